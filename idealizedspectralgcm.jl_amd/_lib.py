@@ -1,0 +1,103 @@
+"""ctypes binding of libjgcm_hip.so (declarations in include/jgcm.h).
+
+The library is the only compute path: if it is missing or a GPU is absent the
+calls raise; nothing here falls back to CPU arithmetic.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjgcm_hip.so")
+
+JGCM_OK = 0
+JGCM_ERR_ARG = -1
+JGCM_ERR_HIP = -2
+JGCM_ERR_STATE = -3
+JGCM_ERR_NOMEM = -4
+JGCM_ERR_UNSUPPORTED = -5
+
+PREC = {"fp64": 0, "fp32": 1, "split32": 2}
+
+TABLES = {
+    "sinθ": 0, "cosθ": 1, "wts": 2, "λc": 3, "θc": 4, "λe": 5, "θe": 6,
+    "qnm": 7, "dqnm": 8, "qwg": 9, "epsilon": 10, "laplacian_eig": 11,
+    "coef_uvm": 12, "coef_uvc": 13, "coef_uvp": 14, "coef_dλ": 15,
+    "coef_dθm": 16, "coef_dθp": 17, "coef_alp_a": 18, "coef_alp_b": 19,
+}
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+SZ = ctypes.c_size_t
+
+# name -> argtypes (all return int unless listed in _RESTYPE)
+SIGNATURES = {
+    "jgcm_last_error": [],
+    "jgcm_device_count": [ctypes.POINTER(I)],
+    "jgcm_malloc": [ctypes.POINTER(P), SZ],
+    "jgcm_free": [P],
+    "jgcm_memcpy_h2d": [P, P, SZ],
+    "jgcm_memcpy_d2h": [P, P, SZ],
+    "jgcm_copy": [P, P, P, SZ],
+    "jgcm_fill": [P, P, D, SZ],
+    "jgcm_mesh_create": [I, I, I, I, I, D, ctypes.POINTER(P)],
+    "jgcm_mesh_destroy": [P],
+    "jgcm_mesh_synchronize": [P],
+    "jgcm_mesh_stream": [P],
+    "jgcm_mesh_set_precision": [P, I],
+    "jgcm_mesh_get_precision": [P, ctypes.POINTER(I)],
+    "jgcm_mesh_table_size": [P, I, ctypes.POINTER(SZ)],
+    "jgcm_mesh_get_table": [P, I, P, SZ],
+    "jgcm_trans_grid_to_spherical": [P, P, P, I],
+    "jgcm_trans_spherical_to_grid": [P, P, P, I],
+    "jgcm_trans_grid_to_fourier": [P, P, P, I],
+    "jgcm_divide_by_cos": [P, P, I],
+    "jgcm_multiply_by_cos": [P, P, P, I],
+    "jgcm_vor_div_from_grid_uv": [P, P, P, P, P, I],
+    "jgcm_compute_alpha_operator": [P, P, P, D, P, I],
+    "jgcm_uv_grid_from_vor_div": [P, P, P, P, P, I],
+    "jgcm_compute_ucos_vcos_from_vor_div": [P, P, P, P, P, I],
+    "jgcm_apply_laplacian": [P, P, I],
+    "jgcm_compute_gradient_cos": [P, P, P, P, I],
+    "jgcm_compute_gradients": [P, P, P, P, I],
+    "jgcm_add_horizontal_advection": [P, P, P, P, P, I],
+    "jgcm_area_weighted_global_mean": [P, P, ctypes.POINTER(D)],
+}
+_RESTYPE = {"jgcm_last_error": ctypes.c_char_p, "jgcm_mesh_stream": P}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                "(the HIP library is the only compute path)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, I)
+        _lib = L
+    return _lib
+
+
+class JGCMError(RuntimeError):
+    pass
+
+
+def check(rc):
+    """Map a status code to the reference's exception kinds: argument/shape
+    failures -> AssertionError (reference @assert), the rest -> JGCMError."""
+    if rc == JGCM_OK:
+        return
+    msg = lib().jgcm_last_error().decode(errors="replace")
+    if rc == JGCM_ERR_ARG:
+        raise AssertionError(msg)
+    raise JGCMError(f"jgcm error {rc}: {msg}")
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args))

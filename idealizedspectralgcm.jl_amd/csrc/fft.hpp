@@ -1,0 +1,103 @@
+// LDS-resident Stockham FFT (radix-4 with one radix-2 stage when log2(L) is
+// odd) used by the longitude transforms.  One FFT of length L is done by
+// T = L/4 threads that hold four points each per stage; a 256-thread
+// workgroup therefore runs 256/T FFTs side by side.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace jgcm {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// twiddle w^k with w = exp(SIGN * 2 pi i / L); tw[] holds exp(-2 pi i k / L)
+template <int SIGN>
+__device__ __forceinline__ double2 twid(const double2* tw, int k) {
+  double2 t = tw[k];
+  return SIGN < 0 ? t : cconj(t);
+}
+
+template <int L, int SIGN>
+__device__ __forceinline__ void stage_r4(double2* buf, const double2* tw, int j, int Ns) {
+  constexpr int T = L / 4;
+  double2 v0 = buf[j], v1 = buf[j + T], v2 = buf[j + 2 * T], v3 = buf[j + 3 * T];
+  int k = j & (Ns - 1);
+  if (Ns > 1) {
+    int step = L / (Ns * 4);
+    v1 = cmul(v1, twid<SIGN>(tw, k * step));
+    v2 = cmul(v2, twid<SIGN>(tw, 2 * k * step));
+    v3 = cmul(v3, twid<SIGN>(tw, 3 * k * step));
+  }
+  double2 a = cadd(v0, v2), b = csub(v0, v2), c = cadd(v1, v3), d = csub(v1, v3);
+  double2 y0 = cadd(a, c), y2 = csub(a, c);
+  double2 y1, y3;
+  if (SIGN < 0) {  // y1 = b - i d, y3 = b + i d
+    y1 = make_double2(b.x + d.y, b.y - d.x);
+    y3 = make_double2(b.x - d.y, b.y + d.x);
+  } else {         // y1 = b + i d, y3 = b - i d
+    y1 = make_double2(b.x - d.y, b.y + d.x);
+    y3 = make_double2(b.x + d.y, b.y - d.x);
+  }
+  __syncthreads();
+  int o = (j - k) * 4 + k;
+  buf[o] = y0;
+  buf[o + Ns] = y1;
+  buf[o + 2 * Ns] = y2;
+  buf[o + 3 * Ns] = y3;
+  __syncthreads();
+}
+
+template <int L, int SIGN>
+__device__ __forceinline__ void stage_r2(double2* buf, const double2* tw, int j, int Ns) {
+  constexpr int T = L / 4;
+  double2 y[2][2];
+  int o[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    int q = j + h * T;
+    double2 v0 = buf[q], v1 = buf[q + L / 2];
+    int k = q & (Ns - 1);
+    if (Ns > 1) v1 = cmul(v1, twid<SIGN>(tw, k * (L / (Ns * 2))));
+    y[h][0] = cadd(v0, v1);
+    y[h][1] = csub(v0, v1);
+    o[h] = (q - k) * 2 + k;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    buf[o[h]] = y[h][0];
+    buf[o[h] + Ns] = y[h][1];
+  }
+  __syncthreads();
+}
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
+// Full FFT of buf[0..L) (natural order in and out), called by every thread
+// of the workgroup (the barriers inside are workgroup-wide); thread j in
+// [0, L/4) of its FFT group does the work.  Radix sequence: 4, [2], 4, 4, ...
+template <int L, int SIGN>
+__device__ __forceinline__ void fft_lds(double2* buf, const double2* tw, int j) {
+  constexpr int LG = ilog2(L);
+  constexpr bool R2 = (LG & 1) != 0;
+  constexpr int NR4 = LG / 2;
+  int Ns = 1;
+  stage_r4<L, SIGN>(buf, tw, j, Ns);
+  Ns *= 4;
+  if (R2) {
+    stage_r2<L, SIGN>(buf, tw, j, Ns);
+    Ns *= 2;
+  }
+#pragma unroll
+  for (int s = 1; s < NR4; ++s) {
+    stage_r4<L, SIGN>(buf, tw, j, Ns);
+    Ns *= 4;
+  }
+}
+
+}  // namespace jgcm

@@ -1,0 +1,119 @@
+// Internal definitions shared by the libjgcm_hip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/jgcm.h"
+
+namespace jgcm {
+
+void set_error(const char* fmt, ...);
+
+#define JGCM_CHECK_HIP(expr)                                                     \
+  do {                                                                           \
+    hipError_t _e = (expr);                                                      \
+    if (_e != hipSuccess) {                                                      \
+      ::jgcm::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),  \
+                        __FILE__, __LINE__);                                     \
+      return JGCM_ERR_HIP;                                                       \
+    }                                                                            \
+  } while (0)
+
+#define JGCM_REQUIRE(cond, ...)                                                  \
+  do {                                                                           \
+    if (!(cond)) {                                                               \
+      ::jgcm::set_error(__VA_ARGS__);                                            \
+      return JGCM_ERR_ARG;                                                       \
+    }                                                                            \
+  } while (0)
+
+#define JGCM_LAUNCH_CHECK()                                                      \
+  do {                                                                           \
+    hipError_t _e = hipGetLastError();                                           \
+    if (_e != hipSuccess) {                                                      \
+      ::jgcm::set_error("kernel launch failed: %s (%s:%d)",                     \
+                        hipGetErrorString(_e), __FILE__, __LINE__);              \
+      return JGCM_ERR_HIP;                                                       \
+    }                                                                            \
+  } while (0)
+
+// Packed per-m row tables: for each zonal wavenumber m = 0..N the rows are
+// the degrees n = m..ns (ns = num_spherical), each row holding `half` (=nθ/2)
+// values on the southern latitudes j' = 0..half-1 (the northern values follow
+// from the equatorial symmetry P(-mu) = (-1)^(n-m) P(mu)).
+struct RowTables {
+  // FP64
+  double* pinv = nullptr;   // qnm                        (inverse transform)
+  double* pfwd = nullptr;   // qnm * w / 2                (forward transform)
+  double* alpa = nullptr;   // w qnm m / cos^2 / (2 r)    (alpha operator, term a)
+  double* alpb = nullptr;   // -w dqnm / (2 r)            (alpha operator, term b)
+  // FP32 hi/lo split copies: x = hi + lo, hi = fl32(x), lo = fl32(x - hi)
+  float* pinv_hi = nullptr; float* pinv_lo = nullptr;
+  float* pfwd_hi = nullptr; float* pfwd_lo = nullptr;
+  float* alpa_hi = nullptr; float* alpa_lo = nullptr;
+  float* alpb_hi = nullptr; float* alpb_lo = nullptr;
+};
+
+}  // namespace jgcm
+
+struct jgcm_mesh {
+  int N = 0;        // num_fourier
+  int ns = 0;       // num_spherical
+  int nlat = 0, nlon = 0, nd = 0, half = 0;
+  int nm = 0;       // N + 1 zonal wavenumbers
+  int nrows = 0;    // packed rows: sum_m (ns + 1 - m)
+  double radius = 0.0;
+  int precision = JGCM_PREC_FP64;
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  // host copies of public fields
+  std::vector<double> sin_theta, cos_theta, wts, lambda_c, theta_c, lambda_e, theta_e;
+  std::vector<double> qnm, dqnm;               // [N+1][ns+1][nlat]
+  std::vector<double> epsilon, laplacian_eig;  // [N+1][ns+1]
+  std::vector<double> coef_uvm, coef_uvc, coef_uvp, coef_dlam, coef_dthm, coef_dthp;
+  std::vector<int> row_start_h;                // [N+2]
+
+  // device tables
+  double* d_sin = nullptr;
+  double* d_cos = nullptr;
+  double* d_wts = nullptr;
+  double* d_qnm_full = nullptr;   // [N+1][ns+2][nlat]   (recurrence workspace)
+  double* d_dqnm_full = nullptr;  // [N+1][ns+1][nlat]
+  int* d_row_start = nullptr;     // [N+2]
+  jgcm::RowTables rt;
+  double* d_twiddle = nullptr;    // nlon complex: exp(-2 pi i k / nlon)
+  // [N+1][ns+1] spectral coefficient tables (reference layout [m][n] read as
+  // row-major here, i.e. index m*(ns+1)+n)
+  double* d_eig = nullptr;
+  double* d_uvm = nullptr;
+  double* d_uvc = nullptr;
+  double* d_uvp = nullptr;
+  double* d_dlam = nullptr;
+  double* d_dthm = nullptr;
+  double* d_dthp = nullptr;
+
+  // scratch (grown on demand, capacity in level-fields)
+  int cap_fields = 0;
+  double* d_four_a = nullptr;   // E/O Fourier block [B][nm][2][half] complex
+  double* d_four_b = nullptr;
+  double* d_spec_a = nullptr;   // spectral scratch [B][ns+1][N+1] complex
+  double* d_spec_b = nullptr;
+  double* d_grid_a = nullptr;   // grid scratch [B][nlat][nlon]
+  double* d_grid_b = nullptr;
+  double* d_scalar = nullptr;   // small device scalars
+
+  size_t spec_elems() const { return (size_t)(ns + 1) * (N + 1); }
+  size_t grid_elems() const { return (size_t)nlat * nlon; }
+  size_t four_elems() const { return (size_t)nm * 2 * half; }  // complex elements
+};
+
+namespace jgcm {
+int ensure_scratch(jgcm_mesh* m, int nfields);
+int check_mesh(const jgcm_mesh* m);
+}  // namespace jgcm

@@ -1,0 +1,297 @@
+// Spectral-space and grid-space operators of the transform core.
+// Reference: src/Atmos_Spectral/Spectral_Spherical_Mesh.jl (SSM), lines cited
+// per kernel.  All kernels are elementwise (or an n +- 1 stencil in spectral
+// space) and HBM-bound; they run one thread per element with grid-stride loops.
+#include "jgcm_internal.hpp"
+
+namespace jgcm {
+
+int fft_fold(const jgcm_mesh* M, const double* grid, double2* eo, const double* cosmul, int B);
+int ifft_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const double* divcos, int B);
+int legendre_inv(const jgcm_mesh* M, const double2* snm, double2* eo, int B);
+
+static inline int nblocks(size_t n, int per = 256) {
+  size_t b = (n + per - 1) / per;
+  return (int)(b > 8192 ? 8192 : (b == 0 ? 1 : b));
+}
+
+#define GRID_STRIDE(i, n) \
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
+
+__global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, size_t n) {
+  GRID_STRIDE(i, n) dst[i] = src[i];
+}
+
+__global__ void copy_bytes_kernel(unsigned char* __restrict__ dst, const unsigned char* __restrict__ src,
+                                  size_t n) {
+  GRID_STRIDE(i, n) dst[i] = src[i];
+}
+
+__global__ void fill_kernel(double* __restrict__ dst, double v, size_t n) {
+  GRID_STRIDE(i, n) dst[i] = v;
+}
+
+// grid [B][nlat][nlon] scaled by cos θ_j: divide (SSM:333-338) or multiply (SSM:341-346)
+__global__ void cos_scale_kernel(const double* __restrict__ in, double* __restrict__ out,
+                                 const double* __restrict__ cost, int nlat, int nlon, size_t n,
+                                 int divide) {
+  GRID_STRIDE(i, n) {
+    int j = (int)((i / nlon) % nlat);
+    out[i] = divide ? in[i] / cost[j] : in[i] * cost[j];
+  }
+}
+
+// SSM:517-580.  Spectral layout [b][n][m]; coefficient tables [m][n].
+__global__ void ucos_vcos_kernel(const double2* __restrict__ vor, const double2* __restrict__ div,
+                                 double2* __restrict__ ucos, double2* __restrict__ vcos,
+                                 const double* __restrict__ uvc, const double* __restrict__ uvm,
+                                 const double* __restrict__ uvp, int nm, int ns, size_t n_all) {
+  GRID_STRIDE(i, n_all) {
+    int m = (int)(i % nm);
+    int n = (int)((i / nm) % (ns + 1));
+    size_t base = i - (size_t)n * nm;  // element (b, 0, m)
+    int c = m * (ns + 1) + n;
+    double2 d = div[i], v = vor[i];
+    double cc = uvc[c];
+    double2 u = make_double2(cc * (-d.y), cc * d.x);   // uvc * (i div)
+    double2 w = make_double2(cc * (-v.y), cc * v.x);   // uvc * (i vor)
+    if (n >= 1) {
+      double a = uvm[c];
+      double2 vm = vor[base + (size_t)(n - 1) * nm], dm = div[base + (size_t)(n - 1) * nm];
+      u = make_double2(u.x - a * vm.x, u.y - a * vm.y);
+      w = make_double2(w.x + a * dm.x, w.y + a * dm.y);
+    }
+    if (n <= ns - 1) {
+      double a = uvp[c];
+      double2 vp = vor[base + (size_t)(n + 1) * nm], dp = div[base + (size_t)(n + 1) * nm];
+      u = make_double2(u.x - a * vp.x, u.y - a * vp.y);
+      w = make_double2(w.x + a * dp.x, w.y + a * dp.y);
+    }
+    ucos[i] = u;
+    vcos[i] = w;
+  }
+}
+
+// SSM:614-621
+__global__ void laplacian_kernel(double2* __restrict__ s, const double* __restrict__ eig, int nm,
+                                 int ns, size_t n_all) {
+  GRID_STRIDE(i, n_all) {
+    int m = (int)(i % nm);
+    int n = (int)((i / nm) % (ns + 1));
+    double e = eig[m * (ns + 1) + n];
+    double2 v = s[i];
+    s[i] = make_double2(e * v.x, e * v.y);
+  }
+}
+
+// SSM:655-692
+__global__ void gradient_cos_kernel(const double2* __restrict__ h, double2* __restrict__ dl,
+                                    double2* __restrict__ dt, const double* __restrict__ cdl,
+                                    const double* __restrict__ cdm, const double* __restrict__ cdp,
+                                    int nm, int ns, size_t n_all) {
+  GRID_STRIDE(i, n_all) {
+    int m = (int)(i % nm);
+    int n = (int)((i / nm) % (ns + 1));
+    size_t base = i - (size_t)n * nm;
+    int c = m * (ns + 1) + n;
+    double2 v = h[i];
+    double a = cdl[c];
+    dl[i] = make_double2(a * (-v.y), a * v.x);
+    double2 t = make_double2(0.0, 0.0);
+    if (n <= ns - 1) {
+      double2 hp = h[base + (size_t)(n + 1) * nm];
+      t = make_double2(cdp[c] * hp.x, cdp[c] * hp.y);
+    }
+    if (n >= 1) {
+      double2 hm = h[base + (size_t)(n - 1) * nm];
+      t = make_double2(t.x + cdm[c] * hm.x, t.y + cdm[c] * hm.y);
+    }
+    dt[i] = t;
+  }
+}
+
+// SSM:716: δh -= u dλh + v dθh
+__global__ void advection_kernel(double* __restrict__ dh, const double* __restrict__ u,
+                                 const double* __restrict__ v, const double* __restrict__ gl,
+                                 const double* __restrict__ gt, size_t n) {
+  GRID_STRIDE(i, n) dh[i] = dh[i] - (u[i] * gl[i] + v[i] * gt[i]);
+}
+
+// SSM:746-760 (level 1): sum_i sum_j f[j][i] w_j / (2 nλ), one workgroup,
+// fixed summation order (deterministic).
+__global__ __launch_bounds__(256) void area_mean_kernel(const double* __restrict__ f,
+                                                        const double* __restrict__ w, int nlat,
+                                                        int nlon, double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nlon; i += 256) {
+    double col = 0.0;
+    for (int j = 0; j < nlat; ++j) col += f[(size_t)j * nlon + i] * w[j];
+    s += col;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0] / (2.0 * nlon);
+}
+
+int area_mean_device(const jgcm_mesh* M, const double* grid, double* out_dev) {
+  hipLaunchKernelGGL(area_mean_kernel, dim3(1), dim3(256), 0, M->stream, grid, M->d_wts, M->nlat,
+                     M->nlon, out_dev);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int ucos_vcos(const jgcm_mesh* M, const double2* vor, const double2* div, double2* ucos,
+              double2* vcos, int B) {
+  size_t n = (size_t)B * M->spec_elems();
+  hipLaunchKernelGGL(ucos_vcos_kernel, dim3(nblocks(n)), dim3(256), 0, M->stream, vor, div, ucos,
+                     vcos, M->d_uvc, M->d_uvm, M->d_uvp, M->nm, M->ns, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int gradient_cos(const jgcm_mesh* M, const double2* h, double2* dl, double2* dt, int B) {
+  size_t n = (size_t)B * M->spec_elems();
+  hipLaunchKernelGGL(gradient_cos_kernel, dim3(nblocks(n)), dim3(256), 0, M->stream, h, dl, dt,
+                     M->d_dlam, M->d_dthm, M->d_dthp, M->nm, M->ns, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+// inverse transform of a spectral field with the rows divided by cos θ
+int inverse_divcos(const jgcm_mesh* M, const double2* spec, double* grid, int B) {
+  double2* eo = reinterpret_cast<double2*>(M->d_four_a);
+  if (int rc = legendre_inv(M, spec, eo, B)) return rc;
+  return ifft_unfold(M, eo, grid, M->d_cos, B);
+}
+
+}  // namespace jgcm
+
+using namespace jgcm;
+
+#define JGCM_MESH_PROLOGUE(mesh, nlev)                                           \
+  if (int _rc = check_mesh(mesh)) return _rc;                                    \
+  JGCM_REQUIRE((nlev) >= 1, "nlev must be >= 1, got %d", (nlev));                \
+  if (int _rc = ensure_scratch(mesh, (nlev))) return _rc;
+
+extern "C" {
+
+int jgcm_copy(jgcm_mesh_t* mesh, void* dst, const void* src, size_t bytes) {
+  if (int rc = check_mesh(mesh)) return rc;
+  if (bytes == 0) return JGCM_OK;
+  JGCM_REQUIRE(dst && src, "NULL pointer");
+  if (bytes % 8 == 0 && ((uintptr_t)dst % 8 == 0) && ((uintptr_t)src % 8 == 0)) {
+    size_t n = bytes / 8;
+    hipLaunchKernelGGL(copy_kernel, dim3(nblocks(n)), dim3(256), 0, mesh->stream, (double*)dst,
+                       (const double*)src, n);
+  } else {
+    hipLaunchKernelGGL(copy_bytes_kernel, dim3(nblocks(bytes)), dim3(256), 0, mesh->stream,
+                       (unsigned char*)dst, (const unsigned char*)src, bytes);
+  }
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_fill(jgcm_mesh_t* mesh, double* dst, double value, size_t count) {
+  if (int rc = check_mesh(mesh)) return rc;
+  if (count == 0) return JGCM_OK;
+  JGCM_REQUIRE(dst, "NULL pointer");
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(count)), dim3(256), 0, mesh->stream, dst, value, count);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_divide_by_cos(jgcm_mesh_t* mesh, double* grid, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(grid, "NULL field pointer");
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  hipLaunchKernelGGL(cos_scale_kernel, dim3(nblocks(n)), dim3(256), 0, mesh->stream, grid, grid,
+                     mesh->d_cos, mesh->nlat, mesh->nlon, n, 1);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_multiply_by_cos(jgcm_mesh_t* mesh, const double* grid, double* grid_cos, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(grid && grid_cos, "NULL field pointer");
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  hipLaunchKernelGGL(cos_scale_kernel, dim3(nblocks(n)), dim3(256), 0, mesh->stream, grid, grid_cos,
+                     mesh->d_cos, mesh->nlat, mesh->nlon, n, 0);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_compute_ucos_vcos_from_vor_div(jgcm_mesh_t* mesh, const double* vor, const double* div,
+                                        double* ucos, double* vcos, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(vor && div && ucos && vcos, "NULL field pointer");
+  return ucos_vcos(mesh, (const double2*)vor, (const double2*)div, (double2*)ucos, (double2*)vcos, nlev);
+}
+
+int jgcm_uv_grid_from_vor_div(jgcm_mesh_t* mesh, const double* vor, const double* div,
+                              double* grid_u, double* grid_v, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(vor && div && grid_u && grid_v, "NULL field pointer");
+  double2* su = (double2*)mesh->d_spec_a;
+  double2* sv = (double2*)mesh->d_spec_b;
+  if (int rc = ucos_vcos(mesh, (const double2*)vor, (const double2*)div, su, sv, nlev)) return rc;
+  if (int rc = inverse_divcos(mesh, su, grid_u, nlev)) return rc;
+  return inverse_divcos(mesh, sv, grid_v, nlev);
+}
+
+int jgcm_apply_laplacian(jgcm_mesh_t* mesh, double* snm, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(snm, "NULL field pointer");
+  size_t n = (size_t)nlev * mesh->spec_elems();
+  hipLaunchKernelGGL(laplacian_kernel, dim3(nblocks(n)), dim3(256), 0, mesh->stream, (double2*)snm,
+                     mesh->d_eig, mesh->nm, mesh->ns, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_compute_gradient_cos(jgcm_mesh_t* mesh, const double* spe_hs, double* spe_cos_dlambda,
+                              double* spe_cos_dtheta, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(spe_hs && spe_cos_dlambda && spe_cos_dtheta, "NULL field pointer");
+  return gradient_cos(mesh, (const double2*)spe_hs, (double2*)spe_cos_dlambda,
+                      (double2*)spe_cos_dtheta, nlev);
+}
+
+int jgcm_compute_gradients(jgcm_mesh_t* mesh, const double* spe_hs, double* grid_dlambda,
+                           double* grid_dtheta, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(spe_hs && grid_dlambda && grid_dtheta, "NULL field pointer");
+  double2* sl = (double2*)mesh->d_spec_a;
+  double2* st = (double2*)mesh->d_spec_b;
+  if (int rc = gradient_cos(mesh, (const double2*)spe_hs, sl, st, nlev)) return rc;
+  if (int rc = inverse_divcos(mesh, sl, grid_dlambda, nlev)) return rc;
+  return inverse_divcos(mesh, st, grid_dtheta, nlev);
+}
+
+int jgcm_add_horizontal_advection(jgcm_mesh_t* mesh, const double* spe_hs, const double* grid_u,
+                                  const double* grid_v, double* grid_dhs, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(spe_hs && grid_u && grid_v && grid_dhs, "NULL field pointer");
+  if (int rc = jgcm_compute_gradients(mesh, spe_hs, mesh->d_grid_a, mesh->d_grid_b, nlev)) return rc;
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  hipLaunchKernelGGL(advection_kernel, dim3(nblocks(n)), dim3(256), 0, mesh->stream, grid_dhs, grid_u,
+                     grid_v, mesh->d_grid_a, mesh->d_grid_b, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_area_weighted_global_mean(jgcm_mesh_t* mesh, const double* grid, double* result) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(grid && result, "NULL pointer");
+  if (int rc = area_mean_device(mesh, grid, mesh->d_scalar)) return rc;
+  JGCM_CHECK_HIP(hipMemcpyAsync(result, mesh->d_scalar, sizeof(double), hipMemcpyDeviceToHost,
+                                mesh->stream));
+  JGCM_CHECK_HIP(hipStreamSynchronize(mesh->stream));
+  return JGCM_OK;
+}
+
+}  // extern "C"

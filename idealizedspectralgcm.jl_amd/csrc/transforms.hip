@@ -1,0 +1,722 @@
+// Spherical-harmonic transforms on gfx950.
+//
+// Forward  (Trans_Grid_To_Spherical!, Spectral_Spherical_Mesh.jl:221-293):
+//   1. fft_fold_kernel: each workgroup takes `ppw` southern latitudes j' of
+//      one level-field and their mirrors nθ-1-j'.  Row j' and its mirror go
+//      into ONE complex FFT (z = a + i c); the wavenumbers m = 0..N of the
+//      equatorially even / odd combinations E = g_j' + g_mirror and
+//      O = g_j' - g_mirror come straight out of that spectrum.  Output: the
+//      E/O block [field][m][parity][j'] (complex, FP64).
+//   2. legendre_fwd_kernel: per (m, parity) segment, rows n = m+p, m+p+2, ..
+//      S[n] = sum_j' P~[n][j'] X[j'] with the quadrature weight and the 1/2
+//      folded into P~.  Lanes are the 64 real columns (32 complex fields);
+//      each wave owns 8 rows whose table entries are wave-uniform (scalar
+//      loads), so every LDS read feeds 8 FMAs.
+// Inverse (Trans_Spherical_To_Grid!, Spectral_Spherical_Mesh.jl:143-217):
+//   1. legendre_inv_kernel: per (m, parity) segment, E/O[j'] = sum_n P[n][j'] S[n]
+//      over degrees m..N+1.
+//   2. ifft_unfold_kernel: rebuilds g_j' = E+O, g_mirror = E-O, packs both
+//      rows into one complex spectrum and runs one inverse FFT per pair.
+// vor/div (Vor_Div_From_Grid_UV!, :349-393) reuse both stages: the FFT stage
+// multiplies by cos θ on load, the Legendre stage contracts two tables
+// (coef_alp_a against i*X^p and coef_alp_b against X^(1-p), the parity of
+// dP/dμ being opposite to that of P).
+//
+// Precision variants of the Legendre stage (the FFT stage is always FP64):
+//   FP64    : FP64 tables and accumulation.
+//   FP32    : FP32 tables and operands, FP32 accumulation.
+//   SPLIT32 : x = x_hi + x_lo, P = P_hi + P_lo (FP32 pieces); per block of 16
+//             latitudes the products P_hi x_hi and the cross terms
+//             P_hi x_lo + P_lo x_hi are summed in FP32, the block sums are
+//             accumulated in FP64 (compensated summation over latitude).
+#include "fft.hpp"
+#include "jgcm_internal.hpp"
+
+namespace jgcm {
+
+enum { P_FP64 = 0, P_FP32 = 1, P_SPLIT = 2 };
+
+// ---------------------------------------------------------------- FFT stage
+template <int L>
+__global__ __launch_bounds__(256) void fft_fold_kernel(
+    const double* __restrict__ grid,   // [B][nlat][L]
+    double2* __restrict__ eo,          // [B][nm][2][half]
+    const double2* __restrict__ twg,   // [L]
+    const double* __restrict__ cosmul, // [nlat] or nullptr: multiply rows by cos θ
+    int nlat, int nm, int ppw) {
+  constexpr int T = L / 4;
+  constexpr int FPW = 256 / T;
+  __shared__ double2 buf[FPW][L];
+  __shared__ double2 tw[L];
+  extern __shared__ double2 stage[];  // [nm][2][ppw]
+  const int half = nlat / 2;
+  const int b = blockIdx.y;
+  const int j0 = blockIdx.x * ppw;
+  for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const double scale = 1.0 / L;
+  for (int pass = 0; pass < ppw; pass += FPW) {
+    int jj = pass + g;
+    int jp = j0 + jj;
+    bool valid = (jj < ppw) && (jp < half);
+    int jm = nlat - 1 - jp;
+    if (valid) {
+      double s = cosmul ? cosmul[jp] : 1.0;
+      const double* ra = grid + ((size_t)b * nlat + jp) * L;
+      const double* rc = grid + ((size_t)b * nlat + jm) * L;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int i = t + r * T;
+        double a = ra[i], c = rc[i];
+        if (cosmul) { a = a * s; c = c * s; }
+        buf[g][i] = make_double2(a, c);
+      }
+    }
+    __syncthreads();
+    fft_lds<L, -1>(buf[g], tw, t);
+    if (valid) {
+      for (int m = t; m < nm; m += T) {
+        double2 zm = buf[g][m];
+        double2 zc = cconj(buf[g][(L - m) & (L - 1)]);
+        double2 A = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+        double2 d = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
+        double2 C = make_double2(d.y, -d.x);  // (zm - zc) / (2i)
+        stage[(m * 2 + 0) * ppw + jj] = make_double2((A.x + C.x) * scale, (A.y + C.y) * scale);
+        stage[(m * 2 + 1) * ppw + jj] = make_double2((A.x - C.x) * scale, (A.y - C.y) * scale);
+      }
+    }
+    __syncthreads();
+  }
+  const int total = nm * 2 * ppw;
+  for (int idx = threadIdx.x; idx < total; idx += 256) {
+    int jj = idx % ppw;
+    int mp = idx / ppw;
+    int jp = j0 + jj;
+    if (jp < half) eo[((size_t)b * nm * 2 + mp) * half + jp] = stage[idx];
+  }
+}
+
+// Inverse: E/O block -> two grid rows per FFT.  `divcos` divides the output
+// rows by cos θ (Divide_By_Cos!, Spectral_Spherical_Mesh.jl:333-338).
+template <int L>
+__global__ __launch_bounds__(256) void ifft_unfold_kernel(
+    const double2* __restrict__ eo,    // [B][nm][2][half]
+    double* __restrict__ grid,         // [B][nlat][L]
+    const double2* __restrict__ twg, const double* __restrict__ divcos,
+    int nlat, int nm, int ppw) {
+  constexpr int T = L / 4;
+  constexpr int FPW = 256 / T;
+  __shared__ double2 buf[FPW][L];
+  __shared__ double2 tw[L];
+  extern __shared__ double2 stage[];  // [nm][2][ppw]
+  const int half = nlat / 2;
+  const int b = blockIdx.y;
+  const int j0 = blockIdx.x * ppw;
+  for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
+  const int total = nm * 2 * ppw;
+  for (int idx = threadIdx.x; idx < total; idx += 256) {
+    int jj = idx % ppw;
+    int mp = idx / ppw;
+    int jp = j0 + jj;
+    stage[idx] = (jp < half) ? eo[((size_t)b * nm * 2 + mp) * half + jp] : make_double2(0.0, 0.0);
+  }
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  for (int pass = 0; pass < ppw; pass += FPW) {
+    int jj = pass + g;
+    int jp = j0 + jj;
+    bool valid = (jj < ppw) && (jp < half);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[g][t + r * T] = make_double2(0.0, 0.0);
+    __syncthreads();
+    if (valid) {
+      for (int m = t; m < nm; m += T) {
+        double2 E = stage[(m * 2 + 0) * ppw + jj];
+        double2 O = stage[(m * 2 + 1) * ppw + jj];
+        double2 ga = cadd(E, O), gc = csub(E, O);
+        if (m == 0) {
+          // g_0 halved, then 2 Re(.) -> Re(g_0) for each row (SSM:211,214)
+          buf[g][0] = make_double2(ga.x, gc.x);
+        } else {
+          buf[g][m] = make_double2(ga.x - gc.y, ga.y + gc.x);          // ga + i gc
+          buf[g][L - m] = make_double2(ga.x + gc.y, gc.x - ga.y);      // conj(ga) + i conj(gc)
+        }
+      }
+    }
+    __syncthreads();
+    fft_lds<L, +1>(buf[g], tw, t);
+    if (valid) {
+      int jm = nlat - 1 - jp;
+      double* ra = grid + ((size_t)b * nlat + jp) * L;
+      double* rc = grid + ((size_t)b * nlat + jm) * L;
+      double c = divcos ? divcos[jp] : 1.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int i = t + r * T;
+        double2 z = buf[g][i];
+        if (divcos) {
+          ra[i] = z.x / c;
+          rc[i] = z.y / c;
+        } else {
+          ra[i] = z.x;
+          rc[i] = z.y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Full Fourier output (Trans_Grid_To_Fourier!, SSM:295-330): four[b][j][k] =
+// fft(row)/nλ for all k, two rows per complex FFT.
+template <int L>
+__global__ __launch_bounds__(256) void fft_full_kernel(const double* __restrict__ grid,
+                                                       double2* __restrict__ four,
+                                                       const double2* __restrict__ twg,
+                                                       int nlat) {
+  constexpr int T = L / 4;
+  constexpr int FPW = 256 / T;
+  __shared__ double2 buf[FPW][L];
+  __shared__ double2 tw[L];
+  const int half = nlat / 2;
+  const int b = blockIdx.y;
+  for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  int jp = blockIdx.x * FPW + g;
+  bool valid = jp < half;
+  int jm = nlat - 1 - jp;
+  if (valid) {
+    for (int r = 0; r < 4; ++r) {
+      int i = t + r * T;
+      buf[g][i] = make_double2(grid[((size_t)b * nlat + jp) * L + i], grid[((size_t)b * nlat + jm) * L + i]);
+    }
+  }
+  __syncthreads();
+  fft_lds<L, -1>(buf[g], tw, t);
+  if (valid) {
+    const double scale = 1.0 / L;
+    for (int r = 0; r < 4; ++r) {
+      int k = t + r * T;
+      double2 zm = buf[g][k];
+      double2 zc = cconj(buf[g][(L - k) & (L - 1)]);
+      double2 A = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      double2 d = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
+      double2 C = make_double2(d.y, -d.x);
+      four[((size_t)b * nlat + jp) * L + k] = make_double2(A.x * scale, A.y * scale);
+      four[((size_t)b * nlat + jm) * L + k] = make_double2(C.x * scale, C.y * scale);
+    }
+  }
+}
+
+// Full Fourier array [B][nlat][nlon] -> E/O block (for Compute_Alpha_Operator!).
+__global__ void fold_full_kernel(const double2* __restrict__ four, double2* __restrict__ eo,
+                                 int nlat, int nlon, int nm, int B) {
+  int half = nlat / 2;
+  size_t total = (size_t)B * nm * half;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    int jp = idx % half;
+    int m = (idx / half) % nm;
+    int b = idx / ((size_t)half * nm);
+    double2 a = four[((size_t)b * nlat + jp) * nlon + m];
+    double2 c = four[((size_t)b * nlat + (nlat - 1 - jp)) * nlon + m];
+    eo[(((size_t)b * nm + m) * 2 + 0) * half + jp] = cadd(a, c);
+    eo[(((size_t)b * nm + m) * 2 + 1) * half + jp] = csub(a, c);
+  }
+}
+
+// ------------------------------------------------------------ Legendre stage
+constexpr int LCOLS = 64;   // real columns per workgroup (32 complex fields)
+constexpr int LROWS = 8;    // rows per wave
+constexpr int LWAVES = 4;
+constexpr int LKB = 16;     // latitude block for the split accumulation
+
+template <int PREC> struct LdsT { using T = double; };
+template <> struct LdsT<P_FP32> { using T = float; };
+template <> struct LdsT<P_SPLIT> { using T = float2; };  // (hi, lo)
+
+template <int PREC>
+__device__ __forceinline__ typename LdsT<PREC>::T to_lds(double v) {
+  if constexpr (PREC == P_FP64) return v;
+  else if constexpr (PREC == P_FP32) return (float)v;
+  else {
+    float h = (float)v;
+    return make_float2(h, (float)(v - (double)h));
+  }
+}
+
+struct FwdTables {
+  const double* d;      // FP64 rows
+  const float* hi;      // FP32 / split hi
+  const float* lo;      // split lo
+};
+
+struct FwdJob {
+  // term A: coefficient table tA against mulA * X_A^(par)
+  const double2* srcA;
+  FwdTables tA;
+  int mulA_i;           // 1: X_A multiplied by i
+  // term B (optional): table tB against signB * X_B^(1-par)
+  const double2* srcB;
+  FwdTables tB;
+  double signB;
+  double2* out;         // spectral [B][ns+1][nm]
+};
+
+struct FwdJobs {
+  FwdJob job[2];
+};
+
+// Stage one (m, parity) tile of the E/O block into LDS, column-major
+// [col][half+1]; col = 2*field + component.
+template <int PREC>
+__device__ __forceinline__ void stage_x(typename LdsT<PREC>::T* X, const double2* __restrict__ src,
+                                        int f0, int nf, int m, int par, int half, int nm,
+                                        int mul_i, double sgn) {
+  const int XS = half + 1;
+  for (int idx = threadIdx.x; idx < nf * half; idx += blockDim.x) {
+    int jp = idx % half;
+    int fl = idx / half;
+    double2 v = src[(((size_t)(f0 + fl) * nm + m) * 2 + par) * half + jp];
+    double re = v.x, im = v.y;
+    if (mul_i) { double t = re; re = -im; im = t; }
+    re *= sgn;
+    im *= sgn;
+    X[(2 * fl) * XS + jp] = to_lds<PREC>(re);
+    X[(2 * fl + 1) * XS + jp] = to_lds<PREC>(im);
+  }
+  for (int idx = threadIdx.x + nf * half; idx < (LCOLS / 2) * half; idx += blockDim.x) {
+    int jp = idx % half;
+    int fl = idx / half;
+    X[(2 * fl) * XS + jp] = to_lds<PREC>(0.0);
+    X[(2 * fl + 1) * XS + jp] = to_lds<PREC>(0.0);
+  }
+}
+
+// Accumulate sum_j' P[row_r][j'] * X[c][j'] into acc[r] for the LROWS rows
+// of this wave.  `rowp[r]` points at the table row (wave-uniform).
+template <int PREC>
+__device__ __forceinline__ void contract_rows(double (&acc)[LROWS], const typename LdsT<PREC>::T* Xc,
+                                              const FwdTables& tab, const size_t (&rowoff)[LROWS],
+                                              int half) {
+  if constexpr (PREC == P_FP64) {
+    for (int j = 0; j < half; ++j) {
+      double x = Xc[j];
+#pragma unroll
+      for (int r = 0; r < LROWS; ++r) acc[r] = fma(tab.d[rowoff[r] + j], x, acc[r]);
+    }
+  } else if constexpr (PREC == P_FP32) {
+    float a[LROWS];
+#pragma unroll
+    for (int r = 0; r < LROWS; ++r) a[r] = 0.0f;
+    for (int j = 0; j < half; ++j) {
+      float x = Xc[j];
+#pragma unroll
+      for (int r = 0; r < LROWS; ++r) a[r] = fmaf(tab.hi[rowoff[r] + j], x, a[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < LROWS; ++r) acc[r] += (double)a[r];
+  } else {
+    for (int j0 = 0; j0 < half; j0 += LKB) {
+      int je = min(half, j0 + LKB);
+      float hh[LROWS], cr[LROWS];
+#pragma unroll
+      for (int r = 0; r < LROWS; ++r) { hh[r] = 0.0f; cr[r] = 0.0f; }
+      for (int j = j0; j < je; ++j) {
+        float2 x = Xc[j];
+#pragma unroll
+        for (int r = 0; r < LROWS; ++r) {
+          float ph = tab.hi[rowoff[r] + j], pl = tab.lo[rowoff[r] + j];
+          hh[r] = fmaf(ph, x.x, hh[r]);
+          cr[r] = fmaf(ph, x.y, fmaf(pl, x.x, cr[r]));
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < LROWS; ++r) acc[r] += (double)hh[r] + (double)cr[r];
+    }
+  }
+}
+
+// grid: x = segment index (m, par) with m = x/2, par = x%2; y = row chunk of
+// LWAVES*LROWS rows; z = field chunk * njobs + job.
+template <int PREC>
+__global__ __launch_bounds__(256) void legendre_fwd_kernel(FwdJobs jobs, int njobs, int B, int N,
+                                                           int ns, int nm, int half,
+                                                           const int* __restrict__ row_start,
+                                                           int nmax, int zero_top) {
+  using LT = typename LdsT<PREC>::T;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  LT* X = reinterpret_cast<LT*>(smem_raw);
+  const int m = blockIdx.x >> 1;
+  const int par = blockIdx.x & 1;
+  const int rc = blockIdx.y;
+  const int jobi = blockIdx.z % njobs;
+  const int fc = blockIdx.z / njobs;
+  const FwdJob& job = jobs.job[jobi];
+  const int f0 = fc * (LCOLS / 2);
+  const int nf = min(LCOLS / 2, B - f0);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nfirst = m + par;
+  // rows of this segment: n = nfirst + 2k, n <= nmax
+  const int nr = (nfirst <= nmax) ? (nmax - nfirst) / 2 + 1 : 0;
+  const int k0 = rc * (LWAVES * LROWS) + wave * LROWS;
+  if (zero_top && par == 0 && rc == 0 && threadIdx.x < 2 * nf) {
+    int fl = threadIdx.x >> 1;
+    double* o = reinterpret_cast<double*>(job.out + ((size_t)(f0 + fl) * (ns + 1) + ns) * nm + m);
+    o[threadIdx.x & 1] = 0.0;
+  }
+  if (rc * LWAVES * LROWS >= nr) return;  // whole workgroup idle (uniform)
+  size_t rowoff[LROWS];
+  const int rs = row_start[m];
+#pragma unroll
+  for (int r = 0; r < LROWS; ++r) {
+    int k = min(k0 + r, nr - 1);
+    rowoff[r] = (size_t)(rs + (nfirst + 2 * k - m)) * half;
+  }
+  double acc[LROWS];
+#pragma unroll
+  for (int r = 0; r < LROWS; ++r) acc[r] = 0.0;
+  const int XS = half + 1;
+  stage_x<PREC>(X, job.srcA, f0, nf, m, par, half, nm, job.mulA_i, 1.0);
+  __syncthreads();
+  if (k0 < nr) contract_rows<PREC>(acc, X + lane * XS, job.tA, rowoff, half);
+  if (job.srcB) {
+    __syncthreads();
+    stage_x<PREC>(X, job.srcB, f0, nf, m, par ^ 1, half, nm, 0, job.signB);
+    __syncthreads();
+    if (k0 < nr) contract_rows<PREC>(acc, X + lane * XS, job.tB, rowoff, half);
+  }
+  if (k0 < nr && lane < 2 * nf) {
+    int fl = lane >> 1;
+    int comp = lane & 1;
+#pragma unroll
+    for (int r = 0; r < LROWS; ++r) {
+      int k = k0 + r;
+      if (k < nr) {
+        int n = nfirst + 2 * k;
+        double* o = reinterpret_cast<double*>(job.out + ((size_t)(f0 + fl) * (ns + 1) + n) * nm + m);
+        o[comp] = acc[r];
+      }
+    }
+  }
+}
+
+// Inverse Legendre: E/O[b][m][par][j'] = sum_{n = m+par+2k <= ns} P[row(m,n)][j'] S[b][n][m].
+// grid: x = segment (m, par), y = field chunk.  Each wave owns 16 latitudes.
+constexpr int IJB = 16;
+
+template <int PREC>
+__global__ __launch_bounds__(512) void legendre_inv_kernel(const double2* __restrict__ snm,
+                                                           double2* __restrict__ eo,
+                                                           FwdTables tab, int B, int N, int ns,
+                                                           int nm, int half,
+                                                           const int* __restrict__ row_start) {
+  using LT = typename LdsT<PREC>::T;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int m = blockIdx.x >> 1;
+  const int par = blockIdx.x & 1;
+  const int f0 = blockIdx.y * (LCOLS / 2);
+  const int nf = min(LCOLS / 2, B - f0);
+  const int nfirst = m + par;
+  const int nr = (nfirst <= ns) ? (ns - nfirst) / 2 + 1 : 0;
+  const int SS = nr + 1;
+  LT* S = reinterpret_cast<LT*>(smem_raw);                          // [col][nr+1]
+  double* Otile = reinterpret_cast<double*>(smem_raw + ((sizeof(LT) * LCOLS * SS + 15) / 16) * 16);  // [col][half+1]
+  const int OS = half + 1;
+  // stage the segment's spectral coefficients, column-major
+  for (int idx = threadIdx.x; idx < (LCOLS / 2) * nr; idx += blockDim.x) {
+    int k = idx % nr;
+    int fl = idx / nr;
+    double2 v = make_double2(0.0, 0.0);
+    if (fl < nf) v = snm[((size_t)(f0 + fl) * (ns + 1) + nfirst + 2 * k) * nm + m];
+    S[(2 * fl) * SS + k] = to_lds<PREC>(v.x);
+    S[(2 * fl + 1) * SS + k] = to_lds<PREC>(v.y);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int j0 = wave * IJB;
+  if (j0 < half) {
+    const int jn = min(IJB, half - j0);
+    double acc[IJB];
+#pragma unroll
+    for (int q = 0; q < IJB; ++q) acc[q] = 0.0;
+    const int rs = row_start[m];
+    const LT* Sc = S + lane * SS;
+    if constexpr (PREC == P_FP64) {
+      for (int k = 0; k < nr; ++k) {
+        double s = Sc[k];
+        const double* p = tab.d + (size_t)(rs + nfirst + 2 * k - m) * half + j0;
+        if (jn == IJB) {
+#pragma unroll
+          for (int q = 0; q < IJB; ++q) acc[q] = fma(p[q], s, acc[q]);
+        } else {
+          for (int q = 0; q < jn; ++q) acc[q] = fma(p[q], s, acc[q]);
+        }
+      }
+    } else if constexpr (PREC == P_FP32) {
+      float a[IJB];
+#pragma unroll
+      for (int q = 0; q < IJB; ++q) a[q] = 0.0f;
+      for (int k = 0; k < nr; ++k) {
+        float s = Sc[k];
+        const float* p = tab.hi + (size_t)(rs + nfirst + 2 * k - m) * half + j0;
+#pragma unroll
+        for (int q = 0; q < IJB; ++q) if (q < jn) a[q] = fmaf(p[q], s, a[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < IJB; ++q) acc[q] = a[q];
+    } else {
+      for (int kb = 0; kb < nr; kb += LKB) {
+        int ke = min(nr, kb + LKB);
+        float hh[IJB], cr[IJB];
+#pragma unroll
+        for (int q = 0; q < IJB; ++q) { hh[q] = 0.0f; cr[q] = 0.0f; }
+        for (int k = kb; k < ke; ++k) {
+          float2 s = Sc[k];
+          size_t ro = (size_t)(rs + nfirst + 2 * k - m) * half + j0;
+          const float* ph = tab.hi + ro;
+          const float* pl = tab.lo + ro;
+#pragma unroll
+          for (int q = 0; q < IJB; ++q) {
+            if (q < jn) {
+              hh[q] = fmaf(ph[q], s.x, hh[q]);
+              cr[q] = fmaf(ph[q], s.y, fmaf(pl[q], s.x, cr[q]));
+            }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < IJB; ++q) acc[q] += (double)hh[q] + (double)cr[q];
+      }
+    }
+    for (int q = 0; q < jn; ++q) Otile[lane * OS + j0 + q] = acc[q];
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < nf * half; idx += blockDim.x) {
+    int jp = idx % half;
+    int fl = idx / half;
+    eo[(((size_t)(f0 + fl) * nm + m) * 2 + par) * half + jp] =
+        make_double2(Otile[(2 * fl) * OS + jp], Otile[(2 * fl + 1) * OS + jp]);
+  }
+}
+
+// ------------------------------------------------------------------ drivers
+static int pick_ppw(int L) {
+  int fpw = 256 / (L / 4);
+  return fpw >= 8 ? fpw : 8;
+}
+
+template <int L>
+static void launch_fold(const jgcm_mesh* M, const double* grid, double2* eo, const double* cosmul,
+                        int B) {
+  int ppw = pick_ppw(L);
+  dim3 grd((M->half + ppw - 1) / ppw, B);
+  size_t lds = (size_t)M->nm * 2 * ppw * sizeof(double2);
+  hipLaunchKernelGGL(fft_fold_kernel<L>, grd, dim3(256), lds, M->stream, grid, eo,
+                     reinterpret_cast<const double2*>(M->d_twiddle), cosmul, M->nlat, M->nm, ppw);
+}
+
+template <int L>
+static void launch_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const double* divcos,
+                          int B) {
+  int ppw = pick_ppw(L);
+  dim3 grd((M->half + ppw - 1) / ppw, B);
+  size_t lds = (size_t)M->nm * 2 * ppw * sizeof(double2);
+  hipLaunchKernelGGL(ifft_unfold_kernel<L>, grd, dim3(256), lds, M->stream, eo, grid,
+                     reinterpret_cast<const double2*>(M->d_twiddle), divcos, M->nlat, M->nm, ppw);
+}
+
+template <int L>
+static void launch_full(const jgcm_mesh* M, const double* grid, double2* four, int B) {
+  constexpr int FPW = 256 / (L / 4);
+  dim3 grd((M->half + FPW - 1) / FPW, B);
+  hipLaunchKernelGGL(fft_full_kernel<L>, grd, dim3(256), 0, M->stream, grid, four,
+                     reinterpret_cast<const double2*>(M->d_twiddle), M->nlat);
+}
+
+#define JGCM_DISPATCH_L(L_, CALL)                                              \
+  switch (L_) {                                                                \
+    case 16: { constexpr int LL = 16; CALL; } break;                           \
+    case 32: { constexpr int LL = 32; CALL; } break;                           \
+    case 64: { constexpr int LL = 64; CALL; } break;                           \
+    case 128: { constexpr int LL = 128; CALL; } break;                         \
+    case 256: { constexpr int LL = 256; CALL; } break;                         \
+    case 512: { constexpr int LL = 512; CALL; } break;                         \
+    case 1024: { constexpr int LL = 1024; CALL; } break;                       \
+    default: break;                                                            \
+  }
+
+int fft_fold(const jgcm_mesh* M, const double* grid, double2* eo, const double* cosmul, int B) {
+  JGCM_DISPATCH_L(M->nlon, launch_fold<LL>(M, grid, eo, cosmul, B));
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int ifft_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const double* divcos, int B) {
+  JGCM_DISPATCH_L(M->nlon, launch_unfold<LL>(M, eo, grid, divcos, B));
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+static FwdTables tables_for(const jgcm_mesh* M, int which) {
+  const RowTables& r = M->rt;
+  switch (which) {
+    case 0: return {r.pfwd, r.pfwd_hi, r.pfwd_lo};
+    case 1: return {r.alpa, r.alpa_hi, r.alpa_lo};
+    case 2: return {r.alpb, r.alpb_hi, r.alpb_lo};
+    default: return {r.pinv, r.pinv_hi, r.pinv_lo};
+  }
+}
+
+static int launch_legendre_fwd(const jgcm_mesh* M, const FwdJobs& jobs, int njobs, int B, int nmax,
+                               int zero_top) {
+  int maxrows = (nmax - 0) / 2 + 1;
+  int rchunks = (maxrows + LWAVES * LROWS - 1) / (LWAVES * LROWS);
+  int fchunks = (B + LCOLS / 2 - 1) / (LCOLS / 2);
+  dim3 grd(2 * M->nm, rchunks, fchunks * njobs);
+  int XS = M->half + 1;
+  switch (M->precision) {
+    case JGCM_PREC_FP64:
+      hipLaunchKernelGGL(legendre_fwd_kernel<P_FP64>, grd, dim3(256), LCOLS * XS * sizeof(double),
+                         M->stream, jobs, njobs, B, M->N, M->ns, M->nm, M->half, M->d_row_start,
+                         nmax, zero_top);
+      break;
+    case JGCM_PREC_FP32:
+      hipLaunchKernelGGL(legendre_fwd_kernel<P_FP32>, grd, dim3(256), LCOLS * XS * sizeof(float),
+                         M->stream, jobs, njobs, B, M->N, M->ns, M->nm, M->half, M->d_row_start,
+                         nmax, zero_top);
+      break;
+    default:
+      hipLaunchKernelGGL(legendre_fwd_kernel<P_SPLIT>, grd, dim3(256), LCOLS * XS * sizeof(float2),
+                         M->stream, jobs, njobs, B, M->N, M->ns, M->nm, M->half, M->d_row_start,
+                         nmax, zero_top);
+      break;
+  }
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int legendre_inv(const jgcm_mesh* M, const double2* snm, double2* eo, int B) {
+  int fchunks = (B + LCOLS / 2 - 1) / (LCOLS / 2);
+  dim3 grd(2 * M->nm, fchunks);
+  int waves = (M->half + IJB - 1) / IJB;
+  int nrmax = M->ns / 2 + 1;
+  FwdTables tab = tables_for(M, 3);
+  size_t esz = M->precision == JGCM_PREC_FP64 ? 8 : (M->precision == JGCM_PREC_FP32 ? 4 : 8);
+  size_t lds = ((esz * LCOLS * (nrmax + 1) + 15) / 16) * 16 + sizeof(double) * LCOLS * (M->half + 1);
+  dim3 blk(64 * waves);
+  switch (M->precision) {
+    case JGCM_PREC_FP64:
+      hipLaunchKernelGGL(legendre_inv_kernel<P_FP64>, grd, blk, lds, M->stream, snm, eo, tab, B,
+                         M->N, M->ns, M->nm, M->half, M->d_row_start);
+      break;
+    case JGCM_PREC_FP32:
+      hipLaunchKernelGGL(legendre_inv_kernel<P_FP32>, grd, blk, lds, M->stream, snm, eo, tab, B,
+                         M->N, M->ns, M->nm, M->half, M->d_row_start);
+      break;
+    default:
+      hipLaunchKernelGGL(legendre_inv_kernel<P_SPLIT>, grd, blk, lds, M->stream, snm, eo, tab, B,
+                         M->N, M->ns, M->nm, M->half, M->d_row_start);
+      break;
+  }
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int legendre_fwd_plain(const jgcm_mesh* M, const double2* eo, double2* snm, int B) {
+  FwdJobs jobs{};
+  jobs.job[0] = FwdJob{eo, tables_for(M, 0), 0, nullptr, FwdTables{}, 0.0, snm};
+  return launch_legendre_fwd(M, jobs, 1, B, M->N, 1);
+}
+
+// vor = α(V, U, -1), div = α(U, V, +1) with the top degree zeroed (SSM:387-392)
+int legendre_vordiv(const jgcm_mesh* M, const double2* eoU, const double2* eoV, double2* vor,
+                    double2* div, int B) {
+  FwdJobs jobs{};
+  jobs.job[0] = FwdJob{eoV, tables_for(M, 1), 1, eoU, tables_for(M, 2), -1.0, vor};
+  jobs.job[1] = FwdJob{eoU, tables_for(M, 1), 1, eoV, tables_for(M, 2), +1.0, div};
+  return launch_legendre_fwd(M, jobs, 2, B, M->N, 1);
+}
+
+int legendre_alpha(const jgcm_mesh* M, const double2* eoA, const double2* eoB, double isign,
+                   double2* alpha, int B) {
+  FwdJobs jobs{};
+  jobs.job[0] = FwdJob{eoA, tables_for(M, 1), 1, eoB, tables_for(M, 2), isign, alpha};
+  return launch_legendre_fwd(M, jobs, 1, B, M->ns, 0);
+}
+
+int fft_full(const jgcm_mesh* M, const double* grid, double2* four, int B) {
+  JGCM_DISPATCH_L(M->nlon, launch_full<LL>(M, grid, four, B));
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int fold_full(const jgcm_mesh* M, const double2* four, double2* eo, int B) {
+  size_t total = (size_t)B * M->nm * M->half;
+  int blocks = (int)std::min<size_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(fold_full_kernel, dim3(blocks), dim3(256), 0, M->stream, four, eo, M->nlat,
+                     M->nlon, M->nm, B);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+}  // namespace jgcm
+
+using namespace jgcm;
+
+#define JGCM_MESH_PROLOGUE(mesh, nlev)                                           \
+  if (int _rc = check_mesh(mesh)) return _rc;                                    \
+  JGCM_REQUIRE((nlev) >= 1, "nlev must be >= 1, got %d", (nlev));                \
+  if (int _rc = ensure_scratch(mesh, (nlev))) return _rc;
+
+extern "C" {
+
+int jgcm_trans_grid_to_spherical(jgcm_mesh_t* mesh, const double* pfield, double* snm, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(pfield && snm, "NULL field pointer");
+  double2* eo = reinterpret_cast<double2*>(mesh->d_four_a);
+  if (int rc = fft_fold(mesh, pfield, eo, nullptr, nlev)) return rc;
+  return legendre_fwd_plain(mesh, eo, reinterpret_cast<double2*>(snm), nlev);
+}
+
+int jgcm_trans_spherical_to_grid(jgcm_mesh_t* mesh, const double* snm, double* pfield, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(pfield && snm, "NULL field pointer");
+  double2* eo = reinterpret_cast<double2*>(mesh->d_four_a);
+  if (int rc = legendre_inv(mesh, reinterpret_cast<const double2*>(snm), eo, nlev)) return rc;
+  return ifft_unfold(mesh, eo, pfield, nullptr, nlev);
+}
+
+int jgcm_trans_grid_to_fourier(jgcm_mesh_t* mesh, const double* pfield, double* fourier_g, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(pfield && fourier_g, "NULL field pointer");
+  return fft_full(mesh, pfield, reinterpret_cast<double2*>(fourier_g), nlev);
+}
+
+int jgcm_vor_div_from_grid_uv(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
+                              double* vor, double* div, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(grid_u && grid_v && vor && div, "NULL field pointer");
+  double2* eu = reinterpret_cast<double2*>(mesh->d_four_a);
+  double2* ev = reinterpret_cast<double2*>(mesh->d_four_b);
+  if (int rc = fft_fold(mesh, grid_u, eu, mesh->d_cos, nlev)) return rc;
+  if (int rc = fft_fold(mesh, grid_v, ev, mesh->d_cos, nlev)) return rc;
+  return legendre_vordiv(mesh, eu, ev, reinterpret_cast<double2*>(vor),
+                         reinterpret_cast<double2*>(div), nlev);
+}
+
+int jgcm_compute_alpha_operator(jgcm_mesh_t* mesh, const double* fourier_a, const double* fourier_b,
+                                double isign, double* alpha, int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(fourier_a && fourier_b && alpha, "NULL field pointer");
+  double2* ea = reinterpret_cast<double2*>(mesh->d_four_a);
+  double2* eb = reinterpret_cast<double2*>(mesh->d_four_b);
+  if (int rc = fold_full(mesh, reinterpret_cast<const double2*>(fourier_a), ea, nlev)) return rc;
+  if (int rc = fold_full(mesh, reinterpret_cast<const double2*>(fourier_b), eb, nlev)) return rc;
+  return legendre_alpha(mesh, ea, eb, isign, reinterpret_cast<double2*>(alpha), nlev);
+}
+
+}  // extern "C"

@@ -1,0 +1,265 @@
+"""Host-side mirror of JGCM's Spectral_Spherical_Mesh API over libjgcm_hip.
+
+Reference: src/Atmos_Spectral/Spectral_Spherical_Mesh.jl (SSM).  Names and
+argument order follow the reference; a trailing ``!`` of the Julia name is
+dropped (Python identifiers cannot carry it) and the function still mutates
+its trailing output arguments in place.  Functions the reference gives a
+``cosθ`` argument (Divide_By_Cos!, Multiply_By_Cos!) take the mesh instead.
+
+Fields are device tensors (torch is used only to hold GPU memory) in the
+reference's memory layout:
+
+* spectral: ``torch.complex128`` of shape ``(nlev, num_spherical+1, num_fourier+1)``
+* grid:     ``torch.float64`` of shape ``(nlev, nθ, nλ)``
+
+with ``nlev == nd`` or ``nlev == 1`` as the reference asserts
+(SSM:175,255,370,466,707,724).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Spectral_Spherical_Mesh:
+    """``Spectral_Spherical_Mesh(num_fourier, num_spherical, nθ, nλ, nd, radius)``
+    (SSM:69-139).  ``precision`` selects the Legendre-stage arithmetic
+    ("fp64", "fp32", "split32"); the reference is FP64."""
+
+    def __init__(self, num_fourier, num_spherical, nθ, nλ, nd, radius, precision="fp64",
+                 device="cuda"):
+        self.num_fourier, self.num_spherical = int(num_fourier), int(num_spherical)
+        self.nθ, self.nλ, self.nd = int(nθ), int(nλ), int(nd)
+        self.radius = float(radius)
+        self.device = torch.device(device)
+        h = ctypes.c_void_p()
+        call("jgcm_mesh_create", self.num_fourier, self.num_spherical, self.nθ, self.nλ,
+             self.nd, self.radius, ctypes.byref(h))
+        self._h = h
+        self.set_precision(precision)
+        N1, S1 = self.num_fourier + 1, self.num_spherical + 1
+        shapes = {"sinθ": (self.nθ,), "cosθ": (self.nθ,), "wts": (self.nθ,),
+                  "λc": (self.nλ,), "θc": (self.nθ,), "λe": (self.nλ + 1,),
+                  "θe": (self.nθ + 1,), "qnm": (N1, S1, self.nθ), "dqnm": (N1, S1, self.nθ),
+                  "qwg": (N1, S1, self.nθ), "epsilon": (N1, S1), "laplacian_eig": (N1, S1),
+                  "coef_uvm": (N1, S1), "coef_uvc": (N1, S1), "coef_uvp": (N1, S1),
+                  "coef_dλ": (N1, S1), "coef_dθm": (N1, S1), "coef_dθp": (N1, S1),
+                  "coef_alp_a": (N1, S1, self.nθ), "coef_alp_b": (N1, S1, self.nθ)}
+        self._shapes = shapes
+        self._tables = {}
+        for k in ("sinθ", "cosθ", "wts", "λc", "θc", "λe", "θe", "laplacian_eig", "epsilon"):
+            setattr(self, k, self.table(k))
+        wn = np.zeros((N1, S1), dtype=np.int64)
+        for m in range(N1):
+            wn[m, m:] = np.arange(m, S1)
+        self.wave_numbers = wn                       # SSM:583-597
+        # device copy of cos θ for host-side callers that need it
+        self.cosθ_dev = torch.tensor(self.cosθ, dtype=torch.float64, device=self.device)
+
+    # ---- tables (host copies of the reference struct's fields) ----
+    def table(self, name):
+        if name not in self._tables:
+            which = _lib.TABLES[name]
+            n = ctypes.c_size_t()
+            call("jgcm_mesh_table_size", self._h, which, ctypes.byref(n))
+            out = np.empty(n.value, dtype=np.float64)
+            call("jgcm_mesh_get_table", self._h, which, out.ctypes.data_as(ctypes.c_void_p), n.value)
+            self._tables[name] = out.reshape(self._shapes[name])
+        return self._tables[name]
+
+    def __getattr__(self, name):
+        if name in ("qnm", "dqnm", "qwg", "coef_uvm", "coef_uvc", "coef_uvp", "coef_dλ",
+                    "coef_dθm", "coef_dθp", "coef_alp_a", "coef_alp_b"):
+            return self.table(name)
+        raise AttributeError(name)
+
+    # ---- handle management ----
+    @property
+    def handle(self):
+        return self._h
+
+    def set_precision(self, precision):
+        call("jgcm_mesh_set_precision", self._h, _lib.PREC[precision])
+        self.precision = precision
+
+    def stream(self):
+        return _lib.lib().jgcm_mesh_stream(self._h)
+
+    def synchronize(self):
+        call("jgcm_mesh_synchronize", self._h)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            call("jgcm_mesh_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- allocation helpers ----
+    def spectral_zeros(self, nlev=None):
+        nlev = self.nd if nlev is None else nlev
+        return torch.zeros((nlev, self.num_spherical + 1, self.num_fourier + 1),
+                           dtype=torch.complex128, device=self.device)
+
+    def grid_zeros(self, nlev=None):
+        nlev = self.nd if nlev is None else nlev
+        return torch.zeros((nlev, self.nθ, self.nλ), dtype=torch.float64, device=self.device)
+
+    # ---- argument checks mirroring the reference asserts ----
+    def _nlev(self, t, kind, batch=False):
+        assert isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous(), \
+            f"{kind} field must be a contiguous device tensor"
+        if kind == "spec":
+            assert t.dtype == torch.complex128 and t.dim() == 3 and \
+                tuple(t.shape[1:]) == (self.num_spherical + 1, self.num_fourier + 1), \
+                f"spectral field shape {tuple(t.shape)}"
+        elif kind == "grid":
+            assert t.dtype == torch.float64 and t.dim() == 3 and \
+                tuple(t.shape[1:]) == (self.nθ, self.nλ), f"grid field shape {tuple(t.shape)}"
+        elif kind == "four":
+            assert t.dtype == torch.complex128 and t.dim() == 3 and \
+                tuple(t.shape[1:]) == (self.nθ, self.nλ), f"fourier field shape {tuple(t.shape)}"
+        nlev = t.shape[0]
+        if not batch:
+            assert nlev == self.nd or nlev == 1, f"level count {nlev} is neither nd={self.nd} nor 1"
+        return nlev
+
+
+def _same(*ts):
+    n = ts[0].shape[0]
+    assert all(t.shape[0] == n for t in ts), "level counts differ"
+    return n
+
+
+# --------------------------------------------------------------- transforms
+def Trans_Spherical_To_Grid(mesh, snm, pfield, batch=False):
+    """SSM:143-217 (``batch=True`` lifts the nd-or-1 level restriction)."""
+    n = mesh._nlev(snm, "spec", batch)
+    mesh._nlev(pfield, "grid", batch)
+    _same(snm, pfield)
+    call("jgcm_trans_spherical_to_grid", mesh.handle, _ptr(snm), _ptr(pfield), n)
+
+
+def Trans_Grid_To_Spherical(mesh, pfield, snm, batch=False):
+    """SSM:221-293."""
+    n = mesh._nlev(pfield, "grid", batch)
+    mesh._nlev(snm, "spec", batch)
+    _same(snm, pfield)
+    call("jgcm_trans_grid_to_spherical", mesh.handle, _ptr(pfield), _ptr(snm), n)
+
+
+def Trans_Grid_To_Fourier(mesh, pfield, fourier_g):
+    """SSM:295-330."""
+    n = mesh._nlev(pfield, "grid", True)
+    mesh._nlev(fourier_g, "four", True)
+    _same(pfield, fourier_g)
+    call("jgcm_trans_grid_to_fourier", mesh.handle, _ptr(pfield), _ptr(fourier_g), n)
+
+
+def Divide_By_Cos(mesh, grid_d):
+    """SSM:333-338."""
+    n = mesh._nlev(grid_d, "grid", True)
+    call("jgcm_divide_by_cos", mesh.handle, _ptr(grid_d), n)
+
+
+def Multiply_By_Cos(mesh, grid_d, grid_d_cos):
+    """SSM:341-346."""
+    n = mesh._nlev(grid_d, "grid", True)
+    mesh._nlev(grid_d_cos, "grid", True)
+    _same(grid_d, grid_d_cos)
+    call("jgcm_multiply_by_cos", mesh.handle, _ptr(grid_d), _ptr(grid_d_cos), n)
+
+
+def Vor_Div_From_Grid_UV(mesh, grid_u, grid_v, vor, div, batch=False):
+    """SSM:349-393."""
+    n = mesh._nlev(grid_u, "grid", batch)
+    for t, k in ((grid_v, "grid"), (vor, "spec"), (div, "spec")):
+        mesh._nlev(t, k, batch)
+    _same(grid_u, grid_v, vor, div)
+    call("jgcm_vor_div_from_grid_uv", mesh.handle, _ptr(grid_u), _ptr(grid_v), _ptr(vor),
+         _ptr(div), n)
+
+
+def Compute_Alpha_Operator(mesh, fourier_a, fourier_b, isign, alpha):
+    """SSM:424-459 (fourier_a/b as produced by Trans_Grid_To_Fourier)."""
+    n = mesh._nlev(fourier_a, "four", True)
+    mesh._nlev(fourier_b, "four", True)
+    mesh._nlev(alpha, "spec", True)
+    _same(fourier_a, fourier_b, alpha)
+    call("jgcm_compute_alpha_operator", mesh.handle, _ptr(fourier_a), _ptr(fourier_b),
+         float(isign), _ptr(alpha), n)
+
+
+def UV_Grid_From_Vor_Div(mesh, vor, div, grid_u, grid_v, batch=False):
+    """SSM:462-482."""
+    n = mesh._nlev(vor, "spec", batch)
+    for t, k in ((div, "spec"), (grid_u, "grid"), (grid_v, "grid")):
+        mesh._nlev(t, k, batch)
+    _same(vor, div, grid_u, grid_v)
+    call("jgcm_uv_grid_from_vor_div", mesh.handle, _ptr(vor), _ptr(div), _ptr(grid_u),
+         _ptr(grid_v), n)
+
+
+def Compute_Ucos_Vcos_From_Vor_Div(mesh, vor, div, spherical_ucos, spherical_vcos):
+    """SSM:517-580."""
+    n = mesh._nlev(vor, "spec", True)
+    for t in (div, spherical_ucos, spherical_vcos):
+        mesh._nlev(t, "spec", True)
+    _same(vor, div, spherical_ucos, spherical_vcos)
+    call("jgcm_compute_ucos_vcos_from_vor_div", mesh.handle, _ptr(vor), _ptr(div),
+         _ptr(spherical_ucos), _ptr(spherical_vcos), n)
+
+
+def Apply_Laplacian(mesh, spherical_u):
+    """SSM:614-621."""
+    n = mesh._nlev(spherical_u, "spec", True)
+    call("jgcm_apply_laplacian", mesh.handle, _ptr(spherical_u), n)
+
+
+def Compute_Gradient_Cos(mesh, spe_hs, spe_cos_dλ_hs, spe_cos_dθ_hs):
+    """SSM:655-692."""
+    n = mesh._nlev(spe_hs, "spec", True)
+    mesh._nlev(spe_cos_dλ_hs, "spec", True)
+    mesh._nlev(spe_cos_dθ_hs, "spec", True)
+    _same(spe_hs, spe_cos_dλ_hs, spe_cos_dθ_hs)
+    call("jgcm_compute_gradient_cos", mesh.handle, _ptr(spe_hs), _ptr(spe_cos_dλ_hs),
+         _ptr(spe_cos_dθ_hs), n)
+
+
+def Compute_Gradients(mesh, spe_hs, grid_dλ_hs, grid_dθ_hs):
+    """SSM:721-741."""
+    n = mesh._nlev(spe_hs, "spec")
+    mesh._nlev(grid_dλ_hs, "grid")
+    mesh._nlev(grid_dθ_hs, "grid")
+    _same(spe_hs, grid_dλ_hs, grid_dθ_hs)
+    call("jgcm_compute_gradients", mesh.handle, _ptr(spe_hs), _ptr(grid_dλ_hs),
+         _ptr(grid_dθ_hs), n)
+
+
+def Add_Horizontal_Advection(mesh, spe_hs, grid_u, grid_v, grid_δhs):
+    """SSM:694-718."""
+    n = mesh._nlev(spe_hs, "spec")
+    for t in (grid_u, grid_v, grid_δhs):
+        mesh._nlev(t, "grid")
+    _same(spe_hs, grid_u, grid_v, grid_δhs)
+    call("jgcm_add_horizontal_advection", mesh.handle, _ptr(spe_hs), _ptr(grid_u), _ptr(grid_v),
+         _ptr(grid_δhs), n)
+
+
+def Area_Weighted_Global_Mean(mesh, grid_datas):
+    """SSM:746-760 (level 1)."""
+    mesh._nlev(grid_datas, "grid", True)
+    out = ctypes.c_double()
+    call("jgcm_area_weighted_global_mean", mesh.handle, _ptr(grid_datas), ctypes.byref(out))
+    return out.value

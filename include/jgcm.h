@@ -1,0 +1,163 @@
+/*
+ * jgcm.h -- C ABI of libjgcm_hip.so, the MI355X (gfx950) spectral-transform
+ * core of JGCM (CliMA/IdealizedSpectralGCM.jl, src/Atmos_Spectral).
+ *
+ * Every entry point mirrors one reference Julia function; the comment above
+ * each declaration names it (file:line under the reference tree).  Argument
+ * order and in-place semantics follow the reference: trailing pointer
+ * arguments are outputs, inputs are never modified unless the reference
+ * modifies them.
+ *
+ * Memory: every field pointer is a DEVICE pointer (hipMalloc'd, or a torch /
+ * Julia GPU array), laid out exactly like the reference's Julia arrays:
+ *   spectral  complex128 [nlev][num_spherical+1][num_fourier+1]  (m fastest,
+ *             interleaved re/im, i.e. Julia ComplexF64[N+1, N+2, nlev])
+ *   grid      float64    [nlev][nlat][nlon]  (longitude fastest, latitude
+ *             index south -> north, Julia Float64[nλ, nθ, nlev])
+ * `nlev` is the number of level-fields in the call.  The reference accepts
+ * only nlev == nd or 1; this library accepts any nlev >= 1 (a batch), which is
+ * how several prognostic fields go through one launch.
+ *
+ * Errors: every int-returning call returns JGCM_OK (0) or a negative code;
+ * jgcm_last_error() gives a thread-local message.  Shape/argument failures
+ * correspond to the reference's @assert sites, state failures to its error()
+ * sites.
+ *
+ * Threading: each mesh owns one HIP stream and scratch buffers; calls on one
+ * mesh are ordered on that stream and a mesh is not re-entrant (the reference
+ * mesh shares scratch the same way, Spectral_Spherical_Mesh.jl:51-64).  The
+ * stream is a blocking stream, so work on the legacy default stream is
+ * ordered with it.
+ */
+#ifndef JGCM_H
+#define JGCM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JGCM_OK 0
+#define JGCM_ERR_ARG (-1)     /* shape / argument assertion (reference @assert) */
+#define JGCM_ERR_HIP (-2)     /* HIP runtime failure                            */
+#define JGCM_ERR_STATE (-3)   /* invalid state (reference error())              */
+#define JGCM_ERR_NOMEM (-4)
+#define JGCM_ERR_UNSUPPORTED (-5)
+
+/* Legendre-contraction precision (the FFT stage and all I/O stay FP64). */
+#define JGCM_PREC_FP64 0      /* FP64 tables, FP64 accumulation                 */
+#define JGCM_PREC_FP32 1      /* FP32 tables and operands, FP32 accumulation    */
+#define JGCM_PREC_SPLIT32 2   /* hi/lo FP32 split, 3 products, compensated sums */
+
+typedef struct jgcm_mesh jgcm_mesh_t;
+typedef struct jgcm_leapfrog jgcm_leapfrog_t;
+typedef struct jgcm_graph jgcm_graph_t;
+
+const char* jgcm_last_error(void);
+int jgcm_device_count(int* count);
+
+/* ---- device memory helpers (for callers without their own GPU arrays) ---- */
+int jgcm_malloc(void** ptr, size_t bytes);
+int jgcm_free(void* ptr);
+int jgcm_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int jgcm_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* Ordered device-to-device copy on the mesh stream (a kernel, so it can be
+ * graph-captured).  Used for Time_Advance! (Dyn_Data.jl:326-364).          */
+int jgcm_copy(jgcm_mesh_t* mesh, void* dst, const void* src, size_t bytes);
+int jgcm_fill(jgcm_mesh_t* mesh, double* dst, double value, size_t count);
+
+/* ---- mesh: Spectral_Spherical_Mesh(num_fourier, num_spherical, nθ, nλ, nd,
+ *      radius)  Spectral_Spherical_Mesh.jl:69-139.  Gaussian nodes/weights
+ *      (Gauss_And_Legendre.jl:69-118) and the Legendre recurrences
+ *      (Gauss_And_Legendre.jl:4-66) are computed in FP64 on the GPU.          */
+int jgcm_mesh_create(int num_fourier, int num_spherical, int nlat, int nlon,
+                     int nd, double radius, jgcm_mesh_t** out);
+int jgcm_mesh_destroy(jgcm_mesh_t* mesh);
+int jgcm_mesh_synchronize(jgcm_mesh_t* mesh);
+/* hipStream_t of the mesh, returned as void*. */
+void* jgcm_mesh_stream(jgcm_mesh_t* mesh);
+int jgcm_mesh_set_precision(jgcm_mesh_t* mesh, int precision);
+int jgcm_mesh_get_precision(jgcm_mesh_t* mesh, int* precision);
+
+/* Host copies of the mesh's public fields (the Julia struct's fields). */
+#define JGCM_TABLE_SIN_THETA 0     /* [nθ]                         */
+#define JGCM_TABLE_COS_THETA 1     /* [nθ]                         */
+#define JGCM_TABLE_WTS 2           /* [nθ]                         */
+#define JGCM_TABLE_LAMBDA_C 3      /* [nλ]                         */
+#define JGCM_TABLE_THETA_C 4       /* [nθ]                         */
+#define JGCM_TABLE_LAMBDA_E 5      /* [nλ+1]                       */
+#define JGCM_TABLE_THETA_E 6       /* [nθ+1]                       */
+#define JGCM_TABLE_QNM 7           /* [N+1][ns+1][nθ]  (C order: m, n, j) */
+#define JGCM_TABLE_DQNM 8          /* [N+1][ns+1][nθ]              */
+#define JGCM_TABLE_QWG 9           /* [N+1][ns+1][nθ]              */
+#define JGCM_TABLE_EPSILON 10      /* [N+1][ns+1]                  */
+#define JGCM_TABLE_LAPLACIAN_EIG 11 /* [N+1][ns+1]                 */
+#define JGCM_TABLE_COEF_UVM 12
+#define JGCM_TABLE_COEF_UVC 13
+#define JGCM_TABLE_COEF_UVP 14
+#define JGCM_TABLE_COEF_DLAMBDA 15
+#define JGCM_TABLE_COEF_DTHETAM 16
+#define JGCM_TABLE_COEF_DTHETAP 17
+#define JGCM_TABLE_COEF_ALP_A 18   /* [N+1][ns+1][nθ]              */
+#define JGCM_TABLE_COEF_ALP_B 19   /* [N+1][ns+1][nθ]              */
+/* Number of doubles in table `which` (so callers can size buffers). */
+int jgcm_mesh_table_size(jgcm_mesh_t* mesh, int which, size_t* count);
+int jgcm_mesh_get_table(jgcm_mesh_t* mesh, int which, double* host_out, size_t count);
+
+/* ---- transforms ----------------------------------------------------------
+ * Trans_Grid_To_Spherical!(mesh, pfield, snm)   Spectral_Spherical_Mesh.jl:221-293
+ *   forward: FFT/nλ, even/odd-folded Gaussian quadrature for degrees m..N,
+ *   degree num_spherical (N+1) row set to 0, lower triangle (n<m) untouched. */
+int jgcm_trans_grid_to_spherical(jgcm_mesh_t* mesh, const double* pfield,
+                                 double* snm, int nlev);
+/* Trans_Spherical_To_Grid!(mesh, snm, pfield)   Spectral_Spherical_Mesh.jl:143-217
+ *   inverse: sums degrees m..N+1, g_0 halved, pfield = 2nλ Re(ifft(g)).      */
+int jgcm_trans_spherical_to_grid(jgcm_mesh_t* mesh, const double* snm,
+                                 double* pfield, int nlev);
+/* Trans_Grid_To_Fourier!(mesh, pfield, fourier_g)  Spectral_Spherical_Mesh.jl:295-330
+ *   fourier_g complex128 [nlev][nθ][nλ] = fft(pfield)/nλ.                    */
+int jgcm_trans_grid_to_fourier(jgcm_mesh_t* mesh, const double* pfield,
+                               double* fourier_g, int nlev);
+
+/* ---- operators ---------------------------------------------------------- */
+/* Divide_By_Cos!(cosθ, grid)                    Spectral_Spherical_Mesh.jl:333-338 */
+int jgcm_divide_by_cos(jgcm_mesh_t* mesh, double* grid, int nlev);
+/* Multiply_By_Cos!(cosθ, grid, grid_cos)        Spectral_Spherical_Mesh.jl:341-346 */
+int jgcm_multiply_by_cos(jgcm_mesh_t* mesh, const double* grid, double* grid_cos, int nlev);
+/* Vor_Div_From_Grid_UV!(mesh, u, v, vor, div)  Spectral_Spherical_Mesh.jl:349-393 */
+int jgcm_vor_div_from_grid_uv(jgcm_mesh_t* mesh, const double* grid_u,
+                              const double* grid_v, double* vor, double* div, int nlev);
+/* Compute_Alpha_Operator!(mesh, a, b, isign, α) Spectral_Spherical_Mesh.jl:424-459
+ *   fourier_a/b are full Fourier arrays as produced by Trans_Grid_To_Fourier!. */
+int jgcm_compute_alpha_operator(jgcm_mesh_t* mesh, const double* fourier_a,
+                                const double* fourier_b, double isign,
+                                double* alpha, int nlev);
+/* UV_Grid_From_Vor_Div!(mesh, vor, div, u, v)  Spectral_Spherical_Mesh.jl:462-482 */
+int jgcm_uv_grid_from_vor_div(jgcm_mesh_t* mesh, const double* vor,
+                              const double* div, double* grid_u, double* grid_v, int nlev);
+/* Compute_Ucos_Vcos_From_Vor_Div!               Spectral_Spherical_Mesh.jl:517-580 */
+int jgcm_compute_ucos_vcos_from_vor_div(jgcm_mesh_t* mesh, const double* vor,
+                                        const double* div, double* ucos,
+                                        double* vcos, int nlev);
+/* Apply_Laplacian!(mesh, s)                     Spectral_Spherical_Mesh.jl:614-621 */
+int jgcm_apply_laplacian(jgcm_mesh_t* mesh, double* snm, int nlev);
+/* Compute_Gradient_Cos!(mesh, hs, dλ, dθ)       Spectral_Spherical_Mesh.jl:655-692 */
+int jgcm_compute_gradient_cos(jgcm_mesh_t* mesh, const double* spe_hs,
+                              double* spe_cos_dlambda, double* spe_cos_dtheta, int nlev);
+/* Compute_Gradients!(mesh, hs, grid_dλ, grid_dθ) Spectral_Spherical_Mesh.jl:721-741 */
+int jgcm_compute_gradients(jgcm_mesh_t* mesh, const double* spe_hs,
+                           double* grid_dlambda, double* grid_dtheta, int nlev);
+/* Add_Horizontal_Advection!(mesh, hs, u, v, δh) Spectral_Spherical_Mesh.jl:694-718 */
+int jgcm_add_horizontal_advection(jgcm_mesh_t* mesh, const double* spe_hs,
+                                  const double* grid_u, const double* grid_v,
+                                  double* grid_dhs, int nlev);
+/* Area_Weighted_Global_Mean(mesh, grid) (level 1) Spectral_Spherical_Mesh.jl:746-760.
+ * Synchronises the mesh stream and returns the value on the host.          */
+int jgcm_area_weighted_global_mean(jgcm_mesh_t* mesh, const double* grid, double* result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JGCM_H */
