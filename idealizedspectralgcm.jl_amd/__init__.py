@@ -11,6 +11,12 @@ from .mesh import (Spectral_Spherical_Mesh, Trans_Spherical_To_Grid, Trans_Grid_
                    Compute_Alpha_Operator, UV_Grid_From_Vor_Div, Compute_Ucos_Vcos_From_Vor_Div,
                    Apply_Laplacian, Compute_Gradient_Cos, Compute_Gradients,
                    Add_Horizontal_Advection, Area_Weighted_Global_Mean)
+from .time_integrator import (Filtered_Leapfrog, Update_Init_Step, Get_Δt, Get_ξ,
+                              Compute_Spectral_Damping, Filtered_Leapfrog_step, Damped_Leapfrog)
+from .dyn_data import Atmo_Data, Compute_Abs_Vor, Dyn_Data, Time_Advance
+from .dynamics import (Barotropic_Dynamics, Shallow_Water_Physics, Implicit_Correction,
+                       Shallow_Water_Dynamics)
+from .graph import Step_Graph
 
 __all__ = [
     "JGCMError", "lib", "Spectral_Spherical_Mesh", "Trans_Spherical_To_Grid",
@@ -18,4 +24,8 @@ __all__ = [
     "Vor_Div_From_Grid_UV", "Compute_Alpha_Operator", "UV_Grid_From_Vor_Div",
     "Compute_Ucos_Vcos_From_Vor_Div", "Apply_Laplacian", "Compute_Gradient_Cos",
     "Compute_Gradients", "Add_Horizontal_Advection", "Area_Weighted_Global_Mean",
+    "Filtered_Leapfrog", "Update_Init_Step", "Get_Δt", "Get_ξ", "Compute_Spectral_Damping",
+    "Filtered_Leapfrog_step", "Damped_Leapfrog", "Atmo_Data", "Compute_Abs_Vor", "Dyn_Data",
+    "Time_Advance", "Barotropic_Dynamics", "Shallow_Water_Physics", "Implicit_Correction",
+    "Shallow_Water_Dynamics", "Step_Graph",
 ]

@@ -62,6 +62,28 @@ SIGNATURES = {
     "jgcm_compute_gradients": [P, P, P, P, I],
     "jgcm_add_horizontal_advection": [P, P, P, P, P, I],
     "jgcm_area_weighted_global_mean": [P, P, ctypes.POINTER(D)],
+    "jgcm_leapfrog_create": [P, D, I, D, D, ctypes.c_int64, I, ctypes.c_int64, ctypes.c_int64,
+                             ctypes.POINTER(P)],
+    "jgcm_leapfrog_destroy": [P],
+    "jgcm_leapfrog_update_init_step": [P],
+    "jgcm_leapfrog_state": [P, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_int64),
+                            ctypes.POINTER(ctypes.c_int64)],
+    "jgcm_leapfrog_set_time": [P, ctypes.c_int64],
+    "jgcm_leapfrog_damping": [P, P, SZ],
+    "jgcm_compute_spectral_damping": [P, P, P, P, I],
+    "jgcm_filtered_leapfrog": [P, P, P, P, P, I],
+    "jgcm_damped_leapfrog": [P, P, P, P, P, I],
+    "jgcm_sw_implicit_correction": [P, P, P, P, P, D, P, P, I],
+    "jgcm_vorticity_flux": [P, P, P, P, D, P, P, P, I, I],
+    "jgcm_axpby": [P, SZ, D, P, D, P, P],
+    "jgcm_xmy": [P, SZ, D, P, P, P, I],
+    "jgcm_kinetic_energy": [P, P, P, P, P, I],
+    "jgcm_shallow_water_physics": [P, P, P, P, P, D, D, P, P, P, I],
+    "jgcm_mesh_reserve": [P, I],
+    "jgcm_graph_begin": [P],
+    "jgcm_graph_end": [P, ctypes.POINTER(P)],
+    "jgcm_graph_launch": [P],
+    "jgcm_graph_destroy": [P],
 }
 _RESTYPE = {"jgcm_last_error": ctypes.c_char_p, "jgcm_mesh_stream": P}
 

@@ -22,6 +22,21 @@ __device__ __forceinline__ double2 twid(const double2* tw, int k) {
   return SIGN < 0 ? t : cconj(t);
 }
 
+// Barrier among the threads of one FFT group.  When the group fits in one
+// wavefront (L <= 256) a wavefront-scope fence suffices: LDS operations of a
+// wave are processed in order, the fence makes the compiler wait for them and
+// stops it from moving LDS accesses across.
+template <int L>
+__device__ __forceinline__ void gsync() {
+  if constexpr (L / 4 <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 template <int L, int SIGN>
 __device__ __forceinline__ void stage_r4(double2* buf, const double2* tw, int j, int Ns) {
   constexpr int T = L / 4;
@@ -43,13 +58,13 @@ __device__ __forceinline__ void stage_r4(double2* buf, const double2* tw, int j,
     y1 = make_double2(b.x - d.y, b.y + d.x);
     y3 = make_double2(b.x + d.y, b.y - d.x);
   }
-  __syncthreads();
+  gsync<L>();
   int o = (j - k) * 4 + k;
   buf[o] = y0;
   buf[o + Ns] = y1;
   buf[o + 2 * Ns] = y2;
   buf[o + 3 * Ns] = y3;
-  __syncthreads();
+  gsync<L>();
 }
 
 template <int L, int SIGN>
@@ -67,20 +82,19 @@ __device__ __forceinline__ void stage_r2(double2* buf, const double2* tw, int j,
     y[h][1] = csub(v0, v1);
     o[h] = (q - k) * 2 + k;
   }
-  __syncthreads();
+  gsync<L>();
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     buf[o[h]] = y[h][0];
     buf[o[h] + Ns] = y[h][1];
   }
-  __syncthreads();
+  gsync<L>();
 }
 
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
 
 // Full FFT of buf[0..L) (natural order in and out), called by every thread
-// of the workgroup (the barriers inside are workgroup-wide); thread j in
-// [0, L/4) of its FFT group does the work.  Radix sequence: 4, [2], 4, 4, ...
+// of the FFT group (gsync inside); thread j in [0, L/4) of the group works.  Radix sequence: 4, [2], 4, 4, ...
 template <int L, int SIGN>
 __device__ __forceinline__ void fft_lds(double2* buf, const double2* tw, int j) {
   constexpr int LG = ilog2(L);

@@ -374,6 +374,17 @@ int jgcm_mesh_create(int num_fourier, int num_spherical, int nlat, int nlon, int
     set_error("nθ=%d exceeds the supported maximum of 512", nlat);
     return JGCM_ERR_UNSUPPORTED;
   }
+  // LDS budget of the FFT stage: 16 KB work buffer + twiddles + the E/O
+  // staging of 8 latitude pairs (or 256/(nλ/4) when larger) x (N+1) x 2.
+  {
+    int fpw = 256 / (nlon / 4);
+    int ppw = fpw >= 8 ? fpw : 8;
+    size_t lds = 16384 + (size_t)nlon * 16 + (size_t)((num_fourier + 1) * 2 + 1) * ppw * 16;
+    if (lds > 65536) {
+      set_error("T%d on nλ=%d exceeds the FFT stage's 64 KB LDS budget", num_fourier, nlon);
+      return JGCM_ERR_UNSUPPORTED;
+    }
+  }
   jgcm_mesh* m = new (std::nothrow) jgcm_mesh();
   if (!m) {
     set_error("out of host memory");

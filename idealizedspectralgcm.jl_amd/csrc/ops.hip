@@ -2,6 +2,8 @@
 // Reference: src/Atmos_Spectral/Spectral_Spherical_Mesh.jl (SSM), lines cited
 // per kernel.  All kernels are elementwise (or an n +- 1 stencil in spectral
 // space) and HBM-bound; they run one thread per element with grid-stride loops.
+#include <new>
+
 #include "jgcm_internal.hpp"
 
 namespace jgcm {
@@ -291,6 +293,65 @@ int jgcm_area_weighted_global_mean(jgcm_mesh_t* mesh, const double* grid, double
   JGCM_CHECK_HIP(hipMemcpyAsync(result, mesh->d_scalar, sizeof(double), hipMemcpyDeviceToHost,
                                 mesh->stream));
   JGCM_CHECK_HIP(hipStreamSynchronize(mesh->stream));
+  return JGCM_OK;
+}
+
+}  // extern "C"
+
+// ---- graph capture -----------------------------------------------------------
+struct jgcm_graph {
+  jgcm_mesh* mesh = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int jgcm_mesh_reserve(jgcm_mesh_t* mesh, int nfields) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nfields >= 1, "nfields must be >= 1");
+  return ensure_scratch(mesh, nfields);
+}
+
+int jgcm_graph_begin(jgcm_mesh_t* mesh) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_CHECK_HIP(hipStreamBeginCapture(mesh->stream, hipStreamCaptureModeThreadLocal));
+  return JGCM_OK;
+}
+
+int jgcm_graph_end(jgcm_mesh_t* mesh, jgcm_graph_t** out) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(out != nullptr, "out is NULL");
+  jgcm_graph* g = new (std::nothrow) jgcm_graph();
+  if (!g) {
+    set_error("out of host memory");
+    return JGCM_ERR_NOMEM;
+  }
+  g->mesh = mesh;
+  hipError_t e = hipStreamEndCapture(mesh->stream, &g->graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    set_error("graph capture failed: %s", hipGetErrorString(e));
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return JGCM_ERR_HIP;
+  }
+  *out = g;
+  return JGCM_OK;
+}
+
+int jgcm_graph_launch(jgcm_graph_t* graph) {
+  JGCM_REQUIRE(graph != nullptr, "graph is NULL");
+  JGCM_CHECK_HIP(hipGraphLaunch(graph->exec, graph->mesh->stream));
+  return JGCM_OK;
+}
+
+int jgcm_graph_destroy(jgcm_graph_t* graph) {
+  if (!graph) return JGCM_OK;
+  (void)hipStreamSynchronize(graph->mesh->stream);
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  delete graph;
   return JGCM_OK;
 }
 

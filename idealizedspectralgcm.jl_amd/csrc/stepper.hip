@@ -1,0 +1,366 @@
+// Time integration and grid-tendency kernels used by the barotropic and
+// shallow-water steppers.
+//
+// Reference:
+//   Time_Integrator.jl:20-101        Filtered_Leapfrog, damping, leapfrog + RA filter
+//   Shallow_Water_Dynamics.jl:7-54   Shallow_Water_Physics!, SW Implicit_Correction!
+//   Atmo_Data.jl:45,59-65            Coriolis parameter, Compute_Abs_Vor!
+// Operation order inside each kernel follows the reference's broadcast
+// expressions so the rounding matches term by term (FP contraction off).
+#pragma clang fp contract(off)
+
+#include <cmath>
+#include <new>
+
+#include "jgcm_internal.hpp"
+
+struct jgcm_leapfrog {
+  jgcm_mesh* mesh = nullptr;
+  double robert_coef = 0.0;
+  int damping_order = 0;
+  double damping_coef = 0.0;
+  double implicit_coef = 0.0;
+  int64_t dt = 0;
+  bool init_step = true;
+  int64_t time = 0, start_time = 0, end_time = 0;
+  double* d_damping = nullptr;   // [N+1][ns+1]
+  std::vector<double> damping;
+};
+
+namespace jgcm {
+
+static inline int nblk(size_t n) {
+  size_t b = (n + 255) / 256;
+  return (int)(b > 8192 ? 8192 : (b == 0 ? 1 : b));
+}
+
+#define GRID_STRIDE(i, n) \
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
+
+// Compute_Spectral_Damping! (TI:76-80) fused with Filtered_Leapfrog! (TI:93-100).
+// `mode` bit 0: apply damping, bit 1: apply leapfrog.
+__global__ void damped_leapfrog_kernel(double2* __restrict__ dQ, const double2* __restrict__ Qp,
+                                       double2* __restrict__ Qc, double2* __restrict__ Qn,
+                                       const double* __restrict__ damping, int nm, int ns,
+                                       size_t n_all, int init_step, double dt, double r, int mode) {
+  GRID_STRIDE(i, n_all) {
+    int m = (int)(i % nm);
+    int n = (int)((i / nm) % (ns + 1));
+    double2 q = dQ[i];
+    if (mode & 1) {
+      double d = damping[m * (ns + 1) + n];
+      if (init_step) {
+        double2 c = Qc[i];
+        double den = 1.0 + dt * d;
+        q = make_double2((q.x - c.x * d) / den, (q.y - c.y * d) / den);
+      } else {
+        double2 p = Qp[i];
+        double den = 1.0 + (2.0 * dt) * d;
+        q = make_double2((q.x - p.x * d) / den, (q.y - p.y * d) / den);
+      }
+      dQ[i] = q;
+    }
+    if (mode & 2) {
+      double2 c = Qc[i];
+      if (init_step) {
+        double2 nx = make_double2(c.x + dt * q.x, c.y + dt * q.y);
+        Qn[i] = nx;
+        Qc[i] = make_double2(c.x + r * (-1.0 * c.x + nx.x), c.y + r * (-1.0 * c.y + nx.y));
+      } else {
+        double2 p = Qp[i];
+        double2 c1 = make_double2(c.x + r * (p.x - 2.0 * c.x), c.y + r * (p.y - 2.0 * c.y));
+        double2 nx = make_double2(p.x + (2.0 * dt) * q.x, p.y + (2.0 * dt) * q.y);
+        Qn[i] = nx;
+        Qc[i] = make_double2(c1.x + r * nx.x, c1.y + r * nx.y);
+      }
+    }
+  }
+}
+
+// SW Implicit_Correction! (Shallow_Water_Dynamics.jl:36-51)
+__global__ void sw_implicit_kernel(const double2* __restrict__ div_c, const double2* __restrict__ div_p,
+                                   const double2* __restrict__ h_c, const double2* __restrict__ h_p,
+                                   double h0, double2* __restrict__ ddiv, double2* __restrict__ dh,
+                                   const double* __restrict__ eig, int nm, int ns, size_t n_all,
+                                   double mu) {
+  double mu2 = mu * mu;
+  GRID_STRIDE(i, n_all) {
+    int m = (int)(i % nm);
+    int n = (int)((i / nm) % (ns + 1));
+    double e = eig[m * (ns + 1) + n];
+    double2 D = ddiv[i], H = dh[i];
+    double2 hc = h_c[i], hp = h_p[i], dc = div_c[i], dp = div_p[i];
+    D = make_double2(D.x + e * (hc.x - hp.x), D.y + e * (hc.y - hp.y));
+    H = make_double2(H.x + h0 * (dc.x - dp.x), H.y + h0 * (dc.y - dp.y));
+    double me = mu * e;
+    double den = 1.0 - (mu2 * e) * h0;
+    D = make_double2((D.x - me * H.x) / den, (D.y - me * H.y) / den);
+    double mh = mu * h0;
+    H = make_double2(H.x - mh * D.x, H.y - mh * D.y);
+    ddiv[i] = D;
+    dh[i] = H;
+  }
+}
+
+// absvor = vor + f_j (Compute_Abs_Vor!, Atmo_Data.jl:59-65) and the vorticity
+// flux  δu (op)= absvor v,  δv (op)= -absvor u  (Barotropic_Dynamics.jl:28-29,
+// Shallow_Water_Dynamics.jl:129-130, Spectral_Dynamics.jl:300-301).
+// accumulate = 0: assign, 1: add.
+__global__ void vorticity_flux_kernel(const double* __restrict__ vor, const double* __restrict__ u,
+                                      const double* __restrict__ v, const double* __restrict__ sint,
+                                      double two_omega, double* __restrict__ absvor,
+                                      double* __restrict__ du, double* __restrict__ dv, int nlat,
+                                      int nlon, size_t n, int accumulate) {
+  GRID_STRIDE(i, n) {
+    int j = (int)((i / nlon) % nlat);
+    double f = two_omega * sint[j];
+    double a = vor[i] + f;
+    if (absvor) absvor[i] = a;
+    if (!du) continue;
+    double fu = a * v[i];
+    double fv = -a * u[i];
+    if (accumulate) {
+      du[i] = du[i] + fu;
+      dv[i] = dv[i] + fv;
+    } else {
+      du[i] = fu;
+      dv[i] = fv;
+    }
+  }
+}
+
+// out = a x + b y  (x or y may be NULL)
+__global__ void axpby_kernel(double a, const double* __restrict__ x, double b,
+                             const double* __restrict__ y, double* __restrict__ out, size_t n) {
+  GRID_STRIDE(i, n) {
+    double r;
+    if (x && y) r = a * x[i] + b * y[i];
+    else if (x) r = a * x[i];
+    else if (y) r = b * y[i];
+    else r = 0.0;
+    out[i] = r;
+  }
+}
+
+// out = (accumulate ? out : 0) + sign * x * y
+__global__ void xmy_kernel(double sign, const double* __restrict__ x, const double* __restrict__ y,
+                           double* __restrict__ out, size_t n, int accumulate) {
+  GRID_STRIDE(i, n) {
+    double p = x[i] * y[i];
+    double r = sign < 0 ? -p : p;
+    out[i] = accumulate ? out[i] + r : r;
+  }
+}
+
+// 0.5 (u^2 + v^2) [+ base]   (Shallow_Water_Dynamics.jl:139, Spectral_Dynamics.jl:307)
+__global__ void kinetic_energy_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                                      const double* __restrict__ base, double* __restrict__ out,
+                                      size_t n) {
+  GRID_STRIDE(i, n) {
+    double k = 0.5 * (u[i] * u[i] + v[i] * v[i]);
+    out[i] = base ? base[i] + k : k;
+  }
+}
+
+// Shallow_Water_Physics! (Shallow_Water_Dynamics.jl:16-18)
+__global__ void sw_physics_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                                  const double* __restrict__ h, const double* __restrict__ h_eq,
+                                  double km, double kt, double* __restrict__ du,
+                                  double* __restrict__ dv, double* __restrict__ dh, size_t n) {
+  GRID_STRIDE(i, n) {
+    du[i] = -km * u[i];
+    dv[i] = -km * v[i];
+    dh[i] = -kt * (h[i] - h_eq[i]);
+  }
+}
+
+}  // namespace jgcm
+
+using namespace jgcm;
+
+extern "C" {
+
+int jgcm_leapfrog_create(jgcm_mesh_t* mesh, double robert_coef, int damping_order,
+                         double damping_coef, double implicit_coef, int64_t dt, int init_step,
+                         int64_t start_time, int64_t end_time, jgcm_leapfrog_t** out) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(out != nullptr, "out is NULL");
+  JGCM_REQUIRE(damping_order % 2 == 0, "damping_order must be even (Time_Integrator.jl:24)");
+  jgcm_leapfrog* L = new (std::nothrow) jgcm_leapfrog();
+  if (!L) {
+    set_error("out of host memory");
+    return JGCM_ERR_NOMEM;
+  }
+  L->mesh = mesh;
+  L->robert_coef = robert_coef;
+  L->damping_order = damping_order;
+  L->damping_coef = damping_coef;
+  L->implicit_coef = implicit_coef;
+  L->dt = dt;
+  L->init_step = init_step != 0;
+  L->time = start_time;
+  L->start_time = start_time;
+  L->end_time = end_time;
+  int N = mesh->N, ns = mesh->ns;
+  size_t ne = (size_t)(N + 1) * (ns + 1);
+  L->damping.assign(ne, 0.0);
+  double e_ref = mesh->laplacian_eig[(size_t)0 * (ns + 1) + (ns - 1)];  // eigen[1, num_spherical]
+  for (size_t k = 0; k < ne; ++k)
+    L->damping[k] = damping_coef * std::pow(mesh->laplacian_eig[k] / e_ref, (double)damping_order);
+  hipError_t e = hipMalloc(&L->d_damping, ne * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMemcpy(L->d_damping, L->damping.data(), ne * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error("leapfrog table upload failed: %s", hipGetErrorString(e));
+    if (L->d_damping) (void)hipFree(L->d_damping);
+    delete L;
+    return JGCM_ERR_HIP;
+  }
+  *out = L;
+  return JGCM_OK;
+}
+
+int jgcm_leapfrog_destroy(jgcm_leapfrog_t* lf) {
+  if (!lf) return JGCM_OK;
+  if (lf->mesh) (void)hipStreamSynchronize(lf->mesh->stream);
+  if (lf->d_damping) (void)hipFree(lf->d_damping);
+  delete lf;
+  return JGCM_OK;
+}
+
+int jgcm_leapfrog_update_init_step(jgcm_leapfrog_t* lf) {
+  JGCM_REQUIRE(lf != nullptr, "leapfrog is NULL");
+  JGCM_REQUIRE(lf->init_step, "Update_Init_Step! called twice (Time_Integrator.jl:41)");
+  lf->init_step = false;
+  return JGCM_OK;
+}
+
+int jgcm_leapfrog_state(jgcm_leapfrog_t* lf, int* init_step, int64_t* dt, int64_t* time) {
+  JGCM_REQUIRE(lf != nullptr, "leapfrog is NULL");
+  if (init_step) *init_step = lf->init_step ? 1 : 0;
+  if (dt) *dt = lf->dt;
+  if (time) *time = lf->time;
+  return JGCM_OK;
+}
+
+int jgcm_leapfrog_set_time(jgcm_leapfrog_t* lf, int64_t time) {
+  JGCM_REQUIRE(lf != nullptr, "leapfrog is NULL");
+  lf->time = time;
+  return JGCM_OK;
+}
+
+int jgcm_leapfrog_damping(jgcm_leapfrog_t* lf, double* host_out, size_t count) {
+  JGCM_REQUIRE(lf != nullptr && host_out != nullptr, "NULL argument");
+  JGCM_REQUIRE(count >= lf->damping.size(), "damping table needs %zu doubles", lf->damping.size());
+  for (size_t k = 0; k < lf->damping.size(); ++k) host_out[k] = lf->damping[k];
+  return JGCM_OK;
+}
+
+static int launch_damped_leapfrog(jgcm_leapfrog_t* lf, double* dQ, const double* Qp, double* Qc,
+                                  double* Qn, int nlev, int mode) {
+  JGCM_REQUIRE(lf != nullptr, "leapfrog is NULL");
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1");
+  jgcm_mesh* M = lf->mesh;
+  size_t n = (size_t)nlev * M->spec_elems();
+  hipLaunchKernelGGL(damped_leapfrog_kernel, dim3(nblk(n)), dim3(256), 0, M->stream, (double2*)dQ,
+                     (const double2*)Qp, (double2*)Qc, (double2*)Qn, lf->d_damping, M->nm, M->ns, n,
+                     lf->init_step ? 1 : 0, (double)lf->dt, lf->robert_coef, mode);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_compute_spectral_damping(jgcm_leapfrog_t* lf, const double* Qc, const double* Qp,
+                                  double* dQ, int nlev) {
+  JGCM_REQUIRE(Qc && Qp && dQ, "NULL field pointer");
+  return launch_damped_leapfrog(lf, dQ, Qp, const_cast<double*>(Qc), nullptr, nlev, 1);
+}
+
+int jgcm_filtered_leapfrog(jgcm_leapfrog_t* lf, const double* dQ, const double* Qp, double* Qc,
+                           double* Qn, int nlev) {
+  JGCM_REQUIRE(dQ && Qp && Qc && Qn, "NULL field pointer");
+  return launch_damped_leapfrog(lf, const_cast<double*>(dQ), Qp, Qc, Qn, nlev, 2);
+}
+
+int jgcm_damped_leapfrog(jgcm_leapfrog_t* lf, double* dQ, const double* Qp, double* Qc, double* Qn,
+                         int nlev) {
+  JGCM_REQUIRE(dQ && Qp && Qc && Qn, "NULL field pointer");
+  return launch_damped_leapfrog(lf, dQ, Qp, Qc, Qn, nlev, 3);
+}
+
+int jgcm_sw_implicit_correction(jgcm_leapfrog_t* lf, const double* div_c, const double* div_p,
+                                const double* h_c, const double* h_p, double h_0, double* ddiv,
+                                double* dh, int nlev) {
+  JGCM_REQUIRE(lf != nullptr, "leapfrog is NULL");
+  JGCM_REQUIRE(div_c && div_p && h_c && h_p && ddiv && dh, "NULL field pointer");
+  jgcm_mesh* M = lf->mesh;
+  size_t n = (size_t)nlev * M->spec_elems();
+  double dtv = lf->init_step ? (double)lf->dt : 2.0 * lf->dt;
+  double mu = lf->implicit_coef * dtv;
+  hipLaunchKernelGGL(sw_implicit_kernel, dim3(nblk(n)), dim3(256), 0, M->stream, (const double2*)div_c,
+                     (const double2*)div_p, (const double2*)h_c, (const double2*)h_p, h_0,
+                     (double2*)ddiv, (double2*)dh, M->d_eig, M->nm, M->ns, n, mu);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_vorticity_flux(jgcm_mesh_t* mesh, const double* grid_vor, const double* grid_u,
+                        const double* grid_v, double omega, double* grid_absvor, double* grid_du,
+                        double* grid_dv, int nlev, int accumulate) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(grid_vor, "NULL field pointer");
+  JGCM_REQUIRE((grid_du && grid_dv && grid_u && grid_v) || (!grid_du && !grid_dv && grid_absvor),
+               "need (u, v, δu, δv) or an absvor output only");
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  hipLaunchKernelGGL(vorticity_flux_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, grid_vor, grid_u,
+                     grid_v, mesh->d_sin, 2 * omega, grid_absvor, grid_du, grid_dv, mesh->nlat,
+                     mesh->nlon, n, accumulate);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_axpby(jgcm_mesh_t* mesh, size_t count, double a, const double* x, double b,
+               const double* y, double* out) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(out, "NULL output");
+  if (count == 0) return JGCM_OK;
+  hipLaunchKernelGGL(axpby_kernel, dim3(nblk(count)), dim3(256), 0, mesh->stream, a, x, b, y, out, count);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_xmy(jgcm_mesh_t* mesh, size_t count, double sign, const double* x, const double* y,
+             double* out, int accumulate) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(x && y && out, "NULL pointer");
+  if (count == 0) return JGCM_OK;
+  hipLaunchKernelGGL(xmy_kernel, dim3(nblk(count)), dim3(256), 0, mesh->stream, sign, x, y, out, count,
+                     accumulate);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_kinetic_energy(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
+                        const double* grid_base, double* grid_out, int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(grid_u && grid_v && grid_out, "NULL pointer");
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  hipLaunchKernelGGL(kinetic_energy_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, grid_u, grid_v,
+                     grid_base, grid_out, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_shallow_water_physics(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
+                               const double* grid_h, const double* h_eq, double kappa_m,
+                               double kappa_t, double* grid_du, double* grid_dv, double* grid_dh,
+                               int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(grid_u && grid_v && grid_h && h_eq && grid_du && grid_dv && grid_dh, "NULL pointer");
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  hipLaunchKernelGGL(sw_physics_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, grid_u, grid_v, grid_h,
+                     h_eq, kappa_m, kappa_t, grid_du, grid_dv, grid_dh, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+}  // extern "C"

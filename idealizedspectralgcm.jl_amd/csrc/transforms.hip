@@ -48,11 +48,13 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
   constexpr int FPW = 256 / T;
   __shared__ double2 buf[FPW][L];
   __shared__ double2 tw[L];
-  extern __shared__ double2 stage[];  // [nm][2][ppw]
+  extern __shared__ double2 stage[];  // [ppw][2 nm + 1]: (m, parity) per latitude, padded
   const int half = nlat / 2;
+  const int SP = 2 * nm + 1;
   const int b = blockIdx.y;
   const int j0 = blockIdx.x * ppw;
   for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
+  __syncthreads();
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const double scale = 1.0 / L;
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
         buf[g][i] = make_double2(a, c);
       }
     }
-    __syncthreads();
+    gsync<L>();
     fft_lds<L, -1>(buf[g], tw, t);
     if (valid) {
       for (int m = t; m < nm; m += T) {
@@ -82,18 +84,19 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
         double2 A = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
         double2 d = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
         double2 C = make_double2(d.y, -d.x);  // (zm - zc) / (2i)
-        stage[(m * 2 + 0) * ppw + jj] = make_double2((A.x + C.x) * scale, (A.y + C.y) * scale);
-        stage[(m * 2 + 1) * ppw + jj] = make_double2((A.x - C.x) * scale, (A.y - C.y) * scale);
+        stage[jj * SP + m * 2 + 0] = make_double2((A.x + C.x) * scale, (A.y + C.y) * scale);
+        stage[jj * SP + m * 2 + 1] = make_double2((A.x - C.x) * scale, (A.y - C.y) * scale);
       }
     }
-    __syncthreads();
+    gsync<L>();
   }
+  __syncthreads();
   const int total = nm * 2 * ppw;
   for (int idx = threadIdx.x; idx < total; idx += 256) {
     int jj = idx % ppw;
     int mp = idx / ppw;
     int jp = j0 + jj;
-    if (jp < half) eo[((size_t)b * nm * 2 + mp) * half + jp] = stage[idx];
+    if (jp < half) eo[((size_t)b * nm * 2 + mp) * half + jp] = stage[jj * SP + mp];
   }
 }
 
@@ -109,8 +112,9 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
   constexpr int FPW = 256 / T;
   __shared__ double2 buf[FPW][L];
   __shared__ double2 tw[L];
-  extern __shared__ double2 stage[];  // [nm][2][ppw]
+  extern __shared__ double2 stage[];  // [ppw][2 nm + 1]
   const int half = nlat / 2;
+  const int SP = 2 * nm + 1;
   const int b = blockIdx.y;
   const int j0 = blockIdx.x * ppw;
   for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
@@ -119,8 +123,9 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
     int jj = idx % ppw;
     int mp = idx / ppw;
     int jp = j0 + jj;
-    stage[idx] = (jp < half) ? eo[((size_t)b * nm * 2 + mp) * half + jp] : make_double2(0.0, 0.0);
+    stage[jj * SP + mp] = (jp < half) ? eo[((size_t)b * nm * 2 + mp) * half + jp] : make_double2(0.0, 0.0);
   }
+  __syncthreads();
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
   for (int pass = 0; pass < ppw; pass += FPW) {
@@ -129,11 +134,11 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
     bool valid = (jj < ppw) && (jp < half);
 #pragma unroll
     for (int r = 0; r < 4; ++r) buf[g][t + r * T] = make_double2(0.0, 0.0);
-    __syncthreads();
+    gsync<L>();
     if (valid) {
       for (int m = t; m < nm; m += T) {
-        double2 E = stage[(m * 2 + 0) * ppw + jj];
-        double2 O = stage[(m * 2 + 1) * ppw + jj];
+        double2 E = stage[jj * SP + m * 2 + 0];
+        double2 O = stage[jj * SP + m * 2 + 1];
         double2 ga = cadd(E, O), gc = csub(E, O);
         if (m == 0) {
           // g_0 halved, then 2 Re(.) -> Re(g_0) for each row (SSM:211,214)
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
         }
       }
     }
-    __syncthreads();
+    gsync<L>();
     fft_lds<L, +1>(buf[g], tw, t);
     if (valid) {
       int jm = nlat - 1 - jp;
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
         }
       }
     }
-    __syncthreads();
+    gsync<L>();
   }
 }
 
@@ -182,6 +187,7 @@ __global__ __launch_bounds__(256) void fft_full_kernel(const double* __restrict_
   const int half = nlat / 2;
   const int b = blockIdx.y;
   for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
+  __syncthreads();
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
   int jp = blockIdx.x * FPW + g;
@@ -232,6 +238,14 @@ constexpr int LCOLS = 64;   // real columns per workgroup (32 complex fields)
 constexpr int LROWS = 8;    // rows per wave
 constexpr int LWAVES = 4;
 constexpr int LKB = 16;     // latitude block for the split accumulation
+
+// Table rows are read at wave-uniform addresses; going through the constant
+// address space lets the compiler use scalar loads (one load per wave,
+// operand in SGPRs) instead of per-lane vector loads.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* p) {
+  return (const __attribute__((address_space(4))) T*)p;
+}
 
 template <int PREC> struct LdsT { using T = double; };
 template <> struct LdsT<P_FP32> { using T = float; };
@@ -302,23 +316,27 @@ __device__ __forceinline__ void contract_rows(double (&acc)[LROWS], const typena
                                               const FwdTables& tab, const size_t (&rowoff)[LROWS],
                                               int half) {
   if constexpr (PREC == P_FP64) {
+    const auto* td = as_const(tab.d);
     for (int j = 0; j < half; ++j) {
       double x = Xc[j];
 #pragma unroll
-      for (int r = 0; r < LROWS; ++r) acc[r] = fma(tab.d[rowoff[r] + j], x, acc[r]);
+      for (int r = 0; r < LROWS; ++r) acc[r] = fma(td[rowoff[r] + j], x, acc[r]);
     }
   } else if constexpr (PREC == P_FP32) {
+    const auto* th = as_const(tab.hi);
     float a[LROWS];
 #pragma unroll
     for (int r = 0; r < LROWS; ++r) a[r] = 0.0f;
     for (int j = 0; j < half; ++j) {
       float x = Xc[j];
 #pragma unroll
-      for (int r = 0; r < LROWS; ++r) a[r] = fmaf(tab.hi[rowoff[r] + j], x, a[r]);
+      for (int r = 0; r < LROWS; ++r) a[r] = fmaf(th[rowoff[r] + j], x, a[r]);
     }
 #pragma unroll
     for (int r = 0; r < LROWS; ++r) acc[r] += (double)a[r];
   } else {
+    const auto* th = as_const(tab.hi);
+    const auto* tl = as_const(tab.lo);
     for (int j0 = 0; j0 < half; j0 += LKB) {
       int je = min(half, j0 + LKB);
       float hh[LROWS], cr[LROWS];
@@ -328,7 +346,7 @@ __device__ __forceinline__ void contract_rows(double (&acc)[LROWS], const typena
         float2 x = Xc[j];
 #pragma unroll
         for (int r = 0; r < LROWS; ++r) {
-          float ph = tab.hi[rowoff[r] + j], pl = tab.lo[rowoff[r] + j];
+          float ph = th[rowoff[r] + j], pl = tl[rowoff[r] + j];
           hh[r] = fmaf(ph, x.x, hh[r]);
           cr[r] = fmaf(ph, x.y, fmaf(pl, x.x, cr[r]));
         }
@@ -374,7 +392,7 @@ __global__ __launch_bounds__(256) void legendre_fwd_kernel(FwdJobs jobs, int njo
 #pragma unroll
   for (int r = 0; r < LROWS; ++r) {
     int k = min(k0 + r, nr - 1);
-    rowoff[r] = (size_t)(rs + (nfirst + 2 * k - m)) * half;
+    rowoff[r] = (size_t)__builtin_amdgcn_readfirstlane((rs + (nfirst + 2 * k - m)) * half);
   }
   double acc[LROWS];
 #pragma unroll
@@ -449,7 +467,7 @@ __global__ __launch_bounds__(512) void legendre_inv_kernel(const double2* __rest
     if constexpr (PREC == P_FP64) {
       for (int k = 0; k < nr; ++k) {
         double s = Sc[k];
-        const double* p = tab.d + (size_t)(rs + nfirst + 2 * k - m) * half + j0;
+        const auto* p = as_const(tab.d) + (size_t)(rs + nfirst + 2 * k - m) * half + j0;
         if (jn == IJB) {
 #pragma unroll
           for (int q = 0; q < IJB; ++q) acc[q] = fma(p[q], s, acc[q]);
@@ -463,7 +481,7 @@ __global__ __launch_bounds__(512) void legendre_inv_kernel(const double2* __rest
       for (int q = 0; q < IJB; ++q) a[q] = 0.0f;
       for (int k = 0; k < nr; ++k) {
         float s = Sc[k];
-        const float* p = tab.hi + (size_t)(rs + nfirst + 2 * k - m) * half + j0;
+        const auto* p = as_const(tab.hi) + (size_t)(rs + nfirst + 2 * k - m) * half + j0;
 #pragma unroll
         for (int q = 0; q < IJB; ++q) if (q < jn) a[q] = fmaf(p[q], s, a[q]);
       }
@@ -478,8 +496,8 @@ __global__ __launch_bounds__(512) void legendre_inv_kernel(const double2* __rest
         for (int k = kb; k < ke; ++k) {
           float2 s = Sc[k];
           size_t ro = (size_t)(rs + nfirst + 2 * k - m) * half + j0;
-          const float* ph = tab.hi + ro;
-          const float* pl = tab.lo + ro;
+          const auto* ph = as_const(tab.hi) + ro;
+          const auto* pl = as_const(tab.lo) + ro;
 #pragma unroll
           for (int q = 0; q < IJB; ++q) {
             if (q < jn) {
@@ -514,7 +532,7 @@ static void launch_fold(const jgcm_mesh* M, const double* grid, double2* eo, con
                         int B) {
   int ppw = pick_ppw(L);
   dim3 grd((M->half + ppw - 1) / ppw, B);
-  size_t lds = (size_t)M->nm * 2 * ppw * sizeof(double2);
+  size_t lds = (size_t)(M->nm * 2 + 1) * ppw * sizeof(double2);
   hipLaunchKernelGGL(fft_fold_kernel<L>, grd, dim3(256), lds, M->stream, grid, eo,
                      reinterpret_cast<const double2*>(M->d_twiddle), cosmul, M->nlat, M->nm, ppw);
 }
@@ -524,7 +542,7 @@ static void launch_unfold(const jgcm_mesh* M, const double2* eo, double* grid, c
                           int B) {
   int ppw = pick_ppw(L);
   dim3 grd((M->half + ppw - 1) / ppw, B);
-  size_t lds = (size_t)M->nm * 2 * ppw * sizeof(double2);
+  size_t lds = (size_t)(M->nm * 2 + 1) * ppw * sizeof(double2);
   hipLaunchKernelGGL(ifft_unfold_kernel<L>, grd, dim3(256), lds, M->stream, eo, grid,
                      reinterpret_cast<const double2*>(M->d_twiddle), divcos, M->nlat, M->nm, ppw);
 }

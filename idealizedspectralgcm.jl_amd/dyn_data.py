@@ -1,0 +1,102 @@
+"""Atmo_Data and Dyn_Data mirrors (device-resident state).
+
+Reference: src/Atmos_Spectral/Atmo_Data.jl, src/Atmos_Spectral/Dyn_Data.jl.
+Every field is a device tensor in the reference layout; ``Time_Advance`` is
+done with ordered device copies on the mesh stream, with the reference's
+semantics (p <- c, c <- n, the ``_n`` fields keep their values,
+Dyn_Data.jl:326-364).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import call
+from .mesh import _ptr, current_mesh
+
+
+class Atmo_Data:
+    """``Atmo_Data(name, nλ, nθ, nd, do_mass_correction, do_energy_correction,
+    do_water_correction, use_virtual_temperature, sinθ, radius, omega=7.292e-5,
+    grav=9.80, rdgas=287.04, kappa=2/7, rvgas=461.50)`` (Atmo_Data.jl:34-55)."""
+
+    def __init__(self, name, nλ, nθ, nd, do_mass_correction, do_energy_correction,
+                 do_water_correction, use_virtual_temperature, sinθ, radius, omega=7.292e-5,
+                 grav=9.80, rdgas=287.04, kappa=2.0 / 7.0, rvgas=461.50):
+        self.name = name
+        self.nλ, self.nθ, self.nd = nλ, nθ, nd
+        self.do_mass_correction = bool(do_mass_correction)
+        self.do_energy_correction = bool(do_energy_correction)
+        self.do_water_correction = bool(do_water_correction)
+        self.use_virtual_temperature = bool(use_virtual_temperature)
+        self.radius, self.omega, self.grav = float(radius), float(omega), float(grav)
+        self.rdgas, self.kappa, self.rvgas = float(rdgas), float(kappa), float(rvgas)
+        self.coriolis = 2 * self.omega * np.asarray(sinθ)                   # :45
+        self.cp_air = self.rdgas / self.kappa                                # :46
+
+
+def Compute_Abs_Vor(mesh, grid_vor, atmo_data, grid_absvor):
+    """Atmo_Data.jl:59-65 (takes the mesh and Atmo_Data in place of the
+    coriolis array: f = 2Ω sinθ is formed on the device)."""
+    n = mesh._nlev(grid_vor, "grid", True)
+    call("jgcm_vorticity_flux", mesh.handle, _ptr(grid_vor), None, None, atmo_data.omega,
+         _ptr(grid_absvor), None, None, n, 0)
+
+
+_SPEC_ND = ["spe_vor_n", "spe_vor_c", "spe_vor_p", "spe_div_n", "spe_div_c", "spe_div_p",
+            "spe_t_n", "spe_t_c", "spe_t_p", "spe_δvor", "spe_δdiv", "spec_δu", "spec_δv",
+            "spe_δt", "spe_energy", "spe_d1", "spe_d2", "spe_zeros"]
+_SPEC_1 = ["spe_lnps_n", "spe_lnps_c", "spe_lnps_p", "spe_δlnps"]
+_GRID_ND = ["grid_u_n", "grid_u_c", "grid_u_p", "grid_v_n", "grid_v_c", "grid_v_p",
+            "grid_t_n", "grid_t_c", "grid_t_p", "grid_vor", "grid_δvor", "grid_div", "grid_δdiv",
+            "grid_δu", "grid_δv", "grid_p_full", "grid_lnp_full", "grid_Δp", "grid_δt",
+            "grid_absvor", "grid_w_full", "grid_energy_full", "grid_geopot_full", "grid_z_full",
+            "grid_t_eq", "grid_d_full1", "grid_d_full2"]
+_GRID_1 = ["grid_ps_n", "grid_ps_c", "grid_ps_p", "grid_lnps", "grid_δlnps", "grid_δps",
+           "grid_dλ_ps", "grid_dθ_ps", "grid_geopots"]
+_GRID_HALF = ["grid_p_half", "grid_lnp_half", "grid_M_half", "grid_geopot_half", "grid_z_half",
+              "grid_d_half1", "grid_d_half2"]
+
+
+class Dyn_Data:
+    """``Dyn_Data(name, num_fourier, num_spherical, nλ, nθ, nd,
+    num_grid_tracters=0, num_spe_tracters=0)`` (Dyn_Data.jl:158-324).  The
+    mesh whose stream orders Time_Advance defaults to the last mesh created
+    with the same shape (``mesh=`` selects one explicitly)."""
+
+    def __init__(self, name, num_fourier, num_spherical, nλ, nθ, nd, num_grid_tracters=0,
+                 num_spe_tracters=0, mesh=None, device="cuda"):
+        self.name = name
+        self.num_fourier, self.num_spherical = num_fourier, num_spherical
+        self.nλ, self.nθ, self.nd = nλ, nθ, nd
+        self.num_grid_tracters, self.num_spe_tracters = num_grid_tracters, num_spe_tracters
+        self.mesh = mesh if mesh is not None else current_mesh(num_fourier, nθ, nλ)
+        dev = torch.device(device)
+        sp = lambda k: torch.zeros((k, num_spherical + 1, num_fourier + 1), dtype=torch.complex128,
+                                   device=dev)
+        gr = lambda k: torch.zeros((k, nθ, nλ), dtype=torch.float64, device=dev)
+        for f in _SPEC_ND:
+            setattr(self, f, sp(nd))
+        for f in _SPEC_1:
+            setattr(self, f, sp(1))
+        for f in _GRID_ND:
+            setattr(self, f, gr(nd))
+        for f in _GRID_1:
+            setattr(self, f, gr(1))
+        for f in _GRID_HALF:
+            setattr(self, f, gr(nd + 1))
+
+
+def _copy(mesh, dst, src):
+    call("jgcm_copy", mesh.handle, _ptr(dst), _ptr(src), dst.numel() * dst.element_size())
+
+
+def Time_Advance(dyn_data):
+    """Dyn_Data.jl:326-364."""
+    m = dyn_data.mesh
+    for f in ("vor", "div", "lnps", "t"):
+        _copy(m, getattr(dyn_data, f"spe_{f}_p"), getattr(dyn_data, f"spe_{f}_c"))
+        _copy(m, getattr(dyn_data, f"spe_{f}_c"), getattr(dyn_data, f"spe_{f}_n"))
+    for f in ("u", "v", "ps", "t"):
+        _copy(m, getattr(dyn_data, f"grid_{f}_p"), getattr(dyn_data, f"grid_{f}_c"))
+        _copy(m, getattr(dyn_data, f"grid_{f}_c"), getattr(dyn_data, f"grid_{f}_n"))

@@ -1,0 +1,46 @@
+"""HIP-graph capture of whole steps (no reference counterpart: the reference
+runs on the CPU).  Any sequence of library calls on one mesh can be captured
+and replayed; the captured kernels keep the argument values of capture time,
+so a leapfrog step is captured once for the first (init) step and once for
+the later steps, whose Δt no longer changes.
+"""
+import ctypes
+
+from ._lib import call
+
+
+class Step_Graph:
+    def __init__(self, mesh, fn, nfields=None):
+        """Capture ``fn()`` (which must only enqueue library calls on `mesh`)."""
+        self.mesh = mesh
+        if nfields:
+            mesh.reserve(nfields)
+        mesh.synchronize()
+        call("jgcm_graph_begin", mesh.handle)
+        try:
+            fn()
+        except Exception:
+            g = ctypes.c_void_p()
+            try:
+                call("jgcm_graph_end", mesh.handle, ctypes.byref(g))
+                call("jgcm_graph_destroy", g)
+            except Exception:
+                pass
+            raise
+        g = ctypes.c_void_p()
+        call("jgcm_graph_end", mesh.handle, ctypes.byref(g))
+        self._g = g
+
+    def launch(self):
+        call("jgcm_graph_launch", self._g)
+
+    def close(self):
+        if getattr(self, "_g", None) is not None and self._g.value:
+            call("jgcm_graph_destroy", self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -28,6 +28,32 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+class _MeshTable(np.ndarray):
+    """Host table that remembers its mesh, so reference-style calls that pass
+    a table (e.g. ``Filtered_Leapfrog(..., mesh.laplacian_eig, ...)``) can
+    reach the device objects."""
+    mesh = None
+
+
+def _with_mesh(arr, mesh):
+    t = np.asarray(arr).view(_MeshTable)
+    t.mesh = mesh
+    return t
+
+
+_MESHES = []
+
+
+def current_mesh(num_fourier, nθ, nλ):
+    """The most recently created live mesh of this shape."""
+    for ref in reversed(_MESHES):
+        m = ref()
+        if m is not None and m._h is not None and (m.num_fourier, m.nθ, m.nλ) == (num_fourier, nθ, nλ):
+            return m
+    raise AssertionError(f"no Spectral_Spherical_Mesh with num_fourier={num_fourier}, "
+                         f"nθ={nθ}, nλ={nλ}")
+
+
 class Spectral_Spherical_Mesh:
     """``Spectral_Spherical_Mesh(num_fourier, num_spherical, nθ, nλ, nd, radius)``
     (SSM:69-139).  ``precision`` selects the Legendre-stage arithmetic
@@ -55,13 +81,15 @@ class Spectral_Spherical_Mesh:
         self._shapes = shapes
         self._tables = {}
         for k in ("sinθ", "cosθ", "wts", "λc", "θc", "λe", "θe", "laplacian_eig", "epsilon"):
-            setattr(self, k, self.table(k))
+            setattr(self, k, _with_mesh(self.table(k), self))
         wn = np.zeros((N1, S1), dtype=np.int64)
         for m in range(N1):
             wn[m, m:] = np.arange(m, S1)
         self.wave_numbers = wn                       # SSM:583-597
         # device copy of cos θ for host-side callers that need it
-        self.cosθ_dev = torch.tensor(self.cosθ, dtype=torch.float64, device=self.device)
+        self.cosθ_dev = torch.tensor(np.asarray(self.cosθ), dtype=torch.float64, device=self.device)
+        import weakref
+        _MESHES.append(weakref.ref(self))
 
     # ---- tables (host copies of the reference struct's fields) ----
     def table(self, name):
@@ -94,6 +122,10 @@ class Spectral_Spherical_Mesh:
 
     def synchronize(self):
         call("jgcm_mesh_synchronize", self._h)
+
+    def reserve(self, nfields):
+        """Grow the mesh scratch to `nfields` level-fields (needed before graph capture)."""
+        call("jgcm_mesh_reserve", self._h, int(nfields))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

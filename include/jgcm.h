@@ -157,6 +157,74 @@ int jgcm_add_horizontal_advection(jgcm_mesh_t* mesh, const double* spe_hs,
  * Synchronises the mesh stream and returns the value on the host.          */
 int jgcm_area_weighted_global_mean(jgcm_mesh_t* mesh, const double* grid, double* result);
 
+/* ---- time integration: Filtered_Leapfrog ------------------------------------
+ * Filtered_Leapfrog(robert_coef, damping_order, damping_coef, eigen, implicit_coef,
+ *                   Δt, init_step, start_time, end_time)   Time_Integrator.jl:20-38
+ * `eigen` is the mesh's laplacian_eig; the ∇^(2 order) damping table is
+ * damping_coef (eig / eig[m=0, n=num_spherical-1])^order (Time_Integrator.jl:31). */
+int jgcm_leapfrog_create(jgcm_mesh_t* mesh, double robert_coef, int damping_order,
+                         double damping_coef, double implicit_coef, int64_t dt, int init_step,
+                         int64_t start_time, int64_t end_time, jgcm_leapfrog_t** out);
+int jgcm_leapfrog_destroy(jgcm_leapfrog_t* lf);
+/* Update_Init_Step!(integrator)   Time_Integrator.jl:40-43 (error when called twice) */
+int jgcm_leapfrog_update_init_step(jgcm_leapfrog_t* lf);
+int jgcm_leapfrog_state(jgcm_leapfrog_t* lf, int* init_step, int64_t* dt, int64_t* time);
+int jgcm_leapfrog_set_time(jgcm_leapfrog_t* lf, int64_t time);
+int jgcm_leapfrog_damping(jgcm_leapfrog_t* lf, double* host_out, size_t count);
+/* Compute_Spectral_Damping!(integrator, Qc, Qp, δQ)   Time_Integrator.jl:58-81 */
+int jgcm_compute_spectral_damping(jgcm_leapfrog_t* lf, const double* Qc, const double* Qp,
+                                  double* dQ, int nlev);
+/* Filtered_Leapfrog!(integrator, δQ, Qp, Qc, Qn)      Time_Integrator.jl:86-101 */
+int jgcm_filtered_leapfrog(jgcm_leapfrog_t* lf, const double* dQ, const double* Qp, double* Qc,
+                           double* Qn, int nlev);
+/* Compute_Spectral_Damping! followed by Filtered_Leapfrog! in one pass (same results). */
+int jgcm_damped_leapfrog(jgcm_leapfrog_t* lf, double* dQ, const double* Qp, double* Qc,
+                         double* Qn, int nlev);
+/* Implicit_Correction!(integrator, div_c, div_p, h_c, h_p, h_0, δdiv, δh)
+ *                                         Shallow_Water_Dynamics.jl:25-54 */
+int jgcm_sw_implicit_correction(jgcm_leapfrog_t* lf, const double* div_c, const double* div_p,
+                                const double* h_c, const double* h_p, double h_0, double* ddiv,
+                                double* dh, int nlev);
+
+/* ---- grid-space tendency helpers of the steppers --------------------------- */
+/* Compute_Abs_Vor!(vor, coriolis, absvor) (Atmo_Data.jl:59-65, f = 2Ω sinθ :45) fused with
+ * δu (=|+=) absvor v, δv (=|+=) -absvor u  (Barotropic_Dynamics.jl:28-29,
+ * Shallow_Water_Dynamics.jl:129-130, Spectral_Dynamics.jl:300-301).
+ * grid_absvor may be NULL; with grid_du = grid_dv = NULL only absvor is written
+ * (plain Compute_Abs_Vor!).  accumulate: 0 assigns, 1 adds.                 */
+int jgcm_vorticity_flux(jgcm_mesh_t* mesh, const double* grid_vor, const double* grid_u,
+                        const double* grid_v, double omega, double* grid_absvor, double* grid_du,
+                        double* grid_dv, int nlev, int accumulate);
+/* out = a x + b y over `count` doubles (x or y may be NULL).  Reference broadcast
+ * updates such as spe_energy .= spe_kin + spe_h_c (Shallow_Water_Dynamics.jl:141). */
+int jgcm_axpby(jgcm_mesh_t* mesh, size_t count, double a, const double* x, double b,
+               const double* y, double* out);
+/* out = (accumulate ? out : 0) + sign * x * y, e.g. grid_δh .-= grid_h .* grid_div
+ * (Shallow_Water_Dynamics.jl:148).                                           */
+int jgcm_xmy(jgcm_mesh_t* mesh, size_t count, double sign, const double* x, const double* y,
+             double* out, int accumulate);
+/* grid_out = [grid_base +] 0.5 (u^2 + v^2)  (Shallow_Water_Dynamics.jl:139,
+ * Spectral_Dynamics.jl:307); grid_base may be NULL.                           */
+int jgcm_kinetic_energy(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
+                        const double* grid_base, double* grid_out, int nlev);
+/* Shallow_Water_Physics!(dyn_data, kappa_m, kappa_t, h_eq)  Shallow_Water_Dynamics.jl:7-21 */
+int jgcm_shallow_water_physics(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
+                               const double* grid_h, const double* h_eq, double kappa_m,
+                               double kappa_t, double* grid_du, double* grid_dv, double* grid_dh,
+                               int nlev);
+
+/* ---- graph capture of a sequence of calls on one mesh's stream ---------------
+ * No reference counterpart (the reference has no device).  Capture records
+ * every kernel the library enqueues on the mesh stream between begin and end;
+ * replays re-run them with the argument values of capture time.  Scratch must
+ * already be large enough (jgcm_mesh_reserve) and no synchronising call (e.g.
+ * jgcm_area_weighted_global_mean) may be made while capturing.              */
+int jgcm_mesh_reserve(jgcm_mesh_t* mesh, int nfields);
+int jgcm_graph_begin(jgcm_mesh_t* mesh);
+int jgcm_graph_end(jgcm_mesh_t* mesh, jgcm_graph_t** out);
+int jgcm_graph_launch(jgcm_graph_t* graph);
+int jgcm_graph_destroy(jgcm_graph_t* graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,6 +103,8 @@ def test_reduced_precision_legendre(jgcm, N, nlat, precision):
         jgcm.Trans_Grid_To_Spherical(dev, cuda(p_ref), sd)
         e_fwd = field_rel(sd.cpu().numpy(), sp)
         assert e_inv < 1e-6 and e_fwd < 1e-6, (e_inv, e_fwd)
+        # a reduced-precision path really ran (FP64 would sit near 1e-15)
+        assert e_inv > 1e-11 and e_fwd > 1e-11, (e_inv, e_fwd)
         if precision == "split32":
             assert e_inv < 2e-7 and e_fwd < 2e-7, (e_inv, e_fwd)
     finally:
