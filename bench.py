@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-hs", action="store_true")
+    p.add_argument("--hs-steps", type=int, default=50)
     return p.parse_args()
 
 
@@ -59,6 +61,79 @@ def cpu_baseline(seconds):
             "kind": "port",
             "sample": f"{reps} forward+inverse passes of 61 T85 level-fields "
                       f"(numpy FP64 oracle, 1 thread, {el:.1f} s)"}
+
+
+def cpu_hs_baseline(steps=3):
+    """The FP64 oracle's Held-Suarez step at T85 L20 (Δt = 300 s), 1 thread."""
+    from threadpoolctl import threadpool_limits
+    from oracle import FilteredLeapfrog, SpectralSphericalMesh
+    from oracle import dynamics as odyn
+    from oracle import primitive as P
+    N, nlat, nd, dt = 85, 128, 20, 300
+    with threadpool_limits(1):
+        M = SpectralSphericalMesh(N, N + 1, nlat, 2 * nlat, nd, 6371000.0)
+        vc = P.VertCoordinate(2 * nlat, nlat, nd, "even_sigma", "simmons_and_burridge",
+                              "second_centered_wts", 1e5)
+        atmo = odyn.AtmoData("hs", 2 * nlat, nlat, nd, True, True, False, False, M.sin_theta,
+                             6371000.0, 7.292e-5)
+        I = FilteredLeapfrog(0.04, 4, 1.15741e-4, M.laplacian_eig, 0.5, dt, True, 0, 86400)
+        si = P.SemiImplicitSolver(vc, atmo, I, 1e5, np.full(nd, 300.0), M.wave_numbers)
+        d = odyn.DynData("hs", N, N + 1, 2 * nlat, nlat, nd)
+        noise = 0.1 * np.random.default_rng(5).standard_normal((nd, nlat, 2 * nlat))
+        P.spectral_initialize_fields(M, atmo, vc, 1e5, 300.0, d.grid_geopots, d, grid_t_noise=noise)
+        P.atmosphere_update(M, atmo, vc, si, d, HS_PARAMS)
+        si.update_init_step()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            P.atmosphere_update(M, atmo, vc, si, d, HS_PARAMS)
+        el = (time.perf_counter() - t0) / steps
+    return {"s_per_step": el, "sim_days_per_day": dt / el, "cores": 1, "kind": "port",
+            "sample": f"{steps} Atmosphere_Update! steps of the numpy FP64 oracle, T85 L20"}
+
+
+HS_PARAMS = {"σ_b": 0.7, "k_f": 1.0, "k_a": 1.0 / 40.0, "k_s": 1.0 / 4.0, "ΔT_y": 60.0,
+             "Δθ_z": 10.0}
+
+
+def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):
+    """Held-Suarez primitive-equation step (Atmosphere_Update!) on the device,
+    T85 L20 with Δt = 300 s by default; isothermal 300 K at rest, ps = 1e5 Pa,
+    seeded N(0, 0.1 K) temperature noise.  Later steps are captured into one
+    HIP graph and replayed.  Returns ms/step and sim-days/day."""
+    nlon = 2 * nlat
+    a = 6371000.0
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, nlon, nd, a, precision=precision)
+    vc = jgcm.Vert_Coordinate(nlon, nlat, nd, "even_sigma", "simmons_and_burridge",
+                              "second_centered_wts", 1e5, mesh=mesh)
+    atmo = jgcm.Atmo_Data("hs", nlon, nlat, nd, True, True, False, False, mesh.sinθ, a, 7.292e-5)
+    integ = jgcm.Filtered_Leapfrog(0.04, 4, 1.15741e-4, mesh.laplacian_eig, 0.5, dt, True, 0,
+                                   200 * 86400)
+    si = jgcm.Semi_Implicit_Solver(vc, atmo, integ, 1e5, np.full(nd, 300.0), mesh.wave_numbers)
+    d = jgcm.Dyn_Data("hs", N, N + 1, nlon, nlat, nd, mesh=mesh)
+    noise = torch.from_numpy(0.1 * np.random.default_rng(5).standard_normal((nd, nlat, nlon))).cuda()
+    jgcm.Spectral_Initialize_Fields(mesh, atmo, vc, 1.0e5, 300.0, d.grid_geopots, d,
+                                    grid_t_noise=noise)
+    jgcm.Atmosphere_Update(mesh, atmo, vc, si, d, HS_PARAMS)
+    jgcm.Update_Init_Step(si)
+    g = jgcm.Step_Graph(mesh, lambda: jgcm.Atmosphere_Update(mesh, atmo, vc, si, d, HS_PARAMS),
+                        nfields=2 * nd)
+    for _ in range(warmup):
+        g.launch()
+    mesh.synchronize()
+    stream = torch.cuda.ExternalStream(mesh.stream())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        g.launch()
+    e1.record(stream)
+    mesh.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    t = d.grid_t_c
+    finite = bool(torch.isfinite(t).all())
+    g.close()
+    return {"ms_per_step": ms, "sim_days_per_day": dt / (ms * 1e-3), "dt_s": dt,
+            "config": f"T{N} L{nd} ({nlon}x{nlat})", "finite": finite,
+            "mean_T": float(t.mean())}
 
 
 def main():
@@ -167,8 +242,17 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "kernel": f"{dom} transform ({use})"},
     }
+    if not a.no_hs:
+        hs = {}
+        for prec in ("fp64", "split32"):
+            hs[prec] = hs_bench(jgcm, torch, 85, 128, 20, 300, a.hs_steps, 5, prec)
+        hs["t42_l20_fp64"] = hs_bench(jgcm, torch, 42, 64, 20, 600, a.hs_steps, 5, "fp64")
+        hs["split32_speedup_vs_fp64"] = hs["fp64"]["ms_per_step"] / hs["split32"]["ms_per_step"]
+        out["held_suarez"] = hs
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        if not a.no_hs:
+            out["cpu_baseline"]["held_suarez_t85_l20"] = cpu_hs_baseline()
     print(json.dumps(out))
 
 

@@ -51,6 +51,8 @@ SIGNATURES = {
     "jgcm_trans_grid_to_spherical": [P, P, P, I],
     "jgcm_trans_spherical_to_grid": [P, P, P, I],
     "jgcm_trans_grid_to_fourier": [P, P, P, I],
+    "jgcm_trans_grid_to_spherical_adjoint": [P, P, P, I],
+    "jgcm_trans_spherical_to_grid_adjoint": [P, P, P, I],
     "jgcm_divide_by_cos": [P, P, I],
     "jgcm_multiply_by_cos": [P, P, P, I],
     "jgcm_vor_div_from_grid_uv": [P, P, P, P, P, I],
@@ -79,6 +81,24 @@ SIGNATURES = {
     "jgcm_xmy": [P, SZ, D, P, P, P, I],
     "jgcm_kinetic_energy": [P, P, P, P, P, I],
     "jgcm_shallow_water_physics": [P, P, P, P, P, D, D, P, P, P, I],
+    "jgcm_map": [P, I, SZ, P, D, D, P],
+    "jgcm_vert_coord_create": [P, I, I, I, I, D, I, ctypes.POINTER(P)],
+    "jgcm_vert_coord_destroy": [P],
+    "jgcm_vert_coord_get": [P, P, P],
+    "jgcm_pressure_variables": [P, P, P, P, P, P, P],
+    "jgcm_compute_pressures_and_heights": [P, P, P, P, P, P, P, P, P, P, P, P],
+    "jgcm_compute_geopotential": [P, P, P, P, P, P, P, P],
+    "jgcm_vert_advection": [P, P, P, P, P],
+    "jgcm_four_in_one": [P, P] + [P] * 17,
+    "jgcm_semi_implicit_create": [P, P, P, D, P, ctypes.POINTER(P)],
+    "jgcm_semi_implicit_destroy": [P],
+    "jgcm_semi_implicit_update_init_step": [P],
+    "jgcm_semi_implicit_get": [P, I, P, SZ],
+    "jgcm_implicit_correction": [P] * 10,
+    "jgcm_hs_physics": [P, P, ctypes.c_int64] + [P] * 15 + [I],
+    "jgcm_hs_column_dynamics": [P] * 25,
+    "jgcm_hs_corrections": [P] * 8,
+    "jgcm_hs_scalars": [P, P, I],
     "jgcm_mesh_reserve": [P, I],
     "jgcm_graph_begin": [P],
     "jgcm_graph_end": [P, ctypes.POINTER(P)],

@@ -93,6 +93,28 @@ __device__ __forceinline__ void stage_r2(double2* buf, const double2* tw, int j,
 
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
 
+// Workgroup-wide gather: every thread first issues up to MAXIT independent
+// loads, then performs the matching stores, so MAXIT memory requests are in
+// flight per thread instead of one load-wait-store at a time.
+template <int MAXIT, typename LoadF, typename StoreF>
+__device__ __forceinline__ void staged_copy(int total, LoadF load, StoreF store) {
+  using T = decltype(load(0));
+  const int nt = blockDim.x;
+  for (int base = 0; base < total; base += MAXIT * nt) {
+    T tmp[MAXIT];
+#pragma unroll
+    for (int it = 0; it < MAXIT; ++it) {
+      int idx = base + it * nt + (int)threadIdx.x;
+      if (idx < total) tmp[it] = load(idx);
+    }
+#pragma unroll
+    for (int it = 0; it < MAXIT; ++it) {
+      int idx = base + it * nt + (int)threadIdx.x;
+      if (idx < total) store(idx, tmp[it]);
+    }
+  }
+}
+
 // Full FFT of buf[0..L) (natural order in and out), called by every thread
 // of the FFT group (gsync inside); thread j in [0, L/4) of the group works.  Radix sequence: 4, [2], 4, 4, ...
 template <int L, int SIGN>

@@ -113,6 +113,21 @@ struct jgcm_mesh {
   size_t four_elems() const { return (size_t)nm * 2 * half; }  // complex elements
 };
 
+// Filtered_Leapfrog state (Time_Integrator.jl:2-18); the damping table lives
+// on the device.
+struct jgcm_leapfrog {
+  jgcm_mesh* mesh = nullptr;
+  double robert_coef = 0.0;
+  int damping_order = 0;
+  double damping_coef = 0.0;
+  double implicit_coef = 0.0;
+  int64_t dt = 0;
+  bool init_step = true;
+  int64_t time = 0, start_time = 0, end_time = 0;
+  double* d_damping = nullptr;   // [N+1][ns+1]
+  std::vector<double> damping;
+};
+
 namespace jgcm {
 int ensure_scratch(jgcm_mesh* m, int nfields);
 int check_mesh(const jgcm_mesh* m);

@@ -14,19 +14,6 @@
 
 #include "jgcm_internal.hpp"
 
-struct jgcm_leapfrog {
-  jgcm_mesh* mesh = nullptr;
-  double robert_coef = 0.0;
-  int damping_order = 0;
-  double damping_coef = 0.0;
-  double implicit_coef = 0.0;
-  int64_t dt = 0;
-  bool init_step = true;
-  int64_t time = 0, start_time = 0, end_time = 0;
-  double* d_damping = nullptr;   // [N+1][ns+1]
-  std::vector<double> damping;
-};
-
 namespace jgcm {
 
 static inline int nblk(size_t n) {
@@ -159,6 +146,15 @@ __global__ void kinetic_energy_kernel(const double* __restrict__ u, const double
   GRID_STRIDE(i, n) {
     double k = 0.5 * (u[i] * u[i] + v[i] * v[i]);
     out[i] = base ? base[i] + k : k;
+  }
+}
+
+// out = f(x): 0 exp(x), 1 log(x), 2 a x + b
+__global__ void map_kernel(int op, const double* __restrict__ x, double a, double b,
+                           double* __restrict__ out, size_t n) {
+  GRID_STRIDE(i, n) {
+    double v = x[i];
+    out[i] = op == 0 ? exp(v) : (op == 1 ? log(v) : a * v + b);
   }
 }
 
@@ -335,6 +331,17 @@ int jgcm_xmy(jgcm_mesh_t* mesh, size_t count, double sign, const double* x, cons
   if (count == 0) return JGCM_OK;
   hipLaunchKernelGGL(xmy_kernel, dim3(nblk(count)), dim3(256), 0, mesh->stream, sign, x, y, out, count,
                      accumulate);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_map(jgcm_mesh_t* mesh, int op, size_t count, const double* x, double a, double b,
+             double* out) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(x && out, "NULL pointer");
+  JGCM_REQUIRE(op >= 0 && op <= 2, "unknown map op %d", op);
+  if (count == 0) return JGCM_OK;
+  hipLaunchKernelGGL(map_kernel, dim3(nblk(count)), dim3(256), 0, mesh->stream, op, x, a, b, out, count);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

@@ -119,12 +119,15 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
   const int j0 = blockIdx.x * ppw;
   for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
   const int total = nm * 2 * ppw;
-  for (int idx = threadIdx.x; idx < total; idx += 256) {
-    int jj = idx % ppw;
-    int mp = idx / ppw;
-    int jp = j0 + jj;
-    stage[jj * SP + mp] = (jp < half) ? eo[((size_t)b * nm * 2 + mp) * half + jp] : make_double2(0.0, 0.0);
-  }
+  staged_copy<8>(
+      total,
+      [&](int idx) {
+        int jj = idx % ppw;
+        int mp = idx / ppw;
+        int jp = j0 + jj;
+        return (jp < half) ? eo[((size_t)b * nm * 2 + mp) * half + jp] : make_double2(0.0, 0.0);
+      },
+      [&](int idx, double2 v) { stage[(idx % ppw) * SP + idx / ppw] = v; });
   __syncthreads();
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -290,17 +293,23 @@ __device__ __forceinline__ void stage_x(typename LdsT<PREC>::T* X, const double2
                                         int f0, int nf, int m, int par, int half, int nm,
                                         int mul_i, double sgn) {
   const int XS = half + 1;
-  for (int idx = threadIdx.x; idx < nf * half; idx += blockDim.x) {
-    int jp = idx % half;
-    int fl = idx / half;
-    double2 v = src[(((size_t)(f0 + fl) * nm + m) * 2 + par) * half + jp];
-    double re = v.x, im = v.y;
-    if (mul_i) { double t = re; re = -im; im = t; }
-    re *= sgn;
-    im *= sgn;
-    X[(2 * fl) * XS + jp] = to_lds<PREC>(re);
-    X[(2 * fl + 1) * XS + jp] = to_lds<PREC>(im);
-  }
+  staged_copy<8>(
+      nf * half,
+      [&](int idx) {
+        int jp = idx % half;
+        int fl = idx / half;
+        return src[(((size_t)(f0 + fl) * nm + m) * 2 + par) * half + jp];
+      },
+      [&](int idx, double2 v) {
+        int jp = idx % half;
+        int fl = idx / half;
+        double re = v.x, im = v.y;
+        if (mul_i) { double t = re; re = -im; im = t; }
+        re *= sgn;
+        im *= sgn;
+        X[(2 * fl) * XS + jp] = to_lds<PREC>(re);
+        X[(2 * fl + 1) * XS + jp] = to_lds<PREC>(im);
+      });
   for (int idx = threadIdx.x + nf * half; idx < (LCOLS / 2) * half; idx += blockDim.x) {
     int jp = idx % half;
     int fl = idx / half;
@@ -429,7 +438,7 @@ constexpr int IJB = 16;
 template <int PREC>
 __global__ __launch_bounds__(512) void legendre_inv_kernel(const double2* __restrict__ snm,
                                                            double2* __restrict__ eo,
-                                                           FwdTables tab, int B, int N, int ns,
+                                                           FwdTables tab, int B, int nmax, int ns,
                                                            int nm, int half,
                                                            const int* __restrict__ row_start) {
   using LT = typename LdsT<PREC>::T;
@@ -439,20 +448,28 @@ __global__ __launch_bounds__(512) void legendre_inv_kernel(const double2* __rest
   const int f0 = blockIdx.y * (LCOLS / 2);
   const int nf = min(LCOLS / 2, B - f0);
   const int nfirst = m + par;
-  const int nr = (nfirst <= ns) ? (ns - nfirst) / 2 + 1 : 0;
+  // degrees n = nfirst, nfirst + 2, ... <= nmax (nmax = N+1 for the inverse
+  // transform, N for the adjoint of the forward transform)
+  const int nr = (nfirst <= nmax) ? (nmax - nfirst) / 2 + 1 : 0;
   const int SS = nr + 1;
   LT* S = reinterpret_cast<LT*>(smem_raw);                          // [col][nr+1]
   double* Otile = reinterpret_cast<double*>(smem_raw + ((sizeof(LT) * LCOLS * SS + 15) / 16) * 16);  // [col][half+1]
   const int OS = half + 1;
   // stage the segment's spectral coefficients, column-major
-  for (int idx = threadIdx.x; idx < (LCOLS / 2) * nr; idx += blockDim.x) {
-    int k = idx % nr;
-    int fl = idx / nr;
-    double2 v = make_double2(0.0, 0.0);
-    if (fl < nf) v = snm[((size_t)(f0 + fl) * (ns + 1) + nfirst + 2 * k) * nm + m];
-    S[(2 * fl) * SS + k] = to_lds<PREC>(v.x);
-    S[(2 * fl + 1) * SS + k] = to_lds<PREC>(v.y);
-  }
+  staged_copy<8>(
+      (LCOLS / 2) * nr,
+      [&](int idx) {
+        int k = idx % nr;
+        int fl = idx / nr;
+        return fl < nf ? snm[((size_t)(f0 + fl) * (ns + 1) + nfirst + 2 * k) * nm + m]
+                       : make_double2(0.0, 0.0);
+      },
+      [&](int idx, double2 v) {
+        int k = idx % nr;
+        int fl = idx / nr;
+        S[(2 * fl) * SS + k] = to_lds<PREC>(v.x);
+        S[(2 * fl + 1) * SS + k] = to_lds<PREC>(v.y);
+      });
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -465,13 +482,28 @@ __global__ __launch_bounds__(512) void legendre_inv_kernel(const double2* __rest
     const int rs = row_start[m];
     const LT* Sc = S + lane * SS;
     if constexpr (PREC == P_FP64) {
-      for (int k = 0; k < nr; ++k) {
-        double s = Sc[k];
-        const auto* p = as_const(tab.d) + (size_t)(rs + nfirst + 2 * k - m) * half + j0;
-        if (jn == IJB) {
+      // rows of one parity are every other packed row: stride 2*half
+      const auto* p0 = as_const(tab.d) + (size_t)(rs + nfirst - m) * half + j0;
+      const size_t rstride = 2 * (size_t)half;
+      if (jn == IJB) {
+        int k = 0;
+        for (; k + 1 < nr; k += 2) {
+          double s0 = Sc[k], s1 = Sc[k + 1];
+          const auto* pa = p0 + k * rstride;
+          const auto* pb = pa + rstride;
 #pragma unroll
-          for (int q = 0; q < IJB; ++q) acc[q] = fma(p[q], s, acc[q]);
-        } else {
+          for (int q = 0; q < IJB; ++q) acc[q] = fma(pb[q], s1, fma(pa[q], s0, acc[q]));
+        }
+        if (k < nr) {
+          double s0 = Sc[k];
+          const auto* pa = p0 + k * rstride;
+#pragma unroll
+          for (int q = 0; q < IJB; ++q) acc[q] = fma(pa[q], s0, acc[q]);
+        }
+      } else {
+        for (int k = 0; k < nr; ++k) {
+          double s = Sc[k];
+          const auto* p = p0 + k * rstride;
           for (int q = 0; q < jn; ++q) acc[q] = fma(p[q], s, acc[q]);
         }
       }
@@ -617,31 +649,36 @@ static int launch_legendre_fwd(const jgcm_mesh* M, const FwdJobs& jobs, int njob
   return JGCM_OK;
 }
 
-int legendre_inv(const jgcm_mesh* M, const double2* snm, double2* eo, int B) {
+int legendre_inv_t(const jgcm_mesh* M, const double2* snm, double2* eo, int B, int table,
+                   int nmax) {
   int fchunks = (B + LCOLS / 2 - 1) / (LCOLS / 2);
   dim3 grd(2 * M->nm, fchunks);
   int waves = (M->half + IJB - 1) / IJB;
-  int nrmax = M->ns / 2 + 1;
-  FwdTables tab = tables_for(M, 3);
+  int nrmax = nmax / 2 + 1;
+  FwdTables tab = tables_for(M, table);
   size_t esz = M->precision == JGCM_PREC_FP64 ? 8 : (M->precision == JGCM_PREC_FP32 ? 4 : 8);
   size_t lds = ((esz * LCOLS * (nrmax + 1) + 15) / 16) * 16 + sizeof(double) * LCOLS * (M->half + 1);
   dim3 blk(64 * waves);
   switch (M->precision) {
     case JGCM_PREC_FP64:
       hipLaunchKernelGGL(legendre_inv_kernel<P_FP64>, grd, blk, lds, M->stream, snm, eo, tab, B,
-                         M->N, M->ns, M->nm, M->half, M->d_row_start);
+                         nmax, M->ns, M->nm, M->half, M->d_row_start);
       break;
     case JGCM_PREC_FP32:
       hipLaunchKernelGGL(legendre_inv_kernel<P_FP32>, grd, blk, lds, M->stream, snm, eo, tab, B,
-                         M->N, M->ns, M->nm, M->half, M->d_row_start);
+                         nmax, M->ns, M->nm, M->half, M->d_row_start);
       break;
     default:
       hipLaunchKernelGGL(legendre_inv_kernel<P_SPLIT>, grd, blk, lds, M->stream, snm, eo, tab, B,
-                         M->N, M->ns, M->nm, M->half, M->d_row_start);
+                         nmax, M->ns, M->nm, M->half, M->d_row_start);
       break;
   }
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
+}
+
+int legendre_inv(const jgcm_mesh* M, const double2* snm, double2* eo, int B) {
+  return legendre_inv_t(M, snm, eo, B, 3, M->ns);
 }
 
 int legendre_fwd_plain(const jgcm_mesh* M, const double2* eo, double2* snm, int B) {
@@ -663,6 +700,32 @@ int legendre_alpha(const jgcm_mesh* M, const double2* eoA, const double2* eoB, d
                    double2* alpha, int B) {
   FwdJobs jobs{};
   jobs.job[0] = FwdJob{eoA, tables_for(M, 1), 1, eoB, tables_for(M, 2), isign, alpha};
+  return launch_legendre_fwd(M, jobs, 1, B, M->ns, 0);
+}
+
+// E/O block scaled per wavenumber: m = 0 by s0, m >= 1 by s1
+__global__ void scale_eo_kernel(double2* eo, size_t total, int nm, int half, double s0, double s1) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    int m = (int)((i / (2 * (size_t)half)) % nm);
+    double s = m == 0 ? s0 : s1;
+    eo[i] = make_double2(eo[i].x * s, eo[i].y * s);
+  }
+}
+
+int scale_eo(const jgcm_mesh* M, double2* eo, int B, double s0, double s1) {
+  size_t total = (size_t)B * M->nm * 2 * M->half;
+  int blocks = (int)std::min<size_t>((total + 255) / 256, 8192);
+  hipLaunchKernelGGL(scale_eo_kernel, dim3(blocks), dim3(256), 0, M->stream, eo, total, M->nm,
+                     M->half, s0, s1);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+// transpose of the inverse Legendre stage: s[n] = sum_j' Pinv[n][j'] eo^(n-m)[j'], n = m..ns
+int legendre_inv_adjoint(const jgcm_mesh* M, const double2* eo, double2* snm, int B) {
+  FwdJobs jobs{};
+  jobs.job[0] = FwdJob{eo, tables_for(M, 3), 0, nullptr, FwdTables{}, 0.0, snm};
   return launch_legendre_fwd(M, jobs, 1, B, M->ns, 0);
 }
 
@@ -724,6 +787,35 @@ int jgcm_vor_div_from_grid_uv(jgcm_mesh_t* mesh, const double* grid_u, const dou
   if (int rc = fft_fold(mesh, grid_v, ev, mesh->d_cos, nlev)) return rc;
   return legendre_vordiv(mesh, eu, ev, reinterpret_cast<double2*>(vor),
                          reinterpret_cast<double2*>(div), nlev);
+}
+
+// Adjoint (transpose w.r.t. sum_ij x y on the grid and sum Re(conj(a) b) over
+// the spectral array) of Trans_Grid_To_Spherical!: weighted-transpose
+// Legendre (qwg/2 rows, degrees m..N), per-m scaling, conjugate FFT.
+int jgcm_trans_grid_to_spherical_adjoint(jgcm_mesh_t* mesh, const double* snm, double* pfield,
+                                         int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(pfield && snm, "NULL field pointer");
+  double2* eo = reinterpret_cast<double2*>(mesh->d_four_a);
+  if (int rc = legendre_inv_t(mesh, reinterpret_cast<const double2*>(snm), eo, nlev, 0, mesh->N)) return rc;
+  double L = mesh->nlon;
+  if (int rc = scale_eo(mesh, eo, nlev, 1.0 / L, 1.0 / (2.0 * L))) return rc;
+  return ifft_unfold(mesh, eo, pfield, nullptr, nlev);
+}
+
+// Adjoint of Trans_Spherical_To_Grid!: FFT with the transposed Hermitian
+// weights, then the transposed inverse Legendre over degrees m..N+1.  The
+// lower triangle (n < m) of the output is set to zero.
+int jgcm_trans_spherical_to_grid_adjoint(jgcm_mesh_t* mesh, const double* pfield, double* snm,
+                                         int nlev) {
+  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_REQUIRE(pfield && snm, "NULL field pointer");
+  double2* eo = reinterpret_cast<double2*>(mesh->d_four_a);
+  if (int rc = fft_fold(mesh, pfield, eo, nullptr, nlev)) return rc;
+  double L = mesh->nlon;
+  if (int rc = scale_eo(mesh, eo, nlev, L, 2.0 * L)) return rc;
+  if (int rc = jgcm_fill(mesh, snm, 0.0, (size_t)nlev * mesh->spec_elems() * 2)) return rc;
+  return legendre_inv_adjoint(mesh, eo, reinterpret_cast<double2*>(snm), nlev);
 }
 
 int jgcm_compute_alpha_operator(jgcm_mesh_t* mesh, const double* fourier_a, const double* fourier_b,

@@ -54,6 +54,27 @@ extern "C" {
 typedef struct jgcm_mesh jgcm_mesh_t;
 typedef struct jgcm_leapfrog jgcm_leapfrog_t;
 typedef struct jgcm_graph jgcm_graph_t;
+typedef struct jgcm_vert_coord jgcm_vert_coord_t;
+typedef struct jgcm_semi_implicit jgcm_semi_implicit_t;
+
+/* Atmo_Data scalars (Atmo_Data.jl:4-31); cp_air = rdgas / kappa (:46). */
+typedef struct jgcm_atmo_params {
+  double radius, omega, grav, rdgas, kappa, cp_air;
+  int do_mass_correction, do_energy_correction;
+} jgcm_atmo_params;
+
+/* Held-Suarez physics_params (exp/HSt42/Run_HS.jl:12, HS_Forcing.jl:8-13);
+ * rates in day^-1, temperatures in K. */
+typedef struct jgcm_hs_params {
+  double sigma_b, k_f, k_a, k_s, delta_T_y, delta_theta_z;
+} jgcm_hs_params;
+
+/* Vert_Coordinate options (Vert_Coordinate.jl:61-87, Press_And_Geopot.jl:63,
+ * Vert_Coordinate.jl:195-203). */
+#define JGCM_VC_EVEN_SIGMA 0
+#define JGCM_VD_SIMMONS_BURRIDGE 0
+#define JGCM_VA_SECOND_CENTERED_WTS 0
+#define JGCM_VA_SECOND_CENTERED 1
 
 const char* jgcm_last_error(void);
 int jgcm_device_count(int* count);
@@ -120,6 +141,18 @@ int jgcm_trans_spherical_to_grid(jgcm_mesh_t* mesh, const double* snm,
  *   fourier_g complex128 [nlev][nθ][nλ] = fft(pfield)/nλ.                    */
 int jgcm_trans_grid_to_fourier(jgcm_mesh_t* mesh, const double* pfield,
                                double* fourier_g, int nlev);
+
+/* Adjoints of the transform pair (no reference counterpart; the building
+ * block of the reverse-mode derivative of a step).  Transposes with respect
+ * to <x, y> = sum x y over grid values and <a, b> = sum Re(conj(a) b) over
+ * the spectral array: weighted-transpose Legendre plus conjugate FFT.
+ * grid_to_spherical_adjoint: spectral in -> grid out (degrees m..N read).
+ * spherical_to_grid_adjoint: grid in -> spectral out (degrees m..N+1
+ * written, lower triangle set to zero).                                     */
+int jgcm_trans_grid_to_spherical_adjoint(jgcm_mesh_t* mesh, const double* snm, double* pfield,
+                                         int nlev);
+int jgcm_trans_spherical_to_grid_adjoint(jgcm_mesh_t* mesh, const double* pfield, double* snm,
+                                         int nlev);
 
 /* ---- operators ---------------------------------------------------------- */
 /* Divide_By_Cos!(cosθ, grid)                    Spectral_Spherical_Mesh.jl:333-338 */
@@ -203,6 +236,13 @@ int jgcm_axpby(jgcm_mesh_t* mesh, size_t count, double a, const double* x, doubl
  * (Shallow_Water_Dynamics.jl:148).                                           */
 int jgcm_xmy(jgcm_mesh_t* mesh, size_t count, double sign, const double* x, const double* y,
              double* out, int accumulate);
+/* out = exp(x) (op 0, e.g. grid_ps_n .= exp.(grid_lnps), Spectral_Dynamics.jl:336),
+ * log(x) (op 1) or a x + b (op 2), over `count` doubles.                      */
+#define JGCM_MAP_EXP 0
+#define JGCM_MAP_LOG 1
+#define JGCM_MAP_AFFINE 2
+int jgcm_map(jgcm_mesh_t* mesh, int op, size_t count, const double* x, double a, double b,
+             double* out);
 /* grid_out = [grid_base +] 0.5 (u^2 + v^2)  (Shallow_Water_Dynamics.jl:139,
  * Spectral_Dynamics.jl:307); grid_base may be NULL.                           */
 int jgcm_kinetic_energy(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
@@ -212,6 +252,91 @@ int jgcm_shallow_water_physics(jgcm_mesh_t* mesh, const double* grid_u, const do
                                const double* grid_h, const double* h_eq, double kappa_m,
                                double kappa_t, double* grid_du, double* grid_dv, double* grid_dh,
                                int nlev);
+
+/* ---- primitive equations (Held-Suarez) ------------------------------------- */
+/* Vert_Coordinate(nλ, nθ, nd, vert_coord_option, vert_difference_option,
+ *                 vert_advect_scheme, p_ref, zero_top)   Vert_Coordinate.jl:42-56
+ * (even sigma only, :112-117; the uneven/hybrid options are broken in the
+ * reference, :69-76).  nd <= 64.                                           */
+int jgcm_vert_coord_create(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
+                           int vert_difference_option, int vert_advect_scheme, double p_ref,
+                           int zero_top, jgcm_vert_coord_t** out);
+int jgcm_vert_coord_destroy(jgcm_vert_coord_t* vc);
+/* host copies of ak, bk ([nd+1] each; either may be NULL) */
+int jgcm_vert_coord_get(jgcm_vert_coord_t* vc, double* ak, double* bk);
+/* Pressure_Variables!(vert_coord, ps, p_half, Δp, lnp_half, p_full, lnp_full)
+ *                                                       Press_And_Geopot.jl:36-67 */
+int jgcm_pressure_variables(jgcm_vert_coord_t* vc, const double* grid_ps, double* p_half,
+                            double* dp, double* lnp_half, double* p_full, double* lnp_full);
+/* Compute_Pressures_And_Heights!(atmo, vert_coord, ps, geopots, t, p_half, Δp,
+ *   lnp_half, p_full, lnp_full, z_full, z_half)          Press_And_Geopot.jl:3-19 */
+int jgcm_compute_pressures_and_heights(jgcm_vert_coord_t* vc, const jgcm_atmo_params* atmo,
+                                       const double* grid_ps, const double* grid_geopots,
+                                       const double* grid_t, double* p_half, double* dp,
+                                       double* lnp_half, double* p_full, double* lnp_full,
+                                       double* z_full, double* z_half);
+/* Compute_Geopotential!                                  Press_And_Geopot.jl:71-111 */
+int jgcm_compute_geopotential(jgcm_vert_coord_t* vc, const jgcm_atmo_params* atmo,
+                              const double* lnp_half, const double* lnp_full, const double* grid_t,
+                              const double* grid_geopots, double* geopot_full,
+                              double* geopot_half);
+/* Vert_Advection!(vert_coord, r, dz, w, Δt, scheme, rdt)  Vert_Coordinate.jl:171-210 */
+int jgcm_vert_advection(jgcm_vert_coord_t* vc, const double* r, const double* dz,
+                        const double* w, double* rdt);
+/* Four_In_One!(...)                                      Spectral_Dynamics.jl:88-183 */
+int jgcm_four_in_one(jgcm_vert_coord_t* vc, const jgcm_atmo_params* atmo, const double* grid_div,
+                     const double* grid_u, const double* grid_v, const double* grid_ps,
+                     const double* dp, const double* lnp_half, const double* lnp_full,
+                     const double* p_full, const double* grid_dlambda_ps,
+                     const double* grid_dtheta_ps, const double* grid_t, double* grid_M_half,
+                     double* grid_w_full, double* grid_du, double* grid_dv, double* grid_dps,
+                     double* grid_dt);
+/* Semi_Implicit_Solver(vert_coord, atmo_data, integrator, ps_ref, t_ref, wave_numbers)
+ *   Semi_Implicit.jl:40-160: reference profiles, ν, τ, γ, h, div_mat by unit
+ *   vectors (FP64, host) and the inverted wave matrices W_n uploaded.        */
+int jgcm_semi_implicit_create(jgcm_vert_coord_t* vc, const jgcm_atmo_params* atmo,
+                              jgcm_leapfrog_t* lf, double ps_ref, const double* t_ref,
+                              jgcm_semi_implicit_t** out);
+int jgcm_semi_implicit_destroy(jgcm_semi_implicit_t* si);
+/* Update_Init_Step!(semi_implicit)  Semi_Implicit.jl:182-188 */
+int jgcm_semi_implicit_update_init_step(jgcm_semi_implicit_t* si);
+/* host copies: 0 h[nd], 1 div_mat[nd][nd], 2 wave_matrix[N+1][nd][nd], 3 ν, 4 τ,
+ * 5 γ, 6 Δp_ref, 7 lnp_half_ref[nd+1], 8 lnp_full_ref */
+int jgcm_semi_implicit_get(jgcm_semi_implicit_t* si, int which, double* host_out, size_t count);
+/* Implicit_Correction!(semi_implicit, vert_coord, atmo_data, div_c, div_p,
+ *   lnps_c, lnps_p, t_c, t_p, δdiv, δlnps, δt)           Semi_Implicit.jl:462-513 */
+int jgcm_implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, const double* div_p,
+                             const double* lnps_c, const double* lnps_p, const double* t_c,
+                             const double* t_p, double* ddiv, double* dlnps, double* dt);
+/* Step part 1, one column pass: Pressure_Variables!(ps_c) (Spectral_Dynamics.jl:256),
+ * Spectral_Dynamics_Physics! / HS_Forcing! on the p-fields (:452-467,
+ * HS_Forcing.jl:1-130; skipped when do_forcing = 0, δps = 0 always) and
+ * Compute_Corrections_Init (:5-28, results kept on the device).            */
+int jgcm_hs_physics(jgcm_semi_implicit_t* si, const jgcm_hs_params* hs, int64_t dt,
+                    const double* grid_ps_c, const double* grid_u_p, const double* grid_v_p,
+                    const double* grid_t_p, const double* grid_ps_p, double* p_half, double* dp,
+                    double* lnp_half, double* p_full, double* lnp_full, double* grid_du,
+                    double* grid_dv, double* grid_t_eq, double* grid_dt, double* grid_dps,
+                    int do_forcing);
+/* Step part 2, one column pass (Spectral_Dynamics.jl:260-307 without the
+ * transforms): ∇ps *= ps, Four_In_One!, Compute_Geopotential!, δlnps = δps/ps,
+ * Vert_Advection! of u, v, T, absolute-vorticity flux, energy = Φ + KE.     */
+int jgcm_hs_column_dynamics(jgcm_semi_implicit_t* si, const double* grid_div, const double* grid_u,
+                            const double* grid_v, const double* grid_ps, const double* dp,
+                            const double* lnp_half, const double* lnp_full, const double* p_full,
+                            double* grid_dlambda_ps, double* grid_dtheta_ps, const double* grid_t,
+                            const double* grid_vor, const double* grid_geopots, double* grid_M_half,
+                            double* grid_w_full, double* grid_du, double* grid_dv,
+                            double* grid_dps, double* grid_dt, double* geopot_half,
+                            double* geopot_full, double* grid_dlnps, double* grid_absvor,
+                            double* grid_energy);
+/* ps_n = exp(lnps) then Compute_Corrections!  Spectral_Dynamics.jl:336-343, :30-68 */
+int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, double* grid_ps_n,
+                        const double* grid_u_n, const double* grid_v_n, double* grid_t_n,
+                        double* spe_lnps_n, double* spe_t_n);
+/* device correction scalars to the host: mean_ps_p, mean_energy_p, mean_ps_n,
+ * factor, log(factor), mean_energy_n, temperature correction (synchronises) */
+int jgcm_hs_scalars(jgcm_semi_implicit_t* si, double* host_out, int count);
 
 /* ---- graph capture of a sequence of calls on one mesh's stream ---------------
  * No reference counterpart (the reference has no device).  Capture records

@@ -1,0 +1,79 @@
+"""Adjoint transforms: dot-product tests <A x, y> = <x, A^T y> in FP64 and the
+torch autograd wrappers (the rrule counterpart) against central differences.
+A linear map's adjoint is pinned by the dot-product identity alone."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def rspec(N, nlev, rng, upper_only=True):
+    s = rng.standard_normal((nlev, N + 2, N + 1)) + 1j * rng.standard_normal((nlev, N + 2, N + 1))
+    if upper_only:
+        for m in range(N + 1):
+            s[:, :m, m] = 0
+    return s
+
+
+def dot_spec(a, b):
+    return float(np.sum(a.real * b.real + a.imag * b.imag))
+
+
+@pytest.mark.parametrize("N,nlat", [(21, 32), (42, 64), (85, 128)])
+def test_dot_product_identities(jgcm, N, nlat):
+    from jgcm_amd import adjoint
+    nlev = 3
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, nlev, 6.371e6)
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal((nlev, nlat, 2 * nlat))
+    s = rspec(N, nlev, rng)
+    xd, sd = torch.tensor(x, device="cuda"), torch.tensor(s, device="cuda")
+    # forward transform F: grid -> spectral
+    Fx = mesh.spectral_zeros()
+    jgcm.Trans_Grid_To_Spherical(mesh, xd, Fx)
+    FTs = mesh.grid_zeros()
+    adjoint.trans_grid_to_spherical_adjoint(mesh, sd, FTs)
+    torch.cuda.synchronize()
+    lhs = dot_spec(Fx.cpu().numpy(), s)
+    rhs = float(np.sum(x * FTs.cpu().numpy()))
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + abs(rhs)), (lhs, rhs)
+    # inverse transform G: spectral -> grid (uses degrees m..N+1)
+    Gs = mesh.grid_zeros()
+    jgcm.Trans_Spherical_To_Grid(mesh, sd, Gs)
+    GTx = mesh.spectral_zeros()
+    adjoint.trans_spherical_to_grid_adjoint(mesh, xd, GTx)
+    torch.cuda.synchronize()
+    lhs = float(np.sum(Gs.cpu().numpy() * x))
+    rhs = dot_spec(s, GTx.cpu().numpy())
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + abs(rhs)), (lhs, rhs)
+    g = GTx.cpu().numpy()
+    for m in range(N + 1):
+        assert not g[:, :m, m].any()
+
+
+def test_autograd_matches_central_differences(jgcm):
+    from jgcm_amd.adjoint import GridToSpherical, SphericalToGrid
+    N, nlat, nlev = 21, 32, 2
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, nlev, 6.371e6)
+    rng = np.random.default_rng(3)
+    x = torch.tensor(rng.standard_normal((nlev, nlat, 2 * nlat)), device="cuda", requires_grad=True)
+    w = torch.tensor(rspec(N, nlev, rng), device="cuda")
+
+    def loss(xx):
+        s = GridToSpherical.apply(mesh, xx)
+        y = SphericalToGrid.apply(mesh, s * w)
+        return (y * y).sum()
+
+    L = loss(x)
+    L.backward()
+    g = x.grad.detach().cpu().numpy()
+    for seed in range(4):
+        d = np.random.default_rng(100 + seed).standard_normal(x.shape)
+        d /= np.linalg.norm(d)
+        dd = torch.tensor(d, device="cuda")
+        h = 1e-3
+        with torch.no_grad():
+            fd = (loss(x + h * dd) - loss(x - h * dd)).item() / (2 * h)
+        an = float(np.sum(g * d))
+        assert abs(fd - an) <= 1e-7 * max(1.0, abs(an)), (fd, an)
