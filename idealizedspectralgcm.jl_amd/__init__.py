@@ -17,6 +17,7 @@ from .dyn_data import Atmo_Data, Compute_Abs_Vor, Dyn_Data, Time_Advance
 from .dynamics import (Barotropic_Dynamics, Shallow_Water_Physics, Implicit_Correction,
                        Shallow_Water_Dynamics)
 from .graph import Step_Graph
+from .output import Output_Manager, Update_Output, Finalize_Output, Zonal_Mean
 from .primitive import (Vert_Coordinate, Pressure_Variables, Compute_Pressures_And_Heights,
                         Compute_Geopotential, Vert_Advection, Four_In_One, Semi_Implicit_Solver,
                         Implicit_Correction_HS, Spectral_Dynamics_Physics, Spectral_Dynamics,

@@ -44,6 +44,7 @@ SIGNATURES = {
     "jgcm_mesh_destroy": [P],
     "jgcm_mesh_synchronize": [P],
     "jgcm_mesh_stream": [P],
+    "jgcm_mesh_share_stream": [P, P],
     "jgcm_mesh_set_precision": [P, I],
     "jgcm_mesh_get_precision": [P, ctypes.POINTER(I)],
     "jgcm_mesh_table_size": [P, I, ctypes.POINTER(SZ)],
@@ -99,6 +100,7 @@ SIGNATURES = {
     "jgcm_hs_column_dynamics": [P] * 25,
     "jgcm_hs_corrections": [P] * 8,
     "jgcm_hs_scalars": [P, P, I],
+    "jgcm_zonal_mean_accumulate": [P, P, P, P, P, P, I, P, P, P, P, P],
     "jgcm_mesh_reserve": [P, I],
     "jgcm_graph_begin": [P],
     "jgcm_graph_end": [P, ctypes.POINTER(P)],

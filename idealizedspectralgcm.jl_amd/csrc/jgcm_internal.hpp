@@ -70,6 +70,7 @@ struct jgcm_mesh {
   double radius = 0.0;
   int precision = JGCM_PREC_FP64;
   hipStream_t stream = nullptr;
+  bool stream_borrowed = false;   // stream owned by another mesh (jgcm_mesh_share_stream)
   int device = 0;
 
   // host copies of public fields

@@ -178,7 +178,7 @@ static void free_mesh(jgcm_mesh* m) {
                   m->d_grid_b, m->d_scalar};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  if (m->stream) (void)hipStreamDestroy(m->stream);
+  if (m->stream && !m->stream_borrowed) (void)hipStreamDestroy(m->stream);
   delete m;
 }
 
@@ -411,6 +411,19 @@ int jgcm_mesh_synchronize(jgcm_mesh_t* mesh) {
 }
 
 void* jgcm_mesh_stream(jgcm_mesh_t* mesh) { return mesh ? (void*)mesh->stream : nullptr; }
+
+int jgcm_mesh_share_stream(jgcm_mesh_t* mesh, jgcm_mesh_t* owner) {
+  if (int rc = check_mesh(mesh)) return rc;
+  if (int rc = check_mesh(owner)) return rc;
+  JGCM_REQUIRE(mesh != owner, "a mesh cannot share its own stream");
+  JGCM_REQUIRE(!mesh->stream_borrowed, "mesh already uses another mesh's stream");
+  JGCM_CHECK_HIP(hipStreamSynchronize(mesh->stream));
+  JGCM_CHECK_HIP(hipStreamSynchronize(owner->stream));
+  JGCM_CHECK_HIP(hipStreamDestroy(mesh->stream));
+  mesh->stream = owner->stream;
+  mesh->stream_borrowed = true;
+  return JGCM_OK;
+}
 
 int jgcm_mesh_set_precision(jgcm_mesh_t* mesh, int precision) {
   if (int rc = check_mesh(mesh)) return rc;

@@ -58,21 +58,45 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const double scale = 1.0 / L;
+  // all passes' rows are loaded up front (PASSES x 8 loads in flight per thread)
+  constexpr int PASSES = (L == 256) ? 2 : ((L == 512) ? 4 : ((L == 1024) ? 8 : 1));
+  double2 pre[PASSES][4];
+#pragma unroll
+  for (int q = 0; q < PASSES; ++q) {
+    int jj = q * FPW + g;
+    int jp = j0 + jj;
+    if (q * FPW < ppw && jj < ppw && jp < half) {
+      const double* ra = grid + ((size_t)b * nlat + jp) * L;
+      const double* rc = grid + ((size_t)b * nlat + (nlat - 1 - jp)) * L;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) pre[q][r] = make_double2(ra[t + r * T], rc[t + r * T]);
+    }
+  }
   for (int pass = 0; pass < ppw; pass += FPW) {
     int jj = pass + g;
     int jp = j0 + jj;
     bool valid = (jj < ppw) && (jp < half);
-    int jm = nlat - 1 - jp;
     if (valid) {
       double s = cosmul ? cosmul[jp] : 1.0;
-      const double* ra = grid + ((size_t)b * nlat + jp) * L;
-      const double* rc = grid + ((size_t)b * nlat + jm) * L;
+      double2 v[4];
+      if (pass / FPW < PASSES) {
+#pragma unroll
+        for (int q = 0; q < PASSES; ++q)
+          if (q == pass / FPW) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = pre[q][r];
+          }
+      } else {
+        const double* ra = grid + ((size_t)b * nlat + jp) * L;
+        const double* rc = grid + ((size_t)b * nlat + (nlat - 1 - jp)) * L;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = make_double2(ra[t + r * T], rc[t + r * T]);
+      }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int i = t + r * T;
-        double a = ra[i], c = rc[i];
+        double a = v[r].x, c = v[r].y;
         if (cosmul) { a = a * s; c = c * s; }
-        buf[g][i] = make_double2(a, c);
+        buf[g][t + r * T] = make_double2(a, c);
       }
     }
     gsync<L>();

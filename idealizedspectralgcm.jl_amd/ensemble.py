@@ -1,0 +1,148 @@
+"""Batched ensembles of independent shallow-water instances.
+
+No reference counterpart: the reference steps one model at a time.  All
+instances of one truncation are the "levels" of one mesh (the shallow-water
+step has no coupling between levels, so every kernel treats them as
+independent columns of one batch); the meshes of the different truncations
+share one HIP stream and one step of the whole mixed ensemble is captured
+into one graph.  Per-instance results are identical to running each instance
+alone, because every kernel computes each level-field with the same
+arithmetic whatever the batch size.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import call
+from .dyn_data import Atmo_Data, Dyn_Data
+from .dynamics import Shallow_Water_Physics, Shallow_Water_Dynamics
+from .graph import Step_Graph
+from .mesh import (_ptr, Spectral_Spherical_Mesh, Trans_Spherical_To_Grid, UV_Grid_From_Vor_Div,
+                   Vor_Div_From_Grid_UV)
+from .time_integrator import Filtered_Leapfrog, Update_Init_Step
+
+RADIUS, OMEGA = 6371.0e3, 7.292e-5
+
+
+def random_vorticity(N, count, rng, amplitude=1e-5):
+    """Spectral vorticity with |ζ_mn| ~ N(0, σ_n), σ_n ∝ n^(-3/2) for 1 <= n <= N,
+    m = 0 real, degree N+1 zero; normalised so the rms grid vorticity of an
+    instance is about `amplitude` s^-1."""
+    s = np.zeros((count, N + 2, N + 1), np.complex128)
+    n = np.arange(N + 2, dtype=float)
+    sig = np.zeros(N + 2)
+    sig[1:N + 1] = n[1:N + 1] ** -1.5
+    for m in range(N + 1):
+        k = np.arange(m, N + 1)
+        re = rng.standard_normal((count, k.size)) * sig[k]
+        im = rng.standard_normal((count, k.size)) * sig[k] if m > 0 else 0.0
+        s[:, k, m] = re + 1j * im
+    # rms of the grid field = sqrt(sum |c|^2 weights) with unit-normalised harmonics
+    power = np.sum(np.abs(s[:, :, 0]) ** 2, axis=1) + 2 * np.sum(np.abs(s[:, :, 1:]) ** 2, axis=(1, 2))
+    s *= (amplitude / np.sqrt(power))[:, None, None]
+    return s
+
+
+class SW_Group:
+    """`count` shallow-water instances at one truncation, stepped as one batch."""
+
+    def __init__(self, N, nθ, count, dt, seed, precision="fp32", h_0=3.0e4, owner=None,
+                 vor0=None):
+        nλ = 2 * nθ
+        self.N, self.count, self.dt, self.h_0 = N, count, dt, h_0
+        self.mesh = Spectral_Spherical_Mesh(N, N + 1, nθ, nλ, count, RADIUS, precision=precision)
+        if owner is not None:
+            call("jgcm_mesh_share_stream", self.mesh.handle, owner.handle)
+        m = self.mesh
+        self.atmo = Atmo_Data("sw", nλ, nθ, count, False, False, False, False, m.sinθ, RADIUS, OMEGA)
+        self.integ = Filtered_Leapfrog(0.04, 4, 1e-4, m.laplacian_eig, 0.5, dt, True, 0, 10 ** 9)
+        d = self.d = Dyn_Data("sw", N, N + 1, nλ, nθ, count, mesh=m)
+        if vor0 is None:
+            vor0 = random_vorticity(N, count, np.random.default_rng(seed))
+        d.spe_vor_c.copy_(torch.from_numpy(np.ascontiguousarray(vor0)))
+        self.h_eq = torch.full((count, nθ, nλ), h_0, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        # winds of the vorticity field, then linear balance ∇²Φ = ∇·(f ∇ψ)
+        UV_Grid_From_Vor_Div(m, d.spe_vor_c, d.spe_div_c, d.grid_u_c, d.grid_v_c)
+        zero = d.grid_d_full2
+        zero.zero_()
+        torch.cuda.synchronize()
+        # (f v, -f u) = f ∇ψ for a nondivergent flow; its divergence is ∇·(f∇ψ)
+        call("jgcm_vorticity_flux", m.handle, _ptr(zero), _ptr(d.grid_u_c), _ptr(d.grid_v_c),
+             OMEGA, None, _ptr(d.grid_δu), _ptr(d.grid_δv), count, 0)
+        Vor_Div_From_Grid_UV(m, d.grid_δu, d.grid_δv, d.spe_d1, d.spe_d2)
+        m.synchronize()
+        eig = torch.tensor(np.asarray(m.laplacian_eig).T, device="cuda")
+        inv = torch.where(eig != 0, 1.0 / torch.where(eig != 0, eig, torch.ones_like(eig)),
+                          torch.zeros_like(eig))
+        phi = d.spe_d2 * inv[None]
+        phi[:, 0, 0] += h_0                  # mean geopotential h_0 (P_0^0 = 1)
+        torch.cuda.synchronize()
+        # the SW step keeps h in the lnps/ps slots; one slot per instance
+        d.spe_lnps_n = torch.zeros_like(d.spe_vor_c)
+        d.spe_lnps_c = phi.contiguous()
+        d.spe_lnps_p = phi.clone()
+        d.spe_δlnps = torch.zeros_like(d.spe_vor_c)
+        for f in ("grid_ps_n", "grid_ps_c", "grid_ps_p", "grid_δlnps"):
+            setattr(d, f, torch.zeros_like(d.grid_u_c))
+        d.spe_vor_p.copy_(d.spe_vor_c)
+        torch.cuda.synchronize()
+        Trans_Spherical_To_Grid(m, d.spe_lnps_c, d.grid_ps_c)
+        Trans_Spherical_To_Grid(m, d.spe_vor_c, d.grid_vor)
+        call("jgcm_copy", m.handle, _ptr(d.grid_ps_p), _ptr(d.grid_ps_c),
+             d.grid_ps_c.numel() * 8)
+        m.synchronize()
+        self.steps = 0
+
+    def step(self):
+        """One shallow-water step of every instance of the group."""
+        d, m = self.d, self.mesh
+        Shallow_Water_Physics(d, 0.0, 0.0, self.h_eq)
+        Shallow_Water_Dynamics(m, self.atmo, self.h_0, d, self.integ)
+        if self.steps == 0:
+            Update_Init_Step(self.integ)
+        self.steps += 1
+
+
+class SW_Ensemble:
+    """Mixed-truncation ensemble: groups = [(N, nθ, count, dt), ...]."""
+
+    def __init__(self, groups, seed=0, precision="fp32"):
+        self.groups = []
+        owner = None
+        for i, (N, nθ, count, dt) in enumerate(groups):
+            g = SW_Group(N, nθ, count, dt, seed + 1000 * i, precision=precision,
+                         owner=owner.mesh if owner is not None else None)
+            if owner is None:
+                owner = g
+            self.groups.append(g)
+        self.owner = owner.mesh
+        self.graph = None
+
+    @property
+    def instances(self):
+        return sum(g.count for g in self.groups)
+
+    def step(self):
+        for g in self.groups:
+            g.step()
+
+    def capture(self):
+        """Capture one later step of all groups into one graph (after the first step)."""
+        assert all(g.steps >= 1 for g in self.groups), "take the first (init) step eagerly"
+        for g in self.groups:
+            g.mesh.reserve(4 * g.count)
+
+        def body():
+            for g in self.groups:
+                Shallow_Water_Physics(g.d, 0.0, 0.0, g.h_eq)
+                Shallow_Water_Dynamics(g.mesh, g.atmo, g.h_0, g.d, g.integ)
+
+        self.graph = Step_Graph(self.owner, body)
+        return self.graph
+
+    def launch(self):
+        self.graph.launch()
+        for g in self.groups:
+            g.steps += 1

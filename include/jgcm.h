@@ -99,6 +99,10 @@ int jgcm_mesh_destroy(jgcm_mesh_t* mesh);
 int jgcm_mesh_synchronize(jgcm_mesh_t* mesh);
 /* hipStream_t of the mesh, returned as void*. */
 void* jgcm_mesh_stream(jgcm_mesh_t* mesh);
+/* Make `mesh` enqueue on `owner`'s stream (so several meshes, e.g. the
+ * truncations of a mixed ensemble, can be captured into one graph).  The
+ * owner must outlive the mesh.  No reference counterpart.                   */
+int jgcm_mesh_share_stream(jgcm_mesh_t* mesh, jgcm_mesh_t* owner);
 int jgcm_mesh_set_precision(jgcm_mesh_t* mesh, int precision);
 int jgcm_mesh_get_precision(jgcm_mesh_t* mesh, int* precision);
 
@@ -337,6 +341,15 @@ int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, doubl
 /* device correction scalars to the host: mean_ps_p, mean_energy_p, mean_ps_n,
  * factor, log(factor), mean_energy_n, temperature correction (synchronises) */
 int jgcm_hs_scalars(jgcm_semi_implicit_t* si, double* host_out, int count);
+
+/* ---- diagnostics -------------------------------------------------------------
+ * Update_Output!(output_manager, dyn_data, current_time)  Output_Manager.jl:81-107:
+ * adds Zonal_Mean (:276-280) of T, T_eq, u, v ([nd][nθ] each) and ps
+ * ([nθ][nλ]) into the given day's accumulators, on the device.              */
+int jgcm_zonal_mean_accumulate(jgcm_mesh_t* mesh, const double* grid_t, const double* grid_t_eq,
+                               const double* grid_u, const double* grid_v, const double* grid_ps,
+                               int nd, double* acc_t, double* acc_t_eq, double* acc_u,
+                               double* acc_v, double* acc_ps);
 
 /* ---- graph capture of a sequence of calls on one mesh's stream ---------------
  * No reference counterpart (the reference has no device).  Capture records

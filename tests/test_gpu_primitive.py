@@ -82,3 +82,27 @@ def test_hs_steps_match_oracle(jgcm, N, nlat, nd, dt, steps):
     np.testing.assert_allclose(dsi.wave_matrix, si.wave_matrix, rtol=1e-11, atol=1e-13)
     sc = dsi.scalars()
     assert np.isfinite(sc).all()
+
+
+def test_update_output_matches_oracle(jgcm):
+    """Output_Manager.jl:81-145 on the device vs the oracle, two steps per 'day'."""
+    from oracle.output import OutputManager
+    N, nlat, nd, dt = 21, 32, 10, 600
+    (M, vc, atmo, si, d), (dev, dvc, datmo, dsi, dd) = build(jgcm, N, nlat, nd, dt)
+    om_r = OutputManager(2 * nlat, nlat, nd, vc.bk, 0, 2 * 86400, 1)
+    om_d = jgcm.Output_Manager(dev, dvc, 0, 2 * 86400, 1)
+    times = [600, 1200, 86400 + 600, 86400 + 1200]
+    for i, tnow in enumerate(times):
+        P.atmosphere_update(M, atmo, vc, si, d, HS)
+        jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS)
+        if i == 0:
+            si.update_init_step()
+            jgcm.Update_Init_Step(dsi)
+        om_r.update(d, tnow)
+        jgcm.Update_Output(om_d, dd, tnow)
+    om_r.finalize()
+    jgcm.Finalize_Output(om_d)
+    assert rel(om_d.t_daily_zonal_mean, om_r.t) < 1e-12
+    assert rel(om_d.u_daily_zonal_mean, om_r.u) < 1e-9
+    assert rel(om_d.ps_daily_mean, om_r.ps) < 1e-12
+    assert rel(om_d.t_zonal_mean, om_r.t_zonal_mean) < 1e-12
