@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-hs", action="store_true")
     p.add_argument("--hs-steps", type=int, default=50)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run the library's copy kernel over 256 MiB (PMC byte calibration)")
     return p.parse_args()
 
 
@@ -148,6 +150,18 @@ def main():
     from jgcm_amd import _lib
 
     mesh = jgcm.Spectral_Spherical_Mesh(N_TRUNC, N_TRUNC + 1, NLAT, NLON, NFIELDS, 6371.0e3)
+    if a.calibrate_copy:
+        # known traffic: 256 MiB read + 256 MiB written per launch
+        nbytes = 256 << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            _lib.call("jgcm_copy", mesh.handle, ctypes.c_void_p(dst.data_ptr()),
+                      ctypes.c_void_p(src.data_ptr()), nbytes)
+        mesh.synchronize()
+        print(json.dumps({"calibration": "jgcm_copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "launches": 5}))
+        return
     rng = np.random.default_rng(2024)
     s0 = rng.standard_normal((NFIELDS, N_TRUNC + 2, N_TRUNC + 1)) + \
         1j * rng.standard_normal((NFIELDS, N_TRUNC + 2, N_TRUNC + 1))
