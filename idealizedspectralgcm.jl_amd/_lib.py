@@ -40,6 +40,8 @@ SIGNATURES = {
     "jgcm_memcpy_d2h": [P, P, SZ],
     "jgcm_copy": [P, P, P, SZ],
     "jgcm_fill": [P, P, D, SZ],
+    "jgcm_time_advance": [P, I, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P),
+                          ctypes.POINTER(SZ)],
     "jgcm_mesh_create": [I, I, I, I, I, D, ctypes.POINTER(P)],
     "jgcm_mesh_destroy": [P],
     "jgcm_mesh_synchronize": [P],

@@ -33,6 +33,28 @@ __global__ void fill_kernel(double* __restrict__ dst, double v, size_t n) {
   GRID_STRIDE(i, n) dst[i] = v;
 }
 
+// Time_Advance! (Dyn_Data.jl:326-364): blockIdx.y = field; the same thread
+// moves c -> p and then n -> c for an element, so one launch covers all fields.
+constexpr int MAXADV = 16;
+struct AdvanceSet {
+  double* p[MAXADV];
+  double* c[MAXADV];
+  const double* n[MAXADV];
+  size_t count[MAXADV];
+};
+
+__global__ void time_advance_kernel(AdvanceSet s) {
+  const int f = blockIdx.y;
+  double* __restrict__ p = s.p[f];
+  double* __restrict__ c = s.c[f];
+  const double* __restrict__ n = s.n[f];
+  GRID_STRIDE(i, s.count[f]) {
+    double x = c[i];
+    p[i] = x;
+    c[i] = n[i];
+  }
+}
+
 // grid [B][nlat][nlon] scaled by cos θ_j: divide (SSM:333-338) or multiply (SSM:341-346)
 __global__ void cos_scale_kernel(const double* __restrict__ in, double* __restrict__ out,
                                  const double* __restrict__ cost, int nlat, int nlon, size_t n,
@@ -194,6 +216,28 @@ int jgcm_copy(jgcm_mesh_t* mesh, void* dst, const void* src, size_t bytes) {
     hipLaunchKernelGGL(copy_bytes_kernel, dim3(nblocks(bytes)), dim3(256), 0, mesh->stream,
                        (unsigned char*)dst, (const unsigned char*)src, bytes);
   }
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_time_advance(jgcm_mesh_t* mesh, int nfields, double* const* p, double* const* c,
+                      const double* const* n, const size_t* count) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nfields >= 1 && nfields <= MAXADV, "nfields must be in 1..%d, got %d", MAXADV, nfields);
+  JGCM_REQUIRE(p && c && n && count, "NULL argument");
+  AdvanceSet s{};
+  size_t maxc = 0;
+  for (int f = 0; f < nfields; ++f) {
+    JGCM_REQUIRE(p[f] && c[f] && n[f], "NULL field pointer (field %d)", f);
+    s.p[f] = p[f];
+    s.c[f] = c[f];
+    s.n[f] = n[f];
+    s.count[f] = count[f];
+    maxc = count[f] > maxc ? count[f] : maxc;
+  }
+  if (maxc == 0) return JGCM_OK;
+  hipLaunchKernelGGL(time_advance_kernel, dim3(nblocks(maxc, 1024), nfields), dim3(256), 0,
+                     mesh->stream, s);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

@@ -75,26 +75,9 @@ static VC vc_of(const jgcm_vert_coord* v) {
 }
 
 // ---------------------------------------------------------------- columns
-// Pressure_Variables! (Press_And_Geopot.jl:36-67) for one column; P = plane
-// size (nlat * nlon), c = column index.
-__device__ inline void pressure_column(const VC& v, double ps, double* p_half, double* dp,
-                                       double* lnp_half, double* p_full, double* lnp_full,
-                                       size_t P, size_t c) {
-  int nd = v.nd;
-  for (int k = 0; k <= nd; ++k) p_half[k * P + c] = v.ak[k] + v.bk[k] * ps;
-  for (int k = 0; k < nd; ++k) dp[k * P + c] = p_half[(k + 1) * P + c] - p_half[k * P + c];
-  int kt = v.zero_top ? 1 : 0;
-  for (int k = kt; k <= nd; ++k) lnp_half[k * P + c] = log(p_half[k * P + c]);
-  for (int k = kt; k < nd; ++k)
-    lnp_full[k * P + c] = lnp_half[(k + 1) * P + c] +
-                          p_half[k * P + c] * (lnp_half[(k + 1) * P + c] - lnp_half[k * P + c]) /
-                              dp[k * P + c] - 1.0;
-  if (v.zero_top) {
-    lnp_half[c] = 0.0;
-    lnp_full[c] = lnp_half[P + c] - 1.0;
-  }
-  for (int k = 0; k < nd; ++k) p_full[k * P + c] = exp(lnp_full[k * P + c]);
-}
+// Serial one-thread-per-column forms, used by the stand-alone entry points
+// (Compute_Geopotential!, Vert_Advection!, Four_In_One!).  The fused step
+// kernels further down use the level-parallel column tiles instead.
 
 // Compute_Geopotential! (Press_And_Geopot.jl:71-111); `scale` divides the
 // result (1/g for Compute_Pressures_And_Heights!, :3-19).
@@ -152,48 +135,6 @@ struct HSParams {
   double sigma_b, k_f, k_a, k_s, dT_y, dth_z;
 };
 
-// HS_Forcing! (HS_Forcing.jl:1-33) with Rayleigh_Damping! (:101-130) and
-// Newtonian_Damping! (:37-97) for one column.  Returns 1 if a σ assertion
-// (:85,122) fails.
-__device__ inline int hs_forcing_column(int nd, double kappa, double cp_air, double dt, double sinj,
-                                        const double* u, const double* v, const double* p_half,
-                                        const double* p_full, const double* t, double* du,
-                                        double* dv, double* t_eq, double* dtt, const HSParams& h,
-                                        size_t P, size_t c) {
-  const double day = 86400;
-  double kf = h.k_f / day;
-  double ps = p_half[nd * P + c];
-  int bad = 0;
-  for (int k = 0; k < nd; ++k) {
-    double s = p_full[k * P + c] / ps;
-    if ((s - h.sigma_b) / (1.0 - h.sigma_b) > 1.0) bad = 1;
-    double kv = kf / (1.0 - h.sigma_b) * fmax(0.0, s - h.sigma_b);
-    du[k * P + c] = -kv * u[k * P + c];
-    dv[k * P + c] = -kv * v[k * P + c];
-  }
-  for (int k = 0; k < nd; ++k) {
-    double uu = u[k * P + c], vv = v[k * P + c], a = du[k * P + c], b = dv[k * P + c];
-    dtt[k * P + c] = -((uu + 0.5 * a * dt) * a + (vv + 0.5 * b * dt) * b) / cp_air;
-  }
-  const double t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
-  double ka = h.k_a / day, ks = h.k_s / day;
-  double s2 = sinj * sinj;
-  double c2 = 1.0 - s2;
-  double c4 = c2 * c2;
-  for (int k = 0; k < nd; ++k) {
-    double pn = p_full[k * P + c] / p_ref;
-    double te = (t_zero - h.dT_y * s2 - h.dth_z * c2 * log(pn)) * pow(pn, kappa);
-    t_eq[k * P + c] = fmax(t_strat, te);
-  }
-  for (int k = 0; k < nd; ++k) {
-    double s = p_full[k * P + c] / ps;
-    if ((s - h.sigma_b) / (1.0 - h.sigma_b) > 1.0) bad = 1;
-    double kt = ka + (ks - ka) / (1.0 - h.sigma_b) * fmax(0.0, s - h.sigma_b) * c4;
-    dtt[k * P + c] = dtt[k * P + c] - kt * (t[k * P + c] - t_eq[k * P + c]);
-  }
-  return bad;
-}
-
 // Four_In_One! (Spectral_Dynamics.jl:88-183) for one column.
 __device__ inline void four_in_one_column(const VC& v, double rdgas, double cp_air,
                                           const double* div, const double* u, const double* vv,
@@ -233,125 +174,482 @@ __device__ inline void four_in_one_column(const VC& v, double rdgas, double cp_a
   M_half[nd * P + c] = 0.0;
 }
 
+// -------------------------------------------------------------- column tiles
+// The fused step kernels give each workgroup 64 consecutive columns (one per
+// lane) and nq = ceil(nd / KPT) waves; wave q keeps levels q*KPT ..
+// q*KPT + KPT - 1 of its 64 columns in registers, so the levels of a column
+// are worked on in parallel.  Vertical sums (mass flux divergence,
+// hydrostatic integral, column energy) add per-wave partial sums through LDS.
+// Sums over levels are therefore grouped by wave rather than strictly
+// sequential; the FP64 results differ from a sequential sum only by rounding.
+constexpr int TILE = 64;
+constexpr int MAXNQ = 8;
+
+struct TileIdx {
+  int lane, q, nq, k0;
+  size_t c;   // column (grid point or spectral coefficient)
+  bool col;   // column exists
+};
+
+template <int KPT>
+__device__ __forceinline__ TileIdx tile_idx(size_t ncol) {
+  TileIdx t;
+  t.lane = threadIdx.x & (TILE - 1);
+  t.q = threadIdx.x / TILE;
+  t.nq = blockDim.x / TILE;
+  t.k0 = t.q * KPT;
+  t.c = (size_t)blockIdx.x * TILE + t.lane;
+  t.col = t.c < ncol;
+  return t;
+}
+
+__device__ __forceinline__ double2 d2(double x, double y) { return make_double2(x, y); }
+__device__ __forceinline__ double2 d2add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 d2sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 d2scale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+
+// Sums of a per-thread partial over the waves above (smaller level index) and
+// below this one, per lane.  `red` holds nq * 64 entries; every thread of the
+// workgroup must call it (two barriers, `red` is free again on return).
+__device__ inline void level_sums(double2* red, const TileIdx& t, double2 part, double2& above,
+                                  double2& below) {
+  red[t.q * TILE + t.lane] = part;
+  __syncthreads();
+  double2 a = d2(0.0, 0.0), b = d2(0.0, 0.0);
+  for (int i = 0; i < t.nq; ++i) {
+    double2 x = red[i * TILE + t.lane];
+    if (i < t.q) a = d2add(a, x);
+    if (i > t.q) b = d2add(b, x);
+  }
+  __syncthreads();
+  above = a;
+  below = b;
+}
+
+// Sum over the 64 lanes of a wave in a fixed order (result in lane 0).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = TILE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, TILE);
+  return v;
+}
+
+// Pressure_Variables! (Press_And_Geopot.jl:36-67) for the levels of one tile:
+// ph/lnh at half levels k0 .. k0+KPT, dp/lnf/pf at full levels k0 .. k0+KPT-1.
+template <int KPT>
+struct PressTile {
+  double ph[KPT + 1], lnh[KPT + 1], dp[KPT], lnf[KPT], pf[KPT];
+};
+
+template <int KPT>
+__device__ __forceinline__ void pressure_tile(const VC& v, int k0, double ps, PressTile<KPT>& r) {
+#pragma unroll
+  for (int i = 0; i <= KPT; ++i) {
+    int k = min(k0 + i, v.nd);
+    r.ph[i] = v.ak[k] + v.bk[k] * ps;
+    r.lnh[i] = (v.zero_top && k == 0) ? 0.0 : log(r.ph[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    r.dp[i] = r.ph[i + 1] - r.ph[i];
+    if (v.zero_top && k0 + i == 0)
+      r.lnf[i] = r.lnh[i + 1] - 1.0;
+    else
+      r.lnf[i] = r.lnh[i + 1] + r.ph[i] * (r.lnh[i + 1] - r.lnh[i]) / r.dp[i] - 1.0;
+    r.pf[i] = exp(r.lnf[i]);
+  }
+}
+
+template <int KPT>
+__device__ __forceinline__ void store_pressure(const VC& v, const TileIdx& t, size_t P,
+                                               const PressTile<KPT>& r, double* p_half, double* dp,
+                                               double* lnp_half, double* p_full, double* lnp_full) {
+  if (!t.col) return;
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    int k = t.k0 + i;
+    if (k < v.nd) {
+      size_t o = (size_t)k * P + t.c;
+      p_half[o] = r.ph[i];
+      lnp_half[o] = r.lnh[i];
+      dp[o] = r.dp[i];
+      lnp_full[o] = r.lnf[i];
+      p_full[o] = r.pf[i];
+    }
+    if (k + 1 == v.nd) {
+      size_t o = (size_t)v.nd * P + t.c;
+      p_half[o] = r.ph[i + 1];
+      lnp_half[o] = r.lnh[i + 1];
+    }
+  }
+}
+
+// Compute_Geopotential! (Press_And_Geopot.jl:71-111) for one tile, given the
+// temperature and log-pressures of its levels: g_half[k] = Φs + sum_{k'>=k}
+// Rd T (lnp_half[k'+1] - lnp_half[k']), g_full[k] = g_half[k+1] + Rd T
+// (lnp_half[k+1] - lnp_full[k]); with zero_top, g_half[0] = 0.  Results are
+// divided by `grav_div` when it is non-zero (Compute_Pressures_And_Heights!).
+template <int KPT>
+__device__ __forceinline__ void geopotential_tile(const VC& v, const TileIdx& t, double2* red,
+                                                  double rdgas, const double (&T)[KPT],
+                                                  const double (&lnh)[KPT + 1],
+                                                  const double (&lnf)[KPT], double geopots,
+                                                  double (&gfull)[KPT], double (&ghalf)[KPT + 1],
+                                                  double grav_div) {
+  double term[KPT];
+  double part = 0.0;
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    term[i] = (t.k0 + i < v.nd) ? rdgas * T[i] * (lnh[i + 1] - lnh[i]) : 0.0;
+    part += term[i];
+  }
+  double2 above, below;
+  level_sums(red, t, d2(part, 0.0), above, below);
+  double acc = geopots + below.x;  // g_half at level k0 + KPT (or nd)
+  ghalf[KPT] = acc;
+#pragma unroll
+  for (int i = KPT - 1; i >= 0; --i) {
+    gfull[i] = acc + rdgas * T[i] * (lnh[i + 1] - lnf[i]);
+    acc = acc + term[i];
+    ghalf[i] = (v.zero_top && t.k0 + i == 0) ? 0.0 : acc;
+  }
+  if (grav_div != 0.0) {
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) gfull[i] = gfull[i] / grav_div;
+#pragma unroll
+    for (int i = 0; i <= KPT; ++i) ghalf[i] = ghalf[i] / grav_div;
+  }
+}
+
+// One workgroup's contribution to an area-weighted global mean: sum over its
+// 64 columns of value * w[latitude], written by lane 0 of wave 0.
+__device__ inline void block_area_partial(const TileIdx& t, int nlon, const double* w, double v0,
+                                          double v1, double2* partial) {
+  if (t.q != 0) return;
+  double wj = t.col ? w[t.c / nlon] : 0.0;
+  double s0 = wave_sum(t.col ? v0 * wj : 0.0);
+  double s1 = wave_sum(t.col ? v1 * wj : 0.0);
+  if (t.lane == 0) partial[blockIdx.x] = d2(s0, s1);
+}
+
 // ------------------------------------------------------------------ kernels
 // Step part 1 (Spectral_Dynamics_Physics!, SD:452-467; Compute_Corrections_Init,
-// SD:5-28; Pressure_Variables!, SD:256): pressure variables of ps_c, HS forcing
-// on the p-fields, δps = 0, and the column integrands of the corrections
-// (ps_p and sum_k E_k Δp_k(ps_p)).
-__global__ void hs_physics_kernel(VC v, int nlat, int nlon, const double* __restrict__ sint,
-                                  double kappa, double cp_air, double dt, HSParams hp,
-                                  const double* ps_c, const double* u_p, const double* v_p,
-                                  const double* t_p, const double* ps_p, double* p_half, double* dp,
-                                  double* lnp_half, double* p_full, double* lnp_full, double* du,
-                                  double* dv, double* t_eq, double* dtt, double* dps,
-                                  double* colsum, int forcing, int energy, int* bad) {
-  size_t P = (size_t)nlat * nlon;
-  GRID_STRIDE(c, P) {
-    int j = (int)(c / nlon);
-    pressure_column(v, ps_c[c], p_half, dp, lnp_half, p_full, lnp_full, P, c);
-    dps[c] = 0.0;
-    if (forcing) {
-      if (hs_forcing_column(v.nd, kappa, cp_air, dt, sint[j], u_p, v_p, p_half, p_full, t_p, du, dv,
-                            t_eq, dtt, hp, P, c))
-        atomicAdd(bad, 1);
+// SD:5-28; Pressure_Variables!, SD:256): pressure variables of ps_c, HS_Forcing!
+// (HS_Forcing.jl:1-33 with Rayleigh_Damping! :101-130 and Newtonian_Damping!
+// :37-97) on the p-fields, δps = 0, and the per-workgroup partial sums of the
+// corrections' area means (ps_p; sum_k E_k Δp_k(ps_p)).
+template <int KPT>
+__global__ __launch_bounds__(512) void hs_physics_kernel(
+    VC v, int nlat, int nlon, const double* __restrict__ sint, const double* __restrict__ wts,
+    double kappa, double cp_air, double dt, HSParams hp, const double* ps_c, const double* u_p,
+    const double* v_p, const double* t_p, const double* ps_p, double* p_half, double* dp,
+    double* lnp_half, double* p_full, double* lnp_full, double* du, double* dv, double* t_eq,
+    double* dtt, double* dps, double2* partial, int forcing, int mass, int energy, int* bad) {
+  __shared__ double2 red[MAXNQ * TILE];
+  const size_t P = (size_t)nlat * nlon;
+  const TileIdx t = tile_idx<KPT>(P);
+  const int nd = v.nd;
+  const int j = t.col ? (int)(t.c / nlon) : 0;
+  const double ps = t.col ? ps_c[t.c] : 1.0e5;
+  PressTile<KPT> pr;
+  pressure_tile<KPT>(v, t.k0, ps, pr);
+  store_pressure<KPT>(v, t, P, pr, p_half, dp, lnp_half, p_full, lnp_full);
+  if (t.col && t.q == 0) dps[t.c] = 0.0;
+
+  double U[KPT], V[KPT], T[KPT], DU[KPT], DV[KPT], DT[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    bool ok = t.col && t.k0 + i < nd;
+    size_t o = (size_t)(t.k0 + i) * P + t.c;
+    U[i] = ok ? u_p[o] : 0.0;
+    V[i] = ok ? v_p[o] : 0.0;
+    T[i] = ok ? t_p[o] : 0.0;
+    if (!forcing) {
+      DU[i] = ok ? du[o] : 0.0;
+      DV[i] = ok ? dv[o] : 0.0;
+      DT[i] = ok ? dtt[o] : 0.0;
     }
-    if (energy) {
-      double ps = ps_p[c];
-      double vi = 0.0;
-      for (int k = 0; k < v.nd; ++k) {
-        size_t o = k * P + c;
-        double a = u_p[o] + dt * du[o], b = v_p[o] + dt * dv[o];
-        double e = 0.5 * (a * a + b * b) + cp_air * (t_p[o] + dt * dtt[o]);
-        vi = vi + e * (v.dak[k] + v.dbk[k] * ps);
+  }
+  if (forcing) {
+    const double day = 86400;
+    const double t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
+    const double kf = hp.k_f / day, ka = hp.k_a / day, ks = hp.k_s / day;
+    const double sj = sint[j];
+    const double s2 = sj * sj, c2 = 1.0 - s2, c4 = c2 * c2;
+    const double psb = v.ak[nd] + v.bk[nd] * ps;  // p_half[nd]
+    int isbad = 0;
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      bool ok = t.col && t.k0 + i < nd;
+      double s = pr.pf[i] / psb;
+      if (ok && (s - hp.sigma_b) / (1.0 - hp.sigma_b) > 1.0) isbad = 1;
+      double kv = kf / (1.0 - hp.sigma_b) * fmax(0.0, s - hp.sigma_b);
+      double a = -kv * U[i], b = -kv * V[i];
+      DU[i] = a;
+      DV[i] = b;
+      double d = -((U[i] + 0.5 * a * dt) * a + (V[i] + 0.5 * b * dt) * b) / cp_air;
+      double pn = pr.pf[i] / p_ref;
+      double te = (t_zero - hp.dT_y * s2 - hp.dth_z * c2 * log(pn)) * pow(pn, kappa);
+      double teq = fmax(t_strat, te);
+      double kt = ka + (ks - ka) / (1.0 - hp.sigma_b) * fmax(0.0, s - hp.sigma_b) * c4;
+      DT[i] = d - kt * (T[i] - teq);
+      if (ok) {
+        size_t o = (size_t)(t.k0 + i) * P + t.c;
+        du[o] = DU[i];
+        dv[o] = DV[i];
+        t_eq[o] = teq;
+        dtt[o] = DT[i];
       }
-      colsum[c] = vi;
     }
+    if (isbad) atomicAdd(bad, 1);
   }
-}
-
-// Step part 2 (SD:260-307 minus the transforms): ∇ps = ∇lnps * ps, Four_In_One,
-// geopotential, δlnps = δps / ps, vertical advection of u, v, T, absolute
-// vorticity flux, energy = Φ + KE.
-__global__ void hs_column_dynamics_kernel(VC v, int nlat, int nlon, double rdgas, double cp_air,
-                                          double two_omega, const double* __restrict__ sint,
-                                          const double* div, const double* u, const double* vv,
-                                          const double* ps, const double* dp,
-                                          const double* lnp_half, const double* lnp_full,
-                                          const double* p_full, double* dlps, double* dtps,
-                                          const double* t, const double* vor,
-                                          const double* geopots, double* M_half, double* w_full,
-                                          double* du, double* dv, double* dps, double* dtt,
-                                          double* g_half, double* g_full, double* dlnps,
-                                          double* absvor, double* energy) {
-  size_t P = (size_t)nlat * nlon;
-  GRID_STRIDE(c, P) {
-    int j = (int)(c / nlon);
-    double p = ps[c];
-    double a = dlps[c] * p, b = dtps[c] * p;
-    dlps[c] = a;
-    dtps[c] = b;
-    four_in_one_column(v, rdgas, cp_air, div, u, vv, dp, lnp_half, lnp_full, p_full, a, b, t, M_half,
-                       w_full, du, dv, dps, dtt, P, c);
-    geopotential_column(v, rdgas, lnp_half, lnp_full, t, geopots[c], g_full, g_half, P, c, 0.0);
-    dlnps[c] = dps[c] / p;
-    vert_advection_add(v, u, dp, M_half, du, nullptr, P, c);
-    vert_advection_add(v, vv, dp, M_half, dv, nullptr, P, c);
-    vert_advection_add(v, t, dp, M_half, dtt, nullptr, P, c);
-    double f = two_omega * sint[j];
-    for (int k = 0; k < v.nd; ++k) {
-      size_t o = k * P + c;
-      double av = vor[o] + f;
-      absvor[o] = av;
-      double uk = u[o], vk = vv[o];
-      du[o] = du[o] + av * vk;
-      dv[o] = dv[o] - av * uk;
-      energy[o] = g_full[o] + 0.5 * (uk * uk + vk * vk);
-    }
-  }
-}
-
-// Step part 3 of the corrections (SD:336-343): ps_n = exp(lnps) (times the
-// mass factor when known), and column integrands of the energy correction.
-__global__ void hs_ps_energy_kernel(VC v, int nlat, int nlon, double cp_air, const double* lnps,
-                                    double* ps_n, const double* scal, int factor_idx,
-                                    const double* u, const double* vv, const double* t,
-                                    double* colsum, int energy) {
-  size_t P = (size_t)nlat * nlon;
-  GRID_STRIDE(c, P) {
-    double p;
-    if (factor_idx >= 0) {
-      p = ps_n[c] * scal[factor_idx];
-    } else {
-      p = exp(lnps[c]);
-    }
-    ps_n[c] = p;
+  if (mass || energy) {
+    double part = 0.0;
+    const double psp = t.col ? ps_p[t.c] : 0.0;
     if (energy) {
-      double vi = 0.0;
-      for (int k = 0; k < v.nd; ++k) {
-        size_t o = k * P + c;
-        double e = 0.5 * (u[o] * u[o] + vv[o] * vv[o]) + cp_air * t[o];
-        vi = vi + e * (v.dak[k] + v.dbk[k] * p);
+#pragma unroll
+      for (int i = 0; i < KPT; ++i) {
+        int k = t.k0 + i;
+        if (t.col && k < nd) {
+          double a = U[i] + dt * DU[i], b = V[i] + dt * DV[i];
+          double e = 0.5 * (a * a + b * b) + cp_air * (T[i] + dt * DT[i]);
+          part = part + e * (v.dak[k] + v.dbk[k] * psp);
+        }
       }
-      colsum[c] = vi;
     }
+    double2 above, below;
+    level_sums(red, t, d2(part, 0.0), above, below);
+    block_area_partial(t, nlon, wts, psp, part + below.x, partial);
   }
 }
 
-// scalar finishing steps (one thread); slots: 0 mean_ps_p, 1 mean_energy_p
-// (area mean of the column integral, divided by g here), 2 mean_ps_n,
-// 3 factor, 4 log(factor), 5 mean_energy_n, 6 temperature correction
-__global__ void hs_scalar_kernel(double* scal, int stage, double grav, double cp_air) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Finishes an area-weighted global mean from the per-workgroup partials (one
+// workgroup, fixed summation order) and the scalar steps of the corrections.
+// Scalar slots: 0 mean_ps_p, 1 mean_energy_p (mass-weighted integral, /g),
+// 2 mean_ps_n, 3 mass factor, 4 log(factor), 5 mean_energy_n, 6 temperature
+// correction.  stage 0: after the physics kernel (Compute_Corrections_Init);
+// stage 1: mean of ps_n (mass factor; with energy-only, stage 2 as well);
+// stage 2: energy of the corrected state.
+__global__ __launch_bounds__(256) void area_final_kernel(const double2* __restrict__ partial,
+                                                         int nparts, int nlon, double* scal,
+                                                         int stage, int mass, int energy,
+                                                         double grav, double cp_air) {
+  __shared__ double2 red[256];
+  double2 s = d2(0.0, 0.0);
+  for (int i = threadIdx.x; i < nparts; i += 256) s = d2add(s, partial[i]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = d2add(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double m0 = red[0].x / (2.0 * nlon), m1 = red[0].y / (2.0 * nlon);
   if (stage == 0) {
-    scal[1] = scal[1] / grav;
-  } else if (stage == 1) {
+    if (mass) scal[0] = m0;
+    if (energy) scal[1] = m1 / grav;
+    return;
+  }
+  if (stage == 1 && mass) {
+    scal[2] = m0;
     double f = scal[0] / scal[2];
     scal[3] = f;
     scal[4] = log(f);
-  } else if (stage == 2) {
-    scal[5] = scal[5] / grav;
+    return;
+  }
+  if (energy) {
+    scal[5] = m1 / grav;
     scal[6] = grav * (scal[1] - scal[5]) / (cp_air * scal[0]);
   }
+}
+
+// Step part 2 (SD:260-307 minus the transforms): ∇ps = ∇lnps * ps,
+// Four_In_One! (SD:88-183), Compute_Geopotential!, δlnps = δps / ps,
+// Vert_Advection! (Vert_Coordinate.jl:171-210) of u, v, T added to the
+// tendencies, absolute-vorticity flux (Compute_Abs_Vor!, SD:296-303) and
+// energy = Φ + KE (SD:306).
+template <int KPT>
+__global__ __launch_bounds__(512) void hs_column_dynamics_kernel(
+    VC v, int nlat, int nlon, double rdgas, double cp_air, double two_omega,
+    const double* __restrict__ sint, const double* div, const double* u, const double* vv,
+    const double* ps, const double* dp, const double* lnp_half, const double* lnp_full,
+    const double* p_full, double* dlps, double* dtps, const double* t, const double* vor,
+    const double* geopots, double* M_half, double* w_full, double* du, double* dv, double* dps,
+    double* dtt, double* g_half, double* g_full, double* dlnps, double* absvor, double* energy) {
+  __shared__ double2 red[MAXNQ * TILE];
+  const size_t P = (size_t)nlat * nlon;
+  const TileIdx tx = tile_idx<KPT>(P);
+  const int nd = v.nd;
+  const int k0 = tx.k0;
+  const size_t c = tx.c;
+  const double kappa = rdgas / cp_air;
+  const double p = tx.col ? ps[c] : 1.0;
+  const double a = tx.col ? dlps[c] * p : 0.0;
+  const double b = tx.col ? dtps[c] * p : 0.0;
+  const double ps_tend = tx.col ? dps[c] : 0.0;
+  const double gs = tx.col ? geopots[c] : 0.0;
+  const double f = two_omega * sint[tx.col ? (int)(c / nlon) : 0];
+  // levels k0-1 .. k0+KPT of u, v, T, Δp (index i+1 <-> level k0+i)
+  double U[KPT + 2], V[KPT + 2], T[KPT + 2], D[KPT + 2];
+#pragma unroll
+  for (int i = 0; i < KPT + 2; ++i) {
+    int k = k0 + i - 1;
+    bool ok = tx.col && k >= 0 && k < nd;
+    size_t o = (size_t)k * P + c;
+    U[i] = ok ? u[o] : 0.0;
+    V[i] = ok ? vv[o] : 0.0;
+    T[i] = ok ? t[o] : 0.0;
+    D[i] = ok ? dp[o] : 1.0;
+  }
+  double LH[KPT + 1], LF[KPT], PF[KPT], DIV[KPT], DU[KPT], DV[KPT], DT[KPT], VOR[KPT];
+#pragma unroll
+  for (int i = 0; i <= KPT; ++i) {
+    int k = min(k0 + i, nd);
+    LH[i] = tx.col ? lnp_half[(size_t)k * P + c] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    bool ok = tx.col && k0 + i < nd;
+    size_t o = (size_t)(k0 + i) * P + c;
+    LF[i] = ok ? lnp_full[o] : 0.0;
+    PF[i] = ok ? p_full[o] : 0.0;
+    DIV[i] = ok ? div[o] : 0.0;
+    DU[i] = ok ? du[o] : 0.0;
+    DV[i] = ok ? dv[o] : 0.0;
+    DT[i] = ok ? dtt[o] : 0.0;
+    VOR[i] = ok ? vor[o] : 0.0;
+  }
+
+  // Four_In_One!: pressure-gradient force, mass-flux divergence (prefix sum
+  // over levels), ω-α term and the σ-dot mass flux M_half.
+  double dl[KPT], dth[KPT], dmean[KPT];
+  double part = 0.0;
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    int k = k0 + i;
+    bool ok = tx.col && k < nd;
+    int kk = ok ? k : 0;
+    double Dp = D[i + 1];
+    double dlnp_p = LH[i + 1] - LF[i];
+    double dlnp_m = LF[i] - LH[i];
+    double x1 = (v.bk[kk] * dlnp_m + v.bk[kk + 1] * dlnp_p) / Dp;
+    dl[i] = x1 * a;
+    dth[i] = x1 * b;
+    DU[i] = DU[i] - rdgas * T[i + 1] * dl[i];
+    DV[i] = DV[i] - rdgas * T[i + 1] * dth[i];
+    dmean[i] = ok ? DIV[i] * Dp + v.dbk[kk] * (U[i + 1] * a + V[i + 1] * b) : 0.0;
+    part = part + dmean[i];
+  }
+  double2 above2, below2;
+  level_sums(red, tx, d2(part, 0.0), above2, below2);
+  const double total = above2.x + part + below2.x;
+  // M_half at half levels k0 .. k0+KPT
+  double Mh[KPT + 1];
+  Mh[0] = (k0 == 0 || k0 >= nd) ? 0.0 : -above2.x + total * v.bk[k0];
+  double run = above2.x;
+  double W[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    int k = k0 + i;
+    double dlnp_p = LH[i + 1] - LF[i];
+    double dlnp = LH[i + 1] - LH[i];
+    double Dp = D[i + 1];
+    double x5 = -(run * dlnp + dmean[i] * dlnp_p) / Dp + U[i + 1] * dl[i] + V[i + 1] * dth[i];
+    DT[i] = DT[i] + kappa * T[i + 1] * x5;
+    W[i] = x5 * PF[i];
+    run = run + dmean[i];
+    Mh[i + 1] = (k + 1 < nd) ? -run + total * v.bk[k + 1] : 0.0;
+  }
+  const double dps_new = ps_tend - total;
+  if (tx.col && tx.q == 0) {
+    dlps[c] = a;
+    dtps[c] = b;
+    dps[c] = dps_new;
+    dlnps[c] = dps_new / p;
+    M_half[c] = 0.0;
+  }
+
+  // geopotential
+  double Tk[KPT], LHk[KPT + 1], gfull[KPT], ghalf[KPT + 1];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) Tk[i] = T[i + 1];
+#pragma unroll
+  for (int i = 0; i <= KPT; ++i) LHk[i] = LH[i];
+  geopotential_tile<KPT>(v, tx, red, rdgas, Tk, LHk, LF, gs, gfull, ghalf, 0.0);
+
+  // vertical advection of u, v, T with w = M_half, dz = Δp
+  auto face = [&](const double (&R)[KPT + 2], int i) -> double {
+    // flux through half level k0+i (i = 0 .. KPT), between R[i] and R[i+1]
+    int h = k0 + i;
+    if (h <= 0 || h >= nd) return 0.0;
+    double r0 = R[i], r1 = R[i + 1];
+    if (v.scheme == 0) return Mh[i] * (r0 + (r1 - r0) * D[i] / (D[i] + D[i + 1]));
+    return Mh[i] * (r0 + r1) / 2.0;
+  };
+  double fu[KPT + 1], fv[KPT + 1], ft[KPT + 1];
+#pragma unroll
+  for (int i = 0; i <= KPT; ++i) {
+    fu[i] = face(U, i);
+    fv[i] = face(V, i);
+    ft[i] = face(T, i);
+  }
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    int k = k0 + i;
+    if (!(tx.col && k < nd)) continue;
+    size_t o = (size_t)k * P + c;
+    double dz = D[i + 1];
+    double wd = Mh[i + 1] - Mh[i];
+    double ru = (fu[i] - fu[i + 1] + U[i + 1] * wd) / dz;
+    double rv = (fv[i] - fv[i + 1] + V[i + 1] * wd) / dz;
+    double rt = (ft[i] - ft[i + 1] + T[i + 1] * wd) / dz;
+    double du_k = DU[i] + ru, dv_k = DV[i] + rv;
+    double av = VOR[i] + f;
+    double uk = U[i + 1], vk = V[i + 1];
+    du[o] = du_k + av * vk;
+    dv[o] = dv_k - av * uk;
+    dtt[o] = DT[i] + rt;
+    absvor[o] = av;
+    w_full[o] = W[i];
+    M_half[(size_t)(k + 1) * P + c] = Mh[i + 1];
+    g_full[o] = gfull[i];
+    g_half[o] = ghalf[i];
+    energy[o] = gfull[i] + 0.5 * (uk * uk + vk * vk);
+    if (k + 1 == nd) g_half[(size_t)nd * P + c] = ghalf[i + 1];
+  }
+}
+
+// Step part 3 of the corrections (SD:336-343): ps_n = exp(lnps) (stage 1) or
+// ps_n *= mass factor (stage 2), per-workgroup partials of the area mean of
+// ps_n and of the column energy integral of the new state.
+template <int KPT>
+__global__ __launch_bounds__(512) void hs_ps_energy_kernel(VC v, int nlat, int nlon, double cp_air,
+                                                           const double* __restrict__ wts,
+                                                           const double* lnps, double* ps_n,
+                                                           const double* scal, int use_factor,
+                                                           const double* u, const double* vv,
+                                                           const double* t, double2* partial,
+                                                           int energy) {
+  __shared__ double2 red[MAXNQ * TILE];
+  const size_t P = (size_t)nlat * nlon;
+  const TileIdx tx = tile_idx<KPT>(P);
+  double p = 0.0;
+  if (tx.col) p = use_factor ? ps_n[tx.c] * scal[3] : exp(lnps[tx.c]);
+  double part = 0.0;
+  if (energy) {
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      int k = tx.k0 + i;
+      if (tx.col && k < v.nd) {
+        size_t o = (size_t)k * P + tx.c;
+        double e = 0.5 * (u[o] * u[o] + vv[o] * vv[o]) + cp_air * t[o];
+        part = part + e * (v.dak[k] + v.dbk[k] * p);
+      }
+    }
+  }
+  double2 above, below;
+  level_sums(red, tx, d2(part, 0.0), above, below);  // also orders the ps_n read before the write
+  if (tx.col && tx.q == 0) ps_n[tx.c] = p;
+  block_area_partial(tx, nlon, wts, p, part + below.x, partial);
 }
 
 // apply the corrections (SD:41-57): T_n += corr, spe_t_n[k](0,0) += corr,
@@ -371,21 +669,43 @@ __global__ void hs_apply_corrections_kernel(int nd, size_t P, size_t spec_per_le
   }
 }
 
-// pressure variables and heights (SD:355 and Compute_Pressures_And_Heights!, SD:485)
-__global__ void hs_pressure_heights_kernel(VC v, int nlat, int nlon, double rdgas, double grav,
-                                           const double* ps, const double* t, const double* geopots,
-                                           double* p_half, double* dp, double* lnp_half,
-                                           double* p_full, double* lnp_full, double* z_full,
-                                           double* z_half) {
-  size_t P = (size_t)nlat * nlon;
-  GRID_STRIDE(c, P) {
-    pressure_column(v, ps[c], p_half, dp, lnp_half, p_full, lnp_full, P, c);
-    if (z_full) geopotential_column(v, rdgas, lnp_half, lnp_full, t, geopots[c], z_full, z_half, P, c, grav);
+// Pressure_Variables! (SD:355) and Compute_Pressures_And_Heights! (SD:485,
+// Press_And_Geopot.jl:3-19: geopotential / g) when z_full is given.
+template <int KPT>
+__global__ __launch_bounds__(512) void hs_pressure_heights_kernel(
+    VC v, int nlat, int nlon, double rdgas, double grav, const double* ps, const double* t,
+    const double* geopots, double* p_half, double* dp, double* lnp_half, double* p_full,
+    double* lnp_full, double* z_full, double* z_half) {
+  __shared__ double2 red[MAXNQ * TILE];
+  const size_t P = (size_t)nlat * nlon;
+  const TileIdx tx = tile_idx<KPT>(P);
+  PressTile<KPT> pr;
+  pressure_tile<KPT>(v, tx.k0, tx.col ? ps[tx.c] : 1.0e5, pr);
+  store_pressure<KPT>(v, tx, P, pr, p_half, dp, lnp_half, p_full, lnp_full);
+  if (!z_full) return;  // uniform over the grid
+  double T[KPT], gfull[KPT], ghalf[KPT + 1];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    bool ok = tx.col && tx.k0 + i < v.nd;
+    T[i] = ok ? t[(size_t)(tx.k0 + i) * P + tx.c] : 0.0;
+  }
+  geopotential_tile<KPT>(v, tx, red, rdgas, T, pr.lnh, pr.lnf, tx.col ? geopots[tx.c] : 0.0, gfull,
+                         ghalf, grav);
+  if (!tx.col) return;
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    int k = tx.k0 + i;
+    if (k < v.nd) {
+      size_t o = (size_t)k * P + tx.c;
+      z_full[o] = gfull[i];
+      z_half[o] = ghalf[i];
+      if (k + 1 == v.nd) z_half[(size_t)v.nd * P + tx.c] = ghalf[i + 1];
+    }
   }
 }
 
-// Semi-implicit correction (Semi_Implicit.jl:360-404 and :462-513) for one
-// (m, n) coefficient column over the nd levels.
+// Semi-implicit correction (Semi_Implicit.jl:360-404 and :462-513): one tile
+// column per (m, n) coefficient; complex values per level.
 struct SIRef {
   const double* dp;
   const double* lnp_half;
@@ -395,116 +715,171 @@ struct SIRef {
   const double* bk;
 };
 
-__device__ inline void linear_ps_t(int nd, double kappa, const SIRef& r, const double2* ddiv,
-                                   double2* dps, double2* dtv) {
-  // Linear_Ps_T_δdiv! (Semi_Implicit.jl:271-341)
-  double2 dmt = make_double2(0.0, 0.0);
-  double2 M[MAXND + 1];
-  for (int k = 0; k < nd; ++k) {
-    double dlp = r.lnp_half[k + 1] - r.lnp_full[k];
-    double dl = r.lnp_half[k + 1] - r.lnp_half[k];
-    double2 dm = make_double2(ddiv[k].x * r.dp[k], ddiv[k].y * r.dp[k]);
-    double s = -kappa * r.t_ref[k];
-    dtv[k] = make_double2(s * (dmt.x * dl + dm.x * dlp) / r.dp[k], s * (dmt.y * dl + dm.y * dlp) / r.dp[k]);
-    dmt = make_double2(dmt.x + dm.x, dmt.y + dm.y);
-    M[k + 1] = make_double2(-dmt.x, -dmt.y);
+// Linear_Ps_T_δdiv! (Semi_Implicit.jl:271-341) on one tile: δps and δT of
+// the linearised system for the divergence `A` of this tile's levels.
+template <int KPT>
+__device__ __forceinline__ void linear_ps_t_tile(int nd, double kappa, const SIRef& r,
+                                                 const TileIdx& t, double2* red,
+                                                 const double2 (&A)[KPT], double2& dps,
+                                                 double2 (&B)[KPT]) {
+  const int k0 = t.k0;
+  double2 dm[KPT];
+  double2 part = d2(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    int k = k0 + i;
+    dm[i] = (k < nd) ? d2scale(r.dp[k], A[i]) : d2(0.0, 0.0);
+    part = d2add(part, dm[i]);
   }
-  *dps = make_double2(-dmt.x, -dmt.y);
-  for (int k = 0; k < nd - 1; ++k)
-    M[k + 1] = make_double2(M[k + 1].x + dmt.x * r.bk[k + 1], M[k + 1].y + dmt.y * r.bk[k + 1]);
-  M[0] = make_double2(0.0, 0.0);
-  M[nd] = make_double2(0.0, 0.0);
-  double2 mdt_prev = make_double2(0.0, 0.0);  // Mdt_half[0]
-  for (int k = 0; k < nd; ++k) {
-    double2 mdt_next;
-    if (k + 1 < nd) {
-      double dtr = r.t_ref[k + 1] - r.t_ref[k];
-      mdt_next = make_double2(M[k + 1].x * dtr, M[k + 1].y * dtr);
+  double2 above, below;
+  level_sums(red, t, part, above, below);
+  const double2 total = d2add(d2add(above, part), below);
+  dps = d2(-total.x, -total.y);
+  double2 Mh[KPT + 1];
+  Mh[0] = (k0 == 0 || k0 >= nd) ? d2(0.0, 0.0) : d2add(d2(-above.x, -above.y), d2scale(r.bk[k0], total));
+  double2 run = above;
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    int k = k0 + i;
+    if (k < nd) {
+      double dlp = r.lnp_half[k + 1] - r.lnp_full[k];
+      double dl = r.lnp_half[k + 1] - r.lnp_half[k];
+      double s = -kappa * r.t_ref[k];
+      B[i] = d2(s * (run.x * dl + dm[i].x * dlp) / r.dp[k], s * (run.y * dl + dm[i].y * dlp) / r.dp[k]);
+      run = d2add(run, dm[i]);
+      Mh[i + 1] = (k + 1 < nd) ? d2add(d2(-run.x, -run.y), d2scale(r.bk[k + 1], total)) : d2(0.0, 0.0);
     } else {
-      mdt_next = make_double2(0.0, 0.0);
+      B[i] = d2(0.0, 0.0);
+      Mh[i + 1] = d2(0.0, 0.0);
     }
-    dtv[k] = make_double2(dtv[k].x - 0.5 * (mdt_next.x + mdt_prev.x) / r.dp[k],
-                          dtv[k].y - 0.5 * (mdt_next.y + mdt_prev.y) / r.dp[k]);
-    mdt_prev = mdt_next;
+  }
+  // M_half * ΔT_ref at half levels 1 .. nd-1 (zero at the top and bottom)
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    int k = k0 + i;
+    if (k >= nd) continue;
+    double2 next = (k + 1 < nd) ? d2scale(r.t_ref[k + 1] - r.t_ref[k], Mh[i + 1]) : d2(0.0, 0.0);
+    double2 prev = (k >= 1) ? d2scale(r.t_ref[k] - r.t_ref[k - 1], Mh[i]) : d2(0.0, 0.0);
+    B[i] = d2(B[i].x - 0.5 * (next.x + prev.x) / r.dp[k], B[i].y - 0.5 * (next.y + prev.y) / r.dp[k]);
   }
 }
 
-__device__ inline void linear_geopot(int nd, double rdgas, const SIRef& r, const double2* dtv,
-                                     double2* g) {
-  // Linear_Geopot_δt! (Semi_Implicit.jl:213-241)
-  double2 gh_next = make_double2(0.0, 0.0);  // half level nd
-  double2 gh[MAXND + 1];
-  gh[nd] = gh_next;
-  for (int k = nd - 1; k >= 1; --k) {
-    double d = r.lnp_half[k + 1] - r.lnp_half[k];
-    gh[k] = make_double2(gh[k + 1].x + rdgas * (dtv[k].x * d), gh[k + 1].y + rdgas * (dtv[k].y * d));
+// Linear_Geopot_δt! (Semi_Implicit.jl:213-241) on one tile.
+template <int KPT>
+__device__ __forceinline__ void linear_geopot_tile(int nd, double rdgas, const SIRef& r,
+                                                   const TileIdx& t, double2* red,
+                                                   const double2 (&B)[KPT], double2 (&G)[KPT]) {
+  const int k0 = t.k0;
+  double2 term[KPT];
+  double2 part = d2(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    int k = k0 + i;
+    if (k >= 1 && k < nd) {
+      double d = r.lnp_half[k + 1] - r.lnp_half[k];
+      term[i] = d2(rdgas * (B[i].x * d), rdgas * (B[i].y * d));
+    } else {
+      term[i] = d2(0.0, 0.0);
+    }
+    part = d2add(part, term[i]);
   }
-  for (int k = 0; k < nd; ++k) {
-    double d = r.lnp_half[k + 1] - r.lnp_full[k];
-    g[k] = make_double2(gh[k + 1].x + rdgas * (dtv[k].x * d), gh[k + 1].y + rdgas * (dtv[k].y * d));
+  double2 above, below;
+  level_sums(red, t, part, above, below);
+  double2 acc = below;  // geopotential at half level k0 + KPT
+#pragma unroll
+  for (int i = KPT - 1; i >= 0; --i) {
+    int k = k0 + i;
+    if (k < nd) {
+      double d = r.lnp_half[k + 1] - r.lnp_full[k];
+      G[i] = d2(acc.x + rdgas * (B[i].x * d), acc.y + rdgas * (B[i].y * d));
+    } else {
+      G[i] = d2(0.0, 0.0);
+    }
+    acc = d2add(acc, term[i]);
   }
 }
 
-__global__ __launch_bounds__(128) void hs_implicit_kernel(
+template <int KPT>
+__global__ __launch_bounds__(512) void hs_implicit_kernel(
     int nd, int nm, int ns, size_t spec_per_level, double kappa, double rdgas, double ps_ref,
     double xi, SIRef r, const double* __restrict__ eig, const double* __restrict__ wave,
     int num_wavenumbers, const double2* div_c, const double2* div_p, const double2* lnps_c,
     const double2* lnps_p, const double2* t_c, const double2* t_p, double2* ddiv, double2* dlnps,
     double2* dtt) {
-  GRID_STRIDE(idx, spec_per_level) {
-    int m = (int)(idx % nm);
-    int n = (int)(idx / nm);
-    double2 a[MAXND], b[MAXND], g[MAXND];
-    double2 dps;
-    // Adjust_δlnps_δt_δdiv!: δdiv_temp = div_p - div_c
-    for (int k = 0; k < nd; ++k) {
-      size_t o = k * spec_per_level + idx;
-      a[k] = make_double2(div_p[o].x - div_c[o].x, div_p[o].y - div_c[o].y);
-    }
-    linear_ps_t(nd, kappa, r, a, &dps, b);          // b = δt_temp, dps = δps_temp
-    double2 dl = dlnps[idx];
-    dl = make_double2(dl.x + dps.x / ps_ref, dl.y + dps.y / ps_ref);
-    for (int k = 0; k < nd; ++k) {
-      size_t o = k * spec_per_level + idx;
-      double2 d = dtt[o];
-      d = make_double2(d.x + b[k].x, d.y + b[k].y);
-      dtt[o] = d;
-      // δt_temp = t_p - t_c + ξ δt
-      a[k] = make_double2(t_p[o].x - t_c[o].x + xi * d.x, t_p[o].y - t_c[o].y + xi * d.y);
-    }
-    double2 lt = make_double2(lnps_p[idx].x - lnps_c[idx].x + xi * dl.x,
-                              lnps_p[idx].y - lnps_c[idx].y + xi * dl.y);
-    linear_geopot(nd, rdgas, r, a, g);
-    double e = eig[m * (ns + 1) + n];
-    for (int k = 0; k < nd; ++k) {
-      size_t o = k * spec_per_level + idx;
-      double hp = r.h[k] * ps_ref;
-      double2 d = ddiv[o];
-      a[k] = make_double2(d.x - e * (g[k].x + hp * lt.x), d.y - e * (g[k].y + hp * lt.y));
-    }
-    // wave-matrix solve for n <= num_wavenumbers (Semi_Implicit.jl:487-497)
-    if (n >= m && n <= num_wavenumbers) {
-      const double* W = wave + (size_t)n * nd * nd;
-      for (int k = 0; k < nd; ++k) {
-        double2 s = make_double2(0.0, 0.0);
-        for (int q = 0; q < nd; ++q) {
-          double w = W[k * nd + q];
-          s = make_double2(s.x + w * a[q].x, s.y + w * a[q].y);
-        }
-        b[k] = s;
-      }
-      for (int k = 0; k < nd; ++k) a[k] = b[k];
-    }
-    for (int k = 0; k < nd; ++k) ddiv[k * spec_per_level + idx] = a[k];
-    linear_ps_t(nd, kappa, r, a, &dps, b);
-    for (int k = 0; k < nd; ++k) {
-      size_t o = k * spec_per_level + idx;
-      double2 d = dtt[o];
-      dtt[o] = make_double2(d.x + xi * b[k].x, d.y + xi * b[k].y);
-    }
-    double f = xi / ps_ref;
-    dlnps[idx] = make_double2(dl.x + f * dps.x, dl.y + f * dps.y);
+  extern __shared__ double2 lds[];  // nd * 64 entries: level sums, then the wave-matrix operand
+  const TileIdx tx = tile_idx<KPT>(spec_per_level);
+  const size_t idx = tx.c;
+  const int m = tx.col ? (int)(idx % nm) : 0;
+  const int n = tx.col ? (int)(idx / nm) : 0;
+  const int k0 = tx.k0;
+  double2 A[KPT], B[KPT], G[KPT], DD[KPT];
+  // Adjust_δlnps_δt_δdiv!: δdiv_temp = div_p - div_c
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    bool ok = tx.col && k0 + i < nd;
+    size_t o = (size_t)(k0 + i) * spec_per_level + idx;
+    A[i] = ok ? d2sub(div_p[o], div_c[o]) : d2(0.0, 0.0);
   }
+  double2 dps;
+  linear_ps_t_tile<KPT>(nd, kappa, r, tx, lds, A, dps, B);  // B = δt_temp, dps = δps_temp
+  double2 dl = tx.col ? dlnps[idx] : d2(0.0, 0.0);
+  dl = d2(dl.x + dps.x / ps_ref, dl.y + dps.y / ps_ref);
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    bool ok = tx.col && k0 + i < nd;
+    size_t o = (size_t)(k0 + i) * spec_per_level + idx;
+    double2 d = ok ? dtt[o] : d2(0.0, 0.0);
+    d = d2add(d, B[i]);
+    DD[i] = d;
+    // δt_temp = t_p - t_c + ξ δt
+    A[i] = ok ? d2(t_p[o].x - t_c[o].x + xi * d.x, t_p[o].y - t_c[o].y + xi * d.y) : d2(0.0, 0.0);
+  }
+  const double2 lnc = tx.col ? lnps_c[idx] : d2(0.0, 0.0);
+  const double2 lnp = tx.col ? lnps_p[idx] : d2(0.0, 0.0);
+  const double2 lt = d2(lnp.x - lnc.x + xi * dl.x, lnp.y - lnc.y + xi * dl.y);
+  linear_geopot_tile<KPT>(nd, rdgas, r, tx, lds, A, G);
+  const double e = eig[m * (ns + 1) + n];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    int k = k0 + i;
+    bool ok = tx.col && k < nd;
+    size_t o = (size_t)k * spec_per_level + idx;
+    double hp = ok ? r.h[k] * ps_ref : 0.0;
+    double2 d = ok ? ddiv[o] : d2(0.0, 0.0);
+    A[i] = d2(d.x - e * (G[i].x + hp * lt.x), d.y - e * (G[i].y + hp * lt.y));
+  }
+  // wave-matrix solve for n <= num_wavenumbers (Semi_Implicit.jl:487-497):
+  // every level of the column is needed, so the operand goes through LDS
+#pragma unroll
+  for (int i = 0; i < KPT; ++i)
+    if (k0 + i < nd) lds[(k0 + i) * TILE + tx.lane] = A[i];
+  __syncthreads();
+  if (tx.col && n >= m && n <= num_wavenumbers) {
+    const double* Wn = wave + (size_t)n * nd * nd;
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      int k = k0 + i;
+      if (k >= nd) continue;
+      double2 s = d2(0.0, 0.0);
+      for (int qq = 0; qq < nd; ++qq) {
+        double w = Wn[k * nd + qq];
+        double2 x = lds[qq * TILE + tx.lane];
+        s = d2(s.x + w * x.x, s.y + w * x.y);
+      }
+      A[i] = s;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < KPT; ++i)
+    if (tx.col && k0 + i < nd) ddiv[(size_t)(k0 + i) * spec_per_level + idx] = A[i];
+  linear_ps_t_tile<KPT>(nd, kappa, r, tx, lds, A, dps, B);
+#pragma unroll
+  for (int i = 0; i < KPT; ++i)
+    if (tx.col && k0 + i < nd)
+      dtt[(size_t)(k0 + i) * spec_per_level + idx] = d2(DD[i].x + xi * B[i].x, DD[i].y + xi * B[i].y);
+  const double fx = xi / ps_ref;
+  if (tx.col && tx.q == 0) dlnps[idx] = d2(dl.x + fx * dps.x, dl.y + fx * dps.y);
 }
 
 __global__ void vert_advection_kernel(VC v, size_t P, const double* r, const double* dz,
@@ -530,37 +905,20 @@ __global__ void geopotential_kernel(VC v, size_t P, double rdgas, const double* 
   geopotential_column(v, rdgas, lnp_half, lnp_full, t, geopots[c], g_full, g_half, P, c, 0.0);
 }
 
-// area-weighted mean of two planes at once into scal[s0], scal[s1]
-__global__ __launch_bounds__(256) void area_mean2_kernel(const double* __restrict__ f0,
-                                                         const double* __restrict__ f1,
-                                                         const double* __restrict__ w, int nlat,
-                                                         int nlon, double* scal, int s0, int s1) {
-  __shared__ double red[2][256];
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < nlon; i += 256) {
-    double ca = 0.0, cb = 0.0;
-    for (int j = 0; j < nlat; ++j) {
-      if (f0) ca += f0[(size_t)j * nlon + i] * w[j];
-      if (f1) cb += f1[(size_t)j * nlon + i] * w[j];
-    }
-    a += ca;
-    b += cb;
-  }
-  red[0][threadIdx.x] = a;
-  red[1][threadIdx.x] = b;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + o];
-      red[1][threadIdx.x] += red[1][threadIdx.x + o];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (f0) scal[s0] = red[0][0] / (2.0 * nlon);
-    if (f1) scal[s1] = red[1][0] / (2.0 * nlon);
-  }
+// Tile shape for nd levels: KPT levels per thread, nd / KPT waves (<= MAXNQ).
+static int tile_kpt(int nd) { return nd <= 16 ? 2 : (nd <= 32 ? 4 : 8); }
+static dim3 tile_block(int nd) {
+  int kpt = tile_kpt(nd);
+  return dim3(TILE * ((nd + kpt - 1) / kpt));
 }
+static dim3 tile_grid(size_t ncol) { return dim3((unsigned)((ncol + TILE - 1) / TILE)); }
+
+#define JGCM_TILE_DISPATCH(nd, KERNEL, ...)                                          \
+  switch (tile_kpt(nd)) {                                                            \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                       \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                       \
+    default: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                      \
+  }
 
 // ----------------------------------------------------------- host helpers
 // in-place Gauss-Jordan inverse with partial pivoting (FP64)
@@ -728,9 +1086,9 @@ int jgcm_pressure_variables(jgcm_vert_coord_t* v, const double* grid_ps, double*
   JGCM_REQUIRE(v && grid_ps && p_half && dp && lnp_half && p_full && lnp_full, "NULL argument");
   jgcm_mesh* M = v->mesh;
   size_t P = M->grid_elems();
-  hipLaunchKernelGGL(hs_pressure_heights_kernel, dim3(nblk(P)), dim3(256), 0, M->stream, vc_of(v),
-                     M->nlat, M->nlon, 0.0, 0.0, grid_ps, nullptr, nullptr, p_half, dp, lnp_half,
-                     p_full, lnp_full, nullptr, nullptr);
+  JGCM_TILE_DISPATCH(v->nd, hs_pressure_heights_kernel, tile_grid(P), tile_block(v->nd), 0, M->stream,
+                     vc_of(v), M->nlat, M->nlon, 0.0, 0.0, grid_ps, nullptr, nullptr, p_half, dp,
+                     lnp_half, p_full, lnp_full, nullptr, nullptr);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
@@ -743,9 +1101,9 @@ int jgcm_compute_pressures_and_heights(jgcm_vert_coord_t* v, const jgcm_atmo_par
   JGCM_REQUIRE(v && atmo && grid_ps && grid_geopots && grid_t && z_full && z_half, "NULL argument");
   jgcm_mesh* M = v->mesh;
   size_t P = M->grid_elems();
-  hipLaunchKernelGGL(hs_pressure_heights_kernel, dim3(nblk(P)), dim3(256), 0, M->stream, vc_of(v),
-                     M->nlat, M->nlon, atmo->rdgas, atmo->grav, grid_ps, grid_t, grid_geopots,
-                     p_half, dp, lnp_half, p_full, lnp_full, z_full, z_half);
+  JGCM_TILE_DISPATCH(v->nd, hs_pressure_heights_kernel, tile_grid(P), tile_block(v->nd), 0, M->stream,
+                     vc_of(v), M->nlat, M->nlon, atmo->rdgas, atmo->grav, grid_ps, grid_t,
+                     grid_geopots, p_half, dp, lnp_half, p_full, lnp_full, z_full, z_half);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
@@ -900,12 +1258,28 @@ int jgcm_implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, cons
   jgcm_mesh* M = si->vc->mesh;
   size_t S = M->spec_elems();
   double xi = xi_of(si->lf);
-  hipLaunchKernelGGL(hs_implicit_kernel, dim3((unsigned)((S + 127) / 128)), dim3(128), 0, M->stream,
+  size_t lds = (size_t)si->nd * TILE * sizeof(double2);
+  JGCM_TILE_DISPATCH(si->nd, hs_implicit_kernel, tile_grid(S), tile_block(si->nd), lds, M->stream,
                      si->nd, M->nm, M->ns, S, si->atmo.kappa, si->atmo.rdgas, si->ps_ref, xi,
                      siref(si), M->d_eig, si->d_wave, si->num_wavenumbers, (const double2*)div_c,
                      (const double2*)div_p, (const double2*)lnps_c, (const double2*)lnps_p,
                      (const double2*)t_c, (const double2*)t_p, (double2*)ddiv, (double2*)dlnps,
                      (double2*)dt);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+// per-workgroup partial sums of the area means live in the column-sum scratch
+static double2* area_partials(const jgcm_semi_implicit* si) {
+  return reinterpret_cast<double2*>(si->d_colsum);
+}
+
+static int area_final(const jgcm_semi_implicit* si, int stage) {
+  const jgcm_mesh* M = si->vc->mesh;
+  int nparts = (int)tile_grid(M->grid_elems()).x;
+  hipLaunchKernelGGL(area_final_kernel, dim3(1), dim3(256), 0, M->stream, area_partials(si), nparts,
+                     M->nlon, si->d_scal, stage, si->atmo.do_mass_correction,
+                     si->atmo.do_energy_correction, si->atmo.grav, si->atmo.cp_air);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
@@ -922,22 +1296,17 @@ int jgcm_hs_physics(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t 
   jgcm_mesh* M = si->vc->mesh;
   size_t P = M->grid_elems();
   HSParams h{hp->sigma_b, hp->k_f, hp->k_a, hp->k_s, hp->delta_T_y, hp->delta_theta_z};
-  int energy = si->atmo.do_energy_correction;
+  int mass = si->atmo.do_mass_correction, energy = si->atmo.do_energy_correction;
   int* bad = reinterpret_cast<int*>(si->d_scal + 15);
-  hipLaunchKernelGGL(hs_physics_kernel, dim3(nblk(P)), dim3(256), 0, M->stream, vc_of(si->vc), M->nlat,
-                     M->nlon, M->d_sin, si->atmo.kappa, si->atmo.cp_air, (double)dt, h, grid_ps_c,
-                     grid_u_p, grid_v_p, grid_t_p, grid_ps_p, p_half, dp, lnp_half, p_full, lnp_full,
-                     grid_du, grid_dv, grid_t_eq, grid_dt, grid_dps, si->d_colsum, do_forcing,
-                     energy, bad);
+  int nd = si->nd;
+  JGCM_TILE_DISPATCH(nd, hs_physics_kernel, tile_grid(P), tile_block(nd), 0, M->stream,
+                     vc_of(si->vc), M->nlat, M->nlon, M->d_sin, M->d_wts, si->atmo.kappa,
+                     si->atmo.cp_air, (double)dt, h, grid_ps_c, grid_u_p, grid_v_p, grid_t_p,
+                     grid_ps_p, p_half, dp, lnp_half, p_full, lnp_full, grid_du, grid_dv, grid_t_eq,
+                     grid_dt, grid_dps, area_partials(si), do_forcing, mass, energy, bad);
   JGCM_LAUNCH_CHECK();
   // Compute_Corrections_Init (SD:5-28): area means into the scalar slots
-  hipLaunchKernelGGL(area_mean2_kernel, dim3(1), dim3(256), 0, M->stream,
-                     si->atmo.do_mass_correction ? grid_ps_p : nullptr, energy ? si->d_colsum : nullptr,
-                     M->d_wts, M->nlat, M->nlon, si->d_scal, 0, 1);
-  JGCM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(hs_scalar_kernel, dim3(1), dim3(64), 0, M->stream, si->d_scal, 0, si->atmo.grav,
-                     si->atmo.cp_air);
-  JGCM_LAUNCH_CHECK();
+  if (mass || energy) return area_final(si, 0);
   return JGCM_OK;
 }
 
@@ -953,12 +1322,13 @@ int jgcm_hs_column_dynamics(jgcm_semi_implicit_t* si, const double* grid_div, co
   JGCM_REQUIRE(si, "NULL argument");
   jgcm_mesh* M = si->vc->mesh;
   size_t P = M->grid_elems();
-  hipLaunchKernelGGL(hs_column_dynamics_kernel, dim3(nblk(P)), dim3(256), 0, M->stream, vc_of(si->vc),
-                     M->nlat, M->nlon, si->atmo.rdgas, si->atmo.cp_air, 2 * si->atmo.omega, M->d_sin,
-                     grid_div, grid_u, grid_v, grid_ps, dp, lnp_half, lnp_full, p_full,
-                     grid_dlambda_ps, grid_dtheta_ps, grid_t, grid_vor, grid_geopots, grid_M_half,
-                     grid_w_full, grid_du, grid_dv, grid_dps, grid_dt, geopot_half, geopot_full,
-                     grid_dlnps, grid_absvor, grid_energy);
+  int nd = si->nd;
+  JGCM_TILE_DISPATCH(nd, hs_column_dynamics_kernel, tile_grid(P), tile_block(nd), 0, M->stream,
+                     vc_of(si->vc), M->nlat, M->nlon, si->atmo.rdgas, si->atmo.cp_air,
+                     2 * si->atmo.omega, M->d_sin, grid_div, grid_u, grid_v, grid_ps, dp, lnp_half,
+                     lnp_full, p_full, grid_dlambda_ps, grid_dtheta_ps, grid_t, grid_vor,
+                     grid_geopots, grid_M_half, grid_w_full, grid_du, grid_dv, grid_dps, grid_dt,
+                     geopot_half, geopot_full, grid_dlnps, grid_absvor, grid_energy);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
@@ -971,29 +1341,23 @@ int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, doubl
   jgcm_mesh* M = si->vc->mesh;
   size_t P = M->grid_elems();
   int mass = si->atmo.do_mass_correction, energy = si->atmo.do_energy_correction;
+  int nd = si->nd;
   VC v = vc_of(si->vc);
-  // ps_n = exp(lnps)
-  hipLaunchKernelGGL(hs_ps_energy_kernel, dim3(nblk(P)), dim3(256), 0, M->stream, v, M->nlat, M->nlon,
-                     si->atmo.cp_air, grid_lnps, grid_ps_n, si->d_scal, -1, grid_u_n, grid_v_n,
-                     grid_t_n, si->d_colsum, mass ? 0 : energy);
+  // ps_n = exp(lnps); without the mass correction the energy integrand of this ps_n
+  JGCM_TILE_DISPATCH(nd, hs_ps_energy_kernel, tile_grid(P), tile_block(nd), 0, M->stream, v, M->nlat,
+                     M->nlon, si->atmo.cp_air, M->d_wts, grid_lnps, grid_ps_n, si->d_scal, 0,
+                     grid_u_n, grid_v_n, grid_t_n, area_partials(si), mass ? 0 : energy);
   JGCM_LAUNCH_CHECK();
+  if (mass || energy)
+    if (int rc = area_final(si, 1)) return rc;
   if (mass) {
-    hipLaunchKernelGGL(area_mean2_kernel, dim3(1), dim3(256), 0, M->stream, grid_ps_n, nullptr, M->d_wts,
-                       M->nlat, M->nlon, si->d_scal, 2, 0);
-    hipLaunchKernelGGL(hs_scalar_kernel, dim3(1), dim3(64), 0, M->stream, si->d_scal, 1,
-                       si->atmo.grav, si->atmo.cp_air);
     // ps_n *= factor and the energy integrand with the corrected ps
-    hipLaunchKernelGGL(hs_ps_energy_kernel, dim3(nblk(P)), dim3(256), 0, M->stream, v, M->nlat, M->nlon,
-                       si->atmo.cp_air, grid_lnps, grid_ps_n, si->d_scal, 3, grid_u_n, grid_v_n,
-                       grid_t_n, si->d_colsum, energy);
+    JGCM_TILE_DISPATCH(nd, hs_ps_energy_kernel, tile_grid(P), tile_block(nd), 0, M->stream, v,
+                       M->nlat, M->nlon, si->atmo.cp_air, M->d_wts, grid_lnps, grid_ps_n,
+                       si->d_scal, 1, grid_u_n, grid_v_n, grid_t_n, area_partials(si), energy);
     JGCM_LAUNCH_CHECK();
-  }
-  if (energy) {
-    hipLaunchKernelGGL(area_mean2_kernel, dim3(1), dim3(256), 0, M->stream, nullptr, si->d_colsum,
-                       M->d_wts, M->nlat, M->nlon, si->d_scal, 0, 5);
-    hipLaunchKernelGGL(hs_scalar_kernel, dim3(1), dim3(64), 0, M->stream, si->d_scal, 2,
-                       si->atmo.grav, si->atmo.cp_air);
-    JGCM_LAUNCH_CHECK();
+    if (energy)
+      if (int rc = area_final(si, 2)) return rc;
   }
   if (mass || energy) {
     size_t n = (size_t)si->nd * P;

@@ -36,6 +36,42 @@ namespace jgcm {
 
 enum { P_FP64 = 0, P_FP32 = 1, P_SPLIT = 2 };
 
+// A launch may gather its level-fields from up to MAXSEG separate arrays
+// (e.g. vor, div, u cos, v cos, T and ln ps back-transformed together):
+// global field b lives in segment k with start[k] <= b < start[k+1], at
+// ptr[k] + (b - start[k]) * per-field size.  flag[k] selects the cos θ
+// multiply (forward FFT) or divide (inverse FFT) for that segment.
+constexpr int MAXSEG = 8;
+struct SegMap {
+  int nseg;
+  int start[MAXSEG + 1];
+  double* ptr[MAXSEG];
+  int flag[MAXSEG];
+};
+
+__host__ __device__ __forceinline__ int seg_find(const SegMap& s, int b) {
+  int k = 0;
+#pragma unroll
+  for (int q = 1; q < MAXSEG; ++q)
+    if (q < s.nseg && b >= s.start[q]) k = q;
+  return k;
+}
+
+__host__ __device__ __forceinline__ double* seg_field(const SegMap& s, int b, size_t elems) {
+  int k = seg_find(s, b);
+  return s.ptr[k] + (size_t)(b - s.start[k]) * elems;
+}
+
+static inline SegMap one_seg(const void* p, int nlev, int flag) {
+  SegMap s{};
+  s.nseg = 1;
+  s.start[0] = 0;
+  s.start[1] = nlev;
+  s.ptr[0] = const_cast<double*>(reinterpret_cast<const double*>(p));
+  s.flag[0] = flag;
+  return s;
+}
+
 // ---------------------------------------------------------------- FFT stage
 template <int L>
 __global__ __launch_bounds__(256) void fft_fold_kernel(

@@ -87,16 +87,17 @@ class Dyn_Data:
             setattr(self, f, gr(nd + 1))
 
 
-def _copy(mesh, dst, src):
-    call("jgcm_copy", mesh.handle, _ptr(dst), _ptr(src), dst.numel() * dst.element_size())
-
-
 def Time_Advance(dyn_data):
-    """Dyn_Data.jl:326-364."""
-    m = dyn_data.mesh
-    for f in ("vor", "div", "lnps", "t"):
-        _copy(m, getattr(dyn_data, f"spe_{f}_p"), getattr(dyn_data, f"spe_{f}_c"))
-        _copy(m, getattr(dyn_data, f"spe_{f}_c"), getattr(dyn_data, f"spe_{f}_n"))
-    for f in ("u", "v", "ps", "t"):
-        _copy(m, getattr(dyn_data, f"grid_{f}_p"), getattr(dyn_data, f"grid_{f}_c"))
-        _copy(m, getattr(dyn_data, f"grid_{f}_c"), getattr(dyn_data, f"grid_{f}_n"))
+    """Dyn_Data.jl:326-364: p = c, c = n for the spectral vor, div, lnps, T and
+    the grid u, v, ps, T, as one device launch."""
+    d = dyn_data
+    names = [f"spe_{f}" for f in ("vor", "div", "lnps", "t")] + \
+        [f"grid_{f}" for f in ("u", "v", "ps", "t")]
+    n = len(names)
+    ps, cs, ns = (ctypes.c_void_p * n)(), (ctypes.c_void_p * n)(), (ctypes.c_void_p * n)()
+    counts = (ctypes.c_size_t * n)()
+    for i, f in enumerate(names):
+        p, c, nx = getattr(d, f + "_p"), getattr(d, f + "_c"), getattr(d, f + "_n")
+        ps[i], cs[i], ns[i] = p.data_ptr(), c.data_ptr(), nx.data_ptr()
+        counts[i] = c.numel() * c.element_size() // 8
+    call("jgcm_time_advance", d.mesh.handle, n, ps, cs, ns, counts)

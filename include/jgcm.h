@@ -88,6 +88,11 @@ int jgcm_memcpy_d2h(void* dst, const void* src, size_t bytes);
  * graph-captured).  Used for Time_Advance! (Dyn_Data.jl:326-364).          */
 int jgcm_copy(jgcm_mesh_t* mesh, void* dst, const void* src, size_t bytes);
 int jgcm_fill(jgcm_mesh_t* mesh, double* dst, double value, size_t count);
+/* Time_Advance! (Dyn_Data.jl:326-364) for nfields (<= 16) prognostic fields
+ * in one launch: per element, p = c, then c = n.  count[f] = doubles in
+ * field f (complex fields count twice).                                    */
+int jgcm_time_advance(jgcm_mesh_t* mesh, int nfields, double* const* p, double* const* c,
+                      const double* const* n, const size_t* count);
 
 /* ---- mesh: Spectral_Spherical_Mesh(num_fourier, num_spherical, nθ, nλ, nd,
  *      radius)  Spectral_Spherical_Mesh.jl:69-139.  Gaussian nodes/weights
