@@ -118,7 +118,7 @@ def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):
     jgcm.Atmosphere_Update(mesh, atmo, vc, si, d, HS_PARAMS)
     jgcm.Update_Init_Step(si)
     g = jgcm.Step_Graph(mesh, lambda: jgcm.Atmosphere_Update(mesh, atmo, vc, si, d, HS_PARAMS),
-                        nfields=2 * nd)
+                        nfields=5 * nd + 1)
     for _ in range(warmup):
         g.launch()
     mesh.synchronize()

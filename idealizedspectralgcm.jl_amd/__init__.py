@@ -10,7 +10,9 @@ from .mesh import (Spectral_Spherical_Mesh, Trans_Spherical_To_Grid, Trans_Grid_
                    Trans_Grid_To_Fourier, Divide_By_Cos, Multiply_By_Cos, Vor_Div_From_Grid_UV,
                    Compute_Alpha_Operator, UV_Grid_From_Vor_Div, Compute_Ucos_Vcos_From_Vor_Div,
                    Apply_Laplacian, Compute_Gradient_Cos, Compute_Gradients,
-                   Add_Horizontal_Advection, Area_Weighted_Global_Mean)
+                   Add_Horizontal_Advection, Area_Weighted_Global_Mean,
+                   Trans_Spherical_To_Grid_Batch, Trans_Grid_To_Spherical_Batch, BATCH_DIVCOS,
+                   BATCH_VORDIV)
 from .time_integrator import (Filtered_Leapfrog, Update_Init_Step, Get_Δt, Get_ξ,
                               Compute_Spectral_Damping, Filtered_Leapfrog_step, Damped_Leapfrog)
 from .dyn_data import Atmo_Data, Compute_Abs_Vor, Dyn_Data, Time_Advance
@@ -29,6 +31,8 @@ __all__ = [
     "Vor_Div_From_Grid_UV", "Compute_Alpha_Operator", "UV_Grid_From_Vor_Div",
     "Compute_Ucos_Vcos_From_Vor_Div", "Apply_Laplacian", "Compute_Gradient_Cos",
     "Compute_Gradients", "Add_Horizontal_Advection", "Area_Weighted_Global_Mean",
+    "Trans_Spherical_To_Grid_Batch", "Trans_Grid_To_Spherical_Batch", "BATCH_DIVCOS",
+    "BATCH_VORDIV",
     "Filtered_Leapfrog", "Update_Init_Step", "Get_Δt", "Get_ξ", "Compute_Spectral_Damping",
     "Filtered_Leapfrog_step", "Damped_Leapfrog", "Atmo_Data", "Compute_Abs_Vor", "Dyn_Data",
     "Time_Advance", "Barotropic_Dynamics", "Shallow_Water_Physics", "Implicit_Correction",

@@ -53,6 +53,10 @@ SIGNATURES = {
     "jgcm_mesh_get_table": [P, I, P, SZ],
     "jgcm_trans_grid_to_spherical": [P, P, P, I],
     "jgcm_trans_spherical_to_grid": [P, P, P, I],
+    "jgcm_trans_spherical_to_grid_batch": [P, I, ctypes.POINTER(P), ctypes.POINTER(P),
+                                           ctypes.POINTER(I), ctypes.POINTER(I)],
+    "jgcm_trans_grid_to_spherical_batch": [P, I, ctypes.POINTER(P), ctypes.POINTER(P),
+                                           ctypes.POINTER(I), ctypes.POINTER(I)],
     "jgcm_trans_grid_to_fourier": [P, P, P, I],
     "jgcm_trans_grid_to_spherical_adjoint": [P, P, P, I],
     "jgcm_trans_spherical_to_grid_adjoint": [P, P, P, I],
@@ -99,7 +103,7 @@ SIGNATURES = {
     "jgcm_semi_implicit_get": [P, I, P, SZ],
     "jgcm_implicit_correction": [P] * 10,
     "jgcm_hs_physics": [P, P, ctypes.c_int64] + [P] * 15 + [I],
-    "jgcm_hs_column_dynamics": [P] * 25,
+    "jgcm_hs_column_dynamics": [P] * 27,
     "jgcm_hs_corrections": [P] * 8,
     "jgcm_hs_scalars": [P, P, I],
     "jgcm_zonal_mean_accumulate": [P, P, P, P, P, P, I, P, P, P, P, P],

@@ -11,6 +11,8 @@ namespace jgcm {
 int fft_fold(const jgcm_mesh* M, const double* grid, double2* eo, const double* cosmul, int B);
 int ifft_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const double* divcos, int B);
 int legendre_inv(const jgcm_mesh* M, const double2* snm, double2* eo, int B);
+int inverse_batch(const jgcm_mesh* M, int nseg, const double* const* snm, double* const* grid,
+                  const int* nlev, const int* flags);
 
 static inline int nblocks(size_t n, int per = 256) {
   size_t b = (n + per - 1) / per;
@@ -186,11 +188,15 @@ int gradient_cos(const jgcm_mesh* M, const double2* h, double2* dl, double2* dt,
   return JGCM_OK;
 }
 
-// inverse transform of a spectral field with the rows divided by cos θ
-int inverse_divcos(const jgcm_mesh* M, const double2* spec, double* grid, int B) {
-  double2* eo = reinterpret_cast<double2*>(M->d_four_a);
-  if (int rc = legendre_inv(M, spec, eo, B)) return rc;
-  return ifft_unfold(M, eo, grid, M->d_cos, B);
+// inverse transforms of two spectral arrays with the rows divided by cos θ,
+// in one Legendre launch and one FFT launch
+static int inverse_pair_divcos(const jgcm_mesh* M, const double2* sa, const double2* sb,
+                               double* ga, double* gb, int B) {
+  const double* s[2] = {reinterpret_cast<const double*>(sa), reinterpret_cast<const double*>(sb)};
+  double* g[2] = {ga, gb};
+  int nl[2] = {B, B};
+  int fl[2] = {1, 1};
+  return inverse_batch(M, 2, s, g, nl, fl);
 }
 
 }  // namespace jgcm
@@ -282,11 +288,11 @@ int jgcm_uv_grid_from_vor_div(jgcm_mesh_t* mesh, const double* vor, const double
                               double* grid_u, double* grid_v, int nlev) {
   JGCM_MESH_PROLOGUE(mesh, nlev);
   JGCM_REQUIRE(vor && div && grid_u && grid_v, "NULL field pointer");
+  if (int rc = ensure_scratch(mesh, 2 * nlev)) return rc;
   double2* su = (double2*)mesh->d_spec_a;
   double2* sv = (double2*)mesh->d_spec_b;
   if (int rc = ucos_vcos(mesh, (const double2*)vor, (const double2*)div, su, sv, nlev)) return rc;
-  if (int rc = inverse_divcos(mesh, su, grid_u, nlev)) return rc;
-  return inverse_divcos(mesh, sv, grid_v, nlev);
+  return inverse_pair_divcos(mesh, su, sv, grid_u, grid_v, nlev);
 }
 
 int jgcm_apply_laplacian(jgcm_mesh_t* mesh, double* snm, int nlev) {
@@ -311,16 +317,18 @@ int jgcm_compute_gradients(jgcm_mesh_t* mesh, const double* spe_hs, double* grid
                            double* grid_dtheta, int nlev) {
   JGCM_MESH_PROLOGUE(mesh, nlev);
   JGCM_REQUIRE(spe_hs && grid_dlambda && grid_dtheta, "NULL field pointer");
+  if (int rc = ensure_scratch(mesh, 2 * nlev)) return rc;
   double2* sl = (double2*)mesh->d_spec_a;
   double2* st = (double2*)mesh->d_spec_b;
   if (int rc = gradient_cos(mesh, (const double2*)spe_hs, sl, st, nlev)) return rc;
-  if (int rc = inverse_divcos(mesh, sl, grid_dlambda, nlev)) return rc;
-  return inverse_divcos(mesh, st, grid_dtheta, nlev);
+  return inverse_pair_divcos(mesh, sl, st, grid_dlambda, grid_dtheta, nlev);
 }
 
 int jgcm_add_horizontal_advection(jgcm_mesh_t* mesh, const double* spe_hs, const double* grid_u,
                                   const double* grid_v, double* grid_dhs, int nlev) {
-  JGCM_MESH_PROLOGUE(mesh, nlev);
+  // scratch for 2 nlev fields first: the gradients below must not reallocate
+  // the grid scratch this function hands them
+  JGCM_MESH_PROLOGUE(mesh, 2 * nlev);
   JGCM_REQUIRE(spe_hs && grid_u && grid_v && grid_dhs, "NULL field pointer");
   if (int rc = jgcm_compute_gradients(mesh, spe_hs, mesh->d_grid_a, mesh->d_grid_b, nlev)) return rc;
   size_t n = (size_t)nlev * mesh->grid_elems();

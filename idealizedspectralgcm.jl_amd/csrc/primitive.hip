@@ -474,7 +474,8 @@ __global__ __launch_bounds__(512) void hs_column_dynamics_kernel(
     const double* ps, const double* dp, const double* lnp_half, const double* lnp_full,
     const double* p_full, double* dlps, double* dtps, const double* t, const double* vor,
     const double* geopots, double* M_half, double* w_full, double* du, double* dv, double* dps,
-    double* dtt, double* g_half, double* g_full, double* dlnps, double* absvor, double* energy) {
+    double* dtt, double* g_half, double* g_full, double* dlnps, double* absvor, double* energy,
+    const double* dl_t, const double* dt_t) {
   __shared__ double2 red[MAXNQ * TILE];
   const size_t P = (size_t)nlat * nlon;
   const TileIdx tx = tile_idx<KPT>(P);
@@ -607,7 +608,9 @@ __global__ __launch_bounds__(512) void hs_column_dynamics_kernel(
     double uk = U[i + 1], vk = V[i + 1];
     du[o] = du_k + av * vk;
     dv[o] = dv_k - av * uk;
-    dtt[o] = DT[i] + rt;
+    double dt_k = DT[i] + rt;
+    if (dl_t) dt_k = dt_k - (uk * dl_t[o] + vk * dt_t[o]);  // Add_Horizontal_Advection! (SSM:716)
+    dtt[o] = dt_k;
     absvor[o] = av;
     w_full[o] = W[i];
     M_half[(size_t)(k + 1) * P + c] = Mh[i + 1];
@@ -1318,8 +1321,11 @@ int jgcm_hs_column_dynamics(jgcm_semi_implicit_t* si, const double* grid_div, co
                             double* grid_w_full, double* grid_du, double* grid_dv,
                             double* grid_dps, double* grid_dt, double* geopot_half,
                             double* geopot_full, double* grid_dlnps, double* grid_absvor,
-                            double* grid_energy) {
+                            double* grid_energy, const double* grid_dlambda_t,
+                            const double* grid_dtheta_t) {
   JGCM_REQUIRE(si, "NULL argument");
+  JGCM_REQUIRE((grid_dlambda_t == nullptr) == (grid_dtheta_t == nullptr),
+               "grid_dlambda_t and grid_dtheta_t must both be given or both be NULL");
   jgcm_mesh* M = si->vc->mesh;
   size_t P = M->grid_elems();
   int nd = si->nd;
@@ -1328,7 +1334,8 @@ int jgcm_hs_column_dynamics(jgcm_semi_implicit_t* si, const double* grid_div, co
                      2 * si->atmo.omega, M->d_sin, grid_div, grid_u, grid_v, grid_ps, dp, lnp_half,
                      lnp_full, p_full, grid_dlambda_ps, grid_dtheta_ps, grid_t, grid_vor,
                      grid_geopots, grid_M_half, grid_w_full, grid_du, grid_dv, grid_dps, grid_dt,
-                     geopot_half, geopot_full, grid_dlnps, grid_absvor, grid_energy);
+                     geopot_half, geopot_full, grid_dlnps, grid_absvor, grid_energy,
+                     grid_dlambda_t, grid_dtheta_t);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

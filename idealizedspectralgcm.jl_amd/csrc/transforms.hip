@@ -75,10 +75,10 @@ static inline SegMap one_seg(const void* p, int nlev, int flag) {
 // ---------------------------------------------------------------- FFT stage
 template <int L>
 __global__ __launch_bounds__(256) void fft_fold_kernel(
-    const double* __restrict__ grid,   // [B][nlat][L]
+    SegMap in,                         // grid fields [nlat][L] each
     double2* __restrict__ eo,          // [B][nm][2][half]
     const double2* __restrict__ twg,   // [L]
-    const double* __restrict__ cosmul, // [nlat] or nullptr: multiply rows by cos θ
+    const double* __restrict__ cosv,   // [nlat]: rows multiplied by cos θ where flag & 1
     int nlat, int nm, int ppw) {
   constexpr int T = L / 4;
   constexpr int FPW = 256 / T;
@@ -89,6 +89,9 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
   const int SP = 2 * nm + 1;
   const int b = blockIdx.y;
   const int j0 = blockIdx.x * ppw;
+  const int seg = seg_find(in, b);
+  const double* __restrict__ grid = in.ptr[seg] + (size_t)(b - in.start[seg]) * nlat * L;
+  const double* __restrict__ cosmul = (in.flag[seg] & 1) ? cosv : nullptr;
   for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
   __syncthreads();
   const int g = threadIdx.x / T;
@@ -102,8 +105,8 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
     int jj = q * FPW + g;
     int jp = j0 + jj;
     if (q * FPW < ppw && jj < ppw && jp < half) {
-      const double* ra = grid + ((size_t)b * nlat + jp) * L;
-      const double* rc = grid + ((size_t)b * nlat + (nlat - 1 - jp)) * L;
+      const double* ra = grid + (size_t)jp * L;
+      const double* rc = grid + (size_t)(nlat - 1 - jp) * L;
 #pragma unroll
       for (int r = 0; r < 4; ++r) pre[q][r] = make_double2(ra[t + r * T], rc[t + r * T]);
     }
@@ -123,8 +126,8 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
             for (int r = 0; r < 4; ++r) v[r] = pre[q][r];
           }
       } else {
-        const double* ra = grid + ((size_t)b * nlat + jp) * L;
-        const double* rc = grid + ((size_t)b * nlat + (nlat - 1 - jp)) * L;
+        const double* ra = grid + (size_t)jp * L;
+        const double* rc = grid + (size_t)(nlat - 1 - jp) * L;
 #pragma unroll
         for (int r = 0; r < 4; ++r) v[r] = make_double2(ra[t + r * T], rc[t + r * T]);
       }
@@ -160,13 +163,13 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
   }
 }
 
-// Inverse: E/O block -> two grid rows per FFT.  `divcos` divides the output
-// rows by cos θ (Divide_By_Cos!, Spectral_Spherical_Mesh.jl:333-338).
+// Inverse: E/O block -> two grid rows per FFT.  Segments with flag & 1 have
+// their output rows divided by cos θ (Divide_By_Cos!, Spectral_Spherical_Mesh.jl:333-338).
 template <int L>
 __global__ __launch_bounds__(256) void ifft_unfold_kernel(
     const double2* __restrict__ eo,    // [B][nm][2][half]
-    double* __restrict__ grid,         // [B][nlat][L]
-    const double2* __restrict__ twg, const double* __restrict__ divcos,
+    SegMap out,                        // grid fields [nlat][L] each
+    const double2* __restrict__ twg, const double* __restrict__ cosv,
     int nlat, int nm, int ppw) {
   constexpr int T = L / 4;
   constexpr int FPW = 256 / T;
@@ -177,6 +180,9 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
   const int SP = 2 * nm + 1;
   const int b = blockIdx.y;
   const int j0 = blockIdx.x * ppw;
+  const int seg = seg_find(out, b);
+  double* __restrict__ grid = out.ptr[seg] + (size_t)(b - out.start[seg]) * nlat * L;
+  const double* __restrict__ divcos = (out.flag[seg] & 1) ? cosv : nullptr;
   for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
   const int total = nm * 2 * ppw;
   staged_copy<8>(
@@ -216,8 +222,8 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
     fft_lds<L, +1>(buf[g], tw, t);
     if (valid) {
       int jm = nlat - 1 - jp;
-      double* ra = grid + ((size_t)b * nlat + jp) * L;
-      double* rc = grid + ((size_t)b * nlat + jm) * L;
+      double* ra = grid + (size_t)jp * L;
+      double* rc = grid + (size_t)jm * L;
       double c = divcos ? divcos[jp] : 1.0;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -332,18 +338,22 @@ struct FwdTables {
 
 struct FwdJob {
   // term A: coefficient table tA against mulA * X_A^(par)
-  const double2* srcA;
+  const double2* srcA;  // E/O block of this job's first field
   FwdTables tA;
   int mulA_i;           // 1: X_A multiplied by i
   // term B (optional): table tB against signB * X_B^(1-par)
   const double2* srcB;
   FwdTables tB;
   double signB;
-  double2* out;         // spectral [B][ns+1][nm]
+  int nf;               // fields in this job
+  SegMap out;           // spectral fields [ns+1][nm] (complex) each
 };
 
+constexpr int MAXJOB = 4;
 struct FwdJobs {
-  FwdJob job[2];
+  FwdJob job[MAXJOB];
+  int njobs;
+  int zstart[MAXJOB + 1];  // first blockIdx.z of each job (field chunks of 32)
 };
 
 // Stage one (m, parity) tile of the E/O block into LDS, column-major
@@ -427,10 +437,10 @@ __device__ __forceinline__ void contract_rows(double (&acc)[LROWS], const typena
 }
 
 // grid: x = segment index (m, par) with m = x/2, par = x%2; y = row chunk of
-// LWAVES*LROWS rows; z = field chunk * njobs + job.
+// LWAVES*LROWS rows; z = (job, chunk of 32 fields of that job).
 template <int PREC>
-__global__ __launch_bounds__(256) void legendre_fwd_kernel(FwdJobs jobs, int njobs, int B, int N,
-                                                           int ns, int nm, int half,
+__global__ __launch_bounds__(256) void legendre_fwd_kernel(FwdJobs jobs, int N, int ns, int nm,
+                                                           int half,
                                                            const int* __restrict__ row_start,
                                                            int nmax, int zero_top) {
   using LT = typename LdsT<PREC>::T;
@@ -439,11 +449,15 @@ __global__ __launch_bounds__(256) void legendre_fwd_kernel(FwdJobs jobs, int njo
   const int m = blockIdx.x >> 1;
   const int par = blockIdx.x & 1;
   const int rc = blockIdx.y;
-  const int jobi = blockIdx.z % njobs;
-  const int fc = blockIdx.z / njobs;
+  int jobi = 0;
+#pragma unroll
+  for (int q = 1; q < MAXJOB; ++q)
+    if (q < jobs.njobs && (int)blockIdx.z >= jobs.zstart[q]) jobi = q;
+  const int fc = blockIdx.z - jobs.zstart[jobi];
   const FwdJob& job = jobs.job[jobi];
   const int f0 = fc * (LCOLS / 2);
-  const int nf = min(LCOLS / 2, B - f0);
+  const int nf = min(LCOLS / 2, job.nf - f0);
+  const size_t spec_d = (size_t)(ns + 1) * nm * 2;  // doubles per spectral field
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nfirst = m + par;
@@ -452,7 +466,7 @@ __global__ __launch_bounds__(256) void legendre_fwd_kernel(FwdJobs jobs, int njo
   const int k0 = rc * (LWAVES * LROWS) + wave * LROWS;
   if (zero_top && par == 0 && rc == 0 && threadIdx.x < 2 * nf) {
     int fl = threadIdx.x >> 1;
-    double* o = reinterpret_cast<double*>(job.out + ((size_t)(f0 + fl) * (ns + 1) + ns) * nm + m);
+    double* o = seg_field(job.out, f0 + fl, spec_d) + ((size_t)ns * nm + m) * 2;
     o[threadIdx.x & 1] = 0.0;
   }
   if (rc * LWAVES * LROWS >= nr) return;  // whole workgroup idle (uniform)
@@ -479,13 +493,13 @@ __global__ __launch_bounds__(256) void legendre_fwd_kernel(FwdJobs jobs, int njo
   if (k0 < nr && lane < 2 * nf) {
     int fl = lane >> 1;
     int comp = lane & 1;
+    double* base = seg_field(job.out, f0 + fl, spec_d) + (size_t)m * 2 + comp;
 #pragma unroll
     for (int r = 0; r < LROWS; ++r) {
       int k = k0 + r;
       if (k < nr) {
         int n = nfirst + 2 * k;
-        double* o = reinterpret_cast<double*>(job.out + ((size_t)(f0 + fl) * (ns + 1) + n) * nm + m);
-        o[comp] = acc[r];
+        base[(size_t)n * nm * 2] = acc[r];
       }
     }
   }
@@ -496,7 +510,7 @@ __global__ __launch_bounds__(256) void legendre_fwd_kernel(FwdJobs jobs, int njo
 constexpr int IJB = 16;
 
 template <int PREC>
-__global__ __launch_bounds__(512) void legendre_inv_kernel(const double2* __restrict__ snm,
+__global__ __launch_bounds__(512) void legendre_inv_kernel(SegMap in,  // spectral fields
                                                            double2* __restrict__ eo,
                                                            FwdTables tab, int B, int nmax, int ns,
                                                            int nm, int half,
@@ -516,13 +530,15 @@ __global__ __launch_bounds__(512) void legendre_inv_kernel(const double2* __rest
   double* Otile = reinterpret_cast<double*>(smem_raw + ((sizeof(LT) * LCOLS * SS + 15) / 16) * 16);  // [col][half+1]
   const int OS = half + 1;
   // stage the segment's spectral coefficients, column-major
+  const size_t spec_d = (size_t)(ns + 1) * nm * 2;
   staged_copy<8>(
       (LCOLS / 2) * nr,
       [&](int idx) {
         int k = idx % nr;
         int fl = idx / nr;
-        return fl < nf ? snm[((size_t)(f0 + fl) * (ns + 1) + nfirst + 2 * k) * nm + m]
-                       : make_double2(0.0, 0.0);
+        if (fl >= nf) return make_double2(0.0, 0.0);
+        const double2* s = reinterpret_cast<const double2*>(seg_field(in, f0 + fl, spec_d));
+        return s[(size_t)(nfirst + 2 * k) * nm + m];
       },
       [&](int idx, double2 v) {
         int k = idx % nr;
@@ -620,23 +636,21 @@ static int pick_ppw(int L) {
 }
 
 template <int L>
-static void launch_fold(const jgcm_mesh* M, const double* grid, double2* eo, const double* cosmul,
-                        int B) {
+static void launch_fold(const jgcm_mesh* M, const SegMap& in, double2* eo, int B) {
   int ppw = pick_ppw(L);
   dim3 grd((M->half + ppw - 1) / ppw, B);
   size_t lds = (size_t)(M->nm * 2 + 1) * ppw * sizeof(double2);
-  hipLaunchKernelGGL(fft_fold_kernel<L>, grd, dim3(256), lds, M->stream, grid, eo,
-                     reinterpret_cast<const double2*>(M->d_twiddle), cosmul, M->nlat, M->nm, ppw);
+  hipLaunchKernelGGL(fft_fold_kernel<L>, grd, dim3(256), lds, M->stream, in, eo,
+                     reinterpret_cast<const double2*>(M->d_twiddle), M->d_cos, M->nlat, M->nm, ppw);
 }
 
 template <int L>
-static void launch_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const double* divcos,
-                          int B) {
+static void launch_unfold(const jgcm_mesh* M, const double2* eo, const SegMap& out, int B) {
   int ppw = pick_ppw(L);
   dim3 grd((M->half + ppw - 1) / ppw, B);
   size_t lds = (size_t)(M->nm * 2 + 1) * ppw * sizeof(double2);
-  hipLaunchKernelGGL(ifft_unfold_kernel<L>, grd, dim3(256), lds, M->stream, eo, grid,
-                     reinterpret_cast<const double2*>(M->d_twiddle), divcos, M->nlat, M->nm, ppw);
+  hipLaunchKernelGGL(ifft_unfold_kernel<L>, grd, dim3(256), lds, M->stream, eo, out,
+                     reinterpret_cast<const double2*>(M->d_twiddle), M->d_cos, M->nlat, M->nm, ppw);
 }
 
 template <int L>
@@ -659,16 +673,24 @@ static void launch_full(const jgcm_mesh* M, const double* grid, double2* four, i
     default: break;                                                            \
   }
 
-int fft_fold(const jgcm_mesh* M, const double* grid, double2* eo, const double* cosmul, int B) {
-  JGCM_DISPATCH_L(M->nlon, launch_fold<LL>(M, grid, eo, cosmul, B));
+static int fft_fold_map(const jgcm_mesh* M, const SegMap& in, double2* eo) {
+  JGCM_DISPATCH_L(M->nlon, launch_fold<LL>(M, in, eo, in.start[in.nseg]));
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
 
-int ifft_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const double* divcos, int B) {
-  JGCM_DISPATCH_L(M->nlon, launch_unfold<LL>(M, eo, grid, divcos, B));
+static int ifft_unfold_map(const jgcm_mesh* M, const double2* eo, const SegMap& out) {
+  JGCM_DISPATCH_L(M->nlon, launch_unfold<LL>(M, eo, out, out.start[out.nseg]));
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
+}
+
+int fft_fold(const jgcm_mesh* M, const double* grid, double2* eo, const double* cosmul, int B) {
+  return fft_fold_map(M, one_seg(grid, B, cosmul ? 1 : 0), eo);
+}
+
+int ifft_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const double* divcos, int B) {
+  return ifft_unfold_map(M, eo, one_seg(grid, B, divcos ? 1 : 0));
 }
 
 static FwdTables tables_for(const jgcm_mesh* M, int which) {
@@ -681,36 +703,42 @@ static FwdTables tables_for(const jgcm_mesh* M, int which) {
   }
 }
 
-static int launch_legendre_fwd(const jgcm_mesh* M, const FwdJobs& jobs, int njobs, int B, int nmax,
-                               int zero_top) {
-  int maxrows = (nmax - 0) / 2 + 1;
+static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int zero_top) {
+  int z = 0;
+  for (int j = 0; j < jobs.njobs; ++j) {
+    jobs.zstart[j] = z;
+    z += (jobs.job[j].nf + LCOLS / 2 - 1) / (LCOLS / 2);
+  }
+  jobs.zstart[jobs.njobs] = z;
+  if (z == 0) return JGCM_OK;
+  int maxrows = nmax / 2 + 1;
   int rchunks = (maxrows + LWAVES * LROWS - 1) / (LWAVES * LROWS);
-  int fchunks = (B + LCOLS / 2 - 1) / (LCOLS / 2);
-  dim3 grd(2 * M->nm, rchunks, fchunks * njobs);
+  dim3 grd(2 * M->nm, rchunks, z);
   int XS = M->half + 1;
   switch (M->precision) {
     case JGCM_PREC_FP64:
       hipLaunchKernelGGL(legendre_fwd_kernel<P_FP64>, grd, dim3(256), LCOLS * XS * sizeof(double),
-                         M->stream, jobs, njobs, B, M->N, M->ns, M->nm, M->half, M->d_row_start,
-                         nmax, zero_top);
+                         M->stream, jobs, M->N, M->ns, M->nm, M->half, M->d_row_start, nmax,
+                         zero_top);
       break;
     case JGCM_PREC_FP32:
       hipLaunchKernelGGL(legendre_fwd_kernel<P_FP32>, grd, dim3(256), LCOLS * XS * sizeof(float),
-                         M->stream, jobs, njobs, B, M->N, M->ns, M->nm, M->half, M->d_row_start,
-                         nmax, zero_top);
+                         M->stream, jobs, M->N, M->ns, M->nm, M->half, M->d_row_start, nmax,
+                         zero_top);
       break;
     default:
       hipLaunchKernelGGL(legendre_fwd_kernel<P_SPLIT>, grd, dim3(256), LCOLS * XS * sizeof(float2),
-                         M->stream, jobs, njobs, B, M->N, M->ns, M->nm, M->half, M->d_row_start,
-                         nmax, zero_top);
+                         M->stream, jobs, M->N, M->ns, M->nm, M->half, M->d_row_start, nmax,
+                         zero_top);
       break;
   }
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
 
-int legendre_inv_t(const jgcm_mesh* M, const double2* snm, double2* eo, int B, int table,
-                   int nmax) {
+static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, int table,
+                            int nmax) {
+  int B = snm.start[snm.nseg];
   int fchunks = (B + LCOLS / 2 - 1) / (LCOLS / 2);
   dim3 grd(2 * M->nm, fchunks);
   int waves = (M->half + IJB - 1) / IJB;
@@ -737,30 +765,47 @@ int legendre_inv_t(const jgcm_mesh* M, const double2* snm, double2* eo, int B, i
   return JGCM_OK;
 }
 
+int legendre_inv_t(const jgcm_mesh* M, const double2* snm, double2* eo, int B, int table,
+                   int nmax) {
+  return legendre_inv_map(M, one_seg(snm, B, 0), eo, table, nmax);
+}
+
 int legendre_inv(const jgcm_mesh* M, const double2* snm, double2* eo, int B) {
   return legendre_inv_t(M, snm, eo, B, 3, M->ns);
 }
 
+static FwdJob plain_job(const jgcm_mesh* M, const double2* eo, int nf, const SegMap& out) {
+  return FwdJob{eo, tables_for(M, 0), 0, nullptr, FwdTables{}, 0.0, nf, out};
+}
+
 int legendre_fwd_plain(const jgcm_mesh* M, const double2* eo, double2* snm, int B) {
   FwdJobs jobs{};
-  jobs.job[0] = FwdJob{eo, tables_for(M, 0), 0, nullptr, FwdTables{}, 0.0, snm};
-  return launch_legendre_fwd(M, jobs, 1, B, M->N, 1);
+  jobs.njobs = 1;
+  jobs.job[0] = plain_job(M, eo, B, one_seg(snm, B, 0));
+  return launch_legendre_fwd(M, jobs, M->N, 1);
 }
 
 // vor = α(V, U, -1), div = α(U, V, +1) with the top degree zeroed (SSM:387-392)
+static void vordiv_jobs(const jgcm_mesh* M, const double2* eoU, const double2* eoV, int B,
+                        const SegMap& vor, const SegMap& div, FwdJob* out2) {
+  out2[0] = FwdJob{eoV, tables_for(M, 1), 1, eoU, tables_for(M, 2), -1.0, B, vor};
+  out2[1] = FwdJob{eoU, tables_for(M, 1), 1, eoV, tables_for(M, 2), +1.0, B, div};
+}
+
 int legendre_vordiv(const jgcm_mesh* M, const double2* eoU, const double2* eoV, double2* vor,
                     double2* div, int B) {
   FwdJobs jobs{};
-  jobs.job[0] = FwdJob{eoV, tables_for(M, 1), 1, eoU, tables_for(M, 2), -1.0, vor};
-  jobs.job[1] = FwdJob{eoU, tables_for(M, 1), 1, eoV, tables_for(M, 2), +1.0, div};
-  return launch_legendre_fwd(M, jobs, 2, B, M->N, 1);
+  jobs.njobs = 2;
+  vordiv_jobs(M, eoU, eoV, B, one_seg(vor, B, 0), one_seg(div, B, 0), jobs.job);
+  return launch_legendre_fwd(M, jobs, M->N, 1);
 }
 
 int legendre_alpha(const jgcm_mesh* M, const double2* eoA, const double2* eoB, double isign,
                    double2* alpha, int B) {
   FwdJobs jobs{};
-  jobs.job[0] = FwdJob{eoA, tables_for(M, 1), 1, eoB, tables_for(M, 2), isign, alpha};
-  return launch_legendre_fwd(M, jobs, 1, B, M->ns, 0);
+  jobs.njobs = 1;
+  jobs.job[0] = FwdJob{eoA, tables_for(M, 1), 1, eoB, tables_for(M, 2), isign, B, one_seg(alpha, B, 0)};
+  return launch_legendre_fwd(M, jobs, M->ns, 0);
 }
 
 // E/O block scaled per wavenumber: m = 0 by s0, m >= 1 by s1
@@ -785,8 +830,9 @@ int scale_eo(const jgcm_mesh* M, double2* eo, int B, double s0, double s1) {
 // transpose of the inverse Legendre stage: s[n] = sum_j' Pinv[n][j'] eo^(n-m)[j'], n = m..ns
 int legendre_inv_adjoint(const jgcm_mesh* M, const double2* eo, double2* snm, int B) {
   FwdJobs jobs{};
-  jobs.job[0] = FwdJob{eo, tables_for(M, 3), 0, nullptr, FwdTables{}, 0.0, snm};
-  return launch_legendre_fwd(M, jobs, 1, B, M->ns, 0);
+  jobs.njobs = 1;
+  jobs.job[0] = FwdJob{eo, tables_for(M, 3), 0, nullptr, FwdTables{}, 0.0, B, one_seg(snm, B, 0)};
+  return launch_legendre_fwd(M, jobs, M->ns, 0);
 }
 
 int fft_full(const jgcm_mesh* M, const double* grid, double2* four, int B) {
@@ -802,6 +848,83 @@ int fold_full(const jgcm_mesh* M, const double2* four, double2* eo, int B) {
                      M->nlon, M->nm, B);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
+}
+
+// Back-transform of several spectral arrays in one Legendre launch and one FFT
+// launch; flag & 1 divides the rows of that array by cos θ.
+int inverse_batch(const jgcm_mesh* M, int nseg, const double* const* snm, double* const* grid,
+                  const int* nlev, const int* flags) {
+  SegMap in{}, out{};
+  in.nseg = out.nseg = nseg;
+  int B = 0;
+  for (int k = 0; k < nseg; ++k) {
+    in.start[k] = out.start[k] = B;
+    in.ptr[k] = const_cast<double*>(snm[k]);
+    out.ptr[k] = grid[k];
+    in.flag[k] = 0;
+    out.flag[k] = flags ? (flags[k] & 1) : 0;
+    B += nlev[k];
+  }
+  in.start[nseg] = out.start[nseg] = B;
+  double2* eo = reinterpret_cast<double2*>(M->d_four_a);
+  if (int rc = legendre_inv_map(M, in, eo, 3, M->ns)) return rc;
+  return ifft_unfold_map(M, eo, out);
+}
+
+// Forward transform of several grid arrays in one FFT launch and one Legendre
+// launch.  flags[k] == JGCM_BATCH_VORDIV pairs array k (u) with array k+1 (v)
+// and writes vor to snm[k], div to snm[k+1] (Vor_Div_From_Grid_UV!); other
+// arrays get the plain transform.  Consecutive plain arrays share one job.
+int forward_batch(const jgcm_mesh* M, int nseg, const double* const* grid, double* const* snm,
+                  const int* nlev, const int* flags) {
+  SegMap in{};
+  in.nseg = nseg;
+  int B = 0;
+  for (int k = 0; k < nseg; ++k) {
+    in.start[k] = B;
+    in.ptr[k] = const_cast<double*>(grid[k]);
+    int f = flags ? flags[k] : 0;
+    bool uv = (f == JGCM_BATCH_VORDIV) || (k > 0 && flags && flags[k - 1] == JGCM_BATCH_VORDIV);
+    in.flag[k] = uv ? 1 : 0;
+    B += nlev[k];
+  }
+  in.start[nseg] = B;
+  double2* eo = reinterpret_cast<double2*>(M->d_four_a);
+  const size_t eo_field = (size_t)M->nm * 2 * M->half;
+  FwdJobs jobs{};
+  int nj = 0;
+  for (int k = 0; k < nseg;) {
+    if (nj >= MAXJOB) {
+      set_error("batched forward transform needs more than %d Legendre jobs", MAXJOB);
+      return JGCM_ERR_ARG;
+    }
+    if (flags && flags[k] == JGCM_BATCH_VORDIV) {
+      if (nj + 2 > MAXJOB) {
+        set_error("batched forward transform needs more than %d Legendre jobs", MAXJOB);
+        return JGCM_ERR_ARG;
+      }
+      vordiv_jobs(M, eo + in.start[k] * eo_field, eo + in.start[k + 1] * eo_field, nlev[k],
+                  one_seg(snm[k], nlev[k], 0), one_seg(snm[k + 1], nlev[k], 0), jobs.job + nj);
+      nj += 2;
+      k += 2;
+      continue;
+    }
+    // a run of plain arrays: one job, outputs through a segment map
+    SegMap out{};
+    int k0 = k, nf = 0;
+    while (k < nseg && !(flags && flags[k] == JGCM_BATCH_VORDIV)) {
+      out.start[k - k0] = nf;
+      out.ptr[k - k0] = snm[k];
+      nf += nlev[k];
+      ++k;
+    }
+    out.nseg = k - k0;
+    out.start[out.nseg] = nf;
+    jobs.job[nj++] = plain_job(M, eo + in.start[k0] * eo_field, nf, out);
+  }
+  jobs.njobs = nj;
+  if (int rc = fft_fold_map(M, in, eo)) return rc;
+  return launch_legendre_fwd(M, jobs, M->N, 1);
 }
 
 }  // namespace jgcm
@@ -839,14 +962,54 @@ int jgcm_trans_grid_to_fourier(jgcm_mesh_t* mesh, const double* pfield, double* 
 
 int jgcm_vor_div_from_grid_uv(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
                               double* vor, double* div, int nlev) {
-  JGCM_MESH_PROLOGUE(mesh, nlev);
+  JGCM_MESH_PROLOGUE(mesh, 2 * nlev);
   JGCM_REQUIRE(grid_u && grid_v && vor && div, "NULL field pointer");
-  double2* eu = reinterpret_cast<double2*>(mesh->d_four_a);
-  double2* ev = reinterpret_cast<double2*>(mesh->d_four_b);
-  if (int rc = fft_fold(mesh, grid_u, eu, mesh->d_cos, nlev)) return rc;
-  if (int rc = fft_fold(mesh, grid_v, ev, mesh->d_cos, nlev)) return rc;
-  return legendre_vordiv(mesh, eu, ev, reinterpret_cast<double2*>(vor),
-                         reinterpret_cast<double2*>(div), nlev);
+  const double* g[2] = {grid_u, grid_v};
+  double* s[2] = {vor, div};
+  int nl[2] = {nlev, nlev};
+  int fl[2] = {JGCM_BATCH_VORDIV, 0};
+  return forward_batch(mesh, 2, g, s, nl, fl);
+}
+
+int jgcm_trans_spherical_to_grid_batch(jgcm_mesh_t* mesh, int nseg, const double* const* snm,
+                                       double* const* pfield, const int* nlev, const int* flags) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nseg >= 1 && nseg <= MAXSEG, "nseg must be in 1..%d, got %d", MAXSEG, nseg);
+  JGCM_REQUIRE(snm && pfield && nlev, "NULL argument");
+  int total = 0;
+  for (int k = 0; k < nseg; ++k) {
+    JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
+    JGCM_REQUIRE(nlev[k] >= 1, "nlev[%d] must be >= 1, got %d", k, nlev[k]);
+    JGCM_REQUIRE(!flags || (flags[k] & ~JGCM_BATCH_DIVCOS) == 0, "flags[%d] = %d is not valid", k,
+                 flags[k]);
+    total += nlev[k];
+  }
+  if (int rc = ensure_scratch(mesh, total)) return rc;
+  return inverse_batch(mesh, nseg, snm, pfield, nlev, flags);
+}
+
+int jgcm_trans_grid_to_spherical_batch(jgcm_mesh_t* mesh, int nseg, const double* const* pfield,
+                                       double* const* snm, const int* nlev, const int* flags) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nseg >= 1 && nseg <= MAXSEG, "nseg must be in 1..%d, got %d", MAXSEG, nseg);
+  JGCM_REQUIRE(snm && pfield && nlev, "NULL argument");
+  int total = 0;
+  for (int k = 0; k < nseg; ++k) {
+    JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
+    JGCM_REQUIRE(nlev[k] >= 1, "nlev[%d] must be >= 1, got %d", k, nlev[k]);
+    int f = flags ? flags[k] : 0;
+    JGCM_REQUIRE(f == 0 || f == JGCM_BATCH_VORDIV, "flags[%d] = %d is not valid", k, f);
+    if (f == JGCM_BATCH_VORDIV) {
+      JGCM_REQUIRE(k + 1 < nseg && flags[k + 1] == 0 && nlev[k + 1] == nlev[k],
+                   "array %d (u) needs a following v array with flag 0 and the same nlev", k);
+      total += nlev[k];
+      ++k;
+      JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
+    }
+    total += nlev[k];
+  }
+  if (int rc = ensure_scratch(mesh, total)) return rc;
+  return forward_batch(mesh, nseg, pfield, snm, nlev, flags);
 }
 
 // Adjoint (transpose w.r.t. sum_ij x y on the grid and sum Re(conj(a) b) over

@@ -191,6 +191,44 @@ def Trans_Grid_To_Spherical(mesh, pfield, snm, batch=False):
     call("jgcm_trans_grid_to_spherical", mesh.handle, _ptr(pfield), _ptr(snm), n)
 
 
+BATCH_DIVCOS = 1
+BATCH_VORDIV = 2
+
+
+def _batch_args(ts_a, ts_b, flags):
+    n = len(ts_a)
+    assert 1 <= n <= 8 and len(ts_b) == n, "batched transforms take 1..8 arrays"
+    a = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts_a])
+    b = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts_b])
+    nl = (ctypes.c_int * n)(*[t.shape[0] for t in ts_a])
+    fl = (ctypes.c_int * n)(*(flags or [0] * n))
+    return n, a, b, nl, fl
+
+
+def Trans_Spherical_To_Grid_Batch(mesh, snms, pfields, flags=None):
+    """Trans_Spherical_To_Grid! of several arrays in one Legendre launch and
+    one FFT launch (no reference counterpart).  flags[k] = BATCH_DIVCOS
+    divides array k's rows by cos θ (as UV_Grid_From_Vor_Div! and
+    Compute_Gradients! do after their back-transform)."""
+    for s, g in zip(snms, pfields):
+        mesh._nlev(s, "spec", True)
+        mesh._nlev(g, "grid", True)
+        _same(s, g)
+    call("jgcm_trans_spherical_to_grid_batch", mesh.handle, *_batch_args(snms, pfields, flags))
+
+
+def Trans_Grid_To_Spherical_Batch(mesh, pfields, snms, flags=None):
+    """Trans_Grid_To_Spherical! of several arrays in one FFT launch and one
+    Legendre launch (no reference counterpart).  flags[k] = BATCH_VORDIV
+    makes arrays k, k+1 a (u, v) pair whose outputs are (vor, div), as
+    Vor_Div_From_Grid_UV!."""
+    for s, g in zip(snms, pfields):
+        mesh._nlev(s, "spec", True)
+        mesh._nlev(g, "grid", True)
+        _same(s, g)
+    call("jgcm_trans_grid_to_spherical_batch", mesh.handle, *_batch_args(pfields, snms, flags))
+
+
 def Trans_Grid_To_Fourier(mesh, pfield, fourier_g):
     """SSM:295-330."""
     n = mesh._nlev(pfield, "grid", True)

@@ -151,6 +151,26 @@ int jgcm_trans_spherical_to_grid(jgcm_mesh_t* mesh, const double* snm,
 int jgcm_trans_grid_to_fourier(jgcm_mesh_t* mesh, const double* pfield,
                                double* fourier_g, int nlev);
 
+/* Batched forms of the two transforms (no reference counterpart: the
+ * reference transforms one array per call).  nseg (1..8) arrays of nlev[k]
+ * levels go through ONE Legendre launch and ONE FFT launch, which is how a
+ * time step pushes all its level-fields through the transform at once.
+ *   inverse: flags[k] & JGCM_BATCH_DIVCOS divides the output rows of array k
+ *            by cos θ (the second half of UV_Grid_From_Vor_Div! and
+ *            Compute_Gradients!, Spectral_Spherical_Mesh.jl:396-420, 462-488).
+ *   forward: flags[k] == JGCM_BATCH_VORDIV pairs array k (grid u) with array
+ *            k+1 (grid v, flags[k+1] = 0, same nlev) and writes vor to snm[k],
+ *            div to snm[k+1] (Vor_Div_From_Grid_UV!); other arrays get the
+ *            plain forward transform.  At most 4 Legendre jobs: each run of
+ *            plain arrays is one job, each u/v pair two.
+ * flags may be NULL (all plain).  Results equal the one-array calls.       */
+#define JGCM_BATCH_DIVCOS 1
+#define JGCM_BATCH_VORDIV 2
+int jgcm_trans_spherical_to_grid_batch(jgcm_mesh_t* mesh, int nseg, const double* const* snm,
+                                       double* const* pfield, const int* nlev, const int* flags);
+int jgcm_trans_grid_to_spherical_batch(jgcm_mesh_t* mesh, int nseg, const double* const* pfield,
+                                       double* const* snm, const int* nlev, const int* flags);
+
 /* Adjoints of the transform pair (no reference counterpart; the building
  * block of the reverse-mode derivative of a step).  Transposes with respect
  * to <x, y> = sum x y over grid values and <a, b> = sum Re(conj(a) b) over
@@ -329,7 +349,10 @@ int jgcm_hs_physics(jgcm_semi_implicit_t* si, const jgcm_hs_params* hs, int64_t 
                     int do_forcing);
 /* Step part 2, one column pass (Spectral_Dynamics.jl:260-307 without the
  * transforms): ∇ps *= ps, Four_In_One!, Compute_Geopotential!, δlnps = δps/ps,
- * Vert_Advection! of u, v, T, absolute-vorticity flux, energy = Φ + KE.     */
+ * Vert_Advection! of u, v, T, absolute-vorticity flux, energy = Φ + KE.
+ * With grid_dlambda_t / grid_dtheta_t (the grid gradients of T, may be NULL)
+ * it also applies the grid part of Add_Horizontal_Advection! for T
+ * (Spectral_Spherical_Mesh.jl:716): δT -= u ∂λT + v ∂θT.                    */
 int jgcm_hs_column_dynamics(jgcm_semi_implicit_t* si, const double* grid_div, const double* grid_u,
                             const double* grid_v, const double* grid_ps, const double* dp,
                             const double* lnp_half, const double* lnp_full, const double* p_full,
@@ -338,7 +361,8 @@ int jgcm_hs_column_dynamics(jgcm_semi_implicit_t* si, const double* grid_div, co
                             double* grid_w_full, double* grid_du, double* grid_dv,
                             double* grid_dps, double* grid_dt, double* geopot_half,
                             double* geopot_full, double* grid_dlnps, double* grid_absvor,
-                            double* grid_energy);
+                            double* grid_energy, const double* grid_dlambda_t,
+                            const double* grid_dtheta_t);
 /* ps_n = exp(lnps) then Compute_Corrections!  Spectral_Dynamics.jl:336-343, :30-68 */
 int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, double* grid_ps_n,
                         const double* grid_u_n, const double* grid_v_n, double* grid_t_n,

@@ -1,0 +1,105 @@
+"""Batched transforms (jgcm_trans_*_batch): several arrays of ragged level
+counts in one launch pair must give exactly the one-array results (the
+per-field arithmetic is the same, so the comparison is bit-exact) and match
+the FP64 oracle to 1e-12.  Segment sizes are chosen so that 32-field
+Legendre chunks straddle array boundaries."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import SpectralSphericalMesh
+
+pytestmark = pytest.mark.gpu
+
+RADIUS = 6371.0e3
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def random_spec(N, ns, nlev, rng):
+    s = rng.standard_normal((nlev, ns + 1, N + 1)) + 1j * rng.standard_normal((nlev, ns + 1, N + 1))
+    s[:, :, 0] = s[:, :, 0].real
+    for m in range(N + 1):
+        s[:, :m, m] = 0
+    return s
+
+
+def cuda(x):
+    return torch.tensor(np.ascontiguousarray(x), device="cuda")
+
+
+@pytest.mark.parametrize("N,nlat,prec", [(42, 64, "fp64"), (85, 128, "fp64"), (85, 128, "split32")])
+def test_inverse_batch_equals_single_calls(jgcm, N, nlat, prec):
+    dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 5, RADIUS, precision=prec)
+    rng = np.random.default_rng(N)
+    levels = [5, 1, 40, 3]
+    flags = [0, jgcm.BATCH_DIVCOS, 0, jgcm.BATCH_DIVCOS]
+    specs = [cuda(random_spec(N, N + 1, n, rng)) for n in levels]
+    outs = [dev.grid_zeros(n) for n in levels]
+    jgcm.Trans_Spherical_To_Grid_Batch(dev, specs, outs, flags)
+    for s, o, f in zip(specs, outs, flags):
+        single = dev.grid_zeros(s.shape[0])
+        jgcm.Trans_Spherical_To_Grid(dev, s, single, batch=True)
+        if f:
+            jgcm.Divide_By_Cos(dev, single)
+        torch.cuda.synchronize()
+        # the batch divides by cos θ in the FFT epilogue, the single path in a
+        # separate kernel: the same operation, so the same rounding
+        assert torch.equal(o, single)
+    if prec == "fp64":
+        ref = SpectralSphericalMesh(N, N + 1, nlat, 2 * nlat, 1, RADIUS)
+        for s, o, f in zip(specs, outs, flags):
+            sn = s.cpu().numpy()
+            g = np.concatenate([ref.trans_spherical_to_grid(sn[i:i + 1], ref.grid_zeros(1))
+                                for i in range(sn.shape[0])])
+            if f:
+                g = g / ref.cos_theta[None, :, None]
+            assert rel(o.cpu().numpy(), g) < 1e-12
+
+
+@pytest.mark.parametrize("N,nlat,prec", [(42, 64, "fp64"), (85, 128, "fp64"), (85, 128, "split32")])
+def test_forward_batch_equals_single_calls(jgcm, N, nlat, prec):
+    dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 5, RADIUS, precision=prec)
+    rng = np.random.default_rng(7 + N)
+    levels = [5, 1, 33, 33, 2]
+    flags = [0, 0, jgcm.BATCH_VORDIV, 0, 0]
+    grids = [cuda(rng.standard_normal((n, nlat, 2 * nlat))) for n in levels]
+    outs = [dev.spectral_zeros(n) for n in levels]
+    jgcm.Trans_Grid_To_Spherical_Batch(dev, grids, outs, flags)
+    singles = [dev.spectral_zeros(n) for n in levels]
+    for k in (0, 1, 4):
+        jgcm.Trans_Grid_To_Spherical(dev, grids[k], singles[k], batch=True)
+    jgcm.Vor_Div_From_Grid_UV(dev, grids[2], grids[3], singles[2], singles[3], batch=True)
+    torch.cuda.synchronize()
+    for o, s in zip(outs, singles):
+        assert torch.equal(o, s)
+    if prec == "fp64":
+        ref = SpectralSphericalMesh(N, N + 1, nlat, 2 * nlat, 1, RADIUS)
+        for k in (0, 1, 4):
+            gk = grids[k].cpu().numpy()
+            r = np.concatenate([ref.trans_grid_to_spherical(gk[i:i + 1], ref.spec_zeros(1))
+                                for i in range(levels[k])])
+            assert rel(outs[k].cpu().numpy(), r) < 1e-12
+        gu, gv = grids[2].cpu().numpy(), grids[3].cpu().numpy()
+        for i in range(0, 33, 8):
+            vr, dr = ref.vor_div_from_grid_uv(gu[i:i + 1], gv[i:i + 1], ref.spec_zeros(1),
+                                              ref.spec_zeros(1))
+            assert rel(outs[2][i:i + 1].cpu().numpy(), vr) < 1e-12
+            assert rel(outs[3][i:i + 1].cpu().numpy(), dr) < 1e-12
+
+
+def test_batch_argument_errors(jgcm):
+    dev = jgcm.Spectral_Spherical_Mesh(21, 22, 32, 64, 2, RADIUS)
+    g = [dev.grid_zeros(2) for _ in range(5)]
+    s = [dev.spectral_zeros(2) for _ in range(5)]
+    V = jgcm.BATCH_VORDIV
+    with pytest.raises(AssertionError):   # u without a following v
+        jgcm.Trans_Grid_To_Spherical_Batch(dev, g[:2], s[:2], [0, V])
+    with pytest.raises(AssertionError):   # 2 + 1 + 2 Legendre jobs > 4
+        jgcm.Trans_Grid_To_Spherical_Batch(dev, g, s, [V, 0, 0, V, 0])
+    with pytest.raises(AssertionError):   # unknown inverse flag
+        jgcm.Trans_Spherical_To_Grid_Batch(dev, s[:1], g[:1], [2])
+    with pytest.raises(AssertionError):   # more than 8 arrays
+        jgcm.Trans_Spherical_To_Grid_Batch(dev, s * 2, g * 2)
