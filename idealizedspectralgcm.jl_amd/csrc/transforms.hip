@@ -7,14 +7,12 @@
 //      equatorially even / odd combinations E = g_j' + g_mirror and
 //      O = g_j' - g_mirror come straight out of that spectrum.  Output: the
 //      E/O block [field][m][parity][j'] (complex, FP64).
-//   2. legendre_fwd_kernel: per (m, parity) segment, rows n = m+p, m+p+2, ..
+//   2. legendre_fwd_mfma: per (m, parity) segment, rows n = m+p, m+p+2, ..
 //      S[n] = sum_j' P~[n][j'] X[j'] with the quadrature weight and the 1/2
-//      folded into P~.  Lanes are the 64 real columns (32 complex fields);
-//      each wave owns 8 rows whose table entries are wave-uniform (scalar
-//      loads), so every LDS read feeds 8 FMAs.
+//      folded into P~, as 16 x 16 MFMA tiles (rows x 16 real columns).
 // Inverse (Trans_Spherical_To_Grid!, Spectral_Spherical_Mesh.jl:143-217):
-//   1. legendre_inv_kernel: per (m, parity) segment, E/O[j'] = sum_n P[n][j'] S[n]
-//      over degrees m..N+1.
+//   1. legendre_inv_mfma: per (m, parity) segment, E/O[j'] = sum_n P[n][j'] S[n]
+//      over degrees m..N+1, as MFMA tiles (latitudes x columns).
 //   2. ifft_unfold_kernel: rebuilds g_j' = E+O, g_mirror = E-O, packs both
 //      rows into one complex spectrum and runs one inverse FFT per pair.
 // vor/div (Vor_Div_From_Grid_UV!, :349-393) reuse both stages: the FFT stage
@@ -23,12 +21,12 @@
 // dP/dμ being opposite to that of P).
 //
 // Precision variants of the Legendre stage (the FFT stage is always FP64):
-//   FP64    : FP64 tables and accumulation.
-//   FP32    : FP32 tables and operands, FP32 accumulation.
+//   FP64    : FP64 tables, FP64 MFMA.
+//   FP32    : FP32 tables and operands, FP32 MFMA with FP32 accumulation.
 //   SPLIT32 : x = x_hi + x_lo, P = P_hi + P_lo (FP32 pieces); per block of 16
-//             latitudes the products P_hi x_hi and the cross terms
-//             P_hi x_lo + P_lo x_hi are summed in FP32, the block sums are
-//             accumulated in FP64 (compensated summation over latitude).
+//             contraction steps the products P_hi x_hi and the cross terms
+//             P_hi x_lo + P_lo x_hi are summed by FP32 MFMA, the block sums
+//             are accumulated in FP64 (compensated summation).
 #include "fft.hpp"
 #include "jgcm_internal.hpp"
 
@@ -304,9 +302,6 @@ __global__ void fold_full_kernel(const double2* __restrict__ four, double2* __re
 
 // ------------------------------------------------------------ Legendre stage
 constexpr int LCOLS = 64;   // real columns per workgroup (32 complex fields)
-constexpr int LROWS = 8;    // rows per wave
-constexpr int LWAVES = 4;
-constexpr int LKB = 16;     // latitude block for the split accumulation
 
 // Table rows are read at wave-uniform addresses; going through the constant
 // address space lets the compiler use scalar loads (one load per wave,
@@ -356,276 +351,328 @@ struct FwdJobs {
   int zstart[MAXJOB + 1];  // first blockIdx.z of each job (field chunks of 32)
 };
 
-// Stage one (m, parity) tile of the E/O block into LDS, column-major
-// [col][half+1]; col = 2*field + component.
-template <int PREC>
-__device__ __forceinline__ void stage_x(typename LdsT<PREC>::T* X, const double2* __restrict__ src,
-                                        int f0, int nf, int m, int par, int half, int nm,
-                                        int mul_i, double sgn) {
-  const int XS = half + 1;
-  staged_copy<8>(
-      nf * half,
-      [&](int idx) {
-        int jp = idx % half;
-        int fl = idx / half;
-        return src[(((size_t)(f0 + fl) * nm + m) * 2 + par) * half + jp];
-      },
-      [&](int idx, double2 v) {
-        int jp = idx % half;
-        int fl = idx / half;
-        double re = v.x, im = v.y;
-        if (mul_i) { double t = re; re = -im; im = t; }
-        re *= sgn;
-        im *= sgn;
-        X[(2 * fl) * XS + jp] = to_lds<PREC>(re);
-        X[(2 * fl + 1) * XS + jp] = to_lds<PREC>(im);
-      });
-  for (int idx = threadIdx.x + nf * half; idx < (LCOLS / 2) * half; idx += blockDim.x) {
-    int jp = idx % half;
-    int fl = idx / half;
-    X[(2 * fl) * XS + jp] = to_lds<PREC>(0.0);
-    X[(2 * fl + 1) * XS + jp] = to_lds<PREC>(0.0);
-  }
+// ------------------------------------------------ Legendre stage on MFMA
+// Per (m, parity) segment the Legendre sums are small dense products:
+//   forward  S[k][c] = sum_j P[n_k][j] X[j][c]      (rows k x latitudes j)
+//   inverse  E[j][c] = sum_k P[n_k][j] S[k][c]      (latitudes x rows)
+// with c = 2 * field + (re, im).  Each wave owns 16 columns (8 fields) and
+// accumulates 16 x 16 output tiles with v_mfma_f64_16x16x4f64 (FP64) or
+// v_mfma_f32_16x16x4f32 (FP32; split-FP32 as hi*hi + (hi*lo + lo*hi) in FP32
+// over blocks of 16 contraction steps, block sums added in FP64).  Both
+// operands go through LDS: per chunk of KC contraction steps the workgroup
+// stages the right-hand block (fields) and the segment's table rows with all
+// loads in flight at once, then runs the MFMA loop.
+// Operand layouts (both instructions): A[i][k] in lane i + 16 k, B[k][j] in
+// lane j + 16 k.  Result D[i][j] in lane j + 16 (i % 4), register i / 4 for
+// the FP64 instruction and lane j + 16 (i / 4), register i % 4 for FP32.
+typedef double dv4 __attribute__((ext_vector_type(4)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
+
+constexpr int KC = 64;  // contraction steps (latitudes or rows) staged per pass
+constexpr int KS1 = KC + 1;
+
+__device__ __forceinline__ dv4 mma64(double a, double b, dv4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ fv4 mma32(float a, float b, fv4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Accumulate sum_j' P[row_r][j'] * X[c][j'] into acc[r] for the LROWS rows
-// of this wave.  `rowp[r]` points at the table row (wave-uniform).
+// table element type in LDS: FP64, FP32, or (hi, lo)
+template <int PREC> struct TabT { using T = double; };
+template <> struct TabT<P_FP32> { using T = float; };
+template <> struct TabT<P_SPLIT> { using T = float2; };
+
 template <int PREC>
-__device__ __forceinline__ void contract_rows(double (&acc)[LROWS], const typename LdsT<PREC>::T* Xc,
-                                              const FwdTables& tab, const size_t (&rowoff)[LROWS],
-                                              int half) {
+__device__ __forceinline__ typename TabT<PREC>::T tab_load(const FwdTables& t, size_t i, bool ok) {
+  if constexpr (PREC == P_FP64) return ok ? t.d[i] : 0.0;
+  else if constexpr (PREC == P_FP32) return ok ? t.hi[i] : 0.0f;
+  else return ok ? make_float2(t.hi[i], t.lo[i]) : make_float2(0.0f, 0.0f);
+}
+
+// row of the 16 x 16 result tile held in register r of a lane with lane/16 = lk
+template <int PREC>
+__device__ __forceinline__ int tile_row(int lk, int r) {
+  if constexpr (PREC == P_FP64) return lk + 4 * r;
+  else return 4 * lk + r;
+}
+
+// Up to 4 result tiles per wave, FP64 accumulators in every precision.
+struct TileAcc {
+  double v[4][4];
+};
+
+// acc[t] += sum over steps ks < KS of A_t(ks) B(ks); bval(ks) and aval(t, ks)
+// give this lane's operand values (from LDS).
+template <int PREC, typename BF, typename AF>
+__device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS, BF bval, AF aval) {
   if constexpr (PREC == P_FP64) {
-    const auto* td = as_const(tab.d);
-    for (int j = 0; j < half; ++j) {
-      double x = Xc[j];
+    dv4 c[4];
 #pragma unroll
-      for (int r = 0; r < LROWS; ++r) acc[r] = fma(td[rowoff[r] + j], x, acc[r]);
+    for (int t = 0; t < 4; ++t) c[t] = dv4{0.0, 0.0, 0.0, 0.0};
+    for (int ks = 0; ks < KS; ++ks) {
+      double b = bval(ks);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (t < ntile) c[t] = mma64(aval(t, ks), b, c[t]);
     }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc.v[t][r] += c[t][r];
   } else if constexpr (PREC == P_FP32) {
-    const auto* th = as_const(tab.hi);
-    float a[LROWS];
+    fv4 c[4];
 #pragma unroll
-    for (int r = 0; r < LROWS; ++r) a[r] = 0.0f;
-    for (int j = 0; j < half; ++j) {
-      float x = Xc[j];
+    for (int t = 0; t < 4; ++t) c[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ks = 0; ks < KS; ++ks) {
+      float b = bval(ks);
 #pragma unroll
-      for (int r = 0; r < LROWS; ++r) a[r] = fmaf(th[rowoff[r] + j], x, a[r]);
+      for (int t = 0; t < 4; ++t)
+        if (t < ntile) c[t] = mma32(aval(t, ks), b, c[t]);
     }
 #pragma unroll
-    for (int r = 0; r < LROWS; ++r) acc[r] += (double)a[r];
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc.v[t][r] += (double)c[t][r];
   } else {
-    const auto* th = as_const(tab.hi);
-    const auto* tl = as_const(tab.lo);
-    for (int j0 = 0; j0 < half; j0 += LKB) {
-      int je = min(half, j0 + LKB);
-      float hh[LROWS], cr[LROWS];
+    for (int kb = 0; kb < KS; kb += 4) {
+      fv4 hh[4], cr[4];
 #pragma unroll
-      for (int r = 0; r < LROWS; ++r) { hh[r] = 0.0f; cr[r] = 0.0f; }
-      for (int j = j0; j < je; ++j) {
-        float2 x = Xc[j];
+      for (int t = 0; t < 4; ++t) {
+        hh[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
+        cr[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+      const int ke = min(KS, kb + 4);
+      for (int ks = kb; ks < ke; ++ks) {
+        float2 b = bval(ks);
 #pragma unroll
-        for (int r = 0; r < LROWS; ++r) {
-          float ph = th[rowoff[r] + j], pl = tl[rowoff[r] + j];
-          hh[r] = fmaf(ph, x.x, hh[r]);
-          cr[r] = fmaf(ph, x.y, fmaf(pl, x.x, cr[r]));
+        for (int t = 0; t < 4; ++t) {
+          if (t < ntile) {
+            float2 a = aval(t, ks);
+            hh[t] = mma32(a.x, b.x, hh[t]);
+            cr[t] = mma32(a.x, b.y, cr[t]);
+            cr[t] = mma32(a.y, b.x, cr[t]);
+          }
         }
       }
 #pragma unroll
-      for (int r = 0; r < LROWS; ++r) acc[r] += (double)hh[r] + (double)cr[r];
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc.v[t][r] += (double)hh[t][r] + (double)cr[t][r];
     }
   }
 }
 
-// grid: x = segment index (m, par) with m = x/2, par = x%2; y = row chunk of
-// LWAVES*LROWS rows; z = (job, chunk of 32 fields of that job).
+// Two staged copies whose global loads are issued together (up to 8 + 8 per
+// thread in flight), then written to LDS.
+template <typename L1, typename S1, typename L2, typename S2>
+__device__ __forceinline__ void staged_copy2(int n1, L1 load1, S1 store1, int n2, L2 load2,
+                                             S2 store2) {
+  using T1 = decltype(load1(0));
+  using T2 = decltype(load2(0));
+  constexpr int IT = 8;
+  const int nt = blockDim.x;
+  const int n = max(n1, n2);
+  for (int base = 0; base < n; base += IT * nt) {
+    T1 a[IT];
+    T2 b[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      int idx = base + it * nt + (int)threadIdx.x;
+      if (idx < n1) a[it] = load1(idx);
+      if (idx < n2) b[it] = load2(idx);
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      int idx = base + it * nt + (int)threadIdx.x;
+      if (idx < n1) store1(idx, a[it]);
+      if (idx < n2) store2(idx, b[it]);
+    }
+  }
+}
+
+__host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) / 16 * 16; }
+
+// grid: x = segment (m, par); y = group of 64 rows (4 tiles); z = (job,
+// chunk of 32 fields).  256 threads: wave w owns columns 16w .. 16w+15.
+// LDS: X [64 cols][KC+1], P [64 rows][KC+1].
 template <int PREC>
-__global__ __launch_bounds__(256) void legendre_fwd_kernel(FwdJobs jobs, int N, int ns, int nm,
-                                                           int half,
-                                                           const int* __restrict__ row_start,
-                                                           int nmax, int zero_top) {
+__global__ __launch_bounds__(256) void legendre_fwd_mfma(FwdJobs jobs, int ns, int nm, int half,
+                                                         const int* __restrict__ row_start,
+                                                         int nmax, int zero_top) {
   using LT = typename LdsT<PREC>::T;
+  using TT = typename TabT<PREC>::T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   LT* X = reinterpret_cast<LT*>(smem_raw);
+  TT* Pt = reinterpret_cast<TT*>(smem_raw + align16(sizeof(LT) * LCOLS * KS1));
   const int m = blockIdx.x >> 1;
   const int par = blockIdx.x & 1;
-  const int rc = blockIdx.y;
   int jobi = 0;
 #pragma unroll
   for (int q = 1; q < MAXJOB; ++q)
     if (q < jobs.njobs && (int)blockIdx.z >= jobs.zstart[q]) jobi = q;
-  const int fc = blockIdx.z - jobs.zstart[jobi];
   const FwdJob& job = jobs.job[jobi];
-  const int f0 = fc * (LCOLS / 2);
+  const int f0 = (blockIdx.z - jobs.zstart[jobi]) * (LCOLS / 2);
   const int nf = min(LCOLS / 2, job.nf - f0);
-  const size_t spec_d = (size_t)(ns + 1) * nm * 2;  // doubles per spectral field
+  const size_t spec_d = (size_t)(ns + 1) * nm * 2;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int li = lane & 15, lk = lane >> 4;
   const int nfirst = m + par;
-  // rows of this segment: n = nfirst + 2k, n <= nmax
   const int nr = (nfirst <= nmax) ? (nmax - nfirst) / 2 + 1 : 0;
-  const int k0 = rc * (LWAVES * LROWS) + wave * LROWS;
-  if (zero_top && par == 0 && rc == 0 && threadIdx.x < 2 * nf) {
+  if (zero_top && par == 0 && blockIdx.y == 0 && threadIdx.x < 2 * nf) {
     int fl = threadIdx.x >> 1;
     double* o = seg_field(job.out, f0 + fl, spec_d) + ((size_t)ns * nm + m) * 2;
     o[threadIdx.x & 1] = 0.0;
   }
-  if (rc * LWAVES * LROWS >= nr) return;  // whole workgroup idle (uniform)
-  size_t rowoff[LROWS];
-  const int rs = row_start[m];
+  const int kk0 = blockIdx.y * 64;
+  const int nrg = min(64, nr - kk0);
+  if (nrg <= 0) return;  // uniform over the workgroup
+  const int ntile = (nrg + 15) / 16;
+  const size_t rowbase = (size_t)(row_start[m] + nfirst + 2 * kk0 - m) * half;
+  TileAcc acc;
 #pragma unroll
-  for (int r = 0; r < LROWS; ++r) {
-    int k = min(k0 + r, nr - 1);
-    rowoff[r] = (size_t)__builtin_amdgcn_readfirstlane((rs + (nfirst + 2 * k - m)) * half);
-  }
-  double acc[LROWS];
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-  for (int r = 0; r < LROWS; ++r) acc[r] = 0.0;
-  const int XS = half + 1;
-  stage_x<PREC>(X, job.srcA, f0, nf, m, par, half, nm, job.mulA_i, 1.0);
-  __syncthreads();
-  if (k0 < nr) contract_rows<PREC>(acc, X + lane * XS, job.tA, rowoff, half);
-  if (job.srcB) {
-    __syncthreads();
-    stage_x<PREC>(X, job.srcB, f0, nf, m, par ^ 1, half, nm, 0, job.signB);
-    __syncthreads();
-    if (k0 < nr) contract_rows<PREC>(acc, X + lane * XS, job.tB, rowoff, half);
+    for (int r = 0; r < 4; ++r) acc.v[t][r] = 0.0;
+  const LT* Xc = X + (16 * wave + li) * KS1 + lk;
+  const TT* Pc = Pt + li * KS1 + lk;
+  for (int term = 0; term < 2; ++term) {
+    const double2* src = term == 0 ? job.srcA : job.srcB;
+    if (!src) break;
+    const FwdTables& tab = term == 0 ? job.tA : job.tB;
+    const int pt = term == 0 ? par : par ^ 1;
+    const int mul_i = term == 0 ? job.mulA_i : 0;
+    const double sgn = term == 0 ? 1.0 : job.signB;
+    const double2* srow = src + (((size_t)f0 * nm + m) * 2 + pt) * half;
+    const size_t fstride = (size_t)nm * 2 * half;
+    for (int lat0 = 0; lat0 < half; lat0 += KC) {
+      const int kcl = min(KC, half - lat0);
+      const int KSc = (kcl + 3) / 4;
+      const int kpad = 4 * KSc;
+      __syncthreads();  // the previous chunk's operands are consumed
+      staged_copy2(
+          (LCOLS / 2) * kpad,
+          [&](int idx) {
+            int jp = idx % kpad, fl = idx / kpad;
+            return (fl < nf && jp < kcl) ? srow[fl * fstride + lat0 + jp] : make_double2(0.0, 0.0);
+          },
+          [&](int idx, double2 v) {
+            int jp = idx % kpad, fl = idx / kpad;
+            double re = v.x, im = v.y;
+            if (mul_i) { double t = re; re = -im; im = t; }
+            re *= sgn;
+            im *= sgn;
+            X[(2 * fl) * KS1 + jp] = to_lds<PREC>(re);
+            X[(2 * fl + 1) * KS1 + jp] = to_lds<PREC>(im);
+          },
+          64 * kpad,
+          [&](int idx) {
+            int jp = idx % kpad, k = idx / kpad;
+            bool ok = k < nrg && jp < kcl;
+            return tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * k) * half + lat0 + jp : 0, ok);
+          },
+          [&](int idx, TT v) { Pt[(idx / kpad) * KS1 + idx % kpad] = v; });
+      __syncthreads();
+      mfma_accumulate<PREC>(
+          acc, ntile, KSc, [&](int ks) { return Xc[4 * ks]; },
+          [&](int t, int ks) { return Pc[16 * t * KS1 + 4 * ks]; });
+    }
   }
-  if (k0 < nr && lane < 2 * nf) {
-    int fl = lane >> 1;
-    int comp = lane & 1;
+  const int col = 16 * wave + li;
+  const int fl = col >> 1, comp = col & 1;
+  if (fl < nf) {
     double* base = seg_field(job.out, f0 + fl, spec_d) + (size_t)m * 2 + comp;
 #pragma unroll
-    for (int r = 0; r < LROWS; ++r) {
-      int k = k0 + r;
-      if (k < nr) {
-        int n = nfirst + 2 * k;
-        base[(size_t)n * nm * 2] = acc[r];
+    for (int t = 0; t < 4; ++t) {
+      if (t >= ntile) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int kk = kk0 + 16 * t + tile_row<PREC>(lk, r);
+        if (kk < nr) base[(size_t)(nfirst + 2 * kk) * nm * 2] = acc.v[t][r];
       }
     }
   }
 }
 
-// Inverse Legendre: E/O[b][m][par][j'] = sum_{n = m+par+2k <= ns} P[row(m,n)][j'] S[b][n][m].
-// grid: x = segment (m, par), y = field chunk.  Each wave owns 16 latitudes.
-constexpr int IJB = 16;
-
+// Inverse: grid x = segment (m, par), y = chunk of 32 fields, z = group of
+// 64 latitudes (4 tiles); 256 threads, wave w owns columns 16w .. 16w+15.
+// LDS: S [64 cols][KC+1], P [KC rows][KC+1].
 template <int PREC>
-__global__ __launch_bounds__(512) void legendre_inv_kernel(SegMap in,  // spectral fields
-                                                           double2* __restrict__ eo,
-                                                           FwdTables tab, int B, int nmax, int ns,
-                                                           int nm, int half,
-                                                           const int* __restrict__ row_start) {
+__global__ __launch_bounds__(256) void legendre_inv_mfma(SegMap in, double2* __restrict__ eo,
+                                                         FwdTables tab, int B, int nmax, int ns,
+                                                         int nm, int half,
+                                                         const int* __restrict__ row_start) {
   using LT = typename LdsT<PREC>::T;
+  using TT = typename TabT<PREC>::T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  LT* S = reinterpret_cast<LT*>(smem_raw);
+  TT* Pt = reinterpret_cast<TT*>(smem_raw + align16(sizeof(LT) * LCOLS * KS1));
   const int m = blockIdx.x >> 1;
   const int par = blockIdx.x & 1;
   const int f0 = blockIdx.y * (LCOLS / 2);
   const int nf = min(LCOLS / 2, B - f0);
+  const int lat0 = blockIdx.z * 64;
+  const int nlg = min(64, half - lat0);
+  const int nlt = (nlg + 15) / 16;
   const int nfirst = m + par;
-  // degrees n = nfirst, nfirst + 2, ... <= nmax (nmax = N+1 for the inverse
-  // transform, N for the adjoint of the forward transform)
   const int nr = (nfirst <= nmax) ? (nmax - nfirst) / 2 + 1 : 0;
-  const int SS = nr + 1;
-  LT* S = reinterpret_cast<LT*>(smem_raw);                          // [col][nr+1]
-  double* Otile = reinterpret_cast<double*>(smem_raw + ((sizeof(LT) * LCOLS * SS + 15) / 16) * 16);  // [col][half+1]
-  const int OS = half + 1;
-  // stage the segment's spectral coefficients, column-major
   const size_t spec_d = (size_t)(ns + 1) * nm * 2;
-  staged_copy<8>(
-      (LCOLS / 2) * nr,
-      [&](int idx) {
-        int k = idx % nr;
-        int fl = idx / nr;
-        if (fl >= nf) return make_double2(0.0, 0.0);
-        const double2* s = reinterpret_cast<const double2*>(seg_field(in, f0 + fl, spec_d));
-        return s[(size_t)(nfirst + 2 * k) * nm + m];
-      },
-      [&](int idx, double2 v) {
-        int k = idx % nr;
-        int fl = idx / nr;
-        S[(2 * fl) * SS + k] = to_lds<PREC>(v.x);
-        S[(2 * fl + 1) * SS + k] = to_lds<PREC>(v.y);
-      });
-  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int j0 = wave * IJB;
-  if (j0 < half) {
-    const int jn = min(IJB, half - j0);
-    double acc[IJB];
+  const int li = lane & 15, lk = lane >> 4;
+  const size_t rowbase = (size_t)(row_start[m] + nfirst - m) * half + lat0;
+  TileAcc acc;
 #pragma unroll
-    for (int q = 0; q < IJB; ++q) acc[q] = 0.0;
-    const int rs = row_start[m];
-    const LT* Sc = S + lane * SS;
-    if constexpr (PREC == P_FP64) {
-      // rows of one parity are every other packed row: stride 2*half
-      const auto* p0 = as_const(tab.d) + (size_t)(rs + nfirst - m) * half + j0;
-      const size_t rstride = 2 * (size_t)half;
-      if (jn == IJB) {
-        int k = 0;
-        for (; k + 1 < nr; k += 2) {
-          double s0 = Sc[k], s1 = Sc[k + 1];
-          const auto* pa = p0 + k * rstride;
-          const auto* pb = pa + rstride;
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-          for (int q = 0; q < IJB; ++q) acc[q] = fma(pb[q], s1, fma(pa[q], s0, acc[q]));
-        }
-        if (k < nr) {
-          double s0 = Sc[k];
-          const auto* pa = p0 + k * rstride;
+    for (int r = 0; r < 4; ++r) acc.v[t][r] = 0.0;
+  const LT* Sc = S + (16 * wave + li) * KS1 + lk;
+  const TT* Pc = Pt + lk * KS1 + li;
+  for (int kk0 = 0; kk0 < nr; kk0 += KC) {
+    const int kcr = min(KC, nr - kk0);
+    const int KSc = (kcr + 3) / 4;
+    const int kpad = 4 * KSc;
+    __syncthreads();
+    staged_copy2(
+        (LCOLS / 2) * kpad,
+        [&](int idx) {
+          int k = idx % kpad, fl = idx / kpad;
+          if (fl >= nf || k >= kcr) return make_double2(0.0, 0.0);
+          const double2* s = reinterpret_cast<const double2*>(seg_field(in, f0 + fl, spec_d));
+          return s[(size_t)(nfirst + 2 * (kk0 + k)) * nm + m];
+        },
+        [&](int idx, double2 v) {
+          int k = idx % kpad, fl = idx / kpad;
+          S[(2 * fl) * KS1 + k] = to_lds<PREC>(v.x);
+          S[(2 * fl + 1) * KS1 + k] = to_lds<PREC>(v.y);
+        },
+        kpad * 64,
+        [&](int idx) {
+          int j = idx % 64, k = idx / 64;
+          bool ok = k < kcr && j < nlg;
+          return tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * (kk0 + k)) * half + j : 0, ok);
+        },
+        [&](int idx, TT v) { Pt[(idx / 64) * KS1 + idx % 64] = v; });
+    __syncthreads();
+    mfma_accumulate<PREC>(
+        acc, nlt, KSc, [&](int ks) { return Sc[4 * ks]; },
+        [&](int t, int ks) { return Pc[4 * ks * KS1 + 16 * t]; });
+  }
+  const int col = 16 * wave + li;
+  const int fl = col >> 1, comp = col & 1;
+  if (fl < nf) {
+    double* base = reinterpret_cast<double*>(eo) +
+                   ((((size_t)(f0 + fl) * nm + m) * 2 + par) * half + lat0) * 2 + comp;
 #pragma unroll
-          for (int q = 0; q < IJB; ++q) acc[q] = fma(pa[q], s0, acc[q]);
-        }
-      } else {
-        for (int k = 0; k < nr; ++k) {
-          double s = Sc[k];
-          const auto* p = p0 + k * rstride;
-          for (int q = 0; q < jn; ++q) acc[q] = fma(p[q], s, acc[q]);
-        }
-      }
-    } else if constexpr (PREC == P_FP32) {
-      float a[IJB];
+    for (int t = 0; t < 4; ++t) {
+      if (t >= nlt) continue;
 #pragma unroll
-      for (int q = 0; q < IJB; ++q) a[q] = 0.0f;
-      for (int k = 0; k < nr; ++k) {
-        float s = Sc[k];
-        const auto* p = as_const(tab.hi) + (size_t)(rs + nfirst + 2 * k - m) * half + j0;
-#pragma unroll
-        for (int q = 0; q < IJB; ++q) if (q < jn) a[q] = fmaf(p[q], s, a[q]);
-      }
-#pragma unroll
-      for (int q = 0; q < IJB; ++q) acc[q] = a[q];
-    } else {
-      for (int kb = 0; kb < nr; kb += LKB) {
-        int ke = min(nr, kb + LKB);
-        float hh[IJB], cr[IJB];
-#pragma unroll
-        for (int q = 0; q < IJB; ++q) { hh[q] = 0.0f; cr[q] = 0.0f; }
-        for (int k = kb; k < ke; ++k) {
-          float2 s = Sc[k];
-          size_t ro = (size_t)(rs + nfirst + 2 * k - m) * half + j0;
-          const auto* ph = as_const(tab.hi) + ro;
-          const auto* pl = as_const(tab.lo) + ro;
-#pragma unroll
-          for (int q = 0; q < IJB; ++q) {
-            if (q < jn) {
-              hh[q] = fmaf(ph[q], s.x, hh[q]);
-              cr[q] = fmaf(ph[q], s.y, fmaf(pl[q], s.x, cr[q]));
-            }
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < IJB; ++q) acc[q] += (double)hh[q] + (double)cr[q];
+      for (int r = 0; r < 4; ++r) {
+        int j = 16 * t + tile_row<PREC>(lk, r);
+        if (j < nlg) base[(size_t)j * 2] = acc.v[t][r];
       }
     }
-    for (int q = 0; q < jn; ++q) Otile[lane * OS + j0 + q] = acc[q];
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < nf * half; idx += blockDim.x) {
-    int jp = idx % half;
-    int fl = idx / half;
-    eo[(((size_t)(f0 + fl) * nm + m) * 2 + par) * half + jp] =
-        make_double2(Otile[(2 * fl) * OS + jp], Otile[(2 * fl + 1) * OS + jp]);
   }
 }
 
@@ -703,6 +750,12 @@ static FwdTables tables_for(const jgcm_mesh* M, int which) {
   }
 }
 
+// LDS of the Legendre kernels: operand block [64][KC+1] + table block [64][KC+1]
+static size_t legendre_lds(int precision) {
+  size_t e = precision == JGCM_PREC_FP32 ? sizeof(float) : 8;  // double, float2
+  return align16(e * LCOLS * KS1) + e * 64 * KS1;
+}
+
 static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int zero_top) {
   int z = 0;
   for (int j = 0; j < jobs.njobs; ++j) {
@@ -712,24 +765,21 @@ static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int 
   jobs.zstart[jobs.njobs] = z;
   if (z == 0) return JGCM_OK;
   int maxrows = nmax / 2 + 1;
-  int rchunks = (maxrows + LWAVES * LROWS - 1) / (LWAVES * LROWS);
-  dim3 grd(2 * M->nm, rchunks, z);
-  int XS = M->half + 1;
+  int rgroups = ((maxrows + 15) / 16 + 3) / 4;
+  dim3 grd(2 * M->nm, rgroups, z);
+  size_t lds = legendre_lds(M->precision);
   switch (M->precision) {
     case JGCM_PREC_FP64:
-      hipLaunchKernelGGL(legendre_fwd_kernel<P_FP64>, grd, dim3(256), LCOLS * XS * sizeof(double),
-                         M->stream, jobs, M->N, M->ns, M->nm, M->half, M->d_row_start, nmax,
-                         zero_top);
+      hipLaunchKernelGGL(legendre_fwd_mfma<P_FP64>, grd, dim3(256), lds, M->stream, jobs, M->ns,
+                         M->nm, M->half, M->d_row_start, nmax, zero_top);
       break;
     case JGCM_PREC_FP32:
-      hipLaunchKernelGGL(legendre_fwd_kernel<P_FP32>, grd, dim3(256), LCOLS * XS * sizeof(float),
-                         M->stream, jobs, M->N, M->ns, M->nm, M->half, M->d_row_start, nmax,
-                         zero_top);
+      hipLaunchKernelGGL(legendre_fwd_mfma<P_FP32>, grd, dim3(256), lds, M->stream, jobs, M->ns,
+                         M->nm, M->half, M->d_row_start, nmax, zero_top);
       break;
     default:
-      hipLaunchKernelGGL(legendre_fwd_kernel<P_SPLIT>, grd, dim3(256), LCOLS * XS * sizeof(float2),
-                         M->stream, jobs, M->N, M->ns, M->nm, M->half, M->d_row_start, nmax,
-                         zero_top);
+      hipLaunchKernelGGL(legendre_fwd_mfma<P_SPLIT>, grd, dim3(256), lds, M->stream, jobs, M->ns,
+                         M->nm, M->half, M->d_row_start, nmax, zero_top);
       break;
   }
   JGCM_LAUNCH_CHECK();
@@ -740,24 +790,20 @@ static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, 
                             int nmax) {
   int B = snm.start[snm.nseg];
   int fchunks = (B + LCOLS / 2 - 1) / (LCOLS / 2);
-  dim3 grd(2 * M->nm, fchunks);
-  int waves = (M->half + IJB - 1) / IJB;
-  int nrmax = nmax / 2 + 1;
+  dim3 grd(2 * M->nm, fchunks, (M->half + 63) / 64);
   FwdTables tab = tables_for(M, table);
-  size_t esz = M->precision == JGCM_PREC_FP64 ? 8 : (M->precision == JGCM_PREC_FP32 ? 4 : 8);
-  size_t lds = ((esz * LCOLS * (nrmax + 1) + 15) / 16) * 16 + sizeof(double) * LCOLS * (M->half + 1);
-  dim3 blk(64 * waves);
+  size_t lds = legendre_lds(M->precision);
   switch (M->precision) {
     case JGCM_PREC_FP64:
-      hipLaunchKernelGGL(legendre_inv_kernel<P_FP64>, grd, blk, lds, M->stream, snm, eo, tab, B,
+      hipLaunchKernelGGL(legendre_inv_mfma<P_FP64>, grd, dim3(256), lds, M->stream, snm, eo, tab, B,
                          nmax, M->ns, M->nm, M->half, M->d_row_start);
       break;
     case JGCM_PREC_FP32:
-      hipLaunchKernelGGL(legendre_inv_kernel<P_FP32>, grd, blk, lds, M->stream, snm, eo, tab, B,
+      hipLaunchKernelGGL(legendre_inv_mfma<P_FP32>, grd, dim3(256), lds, M->stream, snm, eo, tab, B,
                          nmax, M->ns, M->nm, M->half, M->d_row_start);
       break;
     default:
-      hipLaunchKernelGGL(legendre_inv_kernel<P_SPLIT>, grd, blk, lds, M->stream, snm, eo, tab, B,
+      hipLaunchKernelGGL(legendre_inv_mfma<P_SPLIT>, grd, dim3(256), lds, M->stream, snm, eo, tab, B,
                          nmax, M->ns, M->nm, M->half, M->d_row_start);
       break;
   }

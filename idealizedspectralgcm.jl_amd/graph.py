@@ -5,30 +5,43 @@ so a leapfrog step is captured once for the first (init) step and once for
 the later steps, whose Δt no longer changes.
 """
 import ctypes
+import gc
 
 from ._lib import call
 
 
 class Step_Graph:
     def __init__(self, mesh, fn, nfields=None):
-        """Capture ``fn()`` (which must only enqueue library calls on `mesh`)."""
+        """Capture ``fn()`` (which must only enqueue library calls on `mesh`).
+
+        The cyclic garbage collector is flushed and paused for the capture: a
+        collection inside it could finalize an unrelated mesh or solver, whose
+        destroy call (stream synchronisation, hipFree) is not allowed while
+        this thread captures and would invalidate the capture."""
         self.mesh = mesh
         if nfields:
             mesh.reserve(nfields)
-        mesh.synchronize()
-        call("jgcm_graph_begin", mesh.handle)
+        gc.collect()
+        was_enabled = gc.isenabled()
+        gc.disable()
         try:
-            fn()
-        except Exception:
-            g = ctypes.c_void_p()
+            mesh.synchronize()
+            call("jgcm_graph_begin", mesh.handle)
             try:
-                call("jgcm_graph_end", mesh.handle, ctypes.byref(g))
-                call("jgcm_graph_destroy", g)
+                fn()
             except Exception:
-                pass
-            raise
-        g = ctypes.c_void_p()
-        call("jgcm_graph_end", mesh.handle, ctypes.byref(g))
+                g = ctypes.c_void_p()
+                try:
+                    call("jgcm_graph_end", mesh.handle, ctypes.byref(g))
+                    call("jgcm_graph_destroy", g)
+                except Exception:
+                    pass
+                raise
+            g = ctypes.c_void_p()
+            call("jgcm_graph_end", mesh.handle, ctypes.byref(g))
+        finally:
+            if was_enabled:
+                gc.enable()
         self._g = g
 
     def launch(self):
