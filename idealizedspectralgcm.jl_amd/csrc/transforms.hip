@@ -301,7 +301,19 @@ __global__ void fold_full_kernel(const double2* __restrict__ four, double2* __re
 }
 
 // ------------------------------------------------------------ Legendre stage
-constexpr int LCOLS = 64;   // real columns per workgroup (32 complex fields)
+// Legendre tile shape: LCOLS real columns per workgroup (LCOLS / 2 complex
+// fields, one wave per 16 columns) and TILES result tiles of 16 rows (forward)
+// or 16 latitudes (inverse) per workgroup.
+#ifndef JGCM_LCOLS
+#define JGCM_LCOLS 64
+#endif
+#ifndef JGCM_TILES
+#define JGCM_TILES 2
+#endif
+constexpr int LCOLS = JGCM_LCOLS;
+constexpr int LTHREADS = LCOLS * 4;
+constexpr int TILES = JGCM_TILES;
+constexpr int TROWS = 16 * TILES;
 
 // Table rows are read at wave-uniform addresses; going through the constant
 // address space lets the compiler use scalar loads (one load per wave,
@@ -493,11 +505,38 @@ __device__ __forceinline__ void staged_copy2(int n1, L1 load1, S1 store1, int n2
 
 __host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) / 16 * 16; }
 
+// Workgroup coordinates (x fastest) after an XCD-aware renumbering.  The
+// dispatcher hands workgroup L to XCD L % 8 in launch order.  Renumbering
+// deals chunks of XCD_CHUNK consecutive logical workgroups round-robin over
+// the XCDs: consecutive segments (m, parity) -- 8 values of m per chunk, one
+// 128-byte line of every spectral row, m being the fastest index of the
+// spectral arrays -- meet in one L2, every XCD still gets chunks from the
+// whole range of m (balanced triangle), and the heavy low-m segments are still
+// launched first.  Indices past the last full round stay in place.
+#ifndef JGCM_XCD_REMAP
+#define JGCM_XCD_REMAP 1
+#endif
+constexpr unsigned NXCD = 8;
+constexpr unsigned XCD_CHUNK = 16;
+__device__ __forceinline__ uint3 wg_coords() {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+#if JGCM_XCD_REMAP
+  const unsigned T = gx * gy * gridDim.z;
+  const unsigned full = T / (NXCD * XCD_CHUNK) * (NXCD * XCD_CHUNK);
+  if (L < full) {
+    const unsigned x = L % NXCD, slot = L / NXCD;
+    L = (x + NXCD * (slot / XCD_CHUNK)) * XCD_CHUNK + slot % XCD_CHUNK;
+  }
+#endif
+  return make_uint3(L % gx, (L / gx) % gy, L / (gx * gy));
+}
+
 // grid: x = segment (m, par); y = group of 64 rows (4 tiles); z = (job,
 // chunk of 32 fields).  256 threads: wave w owns columns 16w .. 16w+15.
 // LDS: X [64 cols][KC+1], P [64 rows][KC+1].
 template <int PREC>
-__global__ __launch_bounds__(256) void legendre_fwd_mfma(FwdJobs jobs, int ns, int nm, int half,
+__global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int ns, int nm, int half,
                                                          const int* __restrict__ row_start,
                                                          int nmax, int zero_top) {
   using LT = typename LdsT<PREC>::T;
@@ -505,14 +544,15 @@ __global__ __launch_bounds__(256) void legendre_fwd_mfma(FwdJobs jobs, int ns, i
   extern __shared__ __align__(16) unsigned char smem_raw[];
   LT* X = reinterpret_cast<LT*>(smem_raw);
   TT* Pt = reinterpret_cast<TT*>(smem_raw + align16(sizeof(LT) * LCOLS * KS1));
-  const int m = blockIdx.x >> 1;
-  const int par = blockIdx.x & 1;
+  const uint3 wg = wg_coords();
+  const int m = wg.x >> 1;
+  const int par = wg.x & 1;
   int jobi = 0;
 #pragma unroll
   for (int q = 1; q < MAXJOB; ++q)
-    if (q < jobs.njobs && (int)blockIdx.z >= jobs.zstart[q]) jobi = q;
+    if (q < jobs.njobs && (int)wg.z >= jobs.zstart[q]) jobi = q;
   const FwdJob& job = jobs.job[jobi];
-  const int f0 = (blockIdx.z - jobs.zstart[jobi]) * (LCOLS / 2);
+  const int f0 = (wg.z - jobs.zstart[jobi]) * (LCOLS / 2);
   const int nf = min(LCOLS / 2, job.nf - f0);
   const size_t spec_d = (size_t)(ns + 1) * nm * 2;
   const int lane = threadIdx.x & 63;
@@ -520,13 +560,13 @@ __global__ __launch_bounds__(256) void legendre_fwd_mfma(FwdJobs jobs, int ns, i
   const int li = lane & 15, lk = lane >> 4;
   const int nfirst = m + par;
   const int nr = (nfirst <= nmax) ? (nmax - nfirst) / 2 + 1 : 0;
-  if (zero_top && par == 0 && blockIdx.y == 0 && threadIdx.x < 2 * nf) {
+  if (zero_top && par == 0 && wg.y == 0 && threadIdx.x < 2 * nf) {
     int fl = threadIdx.x >> 1;
     double* o = seg_field(job.out, f0 + fl, spec_d) + ((size_t)ns * nm + m) * 2;
     o[threadIdx.x & 1] = 0.0;
   }
-  const int kk0 = blockIdx.y * 64;
-  const int nrg = min(64, nr - kk0);
+  const int kk0 = wg.y * TROWS;
+  const int nrg = min(TROWS, nr - kk0);
   if (nrg <= 0) return;  // uniform over the workgroup
   const int ntile = (nrg + 15) / 16;
   const size_t rowbase = (size_t)(row_start[m] + nfirst + 2 * kk0 - m) * half;
@@ -566,7 +606,7 @@ __global__ __launch_bounds__(256) void legendre_fwd_mfma(FwdJobs jobs, int ns, i
             X[(2 * fl) * KS1 + jp] = to_lds<PREC>(re);
             X[(2 * fl + 1) * KS1 + jp] = to_lds<PREC>(im);
           },
-          64 * kpad,
+          TROWS * kpad,
           [&](int idx) {
             int jp = idx % kpad, k = idx / kpad;
             bool ok = k < nrg && jp < kcl;
@@ -599,7 +639,7 @@ __global__ __launch_bounds__(256) void legendre_fwd_mfma(FwdJobs jobs, int ns, i
 // 64 latitudes (4 tiles); 256 threads, wave w owns columns 16w .. 16w+15.
 // LDS: S [64 cols][KC+1], P [KC rows][KC+1].
 template <int PREC>
-__global__ __launch_bounds__(256) void legendre_inv_mfma(SegMap in, double2* __restrict__ eo,
+__global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2* __restrict__ eo,
                                                          FwdTables tab, int B, int nmax, int ns,
                                                          int nm, int half,
                                                          const int* __restrict__ row_start) {
@@ -608,12 +648,13 @@ __global__ __launch_bounds__(256) void legendre_inv_mfma(SegMap in, double2* __r
   extern __shared__ __align__(16) unsigned char smem_raw[];
   LT* S = reinterpret_cast<LT*>(smem_raw);
   TT* Pt = reinterpret_cast<TT*>(smem_raw + align16(sizeof(LT) * LCOLS * KS1));
-  const int m = blockIdx.x >> 1;
-  const int par = blockIdx.x & 1;
-  const int f0 = blockIdx.y * (LCOLS / 2);
+  const uint3 wg = wg_coords();
+  const int m = wg.x >> 1;
+  const int par = wg.x & 1;
+  const int f0 = wg.y * (LCOLS / 2);
   const int nf = min(LCOLS / 2, B - f0);
-  const int lat0 = blockIdx.z * 64;
-  const int nlg = min(64, half - lat0);
+  const int lat0 = wg.z * TROWS;
+  const int nlg = min(TROWS, half - lat0);
   const int nlt = (nlg + 15) / 16;
   const int nfirst = m + par;
   const int nr = (nfirst <= nmax) ? (nmax - nfirst) / 2 + 1 : 0;
@@ -628,7 +669,8 @@ __global__ __launch_bounds__(256) void legendre_inv_mfma(SegMap in, double2* __r
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc.v[t][r] = 0.0;
   const LT* Sc = S + (16 * wave + li) * KS1 + lk;
-  const TT* Pc = Pt + lk * KS1 + li;
+  constexpr int PSI = TROWS + 1;
+  const TT* Pc = Pt + lk * PSI + li;
   for (int kk0 = 0; kk0 < nr; kk0 += KC) {
     const int kcr = min(KC, nr - kk0);
     const int KSc = (kcr + 3) / 4;
@@ -647,17 +689,17 @@ __global__ __launch_bounds__(256) void legendre_inv_mfma(SegMap in, double2* __r
           S[(2 * fl) * KS1 + k] = to_lds<PREC>(v.x);
           S[(2 * fl + 1) * KS1 + k] = to_lds<PREC>(v.y);
         },
-        kpad * 64,
+        kpad * TROWS,
         [&](int idx) {
-          int j = idx % 64, k = idx / 64;
+          int j = idx % TROWS, k = idx / TROWS;
           bool ok = k < kcr && j < nlg;
           return tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * (kk0 + k)) * half + j : 0, ok);
         },
-        [&](int idx, TT v) { Pt[(idx / 64) * KS1 + idx % 64] = v; });
+        [&](int idx, TT v) { Pt[(idx / TROWS) * PSI + idx % TROWS] = v; });
     __syncthreads();
     mfma_accumulate<PREC>(
         acc, nlt, KSc, [&](int ks) { return Sc[4 * ks]; },
-        [&](int t, int ks) { return Pc[4 * ks * KS1 + 16 * t]; });
+        [&](int t, int ks) { return Pc[4 * ks * PSI + 16 * t]; });
   }
   const int col = 16 * wave + li;
   const int fl = col >> 1, comp = col & 1;
@@ -753,7 +795,8 @@ static FwdTables tables_for(const jgcm_mesh* M, int which) {
 // LDS of the Legendre kernels: operand block [64][KC+1] + table block [64][KC+1]
 static size_t legendre_lds(int precision) {
   size_t e = precision == JGCM_PREC_FP32 ? sizeof(float) : 8;  // double, float2
-  return align16(e * LCOLS * KS1) + e * 64 * KS1;
+  size_t pf = (size_t)TROWS * KS1, pi = (size_t)KC * (TROWS + 1);
+  return align16(e * LCOLS * KS1) + e * (pf > pi ? pf : pi);
 }
 
 static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int zero_top) {
@@ -765,20 +808,20 @@ static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int 
   jobs.zstart[jobs.njobs] = z;
   if (z == 0) return JGCM_OK;
   int maxrows = nmax / 2 + 1;
-  int rgroups = ((maxrows + 15) / 16 + 3) / 4;
+  int rgroups = ((maxrows + 15) / 16 + TILES - 1) / TILES;
   dim3 grd(2 * M->nm, rgroups, z);
   size_t lds = legendre_lds(M->precision);
   switch (M->precision) {
     case JGCM_PREC_FP64:
-      hipLaunchKernelGGL(legendre_fwd_mfma<P_FP64>, grd, dim3(256), lds, M->stream, jobs, M->ns,
+      hipLaunchKernelGGL(legendre_fwd_mfma<P_FP64>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
                          M->nm, M->half, M->d_row_start, nmax, zero_top);
       break;
     case JGCM_PREC_FP32:
-      hipLaunchKernelGGL(legendre_fwd_mfma<P_FP32>, grd, dim3(256), lds, M->stream, jobs, M->ns,
+      hipLaunchKernelGGL(legendre_fwd_mfma<P_FP32>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
                          M->nm, M->half, M->d_row_start, nmax, zero_top);
       break;
     default:
-      hipLaunchKernelGGL(legendre_fwd_mfma<P_SPLIT>, grd, dim3(256), lds, M->stream, jobs, M->ns,
+      hipLaunchKernelGGL(legendre_fwd_mfma<P_SPLIT>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
                          M->nm, M->half, M->d_row_start, nmax, zero_top);
       break;
   }
@@ -790,20 +833,20 @@ static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, 
                             int nmax) {
   int B = snm.start[snm.nseg];
   int fchunks = (B + LCOLS / 2 - 1) / (LCOLS / 2);
-  dim3 grd(2 * M->nm, fchunks, (M->half + 63) / 64);
+  dim3 grd(2 * M->nm, fchunks, (M->half + TROWS - 1) / TROWS);
   FwdTables tab = tables_for(M, table);
   size_t lds = legendre_lds(M->precision);
   switch (M->precision) {
     case JGCM_PREC_FP64:
-      hipLaunchKernelGGL(legendre_inv_mfma<P_FP64>, grd, dim3(256), lds, M->stream, snm, eo, tab, B,
+      hipLaunchKernelGGL(legendre_inv_mfma<P_FP64>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
                          nmax, M->ns, M->nm, M->half, M->d_row_start);
       break;
     case JGCM_PREC_FP32:
-      hipLaunchKernelGGL(legendre_inv_mfma<P_FP32>, grd, dim3(256), lds, M->stream, snm, eo, tab, B,
+      hipLaunchKernelGGL(legendre_inv_mfma<P_FP32>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
                          nmax, M->ns, M->nm, M->half, M->d_row_start);
       break;
     default:
-      hipLaunchKernelGGL(legendre_inv_mfma<P_SPLIT>, grd, dim3(256), lds, M->stream, snm, eo, tab, B,
+      hipLaunchKernelGGL(legendre_inv_mfma<P_SPLIT>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
                          nmax, M->ns, M->nm, M->half, M->d_row_start);
       break;
   }
