@@ -12,9 +12,10 @@ from .mesh import (Spectral_Spherical_Mesh, Trans_Spherical_To_Grid, Trans_Grid_
                    Apply_Laplacian, Compute_Gradient_Cos, Compute_Gradients,
                    Add_Horizontal_Advection, Area_Weighted_Global_Mean,
                    Trans_Spherical_To_Grid_Batch, Trans_Grid_To_Spherical_Batch, BATCH_DIVCOS,
-                   BATCH_VORDIV)
+                   BATCH_VORDIV, INV_GRAD_LAMBDA, INV_GRAD_THETA, INV_UCOS, INV_VCOS)
 from .time_integrator import (Filtered_Leapfrog, Update_Init_Step, Get_Δt, Get_ξ,
-                              Compute_Spectral_Damping, Filtered_Leapfrog_step, Damped_Leapfrog)
+                              Compute_Spectral_Damping, Filtered_Leapfrog_step, Damped_Leapfrog,
+                              Damped_Leapfrog_Batch)
 from .dyn_data import Atmo_Data, Compute_Abs_Vor, Dyn_Data, Time_Advance
 from .dynamics import (Barotropic_Dynamics, Shallow_Water_Physics, Implicit_Correction,
                        Shallow_Water_Dynamics)
@@ -32,9 +33,9 @@ __all__ = [
     "Compute_Ucos_Vcos_From_Vor_Div", "Apply_Laplacian", "Compute_Gradient_Cos",
     "Compute_Gradients", "Add_Horizontal_Advection", "Area_Weighted_Global_Mean",
     "Trans_Spherical_To_Grid_Batch", "Trans_Grid_To_Spherical_Batch", "BATCH_DIVCOS",
-    "BATCH_VORDIV",
+    "BATCH_VORDIV", "INV_GRAD_LAMBDA", "INV_GRAD_THETA", "INV_UCOS", "INV_VCOS",
     "Filtered_Leapfrog", "Update_Init_Step", "Get_Δt", "Get_ξ", "Compute_Spectral_Damping",
-    "Filtered_Leapfrog_step", "Damped_Leapfrog", "Atmo_Data", "Compute_Abs_Vor", "Dyn_Data",
+    "Filtered_Leapfrog_step", "Damped_Leapfrog", "Damped_Leapfrog_Batch", "Atmo_Data", "Compute_Abs_Vor", "Dyn_Data",
     "Time_Advance", "Barotropic_Dynamics", "Shallow_Water_Physics", "Implicit_Correction",
     "Shallow_Water_Dynamics", "Step_Graph",
 ]

@@ -54,7 +54,8 @@ SIGNATURES = {
     "jgcm_trans_grid_to_spherical": [P, P, P, I],
     "jgcm_trans_spherical_to_grid": [P, P, P, I],
     "jgcm_trans_spherical_to_grid_batch": [P, I, ctypes.POINTER(P), ctypes.POINTER(P),
-                                           ctypes.POINTER(I), ctypes.POINTER(I)],
+                                           ctypes.POINTER(P), ctypes.POINTER(I),
+                                           ctypes.POINTER(I)],
     "jgcm_trans_grid_to_spherical_batch": [P, I, ctypes.POINTER(P), ctypes.POINTER(P),
                                            ctypes.POINTER(I), ctypes.POINTER(I)],
     "jgcm_trans_grid_to_fourier": [P, P, P, I],
@@ -82,6 +83,7 @@ SIGNATURES = {
     "jgcm_compute_spectral_damping": [P, P, P, P, I],
     "jgcm_filtered_leapfrog": [P, P, P, P, P, I],
     "jgcm_damped_leapfrog": [P, P, P, P, P, I],
+    "jgcm_damped_leapfrog_batch": [P, I] + [ctypes.POINTER(P)] * 4 + [ctypes.POINTER(I)] * 2,
     "jgcm_sw_implicit_correction": [P, P, P, P, P, D, P, P, I],
     "jgcm_vorticity_flux": [P, P, P, P, D, P, P, P, I, I],
     "jgcm_axpby": [P, SZ, D, P, D, P, P],
@@ -102,6 +104,7 @@ SIGNATURES = {
     "jgcm_semi_implicit_update_init_step": [P],
     "jgcm_semi_implicit_get": [P, I, P, SZ],
     "jgcm_implicit_correction": [P] * 10,
+    "jgcm_hs_energy_implicit_correction": [P] * 11,
     "jgcm_hs_physics": [P, P, ctypes.c_int64] + [P] * 15 + [I],
     "jgcm_hs_column_dynamics": [P] * 27,
     "jgcm_hs_corrections": [P] * 8,

@@ -11,8 +11,9 @@ namespace jgcm {
 int fft_fold(const jgcm_mesh* M, const double* grid, double2* eo, const double* cosmul, int B);
 int ifft_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const double* divcos, int B);
 int legendre_inv(const jgcm_mesh* M, const double2* snm, double2* eo, int B);
-int inverse_batch(const jgcm_mesh* M, int nseg, const double* const* snm, double* const* grid,
-                  const int* nlev, const int* flags);
+int inverse_batch(const jgcm_mesh* M, int nseg, const double* const* snm,
+                  const double* const* snm_b, double* const* grid, const int* nlev,
+                  const int* flags);
 
 static inline int nblocks(size_t n, int per = 256) {
   size_t b = (n + per - 1) / per;
@@ -188,15 +189,18 @@ int gradient_cos(const jgcm_mesh* M, const double2* h, double2* dl, double2* dt,
   return JGCM_OK;
 }
 
-// inverse transforms of two spectral arrays with the rows divided by cos θ,
-// in one Legendre launch and one FFT launch
-static int inverse_pair_divcos(const jgcm_mesh* M, const double2* sa, const double2* sb,
-                               double* ga, double* gb, int B) {
-  const double* s[2] = {reinterpret_cast<const double*>(sa), reinterpret_cast<const double*>(sb)};
+// Two inverse transforms with the rows divided by cos θ, in one Legendre launch
+// and one FFT launch; the spectral operators (gradient, u cos / v cos) are
+// applied while the Legendre stage stages its coefficients.
+static int inverse_pair_divcos(const jgcm_mesh* M, const double* sa, const double* sa2, int op_a,
+                               const double* sb, const double* sb2, int op_b, double* ga,
+                               double* gb, int B) {
+  const double* s[2] = {sa, sb};
+  const double* s2[2] = {sa2, sb2};
   double* g[2] = {ga, gb};
   int nl[2] = {B, B};
-  int fl[2] = {1, 1};
-  return inverse_batch(M, 2, s, g, nl, fl);
+  int fl[2] = {1 | op_a, 1 | op_b};
+  return inverse_batch(M, 2, s, s2, g, nl, fl);
 }
 
 }  // namespace jgcm
@@ -289,10 +293,8 @@ int jgcm_uv_grid_from_vor_div(jgcm_mesh_t* mesh, const double* vor, const double
   JGCM_MESH_PROLOGUE(mesh, nlev);
   JGCM_REQUIRE(vor && div && grid_u && grid_v, "NULL field pointer");
   if (int rc = ensure_scratch(mesh, 2 * nlev)) return rc;
-  double2* su = (double2*)mesh->d_spec_a;
-  double2* sv = (double2*)mesh->d_spec_b;
-  if (int rc = ucos_vcos(mesh, (const double2*)vor, (const double2*)div, su, sv, nlev)) return rc;
-  return inverse_pair_divcos(mesh, su, sv, grid_u, grid_v, nlev);
+  return inverse_pair_divcos(mesh, vor, div, JGCM_INV_UCOS, vor, div, JGCM_INV_VCOS, grid_u, grid_v,
+                             nlev);
 }
 
 int jgcm_apply_laplacian(jgcm_mesh_t* mesh, double* snm, int nlev) {
@@ -318,10 +320,8 @@ int jgcm_compute_gradients(jgcm_mesh_t* mesh, const double* spe_hs, double* grid
   JGCM_MESH_PROLOGUE(mesh, nlev);
   JGCM_REQUIRE(spe_hs && grid_dlambda && grid_dtheta, "NULL field pointer");
   if (int rc = ensure_scratch(mesh, 2 * nlev)) return rc;
-  double2* sl = (double2*)mesh->d_spec_a;
-  double2* st = (double2*)mesh->d_spec_b;
-  if (int rc = gradient_cos(mesh, (const double2*)spe_hs, sl, st, nlev)) return rc;
-  return inverse_pair_divcos(mesh, sl, st, grid_dlambda, grid_dtheta, nlev);
+  return inverse_pair_divcos(mesh, spe_hs, nullptr, JGCM_INV_GRAD_LAMBDA, spe_hs, nullptr,
+                             JGCM_INV_GRAD_THETA, grid_dlambda, grid_dtheta, nlev);
 }
 
 int jgcm_add_horizontal_advection(jgcm_mesh_t* mesh, const double* spe_hs, const double* grid_u,

@@ -808,7 +808,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     double xi, SIRef r, const double* __restrict__ eig, const double* __restrict__ wave,
     int num_wavenumbers, const double2* div_c, const double2* div_p, const double2* lnps_c,
     const double2* lnps_p, const double2* t_c, const double2* t_p, double2* ddiv, double2* dlnps,
-    double2* dtt) {
+    double2* dtt, double2* energy) {
   extern __shared__ double2 lds[];  // nd * 64 entries: level sums, then the wave-matrix operand
   const TileIdx tx = tile_idx<KPT>(spec_per_level);
   const size_t idx = tx.c;
@@ -849,6 +849,13 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     size_t o = (size_t)k * spec_per_level + idx;
     double hp = ok ? r.h[k] * ps_ref : 0.0;
     double2 d = ok ? ddiv[o] : d2(0.0, 0.0);
+    if (energy && ok) {
+      // Apply_Laplacian!(spe_energy); spe_δdiv .-= spe_energy  (SD:309-310)
+      double2 en = energy[o];
+      double2 lap = d2(e * en.x, e * en.y);
+      energy[o] = lap;
+      d = d2sub(d, lap);
+    }
     A[i] = d2(d.x - e * (G[i].x + hp * lt.x), d.y - e * (G[i].y + hp * lt.y));
   }
   // wave-matrix solve for n <= num_wavenumbers (Semi_Implicit.jl:487-497):
@@ -1252,10 +1259,10 @@ static SIRef siref(const jgcm_semi_implicit* si) {
                si->vc->d_coef + (nd + 1)};
 }
 
-// Implicit_Correction!(semi_implicit, vert_coord, atmo_data, ...)  Semi_Implicit.jl:462-513
-int jgcm_implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, const double* div_p,
-                             const double* lnps_c, const double* lnps_p, const double* t_c,
-                             const double* t_p, double* ddiv, double* dlnps, double* dt) {
+static int implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, const double* div_p,
+                               const double* lnps_c, const double* lnps_p, const double* t_c,
+                               const double* t_p, double* ddiv, double* dlnps, double* dt,
+                               double* energy) {
   JGCM_REQUIRE(si && div_c && div_p && lnps_c && lnps_p && t_c && t_p && ddiv && dlnps && dt,
                "NULL argument");
   jgcm_mesh* M = si->vc->mesh;
@@ -1267,9 +1274,27 @@ int jgcm_implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, cons
                      siref(si), M->d_eig, si->d_wave, si->num_wavenumbers, (const double2*)div_c,
                      (const double2*)div_p, (const double2*)lnps_c, (const double2*)lnps_p,
                      (const double2*)t_c, (const double2*)t_p, (double2*)ddiv, (double2*)dlnps,
-                     (double2*)dt);
+                     (double2*)dt, (double2*)energy);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
+}
+
+// Implicit_Correction!(semi_implicit, vert_coord, atmo_data, ...)  Semi_Implicit.jl:462-513
+int jgcm_implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, const double* div_p,
+                             const double* lnps_c, const double* lnps_p, const double* t_c,
+                             const double* t_p, double* ddiv, double* dlnps, double* dt) {
+  return implicit_correction(si, div_c, div_p, lnps_c, lnps_p, t_c, t_p, ddiv, dlnps, dt, nullptr);
+}
+
+// Apply_Laplacian!(spe_energy); spe_δdiv .-= spe_energy; Implicit_Correction!  (SD:309-316)
+int jgcm_hs_energy_implicit_correction(jgcm_semi_implicit_t* si, double* spe_energy,
+                                       const double* div_c, const double* div_p,
+                                       const double* lnps_c, const double* lnps_p,
+                                       const double* t_c, const double* t_p, double* ddiv,
+                                       double* dlnps, double* dt) {
+  JGCM_REQUIRE(spe_energy != nullptr, "NULL argument");
+  return implicit_correction(si, div_c, div_p, lnps_c, lnps_p, t_c, t_p, ddiv, dlnps, dt,
+                             spe_energy);
 }
 
 // per-workgroup partial sums of the area means live in the column-sum scratch

@@ -24,44 +24,72 @@ static inline int nblk(size_t n) {
 #define GRID_STRIDE(i, n) \
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
 
-// Compute_Spectral_Damping! (TI:76-80) fused with Filtered_Leapfrog! (TI:93-100).
-// `mode` bit 0: apply damping, bit 1: apply leapfrog.
+// Compute_Spectral_Damping! (TI:76-80) fused with Filtered_Leapfrog! (TI:93-100)
+// for element i.  `mode` bit 0: apply damping, bit 1: apply leapfrog.
+__device__ __forceinline__ void damped_leapfrog_elem(double2* __restrict__ dQ,
+                                                     const double2* __restrict__ Qp,
+                                                     double2* __restrict__ Qc,
+                                                     double2* __restrict__ Qn,
+                                                     const double* __restrict__ damping, int nm,
+                                                     int ns, size_t i, int init_step, double dt,
+                                                     double r, int mode) {
+  int m = (int)(i % nm);
+  int n = (int)((i / nm) % (ns + 1));
+  double2 q = dQ[i];
+  if (mode & 1) {
+    double d = damping[m * (ns + 1) + n];
+    if (init_step) {
+      double2 c = Qc[i];
+      double den = 1.0 + dt * d;
+      q = make_double2((q.x - c.x * d) / den, (q.y - c.y * d) / den);
+    } else {
+      double2 p = Qp[i];
+      double den = 1.0 + (2.0 * dt) * d;
+      q = make_double2((q.x - p.x * d) / den, (q.y - p.y * d) / den);
+    }
+    dQ[i] = q;
+  }
+  if (mode & 2) {
+    double2 c = Qc[i];
+    if (init_step) {
+      double2 nx = make_double2(c.x + dt * q.x, c.y + dt * q.y);
+      Qn[i] = nx;
+      Qc[i] = make_double2(c.x + r * (-1.0 * c.x + nx.x), c.y + r * (-1.0 * c.y + nx.y));
+    } else {
+      double2 p = Qp[i];
+      double2 c1 = make_double2(c.x + r * (p.x - 2.0 * c.x), c.y + r * (p.y - 2.0 * c.y));
+      double2 nx = make_double2(p.x + (2.0 * dt) * q.x, p.y + (2.0 * dt) * q.y);
+      Qn[i] = nx;
+      Qc[i] = make_double2(c1.x + r * nx.x, c1.y + r * nx.y);
+    }
+  }
+}
+
 __global__ void damped_leapfrog_kernel(double2* __restrict__ dQ, const double2* __restrict__ Qp,
                                        double2* __restrict__ Qc, double2* __restrict__ Qn,
                                        const double* __restrict__ damping, int nm, int ns,
                                        size_t n_all, int init_step, double dt, double r, int mode) {
-  GRID_STRIDE(i, n_all) {
-    int m = (int)(i % nm);
-    int n = (int)((i / nm) % (ns + 1));
-    double2 q = dQ[i];
-    if (mode & 1) {
-      double d = damping[m * (ns + 1) + n];
-      if (init_step) {
-        double2 c = Qc[i];
-        double den = 1.0 + dt * d;
-        q = make_double2((q.x - c.x * d) / den, (q.y - c.y * d) / den);
-      } else {
-        double2 p = Qp[i];
-        double den = 1.0 + (2.0 * dt) * d;
-        q = make_double2((q.x - p.x * d) / den, (q.y - p.y * d) / den);
-      }
-      dQ[i] = q;
-    }
-    if (mode & 2) {
-      double2 c = Qc[i];
-      if (init_step) {
-        double2 nx = make_double2(c.x + dt * q.x, c.y + dt * q.y);
-        Qn[i] = nx;
-        Qc[i] = make_double2(c.x + r * (-1.0 * c.x + nx.x), c.y + r * (-1.0 * c.y + nx.y));
-      } else {
-        double2 p = Qp[i];
-        double2 c1 = make_double2(c.x + r * (p.x - 2.0 * c.x), c.y + r * (p.y - 2.0 * c.y));
-        double2 nx = make_double2(p.x + (2.0 * dt) * q.x, p.y + (2.0 * dt) * q.y);
-        Qn[i] = nx;
-        Qc[i] = make_double2(c1.x + r * nx.x, c1.y + r * nx.y);
-      }
-    }
-  }
+  GRID_STRIDE(i, n_all)
+  damped_leapfrog_elem(dQ, Qp, Qc, Qn, damping, nm, ns, i, init_step, dt, r, mode);
+}
+
+// several prognostic fields in one launch: blockIdx.y = field
+constexpr int MAXLF = 8;
+struct LeapfrogSet {
+  double2* dQ[MAXLF];
+  const double2* Qp[MAXLF];
+  double2* Qc[MAXLF];
+  double2* Qn[MAXLF];
+  size_t n[MAXLF];
+  int mode[MAXLF];
+};
+
+__global__ void damped_leapfrog_batch_kernel(LeapfrogSet s, const double* __restrict__ damping,
+                                             int nm, int ns, int init_step, double dt, double r) {
+  const int f = blockIdx.y;
+  GRID_STRIDE(i, s.n[f])
+  damped_leapfrog_elem(s.dQ[f], s.Qp[f], s.Qc[f], s.Qn[f], damping, nm, ns, i, init_step, dt, r,
+                       s.mode[f]);
 }
 
 // SW Implicit_Correction! (Shallow_Water_Dynamics.jl:36-51)
@@ -281,6 +309,35 @@ int jgcm_damped_leapfrog(jgcm_leapfrog_t* lf, double* dQ, const double* Qp, doub
                          int nlev) {
   JGCM_REQUIRE(dQ && Qp && Qc && Qn, "NULL field pointer");
   return launch_damped_leapfrog(lf, dQ, Qp, Qc, Qn, nlev, 3);
+}
+
+int jgcm_damped_leapfrog_batch(jgcm_leapfrog_t* lf, int nfields, double* const* dQ,
+                               const double* const* Qp, double* const* Qc, double* const* Qn,
+                               const int* nlev, const int* mode) {
+  JGCM_REQUIRE(lf != nullptr, "leapfrog is NULL");
+  JGCM_REQUIRE(nfields >= 1 && nfields <= MAXLF, "nfields must be in 1..%d, got %d", MAXLF, nfields);
+  JGCM_REQUIRE(dQ && Qp && Qc && Qn && nlev && mode, "NULL argument");
+  jgcm_mesh* M = lf->mesh;
+  LeapfrogSet s{};
+  size_t maxn = 0;
+  for (int f = 0; f < nfields; ++f) {
+    JGCM_REQUIRE(mode[f] >= 1 && mode[f] <= 3, "mode[%d] must be 1, 2 or 3, got %d", f, mode[f]);
+    JGCM_REQUIRE(nlev[f] >= 1, "nlev[%d] must be >= 1", f);
+    JGCM_REQUIRE(dQ[f] && Qp[f] && Qc[f] && ((mode[f] & 2) == 0 || Qn[f]),
+                 "NULL field pointer (field %d)", f);
+    s.dQ[f] = (double2*)dQ[f];
+    s.Qp[f] = (const double2*)Qp[f];
+    s.Qc[f] = (double2*)Qc[f];
+    s.Qn[f] = (double2*)Qn[f];
+    s.n[f] = (size_t)nlev[f] * M->spec_elems();
+    s.mode[f] = mode[f];
+    maxn = s.n[f] > maxn ? s.n[f] : maxn;
+  }
+  hipLaunchKernelGGL(damped_leapfrog_batch_kernel, dim3(nblk(maxn), nfields), dim3(256), 0, M->stream,
+                     s, lf->d_damping, M->nm, M->ns, lf->init_step ? 1 : 0, (double)lf->dt,
+                     lf->robert_coef);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
 }
 
 int jgcm_sw_implicit_correction(jgcm_leapfrog_t* lf, const double* div_c, const double* div_p,

@@ -37,14 +37,36 @@ enum { P_FP64 = 0, P_FP32 = 1, P_SPLIT = 2 };
 // A launch may gather its level-fields from up to MAXSEG separate arrays
 // (e.g. vor, div, u cos, v cos, T and ln ps back-transformed together):
 // global field b lives in segment k with start[k] <= b < start[k+1], at
-// ptr[k] + (b - start[k]) * per-field size.  flag[k] selects the cos θ
-// multiply (forward FFT) or divide (inverse FFT) for that segment.
+// ptr[k] + (b - start[k]) * per-field size.  flag[k] bit 0 selects the cos θ
+// multiply (forward FFT) or divide (inverse FFT) for that segment; for the
+// inverse Legendre stage bits 1..3 name a spectral operator applied while the
+// coefficients are staged (SPEC_OP_*), ptr2[k] being its second operand.
 constexpr int MAXSEG = 8;
 struct SegMap {
   int nseg;
   int start[MAXSEG + 1];
   double* ptr[MAXSEG];
+  const double* ptr2[MAXSEG];
   int flag[MAXSEG];
+};
+
+// Spectral operators fused into the inverse transform's staging (flag bits 1..3):
+enum {
+  SPEC_OP_NONE = 0,
+  SPEC_OP_GRAD_LAMBDA = 1,  // cos θ ∂λ h / a      (Compute_Gradient_Cos!, SSM:677-680)
+  SPEC_OP_GRAD_THETA = 2,   // cos θ ∂θ h / a      (SSM:682-687)
+  SPEC_OP_UCOS = 3,         // u cos θ from (vor, div)  (Compute_Ucos_Vcos_From_Vor_Div!, SSM:568-576)
+  SPEC_OP_VCOS = 4,         // v cos θ from (vor, div)
+};
+
+// coefficient tables [m][ns+1] of those operators
+struct SpecOpTables {
+  const double* dlam;
+  const double* dthm;
+  const double* dthp;
+  const double* uvc;
+  const double* uvm;
+  const double* uvp;
 };
 
 __host__ __device__ __forceinline__ int seg_find(const SegMap& s, int b) {
@@ -635,6 +657,70 @@ __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int 
   }
 }
 
+// Spectral coefficient (n, m) of global field b of an inverse-transform
+// segment map, after the segment's operator (SPEC_OP_*).  The arithmetic and
+// its order are those of the stand-alone operator kernels.
+__device__ __forceinline__ double2 staged_coefficient(const SegMap& in, const SpecOpTables& t,
+                                                      int b, size_t spec_d, int n, int m, int nm,
+                                                      int ns) {
+  const int k = seg_find(in, b);
+  const size_t off = (size_t)(b - in.start[k]) * spec_d;
+  const double2* h = reinterpret_cast<const double2*>(in.ptr[k] + off);
+  const int op = (in.flag[k] >> 1) & 7;
+  const size_t i = (size_t)n * nm + m;
+  if (op == SPEC_OP_NONE) return h[i];
+  const int c = m * (ns + 1) + n;
+  if (op == SPEC_OP_GRAD_LAMBDA) {
+    double2 v = h[i];
+    double a = t.dlam[c];
+    return make_double2(a * (-v.y), a * v.x);
+  }
+  if (op == SPEC_OP_GRAD_THETA) {
+    double2 r = make_double2(0.0, 0.0);
+    if (n <= ns - 1) {
+      double2 hp = h[i + nm];
+      r = make_double2(t.dthp[c] * hp.x, t.dthp[c] * hp.y);
+    }
+    if (n >= 1) {
+      double2 hm = h[i - nm];
+      r = make_double2(r.x + t.dthm[c] * hm.x, r.y + t.dthm[c] * hm.y);
+    }
+    return r;
+  }
+  // u cos θ / v cos θ: ptr = vor, ptr2 = div
+  const double2* vor = h;
+  const double2* div = reinterpret_cast<const double2*>(in.ptr2[k] + off);
+  const double cc = t.uvc[c];
+  if (op == SPEC_OP_UCOS) {
+    double2 d = div[i];
+    double2 u = make_double2(cc * (-d.y), cc * d.x);
+    if (n >= 1) {
+      double a = t.uvm[c];
+      double2 vm = vor[i - nm];
+      u = make_double2(u.x - a * vm.x, u.y - a * vm.y);
+    }
+    if (n <= ns - 1) {
+      double a = t.uvp[c];
+      double2 vp = vor[i + nm];
+      u = make_double2(u.x - a * vp.x, u.y - a * vp.y);
+    }
+    return u;
+  }
+  double2 v = vor[i];
+  double2 w = make_double2(cc * (-v.y), cc * v.x);
+  if (n >= 1) {
+    double a = t.uvm[c];
+    double2 dm = div[i - nm];
+    w = make_double2(w.x + a * dm.x, w.y + a * dm.y);
+  }
+  if (n <= ns - 1) {
+    double a = t.uvp[c];
+    double2 dp = div[i + nm];
+    w = make_double2(w.x + a * dp.x, w.y + a * dp.y);
+  }
+  return w;
+}
+
 // Inverse: grid x = segment (m, par), y = chunk of 32 fields, z = group of
 // 64 latitudes (4 tiles); 256 threads, wave w owns columns 16w .. 16w+15.
 // LDS: S [64 cols][KC+1], P [KC rows][KC+1].
@@ -642,7 +728,8 @@ template <int PREC>
 __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2* __restrict__ eo,
                                                          FwdTables tab, int B, int nmax, int ns,
                                                          int nm, int half,
-                                                         const int* __restrict__ row_start) {
+                                                         const int* __restrict__ row_start,
+                                                         SpecOpTables ops) {
   using LT = typename LdsT<PREC>::T;
   using TT = typename TabT<PREC>::T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -681,8 +768,7 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
         [&](int idx) {
           int k = idx % kpad, fl = idx / kpad;
           if (fl >= nf || k >= kcr) return make_double2(0.0, 0.0);
-          const double2* s = reinterpret_cast<const double2*>(seg_field(in, f0 + fl, spec_d));
-          return s[(size_t)(nfirst + 2 * (kk0 + k)) * nm + m];
+          return staged_coefficient(in, ops, f0 + fl, spec_d, nfirst + 2 * (kk0 + k), m, nm, ns);
         },
         [&](int idx, double2 v) {
           int k = idx % kpad, fl = idx / kpad;
@@ -836,18 +922,19 @@ static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, 
   dim3 grd(2 * M->nm, fchunks, (M->half + TROWS - 1) / TROWS);
   FwdTables tab = tables_for(M, table);
   size_t lds = legendre_lds(M->precision);
+  SpecOpTables ops{M->d_dlam, M->d_dthm, M->d_dthp, M->d_uvc, M->d_uvm, M->d_uvp};
   switch (M->precision) {
     case JGCM_PREC_FP64:
       hipLaunchKernelGGL(legendre_inv_mfma<P_FP64>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start);
+                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops);
       break;
     case JGCM_PREC_FP32:
       hipLaunchKernelGGL(legendre_inv_mfma<P_FP32>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start);
+                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops);
       break;
     default:
       hipLaunchKernelGGL(legendre_inv_mfma<P_SPLIT>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start);
+                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops);
       break;
   }
   JGCM_LAUNCH_CHECK();
@@ -940,17 +1027,20 @@ int fold_full(const jgcm_mesh* M, const double2* four, double2* eo, int B) {
 }
 
 // Back-transform of several spectral arrays in one Legendre launch and one FFT
-// launch; flag & 1 divides the rows of that array by cos θ.
-int inverse_batch(const jgcm_mesh* M, int nseg, const double* const* snm, double* const* grid,
-                  const int* nlev, const int* flags) {
+// launch; flag & 1 divides the rows of that array by cos θ, flag >> 1 names a
+// spectral operator applied on the way in (SPEC_OP_*, second operand snm_b[k]).
+int inverse_batch(const jgcm_mesh* M, int nseg, const double* const* snm,
+                  const double* const* snm_b, double* const* grid, const int* nlev,
+                  const int* flags) {
   SegMap in{}, out{};
   in.nseg = out.nseg = nseg;
   int B = 0;
   for (int k = 0; k < nseg; ++k) {
     in.start[k] = out.start[k] = B;
     in.ptr[k] = const_cast<double*>(snm[k]);
+    in.ptr2[k] = snm_b ? snm_b[k] : nullptr;
     out.ptr[k] = grid[k];
-    in.flag[k] = 0;
+    in.flag[k] = flags ? (flags[k] & ~1) : 0;
     out.flag[k] = flags ? (flags[k] & 1) : 0;
     B += nlev[k];
   }
@@ -1061,7 +1151,8 @@ int jgcm_vor_div_from_grid_uv(jgcm_mesh_t* mesh, const double* grid_u, const dou
 }
 
 int jgcm_trans_spherical_to_grid_batch(jgcm_mesh_t* mesh, int nseg, const double* const* snm,
-                                       double* const* pfield, const int* nlev, const int* flags) {
+                                       const double* const* snm_b, double* const* pfield,
+                                       const int* nlev, const int* flags) {
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_REQUIRE(nseg >= 1 && nseg <= MAXSEG, "nseg must be in 1..%d, got %d", MAXSEG, nseg);
   JGCM_REQUIRE(snm && pfield && nlev, "NULL argument");
@@ -1069,12 +1160,15 @@ int jgcm_trans_spherical_to_grid_batch(jgcm_mesh_t* mesh, int nseg, const double
   for (int k = 0; k < nseg; ++k) {
     JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
     JGCM_REQUIRE(nlev[k] >= 1, "nlev[%d] must be >= 1, got %d", k, nlev[k]);
-    JGCM_REQUIRE(!flags || (flags[k] & ~JGCM_BATCH_DIVCOS) == 0, "flags[%d] = %d is not valid", k,
-                 flags[k]);
+    int f = flags ? flags[k] : 0;
+    int op = f >> 1;
+    JGCM_REQUIRE(f >= 0 && op <= SPEC_OP_VCOS, "flags[%d] = %d is not valid", k, f);
+    JGCM_REQUIRE(op < SPEC_OP_UCOS || (snm_b && snm_b[k]),
+                 "array %d: u cos / v cos needs the divergence in snm_b[%d]", k, k);
     total += nlev[k];
   }
   if (int rc = ensure_scratch(mesh, total)) return rc;
-  return inverse_batch(mesh, nseg, snm, pfield, nlev, flags);
+  return inverse_batch(mesh, nseg, snm, snm_b, pfield, nlev, flags);
 }
 
 int jgcm_trans_grid_to_spherical_batch(jgcm_mesh_t* mesh, int nseg, const double* const* pfield,

@@ -193,28 +193,42 @@ def Trans_Grid_To_Spherical(mesh, pfield, snm, batch=False):
 
 BATCH_DIVCOS = 1
 BATCH_VORDIV = 2
+INV_GRAD_LAMBDA = 2
+INV_GRAD_THETA = 4
+INV_UCOS = 6
+INV_VCOS = 8
+
+
+def _ptrs(ts):
+    return (ctypes.c_void_p * len(ts))(*[None if t is None else t.data_ptr() for t in ts])
 
 
 def _batch_args(ts_a, ts_b, flags):
     n = len(ts_a)
     assert 1 <= n <= 8 and len(ts_b) == n, "batched transforms take 1..8 arrays"
-    a = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts_a])
-    b = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts_b])
     nl = (ctypes.c_int * n)(*[t.shape[0] for t in ts_a])
     fl = (ctypes.c_int * n)(*(flags or [0] * n))
-    return n, a, b, nl, fl
+    return n, _ptrs(ts_a), _ptrs(ts_b), nl, fl
 
 
-def Trans_Spherical_To_Grid_Batch(mesh, snms, pfields, flags=None):
+def Trans_Spherical_To_Grid_Batch(mesh, snms, pfields, flags=None, snms_b=None):
     """Trans_Spherical_To_Grid! of several arrays in one Legendre launch and
-    one FFT launch (no reference counterpart).  flags[k] = BATCH_DIVCOS
-    divides array k's rows by cos θ (as UV_Grid_From_Vor_Div! and
-    Compute_Gradients! do after their back-transform)."""
-    for s, g in zip(snms, pfields):
+    one FFT launch (no reference counterpart).  flags[k] may combine
+    BATCH_DIVCOS (divide array k's rows by cos θ) with one spectral operator
+    applied on the way in: INV_GRAD_LAMBDA / INV_GRAD_THETA (cos-weighted
+    gradient, SSM:655-692) or INV_UCOS / INV_VCOS (u cos θ, v cos θ from
+    vor = snms[k] and div = snms_b[k], SSM:517-580) -- DIVCOS with those is
+    Compute_Gradients! and UV_Grid_From_Vor_Div!."""
+    for k, (s, g) in enumerate(zip(snms, pfields)):
         mesh._nlev(s, "spec", True)
         mesh._nlev(g, "grid", True)
         _same(s, g)
-    call("jgcm_trans_spherical_to_grid_batch", mesh.handle, *_batch_args(snms, pfields, flags))
+        if snms_b is not None and snms_b[k] is not None:
+            mesh._nlev(snms_b[k], "spec", True)
+            _same(s, snms_b[k])
+    n, a, b, nl, fl = _batch_args(snms, pfields, flags)
+    sb = _ptrs(snms_b) if snms_b is not None else None
+    call("jgcm_trans_spherical_to_grid_batch", mesh.handle, n, a, sb, b, nl, fl)
 
 
 def Trans_Grid_To_Spherical_Batch(mesh, pfields, snms, flags=None):

@@ -19,8 +19,9 @@ from ._lib import call
 from .dyn_data import Time_Advance
 from .mesh import (_ptr, Trans_Grid_To_Spherical, Trans_Spherical_To_Grid,
                    Trans_Grid_To_Spherical_Batch, Trans_Spherical_To_Grid_Batch, BATCH_DIVCOS,
-                   BATCH_VORDIV, Vor_Div_From_Grid_UV, UV_Grid_From_Vor_Div, Apply_Laplacian)
-from .time_integrator import Compute_Spectral_Damping, Filtered_Leapfrog_step, Get_Δt
+                   BATCH_VORDIV, INV_GRAD_LAMBDA, INV_GRAD_THETA, INV_UCOS, INV_VCOS,
+                   Vor_Div_From_Grid_UV, UV_Grid_From_Vor_Div)
+from .time_integrator import Damped_Leapfrog_Batch, Get_Δt
 
 _VC = {"even_sigma": 0}
 _VD = {"simmons_and_burridge": 0}
@@ -228,14 +229,9 @@ def Spectral_Dynamics_Physics(atmo_data, mesh, dyn_data, Δt, physics_params, se
 
 class _StepScratch:
     """Device temporaries of one HS step that the reference keeps inside its
-    mesh helpers: cos-weighted spectral gradients of ln ps and T (reused for
-    u cos θ, v cos θ at the end of the step) and the grid gradients of T."""
+    mesh helpers: the grid gradients of T used by Add_Horizontal_Advection!."""
 
     def __init__(self, mesh, nd):
-        s = mesh.spectral_zeros(2 * nd + 2)
-        self.spe_dl_lnps, self.spe_dt_lnps = s[0:1], s[1:2]
-        self.spe_dl_t, self.spe_dt_t = s[2:2 + nd], s[2 + nd:2 + 2 * nd]
-        self.spe_ucos, self.spe_vcos = self.spe_dl_t, self.spe_dt_t
         g = mesh.grid_zeros(2 * nd)
         self.grid_dl_t, self.grid_dt_t = g[0:nd], g[nd:2 * nd]
 
@@ -247,30 +243,34 @@ def _scratch(mesh, d):
     return sc
 
 
-def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit):
+def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
+                      pressure_variables=True):
     """Spectral_Dynamics.jl:206-365.  The transforms of the step are batched:
     the gradients of ln ps and T (:259, :293) go back to the grid in one
-    launch pair, the tendencies δlnps, δT, Φ+KE and (δu, δv) -> (δvor, δdiv)
-    (:280, :294, :304, :308) go forward in one, and the new state (:331-335)
-    comes back in one; the grid part of Add_Horizontal_Advection! for T is
-    fused into the column pass.  Results equal the unbatched sequence."""
+    launch pair with the spectral gradient operators applied on the way in,
+    the tendencies δlnps, δT, Φ+KE and (δu, δv) -> (δvor, δdiv) (:280, :294,
+    :304, :308) go forward in one, and the new state (:331-335, u and v through
+    the u cos θ / v cos θ operators) comes back in one; the grid part of
+    Add_Horizontal_Advection! for T is fused into the column pass, ∇² of the
+    energy into the semi-implicit solve, and the seven damping / leapfrog
+    calls (:319-328) into one launch.  Results equal the unbatched sequence.
+    ``pressure_variables=False`` skips the closing Pressure_Variables! (:355)
+    when the caller recomputes the same fields next (Atmosphere_Update!)."""
     d, si = dyn_data, semi_implicit
     integrator = si.integrator
     Δt = Get_Δt(integrator)
     sc = _scratch(mesh, d)
     h = mesh.handle
-    nd = d.spe_vor_c.shape[0]
     if not si._corrections_ready:
         # Compute_Corrections_Init + Pressure_Variables! without forcing (:250-256)
         _physics(si, d, _ZERO_HS, Δt, False)
     si._corrections_ready = False
     # :259 Compute_Gradients!(lnps) and the gradient half of :293 Add_Horizontal_Advection!(T)
-    call("jgcm_compute_gradient_cos", h, _ptr(d.spe_lnps_c), _ptr(sc.spe_dl_lnps),
-         _ptr(sc.spe_dt_lnps), 1)
-    call("jgcm_compute_gradient_cos", h, _ptr(d.spe_t_c), _ptr(sc.spe_dl_t), _ptr(sc.spe_dt_t), nd)
     Trans_Spherical_To_Grid_Batch(
-        mesh, [sc.spe_dl_lnps, sc.spe_dt_lnps, sc.spe_dl_t, sc.spe_dt_t],
-        [d.grid_dλ_ps, d.grid_dθ_ps, sc.grid_dl_t, sc.grid_dt_t], [BATCH_DIVCOS] * 4)
+        mesh, [d.spe_lnps_c, d.spe_lnps_c, d.spe_t_c, d.spe_t_c],
+        [d.grid_dλ_ps, d.grid_dθ_ps, sc.grid_dl_t, sc.grid_dt_t],
+        [BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA,
+         BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA])
     call("jgcm_hs_column_dynamics", si.handle, *map(_ptr, (                    # :260-307
         d.grid_div, d.grid_u_c, d.grid_v_c, d.grid_ps_c, d.grid_Δp, d.grid_lnp_half,
         d.grid_lnp_full, d.grid_p_full, d.grid_dλ_ps, d.grid_dθ_ps, d.grid_t_c, d.grid_vor,
@@ -282,38 +282,38 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit):
         mesh, [d.grid_δlnps, d.grid_δt, d.grid_energy_full, d.grid_δu, d.grid_δv],
         [d.spe_δlnps, d.spe_δt, d.spe_energy, d.spe_δvor, d.spe_δdiv],
         [0, 0, 0, BATCH_VORDIV, 0])
-    Apply_Laplacian(mesh, d.spe_energy)
-    n2 = d.spe_δdiv.numel() * 2
-    call("jgcm_axpby", h, n2, 1.0, _ptr(d.spe_δdiv), -1.0, _ptr(d.spe_energy),
-         _ptr(d.spe_δdiv))                                                       # :310
-    Implicit_Correction_HS(si, vert_coord, atmo_data, d.spe_div_c, d.spe_div_p, d.spe_lnps_c,
-                           d.spe_lnps_p, d.spe_t_c, d.spe_t_p, d.spe_δdiv, d.spe_δlnps, d.spe_δt)
-    Compute_Spectral_Damping(integrator, d.spe_vor_c, d.spe_vor_p, d.spe_δvor)   # :319-321
-    Compute_Spectral_Damping(integrator, d.spe_div_c, d.spe_div_p, d.spe_δdiv)
-    Compute_Spectral_Damping(integrator, d.spe_t_c, d.spe_t_p, d.spe_δt)
-    Filtered_Leapfrog_step(integrator, d.spe_δvor, d.spe_vor_p, d.spe_vor_c, d.spe_vor_n)
-    Filtered_Leapfrog_step(integrator, d.spe_δdiv, d.spe_div_p, d.spe_div_c, d.spe_div_n)
-    Filtered_Leapfrog_step(integrator, d.spe_δlnps, d.spe_lnps_p, d.spe_lnps_c, d.spe_lnps_n)
-    Filtered_Leapfrog_step(integrator, d.spe_δt, d.spe_t_p, d.spe_t_c, d.spe_t_n)
+    # :309-316 ∇²(energy), δdiv -= ∇²(energy), Implicit_Correction!
+    call("jgcm_hs_energy_implicit_correction", si.handle, *map(_ptr, (
+        d.spe_energy, d.spe_div_c, d.spe_div_p, d.spe_lnps_c, d.spe_lnps_p, d.spe_t_c, d.spe_t_p,
+        d.spe_δdiv, d.spe_δlnps, d.spe_δt)))
+    # :319-328 Compute_Spectral_Damping! (vor, div, T) and Filtered_Leapfrog! (vor, div, lnps, T)
+    Damped_Leapfrog_Batch(integrator, [
+        (d.spe_δvor, d.spe_vor_p, d.spe_vor_c, d.spe_vor_n, True),
+        (d.spe_δdiv, d.spe_div_p, d.spe_div_c, d.spe_div_n, True),
+        (d.spe_δlnps, d.spe_lnps_p, d.spe_lnps_c, d.spe_lnps_n, False),
+        (d.spe_δt, d.spe_t_p, d.spe_t_c, d.spe_t_n, True)])
     # :331-335: vor, div, (u, v) = UV_Grid_From_Vor_Div!, T, ln ps back to the grid
-    call("jgcm_compute_ucos_vcos_from_vor_div", h, _ptr(d.spe_vor_n), _ptr(d.spe_div_n),
-         _ptr(sc.spe_ucos), _ptr(sc.spe_vcos), nd)
     Trans_Spherical_To_Grid_Batch(
-        mesh, [d.spe_vor_n, d.spe_div_n, sc.spe_ucos, sc.spe_vcos, d.spe_t_n, d.spe_lnps_n],
+        mesh, [d.spe_vor_n, d.spe_div_n, d.spe_vor_n, d.spe_vor_n, d.spe_t_n, d.spe_lnps_n],
         [d.grid_vor, d.grid_div, d.grid_u_n, d.grid_v_n, d.grid_t_n, d.grid_lnps],
-        [0, 0, BATCH_DIVCOS, BATCH_DIVCOS, 0, 0])
+        [0, 0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS, 0, 0],
+        snms_b=[None, None, d.spe_div_n, d.spe_div_n, None, None])
     call("jgcm_hs_corrections", si.handle, *map(_ptr, (                          # :336-343
         d.grid_lnps, d.grid_ps_n, d.grid_u_n, d.grid_v_n, d.grid_t_n, d.spe_lnps_n, d.spe_t_n)))
     Time_Advance(d)                                                              # :346
-    Pressure_Variables(vert_coord, d.grid_ps_c, d.grid_p_half, d.grid_Δp, d.grid_lnp_half,
-                       d.grid_p_full, d.grid_lnp_full)                            # :355
+    if pressure_variables:
+        Pressure_Variables(vert_coord, d.grid_ps_c, d.grid_p_half, d.grid_Δp, d.grid_lnp_half,
+                           d.grid_p_full, d.grid_lnp_full)                        # :355
 
 
 def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, physics_params):
     """Spectral_Dynamics.jl:470-490."""
     Δt = Get_Δt(semi_implicit.integrator)
     Spectral_Dynamics_Physics(atmo_data, mesh, dyn_data, Δt, physics_params, semi_implicit)
-    Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit)
+    # the closing Pressure_Variables! of Spectral_Dynamics! (:355) is recomputed
+    # identically by Compute_Pressures_And_Heights! below
+    Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
+                      pressure_variables=False)
     d = dyn_data
     Compute_Pressures_And_Heights(atmo_data, vert_coord, d.grid_ps_c, d.grid_geopots, d.grid_t_c,
                                   d.grid_p_half, d.grid_Δp, d.grid_lnp_half, d.grid_p_full,

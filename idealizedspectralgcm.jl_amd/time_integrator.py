@@ -107,3 +107,20 @@ def Damped_Leapfrog(integrator, δQ, Qp, Qc, Qn):
     """Compute_Spectral_Damping! then Filtered_Leapfrog! in one kernel (same results)."""
     n = _spec_n(integrator, δQ, Qp, Qc, Qn)
     call("jgcm_damped_leapfrog", integrator.handle, _ptr(δQ), _ptr(Qp), _ptr(Qc), _ptr(Qn), n)
+
+
+def Damped_Leapfrog_Batch(integrator, fields):
+    """Compute_Spectral_Damping! and/or Filtered_Leapfrog! of up to 8 fields in
+    one launch (no reference counterpart).  ``fields`` holds tuples
+    ``(δQ, Qp, Qc, Qn, damp)``: ``damp`` True applies the damping before the
+    leapfrog (Damped_Leapfrog), False only the leapfrog."""
+    n = len(fields)
+    assert 1 <= n <= 8, "1..8 fields per batch"
+    P = ctypes.c_void_p * n
+    dq, qp, qc, qn = P(), P(), P(), P()
+    nl, mode = (ctypes.c_int * n)(), (ctypes.c_int * n)()
+    for i, (δQ, Qp, Qc, Qn, damp) in enumerate(fields):
+        nl[i] = _spec_n(integrator, δQ, Qp, Qc, Qn)
+        dq[i], qp[i], qc[i], qn[i] = δQ.data_ptr(), Qp.data_ptr(), Qc.data_ptr(), Qn.data_ptr()
+        mode[i] = 3 if damp else 2
+    call("jgcm_damped_leapfrog_batch", integrator.handle, n, dq, qp, qc, qn, nl, mode)

@@ -156,8 +156,16 @@ int jgcm_trans_grid_to_fourier(jgcm_mesh_t* mesh, const double* pfield,
  * levels go through ONE Legendre launch and ONE FFT launch, which is how a
  * time step pushes all its level-fields through the transform at once.
  *   inverse: flags[k] & JGCM_BATCH_DIVCOS divides the output rows of array k
- *            by cos θ (the second half of UV_Grid_From_Vor_Div! and
- *            Compute_Gradients!, Spectral_Spherical_Mesh.jl:396-420, 462-488).
+ *            by cos θ; flags[k] may add ONE spectral operator applied to the
+ *            input on the fly (no spectral temporary is written):
+ *              JGCM_INV_GRAD_LAMBDA  cos θ ∂λ h / a  (Compute_Gradient_Cos!,
+ *              JGCM_INV_GRAD_THETA   cos θ ∂θ h / a   SSM:655-692)
+ *              JGCM_INV_UCOS         u cos θ from vor = snm[k], div = snm_b[k]
+ *              JGCM_INV_VCOS         v cos θ  (Compute_Ucos_Vcos_From_Vor_Div!,
+ *                                             SSM:517-580)
+ *            so DIVCOS|GRAD_* is Compute_Gradients! (SSM:721-741) and
+ *            DIVCOS|UCOS, DIVCOS|VCOS is UV_Grid_From_Vor_Div! (SSM:462-482).
+ *            snm_b may be NULL when no array uses UCOS / VCOS.
  *   forward: flags[k] == JGCM_BATCH_VORDIV pairs array k (grid u) with array
  *            k+1 (grid v, flags[k+1] = 0, same nlev) and writes vor to snm[k],
  *            div to snm[k+1] (Vor_Div_From_Grid_UV!); other arrays get the
@@ -166,8 +174,13 @@ int jgcm_trans_grid_to_fourier(jgcm_mesh_t* mesh, const double* pfield,
  * flags may be NULL (all plain).  Results equal the one-array calls.       */
 #define JGCM_BATCH_DIVCOS 1
 #define JGCM_BATCH_VORDIV 2
+#define JGCM_INV_GRAD_LAMBDA 2
+#define JGCM_INV_GRAD_THETA 4
+#define JGCM_INV_UCOS 6
+#define JGCM_INV_VCOS 8
 int jgcm_trans_spherical_to_grid_batch(jgcm_mesh_t* mesh, int nseg, const double* const* snm,
-                                       double* const* pfield, const int* nlev, const int* flags);
+                                       const double* const* snm_b, double* const* pfield,
+                                       const int* nlev, const int* flags);
 int jgcm_trans_grid_to_spherical_batch(jgcm_mesh_t* mesh, int nseg, const double* const* pfield,
                                        double* const* snm, const int* nlev, const int* flags);
 
@@ -242,6 +255,14 @@ int jgcm_filtered_leapfrog(jgcm_leapfrog_t* lf, const double* dQ, const double* 
 /* Compute_Spectral_Damping! followed by Filtered_Leapfrog! in one pass (same results). */
 int jgcm_damped_leapfrog(jgcm_leapfrog_t* lf, double* dQ, const double* Qp, double* Qc,
                          double* Qn, int nlev);
+/* Compute_Spectral_Damping! / Filtered_Leapfrog! for nfields (<= 8) prognostic
+ * fields in one launch (no reference counterpart; Spectral_Dynamics.jl:319-328
+ * makes seven calls).  mode[f]: 1 damping of dQ only, 2 leapfrog only, 3 both
+ * (damping, then the leapfrog with the damped tendency).  Qn[f] may be NULL
+ * for mode 1.                                                              */
+int jgcm_damped_leapfrog_batch(jgcm_leapfrog_t* lf, int nfields, double* const* dQ,
+                               const double* const* Qp, double* const* Qc, double* const* Qn,
+                               const int* nlev, const int* mode);
 /* Implicit_Correction!(integrator, div_c, div_p, h_c, h_p, h_0, δdiv, δh)
  *                                         Shallow_Water_Dynamics.jl:25-54 */
 int jgcm_sw_implicit_correction(jgcm_leapfrog_t* lf, const double* div_c, const double* div_p,
@@ -337,6 +358,14 @@ int jgcm_semi_implicit_get(jgcm_semi_implicit_t* si, int which, double* host_out
 int jgcm_implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, const double* div_p,
                              const double* lnps_c, const double* lnps_p, const double* t_c,
                              const double* t_p, double* ddiv, double* dlnps, double* dt);
+/* Apply_Laplacian!(spe_energy); spe_δdiv .-= spe_energy; Implicit_Correction!
+ * (Spectral_Dynamics.jl:309-316) in one pass: spe_energy is replaced by its
+ * Laplacian, as in the reference.                                          */
+int jgcm_hs_energy_implicit_correction(jgcm_semi_implicit_t* si, double* spe_energy,
+                                       const double* div_c, const double* div_p,
+                                       const double* lnps_c, const double* lnps_p,
+                                       const double* t_c, const double* t_p, double* ddiv,
+                                       double* dlnps, double* dt);
 /* Step part 1, one column pass: Pressure_Variables!(ps_c) (Spectral_Dynamics.jl:256),
  * Spectral_Dynamics_Physics! / HS_Forcing! on the p-fields (:452-467,
  * HS_Forcing.jl:1-130; skipped when do_forcing = 0, δps = 0 always) and

@@ -99,7 +99,60 @@ def test_batch_argument_errors(jgcm):
         jgcm.Trans_Grid_To_Spherical_Batch(dev, g[:2], s[:2], [0, V])
     with pytest.raises(AssertionError):   # 2 + 1 + 2 Legendre jobs > 4
         jgcm.Trans_Grid_To_Spherical_Batch(dev, g, s, [V, 0, 0, V, 0])
-    with pytest.raises(AssertionError):   # unknown inverse flag
-        jgcm.Trans_Spherical_To_Grid_Batch(dev, s[:1], g[:1], [2])
+    with pytest.raises(AssertionError):   # unknown inverse operator
+        jgcm.Trans_Spherical_To_Grid_Batch(dev, s[:1], g[:1], [16])
+    with pytest.raises(AssertionError):   # u cos θ without the divergence
+        jgcm.Trans_Spherical_To_Grid_Batch(dev, s[:1], g[:1], [jgcm.INV_UCOS])
     with pytest.raises(AssertionError):   # more than 8 arrays
         jgcm.Trans_Spherical_To_Grid_Batch(dev, s * 2, g * 2)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "split32"])
+def test_inverse_operators_equal_unfused(jgcm, prec):
+    """Gradient and u cos / v cos operators applied while staging equal the
+    stand-alone operator kernels followed by the plain transform and ÷cos θ."""
+    N, nlat = 42, 64
+    dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 3, RADIUS, precision=prec)
+    rng = np.random.default_rng(3)
+    h = cuda(random_spec(N, N + 1, 3, rng))
+    vor = cuda(random_spec(N, N + 1, 3, rng))
+    div = cuda(random_spec(N, N + 1, 3, rng))
+    D = jgcm.BATCH_DIVCOS
+    outs = [dev.grid_zeros(3) for _ in range(4)]
+    jgcm.Trans_Spherical_To_Grid_Batch(
+        dev, [h, h, vor, vor], outs,
+        [D | jgcm.INV_GRAD_LAMBDA, D | jgcm.INV_GRAD_THETA, D | jgcm.INV_UCOS, D | jgcm.INV_VCOS],
+        snms_b=[None, None, div, div])
+    sl, st = dev.spectral_zeros(3), dev.spectral_zeros(3)
+    jgcm.Compute_Gradient_Cos(dev, h, sl, st)
+    su, sv = dev.spectral_zeros(3), dev.spectral_zeros(3)
+    jgcm.Compute_Ucos_Vcos_From_Vor_Div(dev, vor, div, su, sv)
+    for s, o in zip((sl, st, su, sv), outs):
+        g = dev.grid_zeros(3)
+        jgcm.Trans_Spherical_To_Grid(dev, s, g, batch=True)
+        jgcm.Divide_By_Cos(dev, g)
+        torch.cuda.synchronize()
+        assert torch.equal(o, g)
+
+
+def test_damped_leapfrog_batch_equals_single_calls(jgcm):
+    N, nlat, nd = 42, 64, 4
+    dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, nd, RADIUS)
+    rng = np.random.default_rng(9)
+    for init in (True, False):
+        I = jgcm.Filtered_Leapfrog(0.04, 4, 1e-4, dev.laplacian_eig, 0.5, 600, True, 0, 86400)
+        if not init:
+            jgcm.Update_Init_Step(I)
+        state = [[cuda(random_spec(N, N + 1, n, rng)) for _ in range(4)] for n in (nd, nd, 1)]
+        a = [[t.clone() for t in f] for f in state]
+        b = [[t.clone() for t in f] for f in state]
+        damp = [True, False, True]
+        jgcm.Damped_Leapfrog_Batch(I, [tuple(f) + (dm,) for f, dm in zip(a, damp)])
+        for f, dm in zip(b, damp):
+            if dm:
+                jgcm.Compute_Spectral_Damping(I, f[2], f[1], f[0])
+            jgcm.Filtered_Leapfrog_step(I, *f)
+        torch.cuda.synchronize()
+        for fa, fb in zip(a, b):
+            for x, y in zip(fa, fb):
+                assert torch.equal(x, y)
