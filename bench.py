@@ -138,6 +138,20 @@ def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):
             "mean_T": float(t.mean())}
 
 
+def traffic_from_profile(direction, precision):
+    """HBM bytes of one transform launch (both kernels) as measured by the
+    committed PMC summary (FETCH_SIZE and WRITE_SIZE, calibrated against a
+    copy kernel of known size), or None if the summary is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "sht_t85_61fields_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return float(t["launch_bytes"][direction][precision])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -253,8 +267,10 @@ def main():
                    "path": use},
         "precisions": results,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": f"{dom} transform ({use})"},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": traffic_from_profile(dom, use),
+                     "kernel": f"{dom} transform ({use}): Legendre + FFT kernels, "
+                               f"{NFIELDS * BYTES_PER_FIELD} algorithmic bytes"},
     }
     if not a.no_hs:
         hs = {}
