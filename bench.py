@@ -138,6 +138,55 @@ def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):
             "mean_T": float(t.mean())}
 
 
+def ensemble_bench(jgcm, torch, steps=100):
+    """Mixed shallow-water ensemble: 64 x T21 (64x32), 128 x T42 (128x64) and
+    64 x T85 (256x128) instances, seeded n^-3/2 vorticity spectra, FP32
+    Legendre with FP64 state, one step of all 256 captured into one HIP graph.
+    Reports instance-steps/s and the p50/p99 latency of one ensemble step,
+    beside the cost of looping the plain FP64 path one instance at a time."""
+    from jgcm_amd import ensemble as E
+    groups = [(21, 32, 64, 1800), (42, 64, 128, 1200), (85, 128, 64, 600)]
+    ens = E.SW_Ensemble(groups, seed=11, precision="fp32")
+    ens.step()
+    ens.capture()
+    for _ in range(10):
+        ens.launch()
+    ens.owner.synchronize()
+    stream = torch.cuda.ExternalStream(ens.owner.stream())
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    evs[0].record(stream)
+    for i in range(steps):
+        ens.launch()
+        evs[i + 1].record(stream)
+    ens.owner.synchronize()
+    lat = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(steps)])
+    total = evs[0].elapsed_time(evs[-1]) / steps
+    finite = all(bool(torch.isfinite(g.d.spe_vor_c).all()) for g in ens.groups)
+    ens.graph.close()
+    # the plain FP64 path, one instance per model, eager launches
+    loop_ms = 0.0
+    for N, nθ, count, dt in groups:
+        one = E.SW_Group(N, nθ, 1, dt, seed=5, precision="fp64")
+        one.step()
+        for _ in range(5):
+            one.step()
+        one.mesh.synchronize()
+        st = torch.cuda.ExternalStream(one.mesh.stream())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(20):
+            one.step()
+        e1.record(st)
+        one.mesh.synchronize()
+        loop_ms += count * e0.elapsed_time(e1) / 20
+    n = ens.instances
+    return {"instances": n, "groups": "64xT21, 128xT42, 64xT85", "precision": "fp32 Legendre, fp64 state",
+            "ms_per_ensemble_step": total, "instance_steps_per_s": n / (total * 1e-3),
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "loop_fp64_ms_per_step": loop_ms, "speedup_vs_fp64_loop": loop_ms / total,
+            "finite": finite}
+
+
 def traffic_from_profile(direction, precision):
     """HBM bytes of one transform launch (both kernels) as measured by the
     committed PMC summary (FETCH_SIZE and WRITE_SIZE, calibrated against a
@@ -279,6 +328,7 @@ def main():
         hs["t42_l20_fp64"] = hs_bench(jgcm, torch, 42, 64, 20, 600, a.hs_steps, 5, "fp64")
         hs["split32_speedup_vs_fp64"] = hs["fp64"]["ms_per_step"] / hs["split32"]["ms_per_step"]
         out["held_suarez"] = hs
+        out["ensemble"] = ensemble_bench(jgcm, torch)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         if not a.no_hs:
