@@ -7,8 +7,8 @@ state (vor 20, div 20, T 20, ln ps 1).  Inputs are synthetic, seeded,
 band-limited fields resident in HBM before timing starts.  Throughput counts
 each direction of each level-field as one transform.
 
-The Legendre stage is measured in FP64 and in split-FP32; `value` is the
-split-FP32 path when its measured error against the FP64 path is <= 1e-6,
+The Legendre stage is measured in FP64, split-FP32 and BF16x3; `value` is
+the split-FP32 path when its measured error against the FP64 path is <= 1e-6,
 otherwise the FP64 path.  The CPU baseline is the FP64 oracle restatement
 (numpy, single thread) on a bounded sample of the same workload.
 """
@@ -249,7 +249,7 @@ def main():
 
     results = {}
     ref_grid = ref_spec = None
-    for prec in ("fp64", "split32"):
+    for prec in ("fp64", "split32", "bf16x3"):
         mesh.set_precision(prec)
         for _ in range(a.warmup):
             inv(spec_in, grid)
@@ -323,7 +323,7 @@ def main():
     }
     if not a.no_hs:
         hs = {}
-        for prec in ("fp64", "split32"):
+        for prec in ("fp64", "split32", "bf16x3"):
             hs[prec] = hs_bench(jgcm, torch, 85, 128, 20, 300, a.hs_steps, 5, prec)
         hs["t42_l20_fp64"] = hs_bench(jgcm, torch, 42, 64, 20, 600, a.hs_steps, 5, "fp64")
         hs["split32_speedup_vs_fp64"] = hs["fp64"]["ms_per_step"] / hs["split32"]["ms_per_step"]

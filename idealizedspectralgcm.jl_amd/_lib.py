@@ -16,7 +16,7 @@ JGCM_ERR_STATE = -3
 JGCM_ERR_NOMEM = -4
 JGCM_ERR_UNSUPPORTED = -5
 
-PREC = {"fp64": 0, "fp32": 1, "split32": 2}
+PREC = {"fp64": 0, "fp32": 1, "split32": 2, "bf16x3": 3}
 
 TABLES = {
     "sinθ": 0, "cosθ": 1, "wts": 2, "λc": 3, "θc": 4, "λe": 5, "θe": 6,

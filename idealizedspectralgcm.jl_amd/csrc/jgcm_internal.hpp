@@ -57,7 +57,35 @@ struct RowTables {
   float* pfwd_hi = nullptr; float* pfwd_lo = nullptr;
   float* alpa_hi = nullptr; float* alpa_lo = nullptr;
   float* alpb_hi = nullptr; float* alpb_lo = nullptr;
+  // BF16 three-piece split copies: x = b1 + b2 + b3 (x, y, z of a ushort4; w = 0)
+  ushort4* pinv_b3 = nullptr;
+  ushort4* pfwd_b3 = nullptr;
+  ushort4* alpa_b3 = nullptr;
+  ushort4* alpb_b3 = nullptr;
 };
+
+// BF16 (round to nearest even) from FP32 and back; finite values only.
+__host__ __device__ inline unsigned short f32_to_bf16(float x) {
+  unsigned u;
+  __builtin_memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(unsigned short b) {
+  unsigned u = (unsigned)b << 16;
+  float x;
+  __builtin_memcpy(&x, &u, 4);
+  return x;
+}
+// x = b1 + b2 + b3 + O(2^-24 |x|): each piece the BF16 rounding of the rest.
+__host__ __device__ inline ushort4 split_bf16x3(double v) {
+  unsigned short b1 = f32_to_bf16((float)v);
+  double r = v - (double)bf16_to_f32(b1);
+  unsigned short b2 = f32_to_bf16((float)r);
+  r = r - (double)bf16_to_f32(b2);
+  unsigned short b3 = f32_to_bf16((float)r);
+  return make_ushort4(b1, b2, b3, 0);
+}
 
 }  // namespace jgcm
 

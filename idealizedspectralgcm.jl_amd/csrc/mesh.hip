@@ -132,11 +132,13 @@ __global__ void pack_rows_kernel(int N, int ns, int nlat, double radius,
   double* d64[4] = {rt.pinv, rt.pfwd, rt.alpa, rt.alpb};
   float* hi[4] = {rt.pinv_hi, rt.pfwd_hi, rt.alpa_hi, rt.alpb_hi};
   float* lo[4] = {rt.pinv_lo, rt.pfwd_lo, rt.alpa_lo, rt.alpb_lo};
+  ushort4* b3[4] = {rt.pinv_b3, rt.pfwd_b3, rt.alpa_b3, rt.alpb_b3};
   for (int t = 0; t < 4; ++t) {
     d64[t][o] = vals[t];
     float h = (float)vals[t];
     hi[t][o] = h;
     lo[t][o] = (float)(vals[t] - (double)h);
+    b3[t][o] = split_bf16x3(vals[t]);
   }
 }
 
@@ -173,6 +175,7 @@ static void free_mesh(jgcm_mesh* m) {
                   m->rt.pinv, m->rt.pfwd, m->rt.alpa, m->rt.alpb,
                   m->rt.pinv_hi, m->rt.pinv_lo, m->rt.pfwd_hi, m->rt.pfwd_lo,
                   m->rt.alpa_hi, m->rt.alpa_lo, m->rt.alpb_hi, m->rt.alpb_lo,
+                  m->rt.pinv_b3, m->rt.pfwd_b3, m->rt.alpa_b3, m->rt.alpb_b3,
                   m->d_twiddle, m->d_eig, m->d_uvm, m->d_uvc, m->d_uvp, m->d_dlam, m->d_dthm,
                   m->d_dthp, m->d_four_a, m->d_four_b, m->d_spec_a, m->d_spec_b, m->d_grid_a,
                   m->d_grid_b, m->d_scalar};
@@ -267,6 +270,8 @@ static int create_impl(int N, int ns, int nlat, int nlon, int nd, double radius,
                     &m->rt.alpa_hi, &m->rt.alpa_lo, &m->rt.alpb_hi, &m->rt.alpb_lo};
   for (auto p : d64) JGCM_CHECK_HIP(hipMalloc(p, tab * sizeof(double)));
   for (auto p : f32) JGCM_CHECK_HIP(hipMalloc(p, tab * sizeof(float)));
+  ushort4** b16[4] = {&m->rt.pinv_b3, &m->rt.pfwd_b3, &m->rt.alpa_b3, &m->rt.alpb_b3};
+  for (auto p : b16) JGCM_CHECK_HIP(hipMalloc(p, tab * sizeof(ushort4)));
   hipLaunchKernelGGL(pack_rows_kernel, dim3((half + 63) / 64, ns + 1, N + 1), dim3(64), 0, st, N,
                      ns, nlat, radius, m->d_row_start, m->d_qnm_full, m->d_dqnm_full, m->d_wts,
                      m->d_cos, m->rt);
@@ -428,7 +433,7 @@ int jgcm_mesh_share_stream(jgcm_mesh_t* mesh, jgcm_mesh_t* owner) {
 int jgcm_mesh_set_precision(jgcm_mesh_t* mesh, int precision) {
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_REQUIRE(precision == JGCM_PREC_FP64 || precision == JGCM_PREC_FP32 ||
-                   precision == JGCM_PREC_SPLIT32,
+                   precision == JGCM_PREC_SPLIT32 || precision == JGCM_PREC_BF16X3,
                "unknown precision %d", precision);
   mesh->precision = precision;
   return JGCM_OK;

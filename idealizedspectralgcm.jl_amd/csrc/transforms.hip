@@ -32,7 +32,7 @@
 
 namespace jgcm {
 
-enum { P_FP64 = 0, P_FP32 = 1, P_SPLIT = 2 };
+enum { P_FP64 = 0, P_FP32 = 1, P_SPLIT = 2, P_BF16 = 3 };
 
 // A launch may gather its level-fields from up to MAXSEG separate arrays
 // (e.g. vor, div, u cos, v cos, T and ln ps back-transformed together):
@@ -347,15 +347,18 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(c
 
 template <int PREC> struct LdsT { using T = double; };
 template <> struct LdsT<P_FP32> { using T = float; };
-template <> struct LdsT<P_SPLIT> { using T = float2; };  // (hi, lo)
+template <> struct LdsT<P_SPLIT> { using T = float2; };   // (hi, lo)
+template <> struct LdsT<P_BF16> { using T = ushort4; };   // (b1, b2, b3, 0)
 
 template <int PREC>
 __device__ __forceinline__ typename LdsT<PREC>::T to_lds(double v) {
   if constexpr (PREC == P_FP64) return v;
   else if constexpr (PREC == P_FP32) return (float)v;
-  else {
+  else if constexpr (PREC == P_SPLIT) {
     float h = (float)v;
     return make_float2(h, (float)(v - (double)h));
+  } else {
+    return split_bf16x3(v);
   }
 }
 
@@ -363,6 +366,7 @@ struct FwdTables {
   const double* d;      // FP64 rows
   const float* hi;      // FP32 / split hi
   const float* lo;      // split lo
+  const ushort4* b3;    // BF16 three-piece split
 };
 
 struct FwdJob {
@@ -416,12 +420,17 @@ __device__ __forceinline__ fv4 mma32(float a, float b, fv4 c) {
 template <int PREC> struct TabT { using T = double; };
 template <> struct TabT<P_FP32> { using T = float; };
 template <> struct TabT<P_SPLIT> { using T = float2; };
+template <> struct TabT<P_BF16> { using T = ushort4; };
+
+// contraction steps per MFMA: 4 (f64 / f32 16x16x4), 16 (bf16 16x16x16)
+template <int PREC> constexpr int kstep() { return PREC == P_BF16 ? 16 : 4; }
 
 template <int PREC>
 __device__ __forceinline__ typename TabT<PREC>::T tab_load(const FwdTables& t, size_t i, bool ok) {
   if constexpr (PREC == P_FP64) return ok ? t.d[i] : 0.0;
   else if constexpr (PREC == P_FP32) return ok ? t.hi[i] : 0.0f;
-  else return ok ? make_float2(t.hi[i], t.lo[i]) : make_float2(0.0f, 0.0f);
+  else if constexpr (PREC == P_SPLIT) return ok ? make_float2(t.hi[i], t.lo[i]) : make_float2(0.0f, 0.0f);
+  else return ok ? t.b3[i] : make_ushort4(0, 0, 0, 0);
 }
 
 // row of the 16 x 16 result tile held in register r of a lane with lane/16 = lk
@@ -436,8 +445,15 @@ struct TileAcc {
   double v[4][4];
 };
 
-// acc[t] += sum over steps ks < KS of A_t(ks) B(ks); bval(ks) and aval(t, ks)
-// give this lane's operand values (from LDS).
+typedef short sv4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ fv4 mma16(sv4 a, sv4 b, fv4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// acc[t] += sum over MFMA steps ks < KS of A_t(ks) B(ks); bval(ks, e) and
+// aval(t, ks, e) give this lane's operand values (from LDS): e = 0 for the
+// 16x16x4 instructions, e = 0..3 (four consecutive contraction steps of the
+// lane) for the bf16 16x16x16 instruction.
 template <int PREC, typename BF, typename AF>
 __device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS, BF bval, AF aval) {
   if constexpr (PREC == P_FP64) {
@@ -445,10 +461,10 @@ __device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS,
 #pragma unroll
     for (int t = 0; t < 4; ++t) c[t] = dv4{0.0, 0.0, 0.0, 0.0};
     for (int ks = 0; ks < KS; ++ks) {
-      double b = bval(ks);
+      double b = bval(ks, 0);
 #pragma unroll
       for (int t = 0; t < 4; ++t)
-        if (t < ntile) c[t] = mma64(aval(t, ks), b, c[t]);
+        if (t < ntile) c[t] = mma64(aval(t, ks, 0), b, c[t]);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -459,16 +475,16 @@ __device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS,
 #pragma unroll
     for (int t = 0; t < 4; ++t) c[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int ks = 0; ks < KS; ++ks) {
-      float b = bval(ks);
+      float b = bval(ks, 0);
 #pragma unroll
       for (int t = 0; t < 4; ++t)
-        if (t < ntile) c[t] = mma32(aval(t, ks), b, c[t]);
+        if (t < ntile) c[t] = mma32(aval(t, ks, 0), b, c[t]);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc.v[t][r] += (double)c[t][r];
-  } else {
+  } else if constexpr (PREC == P_SPLIT) {
     for (int kb = 0; kb < KS; kb += 4) {
       fv4 hh[4], cr[4];
 #pragma unroll
@@ -478,11 +494,11 @@ __device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS,
       }
       const int ke = min(KS, kb + 4);
       for (int ks = kb; ks < ke; ++ks) {
-        float2 b = bval(ks);
+        float2 b = bval(ks, 0);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           if (t < ntile) {
-            float2 a = aval(t, ks);
+            float2 a = aval(t, ks, 0);
             hh[t] = mma32(a.x, b.x, hh[t]);
             cr[t] = mma32(a.x, b.y, cr[t]);
             cr[t] = mma32(a.y, b.x, cr[t]);
@@ -493,6 +509,49 @@ __device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS,
       for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc.v[t][r] += (double)hh[t][r] + (double)cr[t][r];
+    }
+  } else {
+    // BF16 x 3: x = x1 + x2 + x3, P = p1 + p2 + p3; the six products with
+    // piece orders summing to <= 2 (the rest are below 2^-24 |P x|), smallest
+    // first, in FP32 over blocks of 64 contraction steps; block sums in FP64.
+    for (int kb = 0; kb < KS; kb += 4) {
+      fv4 c[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) c[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
+      const int ke = min(KS, kb + 4);
+      for (int ks = kb; ks < ke; ++ks) {
+        sv4 b1, b2, b3;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          ushort4 x = bval(ks, e);
+          b1[e] = (short)x.x;
+          b2[e] = (short)x.y;
+          b3[e] = (short)x.z;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          if (t < ntile) {
+            sv4 a1, a2, a3;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              ushort4 p = aval(t, ks, e);
+              a1[e] = (short)p.x;
+              a2[e] = (short)p.y;
+              a3[e] = (short)p.z;
+            }
+            c[t] = mma16(a1, b3, c[t]);
+            c[t] = mma16(a2, b2, c[t]);
+            c[t] = mma16(a3, b1, c[t]);
+            c[t] = mma16(a1, b2, c[t]);
+            c[t] = mma16(a2, b1, c[t]);
+            c[t] = mma16(a1, b1, c[t]);
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc.v[t][r] += (double)c[t][r];
     }
   }
 }
@@ -597,8 +656,10 @@ __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int 
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc.v[t][r] = 0.0;
-  const LT* Xc = X + (16 * wave + li) * KS1 + lk;
-  const TT* Pc = Pt + li * KS1 + lk;
+  constexpr int KSTEP = kstep<PREC>();
+  const int lko = KSTEP == 16 ? 4 * lk : lk;  // this lane's first contraction step
+  const LT* Xc = X + (16 * wave + li) * KS1 + lko;
+  const TT* Pc = Pt + li * KS1 + lko;
   for (int term = 0; term < 2; ++term) {
     const double2* src = term == 0 ? job.srcA : job.srcB;
     if (!src) break;
@@ -610,8 +671,8 @@ __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int 
     const size_t fstride = (size_t)nm * 2 * half;
     for (int lat0 = 0; lat0 < half; lat0 += KC) {
       const int kcl = min(KC, half - lat0);
-      const int KSc = (kcl + 3) / 4;
-      const int kpad = 4 * KSc;
+      const int KSc = (kcl + KSTEP - 1) / KSTEP;
+      const int kpad = KSTEP * KSc;
       __syncthreads();  // the previous chunk's operands are consumed
       staged_copy2(
           (LCOLS / 2) * kpad,
@@ -637,8 +698,8 @@ __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int 
           [&](int idx, TT v) { Pt[(idx / kpad) * KS1 + idx % kpad] = v; });
       __syncthreads();
       mfma_accumulate<PREC>(
-          acc, ntile, KSc, [&](int ks) { return Xc[4 * ks]; },
-          [&](int t, int ks) { return Pc[16 * t * KS1 + 4 * ks]; });
+          acc, ntile, KSc, [&](int ks, int e) { return Xc[KSTEP * ks + e]; },
+          [&](int t, int ks, int e) { return Pc[16 * t * KS1 + KSTEP * ks + e]; });
     }
   }
   const int col = 16 * wave + li;
@@ -755,13 +816,15 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc.v[t][r] = 0.0;
-  const LT* Sc = S + (16 * wave + li) * KS1 + lk;
+  constexpr int KSTEP = kstep<PREC>();
+  const int lko = KSTEP == 16 ? 4 * lk : lk;  // this lane's first contraction step
+  const LT* Sc = S + (16 * wave + li) * KS1 + lko;
   constexpr int PSI = TROWS + 1;
-  const TT* Pc = Pt + lk * PSI + li;
+  const TT* Pc = Pt + lko * PSI + li;
   for (int kk0 = 0; kk0 < nr; kk0 += KC) {
     const int kcr = min(KC, nr - kk0);
-    const int KSc = (kcr + 3) / 4;
-    const int kpad = 4 * KSc;
+    const int KSc = (kcr + KSTEP - 1) / KSTEP;
+    const int kpad = KSTEP * KSc;
     __syncthreads();
     staged_copy2(
         (LCOLS / 2) * kpad,
@@ -784,8 +847,8 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
         [&](int idx, TT v) { Pt[(idx / TROWS) * PSI + idx % TROWS] = v; });
     __syncthreads();
     mfma_accumulate<PREC>(
-        acc, nlt, KSc, [&](int ks) { return Sc[4 * ks]; },
-        [&](int t, int ks) { return Pc[4 * ks * PSI + 16 * t]; });
+        acc, nlt, KSc, [&](int ks, int e) { return Sc[KSTEP * ks + e]; },
+        [&](int t, int ks, int e) { return Pc[(KSTEP * ks + e) * PSI + 16 * t]; });
   }
   const int col = 16 * wave + li;
   const int fl = col >> 1, comp = col & 1;
@@ -871,10 +934,10 @@ int ifft_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const doubl
 static FwdTables tables_for(const jgcm_mesh* M, int which) {
   const RowTables& r = M->rt;
   switch (which) {
-    case 0: return {r.pfwd, r.pfwd_hi, r.pfwd_lo};
-    case 1: return {r.alpa, r.alpa_hi, r.alpa_lo};
-    case 2: return {r.alpb, r.alpb_hi, r.alpb_lo};
-    default: return {r.pinv, r.pinv_hi, r.pinv_lo};
+    case 0: return {r.pfwd, r.pfwd_hi, r.pfwd_lo, r.pfwd_b3};
+    case 1: return {r.alpa, r.alpa_hi, r.alpa_lo, r.alpa_b3};
+    case 2: return {r.alpb, r.alpb_hi, r.alpb_lo, r.alpb_b3};
+    default: return {r.pinv, r.pinv_hi, r.pinv_lo, r.pinv_b3};
   }
 }
 
@@ -906,6 +969,10 @@ static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int 
       hipLaunchKernelGGL(legendre_fwd_mfma<P_FP32>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
                          M->nm, M->half, M->d_row_start, nmax, zero_top);
       break;
+    case JGCM_PREC_BF16X3:
+      hipLaunchKernelGGL(legendre_fwd_mfma<P_BF16>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
+                         M->nm, M->half, M->d_row_start, nmax, zero_top);
+      break;
     default:
       hipLaunchKernelGGL(legendre_fwd_mfma<P_SPLIT>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
                          M->nm, M->half, M->d_row_start, nmax, zero_top);
@@ -930,6 +997,10 @@ static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, 
       break;
     case JGCM_PREC_FP32:
       hipLaunchKernelGGL(legendre_inv_mfma<P_FP32>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
+                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops);
+      break;
+    case JGCM_PREC_BF16X3:
+      hipLaunchKernelGGL(legendre_inv_mfma<P_BF16>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
                          nmax, M->ns, M->nm, M->half, M->d_row_start, ops);
       break;
     default:

@@ -50,6 +50,8 @@ extern "C" {
 #define JGCM_PREC_FP64 0      /* FP64 tables, FP64 accumulation                 */
 #define JGCM_PREC_FP32 1      /* FP32 tables and operands, FP32 accumulation    */
 #define JGCM_PREC_SPLIT32 2   /* hi/lo FP32 split, 3 products, compensated sums */
+#define JGCM_PREC_BF16X3 3    /* 3-piece BF16 split, 6 products on BF16 MFMA,
+                                 FP32 block sums, FP64 accumulation            */
 
 typedef struct jgcm_mesh jgcm_mesh_t;
 typedef struct jgcm_leapfrog jgcm_leapfrog_t;

@@ -30,7 +30,8 @@ def cuda(x):
     return torch.tensor(np.ascontiguousarray(x), device="cuda")
 
 
-@pytest.mark.parametrize("N,nlat,prec", [(42, 64, "fp64"), (85, 128, "fp64"), (85, 128, "split32")])
+@pytest.mark.parametrize("N,nlat,prec", [(42, 64, "fp64"), (85, 128, "fp64"), (85, 128, "split32"),
+                                         (85, 128, "bf16x3")])
 def test_inverse_batch_equals_single_calls(jgcm, N, nlat, prec):
     dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 5, RADIUS, precision=prec)
     rng = np.random.default_rng(N)
@@ -59,7 +60,8 @@ def test_inverse_batch_equals_single_calls(jgcm, N, nlat, prec):
             assert rel(o.cpu().numpy(), g) < 1e-12
 
 
-@pytest.mark.parametrize("N,nlat,prec", [(42, 64, "fp64"), (85, 128, "fp64"), (85, 128, "split32")])
+@pytest.mark.parametrize("N,nlat,prec", [(42, 64, "fp64"), (85, 128, "fp64"), (85, 128, "split32"),
+                                         (85, 128, "bf16x3")])
 def test_forward_batch_equals_single_calls(jgcm, N, nlat, prec):
     dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 5, RADIUS, precision=prec)
     rng = np.random.default_rng(7 + N)
@@ -107,7 +109,7 @@ def test_batch_argument_errors(jgcm):
         jgcm.Trans_Spherical_To_Grid_Batch(dev, s * 2, g * 2)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "split32"])
+@pytest.mark.parametrize("prec", ["fp64", "split32", "bf16x3"])
 def test_inverse_operators_equal_unfused(jgcm, prec):
     """Gradient and u cos / v cos operators applied while staging equal the
     stand-alone operator kernels followed by the plain transform and ÷cos θ."""

@@ -87,7 +87,7 @@ def test_batched_61_fields_fp64(jgcm, N, nlat):
     assert rel(sd.cpu().numpy(), sp) < 1e-11
 
 
-@pytest.mark.parametrize("precision", ["fp32", "split32"])
+@pytest.mark.parametrize("precision", ["fp32", "split32", "bf16x3"])
 @pytest.mark.parametrize("N,nlat", TRUNCS)
 def test_reduced_precision_legendre(jgcm, N, nlat, precision):
     B = 61
@@ -105,7 +105,7 @@ def test_reduced_precision_legendre(jgcm, N, nlat, precision):
         assert e_inv < 1e-6 and e_fwd < 1e-6, (e_inv, e_fwd)
         # a reduced-precision path really ran (FP64 would sit near 1e-15)
         assert e_inv > 1e-11 and e_fwd > 1e-11, (e_inv, e_fwd)
-        if precision == "split32":
+        if precision in ("split32", "bf16x3"):
             assert e_inv < 2e-7 and e_fwd < 2e-7, (e_inv, e_fwd)
     finally:
         dev.set_precision("fp64")
