@@ -61,6 +61,12 @@ SIGNATURES = {
     "jgcm_trans_grid_to_fourier": [P, P, P, I],
     "jgcm_trans_grid_to_spherical_adjoint": [P, P, P, I],
     "jgcm_trans_spherical_to_grid_adjoint": [P, P, P, I],
+    "jgcm_vor_div_from_grid_uv_adjoint": [P, P, P, P, P, I],
+    "jgcm_ucos_vcos_adjoint": [P, P, P, P, P, I],
+    "jgcm_uv_grid_from_vor_div_adjoint": [P, P, P, P, P, I],
+    "jgcm_vorticity_flux_adjoint": [P, P, P, P, D, P, P, P, P, P, I],
+    "jgcm_damped_leapfrog_adjoint": [P, I, P, P, P, P, P, I],
+    "jgcm_barotropic_step_adjoint": [P, I, P, P, P, D, P, P, I],
     "jgcm_divide_by_cos": [P, P, I],
     "jgcm_multiply_by_cos": [P, P, P, I],
     "jgcm_vor_div_from_grid_uv": [P, P, P, P, P, I],

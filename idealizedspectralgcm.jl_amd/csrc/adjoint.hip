@@ -1,0 +1,312 @@
+// Adjoints (transposes with respect to the real inner products
+//   grid:     <x, y> = sum x y
+//   spectral: <a, b> = sum Re(conj(a) b)  over the whole [nlev][ns+1][nm] array)
+// of the operators of the barotropic vorticity step (Barotropic_Dynamics.jl:2-46):
+// Vor_Div_From_Grid_UV!, UV_Grid_From_Vor_Div! (through the u cos θ / v cos θ
+// stencil), the absolute-vorticity flux and the damped, filtered leapfrog with
+// Time_Advance!.  Composed, they give the reverse-mode step
+// jgcm_barotropic_step_adjoint.
+//
+// The elementwise kernels read every input element before writing the
+// outputs of the same index, so their outputs may alias their inputs (the
+// step adjoint runs them in place); no pointer here is __restrict__.
+#pragma clang fp contract(off)
+
+#include "jgcm_internal.hpp"
+
+namespace jgcm {
+
+int ifft_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const double* divcos, int B);
+int scale_eo(const jgcm_mesh* M, double2* eo, int B, double s0, double s1);
+int legendre_alpha_transpose(const jgcm_mesh* M, const double2* h, double2* eo, int B, int table,
+                             int op, int swap, int accumulate);
+
+static inline int nblk(size_t n) {
+  size_t b = (n + 255) / 256;
+  return (int)(b > 8192 ? 8192 : (b == 0 ? 1 : b));
+}
+
+#define GRID_STRIDE(i, n) \
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
+
+// row tables of the α transposes (coef_alp_a, coef_alp_b) and the staging
+// operators applied to the cotangent on the way in
+enum { TAB_ALPA = 1, TAB_ALPB = 2 };
+enum { OP_NONE = 0, OP_NEG_I = 5, OP_NEG = 6 };
+
+__global__ void scale_by_cos_kernel(const double* in, double* out, const double* cost, int nlat,
+                                    int nlon, size_t n, int divide) {
+  GRID_STRIDE(i, n) {
+    double c = cost[(i / nlon) % nlat];
+    out[i] = divide ? in[i] / c : in[i] * c;
+  }
+}
+
+// Transpose of the u cos θ / v cos θ stencil (SSM:568-576, coefficients [m][n], real):
+//   ucos[n] = uvc[n] (i div[n]) - uvm[n] vor[n-1] - uvp[n] vor[n+1]
+//   vcos[n] = uvc[n] (i vor[n]) + uvm[n] div[n-1] + uvp[n] div[n+1]
+// so for row k
+//   vor_bar[k] = uvc[k] (-i vcos_bar[k]) - uvm[k+1] ucos_bar[k+1] - uvp[k-1] ucos_bar[k-1]
+//   div_bar[k] = uvc[k] (-i ucos_bar[k]) + uvm[k+1] vcos_bar[k+1] + uvp[k-1] vcos_bar[k-1]
+// with the forward's guards (the uvm term exists for n >= 1, the uvp term for
+// n <= ns-1).  add_a / add_b (may be NULL, may alias vor_b) are added to vor_bar.
+__global__ void ucos_vcos_adjoint_kernel(const double2* ub, const double2* vb, double2* vor_b,
+                                         double2* div_b, const double2* add_a, const double2* add_b,
+                                         const double* uvc, const double* uvm, const double* uvp,
+                                         int nm, int ns, size_t n_all) {
+  GRID_STRIDE(i, n_all) {
+    int m = (int)(i % nm);
+    int k = (int)((i / nm) % (ns + 1));
+    int c = m * (ns + 1) + k;
+    double cc = uvc[c];
+    double2 u0 = ub[i], v0 = vb[i];
+    double2 zv = make_double2(cc * v0.y, -(cc * v0.x));  // uvc (-i vcos_bar)
+    double2 zd = make_double2(cc * u0.y, -(cc * u0.x));  // uvc (-i ucos_bar)
+    if (k + 1 <= ns) {
+      double a = uvm[c + 1];
+      double2 up = ub[i + nm], vp = vb[i + nm];
+      zv = make_double2(zv.x - a * up.x, zv.y - a * up.y);
+      zd = make_double2(zd.x + a * vp.x, zd.y + a * vp.y);
+    }
+    if (k >= 1) {
+      double a = uvp[c - 1];
+      double2 um = ub[i - nm], vm = vb[i - nm];
+      zv = make_double2(zv.x - a * um.x, zv.y - a * um.y);
+      zd = make_double2(zd.x + a * vm.x, zd.y + a * vm.y);
+    }
+    if (add_a) {
+      double2 t = add_a[i];
+      zv = make_double2(t.x + zv.x, t.y + zv.y);
+    }
+    if (add_b) {
+      double2 t = add_b[i];
+      zv = make_double2(zv.x + t.x, zv.y + t.y);
+    }
+    vor_b[i] = zv;
+    if (div_b) div_b[i] = zd;
+  }
+}
+
+// Transpose of grid_δu = (ζ + f) v, grid_δv = -(ζ + f) u (Barotropic_Dynamics.jl:26-29)
+// at the state (ζ, u, v), f = 2Ω sin θ_j:
+//   ζ_bar = v δu_bar - u δv_bar,  u_bar = -(ζ + f) δv_bar,  v_bar = (ζ + f) δu_bar.
+__global__ void vorticity_flux_adjoint_kernel(const double* vor, const double* u, const double* v,
+                                              const double* sint, double two_omega,
+                                              const double* dub, const double* dvb, double* vor_b,
+                                              double* u_b, double* v_b, int nlat, int nlon,
+                                              size_t n) {
+  GRID_STRIDE(i, n) {
+    double av = vor[i] + two_omega * sint[(i / nlon) % nlat];
+    double a = dub[i], b = dvb[i];
+    double z = v[i] * a - u[i] * b;
+    double ur = -(av * b), vr = av * a;
+    vor_b[i] = z;
+    u_b[i] = ur;
+    v_b[i] = vr;
+  }
+}
+
+// Transpose of Compute_Spectral_Damping! + Filtered_Leapfrog! + Time_Advance!
+// (Time_Integrator.jl:58-101, Dyn_Data.jl:326-364) for one field, from the
+// cotangents of the advanced state (p', c') to those of (p, c) and of the
+// undamped tendency G.  With d = damping, r = Robert coefficient, the step is
+//   later steps:  δ = (G - p d)/(1 + 2Δt d),  c' = p + 2Δt δ,
+//                 p' = (1-2r) c + 2r p + 2rΔt δ
+//   first step:   δ = (G - c d)/(1 + Δt d),   c' = c + Δt δ,   p' = c + rΔt δ
+__global__ void damped_leapfrog_adjoint_kernel(const double2* pb_next, const double2* cb_next,
+                                               double2* pb, double2* cb, double2* gb,
+                                               const double* damping, int nm, int ns, size_t n_all,
+                                               int init_step, double dt, double r) {
+  GRID_STRIDE(i, n_all) {
+    int m = (int)(i % nm);
+    int k = (int)((i / nm) % (ns + 1));
+    double d = damping[m * (ns + 1) + k];
+    double2 P = pb_next[i], C = cb_next[i];
+    if (init_step) {
+      double den = 1.0 + dt * d;
+      double2 db = make_double2((r * dt) * P.x + dt * C.x, (r * dt) * P.y + dt * C.y);
+      double s = d / den;
+      gb[i] = make_double2(db.x / den, db.y / den);
+      cb[i] = make_double2(P.x + C.x - s * db.x, P.y + C.y - s * db.y);
+      pb[i] = make_double2(0.0, 0.0);
+    } else {
+      double den = 1.0 + (2.0 * dt) * d;
+      double2 db = make_double2((2.0 * r * dt) * P.x + (2.0 * dt) * C.x,
+                                (2.0 * r * dt) * P.y + (2.0 * dt) * C.y);
+      double s = d / den;
+      gb[i] = make_double2(db.x / den, db.y / den);
+      double2 np = make_double2((2.0 * r) * P.x + C.x - s * db.x, (2.0 * r) * P.y + C.y - s * db.y);
+      cb[i] = make_double2((1.0 - 2.0 * r) * P.x, (1.0 - 2.0 * r) * P.y);
+      pb[i] = np;
+    }
+  }
+}
+
+// (vor_bar, div_bar) -> (u_bar, v_bar): transposed α contractions into the E/O
+// blocks, transposed FFT stage, then × cos θ (the forward multiplies by cos θ
+// on load).  Uses the mesh's E/O scratch.
+static int vor_div_adjoint(jgcm_mesh* M, const double2* vb, const double2* db, double* u_bar,
+                           double* v_bar, int nlev) {
+  double2* eu = reinterpret_cast<double2*>(M->d_four_a);
+  double2* ev = reinterpret_cast<double2*>(M->d_four_b);
+  // vor = alp_a (i V^p) - alp_b U^(1-p)    div = alp_a (i U^p) + alp_b V^(1-p)   (SSM:387-392)
+  if (int rc = legendre_alpha_transpose(M, vb, ev, nlev, TAB_ALPA, OP_NEG_I, 0, 0)) return rc;
+  if (int rc = legendre_alpha_transpose(M, vb, eu, nlev, TAB_ALPB, OP_NEG, 1, 0)) return rc;
+  if (db) {
+    if (int rc = legendre_alpha_transpose(M, db, eu, nlev, TAB_ALPA, OP_NEG_I, 0, 1)) return rc;
+    if (int rc = legendre_alpha_transpose(M, db, ev, nlev, TAB_ALPB, OP_NONE, 1, 1)) return rc;
+  }
+  double L = M->nlon;
+  if (int rc = scale_eo(M, eu, nlev, 1.0 / L, 1.0 / (2.0 * L))) return rc;
+  if (int rc = scale_eo(M, ev, nlev, 1.0 / L, 1.0 / (2.0 * L))) return rc;
+  if (int rc = ifft_unfold(M, eu, u_bar, nullptr, nlev)) return rc;
+  if (int rc = ifft_unfold(M, ev, v_bar, nullptr, nlev)) return rc;
+  size_t n = (size_t)nlev * M->grid_elems();
+  hipLaunchKernelGGL(scale_by_cos_kernel, dim3(nblk(n)), dim3(256), 0, M->stream, u_bar, u_bar,
+                     M->d_cos, M->nlat, M->nlon, n, 0);
+  hipLaunchKernelGGL(scale_by_cos_kernel, dim3(nblk(n)), dim3(256), 0, M->stream, v_bar, v_bar,
+                     M->d_cos, M->nlat, M->nlon, n, 0);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+static void ucos_vcos_adjoint(jgcm_mesh* M, const double* su, const double* sv, double* vor_bar,
+                              double* div_bar, const double* add_a, const double* add_b,
+                              int nlev) {
+  size_t n = (size_t)nlev * M->spec_elems();
+  hipLaunchKernelGGL(ucos_vcos_adjoint_kernel, dim3(nblk(n)), dim3(256), 0, M->stream,
+                     (const double2*)su, (const double2*)sv, (double2*)vor_bar, (double2*)div_bar,
+                     (const double2*)add_a, (const double2*)add_b, M->d_uvc, M->d_uvm, M->d_uvp,
+                     M->nm, M->ns, n);
+}
+
+}  // namespace jgcm
+
+using namespace jgcm;
+
+extern "C" {
+
+int jgcm_trans_spherical_to_grid_adjoint(jgcm_mesh_t* mesh, const double* pfield, double* snm,
+                                         int nlev);
+
+int jgcm_vor_div_from_grid_uv_adjoint(jgcm_mesh_t* mesh, const double* vor_bar, const double* div_bar,
+                                      double* u_bar, double* v_bar, int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(vor_bar && u_bar && v_bar, "NULL field pointer");
+  if (int rc = ensure_scratch(mesh, nlev)) return rc;
+  return vor_div_adjoint(mesh, reinterpret_cast<const double2*>(vor_bar),
+                         reinterpret_cast<const double2*>(div_bar), u_bar, v_bar, nlev);
+}
+
+int jgcm_ucos_vcos_adjoint(jgcm_mesh_t* mesh, const double* ucos_bar, const double* vcos_bar,
+                           double* vor_bar, double* div_bar, int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(ucos_bar && vcos_bar && vor_bar, "NULL field pointer");
+  ucos_vcos_adjoint(mesh, ucos_bar, vcos_bar, vor_bar, div_bar, nullptr, nullptr, nlev);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_uv_grid_from_vor_div_adjoint(jgcm_mesh_t* mesh, const double* u_bar, const double* v_bar,
+                                      double* vor_bar, double* div_bar, int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(u_bar && v_bar && vor_bar, "NULL field pointer");
+  if (int rc = ensure_scratch(mesh, nlev)) return rc;  // before any scratch pointer is taken
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  double* gu = mesh->d_grid_a;
+  double* gv = mesh->d_grid_b;
+  hipLaunchKernelGGL(scale_by_cos_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, u_bar, gu,
+                     mesh->d_cos, mesh->nlat, mesh->nlon, n, 1);
+  hipLaunchKernelGGL(scale_by_cos_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, v_bar, gv,
+                     mesh->d_cos, mesh->nlat, mesh->nlon, n, 1);
+  JGCM_LAUNCH_CHECK();
+  double* su = mesh->d_spec_a;
+  double* sv = mesh->d_spec_b;
+  if (int rc = jgcm_trans_spherical_to_grid_adjoint(mesh, gu, su, nlev)) return rc;
+  if (int rc = jgcm_trans_spherical_to_grid_adjoint(mesh, gv, sv, nlev)) return rc;
+  ucos_vcos_adjoint(mesh, su, sv, vor_bar, div_bar, nullptr, nullptr, nlev);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_vorticity_flux_adjoint(jgcm_mesh_t* mesh, const double* grid_vor, const double* grid_u,
+                                const double* grid_v, double omega, const double* du_bar,
+                                const double* dv_bar, double* vor_bar, double* u_bar, double* v_bar,
+                                int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(grid_vor && grid_u && grid_v && du_bar && dv_bar && vor_bar && u_bar && v_bar,
+               "NULL field pointer");
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  hipLaunchKernelGGL(vorticity_flux_adjoint_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream,
+                     grid_vor, grid_u, grid_v, mesh->d_sin, 2.0 * omega, du_bar, dv_bar, vor_bar,
+                     u_bar, v_bar, mesh->nlat, mesh->nlon, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_damped_leapfrog_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* p_bar_next,
+                                 const double* c_bar_next, double* p_bar, double* c_bar,
+                                 double* g_bar, int nlev) {
+  JGCM_REQUIRE(lf != nullptr && lf->mesh != nullptr, "leapfrog is NULL");
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(p_bar_next && c_bar_next && p_bar && c_bar && g_bar, "NULL field pointer");
+  jgcm_mesh* M = lf->mesh;
+  size_t n = (size_t)nlev * M->spec_elems();
+  hipLaunchKernelGGL(damped_leapfrog_adjoint_kernel, dim3(nblk(n)), dim3(256), 0, M->stream,
+                     (const double2*)p_bar_next, (const double2*)c_bar_next, (double2*)p_bar,
+                     (double2*)c_bar, (double2*)g_bar, lf->d_damping, M->nm, M->ns, n,
+                     init_step ? 1 : 0, (double)lf->dt, lf->robert_coef);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+// One reverse step of Barotropic_Dynamics! (the first step of a run when
+// init_step != 0), in place on (p_bar, c_bar):
+//   (p_bar, c_bar, G_bar) <- leapfrog^T (p_bar, c_bar)
+//   (δu_bar, δv_bar)      <- VorDiv^T (G_bar, 0)
+//   (ζ_bar, u_bar, v_bar) <- flux^T at (grid_vor, grid_u, grid_v)
+//   c_bar += T^-T ζ_bar + stencil^T(T^-T (u_bar / cos θ), T^-T (v_bar / cos θ))
+// grid_vor, grid_u, grid_v are the grid fields of the step's current state
+// (T^-1 vor_c and UV_Grid_From_Vor_Div!(vor_c, 0)).
+int jgcm_barotropic_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* grid_vor,
+                                 const double* grid_u, const double* grid_v, double omega,
+                                 double* p_bar, double* c_bar, int nlev) {
+  JGCM_REQUIRE(lf != nullptr && lf->mesh != nullptr, "leapfrog is NULL");
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(grid_vor && grid_u && grid_v && p_bar && c_bar, "NULL field pointer");
+  jgcm_mesh* M = lf->mesh;
+  if (int rc = check_mesh(M)) return rc;
+  if (int rc = ensure_scratch(M, 2 * nlev)) return rc;  // before any scratch pointer is taken
+  const size_t S = (size_t)nlev * M->spec_elems() * 2, G = (size_t)nlev * M->grid_elems();
+  double* gbar = M->d_spec_a;
+  double* tz = M->d_spec_a + S;
+  double* su = M->d_spec_b;
+  double* sv = M->d_spec_b + S;
+  double* ga0 = M->d_grid_a;
+  double* ga1 = M->d_grid_a + G;
+  double* gb0 = M->d_grid_b;
+  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, p_bar, c_bar, p_bar, c_bar, gbar, nlev))
+    return rc;
+  if (int rc = vor_div_adjoint(M, reinterpret_cast<const double2*>(gbar), nullptr, ga0, ga1, nlev))
+    return rc;
+  hipLaunchKernelGGL(vorticity_flux_adjoint_kernel, dim3(nblk(G)), dim3(256), 0, M->stream,
+                     grid_vor, grid_u, grid_v, M->d_sin, 2.0 * omega, ga0, ga1, gb0, ga0, ga1,
+                     M->nlat, M->nlon, G);
+  hipLaunchKernelGGL(scale_by_cos_kernel, dim3(nblk(G)), dim3(256), 0, M->stream, ga0, ga0,
+                     M->d_cos, M->nlat, M->nlon, G, 1);
+  hipLaunchKernelGGL(scale_by_cos_kernel, dim3(nblk(G)), dim3(256), 0, M->stream, ga1, ga1,
+                     M->d_cos, M->nlat, M->nlon, G, 1);
+  JGCM_LAUNCH_CHECK();
+  if (int rc = jgcm_trans_spherical_to_grid_adjoint(M, gb0, tz, nlev)) return rc;
+  if (int rc = jgcm_trans_spherical_to_grid_adjoint(M, ga0, su, nlev)) return rc;
+  if (int rc = jgcm_trans_spherical_to_grid_adjoint(M, ga1, sv, nlev)) return rc;
+  ucos_vcos_adjoint(M, su, sv, c_bar, nullptr, c_bar, tz, nlev);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+}  // extern "C"

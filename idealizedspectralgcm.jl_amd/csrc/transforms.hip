@@ -57,6 +57,8 @@ enum {
   SPEC_OP_GRAD_THETA = 2,   // cos θ ∂θ h / a      (SSM:682-687)
   SPEC_OP_UCOS = 3,         // u cos θ from (vor, div)  (Compute_Ucos_Vcos_From_Vor_Div!, SSM:568-576)
   SPEC_OP_VCOS = 4,         // v cos θ from (vor, div)
+  SPEC_OP_NEG_I = 5,        // -i h  (transposes of the α operator, SSM:449-456)
+  SPEC_OP_NEG = 6,          // -h
 };
 
 // coefficient tables [m][ns+1] of those operators
@@ -730,6 +732,14 @@ __device__ __forceinline__ double2 staged_coefficient(const SegMap& in, const Sp
   const int op = (in.flag[k] >> 1) & 7;
   const size_t i = (size_t)n * nm + m;
   if (op == SPEC_OP_NONE) return h[i];
+  if (op == SPEC_OP_NEG_I) {
+    double2 v = h[i];
+    return make_double2(v.y, -v.x);
+  }
+  if (op == SPEC_OP_NEG) {
+    double2 v = h[i];
+    return make_double2(-v.x, -v.y);
+  }
   const int c = m * (ns + 1) + n;
   if (op == SPEC_OP_GRAD_LAMBDA) {
     double2 v = h[i];
@@ -790,7 +800,7 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
                                                          FwdTables tab, int B, int nmax, int ns,
                                                          int nm, int half,
                                                          const int* __restrict__ row_start,
-                                                         SpecOpTables ops) {
+                                                         SpecOpTables ops, int out_mode) {
   using LT = typename LdsT<PREC>::T;
   using TT = typename TabT<PREC>::T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -852,16 +862,23 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
   }
   const int col = 16 * wave + li;
   const int fl = col >> 1, comp = col & 1;
+  // out_mode bit 0: write the opposite parity's block (transposed two-table
+  // contraction); bit 1: add to the block instead of overwriting it
+  const int pout = (out_mode & 1) ? par ^ 1 : par;
   if (fl < nf) {
     double* base = reinterpret_cast<double*>(eo) +
-                   ((((size_t)(f0 + fl) * nm + m) * 2 + par) * half + lat0) * 2 + comp;
+                   ((((size_t)(f0 + fl) * nm + m) * 2 + pout) * half + lat0) * 2 + comp;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       if (t >= nlt) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         int j = 16 * t + tile_row<PREC>(lk, r);
-        if (j < nlg) base[(size_t)j * 2] = acc.v[t][r];
+        if (j < nlg) {
+          double v = acc.v[t][r];
+          if (out_mode & 2) v += base[(size_t)j * 2];
+          base[(size_t)j * 2] = v;
+        }
       }
     }
   }
@@ -983,7 +1000,7 @@ static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int 
 }
 
 static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, int table,
-                            int nmax) {
+                            int nmax, int out_mode = 0) {
   int B = snm.start[snm.nseg];
   int fchunks = (B + LCOLS / 2 - 1) / (LCOLS / 2);
   dim3 grd(2 * M->nm, fchunks, (M->half + TROWS - 1) / TROWS);
@@ -993,19 +1010,23 @@ static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, 
   switch (M->precision) {
     case JGCM_PREC_FP64:
       hipLaunchKernelGGL(legendre_inv_mfma<P_FP64>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops);
+                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops,
+                         out_mode);
       break;
     case JGCM_PREC_FP32:
       hipLaunchKernelGGL(legendre_inv_mfma<P_FP32>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops);
+                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops,
+                         out_mode);
       break;
     case JGCM_PREC_BF16X3:
       hipLaunchKernelGGL(legendre_inv_mfma<P_BF16>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops);
+                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops,
+                         out_mode);
       break;
     default:
       hipLaunchKernelGGL(legendre_inv_mfma<P_SPLIT>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops);
+                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops,
+                         out_mode);
       break;
   }
   JGCM_LAUNCH_CHECK();
@@ -1019,6 +1040,14 @@ int legendre_inv_t(const jgcm_mesh* M, const double2* snm, double2* eo, int B, i
 
 int legendre_inv(const jgcm_mesh* M, const double2* snm, double2* eo, int B) {
   return legendre_inv_t(M, snm, eo, B, 3, M->ns);
+}
+
+// Transposed α-operator contraction: eo (+)= table^T (op h) over rows n <= nmax,
+// written to the parity block of the rows (or the opposite one).
+int legendre_alpha_transpose(const jgcm_mesh* M, const double2* h, double2* eo, int B, int table,
+                             int op, int swap, int accumulate) {
+  SegMap in = one_seg(h, B, op << 1);
+  return legendre_inv_map(M, in, eo, table, M->N, (swap ? 1 : 0) | (accumulate ? 2 : 0));
 }
 
 static FwdJob plain_job(const jgcm_mesh* M, const double2* eo, int nf, const SegMap& out) {

@@ -198,6 +198,41 @@ int jgcm_trans_grid_to_spherical_adjoint(jgcm_mesh_t* mesh, const double* snm, d
 int jgcm_trans_spherical_to_grid_adjoint(jgcm_mesh_t* mesh, const double* pfield, double* snm,
                                          int nlev);
 
+/* Adjoints of the barotropic step's operators (no reference counterpart; the
+ * same inner products).  Outputs may not alias inputs.
+ * vor_div_from_grid_uv_adjoint: transpose of Vor_Div_From_Grid_UV!
+ *   (Spectral_Spherical_Mesh.jl:349-393); div_bar may be NULL (zero).       */
+int jgcm_vor_div_from_grid_uv_adjoint(jgcm_mesh_t* mesh, const double* vor_bar, const double* div_bar,
+                                      double* u_bar, double* v_bar, int nlev);
+/* transpose of the u cos θ / v cos θ stencil of Compute_Ucos_Vcos_From_Vor_Div!
+ * (Spectral_Spherical_Mesh.jl:517-580); div_bar may be NULL (not written).  */
+int jgcm_ucos_vcos_adjoint(jgcm_mesh_t* mesh, const double* ucos_bar, const double* vcos_bar,
+                           double* vor_bar, double* div_bar, int nlev);
+/* transpose of UV_Grid_From_Vor_Div! (Spectral_Spherical_Mesh.jl:462-482);
+ * div_bar may be NULL.                                                      */
+int jgcm_uv_grid_from_vor_div_adjoint(jgcm_mesh_t* mesh, const double* u_bar, const double* v_bar,
+                                      double* vor_bar, double* div_bar, int nlev);
+/* transpose of the absolute-vorticity flux δu = (ζ+f) v, δv = -(ζ+f) u
+ * (Barotropic_Dynamics.jl:26-29) linearised at (grid_vor, grid_u, grid_v).  */
+int jgcm_vorticity_flux_adjoint(jgcm_mesh_t* mesh, const double* grid_vor, const double* grid_u,
+                                const double* grid_v, double omega, const double* du_bar,
+                                const double* dv_bar, double* vor_bar, double* u_bar, double* v_bar,
+                                int nlev);
+/* transpose of Compute_Spectral_Damping! + Filtered_Leapfrog! + Time_Advance!
+ * (Time_Integrator.jl:58-101, Dyn_Data.jl:326-364) for one field:
+ * (p_bar', c_bar') -> (p_bar, c_bar, g_bar), g_bar the cotangent of the
+ * undamped tendency.  init_step selects the first-step form.  In place
+ * (p_bar == p_bar_next, c_bar == c_bar_next) is allowed.                     */
+int jgcm_damped_leapfrog_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* p_bar_next,
+                                 const double* c_bar_next, double* p_bar, double* c_bar,
+                                 double* g_bar, int nlev);
+/* one reverse step of Barotropic_Dynamics! (Barotropic_Dynamics.jl:2-46), in
+ * place on (p_bar, c_bar); grid_vor, grid_u, grid_v are the grid fields of
+ * the step's starting state.                                                */
+int jgcm_barotropic_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* grid_vor,
+                                 const double* grid_u, const double* grid_v, double omega,
+                                 double* p_bar, double* c_bar, int nlev);
+
 /* ---- operators ---------------------------------------------------------- */
 /* Divide_By_Cos!(cosθ, grid)                    Spectral_Spherical_Mesh.jl:333-338 */
 int jgcm_divide_by_cos(jgcm_mesh_t* mesh, double* grid, int nlev);
