@@ -67,6 +67,7 @@ SIGNATURES = {
     "jgcm_vorticity_flux_adjoint": [P, P, P, P, D, P, P, P, P, P, I],
     "jgcm_damped_leapfrog_adjoint": [P, I, P, P, P, P, P, I],
     "jgcm_barotropic_step_adjoint": [P, I, P, P, P, D, P, P, I],
+    "jgcm_sw_step_adjoint": [P, I, P, P, P, P, P, P, D, D, D, D, P, P, P, P, P, P, I],
     "jgcm_divide_by_cos": [P, P, I],
     "jgcm_multiply_by_cos": [P, P, P, I],
     "jgcm_vor_div_from_grid_uv": [P, P, P, P, P, I],

@@ -180,6 +180,131 @@ static void ucos_vcos_adjoint(jgcm_mesh* M, const double* su, const double* sv, 
                      M->nm, M->ns, n);
 }
 
+// ---- shallow-water step (Shallow_Water_Dynamics.jl:7-21, :88-179) ----------
+
+// Transpose of the semi-implicit correction (Shallow_Water_Dynamics.jl:36-51)
+//   D1 = Gd + e (h_c - h_p),  H1 = Gh + h0 (div_c - div_p),
+//   Dn = (D1 - μ e H1) / (1 - μ² e h0),  Hn = H1 - μ h0 Dn
+// and of the energy term Gd -= ∇²E, E = F(KE) + h_c (:137-143), e = -n(n+1)/a².
+// In: (gd, gh) = cotangents of (Dn, Hn).  Out: (gd, gh) = cotangents of the
+// explicit tendencies (Gd, Gh); ebar = cotangent of E; the state cotangents
+// gain their linear terms.  (The h_c terms e D1 and -e D1 cancel: the scheme
+// replaces the explicit -∇²h_c by the implicit average.)
+__global__ void sw_implicit_adjoint_kernel(double2* gd, double2* gh, double2* ebar, double2* cd,
+                                           double2* pd, double2* ch, double2* ph, const double* eig,
+                                           double h0, double mu, int nm, int ns, size_t n_all) {
+  GRID_STRIDE(i, n_all) {
+    int m = (int)(i % nm);
+    int k = (int)((i / nm) % (ns + 1));
+    double e = eig[m * (ns + 1) + k];
+    double2 Dn = gd[i], Hn = gh[i];
+    double mh = mu * h0;
+    double2 Dt = make_double2(Dn.x - mh * Hn.x, Dn.y - mh * Hn.y);
+    double den = 1.0 - (mu * mu * e) * h0;
+    double2 D1 = make_double2(Dt.x / den, Dt.y / den);
+    double me = mu * e;
+    double2 H1 = make_double2(Hn.x - me * D1.x, Hn.y - me * D1.y);
+    gd[i] = D1;
+    gh[i] = H1;
+    double2 eb = make_double2(-(e * D1.x), -(e * D1.y));
+    ebar[i] = eb;
+    double2 c = ch[i], p = ph[i];
+    ch[i] = make_double2(c.x + e * D1.x + eb.x, c.y + e * D1.y + eb.y);
+    ph[i] = make_double2(p.x - e * D1.x, p.y - e * D1.y);
+    double2 a = cd[i], b = pd[i];
+    cd[i] = make_double2(a.x + h0 * H1.x, a.y + h0 * H1.y);
+    pd[i] = make_double2(b.x - h0 * H1.x, b.y - h0 * H1.y);
+  }
+}
+
+// Transpose of the grid part of the SW tendencies at the state (ζ, D, H, u, v):
+//   δu = -κm u + (ζ+f) v,  δv = -κm v - (ζ+f) u,  δH = -κt (H - h_eq)   (:16-18, :126-130)
+//   KE = (u² + v²)/2                                                     (:139)
+//   δH -= u gλ + v gθ   (gλ, gθ the grid gradients of h_c, SSM:716)     (:147)
+//   δH -= H D                                                            (:148)
+// In: cotangents of δu, δv, δH and KE.  Out (any slot may alias an input
+// slot): ζ_bar, D_bar, H_bar, and u_bar, v_bar, gλ_bar, gθ_bar divided by
+// cos θ (the inverse transforms behind them divide by cos θ).
+struct SwGridAdj {
+  const double *vor, *div, *h, *u, *v, *gl, *gt;       // state
+  const double *dub, *dvb, *dhb, *kb;                  // cotangents in
+  double *zb, *db, *hb, *ubc, *vbc, *glbc, *gtbc;      // cotangents out
+};
+
+__global__ void sw_grid_adjoint_kernel(SwGridAdj s, const double* sint, const double* cost,
+                                       double two_omega, double km, double kt, int nlat, int nlon,
+                                       size_t n) {
+  GRID_STRIDE(i, n) {
+    int j = (int)((i / nlon) % nlat);
+    double av = s.vor[i] + two_omega * sint[j];
+    double c = cost[j];
+    double U = s.u[i], V = s.v[i], H = s.h[i], D = s.div[i], GL = s.gl[i], GT = s.gt[i];
+    double a = s.dub[i], b = s.dvb[i], q = s.dhb[i], kk = s.kb[i];
+    double ub = -(km * a) - av * b - GL * q + U * kk;
+    double vb = -(km * b) + av * a - GT * q + V * kk;
+    double zb = V * a - U * b;
+    double db = -(H * q);
+    double hb = -(D * q) - kt * q;
+    double glb = -(U * q), gtb = -(V * q);
+    s.zb[i] = zb;
+    s.db[i] = db;
+    s.hb[i] = hb;
+    s.ubc[i] = ub / c;
+    s.vbc[i] = vb / c;
+    s.glbc[i] = glb / c;
+    s.gtbc[i] = gtb / c;
+  }
+}
+
+// State cotangents gain the spectral transposes:
+//   vor_bar += tz + stencil_vor^T(su, sv),  div_bar += td + stencil_div^T(su, sv)
+//   h_bar   += th + grad^T(sgl, sgt)
+// with the u cos / v cos stencil of ucos_vcos_adjoint_kernel and the gradient
+// stencil of Compute_Gradient_Cos! (SSM:677-687):
+//   gλ[n] = dlam[n] (i h[n]),  gθ[n] = dthp[n] h[n+1] (n <= ns-1) + dthm[n] h[n-1] (n >= 1)
+struct SwSpecAdj {
+  const double2 *tz, *td, *th, *su, *sv, *sgl, *sgt;
+  double2 *vor_b, *div_b, *h_b;
+};
+
+__global__ void sw_spectral_adjoint_kernel(SwSpecAdj s, const double* uvc, const double* uvm,
+                                           const double* uvp, const double* dlam, const double* dthm,
+                                           const double* dthp, int nm, int ns, size_t n_all) {
+  GRID_STRIDE(i, n_all) {
+    int m = (int)(i % nm);
+    int k = (int)((i / nm) % (ns + 1));
+    int c = m * (ns + 1) + k;
+    double cc = uvc[c], dl = dlam[c];
+    double2 u0 = s.su[i], v0 = s.sv[i], l0 = s.sgl[i];
+    double2 zv = make_double2(cc * v0.y, -(cc * v0.x));
+    double2 zd = make_double2(cc * u0.y, -(cc * u0.x));
+    double2 zh = make_double2(dl * l0.y, -(dl * l0.x));
+    if (k + 1 <= ns) {
+      double a = uvm[c + 1], t = dthm[c + 1];
+      double2 up = s.su[i + nm], vp = s.sv[i + nm], gp = s.sgt[i + nm];
+      zv = make_double2(zv.x - a * up.x, zv.y - a * up.y);
+      zd = make_double2(zd.x + a * vp.x, zd.y + a * vp.y);
+      zh = make_double2(zh.x + t * gp.x, zh.y + t * gp.y);
+    }
+    if (k >= 1) {
+      double a = uvp[c - 1], t = dthp[c - 1];
+      double2 um = s.su[i - nm], vm = s.sv[i - nm], gm = s.sgt[i - nm];
+      zv = make_double2(zv.x - a * um.x, zv.y - a * um.y);
+      zd = make_double2(zd.x + a * vm.x, zd.y + a * vm.y);
+      zh = make_double2(zh.x + t * gm.x, zh.y + t * gm.y);
+    }
+    double2 t0 = s.tz[i], t1 = s.td[i], t2 = s.th[i];
+    double2 v = s.vor_b[i], d = s.div_b[i], h = s.h_b[i];
+    s.vor_b[i] = make_double2(v.x + (t0.x + zv.x), v.y + (t0.y + zv.y));
+    s.div_b[i] = make_double2(d.x + (t1.x + zd.x), d.y + (t1.y + zd.y));
+    s.h_b[i] = make_double2(h.x + (t2.x + zh.x), h.y + (t2.y + zh.y));
+  }
+}
+
+int inverse_batch(const jgcm_mesh* M, int nseg, const double* const* snm,
+                  const double* const* snm_b, double* const* grid, const int* nlev,
+                  const int* flags);
+
 }  // namespace jgcm
 
 using namespace jgcm;
@@ -305,6 +430,83 @@ int jgcm_barotropic_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const doubl
   if (int rc = jgcm_trans_spherical_to_grid_adjoint(M, ga0, su, nlev)) return rc;
   if (int rc = jgcm_trans_spherical_to_grid_adjoint(M, ga1, sv, nlev)) return rc;
   ucos_vcos_adjoint(M, su, sv, c_bar, nullptr, c_bar, tz, nlev);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_trans_grid_to_spherical_adjoint(jgcm_mesh_t* mesh, const double* snm, double* pfield,
+                                         int nlev);
+
+// One reverse step of Shallow_Water_Physics! + Shallow_Water_Dynamics!
+// (Shallow_Water_Dynamics.jl:7-21, :88-179), in place on the six state
+// cotangents.  grid_vor, grid_div, grid_h, grid_u, grid_v are the grid fields
+// of the step's starting state and spe_h its spectral height (the field the
+// height advection differentiates).  Scratch: four level-field slots in each
+// of the mesh's spectral and grid buffers (S_a0..3, S_b0..3, G_a0..3, G_b0..3).
+int jgcm_sw_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* grid_vor,
+                         const double* grid_div, const double* grid_h, const double* grid_u,
+                         const double* grid_v, const double* spe_h, double omega, double kappa_m,
+                         double kappa_t, double h_0, double* vor_p_bar, double* vor_c_bar,
+                         double* div_p_bar, double* div_c_bar, double* h_p_bar, double* h_c_bar,
+                         int nlev) {
+  JGCM_REQUIRE(lf != nullptr && lf->mesh != nullptr, "leapfrog is NULL");
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(grid_vor && grid_div && grid_h && grid_u && grid_v && spe_h, "NULL state pointer");
+  JGCM_REQUIRE(vor_p_bar && vor_c_bar && div_p_bar && div_c_bar && h_p_bar && h_c_bar,
+               "NULL cotangent pointer");
+  jgcm_mesh* M = lf->mesh;
+  if (int rc = check_mesh(M)) return rc;
+  if (int rc = ensure_scratch(M, 4 * nlev)) return rc;  // before any scratch pointer is taken
+  const size_t S = (size_t)nlev * M->spec_elems() * 2, G = (size_t)nlev * M->grid_elems();
+  double* Sa[4] = {M->d_spec_a, M->d_spec_a + S, M->d_spec_a + 2 * S, M->d_spec_a + 3 * S};
+  double* Sb[4] = {M->d_spec_b, M->d_spec_b + S, M->d_spec_b + 2 * S, M->d_spec_b + 3 * S};
+  double* Ga[4] = {M->d_grid_a, M->d_grid_a + G, M->d_grid_a + 2 * G, M->d_grid_a + 3 * G};
+  double* Gb[4] = {M->d_grid_b, M->d_grid_b + G, M->d_grid_b + 2 * G, M->d_grid_b + 3 * G};
+  const size_t ns_all = (size_t)nlev * M->spec_elems();
+  // damping + leapfrog + Time_Advance: tendency cotangents Gv, Gd, Gh -> S_a0..2
+  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, vor_p_bar, vor_c_bar, vor_p_bar,
+                                            vor_c_bar, Sa[0], nlev)) return rc;
+  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, div_p_bar, div_c_bar, div_p_bar,
+                                            div_c_bar, Sa[1], nlev)) return rc;
+  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, h_p_bar, h_c_bar, h_p_bar, h_c_bar,
+                                            Sa[2], nlev)) return rc;
+  // semi-implicit correction and the energy term; E_bar -> S_a3
+  const double mu = lf->implicit_coef * (init_step ? (double)lf->dt : 2.0 * (double)lf->dt);
+  hipLaunchKernelGGL(sw_implicit_adjoint_kernel, dim3(nblk(ns_all)), dim3(256), 0, M->stream,
+                     (double2*)Sa[1], (double2*)Sa[2], (double2*)Sa[3], (double2*)div_c_bar,
+                     (double2*)div_p_bar, (double2*)h_c_bar, (double2*)h_p_bar, M->d_eig, h_0, mu,
+                     M->nm, M->ns, ns_all);
+  JGCM_LAUNCH_CHECK();
+  // δH_bar = F^T Gh_bar -> G_a0, KE_bar = F^T E_bar -> G_a1 (one call over S_a2..3)
+  if (int rc = jgcm_trans_grid_to_spherical_adjoint(M, Sa[2], Ga[0], 2 * nlev)) return rc;
+  // (δu_bar, δv_bar) = VorDiv^T(Gv_bar, Gd_bar) -> G_a2, G_a3
+  if (int rc = vor_div_adjoint(M, reinterpret_cast<const double2*>(Sa[0]),
+                               reinterpret_cast<const double2*>(Sa[1]), Ga[2], Ga[3], nlev))
+    return rc;
+  // grid gradients of h (the forward's Add_Horizontal_Advection!) -> G_b0, G_b1
+  {
+    const double* s[2] = {spe_h, spe_h};
+    const double* s2[2] = {nullptr, nullptr};
+    double* g[2] = {Gb[0], Gb[1]};
+    int nl[2] = {nlev, nlev};
+    int fl[2] = {JGCM_BATCH_DIVCOS | JGCM_INV_GRAD_LAMBDA, JGCM_BATCH_DIVCOS | JGCM_INV_GRAD_THETA};
+    if (int rc = inverse_batch(M, 2, s, s2, g, nl, fl)) return rc;
+  }
+  SwGridAdj ga{grid_vor, grid_div, grid_h, grid_u, grid_v, Gb[0], Gb[1],
+               Ga[2], Ga[3], Ga[0], Ga[1],
+               Ga[0], Ga[1], Ga[2], Ga[3], Gb[0], Gb[1], Gb[2]};
+  hipLaunchKernelGGL(sw_grid_adjoint_kernel, dim3(nblk(G)), dim3(256), 0, M->stream, ga, M->d_sin,
+                     M->d_cos, 2.0 * omega, kappa_m, kappa_t, M->nlat, M->nlon, G);
+  JGCM_LAUNCH_CHECK();
+  // T^-T of (ζ, D, H, u/cos) -> S_a0..3 and of (v/cos, gλ/cos, gθ/cos) -> S_b0..2
+  if (int rc = jgcm_trans_spherical_to_grid_adjoint(M, Ga[0], Sa[0], 4 * nlev)) return rc;
+  if (int rc = jgcm_trans_spherical_to_grid_adjoint(M, Gb[0], Sb[0], 3 * nlev)) return rc;
+  SwSpecAdj sa{(const double2*)Sa[0], (const double2*)Sa[1], (const double2*)Sa[2],
+               (const double2*)Sa[3], (const double2*)Sb[0], (const double2*)Sb[1],
+               (const double2*)Sb[2], (double2*)vor_c_bar, (double2*)div_c_bar, (double2*)h_c_bar};
+  hipLaunchKernelGGL(sw_spectral_adjoint_kernel, dim3(nblk(ns_all)), dim3(256), 0, M->stream, sa,
+                     M->d_uvc, M->d_uvm, M->d_uvp, M->d_dlam, M->d_dthm, M->d_dthp, M->nm, M->ns,
+                     ns_all);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

@@ -232,6 +232,16 @@ int jgcm_damped_leapfrog_adjoint(jgcm_leapfrog_t* lf, int init_step, const doubl
 int jgcm_barotropic_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* grid_vor,
                                  const double* grid_u, const double* grid_v, double omega,
                                  double* p_bar, double* c_bar, int nlev);
+/* one reverse step of Shallow_Water_Physics! + Shallow_Water_Dynamics!
+ * (Shallow_Water_Dynamics.jl:7-21, :88-179), in place on the six state
+ * cotangents; the grid fields and spe_h (the spectral height) are those of
+ * the step's starting state.                                                */
+int jgcm_sw_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* grid_vor,
+                         const double* grid_div, const double* grid_h, const double* grid_u,
+                         const double* grid_v, const double* spe_h, double omega, double kappa_m,
+                         double kappa_t, double h_0, double* vor_p_bar, double* vor_c_bar,
+                         double* div_p_bar, double* div_c_bar, double* h_p_bar, double* h_c_bar,
+                         int nlev);
 
 /* ---- operators ---------------------------------------------------------- */
 /* Divide_By_Cos!(cosθ, grid)                    Spectral_Spherical_Mesh.jl:333-338 */
