@@ -84,6 +84,13 @@ __host__ __device__ __forceinline__ double* seg_field(const SegMap& s, int b, si
   return s.ptr[k] + (size_t)(b - s.start[k]) * elems;
 }
 
+// First packed table row of wavenumber m (rows n = m..ns per m, m = 0..N):
+// sum_{k<m} (ns + 1 - k).  Computed rather than loaded, so the table
+// addresses of a Legendre workgroup do not wait on a memory round trip.
+__device__ __forceinline__ int packed_row_start(int m, int ns) {
+  return m * (ns + 1) - (m * (m - 1)) / 2;
+}
+
 static inline SegMap one_seg(const void* p, int nlev, int flag) {
   SegMap s{};
   s.nseg = 1;
@@ -114,12 +121,16 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
   const int seg = seg_find(in, b);
   const double* __restrict__ grid = in.ptr[seg] + (size_t)(b - in.start[seg]) * nlat * L;
   const double* __restrict__ cosmul = (in.flag[seg] & 1) ? cosv : nullptr;
-  for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
-  __syncthreads();
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const double scale = 1.0 / L;
-  // all passes' rows are loaded up front (PASSES x 8 loads in flight per thread)
+  // twiddles and all passes' rows are loaded up front (one round trip, up to
+  // PASSES x 8 + TWI loads in flight per thread)
+  constexpr int TWI = (L + 255) / 256;
+  double2 twr[TWI];
+#pragma unroll
+  for (int q = 0; q < TWI; ++q)
+    if (q * 256 + (int)threadIdx.x < L) twr[q] = twg[q * 256 + threadIdx.x];
   constexpr int PASSES = (L == 256) ? 2 : ((L == 512) ? 4 : ((L == 1024) ? 8 : 1));
   double2 pre[PASSES][4];
 #pragma unroll
@@ -133,6 +144,10 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
       for (int r = 0; r < 4; ++r) pre[q][r] = make_double2(ra[t + r * T], rc[t + r * T]);
     }
   }
+#pragma unroll
+  for (int q = 0; q < TWI; ++q)
+    if (q * 256 + (int)threadIdx.x < L) tw[q * 256 + threadIdx.x] = twr[q];
+  __syncthreads();
   for (int pass = 0; pass < ppw; pass += FPW) {
     int jj = pass + g;
     int jp = j0 + jj;
@@ -205,7 +220,12 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
   const int seg = seg_find(out, b);
   double* __restrict__ grid = out.ptr[seg] + (size_t)(b - out.start[seg]) * nlat * L;
   const double* __restrict__ divcos = (out.flag[seg] & 1) ? cosv : nullptr;
-  for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
+  // twiddle loads in flight together with the E/O gather below
+  constexpr int TWI = (L + 255) / 256;
+  double2 twr[TWI];
+#pragma unroll
+  for (int q = 0; q < TWI; ++q)
+    if (q * 256 + (int)threadIdx.x < L) twr[q] = twg[q * 256 + threadIdx.x];
   const int total = nm * 2 * ppw;
   staged_copy<8>(
       total,
@@ -216,6 +236,9 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
         return (jp < half) ? eo[((size_t)b * nm * 2 + mp) * half + jp] : make_double2(0.0, 0.0);
       },
       [&](int idx, double2 v) { stage[(idx % ppw) * SP + idx / ppw] = v; });
+#pragma unroll
+  for (int q = 0; q < TWI; ++q)
+    if (q * 256 + (int)threadIdx.x < L) tw[q * 256 + threadIdx.x] = twr[q];
   __syncthreads();
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -620,7 +643,6 @@ __device__ __forceinline__ uint3 wg_coords() {
 // LDS: X [64 cols][KC+1], P [64 rows][KC+1].
 template <int PREC>
 __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int ns, int nm, int half,
-                                                         const int* __restrict__ row_start,
                                                          int nmax, int zero_top) {
   using LT = typename LdsT<PREC>::T;
   using TT = typename TabT<PREC>::T;
@@ -652,7 +674,7 @@ __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int 
   const int nrg = min(TROWS, nr - kk0);
   if (nrg <= 0) return;  // uniform over the workgroup
   const int ntile = (nrg + 15) / 16;
-  const size_t rowbase = (size_t)(row_start[m] + nfirst + 2 * kk0 - m) * half;
+  const size_t rowbase = (size_t)(packed_row_start(m, ns) + nfirst + 2 * kk0 - m) * half;
   TileAcc acc;
 #pragma unroll
   for (int t = 0; t < 4; ++t)
@@ -799,7 +821,6 @@ template <int PREC>
 __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2* __restrict__ eo,
                                                          FwdTables tab, int B, int nmax, int ns,
                                                          int nm, int half,
-                                                         const int* __restrict__ row_start,
                                                          SpecOpTables ops, int out_mode) {
   using LT = typename LdsT<PREC>::T;
   using TT = typename TabT<PREC>::T;
@@ -820,7 +841,7 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 15, lk = lane >> 4;
-  const size_t rowbase = (size_t)(row_start[m] + nfirst - m) * half + lat0;
+  const size_t rowbase = (size_t)(packed_row_start(m, ns) + nfirst - m) * half + lat0;
   TileAcc acc;
 #pragma unroll
   for (int t = 0; t < 4; ++t)
@@ -885,9 +906,12 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
 }
 
 // ------------------------------------------------------------------ drivers
+#ifndef JGCM_PPW_MIN
+#define JGCM_PPW_MIN 8
+#endif
 static int pick_ppw(int L) {
   int fpw = 256 / (L / 4);
-  return fpw >= 8 ? fpw : 8;
+  return fpw >= JGCM_PPW_MIN ? fpw : JGCM_PPW_MIN;
 }
 
 template <int L>
@@ -980,19 +1004,19 @@ static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int 
   switch (M->precision) {
     case JGCM_PREC_FP64:
       hipLaunchKernelGGL(legendre_fwd_mfma<P_FP64>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
-                         M->nm, M->half, M->d_row_start, nmax, zero_top);
+                         M->nm, M->half, nmax, zero_top);
       break;
     case JGCM_PREC_FP32:
       hipLaunchKernelGGL(legendre_fwd_mfma<P_FP32>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
-                         M->nm, M->half, M->d_row_start, nmax, zero_top);
+                         M->nm, M->half, nmax, zero_top);
       break;
     case JGCM_PREC_BF16X3:
       hipLaunchKernelGGL(legendre_fwd_mfma<P_BF16>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
-                         M->nm, M->half, M->d_row_start, nmax, zero_top);
+                         M->nm, M->half, nmax, zero_top);
       break;
     default:
       hipLaunchKernelGGL(legendre_fwd_mfma<P_SPLIT>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
-                         M->nm, M->half, M->d_row_start, nmax, zero_top);
+                         M->nm, M->half, nmax, zero_top);
       break;
   }
   JGCM_LAUNCH_CHECK();
@@ -1010,22 +1034,22 @@ static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, 
   switch (M->precision) {
     case JGCM_PREC_FP64:
       hipLaunchKernelGGL(legendre_inv_mfma<P_FP64>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops,
+                         nmax, M->ns, M->nm, M->half, ops,
                          out_mode);
       break;
     case JGCM_PREC_FP32:
       hipLaunchKernelGGL(legendre_inv_mfma<P_FP32>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops,
+                         nmax, M->ns, M->nm, M->half, ops,
                          out_mode);
       break;
     case JGCM_PREC_BF16X3:
       hipLaunchKernelGGL(legendre_inv_mfma<P_BF16>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops,
+                         nmax, M->ns, M->nm, M->half, ops,
                          out_mode);
       break;
     default:
       hipLaunchKernelGGL(legendre_inv_mfma<P_SPLIT>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, M->d_row_start, ops,
+                         nmax, M->ns, M->nm, M->half, ops,
                          out_mode);
       break;
   }
