@@ -65,9 +65,13 @@ SIGNATURES = {
     "jgcm_ucos_vcos_adjoint": [P, P, P, P, P, I],
     "jgcm_uv_grid_from_vor_div_adjoint": [P, P, P, P, P, I],
     "jgcm_vorticity_flux_adjoint": [P, P, P, P, D, P, P, P, P, P, I],
-    "jgcm_damped_leapfrog_adjoint": [P, I, P, P, P, P, P, I],
+    "jgcm_damped_leapfrog_adjoint": [P, I, I, P, P, P, P, P, I],
     "jgcm_barotropic_step_adjoint": [P, I, P, P, P, D, P, P, I],
     "jgcm_sw_step_adjoint": [P, I, P, P, P, P, P, P, D, D, D, D, P, P, P, P, P, P, I],
+    "jgcm_hs_column_adjoint": [P, P, D, D, D, P, P, P, P],
+    "jgcm_hs_corrections_adjoint": [P, P, P, P, P, D, D, D, D, P, P, P, P],
+    "jgcm_implicit_adjoint": [P, I, P, P, P, P, P],
+    "jgcm_gradient_cos_adjoint": [P, P, P, P, I, I],
     "jgcm_divide_by_cos": [P, P, I],
     "jgcm_multiply_by_cos": [P, P, P, I],
     "jgcm_vor_div_from_grid_uv": [P, P, P, P, P, I],

@@ -18,6 +18,8 @@ forward (keeping its grid fields) and runs the C step adjoint
 (``jgcm_barotropic_step_adjoint``, ``jgcm_sw_step_adjoint``) backwards through
 it.  Device memory is O(nsteps / checkpoint_every + checkpoint_every) states.
 """
+import ctypes
+
 import torch
 
 from ._lib import call
@@ -121,15 +123,17 @@ def vorticity_flux_adjoint(mesh, grid_vor, grid_u, grid_v, omega, du_bar, dv_bar
          float(omega), _ptr(du_bar), _ptr(dv_bar), _ptr(vor_bar), _ptr(u_bar), _ptr(v_bar), n)
 
 
-def damped_leapfrog_adjoint(integrator, init_step, p_bar_next, c_bar_next, p_bar, c_bar, g_bar):
+def damped_leapfrog_adjoint(integrator, init_step, p_bar_next, c_bar_next, p_bar, c_bar, g_bar,
+                            damp=True):
     """Transpose of Compute_Spectral_Damping! + Filtered_Leapfrog! + Time_Advance!
-    (Time_Integrator.jl:58-101): (p_bar', c_bar') -> (p_bar, c_bar, g_bar)."""
+    (Time_Integrator.jl:58-101): (p_bar', c_bar') -> (p_bar, c_bar, g_bar);
+    ``damp=False`` for a field the step does not damp."""
     mesh = integrator.mesh
     n = mesh._nlev(p_bar_next, "spec", True)
     for t in (c_bar_next, p_bar, c_bar, g_bar):
         assert mesh._nlev(t, "spec", True) == n
-    call("jgcm_damped_leapfrog_adjoint", integrator.handle, 1 if init_step else 0, _ptr(p_bar_next),
-         _ptr(c_bar_next), _ptr(p_bar), _ptr(c_bar), _ptr(g_bar), n)
+    call("jgcm_damped_leapfrog_adjoint", integrator.handle, 1 if init_step else 0, 1 if damp else 0,
+         _ptr(p_bar_next), _ptr(c_bar_next), _ptr(p_bar), _ptr(c_bar), _ptr(g_bar), n)
 
 
 def sw_step_adjoint(integrator, init_step, grids, spe_h, omega, kappa_m, kappa_t, h_0, bars):
@@ -161,12 +165,12 @@ def barotropic_step_adjoint(integrator, init_step, grid_vor, grid_u, grid_v, ome
 
 
 class _Run_Adjoint:
-    """Checkpointed reverse sweep over a run of a one-level stepper whose
-    state is a tuple of spectral fields.  Subclasses name the state fields
-    (``STATE``, pairs of Dyn_Data attribute names in (p, c) order), rebuild the
-    grid fields from the spectral state (``_load_grids``), run one step
-    (``_step``), snapshot what the step adjoint needs (``_snapshot``) and call
-    the C step adjoint (``_step_adjoint``)."""
+    """Checkpointed reverse sweep over a run of a stepper whose state is a
+    tuple of Dyn_Data fields.  Subclasses name the state fields (``STATE``,
+    Dyn_Data attribute names), rebuild the derived grid fields from the state
+    (``_load_grids``), run one step (``_step``), snapshot what the step adjoint
+    needs before and after the step (``_snapshot``, ``_snapshot_after``) and
+    run the reverse step (``_step_adjoint``)."""
 
     STATE = ()
 
@@ -185,10 +189,13 @@ class _Run_Adjoint:
                                      integrator.start_time, integrator.end_time)
         self._first = clone(self.starts_init)
         self._later = clone(False)
-        self._d = Dyn_Data("adjoint", mesh.num_fourier, mesh.num_spherical, mesh.nλ, mesh.nθ, 1,
-                           mesh=mesh)
+        self._d = Dyn_Data("adjoint", mesh.num_fourier, mesh.num_spherical, mesh.nλ, mesh.nθ,
+                           self._levels(), mesh=mesh)
         self._checkpoints = []
         self.nsteps = 0
+
+    def _levels(self):
+        return 1
 
     def _integrator(self, k):
         return self._first if k == 0 else self._later
@@ -201,12 +208,18 @@ class _Run_Adjoint:
     def _state(self):
         return tuple(getattr(self._d, name).clone() for name in self.STATE)
 
-    def _run(self, state, nsteps):
-        assert nsteps >= 1
-        assert len(state) == len(self.STATE)
+    def _check_state(self, state):
         for t in state:
             self.mesh._nlev(t, "spec", True)
             assert t.shape[0] == 1, "the state has one level"
+
+    def _snapshot_after(self):
+        return None
+
+    def _run(self, state, nsteps):
+        assert nsteps >= 1
+        assert len(state) == len(self.STATE)
+        self._check_state(state)
         self._checkpoints = []
         self.nsteps = int(nsteps)
         self._load(state)
@@ -226,10 +239,13 @@ class _Run_Adjoint:
             self._load(self._checkpoints[s])
             snaps = []
             for k in range(k0, k1):
-                snaps.append(self._snapshot())
+                before = self._snapshot()
                 self._step(k)
+                snaps.append((before, self._snapshot_after()))
             for k in reversed(range(k0, k1)):
-                self._step_adjoint(k, self.starts_init and k == 0, snaps[k - k0], bars)
+                before, after = snaps[k - k0]
+                snap = before if after is None else (before, after)
+                self._step_adjoint(k, self.starts_init and k == 0, snap, bars)
         self.mesh.synchronize()
         return bars
 
@@ -336,3 +352,252 @@ class Shallow_Water_Adjoint(_Run_Adjoint):
 
     def backward(self, bars):
         return self._reverse(tuple(bars))
+
+
+class Held_Suarez_Adjoint(_Run_Adjoint):
+    """Gradient of a function of the final state of an ``nsteps`` Held-Suarez
+    run (Atmosphere_Update! each step, exp/HSt42) with respect to its initial
+    state: the spectral (p, c) pairs of vor, div, T and ln ps and the grid
+    p-fields u_p, v_p, T_p, ps_p that the forcing and Compute_Corrections_Init
+    read (the previous step's unfiltered grid state).  The grid c-fields are
+    rebuilt from the spectral c-fields (inverse transforms, u and v by
+    UV_Grid_From_Vor_Div!, ps = exp(ln ps)), as every step leaves them.
+
+        adj = Held_Suarez_Adjoint(mesh, atmo, vert_coord, semi_implicit, hs_params)
+        final = adj.forward(state, nsteps)    # state: tensors in Held_Suarez_Adjoint.STATE order
+        grads = adj.backward(final_bars)      # same order
+
+    Needs the even-sigma coordinate (the library's only one).  The reverse
+    step composes C entry points: the leapfrog transposes, the implicit
+    correction's transpose from its per-n matrix (built once per solver by
+    applying the forward to unit vectors), the transposed transforms and
+    ``jgcm_hs_column_adjoint`` / ``jgcm_hs_corrections_adjoint`` for the grid
+    columns and the global corrections (whose scalar algebra runs on the
+    host from the seven correction scalars the forward step leaves).
+    """
+
+    STATE = ("spe_vor_p", "spe_vor_c", "spe_div_p", "spe_div_c", "spe_t_p", "spe_t_c",
+             "spe_lnps_p", "spe_lnps_c", "grid_u_p", "grid_v_p", "grid_t_p", "grid_ps_p")
+
+    def __init__(self, mesh, atmo_data, vert_coord, semi_implicit, physics_params,
+                 checkpoint_every=4):
+        from .primitive import Get_Topography, Semi_Implicit_Solver
+        self._nd = vert_coord.nd
+        super().__init__(mesh, atmo_data, semi_implicit.integrator, checkpoint_every)
+        mass, energy = bool(atmo_data.do_mass_correction), bool(atmo_data.do_energy_correction)
+        assert mass == energy, "the adjoint supports both corrections or neither"
+        self._corrections = mass
+        self.vc = vert_coord
+        self.params = dict(physics_params)
+        prev = getattr(mesh, "_semi_implicit", None)
+
+        def solver(I):
+            return Semi_Implicit_Solver(vert_coord, atmo_data, I, semi_implicit.ps_ref,
+                                        semi_implicit.t_ref, semi_implicit.wave_numbers)
+        self._si_first, self._si_later = solver(self._first), solver(self._later)
+        mesh._semi_implicit = prev
+        Get_Topography(self._d.grid_geopots)
+        self._matrices = {}
+        self._ws = None
+
+    def _levels(self):
+        return self._nd
+
+    def _solver(self, k):
+        return self._si_first if k == 0 else self._si_later
+
+    # ---- forward -------------------------------------------------------------
+    def _check_state(self, state):
+        M, nd = self.mesh, self._nd
+        for name, t in zip(self.STATE, state):
+            kind = "spec" if name.startswith("spe_") else "grid"
+            lev = 1 if name in ("spe_lnps_p", "spe_lnps_c", "grid_ps_p") else nd
+            assert M._nlev(t, kind, True) == lev, f"{name}: {lev} levels expected"
+
+    def _load_grids(self):
+        from .mesh import Trans_Spherical_To_Grid, UV_Grid_From_Vor_Div
+        from .primitive import Compute_Pressures_And_Heights
+        d, M = self._d, self.mesh
+        Trans_Spherical_To_Grid(M, d.spe_vor_c, d.grid_vor, batch=True)
+        Trans_Spherical_To_Grid(M, d.spe_div_c, d.grid_div, batch=True)
+        UV_Grid_From_Vor_Div(M, d.spe_vor_c, d.spe_div_c, d.grid_u_c, d.grid_v_c, batch=True)
+        Trans_Spherical_To_Grid(M, d.spe_t_c, d.grid_t_c, batch=True)
+        Trans_Spherical_To_Grid(M, d.spe_lnps_c, d.grid_lnps, batch=True)
+        call("jgcm_map", M.handle, 0, d.grid_lnps.numel(), _ptr(d.grid_lnps), 0.0, 0.0,
+             _ptr(d.grid_ps_c))
+        Compute_Pressures_And_Heights(self.atmo, self.vc, d.grid_ps_c, d.grid_geopots, d.grid_t_c,
+                                      d.grid_p_half, d.grid_Δp, d.grid_lnp_half, d.grid_p_full,
+                                      d.grid_lnp_full, d.grid_z_full, d.grid_z_half)
+
+    def _step(self, k):
+        from .primitive import Atmosphere_Update
+        self._last = self._solver(k)
+        Atmosphere_Update(self.mesh, self.atmo, self.vc, self._last, self._d, self.params)
+
+    def _snapshot(self):
+        return self._state()
+
+    def _snapshot_after(self):
+        d = self._d
+        return (self._last.scalars(),
+                tuple(t.clone() for t in (d.grid_u_c, d.grid_v_c, d.grid_t_c, d.grid_ps_c)))
+
+    def forward(self, state, nsteps):
+        return self._run(tuple(state), nsteps)
+
+    def backward(self, bars):
+        return self._reverse(tuple(bars))
+
+    # ---- reverse ---------------------------------------------------------------
+    def _implicit_matrix(self, si):
+        """[ns+1][2nd+1][7nd+3] matrix of ∇²(energy) + Implicit_Correction! per n,
+        column i = the forward applied to the i-th unit input at m = 0."""
+        key = id(si)
+        if key in self._matrices:
+            return self._matrices[key]
+        M, nd = self.mesh, self._nd
+        names = [("ddiv", nd), ("dlnps", 1), ("dt", nd), ("energy", nd), ("div_c", nd),
+                 ("div_p", nd), ("lnps_c", 1), ("lnps_p", 1), ("t_c", nd), ("t_p", nd)]
+        buf = {n: M.spectral_zeros(L) for n, L in names}
+        nin, nout = 7 * nd + 3, 2 * nd + 1
+        A = torch.zeros((M.num_spherical + 1, nout, nin), dtype=torch.float64, device=M.device)
+        i = 0
+        for name, L in names:
+            for lev in range(L):
+                for t in buf.values():
+                    t.zero_()
+                buf[name][lev, :, 0] = 1.0
+                call("jgcm_hs_energy_implicit_correction", si.handle, *map(_ptr, (
+                    buf["energy"], buf["div_c"], buf["div_p"], buf["lnps_c"], buf["lnps_p"],
+                    buf["t_c"], buf["t_p"], buf["ddiv"], buf["dlnps"], buf["dt"])))
+                M.synchronize()
+                A[:, 0:nd, i] = buf["ddiv"][:, :, 0].real.T
+                A[:, nd, i] = buf["dlnps"][0, :, 0].real
+                A[:, nd + 1:, i] = buf["dt"][:, :, 0].real.T
+                i += 1
+        A = A.contiguous()
+        self._matrices[key] = A
+        return A
+
+    def _step_adjoint(self, k, init, snap, bars):
+        from .mesh import (Divide_By_Cos, Trans_Spherical_To_Grid, Trans_Spherical_To_Grid_Batch,
+                           UV_Grid_From_Vor_Div, BATCH_DIVCOS, INV_GRAD_LAMBDA, INV_GRAD_THETA)
+        from .primitive import _hs
+        from .time_integrator import Get_Δt
+        M, nd, atmo = self.mesh, self._nd, self.atmo
+        h = M.handle
+        st, (sc, nxt) = snap
+        vor_p, vor_c, div_p, div_c, t_p, t_c, lnps_p, lnps_c, u_pg, v_pg, t_pg, ps_pg = st
+        Bvp, Bvc, Bdp, Bdc, Btp, Btc, Blp, Blc, Bup, Bvpg, Btpg, Bpsp = bars
+        si, I = self._solver(k), self._integrator(k)
+        g, cp = atmo.grav, atmo.cp_air
+        S, G = M.spectral_zeros, M.grid_zeros
+
+        def add(dst, src):          # dst += src (same shape), on the mesh stream
+            call("jgcm_axpby", h, dst.numel() * dst.element_size() // 8, 1.0, _ptr(dst), 1.0,
+                 _ptr(src), _ptr(dst))
+
+        def t2g_T(grid, nlev):       # transpose of the inverse transform
+            out = S(nlev)
+            trans_spherical_to_grid_adjoint(M, grid, out)
+            return out
+
+        def f_T(spec, nlev):         # transpose of the forward transform
+            out = G(nlev)
+            trans_grid_to_spherical_adjoint(M, spec, out)
+            return out
+
+        # 1. mass and energy corrections (Spectral_Dynamics.jl:30-68)
+        mps_p, E_p, mps_n, f, _, E_n, corr = (float(x) for x in sc)
+        u_n, v_n, t_nc, ps_nc = nxt
+        Ep_b = mpsp_b = 0.0
+        if self._corrections:
+            corr_b = float(Btc[:, 0, 0].real.cpu().numpy().sum())
+            f_b = float(Blc[0, 0, 0].real.cpu()) / f
+            En_b = -corr_b * g / (cp * mps_p)
+            Ep_b = corr_b * g / (cp * mps_p)
+            mpsp_b = -corr_b * corr / mps_p
+            f_b += En_b * E_n / f
+            mpsp_b += f_b / mps_n
+            mpsn_b = -f_b * f / mps_n
+            ub, vb, tb, lb = G(nd), G(nd), G(nd), G(1)
+            call("jgcm_hs_corrections_adjoint", si.handle, *map(_ptr, (u_n, v_n, t_nc, ps_nc)), corr,
+                 f, En_b, mpsn_b, *map(_ptr, (ub, vb, tb, lb)))
+            add(Btc, t2g_T(tb, nd))
+            add(Blc, t2g_T(lb, 1))
+            sv, sd = S(nd), S(nd)
+            uv_grid_from_vor_div_adjoint(M, ub, vb, sv, sd)
+            add(Bvc, sv)
+            add(Bdc, sd)
+        # 2. damping + leapfrog + Time_Advance (vor, div, T damped; ln ps not)
+        Gv, Gd, Gt, Gl = S(nd), S(nd), S(nd), S(1)
+        damped_leapfrog_adjoint(I, init, Bvp, Bvc, Bvp, Bvc, Gv)
+        damped_leapfrog_adjoint(I, init, Bdp, Bdc, Bdp, Bdc, Gd)
+        damped_leapfrog_adjoint(I, init, Btp, Btc, Btp, Btc, Gt)
+        damped_leapfrog_adjoint(I, init, Blp, Blc, Blp, Blc, Gl, damp=False)
+        # 3. ∇²(energy), δdiv -= energy, Implicit_Correction!
+        out = S(7 * nd + 3)
+        call("jgcm_implicit_adjoint", h, nd, _ptr(self._implicit_matrix(si)), _ptr(Gd), _ptr(Gl),
+             _ptr(Gt), _ptr(out))
+        o_ddiv, o_dl, o_dt, o_en = out[0:nd], out[nd:nd + 1], out[nd + 1:2 * nd + 1], out[2 * nd + 1:3 * nd + 1]
+        base = 3 * nd + 1
+        add(Bdc, out[base:base + nd])
+        add(Bdp, out[base + nd:base + 2 * nd])
+        add(Blc, out[base + 2 * nd:base + 2 * nd + 1])
+        add(Blp, out[base + 2 * nd + 1:base + 2 * nd + 2])
+        add(Btc, out[base + 2 * nd + 2:base + 3 * nd + 2])
+        add(Btp, out[base + 3 * nd + 2:base + 4 * nd + 2])
+        # 4. forward transforms of δln ps, δT, Φ+KE and (δu, δv) -> (δζ, δD)
+        dl_g = f_T(o_dl.contiguous(), 1)
+        dt_g = f_T(o_dt.contiguous(), nd)
+        en_g = f_T(o_en.contiguous(), nd)
+        du_g, dv_g = G(nd), G(nd)
+        vor_div_from_grid_uv_adjoint(M, Gv, o_ddiv.contiguous(), du_g, dv_g)
+        # 5. grid columns at the step's state
+        vor_g, div_g, u_g, v_g, t_g, ln_g, ps_c = G(nd), G(nd), G(nd), G(nd), G(nd), G(1), G(1)
+        Trans_Spherical_To_Grid(M, vor_c, vor_g, batch=True)
+        Trans_Spherical_To_Grid(M, div_c, div_g, batch=True)
+        UV_Grid_From_Vor_Div(M, vor_c, div_c, u_g, v_g, batch=True)
+        Trans_Spherical_To_Grid(M, t_c, t_g, batch=True)
+        Trans_Spherical_To_Grid(M, lnps_c, ln_g, batch=True)
+        call("jgcm_map", h, 0, ln_g.numel(), _ptr(ln_g), 0.0, 0.0, _ptr(ps_c))
+        gl, gt, tl, tt = G(1), G(1), G(nd), G(nd)
+        D = BATCH_DIVCOS
+        Trans_Spherical_To_Grid_Batch(M, [lnps_c, lnps_c, t_c, t_c], [gl, gt, tl, tt],
+                                      [D | INV_GRAD_LAMBDA, D | INV_GRAD_THETA,
+                                       D | INV_GRAD_LAMBDA, D | INV_GRAD_THETA])
+        outs = [G(nd), G(nd), G(nd), G(nd), G(nd), G(nd), G(nd), G(1), G(1), G(nd), G(nd), G(nd),
+                G(1), G(1)]
+        div_b, u_b, v_b, t_b, vor_b, tl_b, tt_b, gl_b, gt_b, up_b, vp_b, tp_b, psp_b, psc_b = outs
+        if self._ws is None or self._ws.numel() < 4 * nd * M.nθ * M.nλ:
+            self._ws = torch.zeros(4 * nd * M.nθ * M.nλ, dtype=torch.float64, device=M.device)
+        arr = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+        hp = _hs(self.params)
+        call("jgcm_hs_column_adjoint", si.handle, ctypes.byref(hp), float(Get_Δt(I)), Ep_b, mpsp_b,
+             arr([div_g, u_g, v_g, t_g, vor_g, tl, tt, gl, gt, u_pg, v_pg, t_pg, ps_pg, ps_c]),
+             arr([du_g, dv_g, dt_g, en_g, dl_g]), arr(outs), _ptr(self._ws))
+        # 6. the previous cotangents of the grid p-fields belong to this step's grid c-fields
+        add(u_b, Bup)
+        add(v_b, Bvpg)
+        add(t_b, Btpg)
+        add(psc_b, Bpsp)
+        for dst, src in ((Bup, up_b), (Bvpg, vp_b), (Btpg, tp_b), (Bpsp, psp_b)):
+            call("jgcm_copy", h, _ptr(dst), _ptr(src), src.numel() * src.element_size())
+        # 7. grid c-fields -> spectral c-fields
+        add(Bvc, t2g_T(vor_b, nd))
+        add(Bdc, t2g_T(div_b, nd))
+        add(Btc, t2g_T(t_b, nd))
+        sv, sd = S(nd), S(nd)
+        uv_grid_from_vor_div_adjoint(M, u_b, v_b, sv, sd)
+        add(Bvc, sv)
+        add(Bdc, sd)
+        lg = G(1)
+        call("jgcm_xmy", h, lg.numel(), 1.0, _ptr(ps_c), _ptr(psc_b), _ptr(lg), 0)
+        add(Blc, t2g_T(lg, 1))
+        for gb_l, gb_t, target, nlev in ((gl_b, gt_b, Blc, 1), (tl_b, tt_b, Btc, nd)):
+            Divide_By_Cos(M, gb_l)
+            Divide_By_Cos(M, gb_t)
+            # both temporaries stay referenced until the call: a freed one could
+            # be handed to the other by the caching allocator
+            s_l, s_t = t2g_T(gb_l, nlev), t2g_T(gb_t, nlev)
+            call("jgcm_gradient_cos_adjoint", h, _ptr(s_l), _ptr(s_t), _ptr(target), 1, nlev)

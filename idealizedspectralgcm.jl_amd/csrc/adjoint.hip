@@ -120,7 +120,7 @@ __global__ void damped_leapfrog_adjoint_kernel(const double2* pb_next, const dou
   GRID_STRIDE(i, n_all) {
     int m = (int)(i % nm);
     int k = (int)((i / nm) % (ns + 1));
-    double d = damping[m * (ns + 1) + k];
+    double d = damping ? damping[m * (ns + 1) + k] : 0.0;
     double2 P = pb_next[i], C = cb_next[i];
     if (init_step) {
       double den = 1.0 + dt * d;
@@ -373,9 +373,9 @@ int jgcm_vorticity_flux_adjoint(jgcm_mesh_t* mesh, const double* grid_vor, const
   return JGCM_OK;
 }
 
-int jgcm_damped_leapfrog_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* p_bar_next,
-                                 const double* c_bar_next, double* p_bar, double* c_bar,
-                                 double* g_bar, int nlev) {
+int jgcm_damped_leapfrog_adjoint(jgcm_leapfrog_t* lf, int init_step, int damp,
+                                 const double* p_bar_next, const double* c_bar_next, double* p_bar,
+                                 double* c_bar, double* g_bar, int nlev) {
   JGCM_REQUIRE(lf != nullptr && lf->mesh != nullptr, "leapfrog is NULL");
   JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
   JGCM_REQUIRE(p_bar_next && c_bar_next && p_bar && c_bar && g_bar, "NULL field pointer");
@@ -383,7 +383,7 @@ int jgcm_damped_leapfrog_adjoint(jgcm_leapfrog_t* lf, int init_step, const doubl
   size_t n = (size_t)nlev * M->spec_elems();
   hipLaunchKernelGGL(damped_leapfrog_adjoint_kernel, dim3(nblk(n)), dim3(256), 0, M->stream,
                      (const double2*)p_bar_next, (const double2*)c_bar_next, (double2*)p_bar,
-                     (double2*)c_bar, (double2*)g_bar, lf->d_damping, M->nm, M->ns, n,
+                     (double2*)c_bar, (double2*)g_bar, damp ? lf->d_damping : nullptr, M->nm, M->ns, n,
                      init_step ? 1 : 0, (double)lf->dt, lf->robert_coef);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
@@ -414,7 +414,7 @@ int jgcm_barotropic_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const doubl
   double* ga0 = M->d_grid_a;
   double* ga1 = M->d_grid_a + G;
   double* gb0 = M->d_grid_b;
-  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, p_bar, c_bar, p_bar, c_bar, gbar, nlev))
+  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, 1, p_bar, c_bar, p_bar, c_bar, gbar, nlev))
     return rc;
   if (int rc = vor_div_adjoint(M, reinterpret_cast<const double2*>(gbar), nullptr, ga0, ga1, nlev))
     return rc;
@@ -464,11 +464,11 @@ int jgcm_sw_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* grid_
   double* Gb[4] = {M->d_grid_b, M->d_grid_b + G, M->d_grid_b + 2 * G, M->d_grid_b + 3 * G};
   const size_t ns_all = (size_t)nlev * M->spec_elems();
   // damping + leapfrog + Time_Advance: tendency cotangents Gv, Gd, Gh -> S_a0..2
-  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, vor_p_bar, vor_c_bar, vor_p_bar,
+  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, 1, vor_p_bar, vor_c_bar, vor_p_bar,
                                             vor_c_bar, Sa[0], nlev)) return rc;
-  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, div_p_bar, div_c_bar, div_p_bar,
+  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, 1, div_p_bar, div_c_bar, div_p_bar,
                                             div_c_bar, Sa[1], nlev)) return rc;
-  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, h_p_bar, h_c_bar, h_p_bar, h_c_bar,
+  if (int rc = jgcm_damped_leapfrog_adjoint(lf, init_step, 1, h_p_bar, h_c_bar, h_p_bar, h_c_bar,
                                             Sa[2], nlev)) return rc;
   // semi-implicit correction and the energy term; E_bar -> S_a3
   const double mu = lf->implicit_coef * (init_step ? (double)lf->dt : 2.0 * (double)lf->dt);

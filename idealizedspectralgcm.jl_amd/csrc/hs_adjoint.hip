@@ -1,0 +1,420 @@
+// Reverse-mode pieces of the Held-Suarez step (Atmosphere_Update!,
+// Spectral_Dynamics.jl:206-490) that have no counterpart among the transform
+// and leapfrog adjoints:
+//
+//   hs_column_adjoint_kernel   transpose of the grid-column part of the step
+//                              linearised at the step's state: HS forcing on
+//                              the p-fields (HS_Forcing.jl), the energy of
+//                              Compute_Corrections_Init (SD:5-28), Four_In_One!
+//                              (SD:88-183), the hydrostatic integral, δln ps,
+//                              vertical advection (Vert_Coordinate.jl:171-210),
+//                              horizontal advection of T, the vorticity flux
+//                              and Φ + KE (SD:259-307)
+//   hs_corrections_adjoint_kernel  transpose of the mass and energy
+//                              corrections (SD:30-68) with respect to the
+//                              new-state grid fields
+//   implicit_adjoint_kernel    transpose of ∇²(energy), δdiv -= and
+//                              Implicit_Correction! (Semi_Implicit.jl:462-513)
+//                              per (m, n) from the operator's matrix, which the
+//                              host builds by applying the forward to unit
+//                              vectors (as Build_Implicit_Matrices does for
+//                              its operators, Semi_Implicit.jl:112-160)
+//   gradient_cos_adjoint_kernel transpose of Compute_Gradient_Cos! (SSM:677-687)
+//
+// The vertical coordinate must be even sigma with a zero top (the only
+// coordinate the library builds): every pressure is then b_k ps, the
+// ln-pressure differences are constants, and ps cancels out of the column
+// dynamics, which leaves ps in the forcing's T_eq and in the corrections.
+#pragma clang fp contract(off)
+
+#include <cmath>
+
+#include "primitive_internal.hpp"
+
+namespace jgcm {
+
+static inline int nblk_h(size_t n) {
+  size_t b = (n + 255) / 256;
+  return (int)(b > 8192 ? 8192 : (b == 0 ? 1 : b));
+}
+
+#define HS_GRID_STRIDE(i, n) \
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
+
+// Per-level constants of an even-sigma, zero-top coordinate (Pressure_Variables!,
+// Press_And_Geopot.jl:36-67, at ps = 1; differences of ln p do not depend on ps):
+//   Dp = ln p_half[k+1] - ln p_full[k],  Dl = ln p_half[k+1] - ln p_half[k] (k >= 1),
+//   C1 = (b_k (ln p_full - ln p_half[k]) + b_{k+1} Dp) / Δb_k  (Four_In_One!'s x1 times ps),
+//   sig = p_full[k] / ps.
+struct LevelConst {
+  double Dp, Dl, C1, sig;
+};
+
+__device__ inline LevelConst level_const(const VC& v, int k) {
+  LevelConst c;
+  double lh1 = log(v.bk[k + 1]);
+  double lf;
+  if (k == 0) {
+    lf = lh1 - 1.0;
+    c.Dl = 0.0;
+    c.Dp = lh1 - lf;
+    c.C1 = (v.bk[1] * c.Dp) / v.dbk[0];
+  } else {
+    double lh0 = log(v.bk[k]);
+    lf = lh1 + v.bk[k] * (lh1 - lh0) / v.dbk[k] - 1.0;
+    c.Dl = lh1 - lh0;
+    c.Dp = lh1 - lf;
+    c.C1 = (v.bk[k] * (lf - lh0) + v.bk[k + 1] * c.Dp) / v.dbk[k];
+  }
+  c.sig = exp(lf);
+  return c;
+}
+
+// weight of r_h in the flux at half level h (second_centered_wts: Δb_{h-1}/(Δb_{h-1}+Δb_h))
+__device__ inline double vadv_alpha(const VC& v, int h) {
+  return v.scheme == 0 ? v.dbk[h - 1] / (v.dbk[h - 1] + v.dbk[h]) : 0.5;
+}
+
+struct HsColAdj {
+  // state: c-fields [nd][P], grid gradients of T [nd][P] and of ln ps [P], p-fields
+  const double *div, *u, *v, *t, *vor, *tl, *tt, *gl, *gt;
+  const double *u_p, *v_p, *t_p, *ps_p, *ps_c;
+  // cotangents of the grid tendencies δu, δv, δT [nd][P], of Φ+KE [nd][P] and δln ps [P]
+  const double *du_b, *dv_b, *dt_b, *e_b, *dlnps_b;
+  // cotangents out
+  double *div_b, *u_b, *v_b, *t_b, *vor_b, *tl_b, *tt_b, *gl_b, *gt_b;
+  double *up_b, *vp_b, *tp_b, *psp_b, *psc_b;
+  double* ws;  // [4][nd][P]: Q, S (forward), S_bar, partial Q_bar
+};
+
+struct HsAdjScalars {
+  double rdgas, cp_air, kappa, grav, two_omega, dt, ep_bar, mpsp_bar;
+  jgcm_hs_params hp;
+};
+
+// One thread per column.  Forward quantities (ps factored out, even sigma):
+//   Q_k = D_k + u_k gλ + v_k gθ,  S_k = Σ_{j<k} Δb_j Q_j,  S = Σ_j Δb_j Q_j
+//   x5_k = -(Dl_k/Δb_k) S_k - Dp_k Q_k + C1_k (u_k gλ + v_k gθ);  δT_k += κ T_k x5_k
+//   δu_k -= R T_k C1_k gλ,  δv_k -= R T_k C1_k gθ,  δln ps = -S
+//   m_h = -S_h + b_h S (h = 1..nd-1; M_half = ps m),  vertical advection
+//   V(r)_k = (f_k - f_{k+1} + r_k (m_{k+1} - m_k)) / Δb_k, f_h = m_h ((1-α_h) r_{h-1} + α_h r_h)
+//   δT_k -= u_k Tλ_k + v_k Tθ_k,  δu_k += (ζ_k + f) v_k,  δv_k -= (ζ_k + f) u_k
+//   E_k = R Σ_{j>k} Dl_j T_j + R Dp_k T_k + KE_k
+// plus the forcing on the p-fields and the energy of Compute_Corrections_Init.
+__global__ void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat, int nlon, const double* sint,
+                                         const double* wts, HsAdjScalars c) {
+  const size_t P = (size_t)nlat * nlon;
+  const int nd = v.nd;
+  double* wQ = s.ws;
+  double* wS = s.ws + (size_t)nd * P;
+  double* wSb = s.ws + (size_t)2 * nd * P;
+  double* wQb = s.ws + (size_t)3 * nd * P;
+  const double R = c.rdgas, kap = c.kappa, cp = c.cp_air, dt = c.dt;
+  HS_GRID_STRIDE(col, P) {
+    const int j = (int)(col / nlon);
+    const double f = c.two_omega * sint[j];
+    const double gl = s.gl[col], gt = s.gt[col];
+    const double aj = wts[j] / (2.0 * nlon);
+    // pass A: Q_k, S_k, S
+    double Ssum = 0.0;
+    for (int k = 0; k < nd; ++k) {
+      size_t o = (size_t)k * P + col;
+      double q = s.div[o] + s.u[o] * gl + s.v[o] * gt;
+      wQ[o] = q;
+      wS[o] = Ssum;
+      Ssum = Ssum + v.dbk[k] * q;
+    }
+    const double Stot = Ssum;
+    auto m_at = [&](int h) -> double {
+      return (h >= 1 && h <= nd - 1) ? -wS[(size_t)h * P + col] + v.bk[h] * Stot : 0.0;
+    };
+    // physics constants of this latitude
+    const double day = 86400.0, t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
+    const double kf = c.hp.k_f / day, ka = c.hp.k_a / day, ks = c.hp.k_s / day;
+    const double sj = sint[j], s2 = sj * sj, c2 = 1.0 - s2, c4 = c2 * c2;
+    const double psc = s.ps_c[col], psp = s.ps_p[col];
+    const double lb = -s.dlnps_b[col];  // S_bar from δln ps = -S
+    double Stot_b = lb;
+    double glb = 0.0, gtb = 0.0, pscb = 0.0, pspb = 0.0;
+    double eprefix = 0.0;
+    // pass B: level-local terms, m_bar, S_bar, first part of Q_bar
+    for (int k = 0; k < nd; ++k) {
+      size_t o = (size_t)k * P + col;
+      const LevelConst L = level_const(v, k);
+      const double db = v.dbk[k];
+      const double u = s.u[o], vv = s.v[o], t = s.t[o], vor = s.vor[o];
+      const double a = s.du_b[o], b = s.dv_b[o], cc = s.dt_b[o], e = s.e_b[o];
+      const double mk = m_at(k), mk1 = m_at(k + 1);
+      // vertical advection of u, v, T: contributions to r_bar at level k
+      const double bu = a / db, bv = b / db, bt = cc / db;
+      double vbu = bu * (mk1 - mk), vbv = bv * (mk1 - mk), vbt = bt * (mk1 - mk);
+      double mb = 0.0;  // m_bar at half level k
+      if (k >= 1) {
+        size_t om = o - P;
+        const double dbm = v.dbk[k - 1];
+        const double bum = s.du_b[om] / dbm, bvm = s.dv_b[om] / dbm, btm = s.dt_b[om] / dbm;
+        const double al = vadv_alpha(v, k);
+        const double um = s.u[om], vm = s.v[om], tm = s.t[om];
+        vbu = vbu + (bu - bum) * mk * al;
+        vbv = vbv + (bv - bvm) * mk * al;
+        vbt = vbt + (bt - btm) * mk * al;
+        mb = mb + (bum * um - bu * u + (bu - bum) * ((1.0 - al) * um + al * u));
+        mb = mb + (bvm * vm - bv * vv + (bv - bvm) * ((1.0 - al) * vm + al * vv));
+        mb = mb + (btm * tm - bt * t + (bt - btm) * ((1.0 - al) * tm + al * t));
+      }
+      if (k + 1 <= nd - 1) {
+        size_t op = o + P;
+        const double dbp = v.dbk[k + 1];
+        const double bup = s.du_b[op] / dbp, bvp = s.dv_b[op] / dbp, btp = s.dt_b[op] / dbp;
+        const double al1 = vadv_alpha(v, k + 1);
+        vbu = vbu + (bup - bu) * mk1 * (1.0 - al1);
+        vbv = vbv + (bvp - bv) * mk1 * (1.0 - al1);
+        vbt = vbt + (btp - bt) * mk1 * (1.0 - al1);
+      }
+      // energy conversion κ T x5
+      const double Sk = wS[o], Qk = wQ[o];
+      const double uvg = u * gl + vv * gt;
+      const double x5 = (k >= 1 ? -(L.Dl / db) * Sk : 0.0) - L.Dp * Qk + L.C1 * uvg;
+      const double xb = kap * t * cc;
+      double Sb = 0.0;
+      if (k >= 1) {
+        Sb = -mb - (L.Dl / db) * xb;
+        Stot_b = Stot_b + v.bk[k] * mb;
+      }
+      wSb[o] = Sb;
+      wQb[o] = -L.Dp * xb;
+      // local cotangents
+      s.vor_b[o] = vv * a - u * b;
+      s.u_b[o] = -((vor + f) * b) - s.tl[o] * cc + u * e + L.C1 * gl * xb + vbu;
+      s.v_b[o] = (vor + f) * a - s.tt[o] * cc + vv * e + L.C1 * gt * xb + vbv;
+      double tb = R * L.Dp * e - R * L.C1 * (gl * a + gt * b) + kap * x5 * cc + vbt;
+      if (k >= 1) tb = tb + R * L.Dl * eprefix;
+      s.t_b[o] = tb;
+      s.tl_b[o] = -(u * cc);
+      s.tt_b[o] = -(vv * cc);
+      glb = glb - R * L.C1 * t * a + L.C1 * u * xb;
+      gtb = gtb - R * L.C1 * t * b + L.C1 * vv * xb;
+      eprefix = eprefix + e;
+      // Held-Suarez forcing on the p-fields and the energy of Compute_Corrections_Init
+      const double sg = L.sig;
+      const double kv = kf / (1.0 - c.hp.sigma_b) * fmax(0.0, sg - c.hp.sigma_b);
+      const double kt = ka + (ks - ka) / (1.0 - c.hp.sigma_b) * fmax(0.0, sg - c.hp.sigma_b) * c4;
+      const double U = s.u_p[o], V = s.v_p[o], T = s.t_p[o];
+      const double pn = sg * psc / p_ref;
+      const double A0 = t_zero - c.hp.delta_T_y * s2 - c.hp.delta_theta_z * c2 * log(pn);
+      const double te = A0 * pow(pn, kap);
+      const double teq = fmax(t_strat, te);
+      const double g2 = kv * (1.0 - 0.5 * kv * dt);
+      const double DT = g2 * (U * U + V * V) / cp - kt * (T - teq);
+      const double uu = U + dt * (-kv * U), vq = V + dt * (-kv * V);
+      const double eb = c.ep_bar / c.grav * aj * psp * db;
+      const double ap = a + eb * dt * uu, bp = b + eb * dt * vq, cpb = cc + cp * dt * eb;
+      s.up_b[o] = eb * uu - kv * ap + 2.0 * g2 * U * cpb / cp;
+      s.vp_b[o] = eb * vq - kv * bp + 2.0 * g2 * V * cpb / cp;
+      s.tp_b[o] = cp * eb - kt * cpb;
+      if (te > t_strat) {
+        const double dte = pow(pn, kap - 1.0) * (kap * A0 - c.hp.delta_theta_z * c2) * sg / p_ref;
+        pscb = pscb + kt * cpb * dte;
+      }
+      const double ep = 0.5 * (uu * uu + vq * vq) + cp * (T + dt * DT);
+      pspb = pspb + c.ep_bar / c.grav * aj * db * ep;
+    }
+    // pass C (bottom up): Q_bar = partial + Δb_k (Σ_{k'>k} S_bar + S_bar_tot)
+    double suffix = 0.0;
+    for (int k = nd - 1; k >= 0; --k) {
+      size_t o = (size_t)k * P + col;
+      const double qb = wQb[o] + v.dbk[k] * (suffix + Stot_b);
+      const double u = s.u[o], vv = s.v[o];
+      s.div_b[o] = qb;
+      s.u_b[o] = s.u_b[o] + gl * qb;
+      s.v_b[o] = s.v_b[o] + gt * qb;
+      glb = glb + u * qb;
+      gtb = gtb + vv * qb;
+      suffix = suffix + wSb[o];
+    }
+    s.gl_b[col] = glb;
+    s.gt_b[col] = gtb;
+    s.psc_b[col] = pscb;
+    s.psp_b[col] = pspb + c.mpsp_bar * aj;
+  }
+}
+
+// Transpose of the corrections (SD:30-68) with respect to the new-state grid
+// fields u_n, v_n, T_n (before its energy correction) and ps_n (before its mass
+// correction), given the cotangents E_n_bar of the energy integral and
+// mps_n_bar of the mean surface pressure.  Inputs are the corrected fields
+// (T_n' = T_n + corr, ps_n' = f ps_n).  Output ln ps cotangent = ps_n ps_n_bar
+// (ps_n = exp(ln ps grid)).
+__global__ void hs_corrections_adjoint_kernel(VC v, int nlat, int nlon, const double* wts,
+                                              const double* u_n, const double* v_n,
+                                              const double* t_nc, const double* ps_nc, double corr,
+                                              double f, double en_bar, double mpsn_bar, double grav,
+                                              double cp, double* un_b, double* vn_b, double* tn_b,
+                                              double* lnps_b) {
+  const size_t P = (size_t)nlat * nlon;
+  const int nd = v.nd;
+  HS_GRID_STRIDE(col, P) {
+    const int j = (int)(col / nlon);
+    const double aj = wts[j] / (2.0 * nlon);
+    const double psc = ps_nc[col];
+    const double psn = psc / f;
+    double acc = 0.0;
+    for (int k = 0; k < nd; ++k) {
+      size_t o = (size_t)k * P + col;
+      const double u = u_n[o], vv = v_n[o], t = t_nc[o] - corr;
+      const double e = 0.5 * (u * u + vv * vv) + cp * t;
+      acc = acc + v.dbk[k] * e;
+      const double eb = en_bar / grav * aj * psc * v.dbk[k];
+      un_b[o] = eb * u;
+      vn_b[o] = eb * vv;
+      tn_b[o] = eb * cp;
+    }
+    const double psb = en_bar * f * aj * acc / grav + mpsn_bar * aj;
+    lnps_b[col] = psn * psb;
+  }
+}
+
+// out[i][n][m] = Σ_o A_n[o][i] in[o][n][m] for n >= m (zero below the
+// diagonal), A_n the real (2nd+1) x (7nd+3) matrix of the linear map
+// (δdiv, δln ps, δT, energy, div_c, div_p, ln ps_c, ln ps_p, T_c, T_p) ->
+// (δdiv, δln ps, δT) of ∇²(energy) + Implicit_Correction!.
+__global__ void implicit_adjoint_kernel(const double* A, int nd, int nm, const double2* ddiv_b,
+                                        const double2* dlnps_b, const double2* dt_b, double2* out,
+                                        size_t spec_per_level) {
+  const int nin = 7 * nd + 3, nout = 2 * nd + 1;
+  HS_GRID_STRIDE(idx, spec_per_level) {
+    const int m = (int)(idx % nm);
+    const int n = (int)(idx / nm);
+    const double* An = A + (size_t)n * nout * nin;
+    for (int i = 0; i < nin; ++i) {
+      double2 acc = make_double2(0.0, 0.0);
+      if (n >= m) {
+        for (int o = 0; o < nout; ++o) {
+          const double2 x = o < nd ? ddiv_b[(size_t)o * spec_per_level + idx]
+                                   : (o == nd ? dlnps_b[idx]
+                                              : dt_b[(size_t)(o - nd - 1) * spec_per_level + idx]);
+          const double w = An[(size_t)o * nin + i];
+          acc = make_double2(acc.x + w * x.x, acc.y + w * x.y);
+        }
+      }
+      out[(size_t)i * spec_per_level + idx] = acc;
+    }
+  }
+}
+
+// Transpose of Compute_Gradient_Cos! (SSM:677-687):
+//   gλ[n] = dlam[n] (i h[n]),  gθ[n] = dthp[n] h[n+1] (n <= ns-1) + dthm[n] h[n-1] (n >= 1)
+// h_bar (+)= dlam (-i gλ_bar) + dthp[k-1] gθ_bar[k-1] + dthm[k+1] gθ_bar[k+1].
+__global__ void gradient_cos_adjoint_kernel(const double2* glb, const double2* gtb, double2* hb,
+                                            const double* dlam, const double* dthm,
+                                            const double* dthp, int nm, int ns, size_t n_all,
+                                            int accumulate) {
+  HS_GRID_STRIDE(i, n_all) {
+    const int m = (int)(i % nm);
+    const int k = (int)((i / nm) % (ns + 1));
+    const int c = m * (ns + 1) + k;
+    const double dl = dlam[c];
+    const double2 l0 = glb[i];
+    double2 z = make_double2(dl * l0.y, -(dl * l0.x));
+    if (k + 1 <= ns) {
+      const double w = dthm[c + 1];
+      const double2 g = gtb[i + nm];
+      z = make_double2(z.x + w * g.x, z.y + w * g.y);
+    }
+    if (k >= 1) {
+      const double w = dthp[c - 1];
+      const double2 g = gtb[i - nm];
+      z = make_double2(z.x + w * g.x, z.y + w * g.y);
+    }
+    if (accumulate) {
+      const double2 h = hb[i];
+      z = make_double2(h.x + z.x, h.y + z.y);
+    }
+    hb[i] = z;
+  }
+}
+
+static int check_even_sigma(const jgcm_vert_coord* v) {
+  JGCM_REQUIRE(v->zero_top == 1, "the adjoint needs a zero-top coordinate");
+  for (double a : v->ak) JGCM_REQUIRE(a == 0.0, "the adjoint needs an even-sigma coordinate (ak = 0)");
+  JGCM_REQUIRE(v->nd >= 1 && v->nd <= MAXND, "nd must be in 1..%d", MAXND);
+  return JGCM_OK;
+}
+
+}  // namespace jgcm
+
+using namespace jgcm;
+
+extern "C" {
+
+int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, double dt,
+                           double ep_bar, double mpsp_bar, const double* const* state,
+                           const double* const* cot_in, double* const* cot_out, double* workspace) {
+  JGCM_REQUIRE(si && hp && state && cot_in && cot_out && workspace, "NULL argument");
+  if (int rc = check_even_sigma(si->vc)) return rc;
+  jgcm_mesh* M = si->vc->mesh;
+  for (int i = 0; i < 14; ++i) JGCM_REQUIRE(state[i], "NULL state array %d", i);
+  for (int i = 0; i < 5; ++i) JGCM_REQUIRE(cot_in[i], "NULL input cotangent %d", i);
+  for (int i = 0; i < 14; ++i) JGCM_REQUIRE(cot_out[i], "NULL output cotangent %d", i);
+  HsColAdj s{state[0], state[1], state[2], state[3], state[4], state[5], state[6], state[7],
+             state[8], state[9], state[10], state[11], state[12], state[13],
+             cot_in[0], cot_in[1], cot_in[2], cot_in[3], cot_in[4],
+             cot_out[0], cot_out[1], cot_out[2], cot_out[3], cot_out[4], cot_out[5], cot_out[6],
+             cot_out[7], cot_out[8], cot_out[9], cot_out[10], cot_out[11], cot_out[12],
+             cot_out[13], workspace};
+  HsAdjScalars c{si->atmo.rdgas, si->atmo.cp_air, si->atmo.kappa, si->atmo.grav,
+                 2.0 * si->atmo.omega, dt, ep_bar, mpsp_bar, *hp};
+  size_t P = M->grid_elems();
+  hipLaunchKernelGGL(hs_column_adjoint_kernel, dim3(nblk_h(P)), dim3(256), 0, M->stream, s,
+                     vc_of(si->vc), M->nlat, M->nlon, M->d_sin, M->d_wts, c);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_hs_corrections_adjoint(jgcm_semi_implicit_t* si, const double* grid_u_n,
+                                const double* grid_v_n, const double* grid_t_n,
+                                const double* grid_ps_n, double corr, double factor, double en_bar,
+                                double mpsn_bar, double* u_n_bar, double* v_n_bar, double* t_n_bar,
+                                double* lnps_n_bar) {
+  JGCM_REQUIRE(si && grid_u_n && grid_v_n && grid_t_n && grid_ps_n && u_n_bar && v_n_bar &&
+                   t_n_bar && lnps_n_bar, "NULL argument");
+  JGCM_REQUIRE(factor > 0.0, "mass factor must be positive");
+  if (int rc = check_even_sigma(si->vc)) return rc;
+  jgcm_mesh* M = si->vc->mesh;
+  size_t P = M->grid_elems();
+  hipLaunchKernelGGL(hs_corrections_adjoint_kernel, dim3(nblk_h(P)), dim3(256), 0, M->stream,
+                     vc_of(si->vc), M->nlat, M->nlon, M->d_wts, grid_u_n, grid_v_n, grid_t_n,
+                     grid_ps_n, corr, factor, en_bar, mpsn_bar, si->atmo.grav, si->atmo.cp_air,
+                     u_n_bar, v_n_bar, t_n_bar, lnps_n_bar);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_implicit_adjoint(jgcm_mesh_t* mesh, int nd, const double* matrix, const double* ddiv_bar,
+                          const double* dlnps_bar, const double* dt_bar, double* out) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nd >= 1 && nd <= MAXND, "nd must be in 1..%d, got %d", MAXND, nd);
+  JGCM_REQUIRE(matrix && ddiv_bar && dlnps_bar && dt_bar && out, "NULL argument");
+  size_t S = mesh->spec_elems();
+  hipLaunchKernelGGL(implicit_adjoint_kernel, dim3(nblk_h(S)), dim3(256), 0, mesh->stream, matrix, nd,
+                     mesh->nm, (const double2*)ddiv_bar, (const double2*)dlnps_bar,
+                     (const double2*)dt_bar, (double2*)out, S);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_gradient_cos_adjoint(jgcm_mesh_t* mesh, const double* grad_lambda_bar,
+                              const double* grad_theta_bar, double* h_bar, int accumulate, int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(grad_lambda_bar && grad_theta_bar && h_bar, "NULL argument");
+  size_t n = (size_t)nlev * mesh->spec_elems();
+  hipLaunchKernelGGL(gradient_cos_adjoint_kernel, dim3(nblk_h(n)), dim3(256), 0, mesh->stream,
+                     (const double2*)grad_lambda_bar, (const double2*)grad_theta_bar,
+                     (double2*)h_bar, mesh->d_dlam, mesh->d_dthm, mesh->d_dthp, mesh->nm, mesh->ns,
+                     n, accumulate ? 1 : 0);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+}  // extern "C"

@@ -17,41 +17,13 @@
 #include <new>
 #include <vector>
 
-#include "jgcm_internal.hpp"
+#include "primitive_internal.hpp"
 
 namespace jgcm {
 int area_mean_device(const jgcm_mesh* M, const double* grid, double* out_dev);
 }
 
-struct jgcm_vert_coord {
-  jgcm_mesh* mesh = nullptr;
-  int nd = 0;
-  int advect_scheme = 0;  // 0: second_centered_wts, 1: second_centered
-  int zero_top = 1;
-  double p_ref = 101325.0;
-  std::vector<double> ak, bk, dak, dbk;
-  double* d_coef = nullptr;  // [ak(nd+1) | bk(nd+1) | dak(nd) | dbk(nd)]
-};
-
-struct jgcm_semi_implicit {
-  jgcm_vert_coord* vc = nullptr;
-  jgcm_leapfrog_t* lf = nullptr;
-  jgcm_atmo_params atmo{};
-  int nd = 0;
-  double ps_ref = 0.0;
-  std::vector<double> dp_ref, p_half_ref, lnp_half_ref, lnp_full_ref, t_ref, h, div_mat;
-  std::vector<double> nu, tau, gamma;
-  int num_wavenumbers = 0;
-  std::vector<double> wave_matrix;  // [num_wavenumbers+1][nd][nd]
-  double* d_ref = nullptr;          // [dp_ref | lnp_half_ref | lnp_full_ref | t_ref | h] on device
-  double* d_wave = nullptr;
-  double* d_scal = nullptr;         // correction scalars
-  double* d_colsum = nullptr;       // column sums [2][nlat][nlon]
-};
-
 namespace jgcm {
-
-constexpr int MAXND = 64;
 
 static inline int nblk(size_t n) {
   size_t b = (n + 255) / 256;
@@ -60,19 +32,6 @@ static inline int nblk(size_t n) {
 
 #define GRID_STRIDE(i, n) \
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
-
-struct VC {
-  int nd, zero_top, scheme;
-  const double* ak;
-  const double* bk;
-  const double* dak;
-  const double* dbk;
-};
-
-static VC vc_of(const jgcm_vert_coord* v) {
-  return VC{v->nd, v->zero_top, v->advect_scheme, v->d_coef, v->d_coef + (v->nd + 1),
-            v->d_coef + 2 * (v->nd + 1), v->d_coef + 2 * (v->nd + 1) + v->nd};
-}
 
 // ---------------------------------------------------------------- columns
 // Serial one-thread-per-column forms, used by the stand-alone entry points

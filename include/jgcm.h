@@ -221,11 +221,12 @@ int jgcm_vorticity_flux_adjoint(jgcm_mesh_t* mesh, const double* grid_vor, const
 /* transpose of Compute_Spectral_Damping! + Filtered_Leapfrog! + Time_Advance!
  * (Time_Integrator.jl:58-101, Dyn_Data.jl:326-364) for one field:
  * (p_bar', c_bar') -> (p_bar, c_bar, g_bar), g_bar the cotangent of the
- * undamped tendency.  init_step selects the first-step form.  In place
- * (p_bar == p_bar_next, c_bar == c_bar_next) is allowed.                     */
-int jgcm_damped_leapfrog_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* p_bar_next,
-                                 const double* c_bar_next, double* p_bar, double* c_bar,
-                                 double* g_bar, int nlev);
+ * undamped tendency.  init_step selects the first-step form; damp = 0 for a
+ * field the step does not damp (ln ps).  In place (p_bar == p_bar_next,
+ * c_bar == c_bar_next) is allowed.                                           */
+int jgcm_damped_leapfrog_adjoint(jgcm_leapfrog_t* lf, int init_step, int damp,
+                                 const double* p_bar_next, const double* c_bar_next, double* p_bar,
+                                 double* c_bar, double* g_bar, int nlev);
 /* one reverse step of Barotropic_Dynamics! (Barotropic_Dynamics.jl:2-46), in
  * place on (p_bar, c_bar); grid_vor, grid_u, grid_v are the grid fields of
  * the step's starting state.                                                */
@@ -242,6 +243,39 @@ int jgcm_sw_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* grid_
                          double kappa_t, double h_0, double* vor_p_bar, double* vor_c_bar,
                          double* div_p_bar, double* div_c_bar, double* h_p_bar, double* h_c_bar,
                          int nlev);
+
+/* Pieces of the reverse Held-Suarez step (no reference counterpart; even-sigma
+ * coordinate).  jgcm_hs_column_adjoint: transpose of the grid-column part of
+ * Atmosphere_Update! (Spectral_Dynamics.jl:206-307, 452-467; HS_Forcing.jl:1-130;
+ * Spectral_Dynamics.jl:5-28 for the energy integral of the p-fields) at a state.
+ *   state[14]  : grid div, u, v, T, vor [nd]; ∂λT, ∂θT [nd]; ∂λ ln ps, ∂θ ln ps [1];
+ *                u_p, v_p, T_p [nd]; ps_p, ps_c [1]
+ *   cot_in[5]  : cotangents of the grid δu, δv, δT, Φ+KE [nd] and δln ps [1]
+ *   cot_out[14]: cotangents of div, u, v, T, vor, ∂λT, ∂θT, ∂λ ln ps, ∂θ ln ps,
+ *                u_p, v_p, T_p, ps_p, ps_c (same shapes as the state)
+ *   ep_bar, mpsp_bar: cotangents of the energy integral and mean ps of the p-fields
+ *   workspace  : 4 nd nθ nλ doubles                                          */
+int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, double dt,
+                           double ep_bar, double mpsp_bar, const double* const* state,
+                           const double* const* cot_in, double* const* cot_out, double* workspace);
+/* transpose of the mass and energy corrections (Spectral_Dynamics.jl:30-68) with
+ * respect to the new-state grid u, v, T and ln ps (T_n and ps_n passed after
+ * their corrections; corr, factor = the step's correction scalars).          */
+int jgcm_hs_corrections_adjoint(jgcm_semi_implicit_t* si, const double* grid_u_n,
+                                const double* grid_v_n, const double* grid_t_n,
+                                const double* grid_ps_n, double corr, double factor, double en_bar,
+                                double mpsn_bar, double* u_n_bar, double* v_n_bar, double* t_n_bar,
+                                double* lnps_n_bar);
+/* transpose of ∇²(energy); δdiv -= energy; Implicit_Correction! (Spectral_Dynamics.jl:309-316,
+ * Semi_Implicit.jl:462-513) from its per-n matrix [ns+1][2nd+1][7nd+3] (outputs
+ * δdiv, δln ps, δT; inputs δdiv, δln ps, δT, energy, div_c, div_p, ln ps_c,
+ * ln ps_p, T_c, T_p).  out: 7nd+3 spectral level-fields in that order.       */
+int jgcm_implicit_adjoint(jgcm_mesh_t* mesh, int nd, const double* matrix, const double* ddiv_bar,
+                          const double* dlnps_bar, const double* dt_bar, double* out);
+/* transpose of Compute_Gradient_Cos! (Spectral_Spherical_Mesh.jl:677-687):
+ * (∂λ_bar, ∂θ_bar) -> h_bar (added when accumulate != 0).                   */
+int jgcm_gradient_cos_adjoint(jgcm_mesh_t* mesh, const double* grad_lambda_bar,
+                              const double* grad_theta_bar, double* h_bar, int accumulate, int nlev);
 
 /* ---- operators ---------------------------------------------------------- */
 /* Divide_By_Cos!(cosθ, grid)                    Spectral_Spherical_Mesh.jl:333-338 */

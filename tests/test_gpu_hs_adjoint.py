@@ -1,0 +1,165 @@
+"""Reverse-mode derivative of the Held-Suarez step (Atmosphere_Update!,
+Spectral_Dynamics.jl:206-490).
+
+The implicit-correction and gradient transposes are pinned by the dot-product
+identity against their forward entry points (1e-12).  The whole reverse sweep
+is pinned by central differences of Atmosphere_Update! runs: along random
+directions in the full state (spectral p, c of vor, div, T, ln ps and the
+grid p-fields the forcing reads) the difference quotient of
+J = Σ_f <W_f, f^K> must converge to the gradient at second order in the step
+and agree with it to 1e-6 at the smallest step."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+A, OMEGA = 6371000.0, 7.292e-5
+HS = {"σ_b": 0.7, "k_f": 1.0, "k_a": 1.0 / 40.0, "k_s": 1.0 / 4.0, "ΔT_y": 60.0, "Δθ_z": 10.0}
+
+
+def rspec(N, nlev, rng, valid=True):
+    s = rng.standard_normal((nlev, N + 2, N + 1)) + 1j * rng.standard_normal((nlev, N + 2, N + 1))
+    for m in range(N + 1):
+        s[:, :m, m] = 0
+    if valid:
+        s[:, :, 0] = s[:, :, 0].real
+        s[:, N + 1, :] = 0
+    return s
+
+
+def dev(x):
+    return torch.tensor(np.ascontiguousarray(x), device="cuda")
+
+
+def dot(a, b):
+    a, b = a.cpu().numpy(), b.cpu().numpy()
+    if np.iscomplexobj(a):
+        return float(np.sum(a.real * b.real + a.imag * b.imag))
+    return float(np.sum(a * b))
+
+
+def model(jgcm, N=21, nlat=32, nd=8, dt=600, spinup=3, seed=11):
+    nlon = 2 * nlat
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, nlon, nd, A)
+    vc = jgcm.Vert_Coordinate(nlon, nlat, nd, "even_sigma", "simmons_and_burridge",
+                              "second_centered_wts", 1e5, mesh=mesh)
+    atmo = jgcm.Atmo_Data("hs", nlon, nlat, nd, True, True, False, False, mesh.sinθ, A, OMEGA)
+
+    def make_solver(init):
+        I = jgcm.Filtered_Leapfrog(0.04, 4, 1.15741e-4, mesh.laplacian_eig, 0.5, dt, True, 0,
+                                   86400 * 200)
+        si = jgcm.Semi_Implicit_Solver(vc, atmo, I, 1e5, np.full(nd, 300.0), mesh.wave_numbers)
+        if not init:
+            jgcm.Update_Init_Step(si)
+        return si
+
+    d = jgcm.Dyn_Data("hs", N, N + 1, nlon, nlat, nd, mesh=mesh)
+    jgcm.Get_Topography(d.grid_geopots)
+    noise = 0.1 * np.random.default_rng(seed).standard_normal((nd, nlat, nlon))
+    jgcm.Spectral_Initialize_Fields(mesh, atmo, vc, 1.0e5, 300.0, d.grid_geopots, d,
+                                    grid_t_noise=dev(noise))
+    si = make_solver(True)
+    for i in range(spinup):
+        jgcm.Atmosphere_Update(mesh, atmo, vc, si, d, HS)
+        if i == 0:
+            jgcm.Update_Init_Step(si)
+    torch.cuda.synchronize()
+    return mesh, vc, atmo, make_solver, d
+
+
+def state_of(jgcm, d):
+    return tuple(getattr(d, n).clone() for n in jgcm.Held_Suarez_Adjoint.STATE)
+
+
+@pytest.mark.parametrize("init", [True, False])
+def test_implicit_adjoint_dot_product(jgcm, init):
+    mesh, vc, atmo, make_solver, _ = model(jgcm, spinup=0)
+    si = make_solver(init)
+    adj = jgcm.Held_Suarez_Adjoint(mesh, atmo, vc, si, HS)
+    Amat = adj._implicit_matrix(si)
+    N, nd = mesh.num_fourier, vc.nd
+    rng = np.random.default_rng(5 + init)
+    names = [("ddiv", nd), ("dlnps", 1), ("dt", nd), ("energy", nd), ("div_c", nd), ("div_p", nd),
+             ("lnps_c", 1), ("lnps_p", 1), ("t_c", nd), ("t_p", nd)]
+    x = {n: dev(rspec(N, L, rng, valid=False)) for n, L in names}
+    y = [dev(rspec(N, L, rng, valid=False)) for L in (nd, 1, nd)]
+    fx = {n: t.clone() for n, t in x.items()}
+    from jgcm_amd._lib import call
+    from jgcm_amd.mesh import _ptr
+    call("jgcm_hs_energy_implicit_correction", si.handle, *map(_ptr, (
+        fx["energy"], fx["div_c"], fx["div_p"], fx["lnps_c"], fx["lnps_p"], fx["t_c"], fx["t_p"],
+        fx["ddiv"], fx["dlnps"], fx["dt"])))
+    out = mesh.spectral_zeros(7 * nd + 3)
+    call("jgcm_implicit_adjoint", mesh.handle, nd, _ptr(Amat), _ptr(y[0]), _ptr(y[1]), _ptr(y[2]),
+         _ptr(out))
+    torch.cuda.synchronize()
+    lhs = dot(fx["ddiv"], y[0]) + dot(fx["dlnps"], y[1]) + dot(fx["dt"], y[2])
+    rhs, o = 0.0, 0
+    for n, L in names:
+        rhs += dot(x[n], out[o:o + L])
+        o += L
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + abs(rhs)), (lhs, rhs)
+
+
+def test_gradient_cos_adjoint_dot_product(jgcm):
+    N, nlat, nlev = 42, 64, 3
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, nlev, A)
+    rng = np.random.default_rng(8)
+    h = dev(rspec(N, nlev, rng, valid=False))
+    a, b = dev(rspec(N, nlev, rng, valid=False)), dev(rspec(N, nlev, rng, valid=False))
+    gl, gt = mesh.spectral_zeros(), mesh.spectral_zeros()
+    jgcm.Compute_Gradient_Cos(mesh, h, gl, gt)
+    hb = dev(rspec(N, nlev, rng, valid=False))
+    hb0 = hb.clone()
+    from jgcm_amd._lib import call
+    from jgcm_amd.mesh import _ptr
+    call("jgcm_gradient_cos_adjoint", mesh.handle, _ptr(a), _ptr(b), _ptr(hb), 1, nlev)
+    torch.cuda.synchronize()
+    lhs = dot(gl, a) + dot(gt, b)
+    rhs = dot(h, hb - hb0)
+    # the sums cancel to ~1e-3 of their terms: scale the bound by the operands' norms
+    nrm = lambda t: float(np.linalg.norm(t.cpu().numpy()))
+    scale = nrm(gl) * nrm(a) + nrm(gt) * nrm(b)
+    assert abs(lhs - rhs) <= 1e-13 * scale, (lhs, rhs, scale)
+
+
+@pytest.mark.parametrize("K,starts_init", [(1, True), (1, False), (3, True)])
+def test_hs_gradient_matches_central_differences(jgcm, K, starts_init):
+    mesh, vc, atmo, make_solver, d = model(jgcm, spinup=3)
+    state = state_of(jgcm, d)
+    N = mesh.num_fourier
+    rng = np.random.default_rng(61 + K)
+    scales = [max(float(t.abs().max()), 1e-30) for t in state]
+
+    def rand_like(t, r):
+        if t.is_complex():
+            return dev(rspec(N, t.shape[0], r))
+        return dev(r.standard_normal(tuple(t.shape)))
+
+    weights = [rand_like(t, rng) / s for t, s in zip(state, scales)]
+
+    def run(st):
+        si = make_solver(starts_init)
+        adj = jgcm.Held_Suarez_Adjoint(mesh, atmo, vc, si, HS, checkpoint_every=2)
+        return adj, adj.forward(st, K)
+
+    adj, _ = run(state)
+    grads = adj.backward(weights)
+    for seed in range(3):
+        r = np.random.default_rng(300 + seed)
+        dirs = []
+        for t, s in zip(state, scales):
+            dk = rand_like(t, r)
+            dirs.append(s * dk / dk.abs().max())
+        an = sum(dot(gk, dk) for gk, dk in zip(grads, dirs))
+        err = []
+        for f in (1e-3, 1e-4, 1e-5):
+            _, fp = run(tuple(x + f * dk for x, dk in zip(state, dirs)))
+            _, fm = run(tuple(x - f * dk for x, dk in zip(state, dirs)))
+            Jp = sum(dot(w, o) for w, o in zip(weights, fp))
+            Jm = sum(dot(w, o) for w, o in zip(weights, fm))
+            err.append((Jp - Jm) / (2 * f) - an)
+        tol = max(1.0, abs(an))
+        assert abs(err[1]) <= 0.05 * abs(err[0]) + 1e-8 * tol, (seed, an, err)
+        assert abs(err[2]) <= 1e-6 * tol, (seed, an, err)
