@@ -250,6 +250,32 @@ class _Run_Adjoint:
         return bars
 
 
+class Run(torch.autograd.Function):
+    """A whole run as a differentiable function (the counterpart of a
+    ChainRulesCore ``rrule`` of the time loop):
+
+        final = Run.apply(adjoint, nsteps, *state)
+
+    ``adjoint`` is a Barotropic_Adjoint, Shallow_Water_Adjoint or
+    Held_Suarez_Adjoint; ``state`` its STATE tensors.  The backward pass is
+    the adjoint's checkpointed reverse sweep; gradients of complex tensors use
+    torch's convention (real and imaginary parts, <a, b> = Re(conj(a) b))."""
+
+    @staticmethod
+    def forward(ctx, adjoint, nsteps, *state):
+        ctx.adjoint = adjoint
+        ctx.shapes = [(t.shape, t.dtype) for t in state]
+        return adjoint._run(tuple(t.detach().contiguous() for t in state), int(nsteps))
+
+    @staticmethod
+    def backward(ctx, *bars):
+        adj = ctx.adjoint
+        full = tuple(b.contiguous() if b is not None else
+                     torch.zeros(shp, dtype=dt, device=adj.mesh.device)
+                     for b, (shp, dt) in zip(bars, ctx.shapes))
+        return (None, None) + tuple(adj._reverse(full))
+
+
 class Barotropic_Adjoint(_Run_Adjoint):
     """Gradient of a function of the final state of an ``nsteps`` barotropic
     run with respect to the initial spectral vorticity (vor_p, vor_c).

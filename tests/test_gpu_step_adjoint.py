@@ -340,3 +340,47 @@ def test_shallow_water_gradient_matches_central_differences(jgcm, K, starts_init
         tol = max(1.0, abs(an))
         assert abs(err[1]) <= 0.05 * abs(err[0]) + 1e-8 * tol, (seed, an, err)
         assert abs(err[2]) <= 1e-6 * tol, (seed, an, err)
+
+
+def test_run_autograd_function_equals_reverse_sweep(jgcm):
+    """Run.apply (the rrule of the time loop) gives exactly the reverse sweep's gradient."""
+    from jgcm_amd.adjoint import Run
+    N, nlat, K = 21, 32, 5
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 1, A)
+    atmo = jgcm.Atmo_Data("b", 2 * nlat, nlat, 1, False, False, False, False, mesh.sinθ, A, OMEGA)
+    I = jgcm.Filtered_Leapfrog(0.04, 4, 1.0 / (0.1 * 86400), mesh.laplacian_eig, 0.0, 1800, True, 0,
+                               864000)
+    rng = np.random.default_rng(71)
+    c0 = rh4_state(jgcm, mesh)
+    W = dev(rspec(N, 1, rng, valid=True))
+    vp = c0.clone().requires_grad_(True)
+    vc = c0.clone().requires_grad_(True)
+    adj = jgcm.Barotropic_Adjoint(mesh, atmo, I, checkpoint_every=2)
+    pK, cK = Run.apply(adj, K, vp, vc)
+    loss = (W.conj() * cK).real.sum()
+    loss.backward()
+    ref = jgcm.Barotropic_Adjoint(mesh, atmo, I, checkpoint_every=2)
+    ref.forward(c0, c0, K)
+    gp, gc = ref.backward(torch.zeros_like(W), W)
+    assert torch.equal(vc.grad, gc) and torch.equal(vp.grad, gp)
+
+
+@pytest.mark.parametrize("prec", ["split32", "bf16x3"])
+def test_gradient_with_reduced_precision_legendre(jgcm, prec):
+    """The reverse sweep on a split-FP32 / BF16x3 Legendre mesh stays within
+    1e-6 (relative, max norm) of the FP64 gradient."""
+    N, nlat, K = 42, 64, 8
+    grads = {}
+    for p in ("fp64", prec):
+        mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 1, A, precision=p)
+        atmo = jgcm.Atmo_Data("b", 2 * nlat, nlat, 1, False, False, False, False, mesh.sinθ, A, OMEGA)
+        I = jgcm.Filtered_Leapfrog(0.04, 4, 1.0 / (0.1 * 86400), mesh.laplacian_eig, 0.0, 1800, True,
+                                   0, 864000)
+        c0 = rh4_state(jgcm, mesh)
+        W = dev(rspec(N, 1, np.random.default_rng(5), valid=True))
+        adj = jgcm.Barotropic_Adjoint(mesh, atmo, I, checkpoint_every=4)
+        adj.forward(c0, c0, K)
+        grads[p] = adj.backward(torch.zeros_like(W), W)[1].cpu().numpy()
+    g0, g1 = grads["fp64"], grads[prec]
+    err = np.abs(g1 - g0).max() / np.abs(g0).max()
+    assert err < 1e-6, err
