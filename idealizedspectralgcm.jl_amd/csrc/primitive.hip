@@ -381,46 +381,6 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
   }
 }
 
-// Finishes an area-weighted global mean from the per-workgroup partials (one
-// workgroup, fixed summation order) and the scalar steps of the corrections.
-// Scalar slots: 0 mean_ps_p, 1 mean_energy_p (mass-weighted integral, /g),
-// 2 mean_ps_n, 3 mass factor, 4 log(factor), 5 mean_energy_n, 6 temperature
-// correction.  stage 0: after the physics kernel (Compute_Corrections_Init);
-// stage 1: mean of ps_n (mass factor; with energy-only, stage 2 as well);
-// stage 2: energy of the corrected state.
-__global__ __launch_bounds__(256) void area_final_kernel(const double2* __restrict__ partial,
-                                                         int nparts, int nlon, double* scal,
-                                                         int stage, int mass, int energy,
-                                                         double grav, double cp_air) {
-  __shared__ double2 red[256];
-  double2 s = d2(0.0, 0.0);
-  for (int i = threadIdx.x; i < nparts; i += 256) s = d2add(s, partial[i]);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] = d2add(red[threadIdx.x], red[threadIdx.x + o]);
-    __syncthreads();
-  }
-  if (threadIdx.x != 0) return;
-  double m0 = red[0].x / (2.0 * nlon), m1 = red[0].y / (2.0 * nlon);
-  if (stage == 0) {
-    if (mass) scal[0] = m0;
-    if (energy) scal[1] = m1 / grav;
-    return;
-  }
-  if (stage == 1 && mass) {
-    scal[2] = m0;
-    double f = scal[0] / scal[2];
-    scal[3] = f;
-    scal[4] = log(f);
-    return;
-  }
-  if (energy) {
-    scal[5] = m1 / grav;
-    scal[6] = grav * (scal[1] - scal[5]) / (cp_air * scal[0]);
-  }
-}
-
 // Step part 2 (SD:260-307 minus the transforms): ∇ps = ∇lnps * ps,
 // Four_In_One! (SD:88-183), Compute_Geopotential!, δlnps = δps / ps,
 // Vert_Advection! (Vert_Coordinate.jl:171-210) of u, v, T added to the
@@ -580,23 +540,20 @@ __global__ __launch_bounds__(512) void hs_column_dynamics_kernel(
   }
 }
 
-// Step part 3 of the corrections (SD:336-343): ps_n = exp(lnps) (stage 1) or
-// ps_n *= mass factor (stage 2), per-workgroup partials of the area mean of
-// ps_n and of the column energy integral of the new state.
+// Compute_Corrections! sums (SD:30-68) in one pass: ps_n = exp(ln ps) (SD:336)
+// and per workgroup Σ w ps_n, Σ w Σ_k E_k Δa_k and Σ w ps_n Σ_k E_k Δb_k with
+// E = KE + c_p T_n, so that the mass-weighted energy of the corrected
+// ps = f ps_n is (Σ_a + f Σ_b) / (2 nλ g) without a second pass.
 template <int KPT>
-__global__ __launch_bounds__(512) void hs_ps_energy_kernel(VC v, int nlat, int nlon, double cp_air,
-                                                           const double* __restrict__ wts,
-                                                           const double* lnps, double* ps_n,
-                                                           const double* scal, int use_factor,
-                                                           const double* u, const double* vv,
-                                                           const double* t, double2* partial,
-                                                           int energy) {
+__global__ __launch_bounds__(512) void hs_corrections_sums_kernel(
+    VC v, int nlat, int nlon, double cp_air, const double* __restrict__ wts, const double* lnps,
+    double* ps_n, const double* u, const double* vv, const double* t, double* partial, int sums,
+    int energy) {
   __shared__ double2 red[MAXNQ * TILE];
   const size_t P = (size_t)nlat * nlon;
   const TileIdx tx = tile_idx<KPT>(P);
-  double p = 0.0;
-  if (tx.col) p = use_factor ? ps_n[tx.c] * scal[3] : exp(lnps[tx.c]);
-  double part = 0.0;
+  const double p = tx.col ? exp(lnps[tx.c]) : 0.0;
+  double pa = 0.0, pb = 0.0;
   if (energy) {
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
@@ -604,30 +561,82 @@ __global__ __launch_bounds__(512) void hs_ps_energy_kernel(VC v, int nlat, int n
       if (tx.col && k < v.nd) {
         size_t o = (size_t)k * P + tx.c;
         double e = 0.5 * (u[o] * u[o] + vv[o] * vv[o]) + cp_air * t[o];
-        part = part + e * (v.dak[k] + v.dbk[k] * p);
+        pa = pa + e * v.dak[k];
+        pb = pb + e * v.dbk[k];
       }
     }
   }
   double2 above, below;
-  level_sums(red, tx, d2(part, 0.0), above, below);  // also orders the ps_n read before the write
+  level_sums(red, tx, d2(pa, pb), above, below);
   if (tx.col && tx.q == 0) ps_n[tx.c] = p;
-  block_area_partial(tx, nlon, wts, p, part + below.x, partial);
+  if (sums && tx.q == 0) {
+    double wj = tx.col ? wts[tx.c / nlon] : 0.0;
+    double s0 = wave_sum(tx.col ? p * wj : 0.0);
+    double s1 = wave_sum(tx.col ? (pa + below.x) * wj : 0.0);
+    double s2 = wave_sum(tx.col ? p * (pb + below.y) * wj : 0.0);
+    if (tx.lane == 0) {
+      partial[3 * (size_t)blockIdx.x] = s0;
+      partial[3 * (size_t)blockIdx.x + 1] = s1;
+      partial[3 * (size_t)blockIdx.x + 2] = s2;
+    }
+  }
 }
 
-// apply the corrections (SD:41-57): T_n += corr, spe_t_n[k](0,0) += corr,
-// spe_lnps_n(0,0) += log(factor)
-__global__ void hs_apply_corrections_kernel(int nd, size_t P, size_t spec_per_level, double* t_n,
-                                            double2* spe_t_n, double2* spe_lnps_n,
-                                            const double* scal, int mass, int energy) {
+// Compute_Corrections! (SD:30-68).  Every workgroup first finishes the area
+// means from the per-workgroup partials of the physics pass (Σ w ps_p, Σ w Σ E_p Δp)
+// and of the sums pass, in the same fixed order in every workgroup (so all
+// agree, bit for bit, and no separate reduction launch is needed); workgroup 0
+// stores the scalars: 0 mean ps_p, 1 mean energy_p, 2 mean ps_n, 3 factor f,
+// 4 log f, 5 mean energy_n, 6 temperature correction.  Then ps_n *= f,
+// T_n += corr, and the (0, 0) spectral coefficients of ln ps_n and T_n.
+constexpr int APPLY_THREADS = 256;
+__global__ __launch_bounds__(APPLY_THREADS) void hs_apply_corrections_kernel(
+    int nd, size_t P, size_t spec_per_level, int nlon, double* t_n, double* ps_n,
+    double2* spe_t_n, double2* spe_lnps_n, const double* part_p, const double* part_n, int nparts,
+    double* scal, double grav, double cp_air, int mass, int energy) {
+  __shared__ double red[5][APPLY_THREADS];
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < nparts; i += APPLY_THREADS) {
+    acc[0] = acc[0] + part_p[2 * (size_t)i];
+    acc[1] = acc[1] + part_p[2 * (size_t)i + 1];
+    acc[2] = acc[2] + part_n[3 * (size_t)i];
+    acc[3] = acc[3] + part_n[3 * (size_t)i + 1];
+    acc[4] = acc[4] + part_n[3 * (size_t)i + 2];
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int st = APPLY_THREADS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  const double mps_p = red[0][0] / (2.0 * nlon);
+  const double e_p = red[1][0] / (2.0 * nlon) / grav;
+  const double mps_n = red[2][0] / (2.0 * nlon);
+  const double f = mass ? mps_p / mps_n : 1.0;
+  const double e_n = (red[3][0] + f * red[4][0]) / (2.0 * nlon) / grav;
+  const double corr = energy ? grav * (e_p - e_n) / (cp_air * mps_p) : 0.0;
   size_t n = (size_t)nd * P;
-  double corr = energy ? scal[6] : 0.0;
   GRID_STRIDE(i, n) {
     if (energy) t_n[i] = t_n[i] + corr;
+    if (mass && i < P) ps_n[i] = ps_n[i] * f;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (mass) spe_lnps_n[0].x = spe_lnps_n[0].x + scal[4];
-    if (energy)
+    scal[0] = mps_p;
+    if (energy) scal[1] = e_p;
+    if (mass) {
+      scal[2] = mps_n;
+      scal[3] = f;
+      scal[4] = log(f);
+      spe_lnps_n[0].x = spe_lnps_n[0].x + scal[4];
+    }
+    if (energy) {
+      scal[5] = e_n;
+      scal[6] = corr;
       for (int k = 0; k < nd; ++k) spe_t_n[k * spec_per_level].x = spe_t_n[k * spec_per_level].x + corr;
+    }
   }
 }
 
@@ -1261,15 +1270,6 @@ static double2* area_partials(const jgcm_semi_implicit* si) {
   return reinterpret_cast<double2*>(si->d_colsum);
 }
 
-static int area_final(const jgcm_semi_implicit* si, int stage) {
-  const jgcm_mesh* M = si->vc->mesh;
-  int nparts = (int)tile_grid(M->grid_elems()).x;
-  hipLaunchKernelGGL(area_final_kernel, dim3(1), dim3(256), 0, M->stream, area_partials(si), nparts,
-                     M->nlon, si->d_scal, stage, si->atmo.do_mass_correction,
-                     si->atmo.do_energy_correction, si->atmo.grav, si->atmo.cp_air);
-  JGCM_LAUNCH_CHECK();
-  return JGCM_OK;
-}
 
 // HS_Forcing!(atmo_data, Δt, sinθ, u, v, p_half, p_full, t, δu, δv, t_eq, δt, params)
 // HS_Forcing.jl:1-33 (pressures are recomputed from grid_ps; see jgcm.h)
@@ -1292,8 +1292,6 @@ int jgcm_hs_physics(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t 
                      grid_ps_p, p_half, dp, lnp_half, p_full, lnp_full, grid_du, grid_dv, grid_t_eq,
                      grid_dt, grid_dps, area_partials(si), do_forcing, mass, energy, bad);
   JGCM_LAUNCH_CHECK();
-  // Compute_Corrections_Init (SD:5-28): area means into the scalar slots
-  if (mass || energy) return area_final(si, 0);
   return JGCM_OK;
 }
 
@@ -1334,27 +1332,23 @@ int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, doubl
   int mass = si->atmo.do_mass_correction, energy = si->atmo.do_energy_correction;
   int nd = si->nd;
   VC v = vc_of(si->vc);
-  // ps_n = exp(lnps); without the mass correction the energy integrand of this ps_n
-  JGCM_TILE_DISPATCH(nd, hs_ps_energy_kernel, tile_grid(P), tile_block(nd), 0, M->stream, v, M->nlat,
-                     M->nlon, si->atmo.cp_air, M->d_wts, grid_lnps, grid_ps_n, si->d_scal, 0,
-                     grid_u_n, grid_v_n, grid_t_n, area_partials(si), mass ? 0 : energy);
+  // ps_n = exp(lnps) and the correction sums (one pass), then the corrections; the
+  // physics pass left the partials of Compute_Corrections_Init at the start of the
+  // column-sum scratch, these go after them
+  const int nparts = (int)tile_grid(P).x;
+  double* part_p = si->d_colsum;
+  double* part_n = si->d_colsum + 2 * (size_t)nparts;
+  JGCM_TILE_DISPATCH(nd, hs_corrections_sums_kernel, tile_grid(P), tile_block(nd), 0, M->stream, v,
+                     M->nlat, M->nlon, si->atmo.cp_air, M->d_wts, grid_lnps, grid_ps_n, grid_u_n,
+                     grid_v_n, grid_t_n, part_n, mass || energy, energy);
   JGCM_LAUNCH_CHECK();
-  if (mass || energy)
-    if (int rc = area_final(si, 1)) return rc;
-  if (mass) {
-    // ps_n *= factor and the energy integrand with the corrected ps
-    JGCM_TILE_DISPATCH(nd, hs_ps_energy_kernel, tile_grid(P), tile_block(nd), 0, M->stream, v,
-                       M->nlat, M->nlon, si->atmo.cp_air, M->d_wts, grid_lnps, grid_ps_n,
-                       si->d_scal, 1, grid_u_n, grid_v_n, grid_t_n, area_partials(si), energy);
-    JGCM_LAUNCH_CHECK();
-    if (energy)
-      if (int rc = area_final(si, 2)) return rc;
-  }
   if (mass || energy) {
     size_t n = (size_t)si->nd * P;
-    hipLaunchKernelGGL(hs_apply_corrections_kernel, dim3(nblk(n)), dim3(256), 0, M->stream, si->nd, P,
-                       M->spec_elems(), grid_t_n, (double2*)spe_t_n, (double2*)spe_lnps_n,
-                       si->d_scal, mass, energy);
+    int blocks = (int)std::min<size_t>((n + APPLY_THREADS - 1) / APPLY_THREADS, 256);
+    hipLaunchKernelGGL(hs_apply_corrections_kernel, dim3(blocks), dim3(APPLY_THREADS), 0, M->stream,
+                       si->nd, P, M->spec_elems(), M->nlon, grid_t_n, grid_ps_n, (double2*)spe_t_n,
+                       (double2*)spe_lnps_n, part_p, part_n, nparts, si->d_scal, si->atmo.grav,
+                       si->atmo.cp_air, mass, energy);
     JGCM_LAUNCH_CHECK();
   }
   return JGCM_OK;
