@@ -906,17 +906,20 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
 }
 
 // ------------------------------------------------------------------ drivers
-#ifndef JGCM_PPW_MIN
-#define JGCM_PPW_MIN 8
-#endif
-static int pick_ppw(int L) {
+// Latitude pairs per FFT workgroup.  A workgroup runs 256/(L/4) FFTs per
+// pass; at least 8 pairs per workgroup, except at L = 256 where one pass of 4
+// measured faster for the fold at every batch size and for the unfold below
+// ~48 level-fields (T85: fold -1.2 us at 20 and 61 level-fields, unfold
+// -1.3 us at 20, +0.9 us at 61).
+static int pick_ppw(int L, int B, bool fold) {
   int fpw = 256 / (L / 4);
-  return fpw >= JGCM_PPW_MIN ? fpw : JGCM_PPW_MIN;
+  if (fpw == 4 && (fold || B < 48)) return 4;
+  return fpw >= 8 ? fpw : 8;
 }
 
 template <int L>
 static void launch_fold(const jgcm_mesh* M, const SegMap& in, double2* eo, int B) {
-  int ppw = pick_ppw(L);
+  int ppw = pick_ppw(L, B, true);
   dim3 grd((M->half + ppw - 1) / ppw, B);
   size_t lds = (size_t)(M->nm * 2 + 1) * ppw * sizeof(double2);
   hipLaunchKernelGGL(fft_fold_kernel<L>, grd, dim3(256), lds, M->stream, in, eo,
@@ -925,7 +928,7 @@ static void launch_fold(const jgcm_mesh* M, const SegMap& in, double2* eo, int B
 
 template <int L>
 static void launch_unfold(const jgcm_mesh* M, const double2* eo, const SegMap& out, int B) {
-  int ppw = pick_ppw(L);
+  int ppw = pick_ppw(L, B, false);
   dim3 grd((M->half + ppw - 1) / ppw, B);
   size_t lds = (size_t)(M->nm * 2 + 1) * ppw * sizeof(double2);
   hipLaunchKernelGGL(ifft_unfold_kernel<L>, grd, dim3(256), lds, M->stream, eo, out,
