@@ -425,6 +425,8 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         Get_Topography(self._d.grid_geopots)
         self._matrices = {}
         self._ws = None
+        self._param_b = torch.zeros((4, mesh.nθ, mesh.nλ), dtype=torch.float64, device=mesh.device)
+        self.parameter_gradients = {}
 
     def _levels(self):
         return self._nd
@@ -469,10 +471,50 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
                 tuple(t.clone() for t in (d.grid_u_c, d.grid_v_c, d.grid_t_c, d.grid_ps_c)))
 
     def forward(self, state, nsteps):
+        self._init_lnps = state[7].clone()
         return self._run(tuple(state), nsteps)
 
     def backward(self, bars):
-        return self._reverse(tuple(bars))
+        """Cotangents of the final state -> those of the initial state (returned)
+        and of the forcing parameters k_f, k_a, k_s, ΔT_y (in
+        ``parameter_gradients``, keyed like the HS parameter dict)."""
+        self._param_b.zero_()
+        out = self._reverse(tuple(bars))
+        sums = self._param_b.cpu().numpy().reshape(4, -1).sum(axis=1)
+        self.parameter_gradients = dict(zip(("k_f", "k_a", "k_s", "ΔT_y"), map(float, sums)))
+        return out
+
+    def initial_state_gradient(self, grads):
+        """Gradient with respect to the spectral coefficients (vor, div, T, ln ps) of a
+        state whose p-fields equal its c-fields and whose grid p-fields are its grid
+        c-fields (the state Spectral_Initialize_Fields! leaves): the c and p
+        cotangents added, and the grid p-field cotangents taken back through
+        UV_Grid_From_Vor_Div!, the inverse transform and ps = exp(ln ps)."""
+        from .mesh import Trans_Spherical_To_Grid
+        M, nd, h = self.mesh, self._nd, self.mesh.handle
+        gvp, gvc, gdp, gdc, gtp, gtc, glp, glc, gup, gvpg, gtpg, gpsp = grads
+        out = [gvc.clone(), gdc.clone(), gtc.clone(), glc.clone()]
+
+        def add(dst, src):
+            call("jgcm_axpby", h, dst.numel() * dst.element_size() // 8, 1.0, _ptr(dst), 1.0,
+                 _ptr(src), _ptr(dst))
+        for dst, src in zip(out, (gvp, gdp, gtp, glp)):
+            add(dst, src)
+        sv, sd = M.spectral_zeros(nd), M.spectral_zeros(nd)
+        uv_grid_from_vor_div_adjoint(M, gup, gvpg, sv, sd)
+        add(out[0], sv)
+        add(out[1], sd)
+        st = M.spectral_zeros(nd)
+        trans_spherical_to_grid_adjoint(M, gtpg, st)
+        add(out[2], st)
+        ln_g, ps_g, lg, sl = M.grid_zeros(1), M.grid_zeros(1), M.grid_zeros(1), M.spectral_zeros(1)
+        Trans_Spherical_To_Grid(M, self._init_lnps, ln_g, batch=True)
+        call("jgcm_map", h, 0, ln_g.numel(), _ptr(ln_g), 0.0, 0.0, _ptr(ps_g))
+        call("jgcm_xmy", h, lg.numel(), 1.0, _ptr(ps_g), _ptr(gpsp), _ptr(lg), 0)
+        trans_spherical_to_grid_adjoint(M, lg, sl)
+        add(out[3], sl)
+        M.synchronize()
+        return tuple(out)
 
     # ---- reverse ---------------------------------------------------------------
     def _implicit_matrix(self, si):
@@ -601,7 +643,7 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         hp = _hs(self.params)
         call("jgcm_hs_column_adjoint", si.handle, ctypes.byref(hp), float(Get_Δt(I)), Ep_b, mpsp_b,
              arr([div_g, u_g, v_g, t_g, vor_g, tl, tt, gl, gt, u_pg, v_pg, t_pg, ps_pg, ps_c]),
-             arr([du_g, dv_g, dt_g, en_g, dl_g]), arr(outs), _ptr(self._ws))
+             arr([du_g, dv_g, dt_g, en_g, dl_g]), arr(outs), _ptr(self._ws), _ptr(self._param_b))
         # 6. the previous cotangents of the grid p-fields belong to this step's grid c-fields
         add(u_b, Bup)
         add(v_b, Bvpg)

@@ -84,7 +84,8 @@ struct HsColAdj {
   // cotangents out
   double *div_b, *u_b, *v_b, *t_b, *vor_b, *tl_b, *tt_b, *gl_b, *gt_b;
   double *up_b, *vp_b, *tp_b, *psp_b, *psc_b;
-  double* ws;  // [4][nd][P]: Q, S (forward), S_bar, partial Q_bar
+  double* ws;       // [4][nd][P]: Q, S (forward), S_bar, partial Q_bar
+  double* param_b;  // [4][P] or NULL: cotangents of k_f, k_a, k_s, ΔT_y, added per column
 };
 
 struct HsAdjScalars {
@@ -136,6 +137,7 @@ __global__ void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat, int nlon, c
     const double lb = -s.dlnps_b[col];  // S_bar from δln ps = -S
     double Stot_b = lb;
     double glb = 0.0, gtb = 0.0, pscb = 0.0, pspb = 0.0;
+    double kfb = 0.0, kab = 0.0, ksb = 0.0, dtyb = 0.0;  // forcing-parameter cotangents
     double eprefix = 0.0;
     // pass B: level-local terms, m_bar, S_bar, first part of Q_bar
     for (int k = 0; k < nd; ++k) {
@@ -215,7 +217,15 @@ __global__ void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat, int nlon, c
       if (te > t_strat) {
         const double dte = pow(pn, kap - 1.0) * (kap * A0 - c.hp.delta_theta_z * c2) * sg / p_ref;
         pscb = pscb + kt * cpb * dte;
+        dtyb = dtyb - cpb * kt * s2 * pow(pn, kap);  // ∂T_eq/∂ΔT_y = -sin²φ pn^κ
       }
+      // k_v = k_f/day max(0, σ-σ_b)/(1-σ_b);  k_T = k_a/day + (k_s-k_a)/day max(0, σ-σ_b)/(1-σ_b) cos⁴φ
+      const double sp = fmax(0.0, sg - c.hp.sigma_b) / (1.0 - c.hp.sigma_b);
+      const double kvb = -(ap * U) - bp * V + cpb * (1.0 - kv * dt) * (U * U + V * V) / cp;
+      const double ktb = -(cpb * (T - teq));
+      kfb = kfb + kvb * sp / day;
+      kab = kab + ktb * (1.0 - sp * c4) / day;
+      ksb = ksb + ktb * (sp * c4) / day;
       const double ep = 0.5 * (uu * uu + vq * vq) + cp * (T + dt * DT);
       pspb = pspb + c.ep_bar / c.grav * aj * db * ep;
     }
@@ -236,6 +246,12 @@ __global__ void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat, int nlon, c
     s.gt_b[col] = gtb;
     s.psc_b[col] = pscb;
     s.psp_b[col] = pspb + c.mpsp_bar * aj;
+    if (s.param_b) {
+      s.param_b[col] = s.param_b[col] + kfb;
+      s.param_b[P + col] = s.param_b[P + col] + kab;
+      s.param_b[2 * P + col] = s.param_b[2 * P + col] + ksb;
+      s.param_b[3 * P + col] = s.param_b[3 * P + col] + dtyb;
+    }
   }
 }
 
@@ -349,7 +365,8 @@ extern "C" {
 
 int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, double dt,
                            double ep_bar, double mpsp_bar, const double* const* state,
-                           const double* const* cot_in, double* const* cot_out, double* workspace) {
+                           const double* const* cot_in, double* const* cot_out, double* workspace,
+                           double* param_bar) {
   JGCM_REQUIRE(si && hp && state && cot_in && cot_out && workspace, "NULL argument");
   if (int rc = check_even_sigma(si->vc)) return rc;
   jgcm_mesh* M = si->vc->mesh;
@@ -361,7 +378,7 @@ int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, d
              cot_in[0], cot_in[1], cot_in[2], cot_in[3], cot_in[4],
              cot_out[0], cot_out[1], cot_out[2], cot_out[3], cot_out[4], cot_out[5], cot_out[6],
              cot_out[7], cot_out[8], cot_out[9], cot_out[10], cot_out[11], cot_out[12],
-             cot_out[13], workspace};
+             cot_out[13], workspace, param_bar};
   HsAdjScalars c{si->atmo.rdgas, si->atmo.cp_air, si->atmo.kappa, si->atmo.grav,
                  2.0 * si->atmo.omega, dt, ep_bar, mpsp_bar, *hp};
   size_t P = M->grid_elems();

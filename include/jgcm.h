@@ -254,10 +254,13 @@ int jgcm_sw_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* grid_
  *   cot_out[14]: cotangents of div, u, v, T, vor, ∂λT, ∂θT, ∂λ ln ps, ∂θ ln ps,
  *                u_p, v_p, T_p, ps_p, ps_c (same shapes as the state)
  *   ep_bar, mpsp_bar: cotangents of the energy integral and mean ps of the p-fields
- *   workspace  : 4 nd nθ nλ doubles                                          */
+ *   workspace  : 4 nd nθ nλ doubles
+ *   param_bar  : NULL, or [4][nθ][nλ] doubles to which each column adds its share
+ *                of the cotangents of k_f, k_a, k_s and ΔT_y (HS_Forcing.jl:8-13)  */
 int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, double dt,
                            double ep_bar, double mpsp_bar, const double* const* state,
-                           const double* const* cot_in, double* const* cot_out, double* workspace);
+                           const double* const* cot_in, double* const* cot_out, double* workspace,
+                           double* param_bar);
 /* transpose of the mass and energy corrections (Spectral_Dynamics.jl:30-68) with
  * respect to the new-state grid u, v, T and ln ps (T_n and ps_n passed after
  * their corrections; corr, factor = the step's correction scalars).          */

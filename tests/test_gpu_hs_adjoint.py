@@ -163,3 +163,147 @@ def test_hs_gradient_matches_central_differences(jgcm, K, starts_init):
         tol = max(1.0, abs(an))
         assert abs(err[1]) <= 0.05 * abs(err[0]) + 1e-8 * tol, (seed, an, err)
         assert abs(err[2]) <= 1e-6 * tol, (seed, an, err)
+
+
+def mass_weighted_mse(jgcm, mesh, vc, t_spec, lnps_spec, target, want_grad=False):
+    """L = Σ_j a_j ps_j Σ_k Δb_k (T - T*)² / Σ_j a_j ps_j (mass-weighted global mean of
+    (T - T*)², a_j = w_j / (2 nλ)), from the spectral T and ln ps; with want_grad also
+    the cotangents of the spectral T and ln ps."""
+    from jgcm_amd import adjoint as ad
+    nd = vc.nd
+    t_g, ln_g = mesh.grid_zeros(nd), mesh.grid_zeros(1)
+    jgcm.Trans_Spherical_To_Grid(mesh, t_spec, t_g, batch=True)
+    jgcm.Trans_Spherical_To_Grid(mesh, lnps_spec, ln_g, batch=True)
+    a = torch.tensor(np.asarray(mesh.wts) / (2.0 * mesh.nλ), device="cuda")[None, :, None]
+    db = torch.tensor(np.diff(np.linspace(0.0, 1.0, nd + 1)), device="cuda")[:, None, None]
+    ps = torch.exp(ln_g)
+    d = t_g - target
+    col = (db * d * d).sum(0, keepdim=True)
+    D = (a * ps).sum()
+    L = float((a * ps * col).sum() / D)
+    if not want_grad:
+        return L
+    gt = (2.0 * a * ps * db * d / D).contiguous()
+    gps = (a * (col - L) / D).contiguous()
+    st, sl = mesh.spectral_zeros(nd), mesh.spectral_zeros(1)
+    ad.trans_spherical_to_grid_adjoint(mesh, gt, st)
+    ad.trans_spherical_to_grid_adjoint(mesh, (ps * gps).contiguous(), sl)
+    return L, st, sl
+
+
+class T42Case:
+    """T42 L20 Held-Suarez, 48 steps of 600 s from the seeded-noise rest state,
+    checkpoints every 8 steps; loss = mass-weighted global mean of (T - T*)², T*
+    the forward final T plus seeded N(0, 0.5 K) noise."""
+    N, nlat, nd, dt, K = 42, 64, 20, 600, 48
+
+    def __init__(self, jgcm, prec):
+        N, nlat, nd, dt = self.N, self.nlat, self.nd, self.dt
+        nlon = 2 * nlat
+        self.jgcm = jgcm
+        self.mesh = mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, nlon, nd, A, precision=prec)
+        self.vc = vc = jgcm.Vert_Coordinate(nlon, nlat, nd, "even_sigma", "simmons_and_burridge",
+                                            "second_centered_wts", 1e5, mesh=mesh)
+        self.atmo = atmo = jgcm.Atmo_Data("hs", nlon, nlat, nd, True, True, False, False, mesh.sinθ,
+                                          A, OMEGA)
+        I = jgcm.Filtered_Leapfrog(0.04, 4, 1.15741e-4, mesh.laplacian_eig, 0.5, dt, True, 0,
+                                   86400 * 200)
+        self.si = jgcm.Semi_Implicit_Solver(vc, atmo, I, 1e5, np.full(nd, 300.0), mesh.wave_numbers)
+        d = jgcm.Dyn_Data("hs", N, N + 1, nlon, nlat, nd, mesh=mesh)
+        jgcm.Get_Topography(d.grid_geopots)
+        noise = 0.1 * np.random.default_rng(2024).standard_normal((nd, nlat, nlon))
+        jgcm.Spectral_Initialize_Fields(mesh, atmo, vc, 1.0e5, 300.0, d.grid_geopots, d,
+                                        grid_t_noise=dev(noise))
+        self.c0 = [d.spe_vor_c.clone(), d.spe_div_c.clone(), d.spe_t_c.clone(),
+                   d.spe_lnps_c.clone()]
+        adj, fin = self.run(self.c0)
+        t_fin = mesh.grid_zeros(nd)
+        jgcm.Trans_Spherical_To_Grid(mesh, fin[5], t_fin, batch=True)
+        self.target = t_fin + dev(0.5 * np.random.default_rng(7).standard_normal((nd, nlat, nlon)))
+        self.L0, gt, gl = mass_weighted_mse(jgcm, mesh, vc, fin[5], fin[7], self.target,
+                                            want_grad=True)
+        bars = [torch.zeros_like(x) for x in fin]
+        bars[5], bars[7] = gt, gl
+        self.grad = adj.initial_state_gradient(adj.backward(bars))
+        self.param_grad = dict(adj.parameter_gradients)
+
+    def full_state(self, c):
+        jgcm, mesh, nd = self.jgcm, self.mesh, self.nd
+        vor, div, t, lnps = c
+        u, v, tg, lg = mesh.grid_zeros(nd), mesh.grid_zeros(nd), mesh.grid_zeros(nd), mesh.grid_zeros(1)
+        jgcm.UV_Grid_From_Vor_Div(mesh, vor, div, u, v, batch=True)
+        jgcm.Trans_Spherical_To_Grid(mesh, t, tg, batch=True)
+        jgcm.Trans_Spherical_To_Grid(mesh, lnps, lg, batch=True)
+        return (vor, vor, div, div, t, t, lnps, lnps, u, v, tg, torch.exp(lg))
+
+    def run(self, c, params=HS):
+        adj = self.jgcm.Held_Suarez_Adjoint(self.mesh, self.atmo, self.vc, self.si, params,
+                                            checkpoint_every=8)
+        return adj, adj.forward(self.full_state(c), self.K)
+
+    def loss(self, c, params=HS):
+        _, f = self.run(c, params)
+        return mass_weighted_mse(self.jgcm, self.mesh, self.vc, f[5], f[7], self.target)
+
+
+def test_hs_loss_gradient_t42_l20_matches_central_differences(jgcm):
+    """FP64: the gradient with respect to the initial vor/div/T/ln ps coefficients
+    matches central differences (relative step 1e-6) along 16 seeded unit
+    directions to 1e-6; the gradients with respect to k_f, k_a, k_s, ΔT_y
+    match them to 2e-6."""
+    case = T42Case(jgcm, "fp64")
+    N, c0, g = case.N, case.c0, case.grad
+    rng = np.random.default_rng(16)
+    scale = [float(x.abs().max()) for x in c0]
+    worst = 0.0
+    for it in range(16):
+        dirs = [dev(rspec(N, x.shape[0], rng)) for x in c0]
+        nrm = np.sqrt(sum(dot(dk, dk) for dk in dirs))
+        # unit direction, each field scaled to the state's size
+        dirs = [dk / nrm * s for dk, s in zip(dirs, scale)]
+        h = 1e-6
+        Lp = case.loss([x + h * dk for x, dk in zip(c0, dirs)])
+        Lm = case.loss([x - h * dk for x, dk in zip(c0, dirs)])
+        fd = (Lp - Lm) / (2 * h)
+        an = sum(dot(gk, dk) for gk, dk in zip(g, dirs))
+        worst = max(worst, abs(fd - an) / max(abs(an), 1e-300))
+    assert worst <= 1e-6, worst
+    # the loss is far less sensitive to the forcing parameters (k_f: 3e-6 per
+    # unit): at a relative step of 1e-6 the quotient is rounding-limited, so the
+    # steps are 1e-3 and 1e-4 relative (truncation error 1e-6 and 1e-8 measured)
+    for key in ("k_f", "k_a", "k_s", "ΔT_y"):
+        errs = []
+        for rel in (1e-3, 1e-4):
+            h = rel * HS[key]
+            Lp = case.loss(c0, dict(HS, **{key: HS[key] + h}))
+            Lm = case.loss(c0, dict(HS, **{key: HS[key] - h}))
+            errs.append(abs((Lp - Lm) / (2 * h) - case.param_grad[key]))
+        assert min(errs) <= 2e-6 * abs(case.param_grad[key]), (key, errs, case.param_grad[key])
+
+
+def test_hs_loss_gradient_split32_matches_fp64(jgcm):
+    """The split-FP32 adjoint of the same run against the FP64 gradient (the
+    reference gradient; the split forward is not smooth at a 1e-6 relative
+    step, so central differences do not apply to it): per field, relative
+    error in the max norm."""
+    ref = T42Case(jgcm, "fp64")
+    sp = T42Case(jgcm, "split32")
+    N = ref.N
+    # compare on the coefficients a state of real fields has: n <= N, n >= m and
+    # m = 0 real.  Degree N+1 is never excited (every forward transform zeroes
+    # it) and, without the wave-matrix solve there, its cotangent grows by many
+    # orders over 48 steps in both precisions.
+    mask = np.zeros((N + 2, N + 1), dtype=bool)
+    for m in range(N + 1):
+        mask[m:N + 1, m] = True
+    errs = []
+    for a, b in zip(sp.grad, ref.grad):
+        a, b = a.cpu().numpy()[:, mask], b.cpu().numpy()[:, mask]
+        errs.append(float(np.abs(a - b).max() / np.abs(b).max()))
+    assert max(errs) <= 1e-5, errs
+    # dL/dk_f (3e-6 per unit) is a small residue of cancelling column
+    # contributions, so it carries the split precision's ~1e-7 state error
+    # amplified (5e-4 relative measured); the others agree to a few 1e-6
+    tol = {"k_f": 1e-3, "k_a": 1e-5, "k_s": 1e-5, "ΔT_y": 1e-5}
+    for key, v in ref.param_grad.items():
+        assert abs(sp.param_grad[key] - v) <= tol[key] * abs(v), (key, sp.param_grad[key], v)
