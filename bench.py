@@ -187,6 +187,113 @@ def ensemble_bench(jgcm, torch, steps=100):
             "finite": finite}
 
 
+def adjoint_bench(jgcm, torch, precision, n_fd=16):
+    """Reverse mode of T42 L20 Held-Suarez over 48 steps of 600 s with
+    checkpoints every 8 steps: loss = mass-weighted global mean of (T - T*)²,
+    T* = the forward final T plus seeded N(0, 0.5 K) noise; gradient with
+    respect to the initial vor/div/T/ln ps coefficients and k_f, k_a, k_s, ΔT_y.
+    Reports the forward (with checkpoints) and reverse wall times and, for FP64,
+    the worst relative error against central differences (relative step 1e-6)
+    along n_fd seeded unit directions."""
+    import time
+    N, nlat, nd, dt, K = 42, 64, 20, 600, 48
+    nlon = 2 * nlat
+    a = 6371000.0
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, nlon, nd, a, precision=precision)
+    vc = jgcm.Vert_Coordinate(nlon, nlat, nd, "even_sigma", "simmons_and_burridge",
+                              "second_centered_wts", 1e5, mesh=mesh)
+    atmo = jgcm.Atmo_Data("hs", nlon, nlat, nd, True, True, False, False, mesh.sinθ, a, 7.292e-5)
+    integ = jgcm.Filtered_Leapfrog(0.04, 4, 1.15741e-4, mesh.laplacian_eig, 0.5, dt, True, 0,
+                                   200 * 86400)
+    si = jgcm.Semi_Implicit_Solver(vc, atmo, integ, 1e5, np.full(nd, 300.0), mesh.wave_numbers)
+    d = jgcm.Dyn_Data("hs", N, N + 1, nlon, nlat, nd, mesh=mesh)
+    jgcm.Get_Topography(d.grid_geopots)
+    noise = torch.from_numpy(0.1 * np.random.default_rng(2024).standard_normal((nd, nlat, nlon))).cuda()
+    jgcm.Spectral_Initialize_Fields(mesh, atmo, vc, 1.0e5, 300.0, d.grid_geopots, d,
+                                    grid_t_noise=noise)
+    c0 = [d.spe_vor_c.clone(), d.spe_div_c.clone(), d.spe_t_c.clone(), d.spe_lnps_c.clone()]
+    wa = torch.tensor(np.asarray(mesh.wts) / (2.0 * nlon), device="cuda")[None, :, None]
+    db = torch.tensor(np.diff(np.linspace(0.0, 1.0, nd + 1)), device="cuda")[:, None, None]
+
+    def full_state(c):
+        vor, div, t, lnps = c
+        u, v, tg, lg = mesh.grid_zeros(nd), mesh.grid_zeros(nd), mesh.grid_zeros(nd), mesh.grid_zeros(1)
+        jgcm.UV_Grid_From_Vor_Div(mesh, vor, div, u, v, batch=True)
+        jgcm.Trans_Spherical_To_Grid(mesh, t, tg, batch=True)
+        jgcm.Trans_Spherical_To_Grid(mesh, lnps, lg, batch=True)
+        return (vor, vor, div, div, t, t, lnps, lnps, u, v, tg, torch.exp(lg))
+
+    def grids(f):
+        tg, lg = mesh.grid_zeros(nd), mesh.grid_zeros(1)
+        jgcm.Trans_Spherical_To_Grid(mesh, f[5], tg, batch=True)
+        jgcm.Trans_Spherical_To_Grid(mesh, f[7], lg, batch=True)
+        return tg, torch.exp(lg)
+
+    def loss(f, target, grad=False):
+        tg, ps = grids(f)
+        dd = tg - target
+        col = (db * dd * dd).sum(0, keepdim=True)
+        D = (wa * ps).sum()
+        L = float((wa * ps * col).sum() / D)
+        if not grad:
+            return L
+        from jgcm_amd.adjoint import trans_spherical_to_grid_adjoint as t2g_T
+        st, sl = mesh.spectral_zeros(nd), mesh.spectral_zeros(1)
+        t2g_T(mesh, (2.0 * wa * ps * db * dd / D).contiguous(), st)
+        t2g_T(mesh, (ps * wa * (col - L) / D).contiguous(), sl)
+        bars = [torch.zeros_like(x) for x in f]
+        bars[5], bars[7] = st, sl
+        return L, bars
+
+    def run(c):
+        adj = jgcm.Held_Suarez_Adjoint(mesh, atmo, vc, si, HS_PARAMS, checkpoint_every=8)
+        return adj, adj.forward(full_state(c), K)
+
+    adj, fin = run(c0)
+    tg, _ = grids(fin)
+    target = tg + torch.from_numpy(0.5 * np.random.default_rng(7).standard_normal((nd, nlat, nlon))).cuda()
+    L0, bars = loss(fin, target, grad=True)
+    adj.backward(bars)            # builds the implicit-correction matrices once
+    mesh.synchronize()
+    t0 = time.perf_counter()
+    adj, fin = run(c0)
+    mesh.synchronize()
+    t1 = time.perf_counter()
+    g = adj.initial_state_gradient(adj.backward(bars))
+    mesh.synchronize()
+    t2 = time.perf_counter()
+    out = {"config": "T42 L20 (128x64), 48 steps of 600 s, checkpoints every 8",
+           "loss": L0, "forward_ms": (t1 - t0) * 1e3, "reverse_ms": (t2 - t1) * 1e3,
+           "reverse_ms_per_step": (t2 - t1) * 1e3 / K,
+           "parameter_gradients": adj.parameter_gradients}
+    if precision == "fp64" and n_fd:
+        def dot(x, y):
+            x, y = x.cpu().numpy(), y.cpu().numpy()
+            return float(np.sum(x.real * y.real + x.imag * y.imag))
+        rng = np.random.default_rng(16)
+        scale = [float(x.abs().max()) for x in c0]
+        worst = 0.0
+        for _ in range(n_fd):
+            dirs = []
+            for x in c0:
+                s = rng.standard_normal(x.shape) + 1j * rng.standard_normal(x.shape)
+                for m in range(N + 1):
+                    s[:, :m, m] = 0
+                s[:, :, 0] = s[:, :, 0].real
+                s[:, N + 1, :] = 0
+                dirs.append(torch.tensor(s, device="cuda"))
+            nrm = np.sqrt(sum(dot(x, x) for x in dirs))
+            dirs = [x / nrm * sc for x, sc in zip(dirs, scale)]
+            h = 1e-6
+            Lp = loss(run([x + h * y for x, y in zip(c0, dirs)])[1], target)
+            Lm = loss(run([x - h * y for x, y in zip(c0, dirs)])[1], target)
+            an = sum(dot(x, y) for x, y in zip(g, dirs))
+            worst = max(worst, abs((Lp - Lm) / (2 * h) - an) / abs(an))
+        out["fd_directions"] = n_fd
+        out["fd_max_rel_err"] = worst
+    return out
+
+
 def traffic_from_profile(direction, precision):
     """HBM bytes of one transform launch (both kernels) as measured by the
     committed PMC summary (FETCH_SIZE and WRITE_SIZE, calibrated against a
@@ -329,6 +436,7 @@ def main():
         hs["split32_speedup_vs_fp64"] = hs["fp64"]["ms_per_step"] / hs["split32"]["ms_per_step"]
         out["held_suarez"] = hs
         out["ensemble"] = ensemble_bench(jgcm, torch)
+        out["adjoint"] = {p: adjoint_bench(jgcm, torch, p) for p in ("fp64", "split32")}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         if not a.no_hs:

@@ -68,8 +68,8 @@ SIGNATURES = {
     "jgcm_damped_leapfrog_adjoint": [P, I, I, P, P, P, P, P, I],
     "jgcm_barotropic_step_adjoint": [P, I, P, P, P, D, P, P, I],
     "jgcm_sw_step_adjoint": [P, I, P, P, P, P, P, P, D, D, D, D, P, P, P, P, P, P, I],
-    "jgcm_hs_column_adjoint": [P, P, D, D, D, P, P, P, P, P],
-    "jgcm_hs_corrections_adjoint": [P, P, P, P, P, D, D, D, D, P, P, P, P],
+    "jgcm_hs_column_adjoint": [P, P, D, P, P, P, P, P, P],
+    "jgcm_hs_corrections_adjoint": [P, P, P, P, P, P, P, P, P, P, P, P, P],
     "jgcm_implicit_adjoint": [P, I, P, P, P, P, P],
     "jgcm_gradient_cos_adjoint": [P, P, P, P, I, I],
     "jgcm_divide_by_cos": [P, P, I],
@@ -120,6 +120,7 @@ SIGNATURES = {
     "jgcm_hs_column_dynamics": [P] * 27,
     "jgcm_hs_corrections": [P] * 8,
     "jgcm_hs_scalars": [P, P, I],
+    "jgcm_hs_scalars_device": [P, P, I],
     "jgcm_zonal_mean_accumulate": [P, P, P, P, P, P, I, P, P, P, P, P],
     "jgcm_mesh_reserve": [P, I],
     "jgcm_graph_begin": [P],

@@ -399,7 +399,7 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
     applying the forward to unit vectors), the transposed transforms and
     ``jgcm_hs_column_adjoint`` / ``jgcm_hs_corrections_adjoint`` for the grid
     columns and the global corrections (whose scalar algebra runs on the
-    host from the seven correction scalars the forward step leaves).
+    device from the seven correction scalars the forward step leaves).
     """
 
     STATE = ("spe_vor_p", "spe_vor_c", "spe_div_p", "spe_div_c", "spe_t_p", "spe_t_c",
@@ -467,8 +467,9 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
 
     def _snapshot_after(self):
         d = self._d
-        return (self._last.scalars(),
-                tuple(t.clone() for t in (d.grid_u_c, d.grid_v_c, d.grid_t_c, d.grid_ps_c)))
+        sc = torch.zeros(7, dtype=torch.float64, device=self.mesh.device)
+        call("jgcm_hs_scalars_device", self._last.handle, _ptr(sc), 7)
+        return sc, tuple(t.clone() for t in (d.grid_u_c, d.grid_v_c, d.grid_t_c, d.grid_ps_c))
 
     def forward(self, state, nsteps):
         self._init_lnps = state[7].clone()
@@ -558,7 +559,6 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         vor_p, vor_c, div_p, div_c, t_p, t_c, lnps_p, lnps_c, u_pg, v_pg, t_pg, ps_pg = st
         Bvp, Bvc, Bdp, Bdc, Btp, Btc, Blp, Blc, Bup, Bvpg, Btpg, Bpsp = bars
         si, I = self._solver(k), self._integrator(k)
-        g, cp = atmo.grav, atmo.cp_air
         S, G = M.spectral_zeros, M.grid_zeros
 
         def add(dst, src):          # dst += src (same shape), on the mesh stream
@@ -575,22 +575,14 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
             trans_grid_to_spherical_adjoint(M, spec, out)
             return out
 
-        # 1. mass and energy corrections (Spectral_Dynamics.jl:30-68)
-        mps_p, E_p, mps_n, f, _, E_n, corr = (float(x) for x in sc)
+        # 1. mass and energy corrections (Spectral_Dynamics.jl:30-68); the scalar
+        #    cotangents stay on the device (no wait for the stream)
         u_n, v_n, t_nc, ps_nc = nxt
-        Ep_b = mpsp_b = 0.0
+        scal_bar = torch.zeros(4, dtype=torch.float64, device=M.device)
         if self._corrections:
-            corr_b = float(Btc[:, 0, 0].real.cpu().numpy().sum())
-            f_b = float(Blc[0, 0, 0].real.cpu()) / f
-            En_b = -corr_b * g / (cp * mps_p)
-            Ep_b = corr_b * g / (cp * mps_p)
-            mpsp_b = -corr_b * corr / mps_p
-            f_b += En_b * E_n / f
-            mpsp_b += f_b / mps_n
-            mpsn_b = -f_b * f / mps_n
             ub, vb, tb, lb = G(nd), G(nd), G(nd), G(1)
-            call("jgcm_hs_corrections_adjoint", si.handle, *map(_ptr, (u_n, v_n, t_nc, ps_nc)), corr,
-                 f, En_b, mpsn_b, *map(_ptr, (ub, vb, tb, lb)))
+            call("jgcm_hs_corrections_adjoint", si.handle, *map(_ptr, (
+                sc, Btc, Blc, u_n, v_n, t_nc, ps_nc, scal_bar, ub, vb, tb, lb)))
             add(Btc, t2g_T(tb, nd))
             add(Blc, t2g_T(lb, 1))
             sv, sd = S(nd), S(nd)
@@ -641,7 +633,7 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
             self._ws = torch.zeros(4 * nd * M.nθ * M.nλ, dtype=torch.float64, device=M.device)
         arr = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
         hp = _hs(self.params)
-        call("jgcm_hs_column_adjoint", si.handle, ctypes.byref(hp), float(Get_Δt(I)), Ep_b, mpsp_b,
+        call("jgcm_hs_column_adjoint", si.handle, ctypes.byref(hp), float(Get_Δt(I)), _ptr(scal_bar),
              arr([div_g, u_g, v_g, t_g, vor_g, tl, tt, gl, gt, u_pg, v_pg, t_pg, ps_pg, ps_c]),
              arr([du_g, dv_g, dt_g, en_g, dl_g]), arr(outs), _ptr(self._ws), _ptr(self._param_b))
         # 6. the previous cotangents of the grid p-fields belong to this step's grid c-fields

@@ -89,7 +89,8 @@ struct HsColAdj {
 };
 
 struct HsAdjScalars {
-  double rdgas, cp_air, kappa, grav, two_omega, dt, ep_bar, mpsp_bar;
+  double rdgas, cp_air, kappa, grav, two_omega, dt;
+  const double* scal_bar;  // device: [1] energy_p cotangent, [2] mean ps_p cotangent
   jgcm_hs_params hp;
 };
 
@@ -111,6 +112,7 @@ __global__ void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat, int nlon, c
   double* wSb = s.ws + (size_t)2 * nd * P;
   double* wQb = s.ws + (size_t)3 * nd * P;
   const double R = c.rdgas, kap = c.kappa, cp = c.cp_air, dt = c.dt;
+  const double ep_bar = c.scal_bar[1], mpsp_bar = c.scal_bar[2];
   HS_GRID_STRIDE(col, P) {
     const int j = (int)(col / nlon);
     const double f = c.two_omega * sint[j];
@@ -209,7 +211,7 @@ __global__ void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat, int nlon, c
       const double g2 = kv * (1.0 - 0.5 * kv * dt);
       const double DT = g2 * (U * U + V * V) / cp - kt * (T - teq);
       const double uu = U + dt * (-kv * U), vq = V + dt * (-kv * V);
-      const double eb = c.ep_bar / c.grav * aj * psp * db;
+      const double eb = ep_bar / c.grav * aj * psp * db;
       const double ap = a + eb * dt * uu, bp = b + eb * dt * vq, cpb = cc + cp * dt * eb;
       s.up_b[o] = eb * uu - kv * ap + 2.0 * g2 * U * cpb / cp;
       s.vp_b[o] = eb * vq - kv * bp + 2.0 * g2 * V * cpb / cp;
@@ -227,7 +229,7 @@ __global__ void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat, int nlon, c
       kab = kab + ktb * (1.0 - sp * c4) / day;
       ksb = ksb + ktb * (sp * c4) / day;
       const double ep = 0.5 * (uu * uu + vq * vq) + cp * (T + dt * DT);
-      pspb = pspb + c.ep_bar / c.grav * aj * db * ep;
+      pspb = pspb + ep_bar / c.grav * aj * db * ep;
     }
     // pass C (bottom up): Q_bar = partial + Δb_k (Σ_{k'>k} S_bar + S_bar_tot)
     double suffix = 0.0;
@@ -245,7 +247,7 @@ __global__ void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat, int nlon, c
     s.gl_b[col] = glb;
     s.gt_b[col] = gtb;
     s.psc_b[col] = pscb;
-    s.psp_b[col] = pspb + c.mpsp_bar * aj;
+    s.psp_b[col] = pspb + mpsp_bar * aj;
     if (s.param_b) {
       s.param_b[col] = s.param_b[col] + kfb;
       s.param_b[P + col] = s.param_b[P + col] + kab;
@@ -263,12 +265,13 @@ __global__ void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat, int nlon, c
 // (ps_n = exp(ln ps grid)).
 __global__ void hs_corrections_adjoint_kernel(VC v, int nlat, int nlon, const double* wts,
                                               const double* u_n, const double* v_n,
-                                              const double* t_nc, const double* ps_nc, double corr,
-                                              double f, double en_bar, double mpsn_bar, double grav,
-                                              double cp, double* un_b, double* vn_b, double* tn_b,
-                                              double* lnps_b) {
+                                              const double* t_nc, const double* ps_nc,
+                                              const double* scal, const double* scal_bar,
+                                              double grav, double cp, double* un_b, double* vn_b,
+                                              double* tn_b, double* lnps_b) {
   const size_t P = (size_t)nlat * nlon;
   const int nd = v.nd;
+  const double corr = scal[6], f = scal[3], en_bar = scal_bar[0], mpsn_bar = scal_bar[3];
   HS_GRID_STRIDE(col, P) {
     const int j = (int)(col / nlon);
     const double aj = wts[j] / (2.0 * nlon);
@@ -288,6 +291,32 @@ __global__ void hs_corrections_adjoint_kernel(VC v, int nlat, int nlon, const do
     const double psb = en_bar * f * aj * acc / grav + mpsn_bar * aj;
     lnps_b[col] = psn * psb;
   }
+}
+
+// Scalar algebra of the corrections' transpose (SD:30-68), one thread:
+//   corr = g (E_p - E_n) / (c_p mps_p),  f = mps_p / mps_n,
+//   T_c'(k, 0, 0) = T_n(k, 0, 0) + corr,  ln ps_c'(0, 0, 0) = ln ps_n(0, 0, 0) + log f,
+//   E_n depends on f through ps_n' = f ps_n  (∂E_n/∂f = E_n / f).
+// scal: the forward's slots (0 mps_p, 1 E_p, 2 mps_n, 3 f, 5 E_n, 6 corr);
+// out: 0 E_n_bar, 1 E_p_bar, 2 mps_p_bar, 3 mps_n_bar.
+__global__ void hs_correction_scalars_adjoint_kernel(const double* scal, const double2* t_c_bar,
+                                                     const double2* lnps_c_bar, int nd,
+                                                     size_t spec_per_level, double grav, double cp,
+                                                     double* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const double mps_p = scal[0], mps_n = scal[2], f = scal[3], e_n = scal[5], corr = scal[6];
+  double corr_b = 0.0;
+  for (int k = 0; k < nd; ++k) corr_b = corr_b + t_c_bar[(size_t)k * spec_per_level].x;
+  double f_b = lnps_c_bar[0].x / f;
+  const double en_b = -corr_b * grav / (cp * mps_p);
+  const double ep_b = corr_b * grav / (cp * mps_p);
+  double mpsp_b = -corr_b * corr / mps_p;
+  f_b = f_b + en_b * e_n / f;
+  mpsp_b = mpsp_b + f_b / mps_n;
+  out[0] = en_b;
+  out[1] = ep_b;
+  out[2] = mpsp_b;
+  out[3] = -f_b * f / mps_n;
 }
 
 // out[i][n][m] = Σ_o A_n[o][i] in[o][n][m] for n >= m (zero below the
@@ -312,6 +341,43 @@ __global__ void implicit_adjoint_kernel(const double* A, int nd, int nm, const d
           const double w = An[(size_t)o * nin + i];
           acc = make_double2(acc.x + w * x.x, acc.y + w * x.y);
         }
+      }
+      out[(size_t)i * spec_per_level + idx] = acc;
+    }
+  }
+}
+
+// Same map with one workgroup per degree n: A_n staged in LDS (shared by every
+// m of the row), the 2nd+1 input cotangents of the thread's m in registers.
+// For nout <= NOUT_MAX.
+template <int NOUT_MAX>
+__global__ __launch_bounds__(128) void implicit_adjoint_row_kernel(
+    const double* A, int nd, int nm, const double2* ddiv_b, const double2* dlnps_b,
+    const double2* dt_b, double2* out, size_t spec_per_level) {
+  extern __shared__ double An[];  // [nout][nin]
+  const int nin = 7 * nd + 3, nout = 2 * nd + 1;
+  const int n = blockIdx.x;
+  const double* src = A + (size_t)n * nout * nin;
+  for (int q = threadIdx.x; q < nout * nin; q += blockDim.x) An[q] = src[q];
+  __syncthreads();
+  for (int m = threadIdx.x; m < nm; m += blockDim.x) {
+    const size_t idx = (size_t)n * nm + m;
+    double2 x[NOUT_MAX];
+#pragma unroll
+    for (int o = 0; o < NOUT_MAX; ++o) {
+      if (o >= nout) break;
+      x[o] = (n < m) ? make_double2(0.0, 0.0)
+                     : (o < nd ? ddiv_b[(size_t)o * spec_per_level + idx]
+                               : (o == nd ? dlnps_b[idx]
+                                          : dt_b[(size_t)(o - nd - 1) * spec_per_level + idx]));
+    }
+    for (int i = 0; i < nin; ++i) {
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int o = 0; o < NOUT_MAX; ++o) {
+        if (o >= nout) break;
+        const double w = An[o * nin + i];
+        acc = make_double2(acc.x + w * x[o].x, acc.y + w * x[o].y);
       }
       out[(size_t)i * spec_per_level + idx] = acc;
     }
@@ -364,10 +430,10 @@ using namespace jgcm;
 extern "C" {
 
 int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, double dt,
-                           double ep_bar, double mpsp_bar, const double* const* state,
+                           const double* scal_bar, const double* const* state,
                            const double* const* cot_in, double* const* cot_out, double* workspace,
                            double* param_bar) {
-  JGCM_REQUIRE(si && hp && state && cot_in && cot_out && workspace, "NULL argument");
+  JGCM_REQUIRE(si && hp && scal_bar && state && cot_in && cot_out && workspace, "NULL argument");
   if (int rc = check_even_sigma(si->vc)) return rc;
   jgcm_mesh* M = si->vc->mesh;
   for (int i = 0; i < 14; ++i) JGCM_REQUIRE(state[i], "NULL state array %d", i);
@@ -380,29 +446,34 @@ int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, d
              cot_out[7], cot_out[8], cot_out[9], cot_out[10], cot_out[11], cot_out[12],
              cot_out[13], workspace, param_bar};
   HsAdjScalars c{si->atmo.rdgas, si->atmo.cp_air, si->atmo.kappa, si->atmo.grav,
-                 2.0 * si->atmo.omega, dt, ep_bar, mpsp_bar, *hp};
+                 2.0 * si->atmo.omega, dt, scal_bar, *hp};
   size_t P = M->grid_elems();
-  hipLaunchKernelGGL(hs_column_adjoint_kernel, dim3(nblk_h(P)), dim3(256), 0, M->stream, s,
-                     vc_of(si->vc), M->nlat, M->nlon, M->d_sin, M->d_wts, c);
+  // one thread per column: 64-thread workgroups spread the columns over the CUs
+  hipLaunchKernelGGL(hs_column_adjoint_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), 0,
+                     M->stream, s, vc_of(si->vc), M->nlat, M->nlon, M->d_sin, M->d_wts, c);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
 
-int jgcm_hs_corrections_adjoint(jgcm_semi_implicit_t* si, const double* grid_u_n,
-                                const double* grid_v_n, const double* grid_t_n,
-                                const double* grid_ps_n, double corr, double factor, double en_bar,
-                                double mpsn_bar, double* u_n_bar, double* v_n_bar, double* t_n_bar,
+int jgcm_hs_corrections_adjoint(jgcm_semi_implicit_t* si, const double* scal,
+                                const double* spe_t_c_bar, const double* spe_lnps_c_bar,
+                                const double* grid_u_n, const double* grid_v_n,
+                                const double* grid_t_n, const double* grid_ps_n, double* scal_bar,
+                                double* u_n_bar, double* v_n_bar, double* t_n_bar,
                                 double* lnps_n_bar) {
-  JGCM_REQUIRE(si && grid_u_n && grid_v_n && grid_t_n && grid_ps_n && u_n_bar && v_n_bar &&
-                   t_n_bar && lnps_n_bar, "NULL argument");
-  JGCM_REQUIRE(factor > 0.0, "mass factor must be positive");
+  JGCM_REQUIRE(si && scal && spe_t_c_bar && spe_lnps_c_bar && grid_u_n && grid_v_n && grid_t_n &&
+                   grid_ps_n && scal_bar && u_n_bar && v_n_bar && t_n_bar && lnps_n_bar,
+               "NULL argument");
   if (int rc = check_even_sigma(si->vc)) return rc;
   jgcm_mesh* M = si->vc->mesh;
+  hipLaunchKernelGGL(hs_correction_scalars_adjoint_kernel, dim3(1), dim3(64), 0, M->stream, scal,
+                     (const double2*)spe_t_c_bar, (const double2*)spe_lnps_c_bar, si->nd,
+                     M->spec_elems(), si->atmo.grav, si->atmo.cp_air, scal_bar);
   size_t P = M->grid_elems();
   hipLaunchKernelGGL(hs_corrections_adjoint_kernel, dim3(nblk_h(P)), dim3(256), 0, M->stream,
                      vc_of(si->vc), M->nlat, M->nlon, M->d_wts, grid_u_n, grid_v_n, grid_t_n,
-                     grid_ps_n, corr, factor, en_bar, mpsn_bar, si->atmo.grav, si->atmo.cp_air,
-                     u_n_bar, v_n_bar, t_n_bar, lnps_n_bar);
+                     grid_ps_n, scal, scal_bar, si->atmo.grav, si->atmo.cp_air, u_n_bar, v_n_bar,
+                     t_n_bar, lnps_n_bar);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
@@ -413,9 +484,17 @@ int jgcm_implicit_adjoint(jgcm_mesh_t* mesh, int nd, const double* matrix, const
   JGCM_REQUIRE(nd >= 1 && nd <= MAXND, "nd must be in 1..%d, got %d", MAXND, nd);
   JGCM_REQUIRE(matrix && ddiv_bar && dlnps_bar && dt_bar && out, "NULL argument");
   size_t S = mesh->spec_elems();
-  hipLaunchKernelGGL(implicit_adjoint_kernel, dim3(nblk_h(S)), dim3(256), 0, mesh->stream, matrix, nd,
-                     mesh->nm, (const double2*)ddiv_bar, (const double2*)dlnps_bar,
-                     (const double2*)dt_bar, (double2*)out, S);
+  const size_t lds = (size_t)(2 * nd + 1) * (7 * nd + 3) * sizeof(double);
+  if (nd <= 24 && lds <= 64 * 1024) {
+    // one workgroup per degree, the row's matrix in LDS
+    hipLaunchKernelGGL(implicit_adjoint_row_kernel<49>, dim3(mesh->ns + 1), dim3(128), lds,
+                       mesh->stream, matrix, nd, mesh->nm, (const double2*)ddiv_bar,
+                       (const double2*)dlnps_bar, (const double2*)dt_bar, (double2*)out, S);
+  } else {
+    hipLaunchKernelGGL(implicit_adjoint_kernel, dim3(nblk_h(S)), dim3(256), 0, mesh->stream, matrix,
+                       nd, mesh->nm, (const double2*)ddiv_bar, (const double2*)dlnps_bar,
+                       (const double2*)dt_bar, (double2*)out, S);
+  }
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

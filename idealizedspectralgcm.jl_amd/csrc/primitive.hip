@@ -1354,6 +1354,14 @@ int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, doubl
   return JGCM_OK;
 }
 
+int jgcm_hs_scalars_device(jgcm_semi_implicit_t* si, double* dev_out, int count) {
+  JGCM_REQUIRE(si && dev_out && count >= 1 && count <= 16, "bad argument");
+  jgcm_mesh* M = si->vc->mesh;
+  JGCM_CHECK_HIP(hipMemcpyAsync(dev_out, si->d_scal, count * sizeof(double), hipMemcpyDeviceToDevice,
+                                M->stream));
+  return JGCM_OK;
+}
+
 int jgcm_hs_scalars(jgcm_semi_implicit_t* si, double* host_out, int count) {
   JGCM_REQUIRE(si && host_out && count >= 1 && count <= 16, "bad argument");
   jgcm_mesh* M = si->vc->mesh;

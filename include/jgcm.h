@@ -253,21 +253,27 @@ int jgcm_sw_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* grid_
  *   cot_in[5]  : cotangents of the grid δu, δv, δT, Φ+KE [nd] and δln ps [1]
  *   cot_out[14]: cotangents of div, u, v, T, vor, ∂λT, ∂θT, ∂λ ln ps, ∂θ ln ps,
  *                u_p, v_p, T_p, ps_p, ps_c (same shapes as the state)
- *   ep_bar, mpsp_bar: cotangents of the energy integral and mean ps of the p-fields
+ *   scal_bar   : device; [1], [2] = cotangents of the energy integral and mean ps
+ *                of the p-fields (from jgcm_hs_corrections_adjoint)
  *   workspace  : 4 nd nθ nλ doubles
  *   param_bar  : NULL, or [4][nθ][nλ] doubles to which each column adds its share
  *                of the cotangents of k_f, k_a, k_s and ΔT_y (HS_Forcing.jl:8-13)  */
 int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, double dt,
-                           double ep_bar, double mpsp_bar, const double* const* state,
+                           const double* scal_bar, const double* const* state,
                            const double* const* cot_in, double* const* cot_out, double* workspace,
                            double* param_bar);
-/* transpose of the mass and energy corrections (Spectral_Dynamics.jl:30-68) with
- * respect to the new-state grid u, v, T and ln ps (T_n and ps_n passed after
- * their corrections; corr, factor = the step's correction scalars).          */
-int jgcm_hs_corrections_adjoint(jgcm_semi_implicit_t* si, const double* grid_u_n,
-                                const double* grid_v_n, const double* grid_t_n,
-                                const double* grid_ps_n, double corr, double factor, double en_bar,
-                                double mpsn_bar, double* u_n_bar, double* v_n_bar, double* t_n_bar,
+/* transpose of the mass and energy corrections (Spectral_Dynamics.jl:30-68).
+ * scal: the step's seven correction scalars (device, jgcm_hs_scalars_device);
+ * spe_t_c_bar, spe_lnps_c_bar: cotangents of the corrected new T and ln ps.
+ * Writes scal_bar (device, 4 doubles: energy_n, energy_p, mean ps_p, mean ps_n
+ * cotangents; the last three are read by jgcm_hs_column_adjoint as [1], [2])
+ * and the cotangents of the new-state grid u, v, T and ln ps (T_n and ps_n
+ * passed after their corrections).                                          */
+int jgcm_hs_corrections_adjoint(jgcm_semi_implicit_t* si, const double* scal,
+                                const double* spe_t_c_bar, const double* spe_lnps_c_bar,
+                                const double* grid_u_n, const double* grid_v_n,
+                                const double* grid_t_n, const double* grid_ps_n, double* scal_bar,
+                                double* u_n_bar, double* v_n_bar, double* t_n_bar,
                                 double* lnps_n_bar);
 /* transpose of ∇²(energy); δdiv -= energy; Implicit_Correction! (Spectral_Dynamics.jl:309-316,
  * Semi_Implicit.jl:462-513) from its per-n matrix [ns+1][2nd+1][7nd+3] (outputs
@@ -483,6 +489,8 @@ int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, doubl
 /* device correction scalars to the host: mean_ps_p, mean_energy_p, mean_ps_n,
  * factor, log(factor), mean_energy_n, temperature correction (synchronises) */
 int jgcm_hs_scalars(jgcm_semi_implicit_t* si, double* host_out, int count);
+/* The same scalars copied to device memory on the mesh stream (no wait).    */
+int jgcm_hs_scalars_device(jgcm_semi_implicit_t* si, double* dev_out, int count);
 
 /* ---- diagnostics -------------------------------------------------------------
  * Update_Output!(output_manager, dyn_data, current_time)  Output_Manager.jl:81-107:
