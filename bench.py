@@ -187,6 +187,27 @@ def ensemble_bench(jgcm, torch, steps=100):
             "finite": finite}
 
 
+def galewsky_bench(jgcm, torch, days=6):
+    """Shallow-water T85 (256x128) Galewsky jet, Δt = 600 s, 6 days, unforced;
+    later steps replayed from one HIP graph.  Per Legendre precision: device
+    ms/step and sim-days/day; per day, the relative L2 error of grid
+    vorticity, divergence and geopotential against the FP64 run (the FP64
+    device path equals the FP64 oracle step for step to 1e-11)."""
+    from jgcm_amd import experiments as X
+    r = X.precision_track(days=days, precisions=("split32", "bf16x3", "fp32"))
+    out = {"config": f"T85 (256x128) shallow water, Galewsky jet + bump, dt 600 s, {days} days",
+           "max_vor_by_day": r["fp64"]["max_vor_by_day"]}
+    for prec, v in r.items():
+        e = {"ms_per_step": v["ms_per_step"], "sim_days_per_day": v["sim_days_per_day"]}
+        if "errors" in v:
+            errs = np.array(v["errors"])
+            e["rel_err_by_day"] = {"vor": [float(f"{x:.3g}") for x in errs[:, 0]],
+                                   "div": [float(f"{x:.3g}") for x in errs[:, 1]],
+                                   "h": [float(f"{x:.3g}") for x in errs[:, 2]]}
+        out[prec] = e
+    return out
+
+
 def adjoint_bench(jgcm, torch, precision, n_fd=16):
     """Reverse mode of T42 L20 Held-Suarez over 48 steps of 600 s with
     checkpoints every 8 steps: loss = mass-weighted global mean of (T - T*)²,
@@ -436,6 +457,7 @@ def main():
         hs["split32_speedup_vs_fp64"] = hs["fp64"]["ms_per_step"] / hs["split32"]["ms_per_step"]
         out["held_suarez"] = hs
         out["ensemble"] = ensemble_bench(jgcm, torch)
+        out["shallow_water_galewsky"] = galewsky_bench(jgcm, torch)
         out["adjoint"] = {p: adjoint_bench(jgcm, torch, p) for p in ("fp64", "split32")}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)

@@ -20,6 +20,7 @@ from .dyn_data import Atmo_Data, Compute_Abs_Vor, Dyn_Data, Time_Advance
 from .dynamics import (Barotropic_Dynamics, Shallow_Water_Physics, Implicit_Correction,
                        Shallow_Water_Dynamics)
 from .graph import Step_Graph
+from .initial_conditions import Galewsky_Jet, Rossby_Haurwitz_Wave4
 from .adjoint import Barotropic_Adjoint, Held_Suarez_Adjoint, Shallow_Water_Adjoint
 from .output import Output_Manager, Update_Output, Finalize_Output, Zonal_Mean
 from .primitive import (Vert_Coordinate, Pressure_Variables, Compute_Pressures_And_Heights,
@@ -39,5 +40,5 @@ __all__ = [
     "Filtered_Leapfrog_step", "Damped_Leapfrog", "Damped_Leapfrog_Batch", "Atmo_Data", "Compute_Abs_Vor", "Dyn_Data",
     "Time_Advance", "Barotropic_Dynamics", "Shallow_Water_Physics", "Implicit_Correction",
     "Shallow_Water_Dynamics", "Step_Graph", "Barotropic_Adjoint",
-    "Shallow_Water_Adjoint", "Held_Suarez_Adjoint",
+    "Shallow_Water_Adjoint", "Held_Suarez_Adjoint", "Galewsky_Jet", "Rossby_Haurwitz_Wave4",
 ]

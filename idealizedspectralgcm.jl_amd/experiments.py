@@ -1,0 +1,124 @@
+"""Driver loops of the shallow-water experiments (exp/Shallow_Water/SW.jl:78-96
+order: physics, dynamics, Update_Init_Step! after the first step), with the
+later steps replayed from one captured HIP graph.
+
+``Galewsky_Experiment`` is the unforced T85 shallow-water run of the Galewsky
+jet (Δt = 600 s; Filtered_Leapfrog with Robert coefficient 0.04, ∇⁸ damping
+1e-4 s^-1 at the truncation and implicit coefficient 0.5, the values of
+exp/Shallow_Water/SW.jl:39-44).  ``run_days`` advances whole days and hands a
+snapshot of the grid state to a callback at the end of each day.
+"""
+import torch
+
+from .dyn_data import Atmo_Data, Dyn_Data
+from .dynamics import Shallow_Water_Dynamics, Shallow_Water_Physics
+from .graph import Step_Graph
+from .initial_conditions import Galewsky_Jet
+from .mesh import Spectral_Spherical_Mesh
+from .time_integrator import Filtered_Leapfrog, Update_Init_Step
+
+RADIUS, OMEGA = 6371.0e3, 7.292e-5
+
+
+class Galewsky_Experiment:
+    def __init__(self, N=85, nθ=128, dt=600, precision="fp64", seed=7, perturb=True,
+                 robert_coef=0.04, damping_order=4, damping_coef=1.0e-4, implicit_coef=0.5,
+                 end_time=6 * 86400):
+        nλ = 2 * nθ
+        self.dt = int(dt)
+        m = self.mesh = Spectral_Spherical_Mesh(N, N + 1, nθ, nλ, 1, RADIUS, precision=precision)
+        self.atmo = Atmo_Data("sw", nλ, nθ, 1, False, False, False, False, m.sinθ, RADIUS, OMEGA)
+        self.integ = Filtered_Leapfrog(robert_coef, damping_order, damping_coef, m.laplacian_eig,
+                                       implicit_coef, self.dt, True, 0, end_time)
+        self.d = Dyn_Data("sw", N, N + 1, nλ, nθ, 1, mesh=m)
+        self.info = Galewsky_Jet(m, self.d, seed=seed, perturb=perturb)
+        self.h_0 = self.info["h_0"]
+        # unforced: the physics call only clears the grid tendencies (κ = 0)
+        self.h_eq = torch.zeros(1, nθ, nλ, dtype=torch.float64, device=m.device)
+        self.steps = 0
+        self.graph = None
+
+    def _step(self):
+        Shallow_Water_Physics(self.d, 0.0, 0.0, self.h_eq)
+        Shallow_Water_Dynamics(self.mesh, self.atmo, self.h_0, self.d, self.integ)
+
+    def _prepare(self):
+        """First (forward) step eagerly, then capture the later step once."""
+        if self.steps == 0:
+            self._step()
+            Update_Init_Step(self.integ)
+            self.steps = 1
+        if self.graph is None:
+            self.graph = Step_Graph(self.mesh, self._step, nfields=4)
+
+    def snapshot(self):
+        """(grid vorticity, divergence, geopotential, u) copies, after the stream drains."""
+        self.mesh.synchronize()
+        d = self.d
+        out = tuple(t.clone() for t in (d.grid_vor, d.grid_div, d.grid_ps_c, d.grid_u_c))
+        torch.cuda.synchronize()
+        return out
+
+    def run_days(self, days, on_day=None):
+        """Advance `days` days (day k ends after k·86400/Δt steps); returns
+        (device ms, steps timed): HIP events on the mesh stream around the
+        graph replays; the eager first step, the capture and the snapshots
+        are outside the timed region."""
+        per_day = 86400 // self.dt
+        stream = torch.cuda.ExternalStream(self.mesh.stream())
+        ms, timed = 0.0, 0
+        start_day = self.steps // per_day
+        self._prepare()
+        for day in range(start_day, start_day + days):
+            target = (day + 1) * per_day
+            self.mesh.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            n = target - self.steps
+            for _ in range(n):
+                self.graph.launch()
+            e1.record(stream)
+            self.mesh.synchronize()
+            self.steps = target
+            ms += e0.elapsed_time(e1)
+            timed += n
+            if on_day is not None:
+                on_day(day + 1, self.snapshot())
+        return ms, timed
+
+    def close(self):
+        if self.graph is not None:
+            self.graph.close()
+            self.graph = None
+
+
+def relative_error(x, ref):
+    """‖x - ref‖₂ / ‖ref‖₂ (the absolute norm where ref is exactly zero)."""
+    n = float(torch.linalg.norm(ref))
+    e = float(torch.linalg.norm(x - ref))
+    return e / n if n > 0 else e
+
+
+def precision_track(days=6, precisions=("split32", "bf16x3", "fp32"), seed=7, **kw):
+    """Run the FP64 experiment and each reduced-precision one for `days` days;
+    per day and field (vorticity, divergence, geopotential), the relative
+    error against the FP64 run.  Returns {precision: {"ms_per_step", "errors":
+    [[vor, div, h] per day 0..days]}} plus the FP64 entry."""
+    out = {}
+    snaps = {}
+    for prec in ("fp64",) + tuple(precisions):
+        e = Galewsky_Experiment(precision=prec, seed=seed, **kw)
+        s = [e.snapshot()]
+        ms, steps = e.run_days(days, lambda day, snap: s.append(snap))
+        e.close()
+        snaps[prec] = s
+        out[prec] = {"ms_per_step": ms / steps, "steps": steps,
+                     "sim_days_per_day": e.dt / (ms / steps * 1e-3)}
+    ref = snaps["fp64"]
+    for prec in precisions:
+        out[prec]["errors"] = [[relative_error(a, b) for a, b in zip(sa[:3], sb[:3])]
+                               for sa, sb in zip(snaps[prec], ref)]
+    out["fp64"]["max_vor_by_day"] = [float(s[0].abs().max()) for s in ref]
+    return out
+

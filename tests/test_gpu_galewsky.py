@@ -1,0 +1,134 @@
+"""Shallow-water Galewsky jet at T85 (256x128, Δt = 600 s), the device path
+against the FP64 oracle and the precisions against FP64.
+
+* The initial state built on the device (Galewsky_Jet) equals the oracle's
+  transforms of the same host profiles (1e-12 relative).
+* 12 device FP64 steps equal 12 oracle steps (1e-11 relative; only summation
+  order differs).
+* Without the bump, damping and time filter, the balanced jet stays steady for
+  a day: u changes by < 1e-6 relative and max|div| < 1e-5 max|vor| (measured
+  3.5e-8 and 8.2e-7; a first step whose previous state is not the current one
+  gives 2e-2).
+* Over 6 days each reduced-precision Legendre stage tracks FP64: vorticity
+  within 1.5e-6 relative for the first 3 days (before the jet's instability
+  amplifies differences; measured <= 6.3e-7) and within 5e-5 up to day 6
+  (measured split32 4.2e-6, BF16x3 1.6e-5, FP32 2.1e-6); geopotential within
+  1e-6 every day (measured <= 2e-7).  The jet breaks: max|vor| grows > 10 %
+  by day 6 (measured 18 %).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import FilteredLeapfrog, SpectralSphericalMesh
+from oracle import dynamics as odyn
+
+pytestmark = pytest.mark.gpu
+
+A, OMEGA, G = 6.371e6, 7.292e-5, 9.80616
+N, NLAT = 85, 128
+NLON = 2 * NLAT
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def ic(jgcm):
+    from jgcm_amd import initial_conditions
+    return initial_conditions
+
+
+@pytest.fixture(scope="module")
+def ex(jgcm):
+    from jgcm_amd import experiments
+    return experiments
+
+
+def oracle_state(ic, M, seed):
+    """The builder's steps with the oracle mesh on the same host profiles."""
+    lat, lon = M.theta_c, M.lam_c
+    d = odyn.DynData("sw", N, N + 1, NLON, NLAT, 1)
+    u = ic.galewsky_zonal_wind(lat)
+    phi = ic.galewsky_geopotential(lat, M.wts, A, OMEGA, G, 1e4)
+    lon_c = float(np.random.default_rng(seed).uniform(0.0, 2 * np.pi))
+    grid_phi = phi[:, None] + G * ic.galewsky_bump(lat, lon, lon_c)
+    d.grid_u_c[...] = u[None, :, None]
+    d.grid_v_c[...] = 0.0
+    d.grid_lnps[...] = grid_phi[None]
+    M.vor_div_from_grid_uv(d.grid_u_c, d.grid_v_c, d.spe_vor_c, d.spe_div_c)
+    M.trans_spherical_to_grid(d.spe_vor_c, d.grid_vor)
+    M.trans_spherical_to_grid(d.spe_div_c, d.grid_div)
+    M.uv_grid_from_vor_div(d.spe_vor_c, d.spe_div_c, d.grid_u_c, d.grid_v_c)
+    M.trans_grid_to_spherical(d.grid_lnps, d.spe_lnps_c)
+    M.trans_spherical_to_grid(d.spe_lnps_c, d.grid_ps_c)
+    d.spe_lnps_n[...] = d.spe_lnps_c
+    for f in ("vor", "div", "lnps"):
+        getattr(d, f"spe_{f}_p")[...] = getattr(d, f"spe_{f}_c")
+    for f in ("u", "v", "ps"):
+        getattr(d, f"grid_{f}_p")[...] = getattr(d, f"grid_{f}_c")
+    return d, lon_c
+
+
+FIELDS = ["spe_vor_c", "spe_div_c", "spe_lnps_c", "spe_lnps_n", "spe_vor_p", "spe_lnps_p",
+          "grid_u_c", "grid_v_c", "grid_vor", "grid_ps_c", "grid_u_p", "grid_ps_p"]
+
+
+def test_galewsky_state_and_steps_match_oracle(jgcm, ic, ex):
+    M = SpectralSphericalMesh(N, N + 1, NLAT, NLON, 1, A)
+    e = ex.Galewsky_Experiment(precision="fp64", seed=7)
+    d, lon_c = oracle_state(ic, M, 7)
+    assert e.info["bump_lon"] == lon_c
+    dd = e.d
+    scale_v = np.abs(d.spe_vor_c).max()
+    for f in FIELDS:
+        a, b = getattr(dd, f).cpu().numpy(), getattr(d, f)
+        if f.startswith("spe_div") or f == "grid_v_c":
+            # zero up to rounding on both sides
+            assert np.abs(a).max() < 1e-12 * (scale_v if f.startswith("spe") else 80.0), f
+        else:
+            assert rel(a, b) < 1e-12, (f, rel(a, b))
+    atmo_r = odyn.AtmoData("sw", NLON, NLAT, 1, False, False, False, False, M.sin_theta, A, OMEGA)
+    Ir = FilteredLeapfrog(0.04, 4, 1e-4, M.laplacian_eig, 0.5, 600, True, 0, 6 * 86400)
+    heq = np.zeros((1, NLAT, NLON))
+    for i in range(12):
+        odyn.shallow_water_physics(d, 0.0, 0.0, heq)
+        odyn.shallow_water_dynamics(M, atmo_r, G * 1e4, d, Ir)
+        if i == 0:
+            Ir.update_init_step()
+    e._prepare()                                 # first step eager, later ones replayed
+    for _ in range(11):
+        e.graph.launch()
+    e.mesh.synchronize()
+    e.close()
+    for f in ("spe_vor_c", "spe_div_c", "spe_lnps_c", "spe_vor_p", "grid_u_c", "grid_v_c",
+              "grid_ps_c", "grid_vor", "grid_div"):
+        err = rel(getattr(dd, f).cpu().numpy(), getattr(d, f))
+        assert err < 1e-11, (f, err)
+
+
+def test_unperturbed_jet_stays_balanced(jgcm, ex):
+    e = ex.Galewsky_Experiment(precision="fp64", perturb=False, robert_coef=0.0,
+                               damping_coef=0.0)
+    v0, d0, h0, u0 = (t.cpu().numpy() for t in e.snapshot())
+    snaps = []
+    e.run_days(1, lambda day, s: snaps.append(s))
+    e.close()
+    v, dv, h, u = (t.cpu().numpy() for t in snaps[-1])
+    du = np.linalg.norm(u - u0) / np.linalg.norm(u0)
+    ratio = np.abs(dv).max() / np.abs(v).max()
+    assert du < 1e-6, du
+    assert ratio < 1e-5, ratio
+
+
+def test_precisions_track_fp64_over_six_days(jgcm, ex):
+    r = ex.precision_track(days=6)
+    for prec in ("split32", "bf16x3", "fp32"):
+        errs = np.array(r[prec]["errors"])          # [day 0..6][vor, div, h]
+        print(prec, "vor", errs[:, 0], "h", errs[:, 2])
+        assert np.all(errs[:4, 0] < 1.5e-6), (prec, errs[:, 0])
+        assert np.all(errs[:, 0] < 5e-5), (prec, errs[:, 0])
+        assert np.all(errs[:, 2] < 1e-6), (prec, errs[:, 2])
+    mv = r["fp64"]["max_vor_by_day"]
+    assert mv[-1] > 1.1 * mv[0], mv

@@ -48,6 +48,17 @@ def rh4_setup(jgcm, N=42, nlat=64):
     return M, dev, d
 
 
+def test_rossby_haurwitz_builder_matches_oracle_state(jgcm):
+    """Rossby_Haurwitz_Wave4 (device) equals the oracle's transforms of the same
+    grid vorticity plus the same seeded spectral noise."""
+    M, dev, d = rh4_setup(jgcm)
+    dd = jgcm.Dyn_Data("b", 42, 43, 128, 64, 1, mesh=dev)
+    jgcm.Rossby_Haurwitz_Wave4(dev, dd, noise=1e-8, seed=42)
+    for f in ("spe_vor_c", "grid_vor", "grid_u_c", "grid_v_c", "spe_vor_p", "grid_u_p"):
+        ref = getattr(d, f) if not f.endswith("_p") else getattr(d, f[:-2] + "_c")
+        assert rel(getattr(dd, f).cpu().numpy(), ref) < 1e-12, f
+
+
 def test_barotropic_steps_match_oracle(jgcm):
     M, dev, d = rh4_setup(jgcm)
     N, nlat = 42, 64
