@@ -68,7 +68,7 @@ SIGNATURES = {
     "jgcm_damped_leapfrog_adjoint": [P, I, I, P, P, P, P, P, I],
     "jgcm_barotropic_step_adjoint": [P, I, P, P, P, D, P, P, I],
     "jgcm_sw_step_adjoint": [P, I, P, P, P, P, P, P, D, D, D, D, P, P, P, P, P, P, I],
-    "jgcm_hs_column_adjoint": [P, P, D, P, P, P, P, P, P],
+    "jgcm_hs_column_adjoint": [P, P, D, P, P, P, P, P],
     "jgcm_hs_corrections_adjoint": [P, P, P, P, P, P, P, P, P, P, P, P, P],
     "jgcm_implicit_adjoint": [P, I, P, P, P, P, P],
     "jgcm_gradient_cos_adjoint": [P, P, P, P, I, I],

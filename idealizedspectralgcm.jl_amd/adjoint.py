@@ -424,7 +424,6 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         mesh._semi_implicit = prev
         Get_Topography(self._d.grid_geopots)
         self._matrices = {}
-        self._ws = None
         self._param_b = torch.zeros((4, mesh.nθ, mesh.nλ), dtype=torch.float64, device=mesh.device)
         self.parameter_gradients = {}
 
@@ -629,13 +628,11 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         outs = [G(nd), G(nd), G(nd), G(nd), G(nd), G(nd), G(nd), G(1), G(1), G(nd), G(nd), G(nd),
                 G(1), G(1)]
         div_b, u_b, v_b, t_b, vor_b, tl_b, tt_b, gl_b, gt_b, up_b, vp_b, tp_b, psp_b, psc_b = outs
-        if self._ws is None or self._ws.numel() < 4 * nd * M.nθ * M.nλ:
-            self._ws = torch.zeros(4 * nd * M.nθ * M.nλ, dtype=torch.float64, device=M.device)
         arr = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
         hp = _hs(self.params)
         call("jgcm_hs_column_adjoint", si.handle, ctypes.byref(hp), float(Get_Δt(I)), _ptr(scal_bar),
              arr([div_g, u_g, v_g, t_g, vor_g, tl, tt, gl, gt, u_pg, v_pg, t_pg, ps_pg, ps_c]),
-             arr([du_g, dv_g, dt_g, en_g, dl_g]), arr(outs), _ptr(self._ws), _ptr(self._param_b))
+             arr([du_g, dv_g, dt_g, en_g, dl_g]), arr(outs), _ptr(self._param_b))
         # 6. the previous cotangents of the grid p-fields belong to this step's grid c-fields
         add(u_b, Bup)
         add(v_b, Bvpg)

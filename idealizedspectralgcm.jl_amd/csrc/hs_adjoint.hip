@@ -2,10 +2,11 @@
 // Spectral_Dynamics.jl:206-490) that have no counterpart among the transform
 // and leapfrog adjoints:
 //
-//   hs_column_adjoint_kernel   transpose of the grid-column part of the step
-//                              linearised at the step's state: HS forcing on
-//                              the p-fields (HS_Forcing.jl), the energy of
-//                              Compute_Corrections_Init (SD:5-28), Four_In_One!
+//   hs_forcing_adjoint_kernel  transpose of the HS forcing on the p-fields
+//                              (HS_Forcing.jl) and of the energy integral of
+//                              Compute_Corrections_Init (SD:5-28)
+//   hs_column_adjoint_kernel   transpose of the grid-column dynamics of the
+//                              step linearised at its state: Four_In_One!
 //                              (SD:88-183), the hydrostatic integral, δln ps,
 //                              vertical advection (Vert_Coordinate.jl:171-210),
 //                              horizontal advection of T, the vorticity flux
@@ -84,7 +85,6 @@ struct HsColAdj {
   // cotangents out
   double *div_b, *u_b, *v_b, *t_b, *vor_b, *tl_b, *tt_b, *gl_b, *gt_b;
   double *up_b, *vp_b, *tp_b, *psp_b, *psc_b;
-  double* ws;       // [4][nd][P]: Q, S (forward), S_bar, partial Q_bar
   double* param_b;  // [4][P] or NULL: cotangents of k_f, k_a, k_s, ΔT_y, added per column
 };
 
@@ -94,7 +94,8 @@ struct HsAdjScalars {
   jgcm_hs_params hp;
 };
 
-// One thread per column.  Forward quantities (ps factored out, even sigma):
+// Column tiles: 64 columns per workgroup (one per lane), wave q on levels
+// q*KPT .. q*KPT+KPT-1.  Forward quantities (ps factored out, even sigma):
 //   Q_k = D_k + u_k gλ + v_k gθ,  S_k = Σ_{j<k} Δb_j Q_j,  S = Σ_j Δb_j Q_j
 //   x5_k = -(Dl_k/Δb_k) S_k - Dp_k Q_k + C1_k (u_k gλ + v_k gθ);  δT_k += κ T_k x5_k
 //   δu_k -= R T_k C1_k gλ,  δv_k -= R T_k C1_k gθ,  δln ps = -S
@@ -102,157 +103,267 @@ struct HsAdjScalars {
 //   V(r)_k = (f_k - f_{k+1} + r_k (m_{k+1} - m_k)) / Δb_k, f_h = m_h ((1-α_h) r_{h-1} + α_h r_h)
 //   δT_k -= u_k Tλ_k + v_k Tθ_k,  δu_k += (ζ_k + f) v_k,  δv_k -= (ζ_k + f) u_k
 //   E_k = R Σ_{j>k} Dl_j T_j + R Dp_k T_k + KE_k
-// plus the forcing on the p-fields and the energy of Compute_Corrections_Init.
-__global__ void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat, int nlon, const double* sint,
-                                         const double* wts, HsAdjScalars c) {
+// (the forcing on the p-fields is hs_forcing_adjoint_kernel).
+// The reverse sweep needs three vertical scans — the prefix S_k, the prefix of
+// the energy cotangent and the suffix of S_bar — and per-column sums; each is
+// formed from per-wave partials in wave order (level_sums_n), so the result
+// differs from a sequential sweep by rounding only.  Outputs must not alias
+// inputs (neighbouring levels of the input cotangents are read by other waves).
+template <int KPT>
+__global__ __launch_bounds__(1024) void hs_column_adjoint_kernel(HsColAdj s, VC v, int nlat,
+                                                                 int nlon, const double* sint,
+                                                                 const double* wts, HsAdjScalars c) {
+  extern __shared__ double red[];  // [4][nq][64]
   const size_t P = (size_t)nlat * nlon;
   const int nd = v.nd;
-  double* wQ = s.ws;
-  double* wS = s.ws + (size_t)nd * P;
-  double* wSb = s.ws + (size_t)2 * nd * P;
-  double* wQb = s.ws + (size_t)3 * nd * P;
-  const double R = c.rdgas, kap = c.kappa, cp = c.cp_air, dt = c.dt;
+  const TileIdx t = tile_idx<KPT>(P);
+  const bool ok = t.col;
+  const size_t col = ok ? t.c : 0;
+  const int k0 = t.k0;
+  const int j = (int)(col / nlon);
+  const double R = c.rdgas, kap = c.kappa;
+  const double f = c.two_omega * sint[j];
+  const double gl = ok ? s.gl[col] : 0.0, gt = ok ? s.gt[col] : 0.0;
+
+  // slot i of U/V/T holds level k0-1+i (i = 0..KPT); slot i of BU/BV/BT holds
+  // δ*_bar / Δb at level k0-1+i (i = 0..KPT+1); out-of-range levels read as 0
+  double U[KPT + 1], V[KPT + 1], T[KPT + 1], BU[KPT + 2], BV[KPT + 2], BT[KPT + 2];
+#pragma unroll
+  for (int i = 0; i < KPT + 2; ++i) {
+    const int k = k0 - 1 + i;
+    const bool in = ok && k >= 0 && k < nd;
+    const size_t o = (size_t)(in ? k : 0) * P + col;
+    const double db = in ? v.dbk[k] : 1.0;
+    BU[i] = in ? s.du_b[o] / db : 0.0;
+    BV[i] = in ? s.dv_b[o] / db : 0.0;
+    BT[i] = in ? s.dt_b[o] / db : 0.0;
+    if (i <= KPT) {
+      U[i] = in ? s.u[o] : 0.0;
+      V[i] = in ? s.v[o] : 0.0;
+      T[i] = in ? s.t[o] : 0.0;
+    }
+  }
+
+  // scan 1: S_k at own levels and at k0+KPT, and the column total S
+  double Q[KPT], Sl[KPT + 1];
+  double part = 0.0;
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    const int k = k0 + i;
+    Sl[i] = part;
+    Q[i] = 0.0;
+    if (ok && k < nd) {
+      const size_t o = (size_t)k * P + col;
+      Q[i] = s.div[o] + U[i + 1] * gl + V[i + 1] * gt;
+      part = part + v.dbk[k] * Q[i];
+    }
+  }
+  Sl[KPT] = part;
+  double Stot;
+  {
+    const double pa[1] = {part};
+    double ab[1], be[1];
+    level_sums_n<1>(red, t, pa, ab, be);
+#pragma unroll
+    for (int i = 0; i <= KPT; ++i) Sl[i] = Sl[i] + ab[0];
+    Stot = ab[0] + part + be[0];
+  }
+  double mh[KPT + 1];  // m at half levels k0 .. k0+KPT
+#pragma unroll
+  for (int i = 0; i <= KPT; ++i) {
+    const int h = k0 + i;
+    mh[i] = (h >= 1 && h <= nd - 1) ? -Sl[i] + v.bk[h] * Stot : 0.0;
+  }
+
+  const double lb = ok ? -s.dlnps_b[col] : 0.0;  // S_bar from δln ps = -S
+
+  // level-local terms
+  double Qbp[KPT], Sbl[KPT], E[KPT], DL[KPT], ubp[KPT], vbp[KPT], tbp[KPT];
+  double esum = 0.0, sbsum = 0.0, mbsum = 0.0;
+  double glb = 0.0, gtb = 0.0;
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    const int k = k0 + i;
+    Qbp[i] = Sbl[i] = E[i] = DL[i] = ubp[i] = vbp[i] = tbp[i] = 0.0;
+    if (!(ok && k < nd)) continue;
+    const size_t o = (size_t)k * P + col;
+    const LevelConst L = level_const(v, k);
+    DL[i] = L.Dl;
+    const double db = v.dbk[k];
+    const double u = U[i + 1], vv = V[i + 1], tk = T[i + 1], vor = s.vor[o];
+    const double a = s.du_b[o], b = s.dv_b[o], cc = s.dt_b[o], e = s.e_b[o];
+    const double mk = mh[i], mk1 = mh[i + 1];
+    // vertical advection of u, v, T: contributions to r_bar at level k
+    const double bu = BU[i + 1], bv = BV[i + 1], bt = BT[i + 1];
+    double vbu = bu * (mk1 - mk), vbv = bv * (mk1 - mk), vbt = bt * (mk1 - mk);
+    double mb = 0.0;  // m_bar at half level k
+    if (k >= 1) {
+      const double bum = BU[i], bvm = BV[i], btm = BT[i];
+      const double al = vadv_alpha(v, k);
+      const double um = U[i], vm = V[i], tm = T[i];
+      vbu = vbu + (bu - bum) * mk * al;
+      vbv = vbv + (bv - bvm) * mk * al;
+      vbt = vbt + (bt - btm) * mk * al;
+      mb = mb + (bum * um - bu * u + (bu - bum) * ((1.0 - al) * um + al * u));
+      mb = mb + (bvm * vm - bv * vv + (bv - bvm) * ((1.0 - al) * vm + al * vv));
+      mb = mb + (btm * tm - bt * tk + (bt - btm) * ((1.0 - al) * tm + al * tk));
+    }
+    if (k + 1 <= nd - 1) {
+      const double al1 = vadv_alpha(v, k + 1);
+      vbu = vbu + (BU[i + 2] - bu) * mk1 * (1.0 - al1);
+      vbv = vbv + (BV[i + 2] - bv) * mk1 * (1.0 - al1);
+      vbt = vbt + (BT[i + 2] - bt) * mk1 * (1.0 - al1);
+    }
+    // energy conversion κ T x5
+    const double uvg = u * gl + vv * gt;
+    const double x5 = (k >= 1 ? -(L.Dl / db) * Sl[i] : 0.0) - L.Dp * Q[i] + L.C1 * uvg;
+    const double xb = kap * tk * cc;
+    if (k >= 1) {
+      Sbl[i] = -mb - (L.Dl / db) * xb;
+      mbsum = mbsum + v.bk[k] * mb;
+    }
+    Qbp[i] = -L.Dp * xb;
+    // local cotangents
+    s.vor_b[o] = vv * a - u * b;
+    ubp[i] = -((vor + f) * b) - s.tl[o] * cc + u * e + L.C1 * gl * xb + vbu;
+    vbp[i] = (vor + f) * a - s.tt[o] * cc + vv * e + L.C1 * gt * xb + vbv;
+    tbp[i] = R * L.Dp * e - R * L.C1 * (gl * a + gt * b) + kap * x5 * cc + vbt;
+    s.tl_b[o] = -(u * cc);
+    s.tt_b[o] = -(vv * cc);
+    glb = glb - R * L.C1 * tk * a + L.C1 * u * xb;
+    gtb = gtb - R * L.C1 * tk * b + L.C1 * vv * xb;
+    E[i] = e;
+    esum = esum + e;
+    sbsum = sbsum + Sbl[i];
+  }
+
+  // scans 2 and 3: prefix of the energy cotangent, suffix of S_bar, S_bar total
+  double ab[3], be[3];
+  {
+    const double pa[3] = {esum, sbsum, mbsum};
+    level_sums_n<3>(red, t, pa, ab, be);
+  }
+  const double Stot_b = lb + (ab[2] + mbsum + be[2]);
+  double eprefix = ab[0];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    if (k0 + i >= 1) tbp[i] = tbp[i] + R * DL[i] * eprefix;
+    eprefix = eprefix + E[i];
+  }
+  // Q_bar = partial + Δb_k (Σ_{k'>k} S_bar + S_bar_tot), bottom up
+  double suffix = be[1];
+#pragma unroll
+  for (int i = KPT - 1; i >= 0; --i) {
+    const int k = k0 + i;
+    if (!(ok && k < nd)) continue;
+    const size_t o = (size_t)k * P + col;
+    const double qb = Qbp[i] + v.dbk[k] * (suffix + Stot_b);
+    const double u = U[i + 1], vv = V[i + 1];
+    s.div_b[o] = qb;
+    s.u_b[o] = ubp[i] + gl * qb;
+    s.v_b[o] = vbp[i] + gt * qb;
+    s.t_b[o] = tbp[i];
+    glb = glb + u * qb;
+    gtb = gtb + vv * qb;
+    suffix = suffix + Sbl[i];
+  }
+
+  // per-column sums over the waves; wave 0 writes
+  double ca[2], cb[2];
+  {
+    const double pa[2] = {glb, gtb};
+    level_sums_n<2>(red, t, pa, ca, cb);
+  }
+  if (ok && t.q == 0) {
+    s.gl_b[col] = glb + cb[0];
+    s.gt_b[col] = gtb + cb[1];
+  }
+}
+
+// Held-Suarez forcing on the p-fields (HS_Forcing.jl:1-130: Rayleigh damping
+// of u_p, v_p below σ_b, Newtonian relaxation of T_p to T_eq(φ, p)) and the
+// energy integral of Compute_Corrections_Init (SD:5-28) — transposed, with the
+// cotangents of ps_c (through T_eq), ps_p (through the energy integral and
+// the mean ps_p) and of k_f, k_a, k_s, ΔT_y summed per column.  Same tiles as
+// hs_column_adjoint_kernel; independent of it (its own inputs and outputs).
+template <int KPT>
+__global__ __launch_bounds__(1024) void hs_forcing_adjoint_kernel(HsColAdj s, VC v, int nlat,
+                                                                  int nlon, const double* sint,
+                                                                  const double* wts,
+                                                                  HsAdjScalars c) {
+  extern __shared__ double red[];  // [4][nq][64]
+  const size_t P = (size_t)nlat * nlon;
+  const int nd = v.nd;
+  const TileIdx t = tile_idx<KPT>(P);
+  const bool ok = t.col;
+  const size_t col = ok ? t.c : 0;
+  const int j = (int)(col / nlon);
+  const double kap = c.kappa, cp = c.cp_air, dt = c.dt;
   const double ep_bar = c.scal_bar[1], mpsp_bar = c.scal_bar[2];
-  HS_GRID_STRIDE(col, P) {
-    const int j = (int)(col / nlon);
-    const double f = c.two_omega * sint[j];
-    const double gl = s.gl[col], gt = s.gt[col];
-    const double aj = wts[j] / (2.0 * nlon);
-    // pass A: Q_k, S_k, S
-    double Ssum = 0.0;
-    for (int k = 0; k < nd; ++k) {
-      size_t o = (size_t)k * P + col;
-      double q = s.div[o] + s.u[o] * gl + s.v[o] * gt;
-      wQ[o] = q;
-      wS[o] = Ssum;
-      Ssum = Ssum + v.dbk[k] * q;
+  const double aj = wts[j] / (2.0 * nlon);
+  const double day = 86400.0, t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
+  const double kf = c.hp.k_f / day, ka = c.hp.k_a / day, ks = c.hp.k_s / day;
+  const double sj = sint[j], s2 = sj * sj, c2 = 1.0 - s2, c4 = c2 * c2;
+  const double psc = ok ? s.ps_c[col] : 1.0, psp = ok ? s.ps_p[col] : 0.0;
+  double pscb = 0.0, pspb = 0.0, kfb = 0.0, kab = 0.0, ksb = 0.0, dtyb = 0.0;
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    const int k = t.k0 + i;
+    if (!(ok && k < nd)) continue;
+    const size_t o = (size_t)k * P + col;
+    const LevelConst L = level_const(v, k);
+    const double db = v.dbk[k];
+    const double a = s.du_b[o], b = s.dv_b[o], cc = s.dt_b[o];
+    const double sg = L.sig;
+    const double kv = kf / (1.0 - c.hp.sigma_b) * fmax(0.0, sg - c.hp.sigma_b);
+    const double kt = ka + (ks - ka) / (1.0 - c.hp.sigma_b) * fmax(0.0, sg - c.hp.sigma_b) * c4;
+    const double Up = s.u_p[o], Vp = s.v_p[o], Tp = s.t_p[o];
+    const double pn = sg * psc / p_ref;
+    const double A0 = t_zero - c.hp.delta_T_y * s2 - c.hp.delta_theta_z * c2 * log(pn);
+    const double pk = pow(pn, kap);
+    const double te = A0 * pk;
+    const double teq = fmax(t_strat, te);
+    const double g2 = kv * (1.0 - 0.5 * kv * dt);
+    const double DT = g2 * (Up * Up + Vp * Vp) / cp - kt * (Tp - teq);
+    const double uu = Up + dt * (-kv * Up), vq = Vp + dt * (-kv * Vp);
+    const double eb = ep_bar / c.grav * aj * psp * db;
+    const double ap = a + eb * dt * uu, bp = b + eb * dt * vq, cpb = cc + cp * dt * eb;
+    s.up_b[o] = eb * uu - kv * ap + 2.0 * g2 * Up * cpb / cp;
+    s.vp_b[o] = eb * vq - kv * bp + 2.0 * g2 * Vp * cpb / cp;
+    s.tp_b[o] = cp * eb - kt * cpb;
+    if (te > t_strat) {
+      // ∂T_eq/∂ps_c = pn^(κ-1) (κ A0 - Δθ_z cos²φ) σ / p_ref
+      const double dte = (pk / pn) * (kap * A0 - c.hp.delta_theta_z * c2) * sg / p_ref;
+      pscb = pscb + kt * cpb * dte;
+      dtyb = dtyb - cpb * kt * s2 * pk;  // ∂T_eq/∂ΔT_y = -sin²φ pn^κ
     }
-    const double Stot = Ssum;
-    auto m_at = [&](int h) -> double {
-      return (h >= 1 && h <= nd - 1) ? -wS[(size_t)h * P + col] + v.bk[h] * Stot : 0.0;
-    };
-    // physics constants of this latitude
-    const double day = 86400.0, t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
-    const double kf = c.hp.k_f / day, ka = c.hp.k_a / day, ks = c.hp.k_s / day;
-    const double sj = sint[j], s2 = sj * sj, c2 = 1.0 - s2, c4 = c2 * c2;
-    const double psc = s.ps_c[col], psp = s.ps_p[col];
-    const double lb = -s.dlnps_b[col];  // S_bar from δln ps = -S
-    double Stot_b = lb;
-    double glb = 0.0, gtb = 0.0, pscb = 0.0, pspb = 0.0;
-    double kfb = 0.0, kab = 0.0, ksb = 0.0, dtyb = 0.0;  // forcing-parameter cotangents
-    double eprefix = 0.0;
-    // pass B: level-local terms, m_bar, S_bar, first part of Q_bar
-    for (int k = 0; k < nd; ++k) {
-      size_t o = (size_t)k * P + col;
-      const LevelConst L = level_const(v, k);
-      const double db = v.dbk[k];
-      const double u = s.u[o], vv = s.v[o], t = s.t[o], vor = s.vor[o];
-      const double a = s.du_b[o], b = s.dv_b[o], cc = s.dt_b[o], e = s.e_b[o];
-      const double mk = m_at(k), mk1 = m_at(k + 1);
-      // vertical advection of u, v, T: contributions to r_bar at level k
-      const double bu = a / db, bv = b / db, bt = cc / db;
-      double vbu = bu * (mk1 - mk), vbv = bv * (mk1 - mk), vbt = bt * (mk1 - mk);
-      double mb = 0.0;  // m_bar at half level k
-      if (k >= 1) {
-        size_t om = o - P;
-        const double dbm = v.dbk[k - 1];
-        const double bum = s.du_b[om] / dbm, bvm = s.dv_b[om] / dbm, btm = s.dt_b[om] / dbm;
-        const double al = vadv_alpha(v, k);
-        const double um = s.u[om], vm = s.v[om], tm = s.t[om];
-        vbu = vbu + (bu - bum) * mk * al;
-        vbv = vbv + (bv - bvm) * mk * al;
-        vbt = vbt + (bt - btm) * mk * al;
-        mb = mb + (bum * um - bu * u + (bu - bum) * ((1.0 - al) * um + al * u));
-        mb = mb + (bvm * vm - bv * vv + (bv - bvm) * ((1.0 - al) * vm + al * vv));
-        mb = mb + (btm * tm - bt * t + (bt - btm) * ((1.0 - al) * tm + al * t));
-      }
-      if (k + 1 <= nd - 1) {
-        size_t op = o + P;
-        const double dbp = v.dbk[k + 1];
-        const double bup = s.du_b[op] / dbp, bvp = s.dv_b[op] / dbp, btp = s.dt_b[op] / dbp;
-        const double al1 = vadv_alpha(v, k + 1);
-        vbu = vbu + (bup - bu) * mk1 * (1.0 - al1);
-        vbv = vbv + (bvp - bv) * mk1 * (1.0 - al1);
-        vbt = vbt + (btp - bt) * mk1 * (1.0 - al1);
-      }
-      // energy conversion κ T x5
-      const double Sk = wS[o], Qk = wQ[o];
-      const double uvg = u * gl + vv * gt;
-      const double x5 = (k >= 1 ? -(L.Dl / db) * Sk : 0.0) - L.Dp * Qk + L.C1 * uvg;
-      const double xb = kap * t * cc;
-      double Sb = 0.0;
-      if (k >= 1) {
-        Sb = -mb - (L.Dl / db) * xb;
-        Stot_b = Stot_b + v.bk[k] * mb;
-      }
-      wSb[o] = Sb;
-      wQb[o] = -L.Dp * xb;
-      // local cotangents
-      s.vor_b[o] = vv * a - u * b;
-      s.u_b[o] = -((vor + f) * b) - s.tl[o] * cc + u * e + L.C1 * gl * xb + vbu;
-      s.v_b[o] = (vor + f) * a - s.tt[o] * cc + vv * e + L.C1 * gt * xb + vbv;
-      double tb = R * L.Dp * e - R * L.C1 * (gl * a + gt * b) + kap * x5 * cc + vbt;
-      if (k >= 1) tb = tb + R * L.Dl * eprefix;
-      s.t_b[o] = tb;
-      s.tl_b[o] = -(u * cc);
-      s.tt_b[o] = -(vv * cc);
-      glb = glb - R * L.C1 * t * a + L.C1 * u * xb;
-      gtb = gtb - R * L.C1 * t * b + L.C1 * vv * xb;
-      eprefix = eprefix + e;
-      // Held-Suarez forcing on the p-fields and the energy of Compute_Corrections_Init
-      const double sg = L.sig;
-      const double kv = kf / (1.0 - c.hp.sigma_b) * fmax(0.0, sg - c.hp.sigma_b);
-      const double kt = ka + (ks - ka) / (1.0 - c.hp.sigma_b) * fmax(0.0, sg - c.hp.sigma_b) * c4;
-      const double U = s.u_p[o], V = s.v_p[o], T = s.t_p[o];
-      const double pn = sg * psc / p_ref;
-      const double A0 = t_zero - c.hp.delta_T_y * s2 - c.hp.delta_theta_z * c2 * log(pn);
-      const double te = A0 * pow(pn, kap);
-      const double teq = fmax(t_strat, te);
-      const double g2 = kv * (1.0 - 0.5 * kv * dt);
-      const double DT = g2 * (U * U + V * V) / cp - kt * (T - teq);
-      const double uu = U + dt * (-kv * U), vq = V + dt * (-kv * V);
-      const double eb = ep_bar / c.grav * aj * psp * db;
-      const double ap = a + eb * dt * uu, bp = b + eb * dt * vq, cpb = cc + cp * dt * eb;
-      s.up_b[o] = eb * uu - kv * ap + 2.0 * g2 * U * cpb / cp;
-      s.vp_b[o] = eb * vq - kv * bp + 2.0 * g2 * V * cpb / cp;
-      s.tp_b[o] = cp * eb - kt * cpb;
-      if (te > t_strat) {
-        const double dte = pow(pn, kap - 1.0) * (kap * A0 - c.hp.delta_theta_z * c2) * sg / p_ref;
-        pscb = pscb + kt * cpb * dte;
-        dtyb = dtyb - cpb * kt * s2 * pow(pn, kap);  // ∂T_eq/∂ΔT_y = -sin²φ pn^κ
-      }
-      // k_v = k_f/day max(0, σ-σ_b)/(1-σ_b);  k_T = k_a/day + (k_s-k_a)/day max(0, σ-σ_b)/(1-σ_b) cos⁴φ
-      const double sp = fmax(0.0, sg - c.hp.sigma_b) / (1.0 - c.hp.sigma_b);
-      const double kvb = -(ap * U) - bp * V + cpb * (1.0 - kv * dt) * (U * U + V * V) / cp;
-      const double ktb = -(cpb * (T - teq));
-      kfb = kfb + kvb * sp / day;
-      kab = kab + ktb * (1.0 - sp * c4) / day;
-      ksb = ksb + ktb * (sp * c4) / day;
-      const double ep = 0.5 * (uu * uu + vq * vq) + cp * (T + dt * DT);
-      pspb = pspb + ep_bar / c.grav * aj * db * ep;
-    }
-    // pass C (bottom up): Q_bar = partial + Δb_k (Σ_{k'>k} S_bar + S_bar_tot)
-    double suffix = 0.0;
-    for (int k = nd - 1; k >= 0; --k) {
-      size_t o = (size_t)k * P + col;
-      const double qb = wQb[o] + v.dbk[k] * (suffix + Stot_b);
-      const double u = s.u[o], vv = s.v[o];
-      s.div_b[o] = qb;
-      s.u_b[o] = s.u_b[o] + gl * qb;
-      s.v_b[o] = s.v_b[o] + gt * qb;
-      glb = glb + u * qb;
-      gtb = gtb + vv * qb;
-      suffix = suffix + wSb[o];
-    }
-    s.gl_b[col] = glb;
-    s.gt_b[col] = gtb;
-    s.psc_b[col] = pscb;
-    s.psp_b[col] = pspb + mpsp_bar * aj;
+    // k_v = k_f/day max(0, σ-σ_b)/(1-σ_b);  k_T = k_a/day + (k_s-k_a)/day max(0, σ-σ_b)/(1-σ_b) cos⁴φ
+    const double sp = fmax(0.0, sg - c.hp.sigma_b) / (1.0 - c.hp.sigma_b);
+    const double kvb = -(ap * Up) - bp * Vp + cpb * (1.0 - kv * dt) * (Up * Up + Vp * Vp) / cp;
+    const double ktb = -(cpb * (Tp - teq));
+    kfb = kfb + kvb * sp / day;
+    kab = kab + ktb * (1.0 - sp * c4) / day;
+    ksb = ksb + ktb * (sp * c4) / day;
+    const double ep = 0.5 * (uu * uu + vq * vq) + cp * (Tp + dt * DT);
+    pspb = pspb + ep_bar / c.grav * aj * db * ep;
+  }
+  double c1a[2], c1b[2], c2a[4], c2b[4];
+  {
+    const double pa[2] = {pscb, pspb};
+    level_sums_n<2>(red, t, pa, c1a, c1b);
+  }
+  {
+    const double pa[4] = {kfb, kab, ksb, dtyb};
+    level_sums_n<4>(red, t, pa, c2a, c2b);
+  }
+  if (ok && t.q == 0) {
+    s.psc_b[col] = pscb + c1b[0];
+    s.psp_b[col] = (pspb + c1b[1]) + mpsp_bar * aj;
     if (s.param_b) {
-      s.param_b[col] = s.param_b[col] + kfb;
-      s.param_b[P + col] = s.param_b[P + col] + kab;
-      s.param_b[2 * P + col] = s.param_b[2 * P + col] + ksb;
-      s.param_b[3 * P + col] = s.param_b[3 * P + col] + dtyb;
+      s.param_b[col] = s.param_b[col] + (kfb + c2b[0]);
+      s.param_b[P + col] = s.param_b[P + col] + (kab + c2b[1]);
+      s.param_b[2 * P + col] = s.param_b[2 * P + col] + (ksb + c2b[2]);
+      s.param_b[3 * P + col] = s.param_b[3 * P + col] + (dtyb + c2b[3]);
     }
   }
 }
@@ -323,65 +434,50 @@ __global__ void hs_correction_scalars_adjoint_kernel(const double* scal, const d
 // diagonal), A_n the real (2nd+1) x (7nd+3) matrix of the linear map
 // (δdiv, δln ps, δT, energy, div_c, div_p, ln ps_c, ln ps_p, T_c, T_p) ->
 // (δdiv, δln ps, δT) of ∇²(energy) + Implicit_Correction!.
-__global__ void implicit_adjoint_kernel(const double* A, int nd, int nm, const double2* ddiv_b,
-                                        const double2* dlnps_b, const double2* dt_b, double2* out,
-                                        size_t spec_per_level) {
-  const int nin = 7 * nd + 3, nout = 2 * nd + 1;
-  HS_GRID_STRIDE(idx, spec_per_level) {
-    const int m = (int)(idx % nm);
-    const int n = (int)(idx / nm);
-    const double* An = A + (size_t)n * nout * nin;
-    for (int i = 0; i < nin; ++i) {
-      double2 acc = make_double2(0.0, 0.0);
-      if (n >= m) {
-        for (int o = 0; o < nout; ++o) {
-          const double2 x = o < nd ? ddiv_b[(size_t)o * spec_per_level + idx]
-                                   : (o == nd ? dlnps_b[idx]
-                                              : dt_b[(size_t)(o - nd - 1) * spec_per_level + idx]);
-          const double w = An[(size_t)o * nin + i];
-          acc = make_double2(acc.x + w * x.x, acc.y + w * x.y);
-        }
-      }
-      out[(size_t)i * spec_per_level + idx] = acc;
-    }
-  }
-}
+// Workgroup (n, output group, m chunk) of four waves: lane = m, each wave
+// owns IPW consecutive outputs i, so the matrix entries it reads are the same
+// for every lane (scalar loads) and each input cotangent is loaded once per
+// wave for IPW products.  Sums run over o in increasing order.
+constexpr int IMPL_IPW = 8;
 
-// Same map with one workgroup per degree n: A_n staged in LDS (shared by every
-// m of the row), the 2nd+1 input cotangents of the thread's m in registers.
-// For nout <= NOUT_MAX.
-template <int NOUT_MAX>
-__global__ __launch_bounds__(128) void implicit_adjoint_row_kernel(
-    const double* A, int nd, int nm, const double2* ddiv_b, const double2* dlnps_b,
-    const double2* dt_b, double2* out, size_t spec_per_level) {
-  extern __shared__ double An[];  // [nout][nin]
+__global__ __launch_bounds__(256) void implicit_adjoint_kernel(const double* A, int nd, int nm,
+                                                               const double2* ddiv_b,
+                                                               const double2* dlnps_b,
+                                                               const double2* dt_b, double2* out,
+                                                               size_t spec_per_level) {
   const int nin = 7 * nd + 3, nout = 2 * nd + 1;
   const int n = blockIdx.x;
-  const double* src = A + (size_t)n * nout * nin;
-  for (int q = threadIdx.x; q < nout * nin; q += blockDim.x) An[q] = src[q];
-  __syncthreads();
-  for (int m = threadIdx.x; m < nm; m += blockDim.x) {
-    const size_t idx = (size_t)n * nm + m;
-    double2 x[NOUT_MAX];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const int lane = threadIdx.x & 63;
+  const int i0 = (blockIdx.y * 4 + wave) * IMPL_IPW;
+  if (i0 >= nin) return;  // the whole wave; no barriers follow
+  const int m = blockIdx.z * 64 + lane;
+  const bool in_row = m < nm;
+  const bool valid = in_row && m <= n;
+  const size_t idx = (size_t)n * nm + (in_row ? m : 0);
+  const double* An = A + (size_t)n * nout * nin;
+  double2 acc[IMPL_IPW];
 #pragma unroll
-    for (int o = 0; o < NOUT_MAX; ++o) {
-      if (o >= nout) break;
-      x[o] = (n < m) ? make_double2(0.0, 0.0)
-                     : (o < nd ? ddiv_b[(size_t)o * spec_per_level + idx]
-                               : (o == nd ? dlnps_b[idx]
-                                          : dt_b[(size_t)(o - nd - 1) * spec_per_level + idx]));
-    }
-    for (int i = 0; i < nin; ++i) {
-      double2 acc = make_double2(0.0, 0.0);
+  for (int r = 0; r < IMPL_IPW; ++r) acc[r] = make_double2(0.0, 0.0);
+#pragma unroll 4
+  for (int o = 0; o < nout; ++o) {
+    double2 x = make_double2(0.0, 0.0);
+    if (valid)
+      x = o < nd ? ddiv_b[(size_t)o * spec_per_level + idx]
+                 : (o == nd ? dlnps_b[idx] : dt_b[(size_t)(o - nd - 1) * spec_per_level + idx]);
+    const double* Ao = An + (size_t)o * nin + i0;
 #pragma unroll
-      for (int o = 0; o < NOUT_MAX; ++o) {
-        if (o >= nout) break;
-        const double w = An[o * nin + i];
-        acc = make_double2(acc.x + w * x[o].x, acc.y + w * x[o].y);
+    for (int r = 0; r < IMPL_IPW; ++r) {
+      if (i0 + r < nin) {
+        const double w = Ao[r];
+        acc[r] = make_double2(acc[r].x + w * x.x, acc[r].y + w * x.y);
       }
-      out[(size_t)i * spec_per_level + idx] = acc;
     }
   }
+  if (!in_row) return;
+#pragma unroll
+  for (int r = 0; r < IMPL_IPW; ++r)
+    if (i0 + r < nin) out[(size_t)(i0 + r) * spec_per_level + idx] = acc[r];
 }
 
 // Transpose of Compute_Gradient_Cos! (SSM:677-687):
@@ -431,9 +527,9 @@ extern "C" {
 
 int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, double dt,
                            const double* scal_bar, const double* const* state,
-                           const double* const* cot_in, double* const* cot_out, double* workspace,
+                           const double* const* cot_in, double* const* cot_out,
                            double* param_bar) {
-  JGCM_REQUIRE(si && hp && scal_bar && state && cot_in && cot_out && workspace, "NULL argument");
+  JGCM_REQUIRE(si && hp && scal_bar && state && cot_in && cot_out, "NULL argument");
   if (int rc = check_even_sigma(si->vc)) return rc;
   jgcm_mesh* M = si->vc->mesh;
   for (int i = 0; i < 14; ++i) JGCM_REQUIRE(state[i], "NULL state array %d", i);
@@ -444,13 +540,28 @@ int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, d
              cot_in[0], cot_in[1], cot_in[2], cot_in[3], cot_in[4],
              cot_out[0], cot_out[1], cot_out[2], cot_out[3], cot_out[4], cot_out[5], cot_out[6],
              cot_out[7], cot_out[8], cot_out[9], cot_out[10], cot_out[11], cot_out[12],
-             cot_out[13], workspace, param_bar};
+             cot_out[13], param_bar};
   HsAdjScalars c{si->atmo.rdgas, si->atmo.cp_air, si->atmo.kappa, si->atmo.grav,
                  2.0 * si->atmo.omega, dt, scal_bar, *hp};
-  size_t P = M->grid_elems();
-  // one thread per column: 64-thread workgroups spread the columns over the CUs
-  hipLaunchKernelGGL(hs_column_adjoint_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), 0,
-                     M->stream, s, vc_of(si->vc), M->nlat, M->nlon, M->d_sin, M->d_wts, c);
+  const int nd = si->vc->nd;
+  const size_t P = M->grid_elems();
+  const dim3 grid((unsigned)((P + TILE - 1) / TILE));
+  // nq = ceil(nd / KPT) waves of 64 lanes per workgroup (nq <= 16)
+  if (nd <= 32) {
+    const int nq = (nd + 1) / 2;
+    const size_t lds = (size_t)4 * nq * TILE * sizeof(double);
+    hipLaunchKernelGGL(hs_forcing_adjoint_kernel<2>, grid, dim3(nq * TILE), lds, M->stream, s,
+                       vc_of(si->vc), M->nlat, M->nlon, M->d_sin, M->d_wts, c);
+    hipLaunchKernelGGL(hs_column_adjoint_kernel<2>, grid, dim3(nq * TILE), lds, M->stream, s,
+                       vc_of(si->vc), M->nlat, M->nlon, M->d_sin, M->d_wts, c);
+  } else {
+    const int nq = (nd + 3) / 4;
+    const size_t lds = (size_t)4 * nq * TILE * sizeof(double);
+    hipLaunchKernelGGL(hs_forcing_adjoint_kernel<4>, grid, dim3(nq * TILE), lds, M->stream, s,
+                       vc_of(si->vc), M->nlat, M->nlon, M->d_sin, M->d_wts, c);
+    hipLaunchKernelGGL(hs_column_adjoint_kernel<4>, grid, dim3(nq * TILE), lds, M->stream, s,
+                       vc_of(si->vc), M->nlat, M->nlon, M->d_sin, M->d_wts, c);
+  }
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
@@ -484,17 +595,12 @@ int jgcm_implicit_adjoint(jgcm_mesh_t* mesh, int nd, const double* matrix, const
   JGCM_REQUIRE(nd >= 1 && nd <= MAXND, "nd must be in 1..%d, got %d", MAXND, nd);
   JGCM_REQUIRE(matrix && ddiv_bar && dlnps_bar && dt_bar && out, "NULL argument");
   size_t S = mesh->spec_elems();
-  const size_t lds = (size_t)(2 * nd + 1) * (7 * nd + 3) * sizeof(double);
-  if (nd <= 24 && lds <= 64 * 1024) {
-    // one workgroup per degree, the row's matrix in LDS
-    hipLaunchKernelGGL(implicit_adjoint_row_kernel<49>, dim3(mesh->ns + 1), dim3(128), lds,
-                       mesh->stream, matrix, nd, mesh->nm, (const double2*)ddiv_bar,
-                       (const double2*)dlnps_bar, (const double2*)dt_bar, (double2*)out, S);
-  } else {
-    hipLaunchKernelGGL(implicit_adjoint_kernel, dim3(nblk_h(S)), dim3(256), 0, mesh->stream, matrix,
-                       nd, mesh->nm, (const double2*)ddiv_bar, (const double2*)dlnps_bar,
-                       (const double2*)dt_bar, (double2*)out, S);
-  }
+  const int nin = 7 * nd + 3;
+  const dim3 grid((unsigned)(mesh->ns + 1), (unsigned)((nin + 4 * IMPL_IPW - 1) / (4 * IMPL_IPW)),
+                  (unsigned)((mesh->nm + 63) / 64));
+  hipLaunchKernelGGL(implicit_adjoint_kernel, grid, dim3(256), 0, mesh->stream, matrix, nd,
+                     mesh->nm, (const double2*)ddiv_bar, (const double2*)dlnps_bar,
+                     (const double2*)dt_bar, (double2*)out, S);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

@@ -141,26 +141,7 @@ __device__ inline void four_in_one_column(const VC& v, double rdgas, double cp_a
 // hydrostatic integral, column energy) add per-wave partial sums through LDS.
 // Sums over levels are therefore grouped by wave rather than strictly
 // sequential; the FP64 results differ from a sequential sum only by rounding.
-constexpr int TILE = 64;
 constexpr int MAXNQ = 8;
-
-struct TileIdx {
-  int lane, q, nq, k0;
-  size_t c;   // column (grid point or spectral coefficient)
-  bool col;   // column exists
-};
-
-template <int KPT>
-__device__ __forceinline__ TileIdx tile_idx(size_t ncol) {
-  TileIdx t;
-  t.lane = threadIdx.x & (TILE - 1);
-  t.q = threadIdx.x / TILE;
-  t.nq = blockDim.x / TILE;
-  t.k0 = t.q * KPT;
-  t.c = (size_t)blockIdx.x * TILE + t.lane;
-  t.col = t.c < ncol;
-  return t;
-}
 
 __device__ __forceinline__ double2 d2(double x, double y) { return make_double2(x, y); }
 __device__ __forceinline__ double2 d2add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
@@ -185,12 +166,6 @@ __device__ inline void level_sums(double2* red, const TileIdx& t, double2 part, 
   below = b;
 }
 
-// Sum over the 64 lanes of a wave in a fixed order (result in lane 0).
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = TILE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, TILE);
-  return v;
-}
 
 // Pressure_Variables! (Press_And_Geopot.jl:36-67) for the levels of one tile:
 // ph/lnh at half levels k0 .. k0+KPT, dp/lnf/pf at full levels k0 .. k0+KPT-1.

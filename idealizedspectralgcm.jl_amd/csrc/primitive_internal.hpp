@@ -50,4 +50,57 @@ inline VC vc_of(const jgcm_vert_coord* v) {
             v->d_coef + 2 * (v->nd + 1), v->d_coef + 2 * (v->nd + 1) + v->nd};
 }
 
+// Column tiles: a workgroup owns 64 consecutive columns (one per lane) and
+// nq = blockDim.x / 64 waves; wave q works on levels q*KPT .. q*KPT + KPT - 1.
+constexpr int TILE = 64;
+
+struct TileIdx {
+  int lane, q, nq, k0;
+  size_t c;   // column (grid point or spectral coefficient)
+  bool col;   // column exists
+};
+
+template <int KPT>
+__device__ __forceinline__ TileIdx tile_idx(size_t ncol) {
+  TileIdx t;
+  t.lane = threadIdx.x & (TILE - 1);
+  t.q = threadIdx.x / TILE;
+  t.nq = blockDim.x / TILE;
+  t.k0 = t.q * KPT;
+  t.c = (size_t)blockIdx.x * TILE + t.lane;
+  t.col = t.c < ncol;
+  return t;
+}
+
+// Sum over the 64 lanes of a wave in a fixed order (result in lane 0).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = TILE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, TILE);
+  return v;
+}
+
+// N per-thread partials summed, per lane, over the waves above (smaller level
+// index) and below this one, in wave order.  `red` holds N * nq * 64 doubles;
+// every thread of the workgroup must call it (two barriers, `red` is free
+// again on return).
+template <int N>
+__device__ inline void level_sums_n(double* red, const TileIdx& t, const double (&part)[N],
+                                    double (&above)[N], double (&below)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) red[(i * t.nq + t.q) * TILE + t.lane] = part[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < t.nq; ++w) {
+      const double x = red[(i * t.nq + w) * TILE + t.lane];
+      if (w < t.q) a += x;
+      if (w > t.q) b += x;
+    }
+    above[i] = a;
+    below[i] = b;
+  }
+  __syncthreads();
+}
+
 }  // namespace jgcm

@@ -255,12 +255,11 @@ int jgcm_sw_step_adjoint(jgcm_leapfrog_t* lf, int init_step, const double* grid_
  *                u_p, v_p, T_p, ps_p, ps_c (same shapes as the state)
  *   scal_bar   : device; [1], [2] = cotangents of the energy integral and mean ps
  *                of the p-fields (from jgcm_hs_corrections_adjoint)
- *   workspace  : 4 nd nθ nλ doubles
  *   param_bar  : NULL, or [4][nθ][nλ] doubles to which each column adds its share
  *                of the cotangents of k_f, k_a, k_s and ΔT_y (HS_Forcing.jl:8-13)  */
 int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, double dt,
                            const double* scal_bar, const double* const* state,
-                           const double* const* cot_in, double* const* cot_out, double* workspace,
+                           const double* const* cot_in, double* const* cot_out,
                            double* param_bar);
 /* transpose of the mass and energy corrections (Spectral_Dynamics.jl:30-68).
  * scal: the step's seven correction scalars (device, jgcm_hs_scalars_device);
