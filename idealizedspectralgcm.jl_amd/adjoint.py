@@ -424,6 +424,8 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         mesh._semi_implicit = prev
         Get_Topography(self._d.grid_geopots)
         self._matrices = {}
+        self._pool = {}
+        self._scal_bar = torch.zeros(4, dtype=torch.float64, device=mesh.device)
         self._param_b = torch.zeros((4, mesh.nθ, mesh.nλ), dtype=torch.float64, device=mesh.device)
         self.parameter_gradients = {}
 
@@ -558,7 +560,25 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         vor_p, vor_c, div_p, div_c, t_p, t_c, lnps_p, lnps_c, u_pg, v_pg, t_pg, ps_pg = st
         Bvp, Bvc, Bdp, Bdc, Btp, Btc, Blp, Blc, Bup, Bvpg, Btpg, Bpsp = bars
         si, I = self._solver(k), self._integrator(k)
-        S, G = M.spectral_zeros, M.grid_zeros
+        # temporaries: one zero-initialised buffer per allocation site, kept
+        # across steps (the n-th request of a step always comes from the same
+        # call, whose producer overwrites it), so no per-step fill is launched
+        site = [0]
+
+        def take(kind, nlev):
+            key = (site[0], kind, nlev)
+            site[0] += 1
+            t = self._pool.get(key)
+            if t is None:
+                t = (M.spectral_zeros if kind == "s" else M.grid_zeros)(nlev)
+                self._pool[key] = t
+            return t
+
+        def S(nlev):
+            return take("s", nlev)
+
+        def G(nlev):
+            return take("g", nlev)
 
         def add(dst, src):          # dst += src (same shape), on the mesh stream
             call("jgcm_axpby", h, dst.numel() * dst.element_size() // 8, 1.0, _ptr(dst), 1.0,
@@ -577,7 +597,7 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         # 1. mass and energy corrections (Spectral_Dynamics.jl:30-68); the scalar
         #    cotangents stay on the device (no wait for the stream)
         u_n, v_n, t_nc, ps_nc = nxt
-        scal_bar = torch.zeros(4, dtype=torch.float64, device=M.device)
+        scal_bar = self._scal_bar          # stays zero without the corrections
         if self._corrections:
             ub, vb, tb, lb = G(nd), G(nd), G(nd), G(1)
             call("jgcm_hs_corrections_adjoint", si.handle, *map(_ptr, (
