@@ -13,6 +13,7 @@
 // the FP64 rounding matches term by term (contraction off).
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <cmath>
 #include <new>
 #include <vector>
@@ -751,13 +752,30 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     double xi, SIRef r, const double* __restrict__ eig, const double* __restrict__ wave,
     int num_wavenumbers, const double2* div_c, const double2* div_p, const double2* lnps_c,
     const double2* lnps_p, const double2* t_c, const double2* t_p, double2* ddiv, double2* dlnps,
-    double2* dtt, double2* energy) {
-  extern __shared__ double2 lds[];  // nd * 64 entries: level sums, then the wave-matrix operand
+    double2* dtt, double2* energy, int stage_w) {
+  // LDS: nd * 64 double2 (level sums, then the wave-matrix operand), then, when
+  // stage_w, the wave matrices W_n of the tile's degrees n_lo .. n_hi
+  extern __shared__ double2 lds[];
   const TileIdx tx = tile_idx<KPT>(spec_per_level);
   const size_t idx = tx.c;
   const int m = tx.col ? (int)(idx % nm) : 0;
   const int n = tx.col ? (int)(idx / nm) : 0;
   const int k0 = tx.k0;
+  // the tile's 64 consecutive columns (idx = n nm + m) span degrees n_lo .. n_hi;
+  // their W_n are copied to LDS up front (one round trip, overlapped with the
+  // level sums below) instead of being read per product term
+  const size_t first = (size_t)blockIdx.x * TILE;
+  const size_t last = min(first + TILE, spec_per_level) - 1;
+  const int n_lo = (int)(first / nm);
+  const int nnd = nd * nd;
+  double* Wl = reinterpret_cast<double*>(lds + (size_t)nd * TILE);
+  if (stage_w) {
+    const int nW = (int)(last / nm) - n_lo + 1;
+    for (int q = threadIdx.x; q < nW * nnd; q += blockDim.x) {
+      const int nn = n_lo + q / nnd;
+      Wl[q] = nn <= num_wavenumbers ? wave[(size_t)nn * nnd + q % nnd] : 0.0;
+    }
+  }
   double2 A[KPT], B[KPT], G[KPT], DD[KPT];
   // Adjust_δlnps_δt_δdiv!: δdiv_temp = div_p - div_c
 #pragma unroll
@@ -808,7 +826,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     if (k0 + i < nd) lds[(k0 + i) * TILE + tx.lane] = A[i];
   __syncthreads();
   if (tx.col && n >= m && n <= num_wavenumbers) {
-    const double* Wn = wave + (size_t)n * nd * nd;
+    const double* Wn = stage_w ? Wl + (size_t)(n - n_lo) * nnd : wave + (size_t)n * nnd;
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
       int k = k0 + i;
@@ -1212,12 +1230,17 @@ static int implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, co
   size_t S = M->spec_elems();
   double xi = xi_of(si->lf);
   size_t lds = (size_t)si->nd * TILE * sizeof(double2);
+  // wave matrices of at most TILE / nm + 2 degrees per tile, staged when they fit
+  const size_t nw = std::min<size_t>((size_t)M->ns + 1, TILE / (size_t)M->nm + 2);
+  const size_t wbytes = nw * si->nd * si->nd * sizeof(double);
+  const int stage_w = lds + wbytes <= 48 * 1024 ? 1 : 0;
+  if (stage_w) lds += wbytes;
   JGCM_TILE_DISPATCH(si->nd, hs_implicit_kernel, tile_grid(S), tile_block(si->nd), lds, M->stream,
                      si->nd, M->nm, M->ns, S, si->atmo.kappa, si->atmo.rdgas, si->ps_ref, xi,
                      siref(si), M->d_eig, si->d_wave, si->num_wavenumbers, (const double2*)div_c,
                      (const double2*)div_p, (const double2*)lnps_c, (const double2*)lnps_p,
                      (const double2*)t_c, (const double2*)t_p, (double2*)ddiv, (double2*)dlnps,
-                     (double2*)dt, (double2*)energy);
+                     (double2*)dt, (double2*)energy, stage_w);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

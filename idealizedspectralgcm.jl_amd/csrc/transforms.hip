@@ -581,14 +581,17 @@ __device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS,
   }
 }
 
-// Two staged copies whose global loads are issued together (up to 8 + 8 per
-// thread in flight), then written to LDS.
+// Two staged copies whose global loads are issued together (up to IT + IT
+// per thread in flight), then written to LDS.
+#ifndef JGCM_STAGE_IT
+#define JGCM_STAGE_IT 8
+#endif
 template <typename L1, typename S1, typename L2, typename S2>
 __device__ __forceinline__ void staged_copy2(int n1, L1 load1, S1 store1, int n2, L2 load2,
                                              S2 store2) {
   using T1 = decltype(load1(0));
   using T2 = decltype(load2(0));
-  constexpr int IT = 8;
+  constexpr int IT = JGCM_STAGE_IT;
   const int nt = blockDim.x;
   const int n = max(n1, n2);
   for (int base = 0; base < n; base += IT * nt) {

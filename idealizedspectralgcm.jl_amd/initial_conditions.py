@@ -30,7 +30,8 @@ from .mesh import (Trans_Grid_To_Spherical, Trans_Spherical_To_Grid, UV_Grid_Fro
 
 
 def _host_to(dst, arr):
-    dst.copy_(torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).reshape(dst.shape))
+    host = np.array(arr, dtype=np.float64, order="C", copy=True)   # writable, contiguous
+    dst.copy_(torch.from_numpy(host).reshape(dst.shape))
 
 
 def _previous_equals_current(d, spectral, grid):
