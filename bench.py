@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-hs", action="store_true")
     p.add_argument("--hs-steps", type=int, default=50)
+    p.add_argument("--no-extra", action="store_true",
+                   help="skip the ensemble, Galewsky and adjoint legs")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run the library's copy kernel over 256 MiB (PMC byte calibration)")
     return p.parse_args()
@@ -456,9 +458,10 @@ def main():
         hs["t42_l20_fp64"] = hs_bench(jgcm, torch, 42, 64, 20, 600, a.hs_steps, 5, "fp64")
         hs["split32_speedup_vs_fp64"] = hs["fp64"]["ms_per_step"] / hs["split32"]["ms_per_step"]
         out["held_suarez"] = hs
-        out["ensemble"] = ensemble_bench(jgcm, torch)
-        out["shallow_water_galewsky"] = galewsky_bench(jgcm, torch)
-        out["adjoint"] = {p: adjoint_bench(jgcm, torch, p) for p in ("fp64", "split32")}
+        if not a.no_extra:
+            out["ensemble"] = ensemble_bench(jgcm, torch)
+            out["shallow_water_galewsky"] = galewsky_bench(jgcm, torch)
+            out["adjoint"] = {p: adjoint_bench(jgcm, torch, p) for p in ("fp64", "split32")}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         if not a.no_hs:

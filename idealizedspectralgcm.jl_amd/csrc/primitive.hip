@@ -148,6 +148,10 @@ __device__ __forceinline__ double2 d2(double x, double y) { return make_double2(
 __device__ __forceinline__ double2 d2add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 d2sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 d2scale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+// a if keep, else 0, selected per component (see hs_implicit_kernel)
+__device__ __forceinline__ double2 d2keep(bool keep, double2 a) {
+  return make_double2(keep ? a.x : 0.0, keep ? a.y : 0.0);
+}
 
 // Sums of a per-thread partial over the waves above (smaller level index) and
 // below this one, per lane.  `red` holds nq * 64 entries; every thread of the
@@ -749,26 +753,39 @@ __device__ __forceinline__ void linear_geopot_tile(int nd, double rdgas, const S
 template <int KPT>
 __global__ __launch_bounds__(512) void hs_implicit_kernel(
     int nd, int nm, int ns, size_t spec_per_level, double kappa, double rdgas, double ps_ref,
-    double xi, SIRef r, const double* __restrict__ eig, const double* __restrict__ wave,
+    double xi, SIRef rg, const double* __restrict__ eig, const double* __restrict__ wave,
     int num_wavenumbers, const double2* div_c, const double2* div_p, const double2* lnps_c,
     const double2* lnps_p, const double2* t_c, const double2* t_p, double2* ddiv, double2* dlnps,
     double2* dtt, double2* energy, int stage_w) {
-  // LDS: nd * 64 double2 (level sums, then the wave-matrix operand), then, when
-  // stage_w, the wave matrices W_n of the tile's degrees n_lo .. n_hi
+  // LDS: nd * 64 double2 (level sums, then the wave-matrix operand); the
+  // reference profiles (6 x (nd+1) doubles); when stage_w, the wave matrices
+  // W_n of the tile's degrees n_lo .. n_hi.  Everything a thread reads from
+  // global memory is requested at entry, before the first barrier, so the
+  // column work below waits for one round trip rather than one per stage.
   extern __shared__ double2 lds[];
   const TileIdx tx = tile_idx<KPT>(spec_per_level);
   const size_t idx = tx.c;
   const int m = tx.col ? (int)(idx % nm) : 0;
   const int n = tx.col ? (int)(idx / nm) : 0;
   const int k0 = tx.k0;
-  // the tile's 64 consecutive columns (idx = n nm + m) span degrees n_lo .. n_hi;
-  // their W_n are copied to LDS up front (one round trip, overlapped with the
-  // level sums below) instead of being read per product term
+  const int n1 = nd + 1;
+  double* refs = reinterpret_cast<double*>(lds + (size_t)nd * TILE);
+  for (int k = threadIdx.x; k < n1; k += blockDim.x) {
+    const bool in = k < nd;
+    refs[k] = in ? rg.dp[k] : 0.0;
+    refs[n1 + k] = rg.lnp_half[k];
+    refs[2 * n1 + k] = in ? rg.lnp_full[k] : 0.0;
+    refs[3 * n1 + k] = in ? rg.t_ref[k] : 0.0;
+    refs[4 * n1 + k] = in ? rg.h[k] : 0.0;
+    refs[5 * n1 + k] = rg.bk[k];
+  }
+  const SIRef r{refs, refs + n1, refs + 2 * n1, refs + 3 * n1, refs + 4 * n1, refs + 5 * n1};
+  // the tile's 64 consecutive columns (idx = n nm + m) span degrees n_lo .. n_hi
   const size_t first = (size_t)blockIdx.x * TILE;
   const size_t last = min(first + TILE, spec_per_level) - 1;
   const int n_lo = (int)(first / nm);
   const int nnd = nd * nd;
-  double* Wl = reinterpret_cast<double*>(lds + (size_t)nd * TILE);
+  double* Wl = refs + 6 * n1;
   if (stage_w) {
     const int nW = (int)(last / nm) - n_lo + 1;
     for (int q = threadIdx.x; q < nW * nnd; q += blockDim.x) {
@@ -776,44 +793,58 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
       Wl[q] = nn <= num_wavenumbers ? wave[(size_t)nn * nnd + q % nnd] : 0.0;
     }
   }
+  // per-thread inputs: every load reads a valid address (element 0 stands in
+  // for the columns and levels this thread does not own) and the components
+  // are selected afterwards.  A conditional on double2 objects
+  // (ok ? x[o] : zero) compiles to a select between the objects' addresses —
+  // generic (flat) loads that also wait for this thread's earlier stores, and
+  // a private copy of the zero
+  double2 Dp[KPT], Dc[KPT], Tp[KPT], Tc[KPT], DT[KPT], DV[KPT], EN[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    const bool ok = tx.col && k0 + i < nd;
+    const size_t o = ok ? (size_t)(k0 + i) * spec_per_level + idx : 0;
+    Dp[i] = d2keep(ok, div_p[o]);
+    Dc[i] = d2keep(ok, div_c[o]);
+    Tp[i] = d2keep(ok, t_p[o]);
+    Tc[i] = d2keep(ok, t_c[o]);
+    DT[i] = d2keep(ok, dtt[o]);
+    DV[i] = d2keep(ok, ddiv[o]);
+    EN[i] = energy ? d2keep(ok, energy[o]) : d2(0.0, 0.0);
+  }
+  const size_t ic = tx.col ? idx : 0;
+  const double2 dl0 = d2keep(tx.col, dlnps[ic]);
+  const double2 lnc = d2keep(tx.col, lnps_c[ic]);
+  const double2 lnp = d2keep(tx.col, lnps_p[ic]);
+  const double e = eig[m * (ns + 1) + n];
+  __syncthreads();  // refs and Wl
   double2 A[KPT], B[KPT], G[KPT], DD[KPT];
   // Adjust_δlnps_δt_δdiv!: δdiv_temp = div_p - div_c
 #pragma unroll
-  for (int i = 0; i < KPT; ++i) {
-    bool ok = tx.col && k0 + i < nd;
-    size_t o = (size_t)(k0 + i) * spec_per_level + idx;
-    A[i] = ok ? d2sub(div_p[o], div_c[o]) : d2(0.0, 0.0);
-  }
+  for (int i = 0; i < KPT; ++i) A[i] = d2sub(Dp[i], Dc[i]);
   double2 dps;
   linear_ps_t_tile<KPT>(nd, kappa, r, tx, lds, A, dps, B);  // B = δt_temp, dps = δps_temp
-  double2 dl = tx.col ? dlnps[idx] : d2(0.0, 0.0);
-  dl = d2(dl.x + dps.x / ps_ref, dl.y + dps.y / ps_ref);
+  const double2 dl = d2(dl0.x + dps.x / ps_ref, dl0.y + dps.y / ps_ref);
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
-    bool ok = tx.col && k0 + i < nd;
-    size_t o = (size_t)(k0 + i) * spec_per_level + idx;
-    double2 d = ok ? dtt[o] : d2(0.0, 0.0);
-    d = d2add(d, B[i]);
+    const bool ok = tx.col && k0 + i < nd;
+    const double2 d = d2add(DT[i], B[i]);
     DD[i] = d;
     // δt_temp = t_p - t_c + ξ δt
-    A[i] = ok ? d2(t_p[o].x - t_c[o].x + xi * d.x, t_p[o].y - t_c[o].y + xi * d.y) : d2(0.0, 0.0);
+    A[i] = ok ? d2(Tp[i].x - Tc[i].x + xi * d.x, Tp[i].y - Tc[i].y + xi * d.y) : d2(0.0, 0.0);
   }
-  const double2 lnc = tx.col ? lnps_c[idx] : d2(0.0, 0.0);
-  const double2 lnp = tx.col ? lnps_p[idx] : d2(0.0, 0.0);
   const double2 lt = d2(lnp.x - lnc.x + xi * dl.x, lnp.y - lnc.y + xi * dl.y);
   linear_geopot_tile<KPT>(nd, rdgas, r, tx, lds, A, G);
-  const double e = eig[m * (ns + 1) + n];
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
-    int k = k0 + i;
-    bool ok = tx.col && k < nd;
-    size_t o = (size_t)k * spec_per_level + idx;
-    double hp = ok ? r.h[k] * ps_ref : 0.0;
-    double2 d = ok ? ddiv[o] : d2(0.0, 0.0);
+    const int k = k0 + i;
+    const bool ok = tx.col && k < nd;
+    const size_t o = (size_t)k * spec_per_level + idx;
+    const double hp = ok ? r.h[k] * ps_ref : 0.0;
+    double2 d = DV[i];
     if (energy && ok) {
       // Apply_Laplacian!(spe_energy); spe_δdiv .-= spe_energy  (SD:309-310)
-      double2 en = energy[o];
-      double2 lap = d2(e * en.x, e * en.y);
+      const double2 lap = d2(e * EN[i].x, e * EN[i].y);
       energy[o] = lap;
       d = d2sub(d, lap);
     }
@@ -826,18 +857,36 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     if (k0 + i < nd) lds[(k0 + i) * TILE + tx.lane] = A[i];
   __syncthreads();
   if (tx.col && n >= m && n <= num_wavenumbers) {
-    const double* Wn = stage_w ? Wl + (size_t)(n - n_lo) * nnd : wave + (size_t)n * nnd;
+    // two loops, so that each reads through one address space (an LDS/global
+    // select would turn every read into a generic access)
+    if (stage_w) {
+      const double* Wn = Wl + (size_t)(n - n_lo) * nnd;
 #pragma unroll
-    for (int i = 0; i < KPT; ++i) {
-      int k = k0 + i;
-      if (k >= nd) continue;
-      double2 s = d2(0.0, 0.0);
-      for (int qq = 0; qq < nd; ++qq) {
-        double w = Wn[k * nd + qq];
-        double2 x = lds[qq * TILE + tx.lane];
-        s = d2(s.x + w * x.x, s.y + w * x.y);
+      for (int i = 0; i < KPT; ++i) {
+        int k = k0 + i;
+        if (k >= nd) continue;
+        double2 s = d2(0.0, 0.0);
+        for (int qq = 0; qq < nd; ++qq) {
+          double w = Wn[k * nd + qq];
+          double2 x = lds[qq * TILE + tx.lane];
+          s = d2(s.x + w * x.x, s.y + w * x.y);
+        }
+        A[i] = s;
       }
-      A[i] = s;
+    } else {
+      const double* Wn = wave + (size_t)n * nnd;
+#pragma unroll
+      for (int i = 0; i < KPT; ++i) {
+        int k = k0 + i;
+        if (k >= nd) continue;
+        double2 s = d2(0.0, 0.0);
+        for (int qq = 0; qq < nd; ++qq) {
+          double w = Wn[k * nd + qq];
+          double2 x = lds[qq * TILE + tx.lane];
+          s = d2(s.x + w * x.x, s.y + w * x.y);
+        }
+        A[i] = s;
+      }
     }
   }
   __syncthreads();
@@ -1229,7 +1278,7 @@ static int implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, co
   jgcm_mesh* M = si->vc->mesh;
   size_t S = M->spec_elems();
   double xi = xi_of(si->lf);
-  size_t lds = (size_t)si->nd * TILE * sizeof(double2);
+  size_t lds = (size_t)si->nd * TILE * sizeof(double2) + (size_t)6 * (si->nd + 1) * sizeof(double);
   // wave matrices of at most TILE / nm + 2 degrees per tile, staged when they fit
   const size_t nw = std::min<size_t>((size_t)M->ns + 1, TILE / (size_t)M->nm + 2);
   const size_t wbytes = nw * si->nd * si->nd * sizeof(double);
