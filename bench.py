@@ -40,6 +40,9 @@ def parse():
     p.add_argument("--hs-steps", type=int, default=50)
     p.add_argument("--no-extra", action="store_true",
                    help="skip the ensemble, Galewsky and adjoint legs")
+    p.add_argument("--no-tune", action="store_true",
+                   help="take the FFT shapes from the tuning cache (no timing runs); "
+                        "the built-in rule where the cache has none")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run the library's copy kernel over 256 MiB (PMC byte calibration)")
     return p.parse_args()
@@ -99,6 +102,15 @@ HS_PARAMS = {"σ_b": 0.7, "k_f": 1.0, "k_a": 1.0 / 40.0, "k_s": 1.0 / 4.0, "ΔT_
              "Δθ_z": 10.0}
 
 
+def tuning_path():
+    """Autotuner cache of this run (JGCM_TUNING_FILE, else under gpurun_out/ or /tmp)."""
+    p = os.environ.get("JGCM_TUNING_FILE")
+    if p:
+        return p
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpurun_out")
+    return os.path.join(d if os.path.isdir(d) else "/tmp", "jgcm_tuning.json")
+
+
 def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):
     """Held-Suarez primitive-equation step (Atmosphere_Update!) on the device,
     T85 L20 with Δt = 300 s by default; isothermal 300 K at rest, ps = 1e5 Pa,
@@ -147,8 +159,10 @@ def ensemble_bench(jgcm, torch, steps=100):
     Reports instance-steps/s and the p50/p99 latency of one ensemble step,
     beside the cost of looping the plain FP64 path one instance at a time."""
     from jgcm_amd import ensemble as E
+    from jgcm_amd import autotune
     groups = [(21, 32, 64, 1800), (42, 64, 128, 1200), (85, 128, 64, 600)]
-    ens = E.SW_Ensemble(groups, seed=11, precision="fp32")
+    cache = autotune.Tuning_Cache(tuning_path())
+    ens = E.SW_Ensemble(groups, seed=11, precision="fp32", tuning=cache)
     ens.step()
     ens.capture()
     for _ in range(10):
@@ -183,6 +197,7 @@ def ensemble_bench(jgcm, torch, steps=100):
         loop_ms += count * e0.elapsed_time(e1) / 20
     n = ens.instances
     return {"instances": n, "groups": "64xT21, 128xT42, 64xT85", "precision": "fp32 Legendre, fp64 state",
+            "fft_shape_fold_unfold": ens.tuning,
             "ms_per_ensemble_step": total, "instance_steps_per_s": n / (total * 1e-3),
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "loop_fp64_ms_per_step": loop_ms, "speedup_vs_fp64_loop": loop_ms / total,
@@ -377,10 +392,19 @@ def main():
         _lib.check(L.jgcm_trans_spherical_to_grid(h, ctypes.c_void_p(s.data_ptr()),
                                                   ctypes.c_void_p(g.data_ptr()), NFIELDS))
 
+    from jgcm_amd import autotune
+    cache = autotune.Tuning_Cache(tuning_path())
     results = {}
     ref_grid = ref_spec = None
     for prec in ("fp64", "split32", "bf16x3"):
         mesh.set_precision(prec)
+        # FFT workgroup shape for this precision and batch, picked by the autotuner
+        if a.no_tune:
+            autotune.set_fft_shape(mesh, 0, 0)
+            shape = autotune.Apply_Tuning(mesh, NFIELDS, cache=cache) or \
+                {"fold_ppw": 0, "unfold_ppw": 0}
+        else:
+            shape = autotune.Tune_FFT_Shape(mesh, NFIELDS, cache=cache)
         for _ in range(a.warmup):
             inv(spec_in, grid)
             fwd(grid, spec)
@@ -426,8 +450,11 @@ def main():
         ms_step = total_ms / a.steps
         results[prec] = {"ms_per_step": ms_step, "fields_per_s": 2 * NFIELDS / (ms_step * 1e-3),
                          "ms_inverse": t_inv, "ms_forward": t_fwd, "wall_s": wall,
-                         "max_rel_err_vs_fp64": err}
+                         "max_rel_err_vs_fp64": err,
+                         "fft_shape": {"fold_ppw": shape["fold_ppw"],
+                                       "unfold_ppw": shape["unfold_ppw"]}}
     mesh.set_precision("fp64")
+    autotune.set_fft_shape(mesh, 0, 0)
     use = "split32" if results["split32"]["max_rel_err_vs_fp64"] <= 1e-6 else "fp64"
     r = results[use]
     # roofline of the dominant launch (the slower direction), algorithmic bytes

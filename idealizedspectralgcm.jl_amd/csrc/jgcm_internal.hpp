@@ -97,6 +97,8 @@ struct jgcm_mesh {
   int nrows = 0;    // packed rows: sum_m (ns + 1 - m)
   double radius = 0.0;
   int precision = JGCM_PREC_FP64;
+  // latitude pairs per FFT workgroup (0: the built-in rule), set by an autotuner
+  int fold_ppw = 0, unfold_ppw = 0;
   hipStream_t stream = nullptr;
   bool stream_borrowed = false;   // stream owned by another mesh (jgcm_mesh_share_stream)
   int device = 0;
@@ -160,4 +162,5 @@ struct jgcm_leapfrog {
 namespace jgcm {
 int ensure_scratch(jgcm_mesh* m, int nfields);
 int check_mesh(const jgcm_mesh* m);
+int fft_max_ppw(const jgcm_mesh* m);  // largest FFT latitude-pair count per workgroup
 }  // namespace jgcm

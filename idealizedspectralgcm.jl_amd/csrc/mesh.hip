@@ -439,6 +439,27 @@ int jgcm_mesh_set_precision(jgcm_mesh_t* mesh, int precision) {
   return JGCM_OK;
 }
 
+int jgcm_mesh_set_fft_shape(jgcm_mesh_t* mesh, int fold_ppw, int unfold_ppw) {
+  if (int rc = check_mesh(mesh)) return rc;
+  const int limit = fft_max_ppw(mesh);
+  JGCM_REQUIRE(fold_ppw >= 0 && fold_ppw <= limit, "fold_ppw must be in 0..%d, got %d", limit,
+               fold_ppw);
+  JGCM_REQUIRE(unfold_ppw >= 0 && unfold_ppw <= limit, "unfold_ppw must be in 0..%d, got %d",
+               limit, unfold_ppw);
+  mesh->fold_ppw = fold_ppw;
+  mesh->unfold_ppw = unfold_ppw;
+  return JGCM_OK;
+}
+
+int jgcm_mesh_get_fft_shape(jgcm_mesh_t* mesh, int* fold_ppw, int* unfold_ppw, int* max_ppw) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(fold_ppw && unfold_ppw && max_ppw, "NULL argument");
+  *fold_ppw = mesh->fold_ppw;
+  *unfold_ppw = mesh->unfold_ppw;
+  *max_ppw = fft_max_ppw(mesh);
+  return JGCM_OK;
+}
+
 int jgcm_mesh_get_precision(jgcm_mesh_t* mesh, int* precision) {
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_REQUIRE(precision != nullptr, "precision is NULL");

@@ -914,6 +914,18 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
 // measured faster for the fold at every batch size and for the unfold below
 // ~48 level-fields (T85: fold -1.2 us at 20 and 61 level-fields, unfold
 // -1.3 us at 20, +0.9 us at 61).
+// Largest latitude-pair count whose staging buffer, with the kernels' static
+// LDS (FPW + 1 rows of L complex values), fits in 64 KB.
+int fft_max_ppw(const jgcm_mesh* M) {
+  const int L = M->nlon;
+  const int fpw = 256 / (L / 4);
+  const long stat = (long)(fpw + 1) * L * 16;
+  const long per = (long)(2 * M->nm + 1) * 16;
+  long mx = (64L * 1024 - stat) / per;
+  if (mx > M->half) mx = M->half;
+  return mx < 1 ? 1 : (int)mx;
+}
+
 static int pick_ppw(int L, int B, bool fold) {
   int fpw = 256 / (L / 4);
   if (fpw == 4 && (fold || B < 48)) return 4;
@@ -922,7 +934,7 @@ static int pick_ppw(int L, int B, bool fold) {
 
 template <int L>
 static void launch_fold(const jgcm_mesh* M, const SegMap& in, double2* eo, int B) {
-  int ppw = pick_ppw(L, B, true);
+  int ppw = M->fold_ppw > 0 ? M->fold_ppw : pick_ppw(L, B, true);
   dim3 grd((M->half + ppw - 1) / ppw, B);
   size_t lds = (size_t)(M->nm * 2 + 1) * ppw * sizeof(double2);
   hipLaunchKernelGGL(fft_fold_kernel<L>, grd, dim3(256), lds, M->stream, in, eo,
@@ -931,7 +943,7 @@ static void launch_fold(const jgcm_mesh* M, const SegMap& in, double2* eo, int B
 
 template <int L>
 static void launch_unfold(const jgcm_mesh* M, const double2* eo, const SegMap& out, int B) {
-  int ppw = pick_ppw(L, B, false);
+  int ppw = M->unfold_ppw > 0 ? M->unfold_ppw : pick_ppw(L, B, false);
   dim3 grd((M->half + ppw - 1) / ppw, B);
   size_t lds = (size_t)(M->nm * 2 + 1) * ppw * sizeof(double2);
   hipLaunchKernelGGL(ifft_unfold_kernel<L>, grd, dim3(256), lds, M->stream, eo, out,

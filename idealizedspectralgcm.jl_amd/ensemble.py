@@ -108,14 +108,22 @@ class SW_Group:
 class SW_Ensemble:
     """Mixed-truncation ensemble: groups = [(N, nθ, count, dt), ...]."""
 
-    def __init__(self, groups, seed=0, precision="fp32"):
+    def __init__(self, groups, seed=0, precision="fp32", tuning=None):
+        """`tuning`: None (built-in FFT shapes) or an autotune.Tuning_Cache; each
+        truncation's FFT shape (for its instance count) is then read from the
+        cache, or tuned and stored when missing, before any step is captured."""
+        from .autotune import Apply_Tuning
         self.groups = []
+        self.tuning = {}
         owner = None
         for i, (N, nθ, count, dt) in enumerate(groups):
             g = SW_Group(N, nθ, count, dt, seed + 1000 * i, precision=precision,
                          owner=owner.mesh if owner is not None else None)
             if owner is None:
                 owner = g
+            if tuning is not None:
+                e = Apply_Tuning(g.mesh, count, cache=tuning, tune_missing=True)
+                self.tuning[f"T{N}"] = None if e is None else (e["fold_ppw"], e["unfold_ppw"])
             self.groups.append(g)
         self.owner = owner.mesh
         self.graph = None

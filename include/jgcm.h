@@ -112,6 +112,15 @@ void* jgcm_mesh_stream(jgcm_mesh_t* mesh);
 int jgcm_mesh_share_stream(jgcm_mesh_t* mesh, jgcm_mesh_t* owner);
 int jgcm_mesh_set_precision(jgcm_mesh_t* mesh, int precision);
 int jgcm_mesh_get_precision(jgcm_mesh_t* mesh, int* precision);
+/* FFT workgroup shape (no reference counterpart: FFTW plans its own): the
+ * number of latitude pairs one workgroup of the longitude FFT handles, for the
+ * grid -> Fourier fold (forward transforms) and the Fourier -> grid unfold
+ * (inverse transforms); 0 restores the built-in rule.  Results do not depend
+ * on the shape (each FFT's arithmetic is the same), only the launch does.
+ * Values above *max_ppw (the LDS limit) are rejected.  Set by the autotuner
+ * (autotune.py), which times the candidates and keeps the fastest.          */
+int jgcm_mesh_set_fft_shape(jgcm_mesh_t* mesh, int fold_ppw, int unfold_ppw);
+int jgcm_mesh_get_fft_shape(jgcm_mesh_t* mesh, int* fold_ppw, int* unfold_ppw, int* max_ppw);
 
 /* Host copies of the mesh's public fields (the Julia struct's fields). */
 #define JGCM_TABLE_SIN_THETA 0     /* [nθ]                         */
