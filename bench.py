@@ -195,8 +195,26 @@ def ensemble_bench(jgcm, torch, steps=100):
         e1.record(st)
         one.mesh.synchronize()
         loop_ms += count * e0.elapsed_time(e1) / 20
+    # the same step with the truncations in sequence on one stream
+    seq = E.SW_Ensemble(groups, seed=11, precision="fp32", tuning=cache, concurrent=False)
+    seq.step()
+    seq.capture()
+    for _ in range(10):
+        seq.launch()
+    seq.owner.synchronize()
+    sst = torch.cuda.ExternalStream(seq.owner.stream())
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s0.record(sst)
+    for _ in range(steps):
+        seq.launch()
+    s1.record(sst)
+    seq.owner.synchronize()
+    seq_ms = s0.elapsed_time(s1) / steps
+    seq.graph.close()
     n = ens.instances
     return {"instances": n, "groups": "64xT21, 128xT42, 64xT85", "precision": "fp32 Legendre, fp64 state",
+            "truncations": "parallel branches of one graph",
+            "sequential_ms_per_ensemble_step": seq_ms,
             "fft_shape_fold_unfold": ens.tuning,
             "ms_per_ensemble_step": total, "instance_steps_per_s": n / (total * 1e-3),
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

@@ -47,6 +47,7 @@ SIGNATURES = {
     "jgcm_mesh_synchronize": [P],
     "jgcm_mesh_stream": [P],
     "jgcm_mesh_share_stream": [P, P],
+    "jgcm_mesh_wait": [P, P],
     "jgcm_mesh_set_precision": [P, I],
     "jgcm_mesh_get_precision": [P, ctypes.POINTER(I)],
     "jgcm_mesh_set_fft_shape": [P, I, I],

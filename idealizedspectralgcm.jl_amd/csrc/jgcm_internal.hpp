@@ -101,6 +101,7 @@ struct jgcm_mesh {
   int fold_ppw = 0, unfold_ppw = 0;
   hipStream_t stream = nullptr;
   bool stream_borrowed = false;   // stream owned by another mesh (jgcm_mesh_share_stream)
+  hipEvent_t signal = nullptr;    // recorded on this mesh's stream by jgcm_mesh_wait
   int device = 0;
 
   // host copies of public fields

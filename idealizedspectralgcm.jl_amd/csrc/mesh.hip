@@ -181,6 +181,7 @@ static void free_mesh(jgcm_mesh* m) {
                   m->d_grid_b, m->d_scalar};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (m->signal) (void)hipEventDestroy(m->signal);
   if (m->stream && !m->stream_borrowed) (void)hipStreamDestroy(m->stream);
   delete m;
 }
@@ -196,6 +197,7 @@ static int create_impl(int N, int ns, int nlat, int nlon, int nd, double radius,
   m->nm = N + 1;
   JGCM_CHECK_HIP(hipGetDevice(&m->device));
   JGCM_CHECK_HIP(hipStreamCreate(&m->stream));
+  JGCM_CHECK_HIP(hipEventCreateWithFlags(&m->signal, hipEventDisableTiming));
   hipStream_t st = m->stream;
 
   // ---- Gaussian nodes and weights (FP64, GPU) ----

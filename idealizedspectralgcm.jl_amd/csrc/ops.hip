@@ -365,6 +365,15 @@ int jgcm_mesh_reserve(jgcm_mesh_t* mesh, int nfields) {
   return ensure_scratch(mesh, nfields);
 }
 
+int jgcm_mesh_wait(jgcm_mesh_t* waiter, jgcm_mesh_t* signaller) {
+  if (int rc = check_mesh(waiter)) return rc;
+  if (int rc = check_mesh(signaller)) return rc;
+  if (waiter->stream == signaller->stream) return JGCM_OK;
+  JGCM_CHECK_HIP(hipEventRecord(signaller->signal, signaller->stream));
+  JGCM_CHECK_HIP(hipStreamWaitEvent(waiter->stream, signaller->signal, 0));
+  return JGCM_OK;
+}
+
 int jgcm_graph_begin(jgcm_mesh_t* mesh) {
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_CHECK_HIP(hipStreamBeginCapture(mesh->stream, hipStreamCaptureModeThreadLocal));

@@ -108,17 +108,23 @@ class SW_Group:
 class SW_Ensemble:
     """Mixed-truncation ensemble: groups = [(N, nθ, count, dt), ...]."""
 
-    def __init__(self, groups, seed=0, precision="fp32", tuning=None):
+    def __init__(self, groups, seed=0, precision="fp32", tuning=None, concurrent=True):
         """`tuning`: None (built-in FFT shapes) or an autotune.Tuning_Cache; each
         truncation's FFT shape (for its instance count) is then read from the
-        cache, or tuned and stored when missing, before any step is captured."""
+        cache, or tuned and stored when missing, before any step is captured.
+        `concurrent`: each truncation keeps its own stream and the captured step
+        runs the truncations as parallel branches of one graph (forked from and
+        joined back to the first group's stream with jgcm_mesh_wait); otherwise
+        all truncations share the first group's stream and run one after
+        another."""
         from .autotune import Apply_Tuning
         self.groups = []
         self.tuning = {}
+        self.concurrent = bool(concurrent)
         owner = None
         for i, (N, nθ, count, dt) in enumerate(groups):
-            g = SW_Group(N, nθ, count, dt, seed + 1000 * i, precision=precision,
-                         owner=owner.mesh if owner is not None else None)
+            share = owner.mesh if (owner is not None and not self.concurrent) else None
+            g = SW_Group(N, nθ, count, dt, seed + 1000 * i, precision=precision, owner=share)
             if owner is None:
                 owner = g
             if tuning is not None:
@@ -133,19 +139,31 @@ class SW_Ensemble:
         return sum(g.count for g in self.groups)
 
     def step(self):
+        """One eager step of every group (each on its stream; see synchronize)."""
         for g in self.groups:
             g.step()
+
+    def synchronize(self):
+        """Wait for every group's stream (graph launches complete on the owner's)."""
+        for g in self.groups:
+            g.mesh.synchronize()
 
     def capture(self):
         """Capture one later step of all groups into one graph (after the first step)."""
         assert all(g.steps >= 1 for g in self.groups), "take the first (init) step eagerly"
         for g in self.groups:
             g.mesh.reserve(4 * g.count)
+        self.synchronize()
+        others = [g for g in self.groups if g.mesh.stream() != self.owner.stream()]
 
         def body():
+            for g in others:                       # fork: branches start after the owner
+                call("jgcm_mesh_wait", g.mesh.handle, self.owner.handle)
             for g in self.groups:
                 Shallow_Water_Physics(g.d, 0.0, 0.0, g.h_eq)
                 Shallow_Water_Dynamics(g.mesh, g.atmo, g.h_0, g.d, g.integ)
+            for g in others:                       # join: the graph ends after every branch
+                call("jgcm_mesh_wait", self.owner.handle, g.mesh.handle)
 
         self.graph = Step_Graph(self.owner, body)
         return self.graph

@@ -520,6 +520,13 @@ int jgcm_graph_begin(jgcm_mesh_t* mesh);
 int jgcm_graph_end(jgcm_mesh_t* mesh, jgcm_graph_t** out);
 int jgcm_graph_launch(jgcm_graph_t* graph);
 int jgcm_graph_destroy(jgcm_graph_t* graph);
+/* Order two meshes' streams (no reference counterpart): work enqueued on
+ * `waiter` after this call starts after everything enqueued on `signaller`
+ * before it.  Inside a capture begun on `signaller` it forks `waiter` into the
+ * capture (its later calls become parallel branches of the same graph); the
+ * reverse call joins the branch back before jgcm_graph_end.  No-op when both
+ * meshes use one stream.                                                      */
+int jgcm_mesh_wait(jgcm_mesh_t* waiter, jgcm_mesh_t* signaller);
 
 #ifdef __cplusplus
 }

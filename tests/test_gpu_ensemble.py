@@ -31,18 +31,21 @@ def test_batched_instances_equal_single_runs(jgcm):
     assert float(g.d.grid_u_c.abs().max()) > 0.1
 
 
-def test_mixed_ensemble_graph_matches_eager(jgcm):
+@pytest.mark.parametrize("concurrent", [True, False])
+def test_mixed_ensemble_graph_matches_eager(jgcm, concurrent):
+    """Replays of the captured step (truncations as parallel graph branches, or
+    in sequence on one stream) equal eager steps bit for bit."""
     from jgcm_amd.ensemble import SW_Ensemble
-    groups = [(21, 32, 3, 600), (42, 64, 2, 600)]
-    A = SW_Ensemble(groups, seed=1, precision="fp32")
-    B = SW_Ensemble(groups, seed=1, precision="fp32")
+    groups = [(21, 32, 3, 600), (42, 64, 2, 600), (85, 128, 2, 300)]
+    A = SW_Ensemble(groups, seed=1, precision="fp32", concurrent=concurrent)
+    B = SW_Ensemble(groups, seed=1, precision="fp32", concurrent=concurrent)
     A.step()
     B.step()
     B.capture()
     for k in range(3):
         A.step()
         B.launch()
-        A.owner.synchronize()
+        A.synchronize()
         B.owner.synchronize()
         for ga, gb in zip(A.groups, B.groups):
             for f in ("spe_vor_c", "spe_lnps_c", "grid_u_c"):
