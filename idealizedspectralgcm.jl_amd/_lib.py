@@ -48,6 +48,8 @@ SIGNATURES = {
     "jgcm_mesh_stream": [P],
     "jgcm_mesh_share_stream": [P, P],
     "jgcm_mesh_wait": [P, P],
+    "jgcm_mesh_branch": [P, I],
+    "jgcm_mesh_join": [P],
     "jgcm_mesh_set_precision": [P, I],
     "jgcm_mesh_get_precision": [P, ctypes.POINTER(I)],
     "jgcm_mesh_set_fft_shape": [P, I, I],

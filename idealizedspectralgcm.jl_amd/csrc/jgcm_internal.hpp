@@ -102,6 +102,10 @@ struct jgcm_mesh {
   hipStream_t stream = nullptr;
   bool stream_borrowed = false;   // stream owned by another mesh (jgcm_mesh_share_stream)
   hipEvent_t signal = nullptr;    // recorded on this mesh's stream by jgcm_mesh_wait
+  // side stream for independent work of one step (jgcm_mesh_branch / jgcm_mesh_join)
+  hipStream_t side = nullptr, main_stream = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  bool on_side = false, forked = false;
   int device = 0;
 
   // host copies of public fields

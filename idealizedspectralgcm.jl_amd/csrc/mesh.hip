@@ -182,6 +182,10 @@ static void free_mesh(jgcm_mesh* m) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->signal) (void)hipEventDestroy(m->signal);
+  if (m->fork_ev) (void)hipEventDestroy(m->fork_ev);
+  if (m->join_ev) (void)hipEventDestroy(m->join_ev);
+  if (m->side) (void)hipStreamDestroy(m->side);
+  if (m->on_side) m->stream = m->main_stream;
   if (m->stream && !m->stream_borrowed) (void)hipStreamDestroy(m->stream);
   delete m;
 }
@@ -198,6 +202,9 @@ static int create_impl(int N, int ns, int nlat, int nlon, int nd, double radius,
   JGCM_CHECK_HIP(hipGetDevice(&m->device));
   JGCM_CHECK_HIP(hipStreamCreate(&m->stream));
   JGCM_CHECK_HIP(hipEventCreateWithFlags(&m->signal, hipEventDisableTiming));
+  JGCM_CHECK_HIP(hipStreamCreate(&m->side));
+  JGCM_CHECK_HIP(hipEventCreateWithFlags(&m->fork_ev, hipEventDisableTiming));
+  JGCM_CHECK_HIP(hipEventCreateWithFlags(&m->join_ev, hipEventDisableTiming));
   hipStream_t st = m->stream;
 
   // ---- Gaussian nodes and weights (FP64, GPU) ----
@@ -414,6 +421,38 @@ int jgcm_mesh_destroy(jgcm_mesh_t* mesh) {
 int jgcm_mesh_synchronize(jgcm_mesh_t* mesh) {
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_CHECK_HIP(hipStreamSynchronize(mesh->stream));
+  if (mesh->side) JGCM_CHECK_HIP(hipStreamSynchronize(mesh->side));
+  return JGCM_OK;
+}
+
+int jgcm_mesh_branch(jgcm_mesh_t* mesh, int side) {
+  if (int rc = check_mesh(mesh)) return rc;
+  if (side) {
+    JGCM_REQUIRE(!mesh->on_side, "the mesh already works on its side stream");
+    JGCM_CHECK_HIP(hipEventRecord(mesh->fork_ev, mesh->stream));
+    JGCM_CHECK_HIP(hipStreamWaitEvent(mesh->side, mesh->fork_ev, 0));
+    mesh->main_stream = mesh->stream;
+    mesh->stream = mesh->side;
+    mesh->on_side = true;
+    mesh->forked = true;
+  } else if (mesh->on_side) {
+    mesh->stream = mesh->main_stream;
+    mesh->on_side = false;
+  }
+  return JGCM_OK;
+}
+
+int jgcm_mesh_join(jgcm_mesh_t* mesh) {
+  if (int rc = check_mesh(mesh)) return rc;
+  if (mesh->on_side) {
+    mesh->stream = mesh->main_stream;
+    mesh->on_side = false;
+  }
+  if (mesh->forked) {
+    JGCM_CHECK_HIP(hipEventRecord(mesh->join_ev, mesh->side));
+    JGCM_CHECK_HIP(hipStreamWaitEvent(mesh->stream, mesh->join_ev, 0));
+    mesh->forked = false;
+  }
   return JGCM_OK;
 }
 

@@ -265,12 +265,16 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
         # Compute_Corrections_Init + Pressure_Variables! without forcing (:250-256)
         _physics(si, d, _ZERO_HS, Δt, False)
     si._corrections_ready = False
+    # the physics pass may still run on the mesh's side stream (Atmosphere_Update
+    # enqueues it there): the gradient batch below does not depend on it
+    call("jgcm_mesh_branch", h, 0)
     # :259 Compute_Gradients!(lnps) and the gradient half of :293 Add_Horizontal_Advection!(T)
     Trans_Spherical_To_Grid_Batch(
         mesh, [d.spe_lnps_c, d.spe_lnps_c, d.spe_t_c, d.spe_t_c],
         [d.grid_dλ_ps, d.grid_dθ_ps, sc.grid_dl_t, sc.grid_dt_t],
         [BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA,
          BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA])
+    call("jgcm_mesh_join", h)                  # the column pass needs both
     call("jgcm_hs_column_dynamics", si.handle, *map(_ptr, (                    # :260-307
         d.grid_div, d.grid_u_c, d.grid_v_c, d.grid_ps_c, d.grid_Δp, d.grid_lnp_half,
         d.grid_lnp_full, d.grid_p_full, d.grid_dλ_ps, d.grid_dθ_ps, d.grid_t_c, d.grid_vor,
@@ -306,9 +310,26 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
                            d.grid_p_full, d.grid_lnp_full)                        # :355
 
 
-def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, physics_params):
-    """Spectral_Dynamics.jl:470-490."""
+# grid size from which the physics pass runs beside the first transforms: on
+# MI355X the overlapped T85 L20 step measured 245 us against 251 us serial, the
+# T42 L20 step 145 us against 139 us (the extra graph dependencies cost more
+# than the smaller kernels gain)
+OVERLAP_MIN_COLUMNS = 256 * 128
+
+
+def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, physics_params,
+                      overlap=None):
+    """Spectral_Dynamics.jl:470-490.  With ``overlap`` the physics pass
+    (forcing on the p-fields, pressure variables, correction partials) goes to
+    the mesh's side stream and runs beside the first transforms of
+    Spectral_Dynamics!, which read and write disjoint memory (parallel branches
+    of a captured step); the results do not change.  ``None`` chooses by grid
+    size (OVERLAP_MIN_COLUMNS)."""
     Δt = Get_Δt(semi_implicit.integrator)
+    if overlap is None:
+        overlap = mesh.nθ * mesh.nλ >= OVERLAP_MIN_COLUMNS
+    if overlap:
+        call("jgcm_mesh_branch", mesh.handle, 1)
     Spectral_Dynamics_Physics(atmo_data, mesh, dyn_data, Δt, physics_params, semi_implicit)
     # the closing Pressure_Variables! of Spectral_Dynamics! (:355) is recomputed
     # identically by Compute_Pressures_And_Heights! below

@@ -527,6 +527,17 @@ int jgcm_graph_destroy(jgcm_graph_t* graph);
  * reverse call joins the branch back before jgcm_graph_end.  No-op when both
  * meshes use one stream.                                                      */
 int jgcm_mesh_wait(jgcm_mesh_t* waiter, jgcm_mesh_t* signaller);
+/* Independent work within one step (no reference counterpart).  After
+ * jgcm_mesh_branch(mesh, 1) the mesh's calls go to its side stream, which
+ * first waits for everything already enqueued; jgcm_mesh_branch(mesh, 0)
+ * sends later calls to the main stream again while the side work runs;
+ * jgcm_mesh_join(mesh) makes the main stream wait for the side work (and
+ * returns to it).  Inside a capture the side work becomes a parallel branch
+ * of the graph, joined before jgcm_graph_end.  The caller guarantees that the
+ * two branches touch disjoint memory (the mesh scratch belongs to the main
+ * branch); jgcm_mesh_synchronize waits for both streams.                     */
+int jgcm_mesh_branch(jgcm_mesh_t* mesh, int side);
+int jgcm_mesh_join(jgcm_mesh_t* mesh);
 
 #ifdef __cplusplus
 }

@@ -112,3 +112,32 @@ def test_update_output_matches_oracle(jgcm):
     assert rel(om_d.u_daily_zonal_mean, om_r.u) < 1e-9
     assert rel(om_d.ps_daily_mean, om_r.ps) < 1e-12
     assert rel(om_d.t_zonal_mean, om_r.t_zonal_mean) < 1e-12
+
+
+def test_overlapped_and_captured_steps_equal_serial_steps(jgcm):
+    """The physics pass on the side stream (eager, and as a parallel branch of a
+    captured step replayed with eager work in between) gives the serial
+    step's results bit for bit."""
+    runs = []
+    for mode in ("serial", "overlap", "graph"):
+        _, (dev, dvc, datmo, dsi, dd) = build(jgcm, 42, 64, 20, 600)
+        jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS, overlap=(mode != "serial"))
+        jgcm.Update_Init_Step(dsi)
+        if mode == "graph":
+            g = jgcm.Step_Graph(dev, lambda: jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS,
+                                                                    overlap=True),
+                                nfields=5 * 20 + 1)
+            for k in range(3):
+                g.launch()
+                dev.synchronize()
+                torch.ones(1000, device="cuda").mul_(k)      # eager work between replays
+                torch.cuda.synchronize()
+            g.close()
+        else:
+            for k in range(3):
+                jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS, overlap=(mode == "overlap"))
+        dev.synchronize()
+        runs.append({f: getattr(dd, f).cpu().numpy() for f in FIELDS + ["grid_δt", "grid_δu"]})
+    for other in runs[1:]:
+        for f, v in runs[0].items():
+            assert np.array_equal(other[f], v), f
