@@ -229,10 +229,14 @@ class _Run_Adjoint:
             self._step(k)
         return self._state()
 
+    def _working_bars(self, bars):
+        """The cotangent buffers the reverse steps update in place."""
+        return tuple(b.clone() for b in bars)
+
     def _reverse(self, bars):
         assert self._checkpoints, "forward() first"
         assert len(bars) == len(self.STATE)
-        bars = tuple(b.clone() for b in bars)
+        bars = self._working_bars(bars)
         for s in reversed(range(len(self._checkpoints))):
             k0 = s * self.every
             k1 = min(k0 + self.every, self.nsteps)
@@ -406,7 +410,12 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
              "spe_lnps_p", "spe_lnps_c", "grid_u_p", "grid_v_p", "grid_t_p", "grid_ps_p")
 
     def __init__(self, mesh, atmo_data, vert_coord, semi_implicit, physics_params,
-                 checkpoint_every=4):
+                 checkpoint_every=4, use_graphs=True):
+        """``use_graphs``: after one eager run of each kind of reverse step
+        (first / later), the step is captured into a HIP graph and replayed;
+        each replay reads the step's snapshot from fixed staging buffers (one
+        batched copy) and updates fixed cotangent buffers.  Results equal the
+        eager steps bit for bit."""
         from .primitive import Get_Topography, Semi_Implicit_Solver
         self._nd = vert_coord.nd
         super().__init__(mesh, atmo_data, semi_implicit.integrator, checkpoint_every)
@@ -424,6 +433,11 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         mesh._semi_implicit = prev
         Get_Topography(self._d.grid_geopots)
         self._matrices = {}
+        self.use_graphs = bool(use_graphs)
+        self._graphs = {}
+        self._graph_runs = {}
+        self._stage = None
+        self._bars_fixed = None
         self._pool = {}
         self._scal_bar = torch.zeros(4, dtype=torch.float64, device=mesh.device)
         self._param_b = torch.zeros((4, mesh.nθ, mesh.nλ), dtype=torch.float64, device=mesh.device)
@@ -549,7 +563,60 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         self._matrices[key] = A
         return A
 
+    def _working_bars(self, bars):
+        if not self.use_graphs:
+            return super()._working_bars(bars)
+        fixed = self._bars_fixed
+        if fixed is None or any(a.shape != b.shape or a.dtype != b.dtype for a, b in zip(fixed, bars)):
+            self._close_graphs()
+            fixed = self._bars_fixed = tuple(torch.empty_like(b) for b in bars)
+        torch.cuda.synchronize()
+        for a, b in zip(fixed, bars):
+            a.copy_(b)
+        torch.cuda.synchronize()
+        return fixed
+
+    def _reverse(self, bars):
+        out = super()._reverse(bars)
+        # with graphs the sweep works in fixed buffers that the next sweep reuses
+        return tuple(t.clone() for t in out) if self.use_graphs else out
+
+    def _close_graphs(self):
+        for g in self._graphs.values():
+            g.close()
+        self._graphs = {}
+        self._graph_runs = {}
+
     def _step_adjoint(self, k, init, snap, bars):
+        if not self.use_graphs:
+            return self._step_adjoint_eager(k, init, snap, bars)
+        from .graph import Step_Graph
+        M = self.mesh
+        st, (sc, nxt) = snap
+        flat = list(st) + [sc] + list(nxt)
+        if self._stage is None or any(a.shape != b.shape for a, b in zip(self._stage, flat)):
+            self._close_graphs()
+            self._stage = [torch.empty_like(t) for t in flat]
+        stage = self._stage
+        ptrs = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+        call("jgcm_copy_batch", M.handle, len(flat), ptrs(stage), ptrs(flat),
+             (ctypes.c_size_t * len(flat))(*[t.numel() * t.element_size() for t in flat]))
+        staged = (tuple(stage[:len(st)]), (stage[len(st)], tuple(stage[len(st) + 1:])))
+        # one graph per kind of step (solver, integrator, forcing parameters)
+        key = (k == 0, bool(init), tuple(sorted((str(a), float(b)) for a, b in self.params.items())))
+        runs = self._graph_runs.get(key, 0)
+        if runs == 0:
+            # eager: fills the temporaries' pool, the implicit matrix and the scratch
+            self._step_adjoint_eager(k, init, staged, bars)
+        else:
+            g = self._graphs.get(key)
+            if g is None:
+                g = self._graphs[key] = Step_Graph(
+                    M, lambda: self._step_adjoint_eager(k, init, staged, bars))
+            g.launch()
+        self._graph_runs[key] = runs + 1
+
+    def _step_adjoint_eager(self, k, init, snap, bars):
         from .mesh import (Divide_By_Cos, Trans_Spherical_To_Grid, Trans_Spherical_To_Grid_Batch,
                            UV_Grid_From_Vor_Div, BATCH_DIVCOS, INV_GRAD_LAMBDA, INV_GRAD_THETA)
         from .primitive import _hs

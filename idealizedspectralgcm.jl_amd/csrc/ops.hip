@@ -38,6 +38,23 @@ __global__ void fill_kernel(double* __restrict__ dst, double v, size_t n) {
 
 // Time_Advance! (Dyn_Data.jl:326-364): blockIdx.y = field; the same thread
 // moves c -> p and then n -> c for an element, so one launch covers all fields.
+constexpr int MAXCOPY = 32;
+struct CopySet {
+  double* dst[MAXCOPY];
+  const double* src[MAXCOPY];
+  size_t count[MAXCOPY];  // doubles
+};
+
+// blockIdx.y = array; a grid-stride loop over its elements
+__global__ void copy_batch_kernel(CopySet s) {
+  const int f = blockIdx.y;
+  double* __restrict__ d = s.dst[f];
+  const double* __restrict__ x = s.src[f];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < s.count[f];
+       i += (size_t)gridDim.x * blockDim.x)
+    d[i] = x[i];
+}
+
 constexpr int MAXADV = 16;
 struct AdvanceSet {
   double* p[MAXADV];
@@ -213,6 +230,29 @@ using namespace jgcm;
   if (int _rc = ensure_scratch(mesh, (nlev))) return _rc;
 
 extern "C" {
+
+int jgcm_copy_batch(jgcm_mesh_t* mesh, int n, double* const* dst, const double* const* src,
+                    const size_t* bytes) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(n >= 1 && n <= MAXCOPY, "n must be in 1..%d, got %d", MAXCOPY, n);
+  JGCM_REQUIRE(dst && src && bytes, "NULL argument");
+  CopySet s{};
+  size_t maxc = 0;
+  for (int f = 0; f < n; ++f) {
+    JGCM_REQUIRE(dst[f] && src[f], "NULL pointer (array %d)", f);
+    JGCM_REQUIRE(bytes[f] % 8 == 0 && (uintptr_t)dst[f] % 8 == 0 && (uintptr_t)src[f] % 8 == 0,
+                 "array %d: sizes and addresses must be multiples of 8 bytes", f);
+    s.dst[f] = dst[f];
+    s.src[f] = src[f];
+    s.count[f] = bytes[f] / 8;
+    if (s.count[f] > maxc) maxc = s.count[f];
+  }
+  if (maxc == 0) return JGCM_OK;
+  hipLaunchKernelGGL(copy_batch_kernel, dim3(nblocks(maxc, 1024), n), dim3(256), 0, mesh->stream,
+                     s);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
 
 int jgcm_copy(jgcm_mesh_t* mesh, void* dst, const void* src, size_t bytes) {
   if (int rc = check_mesh(mesh)) return rc;

@@ -89,6 +89,11 @@ int jgcm_memcpy_d2h(void* dst, const void* src, size_t bytes);
 /* Ordered device-to-device copy on the mesh stream (a kernel, so it can be
  * graph-captured).  Used for Time_Advance! (Dyn_Data.jl:326-364).          */
 int jgcm_copy(jgcm_mesh_t* mesh, void* dst, const void* src, size_t bytes);
+/* n (<= 32) such copies in one launch (no reference counterpart; used to stage
+ * a reverse step's inputs into the buffers its captured graph reads).
+ * Sizes and addresses must be multiples of 8 bytes.                        */
+int jgcm_copy_batch(jgcm_mesh_t* mesh, int n, double* const* dst, const double* const* src,
+                    const size_t* bytes);
 int jgcm_fill(jgcm_mesh_t* mesh, double* dst, double value, size_t count);
 /* Time_Advance! (Dyn_Data.jl:326-364) for nfields (<= 16) prognostic fields
  * in one launch: per element, p = c, then c = n.  count[f] = doubles in
