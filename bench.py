@@ -243,6 +243,42 @@ def galewsky_bench(jgcm, torch, days=6):
     return out
 
 
+def barotropic_bench(jgcm, torch):
+    """Barotropic T42 (128x64) wave-4 Rossby-Haurwitz case with seeded N(0, 1e-8)
+    spectral noise, Δt = 1800 s, Robert 0.04, ∇⁸ damping (0.1-day e-folding at
+    n = 42), 10 days; later steps replayed from one HIP graph.  Reports the
+    device ms/step and sim-days/day and the relative change of the mean
+    kinetic energy over the run (a bounded-run check)."""
+    N, nlat, dt, days = 42, 64, 1800, 10
+    nlon = 2 * nlat
+    a, om = 6371000.0, 7.292e-5
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, nlon, 1, a)
+    atmo = jgcm.Atmo_Data("b", nlon, nlat, 1, False, False, False, False, mesh.sinθ, a, om)
+    integ = jgcm.Filtered_Leapfrog(0.04, 4, 1.0 / (0.1 * 86400), mesh.laplacian_eig, 0.0, dt, True,
+                                   0, days * 86400)
+    d = jgcm.Dyn_Data("b", N, N + 1, nlon, nlat, 1, mesh=mesh)
+    jgcm.Rossby_Haurwitz_Wave4(mesh, d, noise=1.0e-8, seed=42)
+    ke0 = float((d.grid_u_c ** 2 + d.grid_v_c ** 2).mean())
+    jgcm.Barotropic_Dynamics(mesh, atmo, d, integ)
+    jgcm.Update_Init_Step(integ)
+    g = jgcm.Step_Graph(mesh, lambda: jgcm.Barotropic_Dynamics(mesh, atmo, d, integ), nfields=4)
+    steps = days * 86400 // dt - 1
+    stream = torch.cuda.ExternalStream(mesh.stream())
+    mesh.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        g.launch()
+    e1.record(stream)
+    mesh.synchronize()
+    g.close()
+    ms = e0.elapsed_time(e1) / steps
+    ke1 = float((d.grid_u_c ** 2 + d.grid_v_c ** 2).mean())
+    return {"config": "T42 (128x64) barotropic, RH4 + N(0, 1e-8) noise, dt 1800 s, 10 days, FP64",
+            "ms_per_step": ms, "sim_days_per_day": dt / (ms * 1e-3),
+            "kinetic_energy_change": ke1 / ke0 - 1.0}
+
+
 def adjoint_bench(jgcm, torch, precision, n_fd=16):
     """Reverse mode of T42 L20 Held-Suarez over 48 steps of 600 s with
     checkpoints every 8 steps: loss = mass-weighted global mean of (T - T*)²,
@@ -507,6 +543,7 @@ def main():
         hs["split32_speedup_vs_fp64"] = hs["fp64"]["ms_per_step"] / hs["split32"]["ms_per_step"]
         out["held_suarez"] = hs
         if not a.no_extra:
+            out["barotropic_rh4"] = barotropic_bench(jgcm, torch)
             out["ensemble"] = ensemble_bench(jgcm, torch)
             out["shallow_water_galewsky"] = galewsky_bench(jgcm, torch)
             out["adjoint"] = {p: adjoint_bench(jgcm, torch, p) for p in ("fp64", "split32")}
