@@ -279,6 +279,24 @@ def barotropic_bench(jgcm, torch):
             "kinetic_energy_change": ke1 / ke0 - 1.0}
 
 
+def hs_climate_bench(jgcm, torch, days=200):
+    """HSt42 case for its 200 days (T42 L20, Δt 600 s) in FP64 and split-FP32:
+    device ms/step over the run and the climate (zonal-mean u and T averaged
+    over days 100-200): jet maxima (speed, σ, latitude) per hemisphere and the
+    surface temperature at the equator and the poles."""
+    from jgcm_amd import experiments as X
+    out = {"config": f"T42 L20 (128x64), dt 600 s, {days} days, averages over days {days // 2}-{days}"}
+    for prec in ("fp64", "split32"):
+        c = X.held_suarez_climate(days=days, average_from=days // 2, precision=prec)
+        t = c["t_zonal"]
+        nl = t.shape[1]
+        out[prec] = {"ms_per_step": c["ms_per_step"], "sim_days_per_day": 600 / (c["ms_per_step"] * 1e-3),
+                     "finite": c["finite"], "jets_speed_sigma_lat": X.jet_maxima(c),
+                     "surface_t_equator": float(0.5 * (t[-1, nl // 2 - 1] + t[-1, nl // 2])),
+                     "surface_t_poles": float(0.5 * (t[-1, 0] + t[-1, -1]))}
+    return out
+
+
 def adjoint_bench(jgcm, torch, precision, n_fd=16):
     """Reverse mode of T42 L20 Held-Suarez over 48 steps of 600 s with
     checkpoints every 8 steps: loss = mass-weighted global mean of (T - T*)²,
@@ -544,6 +562,7 @@ def main():
         out["held_suarez"] = hs
         if not a.no_extra:
             out["barotropic_rh4"] = barotropic_bench(jgcm, torch)
+            out["held_suarez_200_days"] = hs_climate_bench(jgcm, torch)
             out["ensemble"] = ensemble_bench(jgcm, torch)
             out["shallow_water_galewsky"] = galewsky_bench(jgcm, torch)
             out["adjoint"] = {p: adjoint_bench(jgcm, torch, p) for p in ("fp64", "split32")}

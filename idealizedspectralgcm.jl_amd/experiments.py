@@ -7,6 +7,10 @@ jet (Δt = 600 s; Filtered_Leapfrog with Robert coefficient 0.04, ∇⁸ damping
 1e-4 s^-1 at the truncation and implicit coefficient 0.5, the values of
 exp/Shallow_Water/SW.jl:39-44).  ``run_days`` advances whole days and hands a
 snapshot of the grid state to a callback at the end of each day.
+
+``held_suarez_climate`` runs the Held-Suarez case (exp/HSt42/HS.jl order:
+Spectral_Initialize_Fields!, then Atmosphere_Update! every step) and averages
+its zonal-mean wind and temperature; ``jet_maxima`` locates the jets.
 """
 import torch
 
@@ -122,3 +126,78 @@ def precision_track(days=6, precisions=("split32", "bf16x3", "fp32"), seed=7, **
     out["fp64"]["max_vor_by_day"] = [float(s[0].abs().max()) for s in ref]
     return out
 
+
+
+HS_FORCING = {"σ_b": 0.7, "k_f": 1.0, "k_a": 1.0 / 40.0, "k_s": 1.0 / 4.0, "ΔT_y": 60.0,
+              "Δθ_z": 10.0}
+
+
+def held_suarez_climate(N=42, nθ=64, nd=20, dt=600, days=200, average_from=100,
+                        precision="fp64", seed=5):
+    """Held-Suarez run (exp/HSt42 driver: Spectral_Initialize_Fields! with seeded
+    N(0, 0.1 K) temperature noise, then Atmosphere_Update! every Δt, later steps
+    replayed from one captured graph) and its climate: zonal-mean u and T
+    averaged once a day from `average_from` to `days`.  Returns the averages
+    (numpy [nd][nθ]), the latitudes in degrees, σ at full levels, the device
+    time per step and whether the state stayed finite."""
+    import numpy as np
+    from .graph import Step_Graph
+    from .primitive import (Atmosphere_Update, Semi_Implicit_Solver, Spectral_Initialize_Fields,
+                            Vert_Coordinate)
+    nλ = 2 * nθ
+    mesh = Spectral_Spherical_Mesh(N, N + 1, nθ, nλ, nd, RADIUS, precision=precision)
+    vc = Vert_Coordinate(nλ, nθ, nd, "even_sigma", "simmons_and_burridge", "second_centered_wts",
+                         1e5, mesh=mesh)
+    atmo = Atmo_Data("hs", nλ, nθ, nd, True, True, False, False, mesh.sinθ, RADIUS, OMEGA)
+    integ = Filtered_Leapfrog(0.04, 4, 1.15741e-4, mesh.laplacian_eig, 0.5, dt, True, 0,
+                              days * 86400)
+    si = Semi_Implicit_Solver(vc, atmo, integ, 1e5, np.full(nd, 300.0), mesh.wave_numbers)
+    d = Dyn_Data("hs", N, N + 1, nλ, nθ, nd, mesh=mesh)
+    d.grid_geopots.zero_()
+    noise = torch.from_numpy(0.1 * np.random.default_rng(seed).standard_normal((nd, nθ, nλ)))
+    Spectral_Initialize_Fields(mesh, atmo, vc, 1.0e5, 300.0, d.grid_geopots, d,
+                               grid_t_noise=noise.to(mesh.device))
+    Atmosphere_Update(mesh, atmo, vc, si, d, HS_FORCING)
+    Update_Init_Step(si)
+    g = Step_Graph(mesh, lambda: Atmosphere_Update(mesh, atmo, vc, si, d, HS_FORCING),
+                   nfields=5 * nd + 1)
+    per_day = 86400 // dt
+    um = torch.zeros(nd, nθ, dtype=torch.float64, device=mesh.device)
+    tm = torch.zeros_like(um)
+    stream = torch.cuda.ExternalStream(mesh.stream())
+    ms, timed, count = 0.0, 0, 0
+    for day in range(days):
+        n = per_day - 1 if day == 0 else per_day
+        mesh.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            g.launch()
+        e1.record(stream)
+        mesh.synchronize()
+        ms += e0.elapsed_time(e1)
+        timed += n
+        if day >= average_from:
+            um += d.grid_u_c.mean(dim=2)
+            tm += d.grid_t_c.mean(dim=2)
+            count += 1
+            torch.cuda.synchronize()
+    g.close()
+    finite = bool(torch.isfinite(d.grid_t_c).all()) and bool(torch.isfinite(d.grid_u_c).all())
+    return {"u_zonal": (um / max(count, 1)).cpu().numpy(),
+            "t_zonal": (tm / max(count, 1)).cpu().numpy(),
+            "lat_deg": np.degrees(np.asarray(mesh.θc)),
+            "sigma": (np.arange(nd) + 0.5) / nd,
+            "ms_per_step": ms / timed, "finite": finite}
+
+
+def jet_maxima(climate):
+    """(speed, σ, latitude) of the largest zonal-mean u in each hemisphere."""
+    import numpy as np
+    u, lat, sig = climate["u_zonal"], climate["lat_deg"], climate["sigma"]
+    out = {}
+    for name, sel in (("south", lat < 0), ("north", lat > 0)):
+        sub = u[:, sel]
+        k, j = np.unravel_index(np.argmax(sub), sub.shape)
+        out[name] = (float(sub[k, j]), float(sig[k]), float(lat[sel][j]))
+    return out

@@ -141,3 +141,25 @@ def test_overlapped_and_captured_steps_equal_serial_steps(jgcm):
     for other in runs[1:]:
         for f, v in runs[0].items():
             assert np.array_equal(other[f], v), f
+
+
+def test_hs_t42_l20_two_hundred_day_climate(jgcm):
+    """The HSt42 case run for its 200 days (FP64, Δt = 600 s) reproduces the
+    Held & Suarez (1994) climate: a westerly jet in each hemisphere of 25-40 m/s
+    (measured 33 m/s) near σ = 0.2-0.3 and 35-55° latitude (σ 0.225, 46°),
+    hemispherically symmetric within 5 m/s, and surface temperatures of
+    300-312 K at the equator and 255-272 K at the poles (307 K, 265 K; the
+    forcing's equilibrium is 315 K / 255 K, the eddies carry heat poleward)."""
+    from jgcm_amd import experiments as X
+    c = X.held_suarez_climate(days=200, average_from=100)
+    assert c["finite"]
+    jets = X.jet_maxima(c)
+    print(jets)
+    for speed, sig, lat in jets.values():
+        assert 25.0 < speed < 40.0 and 0.15 < sig < 0.35 and 35.0 < abs(lat) < 55.0, jets
+    assert abs(jets["north"][0] - jets["south"][0]) < 5.0, jets
+    t = c["t_zonal"]
+    nlat = t.shape[1]
+    eq = 0.5 * (t[-1, nlat // 2 - 1] + t[-1, nlat // 2])
+    pole = 0.5 * (t[-1, 0] + t[-1, -1])
+    assert 300.0 < eq < 312.0 and 255.0 < pole < 272.0, (eq, pole)
