@@ -280,20 +280,25 @@ def barotropic_bench(jgcm, torch):
 
 
 def hs_climate_bench(jgcm, torch, days=200):
-    """HSt42 case for its 200 days (T42 L20, Δt 600 s) in FP64 and split-FP32:
-    device ms/step over the run and the climate (zonal-mean u and T averaged
-    over days 100-200): jet maxima (speed, σ, latitude) per hemisphere and the
-    surface temperature at the equator and the poles."""
+    """Held-Suarez L20 runs for 200 days — T42 (128x64, Δt 600 s) and T85
+    (256x128, Δt 300 s) — in FP64 and split-FP32: device ms/step over the run
+    and the climate (zonal-mean u and T averaged over days 100-200): jet maxima
+    (speed, σ, latitude) per hemisphere and the surface temperature at the
+    equator and the poles."""
     from jgcm_amd import experiments as X
-    out = {"config": f"T42 L20 (128x64), dt 600 s, {days} days, averages over days {days // 2}-{days}"}
-    for prec in ("fp64", "split32"):
-        c = X.held_suarez_climate(days=days, average_from=days // 2, precision=prec)
-        t = c["t_zonal"]
-        nl = t.shape[1]
-        out[prec] = {"ms_per_step": c["ms_per_step"], "sim_days_per_day": 600 / (c["ms_per_step"] * 1e-3),
-                     "finite": c["finite"], "jets_speed_sigma_lat": X.jet_maxima(c),
-                     "surface_t_equator": float(0.5 * (t[-1, nl // 2 - 1] + t[-1, nl // 2])),
-                     "surface_t_poles": float(0.5 * (t[-1, 0] + t[-1, -1]))}
+    out = {"config": f"L20, {days} days, averages over days {days // 2}-{days}"}
+    for name, N, nlat, dt in (("t42", 42, 64, 600), ("t85", 85, 128, 300)):
+        for prec in ("fp64", "split32"):
+            c = X.held_suarez_climate(N=N, nθ=nlat, dt=dt, days=days, average_from=days // 2,
+                                      precision=prec)
+            t = c["t_zonal"]
+            nl = t.shape[1]
+            out[f"{name}_{prec}"] = {
+                "dt_s": dt, "ms_per_step": c["ms_per_step"],
+                "sim_days_per_day": dt / (c["ms_per_step"] * 1e-3), "finite": c["finite"],
+                "jets_speed_sigma_lat": X.jet_maxima(c),
+                "surface_t_equator": float(0.5 * (t[-1, nl // 2 - 1] + t[-1, nl // 2])),
+                "surface_t_poles": float(0.5 * (t[-1, 0] + t[-1, -1]))}
     return out
 
 
