@@ -568,6 +568,8 @@ def main():
         if not a.no_extra:
             out["barotropic_rh4"] = barotropic_bench(jgcm, torch)
             out["held_suarez_200_days"] = hs_climate_bench(jgcm, torch)
+            from jgcm_amd import experiments as X
+            out["precision_report_t85_l20"] = X.precision_report()
             out["ensemble"] = ensemble_bench(jgcm, torch)
             out["shallow_water_galewsky"] = galewsky_bench(jgcm, torch)
             out["adjoint"] = {p: adjoint_bench(jgcm, torch, p) for p in ("fp64", "split32")}

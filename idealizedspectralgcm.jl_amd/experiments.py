@@ -201,3 +201,103 @@ def jet_maxima(climate):
         k, j = np.unravel_index(np.argmax(sub), sub.shape)
         out[name] = (float(sub[k, j]), float(sig[k]), float(lat[sel][j]))
     return out
+
+
+def _hs_model(N, nθ, nd, dt, precision, days):
+    import numpy as np
+    from .primitive import Semi_Implicit_Solver, Vert_Coordinate
+    nλ = 2 * nθ
+    mesh = Spectral_Spherical_Mesh(N, N + 1, nθ, nλ, nd, RADIUS, precision=precision)
+    vc = Vert_Coordinate(nλ, nθ, nd, "even_sigma", "simmons_and_burridge", "second_centered_wts",
+                         1e5, mesh=mesh)
+    atmo = Atmo_Data("hs", nλ, nθ, nd, True, True, False, False, mesh.sinθ, RADIUS, OMEGA)
+    integ = Filtered_Leapfrog(0.04, 4, 1.15741e-4, mesh.laplacian_eig, 0.5, dt, True, 0,
+                              days * 86400)
+    si = Semi_Implicit_Solver(vc, atmo, integ, 1e5, np.full(nd, 300.0), mesh.wave_numbers)
+    d = Dyn_Data("hs", N, N + 1, nλ, nθ, nd, mesh=mesh)
+    return mesh, vc, atmo, si, d
+
+
+def _energy_enstrophy(mesh, d):
+    """Mass-weighted global mean kinetic energy (u² + v²)/2 and area-weighted
+    global mean enstrophy ζ²/2 over all levels (even σ: equal Δb)."""
+    import numpy as np
+    w = torch.tensor(np.asarray(mesh.wts), dtype=torch.float64, device=mesh.device)[None, :, None]
+    ps = d.grid_ps_c
+    ke = (0.5 * (d.grid_u_c ** 2 + d.grid_v_c ** 2) * ps * w).sum() / (ps * w).sum() / d.grid_u_c.shape[0]
+    en = (0.5 * d.grid_vor ** 2 * w).sum() / (w.sum() * d.grid_vor.shape[0] * d.grid_vor.shape[2])
+    return float(ke), float(en)
+
+
+def precision_report(N=85, nθ=128, nd=20, dt=300, spinup_days=20, days=10,
+                     precisions=("split32", "bf16x3", "fp32"), seed=5):
+    """Errors of the reduced-precision Legendre stages on the Held-Suarez case,
+    from one FP64 state spun up for `spinup_days`: per field, the relative L2
+    error of one step's spectral tendencies (δζ, δD, δT, δ ln ps after the
+    semi-implicit correction) against the FP64 step; after `days` more days, the
+    relative difference of the global kinetic energy and enstrophy from the
+    FP64 run and each run's relative change over those days.  The reduced
+    precisions accumulate the Legendre sums in FP32, so every spectral
+    coefficient carries an error of ~1e-7 of its field's norm; the small
+    high-degree coefficients carry large relative errors, which ∇² (weight
+    n(n+1)) and the semi-implicit solve bring out in δD (reported also against
+    the norm of ∇²(Φ + KE), its largest explicit term)."""
+    import numpy as np
+    from .graph import Step_Graph
+    from .primitive import Atmosphere_Update, Spectral_Initialize_Fields
+    total = spinup_days + days + 1
+    mesh, vc, atmo, si, d = _hs_model(N, nθ, nd, dt, "fp64", total)
+    d.grid_geopots.zero_()
+    noise = torch.from_numpy(0.1 * np.random.default_rng(seed).standard_normal((nd, nθ, 2 * nθ)))
+    Spectral_Initialize_Fields(mesh, atmo, vc, 1.0e5, 300.0, d.grid_geopots, d,
+                               grid_t_noise=noise.to(mesh.device))
+    Atmosphere_Update(mesh, atmo, vc, si, d, HS_FORCING)
+    Update_Init_Step(si)
+    g = Step_Graph(mesh, lambda: Atmosphere_Update(mesh, atmo, vc, si, d, HS_FORCING),
+                   nfields=5 * nd + 1)
+    for _ in range(spinup_days * 86400 // dt - 1):
+        g.launch()
+    mesh.synchronize()
+    g.close()
+    names = [k for k, v in vars(d).items() if isinstance(v, torch.Tensor)]
+    state = {k: getattr(d, k).clone() for k in names}
+    torch.cuda.synchronize()
+    tend = ("spe_δvor", "spe_δdiv", "spe_δt", "spe_δlnps", "spe_energy")
+    runs = {}
+    for prec in ("fp64",) + tuple(precisions):
+        m2, vc2, atmo2, si2, d2 = _hs_model(N, nθ, nd, dt, prec, total)
+        for k in names:
+            getattr(d2, k).copy_(state[k])
+        Update_Init_Step(si2)                    # later steps: 2Δt leapfrog and wave matrices
+        torch.cuda.synchronize()
+        e0 = _energy_enstrophy(m2, d2)
+        Atmosphere_Update(m2, atmo2, vc2, si2, d2, HS_FORCING)
+        m2.synchronize()
+        t1 = {k: getattr(d2, k).clone() for k in tend}
+        g2 = Step_Graph(m2, lambda: Atmosphere_Update(m2, atmo2, vc2, si2, d2, HS_FORCING),
+                        nfields=5 * nd + 1)
+        for _ in range(days * 86400 // dt - 1):
+            g2.launch()
+        m2.synchronize()
+        g2.close()
+        torch.cuda.synchronize()
+        runs[prec] = (t1, e0, _energy_enstrophy(m2, d2))
+    ref_t, ref_e0, ref_e1 = runs["fp64"]
+    out = {"config": f"Held-Suarez T{N} L{nd}, dt {dt} s, from day {spinup_days} (FP64 spin-up)",
+           "fp64": {"kinetic_energy": ref_e1[0], "enstrophy": ref_e1[1],
+                    "kinetic_energy_change": ref_e1[0] / ref_e0[0] - 1.0,
+                    "enstrophy_change": ref_e1[1] / ref_e0[1] - 1.0}}
+    for prec in precisions:
+        t1, e0, e1 = runs[prec]
+        errs = {k.replace("spe_", ""): float(torch.linalg.norm(t1[k] - ref_t[k]) /
+                                             torch.linalg.norm(ref_t[k])) for k in tend[:4]}
+        # δD's error against its largest explicit term, ∇²(Φ + KE) (left in spe_energy)
+        dd_vs_lap = float(torch.linalg.norm(t1["spe_δdiv"] - ref_t["spe_δdiv"]) /
+                          torch.linalg.norm(ref_t["spe_energy"]))
+        out[prec] = {"tendency_rel_err": errs, "δdiv_err_vs_laplacian_of_energy": dd_vs_lap,
+                     "kinetic_energy_rel_diff": e1[0] / ref_e1[0] - 1.0,
+                     "enstrophy_rel_diff": e1[1] / ref_e1[1] - 1.0,
+                     "kinetic_energy_change": e1[0] / e0[0] - 1.0,
+                     "enstrophy_change": e1[1] / e0[1] - 1.0}
+    out["days_after_spinup"] = days
+    return out

@@ -163,3 +163,20 @@ def test_hs_t42_l20_two_hundred_day_climate(jgcm):
     eq = 0.5 * (t[-1, nlat // 2 - 1] + t[-1, nlat // 2])
     pole = 0.5 * (t[-1, 0] + t[-1, -1])
     assert 300.0 < eq < 312.0 and 255.0 < pole < 272.0, (eq, pole)
+
+
+def test_reduced_precision_tendencies_and_energy_track_fp64(jgcm):
+    """From one FP64 Held-Suarez T42 L20 state (30 days of spin-up), one step's
+    tendencies and the state after 10 more days in split-FP32, BF16x3 and FP32
+    Legendre against FP64 (measured: δζ <= 5e-7 and δT <= 3e-4 relative; δD's
+    error <= 2e-4 of the norm of ∇²(Φ + KE); global kinetic energy and
+    enstrophy after 10 days within 1e-6)."""
+    from jgcm_amd import experiments as X
+    r = X.precision_report(N=42, nθ=64, dt=600, spinup_days=30, days=10)
+    for prec in ("split32", "bf16x3", "fp32"):
+        e = r[prec]
+        assert e["tendency_rel_err"]["δvor"] < 2e-6, (prec, e)
+        assert e["tendency_rel_err"]["δt"] < 2e-3, (prec, e)
+        assert e["δdiv_err_vs_laplacian_of_energy"] < 1e-3, (prec, e)
+        assert abs(e["kinetic_energy_rel_diff"]) < 1e-5, (prec, e)
+        assert abs(e["enstrophy_rel_diff"]) < 1e-5, (prec, e)
