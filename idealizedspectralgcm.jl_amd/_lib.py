@@ -125,6 +125,7 @@ SIGNATURES = {
     "jgcm_hs_physics": [P, P, ctypes.c_int64] + [P] * 15 + [I],
     "jgcm_hs_column_dynamics": [P] * 27,
     "jgcm_hs_corrections": [P] * 8,
+    "jgcm_hs_corrections_advance": [P, P, P, P, P, P],
     "jgcm_hs_scalars": [P, P, I],
     "jgcm_hs_scalars_device": [P, P, I],
     "jgcm_zonal_mean_accumulate": [P, P, P, P, P, P, I, P, P, P, P, P],

@@ -620,6 +620,114 @@ __global__ __launch_bounds__(APPLY_THREADS) void hs_apply_corrections_kernel(
   }
 }
 
+// The correction means (every workgroup that needs them, same fixed order as
+// hs_apply_corrections_kernel): out = {mean ps_p, energy_p, mean ps_n, f, e_n, corr}.
+__device__ void correction_scalars(const double* part_p, const double* part_n, int nparts,
+                                   int nlon, double grav, double cp_air, int mass, int energy,
+                                   double (*red)[APPLY_THREADS], double* out) {
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < nparts; i += APPLY_THREADS) {
+    acc[0] = acc[0] + part_p[2 * (size_t)i];
+    acc[1] = acc[1] + part_p[2 * (size_t)i + 1];
+    acc[2] = acc[2] + part_n[3 * (size_t)i];
+    acc[3] = acc[3] + part_n[3 * (size_t)i + 1];
+    acc[4] = acc[4] + part_n[3 * (size_t)i + 2];
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int st = APPLY_THREADS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  const double mps_p = red[0][0] / (2.0 * nlon);
+  const double e_p = red[1][0] / (2.0 * nlon) / grav;
+  const double mps_n = red[2][0] / (2.0 * nlon);
+  const double f = mass ? mps_p / mps_n : 1.0;
+  const double e_n = (red[3][0] + f * red[4][0]) / (2.0 * nlon) / grav;
+  out[0] = mps_p;
+  out[1] = e_p;
+  out[2] = mps_n;
+  out[3] = f;
+  out[4] = e_n;
+  out[5] = energy ? grav * (e_p - e_n) / (cp_air * mps_p) : 0.0;
+}
+
+// Compute_Corrections! (SD:30-68) fused into Time_Advance! (Dyn_Data.jl:326-364):
+// per field p = c, then c = n where n first receives its correction — kind 1:
+// T_n += corr (grid), 2: ps_n *= f (grid), 3: the (0, 0) coefficient of every
+// level += corr (spectral T, level stride `spec_level` doubles), 4: the (0, 0)
+// coefficient += log f (spectral ln ps).  Only workgroups whose elements need
+// the scalars reduce the partials (all of kinds 1 and 2, the ones holding a
+// (0, 0) coefficient for kinds 3 and 4); workgroup 0 of the first kind-1 field
+// stores the seven scalars as hs_apply_corrections_kernel does.  blockIdx.y = field.
+constexpr int MAXCADV = 16;
+struct CorrAdvanceSet {
+  double* p[MAXCADV];
+  double* c[MAXCADV];
+  double* n[MAXCADV];
+  size_t count[MAXCADV];  // doubles
+  int kind[MAXCADV];
+};
+
+__global__ __launch_bounds__(APPLY_THREADS) void hs_correct_advance_kernel(
+    CorrAdvanceSet s, int per_block, size_t spec_level, int nlon, const double* part_p,
+    const double* part_n, int nparts, double* scal, double grav, double cp_air, int mass,
+    int energy, int scal_field) {
+  __shared__ double red[5][APPLY_THREADS];
+  const int fld = blockIdx.y;
+  const size_t cnt = s.count[fld];
+  const size_t lo = (size_t)blockIdx.x * per_block;
+  if (lo >= cnt) return;  // uniform over the workgroup
+  const size_t hi = min(lo + (size_t)per_block, cnt);
+  const int kind = (mass || energy) ? s.kind[fld] : 0;
+  bool need = kind == 1 || kind == 2;
+  if (kind == 3) need = (lo + spec_level - 1) / spec_level * spec_level < hi;  // holds a multiple
+  if (kind == 4) need = lo == 0;
+  double sc[6] = {0.0, 0.0, 0.0, 1.0, 0.0, 0.0};
+  if (need)
+    correction_scalars(part_p, part_n, nparts, nlon, grav, cp_air, mass, energy, red, sc);
+  const double f = sc[3], corr = sc[5];
+  const double logf = (kind == 4 && mass) ? log(f) : 0.0;
+  double* __restrict__ p = s.p[fld];
+  double* __restrict__ c = s.c[fld];
+  double* __restrict__ n = s.n[fld];
+  for (size_t i = lo + threadIdx.x; i < hi; i += APPLY_THREADS) {
+    const double x = c[i];
+    p[i] = x;
+    double v = n[i];
+    if (kind == 1 && energy) {
+      v = v + corr;
+      n[i] = v;
+    } else if (kind == 2 && mass) {
+      v = v * f;
+      n[i] = v;
+    } else if (kind == 3 && energy && i % spec_level == 0) {
+      v = v + corr;
+      n[i] = v;
+    } else if (kind == 4 && mass && i == 0) {
+      v = v + logf;
+      n[i] = v;
+    }
+    c[i] = v;
+  }
+  if ((mass || energy) && fld == scal_field && blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[0] = sc[0];
+    if (energy) scal[1] = sc[1];
+    if (mass) {
+      scal[2] = sc[2];
+      scal[3] = f;
+      scal[4] = log(f);
+    }
+    if (energy) {
+      scal[5] = sc[4];
+      scal[6] = corr;
+    }
+  }
+}
+
 // Pressure_Variables! (SD:355) and Compute_Pressures_And_Heights! (SD:485,
 // Press_And_Geopot.jl:3-19: geopotential / g) when z_full is given.
 template <int KPT>
@@ -1398,6 +1506,50 @@ int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, doubl
                        si->atmo.cp_air, mass, energy);
     JGCM_LAUNCH_CHECK();
   }
+  return JGCM_OK;
+}
+
+int jgcm_hs_corrections_advance(jgcm_semi_implicit_t* si, const double* grid_lnps,
+                                double* const* p, double* const* c, double* const* n,
+                                const size_t* count) {
+  JGCM_REQUIRE(si && grid_lnps && p && c && n && count, "NULL argument");
+  jgcm_mesh* M = si->vc->mesh;
+  for (int f = 0; f < 8; ++f)
+    JGCM_REQUIRE(p[f] && c[f] && n[f], "NULL field pointer (field %d)", f);
+  const size_t P = M->grid_elems(), S = M->spec_elems();
+  const int nd = si->nd;
+  JGCM_REQUIRE(count[0] == nd * S * 2 && count[1] == nd * S * 2 && count[2] == S * 2 &&
+                   count[3] == nd * S * 2 && count[4] == nd * P && count[5] == nd * P &&
+                   count[6] == P && count[7] == nd * P,
+               "field sizes do not match the Time_Advance! order (vor, div, lnps, T spectral; "
+               "u, v, ps, T grid)");
+  int mass = si->atmo.do_mass_correction, energy = si->atmo.do_energy_correction;
+  VC v = vc_of(si->vc);
+  const int nparts = (int)tile_grid(P).x;
+  double* part_p = si->d_colsum;
+  double* part_n = si->d_colsum + 2 * (size_t)nparts;
+  // ps_n = exp(ln ps) and the sums of the new state (n[4] u, n[5] v, n[7] T, n[6] ps)
+  JGCM_TILE_DISPATCH(nd, hs_corrections_sums_kernel, tile_grid(P), tile_block(nd), 0, M->stream, v,
+                     M->nlat, M->nlon, si->atmo.cp_air, M->d_wts, grid_lnps, n[6], n[4], n[5], n[7],
+                     part_n, mass || energy, energy);
+  JGCM_LAUNCH_CHECK();
+  CorrAdvanceSet s{};
+  static const int kinds[8] = {0, 0, 4, 3, 0, 0, 2, 1};
+  size_t maxc = 0;
+  for (int f = 0; f < 8; ++f) {
+    s.p[f] = p[f];
+    s.c[f] = c[f];
+    s.n[f] = n[f];
+    s.count[f] = count[f];
+    s.kind[f] = kinds[f];
+    maxc = std::max(maxc, count[f]);
+  }
+  const int per_block = 4 * APPLY_THREADS;
+  const dim3 grid((unsigned)((maxc + per_block - 1) / per_block), 8);
+  hipLaunchKernelGGL(hs_correct_advance_kernel, grid, dim3(APPLY_THREADS), 0, M->stream, s,
+                     per_block, 2 * S, M->nlon, part_p, part_n, nparts, si->d_scal, si->atmo.grav,
+                     si->atmo.cp_air, mass, energy, 7);
+  JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
 

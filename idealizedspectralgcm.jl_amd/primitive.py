@@ -16,7 +16,6 @@ import torch
 
 from . import _lib
 from ._lib import call
-from .dyn_data import Time_Advance
 from .mesh import (_ptr, Trans_Grid_To_Spherical, Trans_Spherical_To_Grid,
                    Trans_Grid_To_Spherical_Batch, Trans_Spherical_To_Grid_Batch, BATCH_DIVCOS,
                    BATCH_VORDIV, INV_GRAD_LAMBDA, INV_GRAD_THETA, INV_UCOS, INV_VCOS,
@@ -302,9 +301,14 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
         [d.grid_vor, d.grid_div, d.grid_u_n, d.grid_v_n, d.grid_t_n, d.grid_lnps],
         [0, 0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS, 0, 0],
         snms_b=[None, None, d.spe_div_n, d.spe_div_n, None, None])
-    call("jgcm_hs_corrections", si.handle, *map(_ptr, (                          # :336-343
-        d.grid_lnps, d.grid_ps_n, d.grid_u_n, d.grid_v_n, d.grid_t_n, d.spe_lnps_n, d.spe_t_n)))
-    Time_Advance(d)                                                              # :346
+    # :336-343 Compute_Corrections! and :346 Time_Advance!, the corrections applied
+    # to the new fields while they are copied (one launch after the sums)
+    names = [f"spe_{f}" for f in ("vor", "div", "lnps", "t")] + \
+        [f"grid_{f}" for f in ("u", "v", "ps", "t")]
+    arrs = [[getattr(d, f + s) for f in names] for s in ("_p", "_c", "_n")]
+    ptrs = [(ctypes.c_void_p * 8)(*[t.data_ptr() for t in ts]) for ts in arrs]
+    counts = (ctypes.c_size_t * 8)(*[t.numel() * t.element_size() // 8 for t in arrs[1]])
+    call("jgcm_hs_corrections_advance", si.handle, _ptr(d.grid_lnps), *ptrs, counts)
     if pressure_variables:
         Pressure_Variables(vert_coord, d.grid_ps_c, d.grid_p_half, d.grid_Δp, d.grid_lnp_half,
                            d.grid_p_full, d.grid_lnp_full)                        # :355

@@ -499,6 +499,16 @@ int jgcm_hs_column_dynamics(jgcm_semi_implicit_t* si, const double* grid_div, co
 int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, double* grid_ps_n,
                         const double* grid_u_n, const double* grid_v_n, double* grid_t_n,
                         double* spe_lnps_n, double* spe_t_n);
+/* Compute_Corrections! followed by Time_Advance! (Spectral_Dynamics.jl:336-346,
+ * Dyn_Data.jl:326-364; no single reference function): the sums of the new
+ * state, then one launch that applies the corrections to the new fields while
+ * it copies p = c, c = n.  Fields in Time_Advance! order — spectral vor, div,
+ * ln ps, T, then grid u, v, ps, T — as p/c/n pointer arrays with their sizes
+ * in doubles; grid_lnps as for jgcm_hs_corrections.  Equal to
+ * jgcm_hs_corrections followed by jgcm_time_advance, bit for bit.          */
+int jgcm_hs_corrections_advance(jgcm_semi_implicit_t* si, const double* grid_lnps,
+                                double* const* p, double* const* c, double* const* n,
+                                const size_t* count);
 /* device correction scalars to the host: mean_ps_p, mean_energy_p, mean_ps_n,
  * factor, log(factor), mean_energy_n, temperature correction (synchronises) */
 int jgcm_hs_scalars(jgcm_semi_implicit_t* si, double* host_out, int count);

@@ -180,3 +180,40 @@ def test_reduced_precision_tendencies_and_energy_track_fp64(jgcm):
         assert e["δdiv_err_vs_laplacian_of_energy"] < 1e-3, (prec, e)
         assert abs(e["kinetic_energy_rel_diff"]) < 1e-5, (prec, e)
         assert abs(e["enstrophy_rel_diff"]) < 1e-5, (prec, e)
+
+
+def test_fused_corrections_advance_equals_corrections_then_time_advance(jgcm):
+    """jgcm_hs_corrections_advance equals jgcm_hs_corrections followed by
+    Time_Advance! bit for bit (fields and the seven correction scalars), on the
+    state after one step (the physics partials of that step in place)."""
+    import ctypes
+    from jgcm_amd._lib import call
+    from jgcm_amd.mesh import _ptr
+    _, (dev, dvc, datmo, dsi, dd) = build(jgcm, 42, 64, 20, 600)
+    jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS)
+    dev.synchronize()
+    names = [f"spe_{f}" for f in ("vor", "div", "lnps", "t")] + \
+        [f"grid_{f}" for f in ("u", "v", "ps", "t")]
+    fields = [f + s for f in names for s in ("_p", "_c", "_n")] + ["grid_lnps"]
+    start = {f: getattr(dd, f).clone() for f in fields}
+    results = []
+    for fused in (False, True):
+        for f in fields:
+            getattr(dd, f).copy_(start[f])
+        torch.cuda.synchronize()
+        if fused:
+            arrs = [[getattr(dd, f + s) for f in names] for s in ("_p", "_c", "_n")]
+            ptrs = [(ctypes.c_void_p * 8)(*[t.data_ptr() for t in ts]) for ts in arrs]
+            counts = (ctypes.c_size_t * 8)(*[t.numel() * t.element_size() // 8 for t in arrs[1]])
+            call("jgcm_hs_corrections_advance", dsi.handle, _ptr(dd.grid_lnps), *ptrs, counts)
+        else:
+            call("jgcm_hs_corrections", dsi.handle, *map(_ptr, (
+                dd.grid_lnps, dd.grid_ps_n, dd.grid_u_n, dd.grid_v_n, dd.grid_t_n, dd.spe_lnps_n,
+                dd.spe_t_n)))
+            jgcm.Time_Advance(dd)
+        dev.synchronize()
+        results.append(({f: getattr(dd, f).cpu().numpy() for f in fields}, dsi.scalars()))
+    (a, sa), (b, sb) = results
+    for f in fields:
+        assert np.array_equal(a[f], b[f]), f
+    assert np.array_equal(np.asarray(sa), np.asarray(sb))
