@@ -122,6 +122,7 @@ SIGNATURES = {
     "jgcm_semi_implicit_get": [P, I, P, SZ],
     "jgcm_implicit_correction": [P] * 10,
     "jgcm_hs_energy_implicit_correction": [P] * 11,
+    "jgcm_hs_energy_implicit_leapfrog": [P] * 18,
     "jgcm_hs_physics": [P, P, ctypes.c_int64] + [P] * 15 + [I],
     "jgcm_hs_column_dynamics": [P] * 27,
     "jgcm_hs_corrections": [P] * 8,

@@ -858,13 +858,60 @@ __device__ __forceinline__ void linear_geopot_tile(int nd, double rdgas, const S
   }
 }
 
+// Compute_Spectral_Damping! (Time_Integrator.jl:76-80, when `damp`) and
+// Filtered_Leapfrog! (:93-100) of one element held in registers: q = δQ in,
+// damped δQ out; c_out, n_out = the filtered current and the new value.  The
+// arithmetic and its order are those of stepper.hip's damped_leapfrog_elem.
+__device__ __forceinline__ void leapfrog_reg(double2& q, double2 p, double2 c, double d, bool damp,
+                                             int init_step, double dt, double r, double2& c_out,
+                                             double2& n_out) {
+  if (damp) {
+    if (init_step) {
+      double den = 1.0 + dt * d;
+      q = make_double2((q.x - c.x * d) / den, (q.y - c.y * d) / den);
+    } else {
+      double den = 1.0 + (2.0 * dt) * d;
+      q = make_double2((q.x - p.x * d) / den, (q.y - p.y * d) / den);
+    }
+  }
+  if (init_step) {
+    double2 nx = make_double2(c.x + dt * q.x, c.y + dt * q.y);
+    n_out = nx;
+    c_out = make_double2(c.x + r * (-1.0 * c.x + nx.x), c.y + r * (-1.0 * c.y + nx.y));
+  } else {
+    double2 c1 = make_double2(c.x + r * (p.x - 2.0 * c.x), c.y + r * (p.y - 2.0 * c.y));
+    double2 nx = make_double2(p.x + (2.0 * dt) * q.x, p.y + (2.0 * dt) * q.y);
+    n_out = nx;
+    c_out = make_double2(c1.x + r * nx.x, c1.y + r * nx.y);
+  }
+}
+
+// The step's damping + leapfrog fused into the semi-implicit solve (vorticity,
+// divergence, T damped; ln ps not): the c arrays are written (Robert filter),
+// the n arrays receive the new values.  on = 0: not fused.
+struct LFFuse {
+  double2* dvor;
+  const double2* vor_p;
+  double2* vor_c;
+  double2* vor_n;
+  double2* div_c;
+  double2* div_n;
+  double2* t_c;
+  double2* t_n;
+  double2* lnps_c;
+  double2* lnps_n;
+  const double* damping;  // [m][n]
+  int on, init_step;
+  double dt, r;
+};
+
 template <int KPT>
 __global__ __launch_bounds__(512) void hs_implicit_kernel(
     int nd, int nm, int ns, size_t spec_per_level, double kappa, double rdgas, double ps_ref,
     double xi, SIRef rg, const double* __restrict__ eig, const double* __restrict__ wave,
     int num_wavenumbers, const double2* div_c, const double2* div_p, const double2* lnps_c,
     const double2* lnps_p, const double2* t_c, const double2* t_p, double2* ddiv, double2* dlnps,
-    double2* dtt, double2* energy, int stage_w) {
+    double2* dtt, double2* energy, int stage_w, LFFuse lf) {
   // LDS: nd * 64 double2 (level sums, then the wave-matrix operand); the
   // reference profiles (6 x (nd+1) doubles); when stage_w, the wave matrices
   // W_n of the tile's degrees n_lo .. n_hi.  Everything a thread reads from
@@ -919,6 +966,19 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     DT[i] = d2keep(ok, dtt[o]);
     DV[i] = d2keep(ok, ddiv[o]);
     EN[i] = energy ? d2keep(ok, energy[o]) : d2(0.0, 0.0);
+  }
+  double2 QV[KPT], VP[KPT], VCc[KPT];
+  double dmp = 0.0;
+  if (lf.on) {
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      const bool ok = tx.col && k0 + i < nd;
+      const size_t o = ok ? (size_t)(k0 + i) * spec_per_level + idx : 0;
+      QV[i] = d2keep(ok, lf.dvor[o]);
+      VP[i] = d2keep(ok, lf.vor_p[o]);
+      VCc[i] = d2keep(ok, lf.vor_c[o]);
+    }
+    dmp = lf.damping[m * (ns + 1) + n];
   }
   const size_t ic = tx.col ? idx : 0;
   const double2 dl0 = d2keep(tx.col, dlnps[ic]);
@@ -998,16 +1058,44 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     }
   }
   __syncthreads();
+  double2 Dfin[KPT];
 #pragma unroll
-  for (int i = 0; i < KPT; ++i)
-    if (tx.col && k0 + i < nd) ddiv[(size_t)(k0 + i) * spec_per_level + idx] = A[i];
+  for (int i = 0; i < KPT; ++i) Dfin[i] = A[i];
   linear_ps_t_tile<KPT>(nd, kappa, r, tx, lds, A, dps, B);
-#pragma unroll
-  for (int i = 0; i < KPT; ++i)
-    if (tx.col && k0 + i < nd)
-      dtt[(size_t)(k0 + i) * spec_per_level + idx] = d2(DD[i].x + xi * B[i].x, DD[i].y + xi * B[i].y);
   const double fx = xi / ps_ref;
-  if (tx.col && tx.q == 0) dlnps[idx] = d2(dl.x + fx * dps.x, dl.y + fx * dps.y);
+  double2 lq = d2(dl.x + fx * dps.x, dl.y + fx * dps.y);
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    if (!(tx.col && k0 + i < nd)) continue;
+    const size_t o = (size_t)(k0 + i) * spec_per_level + idx;
+    double2 qd = Dfin[i];
+    double2 qt = d2(DD[i].x + xi * B[i].x, DD[i].y + xi * B[i].y);
+    if (lf.on) {
+      double2 cn, nn;
+      leapfrog_reg(qd, Dp[i], Dc[i], dmp, true, lf.init_step, lf.dt, lf.r, cn, nn);
+      lf.div_c[o] = cn;
+      lf.div_n[o] = nn;
+      leapfrog_reg(qt, Tp[i], Tc[i], dmp, true, lf.init_step, lf.dt, lf.r, cn, nn);
+      lf.t_c[o] = cn;
+      lf.t_n[o] = nn;
+      double2 qv = QV[i];
+      leapfrog_reg(qv, VP[i], VCc[i], dmp, true, lf.init_step, lf.dt, lf.r, cn, nn);
+      lf.dvor[o] = qv;
+      lf.vor_c[o] = cn;
+      lf.vor_n[o] = nn;
+    }
+    ddiv[o] = qd;
+    dtt[o] = qt;
+  }
+  if (tx.col && tx.q == 0) {
+    dlnps[idx] = lq;
+    if (lf.on) {
+      double2 cn, nn;
+      leapfrog_reg(lq, lnp, lnc, 0.0, false, lf.init_step, lf.dt, lf.r, cn, nn);
+      lf.lnps_c[idx] = cn;
+      lf.lnps_n[idx] = nn;
+    }
+  }
 }
 
 __global__ void vert_advection_kernel(VC v, size_t P, const double* r, const double* dz,
@@ -1380,7 +1468,7 @@ static SIRef siref(const jgcm_semi_implicit* si) {
 static int implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, const double* div_p,
                                const double* lnps_c, const double* lnps_p, const double* t_c,
                                const double* t_p, double* ddiv, double* dlnps, double* dt,
-                               double* energy) {
+                               double* energy, LFFuse lf = LFFuse{}) {
   JGCM_REQUIRE(si && div_c && div_p && lnps_c && lnps_p && t_c && t_p && ddiv && dlnps && dt,
                "NULL argument");
   jgcm_mesh* M = si->vc->mesh;
@@ -1397,7 +1485,7 @@ static int implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, co
                      siref(si), M->d_eig, si->d_wave, si->num_wavenumbers, (const double2*)div_c,
                      (const double2*)div_p, (const double2*)lnps_c, (const double2*)lnps_p,
                      (const double2*)t_c, (const double2*)t_p, (double2*)ddiv, (double2*)dlnps,
-                     (double2*)dt, (double2*)energy, stage_w);
+                     (double2*)dt, (double2*)energy, stage_w, lf);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
@@ -1410,6 +1498,22 @@ int jgcm_implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, cons
 }
 
 // Apply_Laplacian!(spe_energy); spe_δdiv .-= spe_energy; Implicit_Correction!  (SD:309-316)
+int jgcm_hs_energy_implicit_leapfrog(jgcm_semi_implicit_t* si, double* spe_energy, double* div_c,
+                                     const double* div_p, double* lnps_c, const double* lnps_p,
+                                     double* t_c, const double* t_p, double* ddiv, double* dlnps,
+                                     double* dt, double* dvor, const double* vor_p, double* vor_c,
+                                     double* vor_n, double* div_n, double* lnps_n, double* t_n) {
+  JGCM_REQUIRE(spe_energy && dvor && vor_p && vor_c && vor_n && div_n && lnps_n && t_n,
+               "NULL argument");
+  jgcm_leapfrog_t* L = si->lf;
+  LFFuse lf{(double2*)dvor, (const double2*)vor_p, (double2*)vor_c, (double2*)vor_n,
+            (double2*)div_c, (double2*)div_n, (double2*)t_c, (double2*)t_n, (double2*)lnps_c,
+            (double2*)lnps_n, L->d_damping, 1, L->init_step ? 1 : 0, (double)L->dt,
+            L->robert_coef};
+  return implicit_correction(si, div_c, div_p, lnps_c, lnps_p, t_c, t_p, ddiv, dlnps, dt,
+                             spe_energy, lf);
+}
+
 int jgcm_hs_energy_implicit_correction(jgcm_semi_implicit_t* si, double* spe_energy,
                                        const double* div_c, const double* div_p,
                                        const double* lnps_c, const double* lnps_p,

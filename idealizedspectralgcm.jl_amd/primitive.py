@@ -20,7 +20,7 @@ from .mesh import (_ptr, Trans_Grid_To_Spherical, Trans_Spherical_To_Grid,
                    Trans_Grid_To_Spherical_Batch, Trans_Spherical_To_Grid_Batch, BATCH_DIVCOS,
                    BATCH_VORDIV, INV_GRAD_LAMBDA, INV_GRAD_THETA, INV_UCOS, INV_VCOS,
                    Vor_Div_From_Grid_UV, UV_Grid_From_Vor_Div)
-from .time_integrator import Damped_Leapfrog_Batch, Get_Δt
+from .time_integrator import Get_Δt
 
 _VC = {"even_sigma": 0}
 _VD = {"simmons_and_burridge": 0}
@@ -285,16 +285,13 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
         mesh, [d.grid_δlnps, d.grid_δt, d.grid_energy_full, d.grid_δu, d.grid_δv],
         [d.spe_δlnps, d.spe_δt, d.spe_energy, d.spe_δvor, d.spe_δdiv],
         [0, 0, 0, BATCH_VORDIV, 0])
-    # :309-316 ∇²(energy), δdiv -= ∇²(energy), Implicit_Correction!
-    call("jgcm_hs_energy_implicit_correction", si.handle, *map(_ptr, (
+    # :309-316 ∇²(energy), δdiv -= ∇²(energy), Implicit_Correction!, and in the same
+    # pass :319-328 Compute_Spectral_Damping! (vor, div, T) and Filtered_Leapfrog!
+    # (vor, div, lnps, T)
+    call("jgcm_hs_energy_implicit_leapfrog", si.handle, *map(_ptr, (
         d.spe_energy, d.spe_div_c, d.spe_div_p, d.spe_lnps_c, d.spe_lnps_p, d.spe_t_c, d.spe_t_p,
-        d.spe_δdiv, d.spe_δlnps, d.spe_δt)))
-    # :319-328 Compute_Spectral_Damping! (vor, div, T) and Filtered_Leapfrog! (vor, div, lnps, T)
-    Damped_Leapfrog_Batch(integrator, [
-        (d.spe_δvor, d.spe_vor_p, d.spe_vor_c, d.spe_vor_n, True),
-        (d.spe_δdiv, d.spe_div_p, d.spe_div_c, d.spe_div_n, True),
-        (d.spe_δlnps, d.spe_lnps_p, d.spe_lnps_c, d.spe_lnps_n, False),
-        (d.spe_δt, d.spe_t_p, d.spe_t_c, d.spe_t_n, True)])
+        d.spe_δdiv, d.spe_δlnps, d.spe_δt, d.spe_δvor, d.spe_vor_p, d.spe_vor_c, d.spe_vor_n,
+        d.spe_div_n, d.spe_lnps_n, d.spe_t_n)))
     # :331-335: vor, div, (u, v) = UV_Grid_From_Vor_Div!, T, ln ps back to the grid
     Trans_Spherical_To_Grid_Batch(
         mesh, [d.spe_vor_n, d.spe_div_n, d.spe_vor_n, d.spe_vor_n, d.spe_t_n, d.spe_lnps_n],

@@ -469,6 +469,18 @@ int jgcm_hs_energy_implicit_correction(jgcm_semi_implicit_t* si, double* spe_ene
                                        const double* lnps_c, const double* lnps_p,
                                        const double* t_c, const double* t_p, double* ddiv,
                                        double* dlnps, double* dt);
+/* The same, followed by the step's Compute_Spectral_Damping! and
+ * Filtered_Leapfrog! (Spectral_Dynamics.jl:319-328; vorticity, divergence and
+ * T damped, ln ps not) in the same pass (no single reference function): the
+ * c arrays of vor, div, ln ps and T receive the Robert filter, the n arrays
+ * the new values, the tendencies the damped values.  Equal to
+ * jgcm_hs_energy_implicit_correction followed by jgcm_damped_leapfrog_batch
+ * bit for bit.                                                              */
+int jgcm_hs_energy_implicit_leapfrog(jgcm_semi_implicit_t* si, double* spe_energy, double* div_c,
+                                     const double* div_p, double* lnps_c, const double* lnps_p,
+                                     double* t_c, const double* t_p, double* ddiv, double* dlnps,
+                                     double* dt, double* dvor, const double* vor_p, double* vor_c,
+                                     double* vor_n, double* div_n, double* lnps_n, double* t_n);
 /* Step part 1, one column pass: Pressure_Variables!(ps_c) (Spectral_Dynamics.jl:256),
  * Spectral_Dynamics_Physics! / HS_Forcing! on the p-fields (:452-467,
  * HS_Forcing.jl:1-130; skipped when do_forcing = 0, δps = 0 always) and

@@ -217,3 +217,46 @@ def test_fused_corrections_advance_equals_corrections_then_time_advance(jgcm):
     for f in fields:
         assert np.array_equal(a[f], b[f]), f
     assert np.array_equal(np.asarray(sa), np.asarray(sb))
+
+
+@pytest.mark.parametrize("init_step", [True, False])
+def test_fused_implicit_leapfrog_equals_separate_calls(jgcm, init_step):
+    """jgcm_hs_energy_implicit_leapfrog equals jgcm_hs_energy_implicit_correction
+    followed by the damped leapfrog of vor, div, T (damped) and ln ps (not) bit
+    for bit — tendencies, filtered c and new n fields — on the first-step and on
+    the later-step integrator."""
+    from jgcm_amd._lib import call
+    from jgcm_amd.mesh import _ptr
+    _, (dev, dvc, datmo, dsi, dd) = build(jgcm, 42, 64, 20, 600)
+    jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS, overlap=False)
+    if not init_step:
+        jgcm.Update_Init_Step(dsi)
+    dev.synchronize()
+    names = ["spe_energy", "spe_δdiv", "spe_δlnps", "spe_δt", "spe_δvor"] + \
+        [f"spe_{f}_{s}" for f in ("vor", "div", "lnps", "t") for s in ("p", "c", "n")]
+    start = {f: getattr(dd, f).clone() for f in names}
+    # a non-trivial energy field (the step left ∇² of it in place)
+    start["spe_energy"] = start["spe_energy"] * 0.37 + start["spe_t_c"] * 2.0
+    out = []
+    for fused in (False, True):
+        for f in names:
+            getattr(dd, f).copy_(start[f])
+        torch.cuda.synchronize()
+        if fused:
+            call("jgcm_hs_energy_implicit_leapfrog", dsi.handle, *map(_ptr, (
+                dd.spe_energy, dd.spe_div_c, dd.spe_div_p, dd.spe_lnps_c, dd.spe_lnps_p, dd.spe_t_c,
+                dd.spe_t_p, dd.spe_δdiv, dd.spe_δlnps, dd.spe_δt, dd.spe_δvor, dd.spe_vor_p,
+                dd.spe_vor_c, dd.spe_vor_n, dd.spe_div_n, dd.spe_lnps_n, dd.spe_t_n)))
+        else:
+            call("jgcm_hs_energy_implicit_correction", dsi.handle, *map(_ptr, (
+                dd.spe_energy, dd.spe_div_c, dd.spe_div_p, dd.spe_lnps_c, dd.spe_lnps_p, dd.spe_t_c,
+                dd.spe_t_p, dd.spe_δdiv, dd.spe_δlnps, dd.spe_δt)))
+            jgcm.Damped_Leapfrog_Batch(dsi.integrator, [
+                (dd.spe_δvor, dd.spe_vor_p, dd.spe_vor_c, dd.spe_vor_n, True),
+                (dd.spe_δdiv, dd.spe_div_p, dd.spe_div_c, dd.spe_div_n, True),
+                (dd.spe_δlnps, dd.spe_lnps_p, dd.spe_lnps_c, dd.spe_lnps_n, False),
+                (dd.spe_δt, dd.spe_t_p, dd.spe_t_c, dd.spe_t_n, True)])
+        dev.synchronize()
+        out.append({f: getattr(dd, f).cpu().numpy() for f in names})
+    for f in names:
+        assert np.array_equal(out[0][f], out[1][f]), f
