@@ -664,6 +664,9 @@ __device__ void correction_scalars(const double* part_p, const double* part_n, i
 // (0, 0) coefficient for kinds 3 and 4); workgroup 0 of the first kind-1 field
 // stores the seven scalars as hs_apply_corrections_kernel does.  blockIdx.y = field.
 constexpr int MAXCADV = 16;
+#ifndef JGCM_CADV_PER_THREAD
+#define JGCM_CADV_PER_THREAD 4  // elements per thread of hs_correct_advance_kernel
+#endif
 struct CorrAdvanceSet {
   double* p[MAXCADV];
   double* c[MAXCADV];
@@ -1648,7 +1651,7 @@ int jgcm_hs_corrections_advance(jgcm_semi_implicit_t* si, const double* grid_lnp
     s.kind[f] = kinds[f];
     maxc = std::max(maxc, count[f]);
   }
-  const int per_block = 4 * APPLY_THREADS;
+  const int per_block = JGCM_CADV_PER_THREAD * APPLY_THREADS;
   const dim3 grid((unsigned)((maxc + per_block - 1) / per_block), 8);
   hipLaunchKernelGGL(hs_correct_advance_kernel, grid, dim3(APPLY_THREADS), 0, M->stream, s,
                      per_block, 2 * S, M->nlon, part_p, part_n, nparts, si->d_scal, si->atmo.grav,
