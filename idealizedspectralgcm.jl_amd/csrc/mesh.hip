@@ -7,6 +7,7 @@
 // contraction is disabled in this file so each operation rounds as written.
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -374,9 +375,11 @@ int jgcm_mesh_create(int num_fourier, int num_spherical, int nlat, int nlon, int
   JGCM_REQUIRE(nlat >= 2 && nlat % 2 == 0, "nθ must be even (Spectral_Spherical_Mesh.jl:193,275), got %d", nlat);
   JGCM_REQUIRE(nd >= 1, "nd must be >= 1");
   JGCM_REQUIRE(radius > 0.0, "radius must be positive");
+  // powers of two 16..1024 take the radix-4 LDS FFT; other even lengths up to
+  // 2048 the direct DFT kernels
   bool pow2 = nlon >= 16 && nlon <= 1024 && (nlon & (nlon - 1)) == 0;
-  if (!pow2) {
-    set_error("nλ=%d: the longitude FFT supports powers of two 16..1024", nlon);
+  if (!pow2 && (nlon < 16 || nlon > 2048 || nlon % 2 != 0)) {
+    set_error("nλ=%d: the longitude transform supports even lengths 16..2048", nlon);
     return JGCM_ERR_UNSUPPORTED;
   }
   if (2 * num_fourier + 1 > nlon) {
@@ -390,7 +393,14 @@ int jgcm_mesh_create(int num_fourier, int num_spherical, int nlat, int nlon, int
   }
   // LDS budget of the FFT stage: 16 KB work buffer + twiddles + the E/O
   // staging of 8 latitude pairs (or 256/(nλ/4) when larger) x (N+1) x 2.
-  {
+  if (!pow2) {
+    // direct DFT: twiddles + two rows (fold) or twiddles + 2 (N+1) coefficients (unfold)
+    size_t lds = (size_t)nlon * 16 + std::max((size_t)nlon * 16, (size_t)(num_fourier + 1) * 32);
+    if (lds > 65536) {
+      set_error("T%d on nλ=%d exceeds the longitude DFT's 64 KB LDS budget", num_fourier, nlon);
+      return JGCM_ERR_UNSUPPORTED;
+    }
+  } else {
     int fpw = 256 / (nlon / 4);
     int ppw = fpw >= 8 ? fpw : 8;
     size_t lds = 16384 + (size_t)nlon * 16 + (size_t)((num_fourier + 1) * 2 + 1) * ppw * 16;

@@ -330,6 +330,124 @@ __global__ __launch_bounds__(256) void fft_full_kernel(const double* __restrict_
   }
 }
 
+// ------------------------------------------------- general-length longitude DFT
+// For nλ that is not a power of two (the reference's FFTW takes any length):
+// direct sums over the twiddle table tw[k] = exp(-2πi k/nλ), indexed by
+// (m λ) mod nλ, one workgroup per latitude pair (fold, unfold) or row (full)
+// and field.  Same inputs, outputs, scaling and cos θ options as the radix-4
+// kernels above; O(nλ (N+1)) per row pair.
+__global__ __launch_bounds__(256) void dft_fold_kernel(SegMap in, double2* __restrict__ eo,
+                                                       const double2* __restrict__ twg,
+                                                       const double* __restrict__ cosv, int nlat,
+                                                       int L, int nm) {
+  extern __shared__ double2 dsh[];  // tw[L] | row a, row c (L doubles each)
+  double2* tw = dsh;
+  double* ra = reinterpret_cast<double*>(dsh + L);
+  double* rc = ra + L;
+  const int half = nlat / 2;
+  const int jp = blockIdx.x, b = blockIdx.y;
+  const int seg = seg_find(in, b);
+  const double* __restrict__ grid = in.ptr[seg] + (size_t)(b - in.start[seg]) * nlat * L;
+  const double s = (in.flag[seg] & 1) ? cosv[jp] : 1.0;  // cos θ is symmetric about the equator
+  const double* a = grid + (size_t)jp * L;
+  const double* c = grid + (size_t)(nlat - 1 - jp) * L;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    tw[i] = twg[i];
+    ra[i] = (in.flag[seg] & 1) ? a[i] * s : a[i];
+    rc[i] = (in.flag[seg] & 1) ? c[i] * s : c[i];
+  }
+  __syncthreads();
+  const double scale = 1.0 / L;
+  for (int m = threadIdx.x; m < nm; m += blockDim.x) {
+    double2 xa = make_double2(0.0, 0.0), xc = make_double2(0.0, 0.0);
+    int k = 0;
+    for (int l = 0; l < L; ++l) {
+      const double2 w = tw[k];
+      xa = make_double2(xa.x + ra[l] * w.x, xa.y + ra[l] * w.y);
+      xc = make_double2(xc.x + rc[l] * w.x, xc.y + rc[l] * w.y);
+      k += m;
+      if (k >= L) k -= L;
+    }
+    eo[(((size_t)b * nm + m) * 2 + 0) * half + jp] =
+        make_double2((xa.x + xc.x) * scale, (xa.y + xc.y) * scale);
+    eo[(((size_t)b * nm + m) * 2 + 1) * half + jp] =
+        make_double2((xa.x - xc.x) * scale, (xa.y - xc.y) * scale);
+  }
+}
+
+__global__ __launch_bounds__(256) void dft_unfold_kernel(const double2* __restrict__ eo,
+                                                         SegMap out,
+                                                         const double2* __restrict__ twg,
+                                                         const double* __restrict__ cosv,
+                                                         int nlat, int L, int nm) {
+  extern __shared__ double2 dsh[];  // tw[L] | ga[nm] | gc[nm]
+  double2* tw = dsh;
+  double2* ga = dsh + L;
+  double2* gc = ga + nm;
+  const int half = nlat / 2;
+  const int jp = blockIdx.x, b = blockIdx.y;
+  const int seg = seg_find(out, b);
+  double* __restrict__ grid = out.ptr[seg] + (size_t)(b - out.start[seg]) * nlat * L;
+  const bool divcos = out.flag[seg] & 1;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) tw[i] = twg[i];
+  for (int m = threadIdx.x; m < nm; m += blockDim.x) {
+    const double2 E = eo[(((size_t)b * nm + m) * 2 + 0) * half + jp];
+    const double2 O = eo[(((size_t)b * nm + m) * 2 + 1) * half + jp];
+    ga[m] = cadd(E, O);
+    gc[m] = csub(E, O);
+  }
+  __syncthreads();
+  const double c = divcos ? cosv[jp] : 1.0;
+  for (int l = threadIdx.x; l < L; l += blockDim.x) {
+    // Re(g_0) + 2 Σ_m Re(g_m e^{+2πi m l / L})  (the reference's g₀/2 and 2 Re, SSM:211-215)
+    double xa = ga[0].x, xc = gc[0].x;
+    int k = l;
+    for (int m = 1; m < nm; ++m) {
+      const double2 w = tw[k];  // e^{-iθ}: Re(g e^{iθ}) = g.x w.x + g.y w.y
+      xa = xa + 2.0 * (ga[m].x * w.x + ga[m].y * w.y);
+      xc = xc + 2.0 * (gc[m].x * w.x + gc[m].y * w.y);
+      k += l;
+      if (k >= L) k -= L;
+    }
+    if (divcos) {
+      xa = xa / c;
+      xc = xc / c;
+    }
+    grid[(size_t)jp * L + l] = xa;
+    grid[(size_t)(nlat - 1 - jp) * L + l] = xc;
+  }
+}
+
+__global__ __launch_bounds__(256) void dft_full_kernel(const double* __restrict__ grid,
+                                                       double2* __restrict__ four,
+                                                       const double2* __restrict__ twg, int nlat,
+                                                       int L) {
+  extern __shared__ double2 dsh[];  // tw[L] | row (L doubles)
+  double2* tw = dsh;
+  double* row = reinterpret_cast<double*>(dsh + L);
+  const int j = blockIdx.x, b = blockIdx.y;
+  const double* r = grid + ((size_t)b * nlat + j) * L;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    tw[i] = twg[i];
+    row[i] = r[i];
+  }
+  __syncthreads();
+  const double scale = 1.0 / L;
+  for (int kk = threadIdx.x; kk < L; kk += blockDim.x) {
+    double2 x = make_double2(0.0, 0.0);
+    int k = 0;
+    for (int l = 0; l < L; ++l) {
+      const double2 w = tw[k];
+      x = make_double2(x.x + row[l] * w.x, x.y + row[l] * w.y);
+      k += kk;
+      if (k >= L) k -= L;
+    }
+    four[((size_t)b * nlat + j) * L + kk] = make_double2(x.x * scale, x.y * scale);
+  }
+}
+
+static bool pow2_fft(int L) { return L >= 16 && L <= 1024 && (L & (L - 1)) == 0; }
+
 // Full Fourier array [B][nlat][nlon] -> E/O block (for Compute_Alpha_Operator!).
 __global__ void fold_full_kernel(const double2* __restrict__ four, double2* __restrict__ eo,
                                  int nlat, int nlon, int nm, int B) {
@@ -971,13 +1089,27 @@ static void launch_full(const jgcm_mesh* M, const double* grid, double2* four, i
   }
 
 static int fft_fold_map(const jgcm_mesh* M, const SegMap& in, double2* eo) {
-  JGCM_DISPATCH_L(M->nlon, launch_fold<LL>(M, in, eo, in.start[in.nseg]));
+  if (pow2_fft(M->nlon)) {
+    JGCM_DISPATCH_L(M->nlon, launch_fold<LL>(M, in, eo, in.start[in.nseg]));
+  } else {
+    const size_t lds = (size_t)M->nlon * (sizeof(double2) + 2 * sizeof(double));
+    hipLaunchKernelGGL(dft_fold_kernel, dim3(M->half, in.start[in.nseg]), dim3(256), lds,
+                       M->stream, in, eo, reinterpret_cast<const double2*>(M->d_twiddle), M->d_cos,
+                       M->nlat, M->nlon, M->nm);
+  }
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
 
 static int ifft_unfold_map(const jgcm_mesh* M, const double2* eo, const SegMap& out) {
-  JGCM_DISPATCH_L(M->nlon, launch_unfold<LL>(M, eo, out, out.start[out.nseg]));
+  if (pow2_fft(M->nlon)) {
+    JGCM_DISPATCH_L(M->nlon, launch_unfold<LL>(M, eo, out, out.start[out.nseg]));
+  } else {
+    const size_t lds = ((size_t)M->nlon + 2 * (size_t)M->nm) * sizeof(double2);
+    hipLaunchKernelGGL(dft_unfold_kernel, dim3(M->half, out.start[out.nseg]), dim3(256), lds,
+                       M->stream, eo, out, reinterpret_cast<const double2*>(M->d_twiddle),
+                       M->d_cos, M->nlat, M->nlon, M->nm);
+  }
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
@@ -1154,7 +1286,13 @@ int legendre_inv_adjoint(const jgcm_mesh* M, const double2* eo, double2* snm, in
 }
 
 int fft_full(const jgcm_mesh* M, const double* grid, double2* four, int B) {
-  JGCM_DISPATCH_L(M->nlon, launch_full<LL>(M, grid, four, B));
+  if (pow2_fft(M->nlon)) {
+    JGCM_DISPATCH_L(M->nlon, launch_full<LL>(M, grid, four, B));
+  } else {
+    const size_t lds = (size_t)M->nlon * (sizeof(double2) + sizeof(double));
+    hipLaunchKernelGGL(dft_full_kernel, dim3(M->nlat, B), dim3(256), lds, M->stream, grid, four,
+                       reinterpret_cast<const double2*>(M->d_twiddle), M->nlat, M->nlon);
+  }
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

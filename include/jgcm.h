@@ -104,7 +104,10 @@ int jgcm_time_advance(jgcm_mesh_t* mesh, int nfields, double* const* p, double* 
 /* ---- mesh: Spectral_Spherical_Mesh(num_fourier, num_spherical, nθ, nλ, nd,
  *      radius)  Spectral_Spherical_Mesh.jl:69-139.  Gaussian nodes/weights
  *      (Gauss_And_Legendre.jl:69-118) and the Legendre recurrences
- *      (Gauss_And_Legendre.jl:4-66) are computed in FP64 on the GPU.          */
+ *      (Gauss_And_Legendre.jl:4-66) are computed in FP64 on the GPU.
+ *      nlon: any even length 16..2048 with nlon >= 2 num_fourier + 1 (powers
+ *      of two up to 1024 take the radix-4 FFT, others a direct DFT); nlat <=
+ *      512.  Otherwise JGCM_ERR_UNSUPPORTED.                                  */
 int jgcm_mesh_create(int num_fourier, int num_spherical, int nlat, int nlon,
                      int nd, double radius, jgcm_mesh_t** out);
 int jgcm_mesh_destroy(jgcm_mesh_t* mesh);

@@ -75,7 +75,8 @@ def test_initial_state_matches_oracle(jgcm):
     assert np.abs(d.spe_div_c).max() < 1e-12 * vmax
 
 
-@pytest.mark.parametrize("N,nlat,nd,dt,steps", [(21, 32, 10, 600, 4), (42, 64, 20, 600, 2)])
+@pytest.mark.parametrize("N,nlat,nd,dt,steps", [(21, 32, 10, 600, 4), (42, 64, 20, 600, 2),
+                                                (31, 48, 10, 600, 2)])
 def test_hs_steps_match_oracle(jgcm, N, nlat, nd, dt, steps):
     (M, vc, atmo, si, d), (dev, dvc, datmo, dsi, dd) = build(jgcm, N, nlat, nd, dt)
     for i in range(steps):

@@ -103,7 +103,7 @@ def sw_setup(N, nlat):
     return M, d, h_0, np.ascontiguousarray(h_eq)
 
 
-@pytest.mark.parametrize("N,nlat", [(42, 64)])
+@pytest.mark.parametrize("N,nlat", [(42, 64), (31, 48)])
 def test_shallow_water_steps_match_oracle(jgcm, N, nlat):
     M, d, h_0, h_eq = sw_setup(N, nlat)
     dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 1, 6.371e6)

@@ -13,7 +13,8 @@ from oracle import SpectralSphericalMesh
 pytestmark = pytest.mark.gpu
 
 RADIUS = 6371.0e3
-TRUNCS = [(21, 32), (42, 64), (85, 128)]
+TRUNCS = [(21, 32), (42, 64), (85, 128),
+          (31, 48), (63, 96)]   # 96 and 192 longitudes: the direct-DFT path (not powers of two)
 _CACHE = {}
 
 
@@ -148,7 +149,7 @@ def test_vor_div_uv(jgcm, N, nlat):
     assert rel(vc.cpu().numpy(), vcr) < 1e-14
 
 
-@pytest.mark.parametrize("N,nlat", TRUNCS[:2])
+@pytest.mark.parametrize("N,nlat", TRUNCS[:2] + TRUNCS[3:4])
 def test_gradients_advection_laplacian(jgcm, N, nlat):
     nd = 2
     dev, ref = pair(jgcm, N, nlat, nd)
@@ -181,7 +182,7 @@ def test_gradients_advection_laplacian(jgcm, N, nlat):
     assert rel(gc.cpu().numpy(), g) < 1e-14
 
 
-@pytest.mark.parametrize("N,nlat", TRUNCS[:2])
+@pytest.mark.parametrize("N,nlat", TRUNCS[:2] + TRUNCS[3:4])
 def test_fourier_and_alpha(jgcm, N, nlat):
     nd = 2
     dev, ref = pair(jgcm, N, nlat, nd)
