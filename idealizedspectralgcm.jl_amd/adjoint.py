@@ -397,7 +397,7 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         final = adj.forward(state, nsteps)    # state: tensors in Held_Suarez_Adjoint.STATE order
         grads = adj.backward(final_bars)      # same order
 
-    Needs the even-sigma coordinate (the library's only one).  The reverse
+    Needs a pure sigma coordinate (even sigma, mcm or v197).  The reverse
     step composes C entry points: the leapfrog transposes, the implicit
     correction's transpose from its per-n matrix (built once per solver by
     applying the forward to unit vectors), the transposed transforms and

@@ -22,10 +22,10 @@
 //                              its operators, Semi_Implicit.jl:112-160)
 //   gradient_cos_adjoint_kernel transpose of Compute_Gradient_Cos! (SSM:677-687)
 //
-// The vertical coordinate must be even sigma with a zero top (the only
-// coordinate the library builds): every pressure is then b_k ps, the
-// ln-pressure differences are constants, and ps cancels out of the column
-// dynamics, which leaves ps in the forcing's T_eq and in the corrections.
+// The vertical coordinate must be pure sigma (ak = 0: even sigma, mcm, v197)
+// with a zero top: every pressure is then b_k ps, the ln-pressure
+// differences are constants, and ps cancels out of the column dynamics, which
+// leaves ps in the forcing's T_eq and in the corrections.
 #pragma clang fp contract(off)
 
 #include <cmath>
@@ -42,7 +42,7 @@ static inline int nblk_h(size_t n) {
 #define HS_GRID_STRIDE(i, n) \
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
 
-// Per-level constants of an even-sigma, zero-top coordinate (Pressure_Variables!,
+// Per-level constants of a pure-sigma, zero-top coordinate (Pressure_Variables!,
 // Press_And_Geopot.jl:36-67, at ps = 1; differences of ln p do not depend on ps):
 //   Dp = ln p_half[k+1] - ln p_full[k],  Dl = ln p_half[k+1] - ln p_half[k] (k >= 1),
 //   C1 = (b_k (ln p_full - ln p_half[k]) + b_{k+1} Dp) / Δb_k  (Four_In_One!'s x1 times ps),
@@ -95,7 +95,7 @@ struct HsAdjScalars {
 };
 
 // Column tiles: 64 columns per workgroup (one per lane), wave q on levels
-// q*KPT .. q*KPT+KPT-1.  Forward quantities (ps factored out, even sigma):
+// q*KPT .. q*KPT+KPT-1.  Forward quantities (ps factored out, pure sigma):
 //   Q_k = D_k + u_k gλ + v_k gθ,  S_k = Σ_{j<k} Δb_j Q_j,  S = Σ_j Δb_j Q_j
 //   x5_k = -(Dl_k/Δb_k) S_k - Dp_k Q_k + C1_k (u_k gλ + v_k gθ);  δT_k += κ T_k x5_k
 //   δu_k -= R T_k C1_k gλ,  δv_k -= R T_k C1_k gθ,  δln ps = -S
@@ -512,9 +512,9 @@ __global__ void gradient_cos_adjoint_kernel(const double2* glb, const double2* g
   }
 }
 
-static int check_even_sigma(const jgcm_vert_coord* v) {
+static int check_pure_sigma(const jgcm_vert_coord* v) {
   JGCM_REQUIRE(v->zero_top == 1, "the adjoint needs a zero-top coordinate");
-  for (double a : v->ak) JGCM_REQUIRE(a == 0.0, "the adjoint needs an even-sigma coordinate (ak = 0)");
+  for (double a : v->ak) JGCM_REQUIRE(a == 0.0, "the adjoint needs a pure sigma coordinate (ak = 0)");
   JGCM_REQUIRE(v->nd >= 1 && v->nd <= MAXND, "nd must be in 1..%d", MAXND);
   return JGCM_OK;
 }
@@ -530,7 +530,7 @@ int jgcm_hs_column_adjoint(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, d
                            const double* const* cot_in, double* const* cot_out,
                            double* param_bar) {
   JGCM_REQUIRE(si && hp && scal_bar && state && cot_in && cot_out, "NULL argument");
-  if (int rc = check_even_sigma(si->vc)) return rc;
+  if (int rc = check_pure_sigma(si->vc)) return rc;
   jgcm_mesh* M = si->vc->mesh;
   for (int i = 0; i < 14; ++i) JGCM_REQUIRE(state[i], "NULL state array %d", i);
   for (int i = 0; i < 5; ++i) JGCM_REQUIRE(cot_in[i], "NULL input cotangent %d", i);
@@ -575,7 +575,7 @@ int jgcm_hs_corrections_adjoint(jgcm_semi_implicit_t* si, const double* scal,
   JGCM_REQUIRE(si && scal && spe_t_c_bar && spe_lnps_c_bar && grid_u_n && grid_v_n && grid_t_n &&
                    grid_ps_n && scal_bar && u_n_bar && v_n_bar && t_n_bar && lnps_n_bar,
                "NULL argument");
-  if (int rc = check_even_sigma(si->vc)) return rc;
+  if (int rc = check_pure_sigma(si->vc)) return rc;
   jgcm_mesh* M = si->vc->mesh;
   hipLaunchKernelGGL(hs_correction_scalars_adjoint_kernel, dim3(1), dim3(64), 0, M->stream, scal,
                      (const double2*)spe_t_c_bar, (const double2*)spe_lnps_c_bar, si->nd,

@@ -1225,6 +1225,18 @@ static void host_pressure(const jgcm_vert_coord* v, double ps, std::vector<doubl
   for (int k = 0; k < nd; ++k) pf[k] = std::exp(lf[k]);
 }
 
+// The reference's two fixed sigma tables (b at the nd+1 half levels, top to
+// surface; a = 0): the old model's 14 levels (Compute_Old_Model_Sigma,
+// Vert_Coordinate.jl:157-165) and the 18 levels of v197 (Compute_V197_Sigma,
+// :143-154).
+static const double kSigmaMcm[15] = {0.0,    0.03,   0.0707, 0.1311, 0.2102,
+                                     0.3036, 0.4062, 0.5138, 0.6226, 0.7284,
+                                     0.8255, 0.9066, 0.9640, 0.9933, 1.0};
+static const double kSigmaV197[19] = {0.0,       0.0089163, 0.0342936, 0.0740741, 0.1262002,
+                                      0.1886145, 0.2592592, 0.3360768, 0.4170096, 0.5000000,
+                                      0.5829904, 0.6639231, 0.7407407, 0.8113854, 0.8737997,
+                                      0.9259259, 0.9657064, 0.9910837, 1.0};
+
 }  // namespace jgcm
 
 using namespace jgcm;
@@ -1237,11 +1249,35 @@ int jgcm_vert_coord_create(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_REQUIRE(out != nullptr, "out is NULL");
   JGCM_REQUIRE(nd >= 1 && nd <= MAXND, "nd must be in 1..%d", MAXND);
-  if (vert_coord_option != JGCM_VC_EVEN_SIGMA) {
-    set_error("Compute_Vert_Coord option %d is not a valid value (only even_sigma is supported)",
-              vert_coord_option);
-    return JGCM_ERR_STATE;
+  // Compute_Vert_Coord (Vert_Coordinate.jl:61-87)
+  const double* table = nullptr;
+  int table_nd = 0;
+  switch (vert_coord_option) {
+    case JGCM_VC_EVEN_SIGMA:
+      break;
+    case JGCM_VC_MCM:
+      table = kSigmaMcm;
+      table_nd = 14;
+      break;
+    case JGCM_VC_V197:
+      table = kSigmaV197;
+      table_nd = 18;
+      break;
+    case JGCM_VC_UNEVEN_SIGMA:
+      set_error("Compute_Vert_Coord uneven_sigma cannot run in the reference: "
+                "Vert_Coordinate.jl:70 passes a and b before they are defined");
+      return JGCM_ERR_STATE;
+    case JGCM_VC_HYBRID:
+      set_error("Compute_Vert_Coord hybrid cannot run in the reference: "
+                "Vert_Coordinate.jl:72-73 call Compute_Uneven_Sigma without nd");
+      return JGCM_ERR_STATE;
+    default:
+      set_error("Compute_Vert_Coord option %d is not a valid value for option", vert_coord_option);
+      return JGCM_ERR_STATE;
   }
+  JGCM_REQUIRE(table == nullptr || nd == table_nd,
+               "the %s coordinate has %d levels, got nd = %d",
+               vert_coord_option == JGCM_VC_MCM ? "mcm" : "v197", table_nd, nd);
   if (vert_difference_option != JGCM_VD_SIMMONS_BURRIDGE) {
     set_error("vert_difference_option %d is not a valid value", vert_difference_option);
     return JGCM_ERR_STATE;
@@ -1257,10 +1293,11 @@ int jgcm_vert_coord_create(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
   v->advect_scheme = vert_advect_scheme == JGCM_VA_SECOND_CENTERED_WTS ? 0 : 1;
   v->zero_top = zero_top ? 1 : 0;
   v->p_ref = p_ref;
-  // Compute_Even_Sigma (Vert_Coordinate.jl:112-117): a = 0, b = LinRange(0, 1, nd+1)
+  // Compute_Even_Sigma (Vert_Coordinate.jl:112-117): a = 0, b = LinRange(0, 1, nd+1);
+  // the tables: a = 0, b as listed
   v->ak.assign(nd + 1, 0.0);
   v->bk.resize(nd + 1);
-  for (int k = 0; k <= nd; ++k) v->bk[k] = (double)k / nd;
+  for (int k = 0; k <= nd; ++k) v->bk[k] = table ? table[k] : (double)k / nd;
   v->bk[nd] = 1.0;
   v->dak.resize(nd);
   v->dbk.resize(nd);

@@ -22,7 +22,7 @@ from .mesh import (_ptr, Trans_Grid_To_Spherical, Trans_Spherical_To_Grid,
                    Vor_Div_From_Grid_UV, UV_Grid_From_Vor_Div)
 from .time_integrator import Get_Δt
 
-_VC = {"even_sigma": 0}
+_VC = {"even_sigma": 0, "uneven_sigma": 1, "hybrid": 2, "mcm": 3, "v197": 4}
 _VD = {"simmons_and_burridge": 0}
 _VA = {"second_centered_wts": 0, "second_centered": 1}
 

@@ -72,8 +72,13 @@ typedef struct jgcm_hs_params {
 } jgcm_hs_params;
 
 /* Vert_Coordinate options (Vert_Coordinate.jl:61-87, Press_And_Geopot.jl:63,
- * Vert_Coordinate.jl:195-203). */
+ * Vert_Coordinate.jl:195-203).  UNEVEN_SIGMA and HYBRID are named so callers
+ * can pass the reference's strings; creation rejects them (see below). */
 #define JGCM_VC_EVEN_SIGMA 0
+#define JGCM_VC_UNEVEN_SIGMA 1
+#define JGCM_VC_HYBRID 2
+#define JGCM_VC_MCM 3
+#define JGCM_VC_V197 4
 #define JGCM_VD_SIMMONS_BURRIDGE 0
 #define JGCM_VA_SECOND_CENTERED_WTS 0
 #define JGCM_VA_SECOND_CENTERED 1
@@ -412,8 +417,13 @@ int jgcm_shallow_water_physics(jgcm_mesh_t* mesh, const double* grid_u, const do
 /* ---- primitive equations (Held-Suarez) ------------------------------------- */
 /* Vert_Coordinate(nλ, nθ, nd, vert_coord_option, vert_difference_option,
  *                 vert_advect_scheme, p_ref, zero_top)   Vert_Coordinate.jl:42-56
- * (even sigma only, :112-117; the uneven/hybrid options are broken in the
- * reference, :69-76).  nd <= 64.                                           */
+ * Pure sigma coordinates (ak = 0): even sigma (:112-117, any nd <= 64) and
+ * the fixed tables mcm (Compute_Old_Model_Sigma, :157-165, nd = 14) and v197
+ * (Compute_V197_Sigma, :143-154, nd = 18); for the tables nd must equal their
+ * length (the reference indexes the table with nd and silently truncates a
+ * smaller one).  uneven_sigma and hybrid fail as in the reference, where
+ * they cannot run (:70 passes undefined a, b; :72-73 omit nd):
+ * JGCM_ERR_STATE with a message naming the defect.                         */
 int jgcm_vert_coord_create(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
                            int vert_difference_option, int vert_advect_scheme, double p_ref,
                            int zero_top, jgcm_vert_coord_t** out);

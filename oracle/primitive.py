@@ -4,7 +4,7 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 
 Restates, line by line (0-based level indices; half levels 0..nd):
 
-* ``VertCoordinate``          <- Vert_Coordinate.jl:42-56, even sigma :112-117
+* ``VertCoordinate``          <- Vert_Coordinate.jl:42-56, even sigma :112-117, tables :143-165
 * ``vert_advection``          <- ``Vert_Advection!`` Vert_Coordinate.jl:171-210
 * ``mass_weighted_global_integral`` <- Vert_Coordinate.jl:215-238
 * ``pressure_variables``      <- ``Pressure_Variables!`` Press_And_Geopot.jl:36-67
@@ -33,10 +33,32 @@ import numpy as np
 from .dynamics import compute_abs_vor, time_advance
 
 
+# Compute_Old_Model_Sigma (Vert_Coordinate.jl:157-165) and Compute_V197_Sigma
+# (:143-154): b at the half levels, a = 0
+SIGMA_TABLES = {
+    "mcm": (0.0, 0.03, 0.0707, 0.1311, 0.2102, 0.3036, 0.4062, 0.5138, 0.6226, 0.7284,
+            0.8255, 0.9066, 0.9640, 0.9933, 1.0),
+    "v197": (0.0, 0.0089163, 0.0342936, 0.0740741, 0.1262002, 0.1886145, 0.2592592,
+             0.3360768, 0.4170096, 0.5000000, 0.5829904, 0.6639231, 0.7407407, 0.8113854,
+             0.8737997, 0.9259259, 0.9657064, 0.9910837, 1.0),
+}
+
+
 class VertCoordinate:
+    """Vert_Coordinate.jl:42-56 with Compute_Vert_Coord (:61-87) for the pure
+    sigma options: even_sigma (:112-117) and the fixed tables, whose nd must
+    equal their length.  uneven_sigma and hybrid cannot run in the reference
+    (:70, :72-73) and raise here."""
+
     def __init__(self, nlon, nlat, nd, vert_coord_option, vert_difference_option,
                  vert_advect_scheme, p_ref=101325.0, zero_top=True):
-        if vert_coord_option != "even_sigma":
+        if vert_coord_option == "even_sigma":
+            bk = np.linspace(0.0, 1.0, nd + 1)
+        elif vert_coord_option in SIGMA_TABLES:
+            bk = np.array(SIGMA_TABLES[vert_coord_option])
+            if nd != len(bk) - 1:
+                raise ValueError(f"{vert_coord_option} has {len(bk) - 1} levels, got nd = {nd}")
+        else:
             raise ValueError(f"Compute_Vert_Coord {vert_coord_option} is not supported")
         self.nlon, self.nlat, self.nd = nlon, nlat, nd
         self.vert_coord_option = vert_coord_option
@@ -44,7 +66,7 @@ class VertCoordinate:
         self.vert_advect_scheme = vert_advect_scheme
         self.p_ref, self.zero_top = p_ref, zero_top
         self.ak = np.zeros(nd + 1)
-        self.bk = np.linspace(0.0, 1.0, nd + 1)
+        self.bk = bk
         self.Δak = self.ak[1:] - self.ak[:-1]
         self.Δbk = self.bk[1:] - self.bk[:-1]
 

@@ -39,10 +39,10 @@ def dot(a, b):
     return float(np.sum(a * b))
 
 
-def model(jgcm, N=21, nlat=32, nd=8, dt=600, spinup=3, seed=11):
+def model(jgcm, N=21, nlat=32, nd=8, dt=600, spinup=3, seed=11, coord="even_sigma"):
     nlon = 2 * nlat
     mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, nlon, nd, A)
-    vc = jgcm.Vert_Coordinate(nlon, nlat, nd, "even_sigma", "simmons_and_burridge",
+    vc = jgcm.Vert_Coordinate(nlon, nlat, nd, coord, "simmons_and_burridge",
                               "second_centered_wts", 1e5, mesh=mesh)
     atmo = jgcm.Atmo_Data("hs", nlon, nlat, nd, True, True, False, False, mesh.sinθ, A, OMEGA)
 
@@ -124,9 +124,11 @@ def test_gradient_cos_adjoint_dot_product(jgcm):
     assert abs(lhs - rhs) <= 1e-13 * scale, (lhs, rhs, scale)
 
 
-@pytest.mark.parametrize("K,starts_init", [(1, True), (1, False), (3, True)])
-def test_hs_gradient_matches_central_differences(jgcm, K, starts_init):
-    mesh, vc, atmo, make_solver, d = model(jgcm, spinup=3)
+@pytest.mark.parametrize("K,starts_init,coord,nd", [
+    (1, True, "even_sigma", 8), (1, False, "even_sigma", 8), (3, True, "even_sigma", 8),
+    (2, True, "mcm", 14), (2, True, "v197", 18)])
+def test_hs_gradient_matches_central_differences(jgcm, K, starts_init, coord, nd):
+    mesh, vc, atmo, make_solver, d = model(jgcm, nd=nd, spinup=3, coord=coord)
     state = state_of(jgcm, d)
     N = mesh.num_fourier
     rng = np.random.default_rng(61 + K)

@@ -20,11 +20,11 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-def build(jgcm, N, nlat, nd, dt, seed=11):
+def build(jgcm, N, nlat, nd, dt, seed=11, coord="even_sigma"):
     nlon = 2 * nlat
     # oracle
     M = SpectralSphericalMesh(N, N + 1, nlat, nlon, nd, A)
-    vc = P.VertCoordinate(nlon, nlat, nd, "even_sigma", "simmons_and_burridge", "second_centered_wts", 1e5)
+    vc = P.VertCoordinate(nlon, nlat, nd, coord, "simmons_and_burridge", "second_centered_wts", 1e5)
     atmo = odyn.AtmoData("hs", nlon, nlat, nd, True, True, False, False, M.sin_theta, A, OMEGA)
     I = FilteredLeapfrog(0.04, 4, 1.15741e-4, M.laplacian_eig, 0.5, dt, True, 0, 86400 * 200)
     si = P.SemiImplicitSolver(vc, atmo, I, 1e5, np.full(nd, 300.0), M.wave_numbers)
@@ -33,7 +33,7 @@ def build(jgcm, N, nlat, nd, dt, seed=11):
     P.spectral_initialize_fields(M, atmo, vc, 1.0e5, 300.0, d.grid_geopots, d, grid_t_noise=noise)
     # device
     dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, nlon, nd, A)
-    dvc = jgcm.Vert_Coordinate(nlon, nlat, nd, "even_sigma", "simmons_and_burridge",
+    dvc = jgcm.Vert_Coordinate(nlon, nlat, nd, coord, "simmons_and_burridge",
                                "second_centered_wts", 1e5, mesh=dev)
     datmo = jgcm.Atmo_Data("hs", nlon, nlat, nd, True, True, False, False, dev.sinθ, A, OMEGA)
     dI = jgcm.Filtered_Leapfrog(0.04, 4, 1.15741e-4, dev.laplacian_eig, 0.5, dt, True, 0, 86400 * 200)
@@ -49,15 +49,24 @@ FIELDS = ["spe_vor_c", "spe_div_c", "spe_t_c", "spe_lnps_c", "grid_u_c", "grid_v
           "grid_ps_c", "grid_p_full", "grid_z_full", "grid_t_eq"]
 
 
+# The divergence starts at zero and stays small: it is formed from terms that
+# nearly cancel, so its error relative to its own maximum is the loosest of
+# the fields (measured 5e-11 to 2.5e-10 over the first four steps at T21 for
+# even sigma, mcm and v197, without growth; T, ps and ln ps agree to 1e-15 and
+# vorticity to 1e-11 in the same runs).
+DIV_TOL = 5e-10
+
+
 def compare(d, dd, tol, fields=FIELDS):
     torch.cuda.synchronize()
     for f in fields:
         e = rel(getattr(dd, f).cpu().numpy(), getattr(d, f))
-        assert e < tol, (f, e)
+        assert e < (max(tol, DIV_TOL) if f == "spe_div_c" else tol), (f, e)
 
 
-def test_semi_implicit_setup_matches_oracle(jgcm):
-    ref, dev = build(jgcm, 21, 32, 10, 600)
+@pytest.mark.parametrize("coord,nd", [("even_sigma", 10), ("mcm", 14), ("v197", 18)])
+def test_semi_implicit_setup_matches_oracle(jgcm, coord, nd):
+    ref, dev = build(jgcm, 21, 32, nd, 600, coord=coord)
     si, dsi = ref[3], dev[3]
     np.testing.assert_allclose(dsi.h, si.h, rtol=1e-13)
     np.testing.assert_allclose(dsi.div_mat, si.div_mat, rtol=1e-12, atol=1e-12 * np.abs(si.div_mat).max())
@@ -75,10 +84,11 @@ def test_initial_state_matches_oracle(jgcm):
     assert np.abs(d.spe_div_c).max() < 1e-12 * vmax
 
 
-@pytest.mark.parametrize("N,nlat,nd,dt,steps", [(21, 32, 10, 600, 4), (42, 64, 20, 600, 2),
-                                                (31, 48, 10, 600, 2)])
-def test_hs_steps_match_oracle(jgcm, N, nlat, nd, dt, steps):
-    (M, vc, atmo, si, d), (dev, dvc, datmo, dsi, dd) = build(jgcm, N, nlat, nd, dt)
+@pytest.mark.parametrize("N,nlat,nd,dt,steps,coord", [
+    (21, 32, 10, 600, 4, "even_sigma"), (42, 64, 20, 600, 2, "even_sigma"),
+    (31, 48, 10, 600, 2, "even_sigma"), (21, 32, 14, 600, 3, "mcm"), (21, 32, 18, 600, 3, "v197")])
+def test_hs_steps_match_oracle(jgcm, N, nlat, nd, dt, steps, coord):
+    (M, vc, atmo, si, d), (dev, dvc, datmo, dsi, dd) = build(jgcm, N, nlat, nd, dt, coord=coord)
     for i in range(steps):
         P.atmosphere_update(M, atmo, vc, si, d, HS)
         jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS)
@@ -89,6 +99,29 @@ def test_hs_steps_match_oracle(jgcm, N, nlat, nd, dt, steps):
     np.testing.assert_allclose(dsi.wave_matrix, si.wave_matrix, rtol=1e-11, atol=1e-13)
     sc = dsi.scalars()
     assert np.isfinite(sc).all()
+
+
+def test_vertical_coordinate_options(jgcm):
+    """Compute_Vert_Coord (Vert_Coordinate.jl:61-87) through the C ABI: the
+    mcm and v197 tables equal the fixture read from the reference's text; a
+    table with the wrong level count is an argument error; uneven_sigma and
+    hybrid, which cannot run in the reference, fail with a state error."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "sigma_tables.json")) as f:
+        tables = json.load(f)
+    mesh = jgcm.Spectral_Spherical_Mesh(21, 22, 32, 64, 18, A)
+    for name, t in tables.items():
+        vc = jgcm.Vert_Coordinate(64, 32, t["nd"], name, "simmons_and_burridge",
+                                  "second_centered_wts", 1e5, mesh=mesh)
+        assert list(vc.bk) == t["b"] and not np.asarray(vc.ak).any(), name
+        with pytest.raises(AssertionError):
+            jgcm.Vert_Coordinate(64, 32, t["nd"] + 1, name, "simmons_and_burridge",
+                                 "second_centered_wts", 1e5, mesh=mesh)
+    for name in ("uneven_sigma", "hybrid"):
+        with pytest.raises(jgcm.JGCMError, match="cannot run in the reference"):
+            jgcm.Vert_Coordinate(64, 32, 10, name, "simmons_and_burridge", "second_centered_wts",
+                                 1e5, mesh=mesh)
 
 
 def test_update_output_matches_oracle(jgcm):
