@@ -85,14 +85,38 @@ class Dyn_Data:
             setattr(self, f, gr(1))
         for f in _GRID_HALF:
             setattr(self, f, gr(nd + 1))
+        # tracers (Dyn_Data.jl:181-184, 208-211; Julia [.., nd, ntracer] =
+        # [ntracer][nd][..] here), allocated even when empty like the reference's
+        for s in ("n", "c", "p"):
+            setattr(self, f"spe_tracers_{s}", torch.zeros(
+                (num_spe_tracters, nd, num_spherical + 1, num_fourier + 1), dtype=torch.complex128,
+                device=dev))
+            setattr(self, f"grid_tracers_{s}", torch.zeros((num_grid_tracters, nd, nθ, nλ),
+                                                           dtype=torch.float64, device=dev))
+
+
+def _tracer_names(d):
+    return [f"{f}_tracers" for f in ("spe", "grid") if getattr(d, f"num_{f}_tracters") > 0]
 
 
 def Time_Advance(dyn_data):
-    """Dyn_Data.jl:326-364: p = c, c = n for the spectral vor, div, lnps, T and
-    the grid u, v, ps, T, as one device launch."""
-    d = dyn_data
-    names = [f"spe_{f}" for f in ("vor", "div", "lnps", "t")] + \
-        [f"grid_{f}" for f in ("u", "v", "ps", "t")]
+    """Dyn_Data.jl:326-364: p = c, c = n for the spectral vor, div, lnps, T,
+    the grid u, v, ps, T and the tracers (when there are any), as one device
+    launch."""
+    _advance(dyn_data, [f"spe_{f}" for f in ("vor", "div", "lnps", "t")] +
+             [f"grid_{f}" for f in ("u", "v", "ps", "t")] + _tracer_names(dyn_data))
+
+
+def Time_Advance_Tracers(dyn_data):
+    """The tracer part of Time_Advance! alone (Dyn_Data.jl:341-344, 360-363);
+    for steps whose other fields are advanced by a fused call.  No launch
+    without tracers."""
+    names = _tracer_names(dyn_data)
+    if names:
+        _advance(dyn_data, names)
+
+
+def _advance(d, names):
     n = len(names)
     ps, cs, ns = (ctypes.c_void_p * n)(), (ctypes.c_void_p * n)(), (ctypes.c_void_p * n)()
     counts = (ctypes.c_size_t * n)()

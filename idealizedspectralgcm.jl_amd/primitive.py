@@ -20,6 +20,7 @@ from .mesh import (_ptr, Trans_Grid_To_Spherical, Trans_Spherical_To_Grid,
                    Trans_Grid_To_Spherical_Batch, Trans_Spherical_To_Grid_Batch, BATCH_DIVCOS,
                    BATCH_VORDIV, INV_GRAD_LAMBDA, INV_GRAD_THETA, INV_UCOS, INV_VCOS,
                    Vor_Div_From_Grid_UV, UV_Grid_From_Vor_Div)
+from .dyn_data import Time_Advance_Tracers
 from .time_integrator import Get_Δt
 
 _VC = {"even_sigma": 0, "uneven_sigma": 1, "hybrid": 2, "mcm": 3, "v197": 4}
@@ -306,6 +307,7 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
     ptrs = [(ctypes.c_void_p * 8)(*[t.data_ptr() for t in ts]) for ts in arrs]
     counts = (ctypes.c_size_t * 8)(*[t.numel() * t.element_size() // 8 for t in arrs[1]])
     call("jgcm_hs_corrections_advance", si.handle, _ptr(d.grid_lnps), *ptrs, counts)
+    Time_Advance_Tracers(d)
     if pressure_variables:
         Pressure_Variables(vert_coord, d.grid_ps_c, d.grid_p_half, d.grid_Δp, d.grid_lnp_half,
                            d.grid_p_full, d.grid_lnp_full)                        # :355

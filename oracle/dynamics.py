@@ -50,11 +50,18 @@ class DynData:
     SPEC3 = ["vor", "div", "t"]
     SPEC1 = ["lnps"]
 
-    def __init__(self, name, num_fourier, num_spherical, nlon, nlat, nd):
+    def __init__(self, name, num_fourier, num_spherical, nlon, nlat, nd, num_grid_tracters=0,
+                 num_spe_tracters=0):
         self.name = name
         self.nd = nd
+        self.num_grid_tracters, self.num_spe_tracters = num_grid_tracters, num_spe_tracters
         sp = lambda k: np.zeros((k, num_spherical + 1, num_fourier + 1), np.complex128)
         gr = lambda k: np.zeros((k, nlat, nlon))
+        # tracers (Dyn_Data.jl:181-184, 208-211): [tracer][level][...]
+        for s in "ncp":
+            setattr(self, f"spe_tracers_{s}", np.zeros(
+                (num_spe_tracters, nd, num_spherical + 1, num_fourier + 1), np.complex128))
+            setattr(self, f"grid_tracers_{s}", np.zeros((num_grid_tracters, nd, nlat, nlon)))
         for f in self.SPEC3:
             for s in "ncp":
                 setattr(self, f"spe_{f}_{s}", sp(nd))
@@ -92,6 +99,11 @@ def time_advance(d):
     for f in ("u", "v", "ps", "t"):
         getattr(d, f"grid_{f}_p")[...] = getattr(d, f"grid_{f}_c")
         getattr(d, f"grid_{f}_c")[...] = getattr(d, f"grid_{f}_n")
+    # tracers only when there are any (:341-344, :360-363)
+    for f in ("spe", "grid"):
+        if getattr(d, f"num_{f}_tracters", 0) > 0:
+            getattr(d, f"{f}_tracers_p")[...] = getattr(d, f"{f}_tracers_c")
+            getattr(d, f"{f}_tracers_c")[...] = getattr(d, f"{f}_tracers_n")
 
 
 def barotropic_dynamics(mesh, atmo, d, integ):

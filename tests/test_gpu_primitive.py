@@ -20,7 +20,7 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-def build(jgcm, N, nlat, nd, dt, seed=11, coord="even_sigma"):
+def build(jgcm, N, nlat, nd, dt, seed=11, coord="even_sigma", tracers=(0, 0)):
     nlon = 2 * nlat
     # oracle
     M = SpectralSphericalMesh(N, N + 1, nlat, nlon, nd, A)
@@ -28,7 +28,7 @@ def build(jgcm, N, nlat, nd, dt, seed=11, coord="even_sigma"):
     atmo = odyn.AtmoData("hs", nlon, nlat, nd, True, True, False, False, M.sin_theta, A, OMEGA)
     I = FilteredLeapfrog(0.04, 4, 1.15741e-4, M.laplacian_eig, 0.5, dt, True, 0, 86400 * 200)
     si = P.SemiImplicitSolver(vc, atmo, I, 1e5, np.full(nd, 300.0), M.wave_numbers)
-    d = odyn.DynData("hs", N, N + 1, nlon, nlat, nd)
+    d = odyn.DynData("hs", N, N + 1, nlon, nlat, nd, *tracers)
     noise = 0.1 * np.random.default_rng(seed).standard_normal((nd, nlat, nlon))
     P.spectral_initialize_fields(M, atmo, vc, 1.0e5, 300.0, d.grid_geopots, d, grid_t_noise=noise)
     # device
@@ -38,7 +38,7 @@ def build(jgcm, N, nlat, nd, dt, seed=11, coord="even_sigma"):
     datmo = jgcm.Atmo_Data("hs", nlon, nlat, nd, True, True, False, False, dev.sinθ, A, OMEGA)
     dI = jgcm.Filtered_Leapfrog(0.04, 4, 1.15741e-4, dev.laplacian_eig, 0.5, dt, True, 0, 86400 * 200)
     dsi = jgcm.Semi_Implicit_Solver(dvc, datmo, dI, 1e5, np.full(nd, 300.0), dev.wave_numbers)
-    dd = jgcm.Dyn_Data("hs", N, N + 1, nlon, nlat, nd, mesh=dev)
+    dd = jgcm.Dyn_Data("hs", N, N + 1, nlon, nlat, nd, *tracers, mesh=dev)
     jgcm.Get_Topography(dd.grid_geopots)
     jgcm.Spectral_Initialize_Fields(dev, datmo, dvc, 1.0e5, 300.0, dd.grid_geopots, dd,
                                     grid_t_noise=torch.from_numpy(noise).cuda())
@@ -99,6 +99,41 @@ def test_hs_steps_match_oracle(jgcm, N, nlat, nd, dt, steps, coord):
     np.testing.assert_allclose(dsi.wave_matrix, si.wave_matrix, rtol=1e-11, atol=1e-13)
     sc = dsi.scalars()
     assert np.isfinite(sc).all()
+
+
+def test_tracers_cycle_with_the_step(jgcm):
+    """Dyn_Data's tracers (Dyn_Data.jl:181-184, 208-211) are cycled by the
+    step's Time_Advance! (:341-344, :360-363) like the oracle's, and carrying
+    them changes nothing else (bitwise against a run without tracers)."""
+    N, nlat, nd, dt = 21, 32, 10, 600
+    ref, dev = build(jgcm, N, nlat, nd, dt, tracers=(2, 1))
+    _, plain = build(jgcm, N, nlat, nd, dt)
+    (M, vc, atmo, si, d), (dm, dvc, datmo, dsi, dd) = ref, dev
+    rng = np.random.default_rng(21)
+    for s in ("n", "c"):
+        g = rng.standard_normal(d.grid_tracers_n.shape)
+        sp = rng.standard_normal(d.spe_tracers_n.shape) + 1j * rng.standard_normal(d.spe_tracers_n.shape)
+        getattr(d, f"grid_tracers_{s}")[...] = g
+        getattr(d, f"spe_tracers_{s}")[...] = sp
+        getattr(dd, f"grid_tracers_{s}").copy_(torch.from_numpy(g))
+        getattr(dd, f"spe_tracers_{s}").copy_(torch.from_numpy(sp))
+    assert tuple(dd.grid_tracers_c.shape) == (2, nd, nlat, 2 * nlat)
+    assert tuple(dd.spe_tracers_c.shape) == (1, nd, N + 2, N + 1)
+    for i in range(2):
+        P.atmosphere_update(M, atmo, vc, si, d, HS)
+        jgcm.Atmosphere_Update(dm, datmo, dvc, dsi, dd, HS)
+        jgcm.Atmosphere_Update(plain[0], plain[2], plain[1], plain[3], plain[4], HS)
+        if i == 0:
+            si.update_init_step()
+            jgcm.Update_Init_Step(dsi)
+            jgcm.Update_Init_Step(plain[3])
+    torch.cuda.synchronize()
+    for f in ("grid_tracers", "spe_tracers"):
+        for s in ("n", "c", "p"):
+            np.testing.assert_array_equal(getattr(dd, f"{f}_{s}").cpu().numpy(), getattr(d, f"{f}_{s}"))
+    for f in FIELDS:
+        assert torch.equal(getattr(dd, f), getattr(plain[4], f)), f
+    compare(d, dd, 1e-10)
 
 
 def test_vertical_coordinate_options(jgcm):
