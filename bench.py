@@ -155,14 +155,15 @@ def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):
 def ensemble_bench(jgcm, torch, steps=100):
     """Mixed shallow-water ensemble: 64 x T21 (64x32), 128 x T42 (128x64) and
     64 x T85 (256x128) instances, seeded n^-3/2 vorticity spectra, FP32
-    Legendre with FP64 state, one step of all 256 captured into one HIP graph.
+    Legendre with FP64 state, one step of all 256 captured into one HIP graph
+    (the truncations in sequence on one stream).
     Reports instance-steps/s and the p50/p99 latency of one ensemble step,
     beside the cost of looping the plain FP64 path one instance at a time."""
     from jgcm_amd import ensemble as E
     from jgcm_amd import autotune
     groups = [(21, 32, 64, 1800), (42, 64, 128, 1200), (85, 128, 64, 600)]
     cache = autotune.Tuning_Cache(tuning_path())
-    ens = E.SW_Ensemble(groups, seed=11, precision="fp32", tuning=cache)
+    ens = E.SW_Ensemble(groups, seed=11, precision="fp32", tuning=cache, concurrent=False)
     ens.step()
     ens.capture()
     for _ in range(10):
@@ -195,26 +196,9 @@ def ensemble_bench(jgcm, torch, steps=100):
         e1.record(st)
         one.mesh.synchronize()
         loop_ms += count * e0.elapsed_time(e1) / 20
-    # the same step with the truncations in sequence on one stream
-    seq = E.SW_Ensemble(groups, seed=11, precision="fp32", tuning=cache, concurrent=False)
-    seq.step()
-    seq.capture()
-    for _ in range(10):
-        seq.launch()
-    seq.owner.synchronize()
-    sst = torch.cuda.ExternalStream(seq.owner.stream())
-    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s0.record(sst)
-    for _ in range(steps):
-        seq.launch()
-    s1.record(sst)
-    seq.owner.synchronize()
-    seq_ms = s0.elapsed_time(s1) / steps
-    seq.graph.close()
     n = ens.instances
     return {"instances": n, "groups": "64xT21, 128xT42, 64xT85", "precision": "fp32 Legendre, fp64 state",
-            "truncations": "parallel branches of one graph",
-            "sequential_ms_per_ensemble_step": seq_ms,
+            "truncations": "in sequence on one stream, one graph",
             "fft_shape_fold_unfold": ens.tuning,
             "ms_per_ensemble_step": total, "instance_steps_per_s": n / (total * 1e-3),
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

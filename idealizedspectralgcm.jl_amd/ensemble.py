@@ -108,15 +108,19 @@ class SW_Group:
 class SW_Ensemble:
     """Mixed-truncation ensemble: groups = [(N, nθ, count, dt), ...]."""
 
-    def __init__(self, groups, seed=0, precision="fp32", tuning=None, concurrent=True):
+    def __init__(self, groups, seed=0, precision="fp32", tuning=None, concurrent=False):
         """`tuning`: None (built-in FFT shapes) or an autotune.Tuning_Cache; each
         truncation's FFT shape (for its instance count) is then read from the
         cache, or tuned and stored when missing, before any step is captured.
-        `concurrent`: each truncation keeps its own stream and the captured step
-        runs the truncations as parallel branches of one graph (forked from and
-        joined back to the first group's stream with jgcm_mesh_wait); otherwise
-        all truncations share the first group's stream and run one after
-        another."""
+        The truncations share the first group's stream and run one after
+        another in the captured step.  `concurrent=True` (each truncation on
+        its own stream, parallel branches of one graph forked and joined with
+        jgcm_mesh_wait) is switched off: one replay of such a graph ended in a
+        segmentation fault inside hipGraphLaunch whose cause has not been
+        found, so it raises."""
+        if concurrent:
+            raise RuntimeError("SW_Ensemble(concurrent=True) is switched off: a replay of the "
+                               "parallel-branch graph crashed in hipGraphLaunch (cause not found)")
         from .autotune import Apply_Tuning
         self.groups = []
         self.tuning = {}

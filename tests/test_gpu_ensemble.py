@@ -8,6 +8,12 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
+def test_concurrent_ensemble_is_switched_off(jgcm):
+    from jgcm_amd.ensemble import SW_Ensemble
+    with pytest.raises(RuntimeError, match="switched off"):
+        SW_Ensemble([(21, 32, 1, 600)], concurrent=True)
+
+
 def test_batched_instances_equal_single_runs(jgcm):
     from jgcm_amd.ensemble import SW_Group, random_vorticity
     N, nlat, count, dt = 21, 32, 4, 600
@@ -31,10 +37,14 @@ def test_batched_instances_equal_single_runs(jgcm):
     assert float(g.d.grid_u_c.abs().max()) > 0.1
 
 
-@pytest.mark.parametrize("concurrent", [True, False])
+@pytest.mark.parametrize("concurrent", [
+    pytest.param(True, marks=pytest.mark.skip(
+        reason="parallel-branch ensemble graph switched off: one replay segfaulted in "
+               "hipGraphLaunch, cause not found")),
+    False])
 def test_mixed_ensemble_graph_matches_eager(jgcm, concurrent):
-    """Replays of the captured step (truncations as parallel graph branches, or
-    in sequence on one stream) equal eager steps bit for bit."""
+    """Replays of the captured step (truncations in sequence on one stream)
+    equal eager steps bit for bit."""
     from jgcm_amd.ensemble import SW_Ensemble
     groups = [(21, 32, 3, 600), (42, 64, 2, 600), (85, 128, 2, 300)]
     A = SW_Ensemble(groups, seed=1, precision="fp32", concurrent=concurrent)
