@@ -67,6 +67,10 @@ SIGNATURES = {
     "jgcm_trans_grid_to_fourier": [P, P, P, I],
     "jgcm_trans_grid_to_spherical_adjoint": [P, P, P, I],
     "jgcm_trans_spherical_to_grid_adjoint": [P, P, P, I],
+    "jgcm_trans_spherical_to_grid_adjoint_batch": [P, I, ctypes.POINTER(P), ctypes.POINTER(P),
+                                                   ctypes.POINTER(I), ctypes.POINTER(I), I],
+    "jgcm_trans_grid_to_spherical_adjoint_batch": [P, I, ctypes.POINTER(P), ctypes.POINTER(P),
+                                                   ctypes.POINTER(I)],
     "jgcm_vor_div_from_grid_uv_adjoint": [P, P, P, P, P, I],
     "jgcm_ucos_vcos_adjoint": [P, P, P, P, P, I],
     "jgcm_uv_grid_from_vor_div_adjoint": [P, P, P, P, P, I],

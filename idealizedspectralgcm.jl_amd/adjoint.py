@@ -38,6 +38,39 @@ def trans_spherical_to_grid_adjoint(mesh, pfield, snm):
     call("jgcm_trans_spherical_to_grid_adjoint", mesh.handle, _ptr(pfield), _ptr(snm), n)
 
 
+def trans_grid_to_spherical_adjoint_batch(mesh, snms, pfields):
+    """trans_grid_to_spherical_adjoint of up to 8 arrays in two launches."""
+    k = len(snms)
+    assert len(pfields) == k and 1 <= k <= 8
+    nlev = []
+    for s, g in zip(snms, pfields):
+        n = mesh._nlev(s, "spec", True)
+        assert mesh._nlev(g, "grid", True) == n
+        nlev.append(n)
+    P = ctypes.c_void_p
+    call("jgcm_trans_grid_to_spherical_adjoint_batch", mesh.handle, k,
+         (P * k)(*[s.data_ptr() for s in snms]), (P * k)(*[g.data_ptr() for g in pfields]),
+         (ctypes.c_int * k)(*nlev))
+
+
+def trans_spherical_to_grid_adjoint_batch(mesh, pfields, snms, divcos=None, accumulate=False):
+    """trans_spherical_to_grid_adjoint of up to 8 arrays in two launches;
+    ``divcos[k]``: array k is divided by cos θ first; ``accumulate``: the
+    results are added to ``snms`` (which must not overlap)."""
+    k = len(pfields)
+    assert len(snms) == k and 1 <= k <= 8
+    nlev = []
+    for g, s in zip(pfields, snms):
+        n = mesh._nlev(g, "grid", True)
+        assert mesh._nlev(s, "spec", True) == n
+        nlev.append(n)
+    P = ctypes.c_void_p
+    flags = (ctypes.c_int * k)(*[1 if (divcos and divcos[i]) else 0 for i in range(k)])
+    call("jgcm_trans_spherical_to_grid_adjoint_batch", mesh.handle, k,
+         (P * k)(*[g.data_ptr() for g in pfields]), (P * k)(*[s.data_ptr() for s in snms]),
+         (ctypes.c_int * k)(*nlev), flags, 1 if accumulate else 0)
+
+
 class GridToSpherical(torch.autograd.Function):
     """y = Trans_Grid_To_Spherical!(mesh, x) as a differentiable function."""
 
@@ -439,6 +472,8 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         self._stage = None
         self._bars_fixed = None
         self._pool = {}
+        self._slots = None
+        self._snap_count = 0
         self._scal_bar = torch.zeros(4, dtype=torch.float64, device=mesh.device)
         self._param_b = torch.zeros((4, mesh.nθ, mesh.nλ), dtype=torch.float64, device=mesh.device)
         self.parameter_gradients = {}
@@ -477,14 +512,40 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         self._last = self._solver(k)
         Atmosphere_Update(self.mesh, self.atmo, self.vc, self._last, self._d, self.params)
 
+    # Snapshots of the recomputed steps go to preallocated buffers, one set per
+    # step of a checkpoint segment (a segment's snapshots are consumed by its
+    # reverse steps, in stream order, before the next segment overwrites them),
+    # each filled by one batched copy.
+    _AFTER = ("grid_u_c", "grid_v_c", "grid_t_c", "grid_ps_c")
+
+    def _slot(self):
+        if self._slots is None or len(self._slots) != self.every:
+            self._slots = [None] * self.every
+        i = self._snap_count % self.every
+        if self._slots[i] is None:
+            d = self._d
+            self._slots[i] = (tuple(torch.empty_like(getattr(d, n)) for n in self.STATE),
+                              torch.zeros(7, dtype=torch.float64, device=self.mesh.device),
+                              tuple(torch.empty_like(getattr(d, n)) for n in self._AFTER))
+        return self._slots[i]
+
+    def _copy_into(self, dsts, srcs):
+        P, n = ctypes.c_void_p, len(srcs)
+        call("jgcm_copy_batch", self.mesh.handle, n, (P * n)(*[t.data_ptr() for t in dsts]),
+             (P * n)(*[t.data_ptr() for t in srcs]),
+             (ctypes.c_size_t * n)(*[t.numel() * t.element_size() for t in srcs]))
+
     def _snapshot(self):
-        return self._state()
+        before = self._slot()[0]
+        self._copy_into(before, [getattr(self._d, n) for n in self.STATE])
+        return before
 
     def _snapshot_after(self):
-        d = self._d
-        sc = torch.zeros(7, dtype=torch.float64, device=self.mesh.device)
+        _, sc, after = self._slot()
         call("jgcm_hs_scalars_device", self._last.handle, _ptr(sc), 7)
-        return sc, tuple(t.clone() for t in (d.grid_u_c, d.grid_v_c, d.grid_t_c, d.grid_ps_c))
+        self._copy_into(after, [getattr(self._d, n) for n in self._AFTER])
+        self._snap_count += 1
+        return sc, after
 
     def forward(self, state, nsteps):
         self._init_lnps = state[7].clone()
@@ -617,8 +678,8 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         self._graph_runs[key] = runs + 1
 
     def _step_adjoint_eager(self, k, init, snap, bars):
-        from .mesh import (Divide_By_Cos, Trans_Spherical_To_Grid, Trans_Spherical_To_Grid_Batch,
-                           UV_Grid_From_Vor_Div, BATCH_DIVCOS, INV_GRAD_LAMBDA, INV_GRAD_THETA)
+        from .mesh import (Trans_Spherical_To_Grid_Batch, BATCH_DIVCOS, INV_GRAD_LAMBDA,
+                           INV_GRAD_THETA, INV_UCOS, INV_VCOS)
         from .primitive import _hs
         from .time_integrator import Get_Δt
         M, nd, atmo = self.mesh, self._nd, self.atmo
@@ -651,15 +712,8 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
             call("jgcm_axpby", h, dst.numel() * dst.element_size() // 8, 1.0, _ptr(dst), 1.0,
                  _ptr(src), _ptr(dst))
 
-        def t2g_T(grid, nlev):       # transpose of the inverse transform
-            out = S(nlev)
-            trans_spherical_to_grid_adjoint(M, grid, out)
-            return out
-
-        def f_T(spec, nlev):         # transpose of the forward transform
-            out = G(nlev)
-            trans_grid_to_spherical_adjoint(M, spec, out)
-            return out
+        def t2g_T_add(grids, targets):   # targets[k] += transpose of the inverse (grids[k])
+            trans_spherical_to_grid_adjoint_batch(M, grids, targets, accumulate=True)
 
         # 1. mass and energy corrections (Spectral_Dynamics.jl:30-68); the scalar
         #    cotangents stay on the device (no wait for the stream)
@@ -669,8 +723,7 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
             ub, vb, tb, lb = G(nd), G(nd), G(nd), G(1)
             call("jgcm_hs_corrections_adjoint", si.handle, *map(_ptr, (
                 sc, Btc, Blc, u_n, v_n, t_nc, ps_nc, scal_bar, ub, vb, tb, lb)))
-            add(Btc, t2g_T(tb, nd))
-            add(Blc, t2g_T(lb, 1))
+            t2g_T_add([tb, lb], [Btc, Blc])
             sv, sd = S(nd), S(nd)
             uv_grid_from_vor_div_adjoint(M, ub, vb, sv, sd)
             add(Bvc, sv)
@@ -694,21 +747,20 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         add(Btc, out[base + 2 * nd + 2:base + 3 * nd + 2])
         add(Btp, out[base + 3 * nd + 2:base + 4 * nd + 2])
         # 4. forward transforms of δln ps, δT, Φ+KE and (δu, δv) -> (δζ, δD)
-        dl_g = f_T(o_dl.contiguous(), 1)
-        dt_g = f_T(o_dt.contiguous(), nd)
-        en_g = f_T(o_en.contiguous(), nd)
+        dl_g, dt_g, en_g = G(1), G(nd), G(nd)
+        trans_grid_to_spherical_adjoint_batch(M, [o_dl, o_dt, o_en], [dl_g, dt_g, en_g])
         du_g, dv_g = G(nd), G(nd)
         vor_div_from_grid_uv_adjoint(M, Gv, o_ddiv.contiguous(), du_g, dv_g)
         # 5. grid columns at the step's state
+        #    (the forward step's two inverse batches: same kernels, same values)
         vor_g, div_g, u_g, v_g, t_g, ln_g, ps_c = G(nd), G(nd), G(nd), G(nd), G(nd), G(1), G(1)
-        Trans_Spherical_To_Grid(M, vor_c, vor_g, batch=True)
-        Trans_Spherical_To_Grid(M, div_c, div_g, batch=True)
-        UV_Grid_From_Vor_Div(M, vor_c, div_c, u_g, v_g, batch=True)
-        Trans_Spherical_To_Grid(M, t_c, t_g, batch=True)
-        Trans_Spherical_To_Grid(M, lnps_c, ln_g, batch=True)
+        D = BATCH_DIVCOS
+        Trans_Spherical_To_Grid_Batch(
+            M, [vor_c, div_c, vor_c, vor_c, t_c, lnps_c], [vor_g, div_g, u_g, v_g, t_g, ln_g],
+            [0, 0, D | INV_UCOS, D | INV_VCOS, 0, 0],
+            snms_b=[None, None, div_c, div_c, None, None])
         call("jgcm_map", h, 0, ln_g.numel(), _ptr(ln_g), 0.0, 0.0, _ptr(ps_c))
         gl, gt, tl, tt = G(1), G(1), G(nd), G(nd)
-        D = BATCH_DIVCOS
         Trans_Spherical_To_Grid_Batch(M, [lnps_c, lnps_c, t_c, t_c], [gl, gt, tl, tt],
                                       [D | INV_GRAD_LAMBDA, D | INV_GRAD_THETA,
                                        D | INV_GRAD_LAMBDA, D | INV_GRAD_THETA])
@@ -728,20 +780,16 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         for dst, src in ((Bup, up_b), (Bvpg, vp_b), (Btpg, tp_b), (Bpsp, psp_b)):
             call("jgcm_copy", h, _ptr(dst), _ptr(src), src.numel() * src.element_size())
         # 7. grid c-fields -> spectral c-fields
-        add(Bvc, t2g_T(vor_b, nd))
-        add(Bdc, t2g_T(div_b, nd))
-        add(Btc, t2g_T(t_b, nd))
+        lg = G(1)
+        call("jgcm_xmy", h, lg.numel(), 1.0, _ptr(ps_c), _ptr(psc_b), _ptr(lg), 0)
+        t2g_T_add([vor_b, div_b, t_b, lg], [Bvc, Bdc, Btc, Blc])
         sv, sd = S(nd), S(nd)
         uv_grid_from_vor_div_adjoint(M, u_b, v_b, sv, sd)
         add(Bvc, sv)
         add(Bdc, sd)
-        lg = G(1)
-        call("jgcm_xmy", h, lg.numel(), 1.0, _ptr(ps_c), _ptr(psc_b), _ptr(lg), 0)
-        add(Blc, t2g_T(lg, 1))
-        for gb_l, gb_t, target, nlev in ((gl_b, gt_b, Blc, 1), (tl_b, tt_b, Btc, nd)):
-            Divide_By_Cos(M, gb_l)
-            Divide_By_Cos(M, gb_t)
-            # both temporaries stay referenced until the call: a freed one could
-            # be handed to the other by the caching allocator
-            s_l, s_t = t2g_T(gb_l, nlev), t2g_T(gb_t, nlev)
-            call("jgcm_gradient_cos_adjoint", h, _ptr(s_l), _ptr(s_t), _ptr(target), 1, nlev)
+        # gradients of ln ps and T (the inverse batch divided them by cos θ)
+        s_ll, s_lt, s_tl, s_tt = S(1), S(1), S(nd), S(nd)
+        trans_spherical_to_grid_adjoint_batch(M, [gl_b, gt_b, tl_b, tt_b],
+                                              [s_ll, s_lt, s_tl, s_tt], divcos=[1, 1, 1, 1])
+        call("jgcm_gradient_cos_adjoint", h, _ptr(s_ll), _ptr(s_lt), _ptr(Blc), 1, 1)
+        call("jgcm_gradient_cos_adjoint", h, _ptr(s_tl), _ptr(s_tt), _ptr(Btc), 1, nd)

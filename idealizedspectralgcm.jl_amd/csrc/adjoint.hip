@@ -17,7 +17,7 @@
 namespace jgcm {
 
 int ifft_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const double* divcos, int B);
-int scale_eo(const jgcm_mesh* M, double2* eo, int B, double s0, double s1);
+int ifft_unfold_flag(const jgcm_mesh* M, const double2* eo, double* grid, int flag, int B);
 int legendre_alpha_transpose(const jgcm_mesh* M, const double2* h, double2* eo, int B, int table,
                              int op, int swap, int accumulate);
 
@@ -156,18 +156,10 @@ static int vor_div_adjoint(jgcm_mesh* M, const double2* vb, const double2* db, d
     if (int rc = legendre_alpha_transpose(M, db, eu, nlev, TAB_ALPA, OP_NEG_I, 0, 1)) return rc;
     if (int rc = legendre_alpha_transpose(M, db, ev, nlev, TAB_ALPB, OP_NONE, 1, 1)) return rc;
   }
-  double L = M->nlon;
-  if (int rc = scale_eo(M, eu, nlev, 1.0 / L, 1.0 / (2.0 * L))) return rc;
-  if (int rc = scale_eo(M, ev, nlev, 1.0 / L, 1.0 / (2.0 * L))) return rc;
-  if (int rc = ifft_unfold(M, eu, u_bar, nullptr, nlev)) return rc;
-  if (int rc = ifft_unfold(M, ev, v_bar, nullptr, nlev)) return rc;
-  size_t n = (size_t)nlev * M->grid_elems();
-  hipLaunchKernelGGL(scale_by_cos_kernel, dim3(nblk(n)), dim3(256), 0, M->stream, u_bar, u_bar,
-                     M->d_cos, M->nlat, M->nlon, n, 0);
-  hipLaunchKernelGGL(scale_by_cos_kernel, dim3(nblk(n)), dim3(256), 0, M->stream, v_bar, v_bar,
-                     M->d_cos, M->nlat, M->nlon, n, 0);
-  JGCM_LAUNCH_CHECK();
-  return JGCM_OK;
+  // the FFT stage applies the transposed Hermitian weights (flag 2) and the
+  // final × cos θ (flags 1 | 4)
+  if (int rc = ifft_unfold_flag(M, eu, u_bar, 1 | 2 | 4, nlev)) return rc;
+  return ifft_unfold_flag(M, ev, v_bar, 1 | 2 | 4, nlev);
 }
 
 static void ucos_vcos_adjoint(jgcm_mesh* M, const double* su, const double* sv, double* vor_bar,

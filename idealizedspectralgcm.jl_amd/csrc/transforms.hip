@@ -120,7 +120,9 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
   const int j0 = blockIdx.x * ppw;
   const int seg = seg_find(in, b);
   const double* __restrict__ grid = in.ptr[seg] + (size_t)(b - in.start[seg]) * nlat * L;
+  // flag bit 0: rows scaled by cos θ; with bit 1 divided by it instead
   const double* __restrict__ cosmul = (in.flag[seg] & 1) ? cosv : nullptr;
+  const bool divcos = (in.flag[seg] & 2) != 0;
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const double scale = 1.0 / L;
@@ -171,7 +173,15 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         double a = v[r].x, c = v[r].y;
-        if (cosmul) { a = a * s; c = c * s; }
+        if (cosmul) {
+          if (divcos) {
+            a = a / s;
+            c = c / s;
+          } else {
+            a = a * s;
+            c = c * s;
+          }
+        }
         buf[g][t + r * T] = make_double2(a, c);
       }
     }
@@ -201,7 +211,10 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
 }
 
 // Inverse: E/O block -> two grid rows per FFT.  Segments with flag & 1 have
-// their output rows divided by cos θ (Divide_By_Cos!, Spectral_Spherical_Mesh.jl:333-338).
+// their output rows divided by cos θ (Divide_By_Cos!, Spectral_Spherical_Mesh.jl:333-338);
+// with flag & 2 the E/O values are first weighted 1/L (m = 0) and 1/(2L)
+// (m > 0), the transposed Hermitian weights of the forward FFT; flag & 4
+// (with & 1) multiplies the rows by cos θ instead of dividing.
 template <int L>
 __global__ __launch_bounds__(256) void ifft_unfold_kernel(
     const double2* __restrict__ eo,    // [B][nm][2][half]
@@ -220,6 +233,9 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
   const int seg = seg_find(out, b);
   double* __restrict__ grid = out.ptr[seg] + (size_t)(b - out.start[seg]) * nlat * L;
   const double* __restrict__ divcos = (out.flag[seg] & 1) ? cosv : nullptr;
+  const bool adjw = (out.flag[seg] & 2) != 0;
+  const bool mulcos = (out.flag[seg] & 4) != 0;
+  const double w0 = 1.0 / L, w1 = 1.0 / (2.0 * L);
   // twiddle loads in flight together with the E/O gather below
   constexpr int TWI = (L + 255) / 256;
   double2 twr[TWI];
@@ -235,7 +251,13 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
         int jp = j0 + jj;
         return (jp < half) ? eo[((size_t)b * nm * 2 + mp) * half + jp] : make_double2(0.0, 0.0);
       },
-      [&](int idx, double2 v) { stage[(idx % ppw) * SP + idx / ppw] = v; });
+      [&](int idx, double2 v) {
+        if (adjw) {
+          const double s = (idx / ppw) < 2 ? w0 : w1;  // m = (idx / ppw) / 2
+          v = make_double2(v.x * s, v.y * s);
+        }
+        stage[(idx % ppw) * SP + idx / ppw] = v;
+      });
 #pragma unroll
   for (int q = 0; q < TWI; ++q)
     if (q * 256 + (int)threadIdx.x < L) tw[q * 256 + threadIdx.x] = twr[q];
@@ -274,7 +296,10 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
       for (int r = 0; r < 4; ++r) {
         int i = t + r * T;
         double2 z = buf[g][i];
-        if (divcos) {
+        if (divcos && mulcos) {
+          ra[i] = z.x * c;
+          rc[i] = z.y * c;
+        } else if (divcos) {
           ra[i] = z.x / c;
           rc[i] = z.y / c;
         } else {
@@ -349,12 +374,13 @@ __global__ __launch_bounds__(256) void dft_fold_kernel(SegMap in, double2* __res
   const int seg = seg_find(in, b);
   const double* __restrict__ grid = in.ptr[seg] + (size_t)(b - in.start[seg]) * nlat * L;
   const double s = (in.flag[seg] & 1) ? cosv[jp] : 1.0;  // cos θ is symmetric about the equator
+  const int cmode = (in.flag[seg] & 1) ? ((in.flag[seg] & 2) ? 2 : 1) : 0;  // x cos, / cos
   const double* a = grid + (size_t)jp * L;
   const double* c = grid + (size_t)(nlat - 1 - jp) * L;
   for (int i = threadIdx.x; i < L; i += blockDim.x) {
     tw[i] = twg[i];
-    ra[i] = (in.flag[seg] & 1) ? a[i] * s : a[i];
-    rc[i] = (in.flag[seg] & 1) ? c[i] * s : c[i];
+    ra[i] = cmode == 1 ? a[i] * s : (cmode == 2 ? a[i] / s : a[i]);
+    rc[i] = cmode == 1 ? c[i] * s : (cmode == 2 ? c[i] / s : c[i]);
   }
   __syncthreads();
   const double scale = 1.0 / L;
@@ -389,10 +415,16 @@ __global__ __launch_bounds__(256) void dft_unfold_kernel(const double2* __restri
   const int seg = seg_find(out, b);
   double* __restrict__ grid = out.ptr[seg] + (size_t)(b - out.start[seg]) * nlat * L;
   const bool divcos = out.flag[seg] & 1;
+  const bool adjw = (out.flag[seg] & 2) != 0;
   for (int i = threadIdx.x; i < L; i += blockDim.x) tw[i] = twg[i];
   for (int m = threadIdx.x; m < nm; m += blockDim.x) {
-    const double2 E = eo[(((size_t)b * nm + m) * 2 + 0) * half + jp];
-    const double2 O = eo[(((size_t)b * nm + m) * 2 + 1) * half + jp];
+    double2 E = eo[(((size_t)b * nm + m) * 2 + 0) * half + jp];
+    double2 O = eo[(((size_t)b * nm + m) * 2 + 1) * half + jp];
+    if (adjw) {  // transposed Hermitian weights, as in ifft_unfold_kernel
+      const double s = m == 0 ? 1.0 / L : 1.0 / (2.0 * L);
+      E = make_double2(E.x * s, E.y * s);
+      O = make_double2(O.x * s, O.y * s);
+    }
     ga[m] = cadd(E, O);
     gc[m] = csub(E, O);
   }
@@ -409,7 +441,10 @@ __global__ __launch_bounds__(256) void dft_unfold_kernel(const double2* __restri
       k += l;
       if (k >= L) k -= L;
     }
-    if (divcos) {
+    if (divcos && (out.flag[seg] & 4)) {
+      xa = xa * c;
+      xc = xc * c;
+    } else if (divcos) {
       xa = xa / c;
       xc = xc / c;
     }
@@ -523,6 +558,10 @@ struct FwdJob {
   double signB;
   int nf;               // fields in this job
   SegMap out;           // spectral fields [ns+1][nm] (complex) each
+  // outputs scaled by scale0 (m = 0) or scale1 (m > 0), then stored or, with
+  // accumulate, added to the output array
+  double scale0 = 1.0, scale1 = 1.0;
+  int accumulate = 0;
 };
 
 constexpr int MAXJOB = 4;
@@ -851,13 +890,19 @@ __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int 
   const int fl = col >> 1, comp = col & 1;
   if (fl < nf) {
     double* base = seg_field(job.out, f0 + fl, spec_d) + (size_t)m * 2 + comp;
+    const double sc = m == 0 ? job.scale0 : job.scale1;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       if (t >= ntile) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         int kk = kk0 + 16 * t + tile_row<PREC>(lk, r);
-        if (kk < nr) base[(size_t)(nfirst + 2 * kk) * nm * 2] = acc.v[t][r];
+        if (kk < nr) {
+          double* o = base + (size_t)(nfirst + 2 * kk) * nm * 2;
+          double v = (double)acc.v[t][r] * sc;
+          if (job.accumulate) v = *o + v;
+          *o = v;
+        }
       }
     }
   }
@@ -1122,6 +1167,11 @@ int ifft_unfold(const jgcm_mesh* M, const double2* eo, double* grid, const doubl
   return ifft_unfold_map(M, eo, one_seg(grid, B, divcos ? 1 : 0));
 }
 
+// ifft_unfold with the segment flag given (bits of ifft_unfold_kernel)
+int ifft_unfold_flag(const jgcm_mesh* M, const double2* eo, double* grid, int flag, int B) {
+  return ifft_unfold_map(M, eo, one_seg(grid, B, flag));
+}
+
 static FwdTables tables_for(const jgcm_mesh* M, int which) {
   const RowTables& r = M->rt;
   switch (which) {
@@ -1258,30 +1308,16 @@ int legendre_alpha(const jgcm_mesh* M, const double2* eoA, const double2* eoB, d
   return launch_legendre_fwd(M, jobs, M->ns, 0);
 }
 
-// E/O block scaled per wavenumber: m = 0 by s0, m >= 1 by s1
-__global__ void scale_eo_kernel(double2* eo, size_t total, int nm, int half, double s0, double s1) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
-       i += (size_t)gridDim.x * blockDim.x) {
-    int m = (int)((i / (2 * (size_t)half)) % nm);
-    double s = m == 0 ? s0 : s1;
-    eo[i] = make_double2(eo[i].x * s, eo[i].y * s);
-  }
-}
-
-int scale_eo(const jgcm_mesh* M, double2* eo, int B, double s0, double s1) {
-  size_t total = (size_t)B * M->nm * 2 * M->half;
-  int blocks = (int)std::min<size_t>((total + 255) / 256, 8192);
-  hipLaunchKernelGGL(scale_eo_kernel, dim3(blocks), dim3(256), 0, M->stream, eo, total, M->nm,
-                     M->half, s0, s1);
-  JGCM_LAUNCH_CHECK();
-  return JGCM_OK;
-}
-
-// transpose of the inverse Legendre stage: s[n] = sum_j' Pinv[n][j'] eo^(n-m)[j'], n = m..ns
-int legendre_inv_adjoint(const jgcm_mesh* M, const double2* eo, double2* snm, int B) {
+// transpose of the inverse Legendre stage: s[n] = sum_j' Pinv[n][j'] eo^(n-m)[j'], n = m..ns,
+// times scale0 (m = 0) or scale1 (m > 0), stored or added to the output
+int legendre_inv_adjoint(const jgcm_mesh* M, const double2* eo, const SegMap& out, int B,
+                         double scale0, double scale1, int accumulate) {
   FwdJobs jobs{};
   jobs.njobs = 1;
-  jobs.job[0] = FwdJob{eo, tables_for(M, 3), 0, nullptr, FwdTables{}, 0.0, B, one_seg(snm, B, 0)};
+  jobs.job[0] = FwdJob{eo, tables_for(M, 3), 0, nullptr, FwdTables{}, 0.0, B, out};
+  jobs.job[0].scale0 = scale0;
+  jobs.job[0].scale1 = scale1;
+  jobs.job[0].accumulate = accumulate;
   return launch_legendre_fwd(M, jobs, M->ns, 0);
 }
 
@@ -1480,28 +1516,94 @@ int jgcm_trans_grid_to_spherical_batch(jgcm_mesh_t* mesh, int nseg, const double
 // Legendre (qwg/2 rows, degrees m..N), per-m scaling, conjugate FFT.
 int jgcm_trans_grid_to_spherical_adjoint(jgcm_mesh_t* mesh, const double* snm, double* pfield,
                                          int nlev) {
-  JGCM_MESH_PROLOGUE(mesh, nlev);
-  JGCM_REQUIRE(pfield && snm, "NULL field pointer");
+  return jgcm_trans_grid_to_spherical_adjoint_batch(mesh, 1, &snm, &pfield, &nlev);
+}
+
+// The same for up to MAXSEG arrays in one Legendre launch and one FFT launch
+// (the weights applied while the FFT stages its input).
+int jgcm_trans_grid_to_spherical_adjoint_batch(jgcm_mesh_t* mesh, int nseg,
+                                               const double* const* snm, double* const* pfield,
+                                               const int* nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nseg >= 1 && nseg <= MAXSEG, "nseg must be in 1..%d, got %d", MAXSEG, nseg);
+  JGCM_REQUIRE(snm && pfield && nlev, "NULL argument");
+  int total = 0;
+  for (int k = 0; k < nseg; ++k) {
+    JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
+    JGCM_REQUIRE(nlev[k] >= 1, "nlev[%d] must be >= 1, got %d", k, nlev[k]);
+    total += nlev[k];
+  }
+  if (int rc = ensure_scratch(mesh, total)) return rc;
+  SegMap in{}, out{};
+  in.nseg = out.nseg = nseg;
+  int B = 0;
+  for (int k = 0; k < nseg; ++k) {
+    in.start[k] = out.start[k] = B;
+    in.ptr[k] = const_cast<double*>(snm[k]);
+    out.ptr[k] = pfield[k];
+    out.flag[k] = 2;  // transposed Hermitian weights
+    B += nlev[k];
+  }
+  in.start[nseg] = out.start[nseg] = B;
   double2* eo = reinterpret_cast<double2*>(mesh->d_four_a);
-  if (int rc = legendre_inv_t(mesh, reinterpret_cast<const double2*>(snm), eo, nlev, 0, mesh->N)) return rc;
-  double L = mesh->nlon;
-  if (int rc = scale_eo(mesh, eo, nlev, 1.0 / L, 1.0 / (2.0 * L))) return rc;
-  return ifft_unfold(mesh, eo, pfield, nullptr, nlev);
+  if (int rc = legendre_inv_map(mesh, in, eo, 0, mesh->N)) return rc;
+  return ifft_unfold_map(mesh, eo, out);
 }
 
 // Adjoint of Trans_Spherical_To_Grid!: FFT with the transposed Hermitian
-// weights, then the transposed inverse Legendre over degrees m..N+1.  The
-// lower triangle (n < m) of the output is set to zero.
+// weights (L for m = 0, 2L for m > 0, applied to the contraction's output),
+// then the transposed inverse Legendre over degrees m..N+1.  The lower
+// triangle (n < m) of the output is set to zero.
 int jgcm_trans_spherical_to_grid_adjoint(jgcm_mesh_t* mesh, const double* pfield, double* snm,
                                          int nlev) {
-  JGCM_MESH_PROLOGUE(mesh, nlev);
-  JGCM_REQUIRE(pfield && snm, "NULL field pointer");
+  return jgcm_trans_spherical_to_grid_adjoint_batch(mesh, 1, &pfield, &snm, &nlev, nullptr, 0);
+}
+
+// The same for up to MAXSEG arrays in one FFT launch and one Legendre launch.
+// flags[k] & JGCM_BATCH_DIVCOS: array k is divided by cos θ on the way in
+// (the transpose of the inverse batch's ÷cos output).  accumulate: the
+// results are added to snm[k] (degrees m..N+1 only; the outputs must not
+// overlap); otherwise snm[k] is overwritten and its other entries zeroed.
+int jgcm_trans_spherical_to_grid_adjoint_batch(jgcm_mesh_t* mesh, int nseg,
+                                               const double* const* pfield, double* const* snm,
+                                               const int* nlev, const int* flags, int accumulate) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nseg >= 1 && nseg <= MAXSEG, "nseg must be in 1..%d, got %d", MAXSEG, nseg);
+  JGCM_REQUIRE(snm && pfield && nlev, "NULL argument");
+  const size_t spec_d = mesh->spec_elems() * 2;
+  int total = 0;
+  for (int k = 0; k < nseg; ++k) {
+    JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
+    JGCM_REQUIRE(nlev[k] >= 1, "nlev[%d] must be >= 1, got %d", k, nlev[k]);
+    int f = flags ? flags[k] : 0;
+    JGCM_REQUIRE(f == 0 || f == JGCM_BATCH_DIVCOS, "flags[%d] = %d is not valid", k, f);
+    if (accumulate) {
+      for (int q = 0; q < k; ++q) {
+        const double *a0 = snm[q], *a1 = snm[q] + nlev[q] * spec_d;
+        const double *b0 = snm[k], *b1 = snm[k] + nlev[k] * spec_d;
+        JGCM_REQUIRE(a1 <= b0 || b1 <= a0, "accumulated outputs %d and %d overlap", q, k);
+      }
+    }
+    total += nlev[k];
+  }
+  if (int rc = ensure_scratch(mesh, total)) return rc;
+  SegMap in{}, out{};
+  in.nseg = out.nseg = nseg;
+  int B = 0;
+  for (int k = 0; k < nseg; ++k) {
+    in.start[k] = out.start[k] = B;
+    in.ptr[k] = const_cast<double*>(pfield[k]);
+    in.flag[k] = (flags && flags[k]) ? 3 : 0;  // cos θ, divided
+    out.ptr[k] = snm[k];
+    B += nlev[k];
+  }
+  in.start[nseg] = out.start[nseg] = B;
   double2* eo = reinterpret_cast<double2*>(mesh->d_four_a);
-  if (int rc = fft_fold(mesh, pfield, eo, nullptr, nlev)) return rc;
-  double L = mesh->nlon;
-  if (int rc = scale_eo(mesh, eo, nlev, L, 2.0 * L)) return rc;
-  if (int rc = jgcm_fill(mesh, snm, 0.0, (size_t)nlev * mesh->spec_elems() * 2)) return rc;
-  return legendre_inv_adjoint(mesh, eo, reinterpret_cast<double2*>(snm), nlev);
+  if (int rc = fft_fold_map(mesh, in, eo)) return rc;
+  if (!accumulate)
+    for (int k = 0; k < nseg; ++k)
+      if (int rc = jgcm_fill(mesh, snm[k], 0.0, (size_t)nlev[k] * spec_d)) return rc;
+  return legendre_inv_adjoint(mesh, eo, out, B, (double)mesh->nlon, 2.0 * mesh->nlon, accumulate);
 }
 
 int jgcm_compute_alpha_operator(jgcm_mesh_t* mesh, const double* fourier_a, const double* fourier_b,

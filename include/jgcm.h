@@ -217,8 +217,22 @@ int jgcm_trans_grid_to_spherical_batch(jgcm_mesh_t* mesh, int nseg, const double
  * written, lower triangle set to zero).                                     */
 int jgcm_trans_grid_to_spherical_adjoint(jgcm_mesh_t* mesh, const double* snm, double* pfield,
                                          int nlev);
+/* grid_to_spherical_adjoint for up to 8 arrays in one Legendre and one FFT
+ * launch (no reference counterpart: the adjoint of the forward batch).     */
+int jgcm_trans_grid_to_spherical_adjoint_batch(jgcm_mesh_t* mesh, int nseg,
+                                               const double* const* snm, double* const* pfield,
+                                               const int* nlev);
 int jgcm_trans_spherical_to_grid_adjoint(jgcm_mesh_t* mesh, const double* pfield, double* snm,
                                          int nlev);
+/* spherical_to_grid_adjoint for up to 8 arrays in one FFT and one Legendre
+ * launch (no reference counterpart: the adjoint of the inverse batch).
+ * flags[k] (or NULL): JGCM_BATCH_DIVCOS divides array k by cos θ on the way
+ * in.  accumulate = 0: snm[k] overwritten (lower triangle zeroed);
+ * accumulate = 1: the result is added to snm[k] (degrees m..N+1), and the
+ * outputs must not overlap.                                                 */
+int jgcm_trans_spherical_to_grid_adjoint_batch(jgcm_mesh_t* mesh, int nseg,
+                                               const double* const* pfield, double* const* snm,
+                                               const int* nlev, const int* flags, int accumulate);
 
 /* Adjoints of the barotropic step's operators (no reference counterpart; the
  * same inner products).  Outputs may not alias inputs.

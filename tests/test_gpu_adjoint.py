@@ -52,6 +52,52 @@ def test_dot_product_identities(jgcm, N, nlat):
         assert not g[:, :m, m].any()
 
 
+@pytest.mark.parametrize("N,nlat", [(21, 32), (85, 128), (31, 48)])
+def test_batched_adjoints_equal_single_calls(jgcm, N, nlat):
+    """The batched adjoint transforms (one FFT and one Legendre launch for
+    several arrays, ÷cos θ on the way in, accumulation into the targets) give
+    the single calls' results bit for bit; 96 x 48 runs the direct-DFT path."""
+    from jgcm_amd import adjoint
+    nd = 3
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, nd, 6.371e6)
+    rng = np.random.default_rng(40 + N)
+
+    def cuda(a):
+        return torch.tensor(a, device="cuda")
+
+    g = [cuda(rng.standard_normal((n, nlat, 2 * nlat))) for n in (nd, 1, nd)]
+    divcos = [0, 1, 0]
+    singles = []
+    for x, dc in zip(g, divcos):
+        xin = x.clone()
+        if dc:
+            jgcm.Divide_By_Cos(mesh, xin)
+        out = mesh.spectral_zeros(x.shape[0])
+        adjoint.trans_spherical_to_grid_adjoint(mesh, xin, out)
+        singles.append(out)
+    outs = [mesh.spectral_zeros(x.shape[0]).fill_(1.0) for x in g]   # overwritten, rest zeroed
+    adjoint.trans_spherical_to_grid_adjoint_batch(mesh, g, outs, divcos=divcos)
+    tgt = [cuda(rspec(N, x.shape[0], rng)) for x in g]
+    acc = [t.clone() for t in tgt]
+    adjoint.trans_spherical_to_grid_adjoint_batch(mesh, g, acc, divcos=divcos, accumulate=True)
+    s = [cuda(rspec(N, n, rng)) for n in (nd, 1, nd)]
+    fs = []
+    for x in s:
+        o = mesh.grid_zeros(x.shape[0])
+        adjoint.trans_grid_to_spherical_adjoint(mesh, x, o)
+        fs.append(o)
+    fb = [mesh.grid_zeros(x.shape[0]) for x in s]
+    adjoint.trans_grid_to_spherical_adjoint_batch(mesh, s, fb)
+    mesh.synchronize()
+    for k in range(3):
+        assert torch.equal(outs[k], singles[k]), k
+        assert torch.equal(acc[k], tgt[k] + singles[k]), k
+        assert torch.equal(fb[k], fs[k]), k
+    with pytest.raises(AssertionError):     # accumulated outputs must not overlap
+        adjoint.trans_spherical_to_grid_adjoint_batch(mesh, [g[0], g[2]], [acc[0], acc[0]],
+                                                      accumulate=True)
+
+
 def test_autograd_matches_central_differences(jgcm):
     from jgcm_amd.adjoint import GridToSpherical, SphericalToGrid
     N, nlat, nlev = 21, 32, 2
