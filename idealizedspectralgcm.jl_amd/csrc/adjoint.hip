@@ -331,19 +331,15 @@ int jgcm_uv_grid_from_vor_div_adjoint(jgcm_mesh_t* mesh, const double* u_bar, co
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
   JGCM_REQUIRE(u_bar && v_bar && vor_bar, "NULL field pointer");
-  if (int rc = ensure_scratch(mesh, nlev)) return rc;  // before any scratch pointer is taken
-  size_t n = (size_t)nlev * mesh->grid_elems();
-  double* gu = mesh->d_grid_a;
-  double* gv = mesh->d_grid_b;
-  hipLaunchKernelGGL(scale_by_cos_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, u_bar, gu,
-                     mesh->d_cos, mesh->nlat, mesh->nlon, n, 1);
-  hipLaunchKernelGGL(scale_by_cos_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, v_bar, gv,
-                     mesh->d_cos, mesh->nlat, mesh->nlon, n, 1);
-  JGCM_LAUNCH_CHECK();
+  if (int rc = ensure_scratch(mesh, 2 * nlev)) return rc;  // before any scratch pointer is taken
+  // ÷ cos θ, then the transposed inverse transform of u and v in one batch
+  // (adjacent outputs: one zero fill)
   double* su = mesh->d_spec_a;
-  double* sv = mesh->d_spec_b;
-  if (int rc = jgcm_trans_spherical_to_grid_adjoint(mesh, gu, su, nlev)) return rc;
-  if (int rc = jgcm_trans_spherical_to_grid_adjoint(mesh, gv, sv, nlev)) return rc;
+  double* sv = su + (size_t)nlev * mesh->spec_elems() * 2;
+  const double* g[2] = {u_bar, v_bar};
+  double* s[2] = {su, sv};
+  const int lev[2] = {nlev, nlev}, fl[2] = {JGCM_BATCH_DIVCOS, JGCM_BATCH_DIVCOS};
+  if (int rc = jgcm_trans_spherical_to_grid_adjoint_batch(mesh, 2, g, s, lev, fl, 0)) return rc;
   ucos_vcos_adjoint(mesh, su, sv, vor_bar, div_bar, nullptr, nullptr, nlev);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;

@@ -1600,9 +1600,16 @@ int jgcm_trans_spherical_to_grid_adjoint_batch(jgcm_mesh_t* mesh, int nseg,
   in.start[nseg] = out.start[nseg] = B;
   double2* eo = reinterpret_cast<double2*>(mesh->d_four_a);
   if (int rc = fft_fold_map(mesh, in, eo)) return rc;
-  if (!accumulate)
-    for (int k = 0; k < nseg; ++k)
-      if (int rc = jgcm_fill(mesh, snm[k], 0.0, (size_t)nlev[k] * spec_d)) return rc;
+  if (!accumulate) {
+    // zero the outputs, one fill per run of adjacent arrays
+    for (int k = 0; k < nseg;) {
+      size_t count = (size_t)nlev[k] * spec_d;
+      int q = k + 1;
+      while (q < nseg && snm[q] == snm[k] + count) count += (size_t)nlev[q++] * spec_d;
+      if (int rc = jgcm_fill(mesh, snm[k], 0.0, count)) return rc;
+      k = q;
+    }
+  }
   return legendre_inv_adjoint(mesh, eo, out, B, (double)mesh->nlon, 2.0 * mesh->nlon, accumulate);
 }
 
