@@ -353,9 +353,12 @@ def adjoint_bench(jgcm, torch, precision, n_fd=16):
     tg, _ = grids(fin)
     target = tg + torch.from_numpy(0.5 * np.random.default_rng(7).standard_normal((nd, nlat, nlon))).cuda()
     L0, bars = loss(fin, target, grad=True)
-    # warm-up sweep: builds the implicit-correction matrices, the temporaries and
-    # the captured reverse steps, which the timed sweep (same object) replays
-    adj.backward(bars)
+    # two warm-up sweeps: the first runs each kind of reverse step eagerly
+    # (implicit-correction matrices, temporaries), the second captures each
+    # kind into a graph (the first step's kind occurs once per sweep); the
+    # timed sweep (same object) only replays
+    for _ in range(2):
+        adj.backward(bars)
     mesh.synchronize()
     t0 = time.perf_counter()
     adj, fin = run(c0, adj)

@@ -40,6 +40,7 @@ SIGNATURES = {
     "jgcm_memcpy_d2h": [P, P, SZ],
     "jgcm_copy": [P, P, P, SZ],
     "jgcm_copy_batch": [P, I, P, P, P],
+    "jgcm_add_batch": [P, I, P, P, P],
     "jgcm_fill": [P, P, D, SZ],
     "jgcm_time_advance": [P, I, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P),
                           ctypes.POINTER(SZ)],

@@ -708,9 +708,11 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         def G(nlev):
             return take("g", nlev)
 
-        def add(dst, src):          # dst += src (same shape), on the mesh stream
-            call("jgcm_axpby", h, dst.numel() * dst.element_size() // 8, 1.0, _ptr(dst), 1.0,
-                 _ptr(src), _ptr(dst))
+        def add(*pairs):            # dst += src for each (dst, src) pair, one launch
+            P, n = ctypes.c_void_p, len(pairs)
+            call("jgcm_add_batch", h, n, (P * n)(*[d.data_ptr() for d, _ in pairs]),
+                 (P * n)(*[s.data_ptr() for _, s in pairs]),
+                 (ctypes.c_size_t * n)(*[d.numel() * d.element_size() // 8 for d, _ in pairs]))
 
         def t2g_T_add(grids, targets):   # targets[k] += transpose of the inverse (grids[k])
             trans_spherical_to_grid_adjoint_batch(M, grids, targets, accumulate=True)
@@ -726,8 +728,7 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
             t2g_T_add([tb, lb], [Btc, Blc])
             sv, sd = S(nd), S(nd)
             uv_grid_from_vor_div_adjoint(M, ub, vb, sv, sd)
-            add(Bvc, sv)
-            add(Bdc, sd)
+            add((Bvc, sv), (Bdc, sd))
         # 2. damping + leapfrog + Time_Advance (vor, div, T damped; ln ps not)
         Gv, Gd, Gt, Gl = S(nd), S(nd), S(nd), S(1)
         damped_leapfrog_adjoint(I, init, Bvp, Bvc, Bvp, Bvc, Gv)
@@ -740,12 +741,11 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
              _ptr(Gt), _ptr(out))
         o_ddiv, o_dl, o_dt, o_en = out[0:nd], out[nd:nd + 1], out[nd + 1:2 * nd + 1], out[2 * nd + 1:3 * nd + 1]
         base = 3 * nd + 1
-        add(Bdc, out[base:base + nd])
-        add(Bdp, out[base + nd:base + 2 * nd])
-        add(Blc, out[base + 2 * nd:base + 2 * nd + 1])
-        add(Blp, out[base + 2 * nd + 1:base + 2 * nd + 2])
-        add(Btc, out[base + 2 * nd + 2:base + 3 * nd + 2])
-        add(Btp, out[base + 3 * nd + 2:base + 4 * nd + 2])
+        add((Bdc, out[base:base + nd]), (Bdp, out[base + nd:base + 2 * nd]),
+            (Blc, out[base + 2 * nd:base + 2 * nd + 1]),
+            (Blp, out[base + 2 * nd + 1:base + 2 * nd + 2]),
+            (Btc, out[base + 2 * nd + 2:base + 3 * nd + 2]),
+            (Btp, out[base + 3 * nd + 2:base + 4 * nd + 2]))
         # 4. forward transforms of δln ps, δT, Φ+KE and (δu, δv) -> (δζ, δD)
         dl_g, dt_g, en_g = G(1), G(nd), G(nd)
         trans_grid_to_spherical_adjoint_batch(M, [o_dl, o_dt, o_en], [dl_g, dt_g, en_g])
@@ -773,22 +773,23 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
              arr([div_g, u_g, v_g, t_g, vor_g, tl, tt, gl, gt, u_pg, v_pg, t_pg, ps_pg, ps_c]),
              arr([du_g, dv_g, dt_g, en_g, dl_g]), arr(outs), _ptr(self._param_b))
         # 6. the previous cotangents of the grid p-fields belong to this step's grid c-fields
-        add(u_b, Bup)
-        add(v_b, Bvpg)
-        add(t_b, Btpg)
-        add(psc_b, Bpsp)
-        for dst, src in ((Bup, up_b), (Bvpg, vp_b), (Btpg, tp_b), (Bpsp, psp_b)):
-            call("jgcm_copy", h, _ptr(dst), _ptr(src), src.numel() * src.element_size())
+        add((u_b, Bup), (v_b, Bvpg), (t_b, Btpg), (psc_b, Bpsp))
+        cp = ((Bup, up_b), (Bvpg, vp_b), (Btpg, tp_b), (Bpsp, psp_b))
+        P4 = ctypes.c_void_p * 4
+        call("jgcm_copy_batch", h, 4, P4(*[d.data_ptr() for d, _ in cp]),
+             P4(*[s.data_ptr() for _, s in cp]),
+             (ctypes.c_size_t * 4)(*[s.numel() * s.element_size() for _, s in cp]))
         # 7. grid c-fields -> spectral c-fields
         lg = G(1)
         call("jgcm_xmy", h, lg.numel(), 1.0, _ptr(ps_c), _ptr(psc_b), _ptr(lg), 0)
         t2g_T_add([vor_b, div_b, t_b, lg], [Bvc, Bdc, Btc, Blc])
         sv, sd = S(nd), S(nd)
         uv_grid_from_vor_div_adjoint(M, u_b, v_b, sv, sd)
-        add(Bvc, sv)
-        add(Bdc, sd)
-        # gradients of ln ps and T (the inverse batch divided them by cos θ)
-        s_ll, s_lt, s_tl, s_tt = S(1), S(1), S(nd), S(nd)
+        add((Bvc, sv), (Bdc, sd))
+        # gradients of ln ps and T (the inverse batch divided them by cos θ);
+        # adjacent outputs, zeroed by one fill
+        s_all = S(2 + 2 * nd)
+        s_ll, s_lt, s_tl, s_tt = s_all[0:1], s_all[1:2], s_all[2:2 + nd], s_all[2 + nd:]
         trans_spherical_to_grid_adjoint_batch(M, [gl_b, gt_b, tl_b, tt_b],
                                               [s_ll, s_lt, s_tl, s_tt], divcos=[1, 1, 1, 1])
         call("jgcm_gradient_cos_adjoint", h, _ptr(s_ll), _ptr(s_lt), _ptr(Blc), 1, 1)

@@ -55,6 +55,16 @@ __global__ void copy_batch_kernel(CopySet s) {
     d[i] = x[i];
 }
 
+// blockIdx.y = array: dst += src
+__global__ void add_batch_kernel(CopySet s) {
+  const int f = blockIdx.y;
+  double* __restrict__ d = s.dst[f];
+  const double* __restrict__ x = s.src[f];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < s.count[f];
+       i += (size_t)gridDim.x * blockDim.x)
+    d[i] = d[i] + x[i];
+}
+
 constexpr int MAXADV = 16;
 struct AdvanceSet {
   double* p[MAXADV];
@@ -249,6 +259,30 @@ int jgcm_copy_batch(jgcm_mesh_t* mesh, int n, double* const* dst, const double* 
   }
   if (maxc == 0) return JGCM_OK;
   hipLaunchKernelGGL(copy_batch_kernel, dim3(nblocks(maxc, 1024), n), dim3(256), 0, mesh->stream,
+                     s);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_add_batch(jgcm_mesh_t* mesh, int n, double* const* dst, const double* const* src,
+                   const size_t* count) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(n >= 1 && n <= MAXCOPY, "n must be in 1..%d, got %d", MAXCOPY, n);
+  JGCM_REQUIRE(dst && src && count, "NULL argument");
+  CopySet s{};
+  size_t maxc = 0;
+  for (int f = 0; f < n; ++f) {
+    JGCM_REQUIRE(dst[f] && src[f], "NULL pointer (array %d)", f);
+    for (int q = 0; q < f; ++q)
+      JGCM_REQUIRE(dst[q] + count[q] <= dst[f] || dst[f] + count[f] <= dst[q],
+                   "destinations %d and %d overlap", q, f);
+    s.dst[f] = dst[f];
+    s.src[f] = src[f];
+    s.count[f] = count[f];
+    if (count[f] > maxc) maxc = count[f];
+  }
+  if (maxc == 0) return JGCM_OK;
+  hipLaunchKernelGGL(add_batch_kernel, dim3(nblocks(maxc, 1024), n), dim3(256), 0, mesh->stream,
                      s);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;

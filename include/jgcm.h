@@ -99,6 +99,11 @@ int jgcm_copy(jgcm_mesh_t* mesh, void* dst, const void* src, size_t bytes);
  * Sizes and addresses must be multiples of 8 bytes.                        */
 int jgcm_copy_batch(jgcm_mesh_t* mesh, int n, double* const* dst, const double* const* src,
                     const size_t* bytes);
+/* dst[k][i] += src[k][i], i < count[k] doubles, for n (<= 32) pairs in one
+ * launch (no reference counterpart; accumulates a reverse step's
+ * cotangents).  The destinations must not overlap.                         */
+int jgcm_add_batch(jgcm_mesh_t* mesh, int n, double* const* dst, const double* const* src,
+                   const size_t* count);
 int jgcm_fill(jgcm_mesh_t* mesh, double* dst, double value, size_t count);
 /* Time_Advance! (Dyn_Data.jl:326-364) for nfields (<= 16) prognostic fields
  * in one launch: per element, p = c, then c = n.  count[f] = doubles in
