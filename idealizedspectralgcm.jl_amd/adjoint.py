@@ -274,17 +274,20 @@ class _Run_Adjoint:
             k0 = s * self.every
             k1 = min(k0 + self.every, self.nsteps)
             self._load(self._checkpoints[s])
-            snaps = []
-            for k in range(k0, k1):
-                before = self._snapshot()
-                self._step(k)
-                snaps.append((before, self._snapshot_after()))
+            snaps = [self._recompute(k) for k in range(k0, k1)]
             for k in reversed(range(k0, k1)):
                 before, after = snaps[k - k0]
                 snap = before if after is None else (before, after)
                 self._step_adjoint(k, self.starts_init and k == 0, snap, bars)
         self.mesh.synchronize()
         return bars
+
+    def _recompute(self, k):
+        """Step k of a checkpoint segment again, with the snapshots its
+        reverse step needs: (before, after)."""
+        before = self._snapshot()
+        self._step(k)
+        return before, self._snapshot_after()
 
 
 class Run(torch.autograd.Function):
@@ -539,6 +542,32 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         before = self._slot()[0]
         self._copy_into(before, [getattr(self._d, n) for n in self.STATE])
         return before
+
+    def _recompute(self, k):
+        """With graphs, the recomputed step and its two snapshots are one
+        captured graph per (kind of step, snapshot buffer), replayed after
+        one eager run; the Python side (snapshot counter, last solver) is
+        advanced here, since a replay runs no Python."""
+        if not self.use_graphs:
+            return super()._recompute(k)
+        from .graph import Step_Graph
+        i = self._snap_count % self.every
+        key = ("forward", k == 0, i,
+               tuple(sorted((str(a), float(b)) for a, b in self.params.items())))
+        runs = self._graph_runs.get(key, 0)
+        self._graph_runs[key] = runs + 1
+        if runs == 0:
+            return super()._recompute(k)   # eager: allocates the snapshot buffers
+        g = self._graphs.get(key)
+        if g is None:
+            count = self._snap_count
+            g = self._graphs[key] = Step_Graph(self.mesh, lambda: _Run_Adjoint._recompute(self, k))
+            self._snap_count = count       # the capture ran the Python side once
+        g.launch()
+        self._last = self._solver(k)
+        before, sc, after = self._slots[i]
+        self._snap_count += 1
+        return before, (sc, after)
 
     def _snapshot_after(self):
         _, sc, after = self._slot()
