@@ -103,12 +103,19 @@ HS_PARAMS = {"σ_b": 0.7, "k_f": 1.0, "k_a": 1.0 / 40.0, "k_s": 1.0 / 4.0, "ΔT_
 
 
 def tuning_path():
-    """Autotuner cache of this run (JGCM_TUNING_FILE, else under gpurun_out/ or /tmp)."""
-    p = os.environ.get("JGCM_TUNING_FILE")
-    if p:
-        return p
-    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpurun_out")
-    return os.path.join(d if os.path.isdir(d) else "/tmp", "jgcm_tuning.json")
+    """Autotuner cache: the autotuner's own default (JGCM_TUNING_FILE, else
+    ~/.cache/jgcm_amd/tuning.json), or /tmp when the home directory is not
+    writable."""
+    from jgcm_amd.autotune import default_cache_path
+    p = default_cache_path()
+    d = os.path.dirname(p)
+    try:
+        os.makedirs(d, exist_ok=True)
+        if os.access(d, os.W_OK):
+            return p
+    except OSError:
+        pass
+    return "/tmp/jgcm_tuning.json"
 
 
 def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):

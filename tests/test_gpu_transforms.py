@@ -2,7 +2,7 @@
 
 Tolerances: FP64 path 1e-12 relative (max-abs error / max-abs value); the
 FP32 and split-FP32 Legendre variants 1e-6 relative per level-field
-(2-norm), the north-star bar for the reduced-precision transforms.
+(2-norm), the accuracy required of the reduced-precision transforms.
 """
 import numpy as np
 import pytest
