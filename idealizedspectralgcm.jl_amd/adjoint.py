@@ -259,8 +259,12 @@ class _Run_Adjoint:
         for k in range(self.nsteps):
             if k % self.every == 0:
                 self._checkpoints.append(self._state())
-            self._step(k)
+            self._run_step(k)
         return self._state()
+
+    def _run_step(self, k):
+        """Step k of the forward run."""
+        self._step(k)
 
     def _working_bars(self, bars):
         """The cotangent buffers the reverse steps update in place."""
@@ -547,13 +551,13 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         """With graphs, the recomputed step and its two snapshots are one
         captured graph per (kind of step, snapshot buffer), replayed after
         one eager run; the Python side (snapshot counter, last solver) is
-        advanced here, since a replay runs no Python."""
-        if not self.use_graphs:
+        advanced here, since a replay runs no Python.  Step 0 (once per
+        sweep) stays eager."""
+        if not self.use_graphs or k == 0:
             return super()._recompute(k)
         from .graph import Step_Graph
         i = self._snap_count % self.every
-        key = ("forward", k == 0, i,
-               tuple(sorted((str(a), float(b)) for a, b in self.params.items())))
+        key = ("forward", i, tuple(sorted((str(a), float(b)) for a, b in self.params.items())))
         runs = self._graph_runs.get(key, 0)
         self._graph_runs[key] = runs + 1
         if runs == 0:
@@ -672,10 +676,12 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
         return tuple(t.clone() for t in out) if self.use_graphs else out
 
     def _close_graphs(self):
-        for g in self._graphs.values():
-            g.close()
-        self._graphs = {}
-        self._graph_runs = {}
+        """Drop the reverse-step graphs (they read the staging and cotangent
+        buffers being replaced); the forward-step graphs stay valid."""
+        for key in [q for q in self._graphs if q[0] == "reverse"]:
+            self._graphs.pop(key).close()
+        for key in [q for q in self._graph_runs if q[0] == "reverse"]:
+            del self._graph_runs[key]
 
     def _step_adjoint(self, k, init, snap, bars):
         if not self.use_graphs:
@@ -693,7 +699,8 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
              (ctypes.c_size_t * len(flat))(*[t.numel() * t.element_size() for t in flat]))
         staged = (tuple(stage[:len(st)]), (stage[len(st)], tuple(stage[len(st) + 1:])))
         # one graph per kind of step (solver, integrator, forcing parameters)
-        key = (k == 0, bool(init), tuple(sorted((str(a), float(b)) for a, b in self.params.items())))
+        key = ("reverse", k == 0, bool(init),
+               tuple(sorted((str(a), float(b)) for a, b in self.params.items())))
         runs = self._graph_runs.get(key, 0)
         if runs == 0:
             # eager: fills the temporaries' pool, the implicit matrix and the scratch
