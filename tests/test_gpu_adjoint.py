@@ -123,3 +123,29 @@ def test_autograd_matches_central_differences(jgcm):
             fd = (loss(x + h * dd) - loss(x - h * dd)).item() / (2 * h)
         an = float(np.sum(g * d))
         assert abs(fd - an) <= 1e-7 * max(1.0, abs(an)), (fd, an)
+
+
+def test_add_batch_equals_axpby_and_rejects_overlap(jgcm):
+    """jgcm_add_batch (the reverse steps' cotangent accumulation) adds each
+    pair exactly like jgcm_axpby with unit coefficients, and refuses
+    destinations that overlap."""
+    import ctypes
+    from jgcm_amd._lib import call
+    mesh = jgcm.Spectral_Spherical_Mesh(21, 22, 32, 64, 1, 6.371e6)
+    rng = np.random.default_rng(5)
+    n = (1000, 37, 4096)
+    dst = [torch.tensor(rng.standard_normal(k), device="cuda") for k in n]
+    src = [torch.tensor(rng.standard_normal(k), device="cuda") for k in n]
+    ref = [d.clone() for d in dst]
+    P = ctypes.c_void_p
+    for d, s in zip(ref, src):
+        call("jgcm_axpby", mesh.handle, d.numel(), 1.0, P(d.data_ptr()), 1.0, P(s.data_ptr()),
+             P(d.data_ptr()))
+    call("jgcm_add_batch", mesh.handle, 3, (P * 3)(*[d.data_ptr() for d in dst]),
+         (P * 3)(*[s.data_ptr() for s in src]), (ctypes.c_size_t * 3)(*n))
+    mesh.synchronize()
+    for a, b in zip(dst, ref):
+        assert torch.equal(a, b)
+    with pytest.raises(AssertionError, match="overlap"):
+        call("jgcm_add_batch", mesh.handle, 2, (P * 2)(dst[2].data_ptr(), dst[2].data_ptr() + 800),
+             (P * 2)(src[2].data_ptr(), src[1].data_ptr()), (ctypes.c_size_t * 2)(200, 37))
