@@ -77,6 +77,7 @@ SIGNATURES = {
     "jgcm_uv_grid_from_vor_div_adjoint": [P, P, P, P, P, I],
     "jgcm_vorticity_flux_adjoint": [P, P, P, P, D, P, P, P, P, P, I],
     "jgcm_damped_leapfrog_adjoint": [P, I, I, P, P, P, P, P, I],
+    "jgcm_damped_leapfrog_adjoint_batch": [P, I, I, P, P, P, P, P, P, P],
     "jgcm_barotropic_step_adjoint": [P, I, P, P, P, D, P, P, I],
     "jgcm_sw_step_adjoint": [P, I, P, P, P, P, P, P, D, D, D, D, P, P, P, P, P, P, I],
     "jgcm_hs_column_adjoint": [P, P, D, P, P, P, P, P],

@@ -156,6 +156,25 @@ def vorticity_flux_adjoint(mesh, grid_vor, grid_u, grid_v, omega, du_bar, dv_bar
          float(omega), _ptr(du_bar), _ptr(dv_bar), _ptr(vor_bar), _ptr(u_bar), _ptr(v_bar), n)
 
 
+def damped_leapfrog_adjoint_batch(integrator, init_step, fields):
+    """damped_leapfrog_adjoint of up to 8 fields in one launch; ``fields`` =
+    [(p_bar_next, c_bar_next, p_bar, c_bar, g_bar, damp), ...]."""
+    mesh = integrator.mesh
+    k = len(fields)
+    assert 1 <= k <= 8
+    nlev = []
+    for f in fields:
+        n = mesh._nlev(f[0], "spec", True)
+        for t in f[1:5]:
+            assert mesh._nlev(t, "spec", True) == n
+        nlev.append(n)
+    P = ctypes.c_void_p
+    arr = lambda j: (P * k)(*[f[j].data_ptr() for f in fields])
+    call("jgcm_damped_leapfrog_adjoint_batch", integrator.handle, 1 if init_step else 0, k,
+         (ctypes.c_int * k)(*[1 if f[5] else 0 for f in fields]), arr(0), arr(1), arr(2), arr(3),
+         arr(4), (ctypes.c_int * k)(*nlev))
+
+
 def damped_leapfrog_adjoint(integrator, init_step, p_bar_next, c_bar_next, p_bar, c_bar, g_bar,
                             damp=True):
     """Transpose of Compute_Spectral_Damping! + Filtered_Leapfrog! + Time_Advance!
@@ -767,10 +786,10 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
             add((Bvc, sv), (Bdc, sd))
         # 2. damping + leapfrog + Time_Advance (vor, div, T damped; ln ps not)
         Gv, Gd, Gt, Gl = S(nd), S(nd), S(nd), S(1)
-        damped_leapfrog_adjoint(I, init, Bvp, Bvc, Bvp, Bvc, Gv)
-        damped_leapfrog_adjoint(I, init, Bdp, Bdc, Bdp, Bdc, Gd)
-        damped_leapfrog_adjoint(I, init, Btp, Btc, Btp, Btc, Gt)
-        damped_leapfrog_adjoint(I, init, Blp, Blc, Blp, Blc, Gl, damp=False)
+        damped_leapfrog_adjoint_batch(I, init, [(Bvp, Bvc, Bvp, Bvc, Gv, True),
+                                                (Bdp, Bdc, Bdp, Bdc, Gd, True),
+                                                (Btp, Btc, Btp, Btc, Gt, True),
+                                                (Blp, Blc, Blp, Blc, Gl, False)])
         # 3. ∇²(energy), δdiv -= energy, Implicit_Correction!
         out = S(7 * nd + 3)
         call("jgcm_implicit_adjoint", h, nd, _ptr(self._implicit_matrix(si)), _ptr(Gd), _ptr(Gl),

@@ -113,11 +113,12 @@ __global__ void vorticity_flux_adjoint_kernel(const double* vor, const double* u
 //   later steps:  δ = (G - p d)/(1 + 2Δt d),  c' = p + 2Δt δ,
 //                 p' = (1-2r) c + 2r p + 2rΔt δ
 //   first step:   δ = (G - c d)/(1 + Δt d),   c' = c + Δt δ,   p' = c + rΔt δ
-__global__ void damped_leapfrog_adjoint_kernel(const double2* pb_next, const double2* cb_next,
-                                               double2* pb, double2* cb, double2* gb,
-                                               const double* damping, int nm, int ns, size_t n_all,
-                                               int init_step, double dt, double r) {
-  GRID_STRIDE(i, n_all) {
+__device__ __forceinline__ void leapfrog_adjoint_elem(size_t i, const double2* pb_next,
+                                                      const double2* cb_next, double2* pb,
+                                                      double2* cb, double2* gb,
+                                                      const double* damping, int nm, int ns,
+                                                      int init_step, double dt, double r) {
+  {
     int m = (int)(i % nm);
     int k = (int)((i / nm) % (ns + 1));
     double d = damping ? damping[m * (ns + 1) + k] : 0.0;
@@ -140,6 +141,36 @@ __global__ void damped_leapfrog_adjoint_kernel(const double2* pb_next, const dou
       pb[i] = np;
     }
   }
+}
+
+__global__ void damped_leapfrog_adjoint_kernel(const double2* pb_next, const double2* cb_next,
+                                               double2* pb, double2* cb, double2* gb,
+                                               const double* damping, int nm, int ns, size_t n_all,
+                                               int init_step, double dt, double r) {
+  GRID_STRIDE(i, n_all) {
+    leapfrog_adjoint_elem(i, pb_next, cb_next, pb, cb, gb, damping, nm, ns, init_step, dt, r);
+  }
+}
+
+// the same for up to MAXLFA fields of one integrator (blockIdx.y = field)
+constexpr int MAXLFA = 8;
+struct LeapAdjSet {
+  const double2* pbn[MAXLFA];
+  const double2* cbn[MAXLFA];
+  double2* pb[MAXLFA];
+  double2* cb[MAXLFA];
+  double2* gb[MAXLFA];
+  const double* damping[MAXLFA];
+  size_t n[MAXLFA];
+};
+
+__global__ void damped_leapfrog_adjoint_batch_kernel(LeapAdjSet s, int nm, int ns, int init_step,
+                                                     double dt, double r) {
+  const int f = blockIdx.y;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < s.n[f];
+       i += (size_t)gridDim.x * blockDim.x)
+    leapfrog_adjoint_elem(i, s.pbn[f], s.cbn[f], s.pb[f], s.cb[f], s.gb[f], s.damping[f], nm, ns,
+                          init_step, dt, r);
 }
 
 // (vor_bar, div_bar) -> (u_bar, v_bar): transposed α contractions into the E/O
@@ -373,6 +404,39 @@ int jgcm_damped_leapfrog_adjoint(jgcm_leapfrog_t* lf, int init_step, int damp,
                      (const double2*)p_bar_next, (const double2*)c_bar_next, (double2*)p_bar,
                      (double2*)c_bar, (double2*)g_bar, damp ? lf->d_damping : nullptr, M->nm, M->ns, n,
                      init_step ? 1 : 0, (double)lf->dt, lf->robert_coef);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_damped_leapfrog_adjoint_batch(jgcm_leapfrog_t* lf, int init_step, int nfields,
+                                       const int* damp, const double* const* p_bar_next,
+                                       const double* const* c_bar_next, double* const* p_bar,
+                                       double* const* c_bar, double* const* g_bar,
+                                       const int* nlev) {
+  JGCM_REQUIRE(lf != nullptr && lf->mesh != nullptr, "leapfrog is NULL");
+  JGCM_REQUIRE(nfields >= 1 && nfields <= MAXLFA, "nfields must be in 1..%d, got %d", MAXLFA,
+               nfields);
+  JGCM_REQUIRE(damp && p_bar_next && c_bar_next && p_bar && c_bar && g_bar && nlev,
+               "NULL argument");
+  jgcm_mesh* M = lf->mesh;
+  LeapAdjSet s{};
+  size_t maxn = 0;
+  for (int f = 0; f < nfields; ++f) {
+    JGCM_REQUIRE(p_bar_next[f] && c_bar_next[f] && p_bar[f] && c_bar[f] && g_bar[f],
+                 "NULL field pointer (field %d)", f);
+    JGCM_REQUIRE(nlev[f] >= 1, "nlev[%d] must be >= 1, got %d", f, nlev[f]);
+    s.pbn[f] = (const double2*)p_bar_next[f];
+    s.cbn[f] = (const double2*)c_bar_next[f];
+    s.pb[f] = (double2*)p_bar[f];
+    s.cb[f] = (double2*)c_bar[f];
+    s.gb[f] = (double2*)g_bar[f];
+    s.damping[f] = damp[f] ? lf->d_damping : nullptr;
+    s.n[f] = (size_t)nlev[f] * M->spec_elems();
+    if (s.n[f] > maxn) maxn = s.n[f];
+  }
+  hipLaunchKernelGGL(damped_leapfrog_adjoint_batch_kernel, dim3(nblk(maxn), nfields), dim3(256), 0,
+                     M->stream, s, M->nm, M->ns, init_step ? 1 : 0, (double)lf->dt,
+                     lf->robert_coef);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

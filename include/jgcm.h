@@ -268,6 +268,14 @@ int jgcm_vorticity_flux_adjoint(jgcm_mesh_t* mesh, const double* grid_vor, const
 int jgcm_damped_leapfrog_adjoint(jgcm_leapfrog_t* lf, int init_step, int damp,
                                  const double* p_bar_next, const double* c_bar_next, double* p_bar,
                                  double* c_bar, double* g_bar, int nlev);
+/* the same for up to 8 fields of one integrator in one launch (no reference
+ * counterpart: the transpose of the batched leapfrog); damp[f] and nlev[f]
+ * per field; bit-identical to the single calls.                            */
+int jgcm_damped_leapfrog_adjoint_batch(jgcm_leapfrog_t* lf, int init_step, int nfields,
+                                       const int* damp, const double* const* p_bar_next,
+                                       const double* const* c_bar_next, double* const* p_bar,
+                                       double* const* c_bar, double* const* g_bar,
+                                       const int* nlev);
 /* one reverse step of Barotropic_Dynamics! (Barotropic_Dynamics.jl:2-46), in
  * place on (p_bar, c_bar); grid_vor, grid_u, grid_v are the grid fields of
  * the step's starting state.                                                */

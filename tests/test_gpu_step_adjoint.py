@@ -145,6 +145,34 @@ def test_damped_leapfrog_adjoint(jgcm, init):
     assert torch.equal(P2, pb) and torch.equal(C2, cb) and torch.equal(g2, gb)
 
 
+@pytest.mark.parametrize("init", [True, False])
+def test_damped_leapfrog_adjoint_batch_equals_single_calls(jgcm, init):
+    """One launch for several fields (levels differ, one field undamped, in
+    place) gives the single calls' results bit for bit."""
+    from jgcm_amd import adjoint as ad
+    N, nlat = 42, 64
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 3, A)
+    I = jgcm.Filtered_Leapfrog(0.04, 4, 1.0 / (0.1 * 86400), mesh.laplacian_eig, 0.0, 1800, True, 0,
+                               864000)
+    if not init:
+        jgcm.Update_Init_Step(I)
+    rng = np.random.default_rng(23)
+    levels, damps = (3, 1, 2), (True, False, True)
+    P = [dev(rspec(N, n, rng, False)) for n in levels]
+    C = [dev(rspec(N, n, rng, False)) for n in levels]
+    single = []
+    for p, c, d in zip(P, C, damps):
+        p2, c2, g2 = p.clone(), c.clone(), mesh.spectral_zeros(p.shape[0])
+        ad.damped_leapfrog_adjoint(I, init, p2, c2, p2, c2, g2, damp=d)
+        single.append((p2, c2, g2))
+    G = [mesh.spectral_zeros(n) for n in levels]
+    ad.damped_leapfrog_adjoint_batch(I, init, [(p, c, p, c, g, d)
+                                               for p, c, g, d in zip(P, C, G, damps)])
+    torch.cuda.synchronize()
+    for (p2, c2, g2), p, c, g in zip(single, P, C, G):
+        assert torch.equal(p, p2) and torch.equal(c, c2) and torch.equal(g, g2)
+
+
 def rh4_state(jgcm, mesh):
     """Rossby-Haurwitz wave 4 vorticity (exp/Barotropic), spectral."""
     w = K = 7.848e-6
