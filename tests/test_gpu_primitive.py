@@ -20,11 +20,12 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-def build(jgcm, N, nlat, nd, dt, seed=11, coord="even_sigma", tracers=(0, 0)):
+def build(jgcm, N, nlat, nd, dt, seed=11, coord="even_sigma", tracers=(0, 0),
+          scheme="second_centered_wts"):
     nlon = 2 * nlat
     # oracle
     M = SpectralSphericalMesh(N, N + 1, nlat, nlon, nd, A)
-    vc = P.VertCoordinate(nlon, nlat, nd, coord, "simmons_and_burridge", "second_centered_wts", 1e5)
+    vc = P.VertCoordinate(nlon, nlat, nd, coord, "simmons_and_burridge", scheme, 1e5)
     atmo = odyn.AtmoData("hs", nlon, nlat, nd, True, True, False, False, M.sin_theta, A, OMEGA)
     I = FilteredLeapfrog(0.04, 4, 1.15741e-4, M.laplacian_eig, 0.5, dt, True, 0, 86400 * 200)
     si = P.SemiImplicitSolver(vc, atmo, I, 1e5, np.full(nd, 300.0), M.wave_numbers)
@@ -33,8 +34,8 @@ def build(jgcm, N, nlat, nd, dt, seed=11, coord="even_sigma", tracers=(0, 0)):
     P.spectral_initialize_fields(M, atmo, vc, 1.0e5, 300.0, d.grid_geopots, d, grid_t_noise=noise)
     # device
     dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, nlon, nd, A)
-    dvc = jgcm.Vert_Coordinate(nlon, nlat, nd, coord, "simmons_and_burridge",
-                               "second_centered_wts", 1e5, mesh=dev)
+    dvc = jgcm.Vert_Coordinate(nlon, nlat, nd, coord, "simmons_and_burridge", scheme, 1e5,
+                               mesh=dev)
     datmo = jgcm.Atmo_Data("hs", nlon, nlat, nd, True, True, False, False, dev.sinθ, A, OMEGA)
     dI = jgcm.Filtered_Leapfrog(0.04, 4, 1.15741e-4, dev.laplacian_eig, 0.5, dt, True, 0, 86400 * 200)
     dsi = jgcm.Semi_Implicit_Solver(dvc, datmo, dI, 1e5, np.full(nd, 300.0), dev.wave_numbers)
@@ -99,6 +100,21 @@ def test_hs_steps_match_oracle(jgcm, N, nlat, nd, dt, steps, coord):
     np.testing.assert_allclose(dsi.wave_matrix, si.wave_matrix, rtol=1e-11, atol=1e-13)
     sc = dsi.scalars()
     assert np.isfinite(sc).all()
+
+
+def test_hs_steps_second_centered_advection_match_oracle(jgcm):
+    """Vert_Advection! with "second_centered" (Vert_Coordinate.jl:194-196,
+    uniform-spacing weights) on the unevenly spaced mcm levels, where it
+    differs from the default "second_centered_wts"."""
+    (M, vc, atmo, si, d), (dev, dvc, datmo, dsi, dd) = build(
+        jgcm, 21, 32, 14, 600, coord="mcm", scheme="second_centered")
+    for i in range(3):
+        P.atmosphere_update(M, atmo, vc, si, d, HS)
+        jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS)
+        if i == 0:
+            si.update_init_step()
+            jgcm.Update_Init_Step(dsi)
+    compare(d, dd, 1e-10)
 
 
 def test_tracers_cycle_with_the_step(jgcm):
