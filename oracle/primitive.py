@@ -26,7 +26,13 @@ Pins: tests/test_oracle_primitive.py (isothermal hydrostatic column exact,
 Held-Suarez T_eq/k_T closed forms, W_n (I - ξ² eig D) = I, the linear
 operators against finite differences of the nonlinear ones they linearise,
 an isothermal resting atmosphere that stays at rest without forcing, mass
-conservation of the corrections).
+conservation of the corrections) and tests/test_oracle_hs_pins.py
+(``vert_advection``: constant and linear profiles, column budget;
+``four_in_one``: pressure-gradient force -R T grad(ln ps) on pure sigma,
+omega = 0 for a uniform wind advecting ps, interface mass fluxes;
+``compute_corrections_init`` / ``compute_corrections``: mean ps restored and
+the new state's mass-weighted energy equal to the target energy computed
+independently; each pin is shown to fail on a planted mistake).
 """
 import numpy as np
 
