@@ -46,6 +46,7 @@ SIGNATURES = {
                           ctypes.POINTER(SZ)],
     "jgcm_mesh_create": [I, I, I, I, I, D, ctypes.POINTER(P)],
     "jgcm_mesh_destroy": [P],
+    "jgcm_mesh_lifetime": [P, ctypes.POINTER(I), ctypes.POINTER(I)],
     "jgcm_mesh_synchronize": [P],
     "jgcm_mesh_stream": [P],
     "jgcm_mesh_share_stream": [P, P],

@@ -285,6 +285,23 @@ class _Run_Adjoint:
         """Step k of the forward run."""
         self._step(k)
 
+    # ---- one recorded trajectory at a time: Run keeps each apply's own ----
+    def _trajectory(self):
+        """What a reverse sweep needs from the last forward run."""
+        return {"checkpoints": self._checkpoints, "nsteps": self.nsteps}
+
+    def _restore(self, traj):
+        self._checkpoints = traj["checkpoints"]
+        self.nsteps = traj["nsteps"]
+
+    def _forward_state(self, state, nsteps):
+        """forward() on a STATE-ordered tuple."""
+        return self._run(tuple(state), nsteps)
+
+    def _backward_state(self, bars):
+        """backward() on STATE-ordered cotangents."""
+        return self._reverse(tuple(bars))
+
     def _working_bars(self, bars):
         """The cotangent buffers the reverse steps update in place."""
         return tuple(b.clone() for b in bars)
@@ -328,7 +345,11 @@ class Run(torch.autograd.Function):
     def forward(ctx, adjoint, nsteps, *state):
         ctx.adjoint = adjoint
         ctx.shapes = [(t.shape, t.dtype) for t in state]
-        return adjoint._run(tuple(t.detach().contiguous() for t in state), int(nsteps))
+        out = adjoint._forward_state(tuple(t.detach().contiguous() for t in state), int(nsteps))
+        # the adjoint object keeps only its latest run: this apply's checkpoints
+        # travel with ctx, so several applies can be differentiated in any order
+        ctx.trajectory = adjoint._trajectory()
+        return out
 
     @staticmethod
     def backward(ctx, *bars):
@@ -336,7 +357,8 @@ class Run(torch.autograd.Function):
         full = tuple(b.contiguous() if b is not None else
                      torch.zeros(shp, dtype=dt, device=adj.mesh.device)
                      for b, (shp, dt) in zip(bars, ctx.shapes))
-        return (None, None) + tuple(adj._reverse(full))
+        adj._restore(ctx.trajectory)
+        return (None, None) + tuple(adj._backward_state(full))
 
 
 class Barotropic_Adjoint(_Run_Adjoint):
@@ -602,6 +624,21 @@ class Held_Suarez_Adjoint(_Run_Adjoint):
     def forward(self, state, nsteps):
         self._init_lnps = state[7].clone()
         return self._run(tuple(state), nsteps)
+
+    def _trajectory(self):
+        t = super()._trajectory()
+        t["init_lnps"] = self._init_lnps
+        return t
+
+    def _restore(self, traj):
+        super()._restore(traj)
+        self._init_lnps = traj["init_lnps"]
+
+    def _forward_state(self, state, nsteps):
+        return self.forward(state, nsteps)
+
+    def _backward_state(self, bars):
+        return self.backward(bars)
 
     def backward(self, bars):
         """Cotangents of the final state -> those of the initial state (returned)

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -108,6 +109,16 @@ struct jgcm_mesh {
   bool on_side = false, forked = false;
   int device = 0;
 
+  // Lifetime.  The creator holds one reference (dropped by jgcm_mesh_destroy);
+  // every leapfrog, vertical coordinate and graph that uses the mesh and every
+  // mesh that borrows its stream holds one more, so the mesh (its stream,
+  // events and buffers) is freed only after the last of them, whatever order
+  // a garbage collector finalizes them in.
+  std::atomic<int> refs{1};
+  std::atomic<int> live_graphs{0};     // graphs whose kernels use this mesh's scratch
+  jgcm_mesh* stream_owner = nullptr;   // the mesh whose stream this one borrows
+  std::vector<void*> retired;          // scratch replaced while graphs still held it
+
   // host copies of public fields
   std::vector<double> sin_theta, cos_theta, wts, lambda_c, theta_c, lambda_e, theta_e;
   std::vector<double> qnm, dqnm;               // [N+1][ns+1][nlat]
@@ -166,6 +177,18 @@ struct jgcm_leapfrog {
 
 namespace jgcm {
 int ensure_scratch(jgcm_mesh* m, int nfields);
+// Validates the handle; while this thread captures a graph it also records
+// the mesh as one whose buffers the graph will use.
 int check_mesh(const jgcm_mesh* m);
+void mesh_retain(jgcm_mesh* m);
+void mesh_release(jgcm_mesh* m);   // frees the mesh with its last reference
+// Meshes used by the capture this thread has open (jgcm_graph_begin..end).
+struct CaptureCtx {
+  jgcm_mesh* origin = nullptr;
+  std::vector<jgcm_mesh*> meshes;
+};
+CaptureCtx* capture_ctx();         // nullptr when this thread captures nothing
+void capture_open(jgcm_mesh* origin);
+void capture_close(std::vector<jgcm_mesh*>* used);
 int fft_max_ppw(const jgcm_mesh* m);  // largest FFT latitude-pair count per workgroup
 }  // namespace jgcm

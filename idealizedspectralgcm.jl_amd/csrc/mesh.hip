@@ -26,12 +26,38 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
+static thread_local CaptureCtx* g_capture = nullptr;
+
+CaptureCtx* capture_ctx() { return g_capture; }
+
+void capture_open(jgcm_mesh* origin) {
+  g_capture = new CaptureCtx();
+  g_capture->origin = origin;
+  g_capture->meshes.push_back(origin);
+}
+
+void capture_close(std::vector<jgcm_mesh*>* used) {
+  if (!g_capture) return;
+  if (used) used->swap(g_capture->meshes);
+  delete g_capture;
+  g_capture = nullptr;
+}
+
 int check_mesh(const jgcm_mesh* m) {
   if (m == nullptr) {
     set_error("mesh handle is NULL");
     return JGCM_ERR_ARG;
   }
+  if (g_capture) {
+    jgcm_mesh* mm = const_cast<jgcm_mesh*>(m);
+    auto& v = g_capture->meshes;
+    if (std::find(v.begin(), v.end(), mm) == v.end()) v.push_back(mm);
+  }
   return JGCM_OK;
+}
+
+void mesh_retain(jgcm_mesh* m) {
+  if (m) m->refs.fetch_add(1);
 }
 
 // Gauss_And_Legendre.jl:81-115, one thread per node pair (i = 1..n/2).
@@ -146,10 +172,26 @@ __global__ void pack_rows_kernel(int N, int ns, int nlat, double radius,
 int ensure_scratch(jgcm_mesh* m, int nfields) {
   if (nfields <= m->cap_fields) return JGCM_OK;
   int cap = nfields;
+  // Growing synchronises and allocates, which a capture cannot record: the
+  // scratch must be reserved before capturing.
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  JGCM_CHECK_HIP(hipStreamIsCapturing(m->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone || g_capture != nullptr) {
+    set_error("mesh scratch must grow from %d to %d level-fields during a graph capture; "
+              "reserve it before capturing (jgcm_mesh_reserve)", m->cap_fields, nfields);
+    return JGCM_ERR_STATE;
+  }
   double* ptrs[6] = {m->d_four_a, m->d_four_b, m->d_spec_a, m->d_spec_b, m->d_grid_a, m->d_grid_b};
   JGCM_CHECK_HIP(hipStreamSynchronize(m->stream));
-  for (double* p : ptrs)
-    if (p) JGCM_CHECK_HIP(hipFree(p));
+  if (m->live_graphs.load() > 0) {
+    // captured graphs keep these addresses as kernel arguments: keep the old
+    // buffers alive until the mesh is freed
+    for (double* p : ptrs)
+      if (p) m->retired.push_back(p);
+  } else {
+    for (double* p : ptrs)
+      if (p) JGCM_CHECK_HIP(hipFree(p));
+  }
   size_t four = (size_t)cap * m->four_elems() * 2 * sizeof(double);
   size_t spec = (size_t)cap * m->spec_elems() * 2 * sizeof(double);
   size_t grid = (size_t)cap * m->grid_elems() * sizeof(double);
@@ -182,13 +224,20 @@ static void free_mesh(jgcm_mesh* m) {
                   m->d_grid_b, m->d_scalar};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (void* p : m->retired) (void)hipFree(p);
   if (m->signal) (void)hipEventDestroy(m->signal);
   if (m->fork_ev) (void)hipEventDestroy(m->fork_ev);
   if (m->join_ev) (void)hipEventDestroy(m->join_ev);
   if (m->side) (void)hipStreamDestroy(m->side);
   if (m->on_side) m->stream = m->main_stream;
   if (m->stream && !m->stream_borrowed) (void)hipStreamDestroy(m->stream);
+  jgcm_mesh* owner = m->stream_owner;
   delete m;
+  mesh_release(owner);
+}
+
+void mesh_release(jgcm_mesh* m) {
+  if (m && m->refs.fetch_sub(1) == 1) free_mesh(m);
 }
 
 static int create_impl(int N, int ns, int nlat, int nlon, int nd, double radius, jgcm_mesh* m) {
@@ -424,7 +473,14 @@ int jgcm_mesh_create(int num_fourier, int num_spherical, int nlat, int nlon, int
 }
 
 int jgcm_mesh_destroy(jgcm_mesh_t* mesh) {
-  if (mesh) free_mesh(mesh);
+  mesh_release(mesh);   // the creator's reference
+  return JGCM_OK;
+}
+
+int jgcm_mesh_lifetime(jgcm_mesh_t* mesh, int* refs, int* live_graphs) {
+  if (int rc = check_mesh(mesh)) return rc;
+  if (refs) *refs = mesh->refs.load();
+  if (live_graphs) *live_graphs = mesh->live_graphs.load();
   return JGCM_OK;
 }
 
@@ -478,6 +534,8 @@ int jgcm_mesh_share_stream(jgcm_mesh_t* mesh, jgcm_mesh_t* owner) {
   JGCM_CHECK_HIP(hipStreamDestroy(mesh->stream));
   mesh->stream = owner->stream;
   mesh->stream_borrowed = true;
+  mesh->stream_owner = owner;   // the owner (and its stream) lives while this mesh does
+  mesh_retain(owner);
   return JGCM_OK;
 }
 

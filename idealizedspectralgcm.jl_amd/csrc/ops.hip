@@ -425,8 +425,14 @@ int jgcm_area_weighted_global_mean(jgcm_mesh_t* mesh, const double* grid, double
 }  // extern "C"
 
 // ---- graph capture -----------------------------------------------------------
+// A graph keeps a reference to every mesh whose calls it recorded (the mesh it
+// was begun on, meshes forked into the capture, meshes sharing its stream):
+// their streams, events and scratch buffers are kernel arguments or launch
+// targets of the graph, so none of them may be freed, and none of their
+// scratch may be released by growth, while the graph exists.
 struct jgcm_graph {
   jgcm_mesh* mesh = nullptr;
+  std::vector<jgcm_mesh*> used;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -450,26 +456,46 @@ int jgcm_mesh_wait(jgcm_mesh_t* waiter, jgcm_mesh_t* signaller) {
 
 int jgcm_graph_begin(jgcm_mesh_t* mesh) {
   if (int rc = check_mesh(mesh)) return rc;
+  if (capture_ctx() != nullptr) {
+    set_error("this thread is already capturing a graph");
+    return JGCM_ERR_STATE;
+  }
   JGCM_CHECK_HIP(hipStreamBeginCapture(mesh->stream, hipStreamCaptureModeThreadLocal));
+  capture_open(mesh);
   return JGCM_OK;
 }
 
 int jgcm_graph_end(jgcm_mesh_t* mesh, jgcm_graph_t** out) {
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_REQUIRE(out != nullptr, "out is NULL");
+  CaptureCtx* ctx = capture_ctx();
+  if (ctx == nullptr || ctx->origin != mesh) {
+    set_error("jgcm_graph_end: no capture was begun on this mesh by this thread");
+    return JGCM_ERR_STATE;
+  }
   jgcm_graph* g = new (std::nothrow) jgcm_graph();
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(mesh->stream, &graph);
+  std::vector<jgcm_mesh*> used;
+  capture_close(&used);
   if (!g) {
+    if (graph) (void)hipGraphDestroy(graph);
     set_error("out of host memory");
     return JGCM_ERR_NOMEM;
   }
   g->mesh = mesh;
-  hipError_t e = hipStreamEndCapture(mesh->stream, &g->graph);
+  g->graph = graph;
   if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
     set_error("graph capture failed: %s", hipGetErrorString(e));
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
     return JGCM_ERR_HIP;
+  }
+  g->used = std::move(used);
+  for (jgcm_mesh* m : g->used) {
+    mesh_retain(m);
+    m->live_graphs.fetch_add(1);
   }
   *out = g;
   return JGCM_OK;
@@ -486,7 +512,12 @@ int jgcm_graph_destroy(jgcm_graph_t* graph) {
   (void)hipStreamSynchronize(graph->mesh->stream);
   if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
   if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  std::vector<jgcm_mesh*> used = std::move(graph->used);
   delete graph;
+  for (jgcm_mesh* m : used) {
+    m->live_graphs.fetch_sub(1);
+    mesh_release(m);
+  }
   return JGCM_OK;
 }
 

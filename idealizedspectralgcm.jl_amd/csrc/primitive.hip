@@ -1317,14 +1317,18 @@ int jgcm_vert_coord_create(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
     delete v;
     return JGCM_ERR_HIP;
   }
+  mesh_retain(mesh);   // released by jgcm_vert_coord_destroy
   *out = v;
   return JGCM_OK;
 }
 
 int jgcm_vert_coord_destroy(jgcm_vert_coord_t* v) {
   if (!v) return JGCM_OK;
+  if (v->mesh) (void)hipStreamSynchronize(v->mesh->stream);
   if (v->d_coef) (void)hipFree(v->d_coef);
+  jgcm_mesh* m = v->mesh;
   delete v;
+  mesh_release(m);
   return JGCM_OK;
 }
 

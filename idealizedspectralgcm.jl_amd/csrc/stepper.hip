@@ -240,6 +240,7 @@ int jgcm_leapfrog_create(jgcm_mesh_t* mesh, double robert_coef, int damping_orde
     delete L;
     return JGCM_ERR_HIP;
   }
+  mesh_retain(mesh);   // released by jgcm_leapfrog_destroy
   *out = L;
   return JGCM_OK;
 }
@@ -248,7 +249,9 @@ int jgcm_leapfrog_destroy(jgcm_leapfrog_t* lf) {
   if (!lf) return JGCM_OK;
   if (lf->mesh) (void)hipStreamSynchronize(lf->mesh->stream);
   if (lf->d_damping) (void)hipFree(lf->d_damping);
+  jgcm_mesh* m = lf->mesh;
   delete lf;
+  mesh_release(m);
   return JGCM_OK;
 }
 

@@ -52,6 +52,7 @@ class SW_Group:
         nλ = 2 * nθ
         self.N, self.count, self.dt, self.h_0 = N, count, dt, h_0
         self.mesh = Spectral_Spherical_Mesh(N, N + 1, nθ, nλ, count, RADIUS, precision=precision)
+        self._stream_owner = owner     # the library also keeps the owner alive
         if owner is not None:
             call("jgcm_mesh_share_stream", self.mesh.handle, owner.handle)
         m = self.mesh
@@ -112,15 +113,13 @@ class SW_Ensemble:
         """`tuning`: None (built-in FFT shapes) or an autotune.Tuning_Cache; each
         truncation's FFT shape (for its instance count) is then read from the
         cache, or tuned and stored when missing, before any step is captured.
-        The truncations share the first group's stream and run one after
-        another in the captured step.  `concurrent=True` (each truncation on
-        its own stream, parallel branches of one graph forked and joined with
-        jgcm_mesh_wait) is switched off: one replay of such a graph ended in a
-        segmentation fault inside hipGraphLaunch whose cause has not been
-        found, so it raises."""
-        if concurrent:
-            raise RuntimeError("SW_Ensemble(concurrent=True) is switched off: a replay of the "
-                               "parallel-branch graph crashed in hipGraphLaunch (cause not found)")
+        By default the truncations share the first group's stream and run one
+        after another in the captured step; with `concurrent=True` each
+        truncation keeps its own stream and the captured step forks them from
+        the first group's stream and joins them back (jgcm_mesh_wait), so the
+        truncations run as parallel branches of one graph.  The graph holds a
+        library reference to every mesh it recorded, so no group's stream,
+        events or scratch can be freed under it."""
         from .autotune import Apply_Tuning
         self.groups = []
         self.tuning = {}

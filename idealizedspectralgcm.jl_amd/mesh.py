@@ -16,6 +16,7 @@ with ``nlev == nd`` or ``nlev == 1`` as the reference asserts
 (SSM:175,255,370,466,707,724).
 """
 import ctypes
+import weakref
 
 import numpy as np
 import torch
@@ -31,13 +32,19 @@ def _ptr(t):
 class _MeshTable(np.ndarray):
     """Host table that remembers its mesh, so reference-style calls that pass
     a table (e.g. ``Filtered_Leapfrog(..., mesh.laplacian_eig, ...)``) can
-    reach the device objects."""
-    mesh = None
+    reach the device objects.  The link is a weak reference: the mesh owns its
+    tables, and a strong link back would put every mesh in a reference cycle
+    that only the cyclic collector frees, in arbitrary order."""
+    _mesh_ref = None
+
+    @property
+    def mesh(self):
+        return None if self._mesh_ref is None else self._mesh_ref()
 
 
 def _with_mesh(arr, mesh):
     t = np.asarray(arr).view(_MeshTable)
-    t.mesh = mesh
+    t._mesh_ref = weakref.ref(mesh)
     return t
 
 
@@ -88,8 +95,18 @@ class Spectral_Spherical_Mesh:
         self.wave_numbers = wn                       # SSM:583-597
         # device copy of cos θ for host-side callers that need it
         self.cosθ_dev = torch.tensor(np.asarray(self.cosθ), dtype=torch.float64, device=self.device)
-        import weakref
+        self._si_ref = None
         _MESHES.append(weakref.ref(self))
+
+    @property
+    def _semi_implicit(self):
+        """The most recent Semi_Implicit_Solver built on this mesh (weakly held;
+        the default solver of Spectral_Dynamics_Physics)."""
+        return None if self._si_ref is None else self._si_ref()
+
+    @_semi_implicit.setter
+    def _semi_implicit(self, si):
+        self._si_ref = None if si is None else weakref.ref(si)
 
     # ---- tables (host copies of the reference struct's fields) ----
     def table(self, name):
@@ -127,7 +144,16 @@ class Spectral_Spherical_Mesh:
         """Grow the mesh scratch to `nfields` level-fields (needed before graph capture)."""
         call("jgcm_mesh_reserve", self._h, int(nfields))
 
+    def lifetime(self):
+        """(references, live graphs) of the library's mesh object."""
+        r, g = ctypes.c_int(), ctypes.c_int()
+        call("jgcm_mesh_lifetime", self._h, ctypes.byref(r), ctypes.byref(g))
+        return r.value, g.value
+
     def close(self):
+        """Drops this object's reference; leapfrogs, vertical coordinates,
+        graphs and meshes borrowing the stream keep the library object alive
+        until they are destroyed."""
         if getattr(self, "_h", None) is not None and self._h.value:
             call("jgcm_mesh_destroy", self._h)
             self._h = None

@@ -120,13 +120,20 @@ int jgcm_time_advance(jgcm_mesh_t* mesh, int nfields, double* const* p, double* 
  *      512.  Otherwise JGCM_ERR_UNSUPPORTED.                                  */
 int jgcm_mesh_create(int num_fourier, int num_spherical, int nlat, int nlon,
                      int nd, double radius, jgcm_mesh_t** out);
+/* Drops the caller's reference.  Leapfrogs, vertical coordinates, graphs and
+ * meshes that borrow this mesh's stream hold references of their own, so the
+ * mesh is freed after the last of them is destroyed, in any order.          */
 int jgcm_mesh_destroy(jgcm_mesh_t* mesh);
+/* Diagnostics (no reference counterpart): the mesh's reference count and the
+ * number of live graphs that recorded calls on it.                          */
+int jgcm_mesh_lifetime(jgcm_mesh_t* mesh, int* refs, int* live_graphs);
 int jgcm_mesh_synchronize(jgcm_mesh_t* mesh);
 /* hipStream_t of the mesh, returned as void*. */
 void* jgcm_mesh_stream(jgcm_mesh_t* mesh);
 /* Make `mesh` enqueue on `owner`'s stream (so several meshes, e.g. the
- * truncations of a mixed ensemble, can be captured into one graph).  The
- * owner must outlive the mesh.  No reference counterpart.                   */
+ * truncations of a mixed ensemble, can be captured into one graph).  `mesh`
+ * keeps a reference to `owner`, whose stream it uses.  No reference
+ * counterpart.                                                              */
 int jgcm_mesh_share_stream(jgcm_mesh_t* mesh, jgcm_mesh_t* owner);
 int jgcm_mesh_set_precision(jgcm_mesh_t* mesh, int precision);
 int jgcm_mesh_get_precision(jgcm_mesh_t* mesh, int* precision);
@@ -580,8 +587,11 @@ int jgcm_zonal_mean_accumulate(jgcm_mesh_t* mesh, const double* grid_t, const do
  * No reference counterpart (the reference has no device).  Capture records
  * every kernel the library enqueues on the mesh stream between begin and end;
  * replays re-run them with the argument values of capture time.  Scratch must
- * already be large enough (jgcm_mesh_reserve) and no synchronising call (e.g.
- * jgcm_area_weighted_global_mean) may be made while capturing.              */
+ * already be large enough (jgcm_mesh_reserve): growth during a capture fails
+ * with JGCM_ERR_STATE, and growth while graphs exist keeps their buffers
+ * until the mesh is freed.  No synchronising call (e.g.
+ * jgcm_area_weighted_global_mean) may be made while capturing.  A graph holds
+ * a reference to every mesh whose calls it recorded.                        */
 int jgcm_mesh_reserve(jgcm_mesh_t* mesh, int nfields);
 int jgcm_graph_begin(jgcm_mesh_t* mesh);
 int jgcm_graph_end(jgcm_mesh_t* mesh, jgcm_graph_t** out);

@@ -51,18 +51,21 @@ FIELDS = ["spe_vor_c", "spe_div_c", "spe_t_c", "spe_lnps_c", "grid_u_c", "grid_v
 
 
 # The divergence starts at zero and stays small: it is formed from terms that
-# nearly cancel, so its error relative to its own maximum is the loosest of
-# the fields (measured 5e-11 to 2.5e-10 over the first four steps at T21 for
-# even sigma, mcm and v197, without growth; T, ps and ln ps agree to 1e-15 and
-# vorticity to 1e-11 in the same runs).
-DIV_TOL = 5e-10
+# nearly cancel (-∇²(Φ + KE) against the pressure-gradient and vorticity-flux
+# divergences), so its error relative to its own maximum is the loosest of
+# the fields and grows with the truncation, as ∇² weights the rounding of the
+# O(1e5) geopotential by n(n+1)/a² (measured 5e-11 to 2.5e-10 over the first
+# four steps at T21 for even sigma, mcm and v197, without growth, and 5.2e-10
+# after three steps at T85 L20; T, ps and ln ps agree to 1e-15 and vorticity
+# to 1e-11 in the same runs).
+DIV_TOL = 1e-9
 
 
 def compare(d, dd, tol, fields=FIELDS):
     torch.cuda.synchronize()
-    for f in fields:
-        e = rel(getattr(dd, f).cpu().numpy(), getattr(d, f))
-        assert e < (max(tol, DIV_TOL) if f == "spe_div_c" else tol), (f, e)
+    errs = {f: rel(getattr(dd, f).cpu().numpy(), getattr(d, f)) for f in fields}
+    for f, e in errs.items():
+        assert e < (max(tol, DIV_TOL) if f == "spe_div_c" else tol), (f, errs)
 
 
 @pytest.mark.parametrize("coord,nd", [("even_sigma", 10), ("mcm", 14), ("v197", 18)])
@@ -87,6 +90,7 @@ def test_initial_state_matches_oracle(jgcm):
 
 @pytest.mark.parametrize("N,nlat,nd,dt,steps,coord", [
     (21, 32, 10, 600, 4, "even_sigma"), (42, 64, 20, 600, 2, "even_sigma"),
+    (85, 128, 20, 300, 3, "even_sigma"),
     (31, 48, 10, 600, 2, "even_sigma"), (21, 32, 14, 600, 3, "mcm"), (21, 32, 18, 600, 3, "v197")])
 def test_hs_steps_match_oracle(jgcm, N, nlat, nd, dt, steps, coord):
     (M, vc, atmo, si, d), (dev, dvc, datmo, dsi, dd) = build(jgcm, N, nlat, nd, dt, coord=coord)

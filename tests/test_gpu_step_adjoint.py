@@ -393,6 +393,35 @@ def test_run_autograd_function_equals_reverse_sweep(jgcm):
     assert torch.equal(vc.grad, gc) and torch.equal(vp.grad, gp)
 
 
+def test_two_runs_on_one_adjoint_keep_their_own_trajectories(jgcm):
+    """Two Run.apply calls on the same adjoint object, then backward of the
+    first: each apply's gradient is that of its own trajectory (the adjoint
+    object keeps only its latest run; Run keeps each apply's checkpoints)."""
+    from jgcm_amd.adjoint import Run
+    N, nlat = 21, 32
+    mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 1, A)
+    atmo = jgcm.Atmo_Data("b", 2 * nlat, nlat, 1, False, False, False, False, mesh.sinθ, A, OMEGA)
+    I = jgcm.Filtered_Leapfrog(0.04, 4, 1.0 / (0.1 * 86400), mesh.laplacian_eig, 0.0, 1800, True, 0,
+                               864000)
+    rng = np.random.default_rng(73)
+    c0 = rh4_state(jgcm, mesh)
+    c1 = c0 + 1e-6 * dev(rspec(N, 1, rng, valid=True))
+    W = dev(rspec(N, 1, rng, valid=True))
+    adj = jgcm.Barotropic_Adjoint(mesh, atmo, I, checkpoint_every=2)
+    runs = []
+    for c, K in ((c0, 5), (c1, 3)):
+        vp = c.clone().requires_grad_(True)
+        vc = c.clone().requires_grad_(True)
+        pK, cK = Run.apply(adj, K, vp, vc)
+        runs.append((c, K, vp, vc, (W.conj() * cK).real.sum()))
+    for c, K, vp, vc, loss in runs:          # the first run's backward comes after both applies
+        loss.backward()
+        ref = jgcm.Barotropic_Adjoint(mesh, atmo, I, checkpoint_every=2)
+        ref.forward(c, c, K)
+        gp, gc = ref.backward(torch.zeros_like(W), W)
+        assert torch.equal(vc.grad, gc) and torch.equal(vp.grad, gp), K
+
+
 @pytest.mark.parametrize("prec", ["split32", "bf16x3"])
 def test_gradient_with_reduced_precision_legendre(jgcm, prec):
     """The reverse sweep on a split-FP32 / BF16x3 Legendre mesh stays within
