@@ -60,6 +60,7 @@ SIGNATURES = {
     "jgcm_mesh_table_size": [P, I, ctypes.POINTER(SZ)],
     "jgcm_mesh_get_table": [P, I, P, SZ],
     "jgcm_trans_grid_to_spherical": [P, P, P, I],
+    "jgcm_trans_grid_to_spherical_noalloc": [P, P, P, I],
     "jgcm_trans_spherical_to_grid": [P, P, P, I],
     "jgcm_trans_spherical_to_grid_batch": [P, I, ctypes.POINTER(P), ctypes.POINTER(P),
                                            ctypes.POINTER(P), ctypes.POINTER(I),

@@ -27,6 +27,8 @@
 //             contraction steps the products P_hi x_hi and the cross terms
 //             P_hi x_lo + P_lo x_hi are summed by FP32 MFMA, the block sums
 //             are accumulated in FP64 (compensated summation).
+#include <mutex>
+
 #include "fft.hpp"
 #include "jgcm_internal.hpp"
 
@@ -1447,6 +1449,68 @@ int jgcm_trans_spherical_to_grid(jgcm_mesh_t* mesh, const double* snm, double* p
   double2* eo = reinterpret_cast<double2*>(mesh->d_four_a);
   if (int rc = legendre_inv(mesh, reinterpret_cast<const double2*>(snm), eo, nlev)) return rc;
   return ifft_unfold(mesh, eo, pfield, nullptr, nlev);
+}
+
+}  // extern "C"
+
+// ---- allocation-free forward transform ---------------------------------------
+// The E/O Fourier block between the FFT and the Legendre stage of
+// jgcm_trans_grid_to_spherical_noalloc lives in this file-scope device array
+// (each GPU holds its own copy for the life of the process).  Host code never
+// passes `scratch` itself to a kernel: its device address on the current GPU is
+// looked up once per device with hipGetSymbolAddress and kept in
+// g_noalloc_ptr.  Every call on one GPU shares the array: calls that overlap
+// on one GPU (different streams, or several host threads) race.  Nothing in
+// it needs clearing: the FFT stage writes every element the Legendre stage
+// reads.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;   // 16 MiB
+static __device__ __align__(256) double2 scratch[NOALLOC_SCRATCH_BYTES / sizeof(double2)];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static double2* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the mesh's GPU (which must be current).
+static int noalloc_scratch(const jgcm_mesh* mesh, double2** out) {
+  int dev = -1;
+  JGCM_CHECK_HIP(hipGetDevice(&dev));
+  if (dev != mesh->device) {
+    set_error("the mesh lives on device %d but device %d is current", mesh->device, dev);
+    return JGCM_ERR_STATE;
+  }
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_error("device %d is beyond the %d devices the static scratch supports", dev,
+              NOALLOC_MAX_DEVICES);
+    return JGCM_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    JGCM_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_noalloc_ptr[dev] = static_cast<double2*>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return JGCM_OK;
+}
+
+extern "C" {
+
+int jgcm_trans_grid_to_spherical_noalloc(jgcm_mesh_t* mesh, const double* pfield, double* snm,
+                                         int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(pfield && snm, "NULL field pointer");
+  const size_t per_field = mesh->four_elems() * sizeof(double2);
+  const size_t max_fields = NOALLOC_SCRATCH_BYTES / per_field;
+  JGCM_REQUIRE((size_t)nlev <= max_fields,
+               "nlev = %d level-fields need %zu bytes of staging; the static scratch holds %zu "
+               "bytes, at most %zu level-fields at T%d on %d latitudes",
+               nlev, (size_t)nlev * per_field, NOALLOC_SCRATCH_BYTES, max_fields, mesh->N,
+               mesh->nlat);
+  double2* eo = nullptr;
+  if (int rc = noalloc_scratch(mesh, &eo)) return rc;
+  if (int rc = fft_fold(mesh, pfield, eo, nullptr, nlev)) return rc;
+  return legendre_fwd_plain(mesh, eo, reinterpret_cast<double2*>(snm), nlev);
 }
 
 int jgcm_trans_grid_to_fourier(jgcm_mesh_t* mesh, const double* pfield, double* fourier_g, int nlev) {

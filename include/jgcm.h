@@ -178,6 +178,16 @@ int jgcm_mesh_get_table(jgcm_mesh_t* mesh, int which, double* host_out, size_t c
  *   degree num_spherical (N+1) row set to 0, lower triangle (n<m) untouched. */
 int jgcm_trans_grid_to_spherical(jgcm_mesh_t* mesh, const double* pfield,
                                  double* snm, int nlev);
+/* Trans_Grid_To_Spherical! without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results (bit for bit) as jgcm_trans_grid_to_spherical;
+ * the FFT -> Legendre staging lives in a static 16 MiB device array (at most
+ * 95 level-fields at T85 on 128 latitudes, 381 at T42; more fails with
+ * JGCM_ERR_ARG).  All calls on one GPU share that array: calls that overlap
+ * on one GPU race.  The mesh's device must be current.                     */
+int jgcm_trans_grid_to_spherical_noalloc(jgcm_mesh_t* mesh, const double* pfield, double* snm,
+                                         int nlev);
 /* Trans_Spherical_To_Grid!(mesh, snm, pfield)   Spectral_Spherical_Mesh.jl:143-217
  *   inverse: sums degrees m..N+1, g_0 halved, pfield = 2nλ Re(ifft(g)).      */
 int jgcm_trans_spherical_to_grid(jgcm_mesh_t* mesh, const double* snm,
