@@ -133,6 +133,7 @@ SIGNATURES = {
     "jgcm_hs_energy_implicit_leapfrog": [P] * 18,
     "jgcm_hs_physics": [P, P, ctypes.c_int64] + [P] * 15 + [I],
     "jgcm_hs_column_dynamics": [P] * 27,
+    "jgcm_hs_physics_column_dynamics": [P, P, ctypes.c_int64] + [P] * 27,
     "jgcm_hs_corrections": [P] * 8,
     "jgcm_hs_corrections_advance": [P, P, P, P, P, P],
     "jgcm_hs_scalars": [P, P, I],

@@ -181,6 +181,24 @@ struct PressTile {
 
 template <int KPT>
 __device__ __forceinline__ void pressure_tile(const VC& v, int k0, double ps, PressTile<KPT>& r) {
+  if (v.pure) {
+    // pure sigma: one logarithm per column, the rest per-level constants
+    const double lps = log(ps);
+#pragma unroll
+    for (int i = 0; i <= KPT; ++i) {
+      int k = min(k0 + i, v.nd);
+      r.ph[i] = v.ak[k] + v.bk[k] * ps;
+      r.lnh[i] = (v.zero_top && k == 0) ? 0.0 : v.lnb[k] + lps;
+    }
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      int k = min(k0 + i, v.nd - 1);
+      r.dp[i] = r.ph[i + 1] - r.ph[i];
+      r.lnf[i] = v.cf[k] + lps;
+      r.pf[i] = v.sf[k] * ps;
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i <= KPT; ++i) {
     int k = min(k0 + i, v.nd);
@@ -291,7 +309,9 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
   const double ps = t.col ? ps_c[t.c] : 1.0e5;
   PressTile<KPT> pr;
   pressure_tile<KPT>(v, t.k0, ps, pr);
-  store_pressure<KPT>(v, t, P, pr, p_half, dp, lnp_half, p_full, lnp_full);
+  // the pressure variables are stored only when asked for (the step's
+  // column pass derives them from ps itself)
+  if (p_half) store_pressure<KPT>(v, t, P, pr, p_half, dp, lnp_half, p_full, lnp_full);
   if (t.col && t.q == 0) dps[t.c] = 0.0;
 
   double U[KPT], V[KPT], T[KPT], DU[KPT], DV[KPT], DT[KPT];
@@ -311,6 +331,7 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
   if (forcing) {
     const double day = 86400;
     const double t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
+    const double ln_pref = 11.512925464970229;   // ln(1e5)
     const double kf = hp.k_f / day, ka = hp.k_a / day, ks = hp.k_s / day;
     const double sj = sint[j];
     const double s2 = sj * sj, c2 = 1.0 - s2, c4 = c2 * c2;
@@ -327,7 +348,11 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
       DV[i] = b;
       double d = -((U[i] + 0.5 * a * dt) * a + (V[i] + 0.5 * b * dt) * b) / cp_air;
       double pn = pr.pf[i] / p_ref;
-      double te = (t_zero - hp.dT_y * s2 - hp.dth_z * c2 * log(pn)) * pow(pn, kappa);
+      // ln(p/p_ref) and (p/p_ref)^κ; on pure sigma ln p_full is ln ps plus a
+      // level constant, so no further logarithm is needed
+      double lpn = v.pure ? pr.lnf[i] - ln_pref : log(pn);
+      double te = (t_zero - hp.dT_y * s2 - hp.dth_z * c2 * lpn) *
+                  (v.pure ? exp(kappa * lpn) : pow(pn, kappa));
       double teq = fmax(t_strat, te);
       double kt = ka + (ks - ka) / (1.0 - hp.sigma_b) * fmax(0.0, s - hp.sigma_b) * c4;
       DT[i] = d - kt * (T[i] - teq);
@@ -366,16 +391,44 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
 // Vert_Advection! (Vert_Coordinate.jl:171-210) of u, v, T added to the
 // tendencies, absolute-vorticity flux (Compute_Abs_Vor!, SD:296-303) and
 // energy = Φ + KE (SD:306).
-template <int KPT>
-__global__ __launch_bounds__(512) void hs_column_dynamics_kernel(
-    VC v, int nlat, int nlon, double rdgas, double cp_air, double two_omega,
-    const double* __restrict__ sint, const double* div, const double* u, const double* vv,
-    const double* ps, const double* dp, const double* lnp_half, const double* lnp_full,
-    const double* p_full, double* dlps, double* dtps, const double* t, const double* vor,
-    const double* geopots, double* M_half, double* w_full, double* du, double* dv, double* dps,
-    double* dtt, double* g_half, double* g_full, double* dlnps, double* absvor, double* energy,
-    const double* dl_t, const double* dt_t) {
-  __shared__ double2 red[MAXNQ * TILE];
+// Arguments of the physics pass when it runs inside the column pass
+// (hs_physics_column_dynamics_kernel): HS_Forcing! on the p-fields and the
+// partial sums of Compute_Corrections_Init.
+struct PhysArgs {
+  const double* wts;
+  double kappa, cp_air, dt;
+  HSParams hp;
+  const double* u_p;
+  const double* v_p;
+  const double* t_p;
+  const double* ps_p;
+  double* t_eq;
+  double2* partial;
+  int mass, energy;
+  int* bad;
+};
+
+#define JGCM_COLUMN_DYNAMICS_ARGS                                                            \
+    VC v, int nlat, int nlon, double rdgas, double cp_air, double two_omega,                  \
+    const double* __restrict__ sint, const double* div, const double* u, const double* vv,    \
+    const double* ps, const double* dp, const double* lnp_half, const double* lnp_full,       \
+    const double* p_full, double* dlps, double* dtps, const double* t, const double* vor,     \
+    const double* geopots, double* M_half, double* w_full, double* du, double* dv, double* dps, \
+    double* dtt, double* g_half, double* g_full, double* dlnps, double* absvor, double* energy, \
+    const double* dl_t, const double* dt_t
+#define JGCM_COLUMN_DYNAMICS_PASS                                                            \
+    v, nlat, nlon, rdgas, cp_air, two_omega, sint, div, u, vv, ps, dp, lnp_half, lnp_full,     \
+    p_full, dlps, dtps, t, vor, geopots, M_half, w_full, du, dv, dps, dtt, g_half, g_full,     \
+    dlnps, absvor, energy, dl_t, dt_t
+
+// OWN_P: the pressure variables are evaluated from ps here instead of read.
+// PHYS: the physics pass (HS forcing of the p-fields, δps = 0, the
+// correction partials) runs first in the same tile, its tendencies kept in
+// registers instead of going through du, dv, dtt and dps (requires OWN_P).
+template <int KPT, bool OWN_P, bool PHYS>
+__device__ __forceinline__ void hs_column_dynamics_body(double2* red, const PhysArgs& pa,
+                                                        JGCM_COLUMN_DYNAMICS_ARGS) {
+  static_assert(OWN_P || !PHYS, "the fused physics pass evaluates the pressures itself");
   const size_t P = (size_t)nlat * nlon;
   const TileIdx tx = tile_idx<KPT>(P);
   const int nd = v.nd;
@@ -385,9 +438,15 @@ __global__ __launch_bounds__(512) void hs_column_dynamics_kernel(
   const double p = tx.col ? ps[c] : 1.0;
   const double a = tx.col ? dlps[c] * p : 0.0;
   const double b = tx.col ? dtps[c] * p : 0.0;
-  const double ps_tend = tx.col ? dps[c] : 0.0;
+  const double ps_tend = (!PHYS && tx.col) ? dps[c] : 0.0;   // the physics pass sets δps = 0
   const double gs = tx.col ? geopots[c] : 0.0;
   const double f = two_omega * sint[tx.col ? (int)(c / nlon) : 0];
+  // Pressure_Variables! of ps (SD:256): read from the given fields, or, when
+  // they are NULL, evaluated here with the arithmetic of the physics pass
+  // (bit-identical to the stored values)
+  constexpr bool own_p = OWN_P;
+  PressTile<KPT> pr;
+  if (own_p) pressure_tile<KPT>(v, k0, p, pr);
   // levels k0-1 .. k0+KPT of u, v, T, Δp (index i+1 <-> level k0+i)
   double U[KPT + 2], V[KPT + 2], T[KPT + 2], D[KPT + 2];
 #pragma unroll
@@ -398,25 +457,94 @@ __global__ __launch_bounds__(512) void hs_column_dynamics_kernel(
     U[i] = ok ? u[o] : 0.0;
     V[i] = ok ? vv[o] : 0.0;
     T[i] = ok ? t[o] : 0.0;
-    D[i] = ok ? dp[o] : 1.0;
+    if (own_p) {
+      int kk = ok ? k : 0;
+      D[i] = ok ? (v.ak[kk + 1] + v.bk[kk + 1] * p) - (v.ak[kk] + v.bk[kk] * p) : 1.0;
+    } else {
+      D[i] = ok ? dp[o] : 1.0;
+    }
   }
   double LH[KPT + 1], LF[KPT], PF[KPT], DIV[KPT], DU[KPT], DV[KPT], DT[KPT], VOR[KPT];
 #pragma unroll
   for (int i = 0; i <= KPT; ++i) {
     int k = min(k0 + i, nd);
-    LH[i] = tx.col ? lnp_half[(size_t)k * P + c] : 0.0;
+    LH[i] = tx.col ? (own_p ? pr.lnh[i] : lnp_half[(size_t)k * P + c]) : 0.0;
   }
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
     bool ok = tx.col && k0 + i < nd;
     size_t o = (size_t)(k0 + i) * P + c;
-    LF[i] = ok ? lnp_full[o] : 0.0;
-    PF[i] = ok ? p_full[o] : 0.0;
+    LF[i] = ok ? (own_p ? pr.lnf[i] : lnp_full[o]) : 0.0;
+    PF[i] = ok ? (own_p ? pr.pf[i] : p_full[o]) : 0.0;
     DIV[i] = ok ? div[o] : 0.0;
-    DU[i] = ok ? du[o] : 0.0;
-    DV[i] = ok ? dv[o] : 0.0;
-    DT[i] = ok ? dtt[o] : 0.0;
+    if constexpr (!PHYS) {
+      DU[i] = ok ? du[o] : 0.0;
+      DV[i] = ok ? dv[o] : 0.0;
+      DT[i] = ok ? dtt[o] : 0.0;
+    }
     VOR[i] = ok ? vor[o] : 0.0;
+  }
+  if constexpr (PHYS) {
+    // Spectral_Dynamics_Physics! (SD:452-467): HS_Forcing! (HS_Forcing.jl:1-130)
+    // on the p-fields with this column's pressures, the same arithmetic as
+    // hs_physics_kernel; then the Compute_Corrections_Init partials (SD:5-28)
+    double Up[KPT], Vp[KPT], Tp[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      bool ok = tx.col && k0 + i < nd;
+      size_t o = (size_t)(k0 + i) * P + c;
+      Up[i] = ok ? pa.u_p[o] : 0.0;
+      Vp[i] = ok ? pa.v_p[o] : 0.0;
+      Tp[i] = ok ? pa.t_p[o] : 0.0;
+    }
+    const HSParams& hp = pa.hp;
+    const double day = 86400;
+    const double t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
+    const double ln_pref = 11.512925464970229;   // ln(1e5)
+    const double kf = hp.k_f / day, ka = hp.k_a / day, ks = hp.k_s / day;
+    const double sj = sint[tx.col ? (int)(c / nlon) : 0];
+    const double s2 = sj * sj, c2 = 1.0 - s2, c4 = c2 * c2;
+    const double psb = v.ak[nd] + v.bk[nd] * p;  // p_half[nd]
+    const double dt = pa.dt;
+    int isbad = 0;
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      bool ok = tx.col && k0 + i < nd;
+      double sg = pr.pf[i] / psb;
+      if (ok && (sg - hp.sigma_b) / (1.0 - hp.sigma_b) > 1.0) isbad = 1;
+      double kv = kf / (1.0 - hp.sigma_b) * fmax(0.0, sg - hp.sigma_b);
+      double fa = -kv * Up[i], fb = -kv * Vp[i];
+      DU[i] = fa;
+      DV[i] = fb;
+      double dh = -((Up[i] + 0.5 * fa * dt) * fa + (Vp[i] + 0.5 * fb * dt) * fb) / pa.cp_air;
+      double pn = pr.pf[i] / p_ref;
+      double lpn = v.pure ? pr.lnf[i] - ln_pref : log(pn);
+      double te = (t_zero - hp.dT_y * s2 - hp.dth_z * c2 * lpn) *
+                  (v.pure ? exp(pa.kappa * lpn) : pow(pn, pa.kappa));
+      double teq = fmax(t_strat, te);
+      double kt = ka + (ks - ka) / (1.0 - hp.sigma_b) * fmax(0.0, sg - hp.sigma_b) * c4;
+      DT[i] = dh - kt * (Tp[i] - teq);
+      if (ok) pa.t_eq[(size_t)(k0 + i) * P + c] = teq;
+    }
+    if (isbad) atomicAdd(pa.bad, 1);
+    if (pa.mass || pa.energy) {
+      double part = 0.0;
+      const double psp = tx.col ? pa.ps_p[c] : 0.0;
+      if (pa.energy) {
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) {
+          int k = k0 + i;
+          if (tx.col && k < nd) {
+            double ea = Up[i] + dt * DU[i], eb = Vp[i] + dt * DV[i];
+            double e = 0.5 * (ea * ea + eb * eb) + pa.cp_air * (Tp[i] + dt * DT[i]);
+            part = part + e * (v.dak[k] + v.dbk[k] * psp);
+          }
+        }
+      }
+      double2 ab, bl;
+      level_sums(red, tx, d2(part, 0.0), ab, bl);
+      block_area_partial(tx, nlon, pa.wts, psp, part + bl.x, pa.partial);
+    }
   }
 
   // Four_In_One!: pressure-gradient force, mass-flux divergence (prefix sum
@@ -518,6 +646,25 @@ __global__ __launch_bounds__(512) void hs_column_dynamics_kernel(
     energy[o] = gfull[i] + 0.5 * (uk * uk + vk * vk);
     if (k + 1 == nd) g_half[(size_t)nd * P + c] = ghalf[i + 1];
   }
+}
+
+template <int KPT>
+__global__ __launch_bounds__(512) void hs_column_dynamics_kernel(JGCM_COLUMN_DYNAMICS_ARGS) {
+  __shared__ double2 red[MAXNQ * TILE];
+  hs_column_dynamics_body<KPT, false, false>(red, PhysArgs{}, JGCM_COLUMN_DYNAMICS_PASS);
+}
+
+template <int KPT>
+__global__ __launch_bounds__(512) void hs_column_dynamics_own_p_kernel(JGCM_COLUMN_DYNAMICS_ARGS) {
+  __shared__ double2 red[MAXNQ * TILE];
+  hs_column_dynamics_body<KPT, true, false>(red, PhysArgs{}, JGCM_COLUMN_DYNAMICS_PASS);
+}
+
+template <int KPT>
+__global__ __launch_bounds__(512) void hs_physics_column_dynamics_kernel(
+    PhysArgs pa, JGCM_COLUMN_DYNAMICS_ARGS) {
+  __shared__ double2 red[MAXNQ * TILE];
+  hs_column_dynamics_body<KPT, true, true>(red, pa, JGCM_COLUMN_DYNAMICS_PASS);
 }
 
 // Compute_Corrections! sums (SD:30-68) in one pass: ps_n = exp(ln ps) (SD:336)
@@ -1305,11 +1452,31 @@ int jgcm_vert_coord_create(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
     v->dak[k] = v->ak[k + 1] - v->ak[k];
     v->dbk[k] = v->bk[k + 1] - v->bk[k];
   }
+  // pure sigma constants: ln p_half_k = ln b_k + ln ps, and the
+  // Simmons-Burridge ln p_full_k = ln p_half_{k+1} + p_half_k (ln p_half_{k+1}
+  // - ln p_half_k) / Δp_k - 1 = ln ps + cf_k (p_half_k / Δp_k = b_k / Δb_k)
+  bool pure = v->zero_top || v->bk[0] > 0.0;
+  for (int k = 0; k <= nd; ++k) pure = pure && v->ak[k] == 0.0;
+  v->pure_sigma = pure ? 1 : 0;
+  std::vector<double> lnb(nd + 1, 0.0), cf(nd, 0.0), sf(nd, 0.0);
+  if (pure) {
+    for (int k = 0; k <= nd; ++k) lnb[k] = v->bk[k] > 0.0 ? std::log(v->bk[k]) : 0.0;
+    for (int k = 0; k < nd; ++k) {
+      if (v->zero_top && k == 0)
+        cf[k] = lnb[1] - 1.0;   // :57-60
+      else
+        cf[k] = lnb[k + 1] + v->bk[k] * (lnb[k + 1] - lnb[k]) / v->dbk[k] - 1.0;
+      sf[k] = std::exp(cf[k]);
+    }
+  }
   std::vector<double> all;
   all.insert(all.end(), v->ak.begin(), v->ak.end());
   all.insert(all.end(), v->bk.begin(), v->bk.end());
   all.insert(all.end(), v->dak.begin(), v->dak.end());
   all.insert(all.end(), v->dbk.begin(), v->dbk.end());
+  all.insert(all.end(), lnb.begin(), lnb.end());
+  all.insert(all.end(), cf.begin(), cf.end());
+  all.insert(all.end(), sf.begin(), sf.end());
   hipError_t e = hipMalloc(&v->d_coef, all.size() * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(v->d_coef, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -1583,6 +1750,12 @@ int jgcm_hs_physics(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t 
                     double* grid_dv, double* grid_t_eq, double* grid_dt, double* grid_dps,
                     int do_forcing) {
   JGCM_REQUIRE(si && hp, "NULL argument");
+  {
+    const int given = (p_half != nullptr) + (dp != nullptr) + (lnp_half != nullptr) +
+                      (p_full != nullptr) + (lnp_full != nullptr);
+    JGCM_REQUIRE(given == 0 || given == 5,
+                 "the five pressure variables must all be given or all be NULL");
+  }
   jgcm_mesh* M = si->vc->mesh;
   size_t P = M->grid_elems();
   HSParams h{hp->sigma_b, hp->k_f, hp->k_a, hp->k_s, hp->delta_T_y, hp->delta_theta_z};
@@ -1611,15 +1784,68 @@ int jgcm_hs_column_dynamics(jgcm_semi_implicit_t* si, const double* grid_div, co
   JGCM_REQUIRE(si, "NULL argument");
   JGCM_REQUIRE((grid_dlambda_t == nullptr) == (grid_dtheta_t == nullptr),
                "grid_dlambda_t and grid_dtheta_t must both be given or both be NULL");
+  {
+    const int given = (dp != nullptr) + (lnp_half != nullptr) + (lnp_full != nullptr) +
+                      (p_full != nullptr);
+    JGCM_REQUIRE(given == 0 || given == 4,
+                 "dp, lnp_half, lnp_full and p_full must all be given or all be NULL");
+  }
   jgcm_mesh* M = si->vc->mesh;
   size_t P = M->grid_elems();
   int nd = si->nd;
-  JGCM_TILE_DISPATCH(nd, hs_column_dynamics_kernel, tile_grid(P), tile_block(nd), 0, M->stream,
-                     vc_of(si->vc), M->nlat, M->nlon, si->atmo.rdgas, si->atmo.cp_air,
-                     2 * si->atmo.omega, M->d_sin, grid_div, grid_u, grid_v, grid_ps, dp, lnp_half,
-                     lnp_full, p_full, grid_dlambda_ps, grid_dtheta_ps, grid_t, grid_vor,
-                     grid_geopots, grid_M_half, grid_w_full, grid_du, grid_dv, grid_dps, grid_dt,
-                     geopot_half, geopot_full, grid_dlnps, grid_absvor, grid_energy,
+#define JGCM_COLUMN_DYNAMICS_LAUNCH(KERNEL)                                                    \
+  JGCM_TILE_DISPATCH(nd, KERNEL, tile_grid(P), tile_block(nd), 0, M->stream, vc_of(si->vc),     \
+                     M->nlat, M->nlon, si->atmo.rdgas, si->atmo.cp_air, 2 * si->atmo.omega,     \
+                     M->d_sin, grid_div, grid_u, grid_v, grid_ps, dp, lnp_half, lnp_full, p_full, \
+                     grid_dlambda_ps, grid_dtheta_ps, grid_t, grid_vor, grid_geopots,           \
+                     grid_M_half, grid_w_full, grid_du, grid_dv, grid_dps, grid_dt, geopot_half, \
+                     geopot_full, grid_dlnps, grid_absvor, grid_energy, grid_dlambda_t,         \
+                     grid_dtheta_t)
+  if (dp == nullptr) {
+    JGCM_COLUMN_DYNAMICS_LAUNCH(hs_column_dynamics_own_p_kernel);
+  } else {
+    JGCM_COLUMN_DYNAMICS_LAUNCH(hs_column_dynamics_kernel);
+  }
+#undef JGCM_COLUMN_DYNAMICS_LAUNCH
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+// Spectral_Dynamics_Physics! and the column part of Spectral_Dynamics! in one
+// pass (the physics tendencies stay in registers); see jgcm.h
+int jgcm_hs_physics_column_dynamics(
+    jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t dt, const double* grid_u_p,
+    const double* grid_v_p, const double* grid_t_p, const double* grid_ps_p, double* grid_t_eq,
+    const double* grid_div, const double* grid_u, const double* grid_v, const double* grid_ps,
+    double* grid_dlambda_ps, double* grid_dtheta_ps, const double* grid_t, const double* grid_vor,
+    const double* grid_geopots, double* grid_M_half, double* grid_w_full, double* grid_du,
+    double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
+    double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
+    const double* grid_dtheta_t) {
+  JGCM_REQUIRE(si && hp, "NULL argument");
+  JGCM_REQUIRE(grid_u_p && grid_v_p && grid_t_p && grid_ps_p && grid_t_eq && grid_div && grid_u &&
+                   grid_v && grid_ps && grid_dlambda_ps && grid_dtheta_ps && grid_t && grid_vor &&
+                   grid_geopots && grid_M_half && grid_w_full && grid_du && grid_dv && grid_dps &&
+                   grid_dt && geopot_half && geopot_full && grid_dlnps && grid_absvor &&
+                   grid_energy,
+               "NULL field pointer");
+  JGCM_REQUIRE((grid_dlambda_t == nullptr) == (grid_dtheta_t == nullptr),
+               "grid_dlambda_t and grid_dtheta_t must both be given or both be NULL");
+  jgcm_mesh* M = si->vc->mesh;
+  size_t P = M->grid_elems();
+  int nd = si->nd;
+  PhysArgs pa{M->d_wts, si->atmo.kappa, si->atmo.cp_air, (double)dt,
+              HSParams{hp->sigma_b, hp->k_f, hp->k_a, hp->k_s, hp->delta_T_y, hp->delta_theta_z},
+              grid_u_p, grid_v_p, grid_t_p, grid_ps_p, grid_t_eq, area_partials(si),
+              si->atmo.do_mass_correction, si->atmo.do_energy_correction,
+              reinterpret_cast<int*>(si->d_scal + 15)};
+  const double* none = nullptr;
+  JGCM_TILE_DISPATCH(nd, hs_physics_column_dynamics_kernel, tile_grid(P), tile_block(nd), 0,
+                     M->stream, pa, vc_of(si->vc), M->nlat, M->nlon, si->atmo.rdgas,
+                     si->atmo.cp_air, 2 * si->atmo.omega, M->d_sin, grid_div, grid_u, grid_v,
+                     grid_ps, none, none, none, none, grid_dlambda_ps, grid_dtheta_ps, grid_t,
+                     grid_vor, grid_geopots, grid_M_half, grid_w_full, grid_du, grid_dv, grid_dps,
+                     grid_dt, geopot_half, geopot_full, grid_dlnps, grid_absvor, grid_energy,
                      grid_dlambda_t, grid_dtheta_t);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;

@@ -13,7 +13,12 @@ struct jgcm_vert_coord {
   int zero_top = 1;
   double p_ref = 101325.0;
   std::vector<double> ak, bk, dak, dbk;
-  double* d_coef = nullptr;  // [ak(nd+1) | bk(nd+1) | dak(nd) | dbk(nd)]
+  // [ak(nd+1) | bk(nd+1) | dak(nd) | dbk(nd) | lnb(nd+1) | cf(nd) | sf(nd)]
+  double* d_coef = nullptr;
+  // pure sigma (every a_k = 0, the top half level usable): the pressure
+  // variables are ps times per-level constants and ln ps plus per-level
+  // constants (lnb, cf, sf below), so a column needs one logarithm
+  int pure_sigma = 0;
 };
 
 struct jgcm_semi_implicit {
@@ -36,18 +41,26 @@ namespace jgcm {
 
 constexpr int MAXND = 64;
 
-// device view of a vertical coordinate: ak, bk (nd+1), Δak, Δbk (nd)
+// device view of a vertical coordinate: ak, bk (nd+1), Δak, Δbk (nd); for a
+// pure sigma coordinate also lnb = ln b_k (nd+1; 0 where b_k = 0),
+// cf_k = ln p_full_k - ln ps and sf_k = p_full_k / ps (nd), the
+// Simmons-Burridge full-level constants (Press_And_Geopot.jl:55-60)
 struct VC {
-  int nd, zero_top, scheme;
+  int nd, zero_top, scheme, pure;
   const double* ak;
   const double* bk;
   const double* dak;
   const double* dbk;
+  const double* lnb;
+  const double* cf;
+  const double* sf;
 };
 
 inline VC vc_of(const jgcm_vert_coord* v) {
-  return VC{v->nd, v->zero_top, v->advect_scheme, v->d_coef, v->d_coef + (v->nd + 1),
-            v->d_coef + 2 * (v->nd + 1), v->d_coef + 2 * (v->nd + 1) + v->nd};
+  const int n1 = v->nd + 1, nd = v->nd;
+  const double* c = v->d_coef;
+  return VC{nd, v->zero_top, v->advect_scheme, v->pure_sigma, c, c + n1, c + 2 * n1,
+            c + 2 * n1 + nd, c + 2 * n1 + 2 * nd, c + 3 * n1 + 2 * nd, c + 3 * n1 + 3 * nd};
 }
 
 // Column tiles: a workgroup owns 64 consecutive columns (one per lane) and

@@ -209,10 +209,14 @@ def Implicit_Correction_HS(semi_implicit, vert_coord, atmo_data, spe_div_c, spe_
 
 
 def _physics(si, d, params, dt, forcing):
-    call("jgcm_hs_physics", si.handle, ctypes.byref(_hs(params)), int(dt), *map(_ptr, (
-        d.grid_ps_c, d.grid_u_p, d.grid_v_p, d.grid_t_p, d.grid_ps_p, d.grid_p_half, d.grid_Δp,
-        d.grid_lnp_half, d.grid_p_full, d.grid_lnp_full, d.grid_δu, d.grid_δv, d.grid_t_eq,
-        d.grid_δt, d.grid_δps)), 1 if forcing else 0)
+    # the pressure variables of ps_c are evaluated inside the column passes and
+    # not stored here: Dyn_Data's pressure fields keep the values the last
+    # Pressure_Variables! / Compute_Pressures_And_Heights! wrote (HS_Forcing!
+    # reads them, Spectral_Dynamics.jl:457-463, without writing them)
+    P = lambda t: None if t is None else _ptr(t)
+    call("jgcm_hs_physics", si.handle, ctypes.byref(_hs(params)), int(dt), *map(P, (
+        d.grid_ps_c, d.grid_u_p, d.grid_v_p, d.grid_t_p, d.grid_ps_p, None, None, None, None,
+        None, d.grid_δu, d.grid_δv, d.grid_t_eq, d.grid_δt, d.grid_δps)), 1 if forcing else 0)
     si._corrections_ready = True
 
 
@@ -244,7 +248,7 @@ def _scratch(mesh, d):
 
 
 def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
-                      pressure_variables=True):
+                      pressure_variables=True, physics_params=None):
     """Spectral_Dynamics.jl:206-365.  The transforms of the step are batched:
     the gradients of ln ps and T (:259, :293) go back to the grid in one
     launch pair with the spectral gradient operators applied on the way in,
@@ -255,13 +259,17 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
     energy into the semi-implicit solve, and the seven damping / leapfrog
     calls (:319-328) into one launch.  Results equal the unbatched sequence.
     ``pressure_variables=False`` skips the closing Pressure_Variables! (:355)
-    when the caller recomputes the same fields next (Atmosphere_Update!)."""
+    when the caller recomputes the same fields next (Atmosphere_Update!).
+    ``physics_params`` (Atmosphere_Update!) runs Spectral_Dynamics_Physics!
+    (:452-467) inside the step's column pass, its tendencies kept in
+    registers (jgcm_hs_physics_column_dynamics)."""
     d, si = dyn_data, semi_implicit
     integrator = si.integrator
     Δt = Get_Δt(integrator)
     sc = _scratch(mesh, d)
     h = mesh.handle
-    if not si._corrections_ready:
+    fused = physics_params is not None
+    if not fused and not si._corrections_ready:
         # Compute_Corrections_Init + Pressure_Variables! without forcing (:250-256)
         _physics(si, d, _ZERO_HS, Δt, False)
     si._corrections_ready = False
@@ -275,12 +283,20 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
         [BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA,
          BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA])
     call("jgcm_mesh_join", h)                  # the column pass needs both
-    call("jgcm_hs_column_dynamics", si.handle, *map(_ptr, (                    # :260-307
-        d.grid_div, d.grid_u_c, d.grid_v_c, d.grid_ps_c, d.grid_Δp, d.grid_lnp_half,
-        d.grid_lnp_full, d.grid_p_full, d.grid_dλ_ps, d.grid_dθ_ps, d.grid_t_c, d.grid_vor,
-        d.grid_geopots, d.grid_M_half, d.grid_w_full, d.grid_δu, d.grid_δv, d.grid_δps, d.grid_δt,
-        d.grid_geopot_half, d.grid_geopot_full, d.grid_δlnps, d.grid_absvor, d.grid_energy_full,
-        sc.grid_dl_t, sc.grid_dt_t)))
+    # :256 Pressure_Variables!(ps_c) is evaluated inside the column pass (NULL fields)
+    P = lambda t: None if t is None else _ptr(t)
+    cols = (d.grid_geopots, d.grid_M_half, d.grid_w_full, d.grid_δu, d.grid_δv, d.grid_δps,
+            d.grid_δt, d.grid_geopot_half, d.grid_geopot_full, d.grid_δlnps, d.grid_absvor,
+            d.grid_energy_full, sc.grid_dl_t, sc.grid_dt_t)
+    if fused:                                                                  # :452-467, :260-307
+        call("jgcm_hs_physics_column_dynamics", si.handle, ctypes.byref(_hs(physics_params)),
+             int(Δt), *map(P, (d.grid_u_p, d.grid_v_p, d.grid_t_p, d.grid_ps_p, d.grid_t_eq,
+                               d.grid_div, d.grid_u_c, d.grid_v_c, d.grid_ps_c, d.grid_dλ_ps,
+                               d.grid_dθ_ps, d.grid_t_c, d.grid_vor) + cols))
+    else:
+        call("jgcm_hs_column_dynamics", si.handle, *map(P, (                   # :260-307
+            d.grid_div, d.grid_u_c, d.grid_v_c, d.grid_ps_c, None, None,
+            None, None, d.grid_dλ_ps, d.grid_dθ_ps, d.grid_t_c, d.grid_vor) + cols))
     # :280, :294, :304, :308
     Trans_Grid_To_Spherical_Batch(
         mesh, [d.grid_δlnps, d.grid_δt, d.grid_energy_full, d.grid_δu, d.grid_δv],
@@ -313,31 +329,29 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
                            d.grid_p_full, d.grid_lnp_full)                        # :355
 
 
-# grid size from which the physics pass runs beside the first transforms: on
-# MI355X the overlapped T85 L20 step measured 245 us against 251 us serial, the
-# T42 L20 step 145 us against 139 us (the extra graph dependencies cost more
-# than the smaller kernels gain)
-OVERLAP_MIN_COLUMNS = 256 * 128
-
-
 def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, physics_params,
-                      overlap=None):
-    """Spectral_Dynamics.jl:470-490.  With ``overlap`` the physics pass
-    (forcing on the p-fields, pressure variables, correction partials) goes to
-    the mesh's side stream and runs beside the first transforms of
-    Spectral_Dynamics!, which read and write disjoint memory (parallel branches
-    of a captured step); the results do not change.  ``None`` chooses by grid
-    size (OVERLAP_MIN_COLUMNS)."""
+                      overlap=False, fused=True):
+    """Spectral_Dynamics.jl:470-490.  By default (``fused``) the physics pass
+    (HS forcing on the p-fields, the correction partials) runs inside the
+    step's column pass, after the gradient transforms, its tendencies kept in
+    registers.  ``fused=False`` runs it as its own pass first; with
+    ``overlap`` as well, that pass goes to the mesh's side stream beside the
+    first transforms (parallel branches of a captured step).  The results
+    agree to rounding.  On MI355X at T85 L20: fused 2xx us, separate serial
+    225 us, separate overlapped 233 us (the branch's fork and join cost more
+    than the 13 us pass)."""
     Δt = Get_Δt(semi_implicit.integrator)
-    if overlap is None:
-        overlap = mesh.nθ * mesh.nλ >= OVERLAP_MIN_COLUMNS
-    if overlap:
-        call("jgcm_mesh_branch", mesh.handle, 1)
-    Spectral_Dynamics_Physics(atmo_data, mesh, dyn_data, Δt, physics_params, semi_implicit)
-    # the closing Pressure_Variables! of Spectral_Dynamics! (:355) is recomputed
-    # identically by Compute_Pressures_And_Heights! below
-    Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
-                      pressure_variables=False)
+    if fused and not overlap:
+        Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
+                          pressure_variables=False, physics_params=physics_params)
+    else:
+        if overlap:
+            call("jgcm_mesh_branch", mesh.handle, 1)
+        Spectral_Dynamics_Physics(atmo_data, mesh, dyn_data, Δt, physics_params, semi_implicit)
+        # the closing Pressure_Variables! of Spectral_Dynamics! (:355) is recomputed
+        # identically by Compute_Pressures_And_Heights! below
+        Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
+                          pressure_variables=False)
     d = dyn_data
     Compute_Pressures_And_Heights(atmo_data, vert_coord, d.grid_ps_c, d.grid_geopots, d.grid_t_c,
                                   d.grid_p_half, d.grid_Δp, d.grid_lnp_half, d.grid_p_full,

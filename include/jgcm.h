@@ -541,7 +541,9 @@ int jgcm_hs_energy_implicit_leapfrog(jgcm_semi_implicit_t* si, double* spe_energ
 /* Step part 1, one column pass: Pressure_Variables!(ps_c) (Spectral_Dynamics.jl:256),
  * Spectral_Dynamics_Physics! / HS_Forcing! on the p-fields (:452-467,
  * HS_Forcing.jl:1-130; skipped when do_forcing = 0, δps = 0 always) and
- * Compute_Corrections_Init (:5-28, results kept on the device).            */
+ * Compute_Corrections_Init (:5-28, results kept on the device).  The
+ * pressure variables are evaluated in registers and stored only when p_half
+ * is not NULL (then all five must be given).                               */
 int jgcm_hs_physics(jgcm_semi_implicit_t* si, const jgcm_hs_params* hs, int64_t dt,
                     const double* grid_ps_c, const double* grid_u_p, const double* grid_v_p,
                     const double* grid_t_p, const double* grid_ps_p, double* p_half, double* dp,
@@ -551,6 +553,9 @@ int jgcm_hs_physics(jgcm_semi_implicit_t* si, const jgcm_hs_params* hs, int64_t 
 /* Step part 2, one column pass (Spectral_Dynamics.jl:260-307 without the
  * transforms): ∇ps *= ps, Four_In_One!, Compute_Geopotential!, δlnps = δps/ps,
  * Vert_Advection! of u, v, T, absolute-vorticity flux, energy = Φ + KE.
+ * When dp is NULL the pressure variables (dp, lnp_half, lnp_full, p_full)
+ * are evaluated from grid_ps in registers, bit-identical to
+ * jgcm_pressure_variables, instead of being read.
  * With grid_dlambda_t / grid_dtheta_t (the grid gradients of T, may be NULL)
  * it also applies the grid part of Add_Horizontal_Advection! for T
  * (Spectral_Spherical_Mesh.jl:716): δT -= u ∂λT + v ∂θT.                    */
@@ -564,6 +569,23 @@ int jgcm_hs_column_dynamics(jgcm_semi_implicit_t* si, const double* grid_div, co
                             double* geopot_full, double* grid_dlnps, double* grid_absvor,
                             double* grid_energy, const double* grid_dlambda_t,
                             const double* grid_dtheta_t);
+/* Spectral_Dynamics_Physics! (Spectral_Dynamics.jl:452-467) followed by the
+ * column part of Spectral_Dynamics! (:250-307), one column pass: the HS
+ * forcing of the p-fields, δps = 0 and the Compute_Corrections_Init partial
+ * sums (as jgcm_hs_physics with do_forcing = 1), then everything
+ * jgcm_hs_column_dynamics does with the pressure variables evaluated from
+ * grid_ps; the physics tendencies stay in registers instead of going
+ * through grid_du, grid_dv, grid_dt and grid_dps, which are only written.
+ * No reference counterpart as one call; equal to the two calls to rounding. */
+int jgcm_hs_physics_column_dynamics(
+    jgcm_semi_implicit_t* si, const jgcm_hs_params* hs, int64_t dt, const double* grid_u_p,
+    const double* grid_v_p, const double* grid_t_p, const double* grid_ps_p, double* grid_t_eq,
+    const double* grid_div, const double* grid_u, const double* grid_v, const double* grid_ps,
+    double* grid_dlambda_ps, double* grid_dtheta_ps, const double* grid_t, const double* grid_vor,
+    const double* grid_geopots, double* grid_M_half, double* grid_w_full, double* grid_du,
+    double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
+    double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
+    const double* grid_dtheta_t);
 /* ps_n = exp(lnps) then Compute_Corrections!  Spectral_Dynamics.jl:336-343, :30-68 */
 int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, double* grid_ps_n,
                         const double* grid_u_n, const double* grid_v_n, double* grid_t_n,

@@ -203,18 +203,21 @@ def test_update_output_matches_oracle(jgcm):
     assert rel(om_d.t_zonal_mean, om_r.t_zonal_mean) < 1e-12
 
 
-def test_overlapped_and_captured_steps_equal_serial_steps(jgcm):
-    """The physics pass on the side stream (eager, and as a parallel branch of a
-    captured step replayed with eager work in between) gives the serial
-    step's results bit for bit."""
+def test_overlapped_fused_and_captured_steps_equal_serial_steps(jgcm):
+    """The physics pass as its own serial pass, on the side stream, fused into
+    the column pass (the default), and captured (fused, and overlapped as a
+    parallel branch) with replays that have eager work in between: all give
+    the same results bit for bit."""
     runs = []
-    for mode in ("serial", "overlap", "graph"):
+    modes = {"separate": dict(fused=False), "overlap": dict(overlap=True), "fused": {},
+             "graph_fused": {}, "graph_overlap": dict(overlap=True)}
+    for mode, kw in modes.items():
         _, (dev, dvc, datmo, dsi, dd) = build(jgcm, 42, 64, 20, 600)
-        jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS, overlap=(mode != "serial"))
+        jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS, **kw)
         jgcm.Update_Init_Step(dsi)
-        if mode == "graph":
+        if mode.startswith("graph"):
             g = jgcm.Step_Graph(dev, lambda: jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS,
-                                                                    overlap=True),
+                                                                    **kw),
                                 nfields=5 * 20 + 1)
             for k in range(3):
                 g.launch()
@@ -224,7 +227,7 @@ def test_overlapped_and_captured_steps_equal_serial_steps(jgcm):
             g.close()
         else:
             for k in range(3):
-                jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS, overlap=(mode == "overlap"))
+                jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS, **kw)
         dev.synchronize()
         runs.append({f: getattr(dd, f).cpu().numpy() for f in FIELDS + ["grid_δt", "grid_δu"]})
     for other in runs[1:]:
