@@ -48,54 +48,102 @@ def parse():
     return p.parse_args()
 
 
+def _cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "cpu_model": model, "threads_all": threads}
+
+
 def cpu_baseline(seconds):
-    from threadpoolctl import threadpool_limits
+    """The transform pair at T85 x 61 level-fields in the C++ FP64 restatement
+    of the reference's loops (oracle/sht_cpu.cpp), on 1 thread (the serial
+    reference's setting) and on all the host cores this job may use (OpenMP
+    over level-fields), each for half of ``seconds``."""
     from oracle import SpectralSphericalMesh
-    with threadpool_limits(1):
-        M = SpectralSphericalMesh(N_TRUNC, N_TRUNC + 1, NLAT, NLON, NFIELDS, 6371.0e3)
-        g = np.random.default_rng(1).standard_normal((NFIELDS, NLAT, NLON))
-        s = M.spec_zeros()
-        reps = 0
-        t0 = time.perf_counter()
+    from oracle.cpu import CpuSHT
+    info = _cpu_info()
+    M = SpectralSphericalMesh(N_TRUNC, N_TRUNC + 1, NLAT, NLON, NFIELDS, 6371.0e3)
+    c = CpuSHT(M)
+    g = np.random.default_rng(1).standard_normal((NFIELDS, NLAT, NLON))
+    s = M.spec_zeros()
+
+    def rate(threads, budget):
+        c.forward(g, s, threads)
+        reps, t0 = 0, time.perf_counter()
         while True:
-            M.trans_grid_to_spherical(g, s)
-            M.trans_spherical_to_grid(s, g)
+            c.forward(g, s, threads)
+            c.inverse(s, g, threads)
             reps += 1
             el = time.perf_counter() - t0
-            if el >= seconds:
-                break
-    return {"value": 2 * NFIELDS * reps / el, "unit": "level-field transforms/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{reps} forward+inverse passes of 61 T85 level-fields "
-                      f"(numpy FP64 oracle, 1 thread, {el:.1f} s)"}
+            if el >= budget:
+                return 2 * NFIELDS * reps / el, reps, el
+
+    v1, r1, e1 = rate(1, seconds / 2)
+    n = info["threads_all"]
+    vn, rn, en = rate(n, seconds / 2)
+    return {"value": vn, "unit": "level-field transforms/s", "cores": n, "kind": "port",
+            "implementation": f"C++ FP64 restatement of SSM:143-293 (oracle/sht_cpu.cpp, "
+                              f"{c.lib.variant} build, radix-2 FFT), not the Julia reference",
+            "single_thread": {"value": v1, "cores": 1},
+            **info,
+            "sample": f"{rn} forward+inverse passes of 61 T85 level-fields on {n} threads "
+                      f"({en:.1f} s) and {r1} on 1 thread ({e1:.1f} s)"}
 
 
 def cpu_hs_baseline(steps=3):
-    """The FP64 oracle's Held-Suarez step at T85 L20 (Δt = 300 s), 1 thread."""
+    """The FP64 oracle's Held-Suarez step at T85 L20 (Δt = 300 s) with its
+    transforms replaced by the C++ restatement: 1 thread, and all cores for
+    the transforms (the rest of the step is numpy on one thread)."""
     from threadpoolctl import threadpool_limits
     from oracle import FilteredLeapfrog, SpectralSphericalMesh
     from oracle import dynamics as odyn
     from oracle import primitive as P
+    from oracle.cpu import CpuSHT
     N, nlat, nd, dt = 85, 128, 20, 300
-    with threadpool_limits(1):
-        M = SpectralSphericalMesh(N, N + 1, nlat, 2 * nlat, nd, 6371000.0)
-        vc = P.VertCoordinate(2 * nlat, nlat, nd, "even_sigma", "simmons_and_burridge",
-                              "second_centered_wts", 1e5)
-        atmo = odyn.AtmoData("hs", 2 * nlat, nlat, nd, True, True, False, False, M.sin_theta,
-                             6371000.0, 7.292e-5)
-        I = FilteredLeapfrog(0.04, 4, 1.15741e-4, M.laplacian_eig, 0.5, dt, True, 0, 86400)
-        si = P.SemiImplicitSolver(vc, atmo, I, 1e5, np.full(nd, 300.0), M.wave_numbers)
-        d = odyn.DynData("hs", N, N + 1, 2 * nlat, nlat, nd)
-        noise = 0.1 * np.random.default_rng(5).standard_normal((nd, nlat, 2 * nlat))
-        P.spectral_initialize_fields(M, atmo, vc, 1e5, 300.0, d.grid_geopots, d, grid_t_noise=noise)
-        P.atmosphere_update(M, atmo, vc, si, d, HS_PARAMS)
-        si.update_init_step()
-        t0 = time.perf_counter()
-        for _ in range(steps):
+    info = _cpu_info()
+    out = {}
+    for label, threads in (("single_thread", 1), ("all_cores", info["threads_all"])):
+        with threadpool_limits(1):
+            M = SpectralSphericalMesh(N, N + 1, nlat, 2 * nlat, nd, 6371000.0)
+            c = CpuSHT(M)
+            M.trans_grid_to_spherical = lambda g, s: c.forward(np.ascontiguousarray(g), s, threads)
+
+            def inv(s, g):
+                out_g = np.ascontiguousarray(g) if g.flags.c_contiguous else np.empty_like(g)
+                c.inverse(np.ascontiguousarray(s), out_g, threads)
+                if out_g is not g:
+                    g[...] = out_g
+                return g
+            M.trans_spherical_to_grid = inv
+            vc = P.VertCoordinate(2 * nlat, nlat, nd, "even_sigma", "simmons_and_burridge",
+                                  "second_centered_wts", 1e5)
+            atmo = odyn.AtmoData("hs", 2 * nlat, nlat, nd, True, True, False, False,
+                                 M.sin_theta, 6371000.0, 7.292e-5)
+            I = FilteredLeapfrog(0.04, 4, 1.15741e-4, M.laplacian_eig, 0.5, dt, True, 0, 86400)
+            si = P.SemiImplicitSolver(vc, atmo, I, 1e5, np.full(nd, 300.0), M.wave_numbers)
+            d = odyn.DynData("hs", N, N + 1, 2 * nlat, nlat, nd)
+            noise = 0.1 * np.random.default_rng(5).standard_normal((nd, nlat, 2 * nlat))
+            P.spectral_initialize_fields(M, atmo, vc, 1e5, 300.0, d.grid_geopots, d,
+                                         grid_t_noise=noise)
             P.atmosphere_update(M, atmo, vc, si, d, HS_PARAMS)
-        el = (time.perf_counter() - t0) / steps
-    return {"s_per_step": el, "sim_days_per_day": dt / el, "cores": 1, "kind": "port",
-            "sample": f"{steps} Atmosphere_Update! steps of the numpy FP64 oracle, T85 L20"}
+            si.update_init_step()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                P.atmosphere_update(M, atmo, vc, si, d, HS_PARAMS)
+            el = (time.perf_counter() - t0) / steps
+        out[label] = {"s_per_step": el, "sim_days_per_day": dt / el, "cores": threads}
+    return {**out, "kind": "port", **info,
+            "sample": f"{steps} Atmosphere_Update! steps, T85 L20: the numpy FP64 oracle with "
+                      f"its transforms in the C++ restatement (oracle/sht_cpu.cpp)"}
 
 
 HS_PARAMS = {"σ_b": 0.7, "k_f": 1.0, "k_a": 1.0 / 40.0, "k_s": 1.0 / 4.0, "ΔT_y": 60.0,
@@ -529,7 +577,13 @@ def main():
                                        "unfold_ppw": shape["unfold_ppw"]}}
     mesh.set_precision("fp64")
     autotune.set_fft_shape(mesh, 0, 0)
-    use = "split32" if results["split32"]["max_rel_err_vs_fp64"] <= 1e-6 else "fp64"
+    # the headline is the fastest Legendre precision within the metric's
+    # 1e-6 gate (FP64 is exact by definition); split-FP32 and BF16x3
+    # accumulate in FP32 blocks, so their accuracy is FP32-class (~1e-7)
+    ok = [p for p, v in results.items() if v["max_rel_err_vs_fp64"] <= 1e-6]
+    use = max(ok, key=lambda p: results[p]["fields_per_s"])
+    for p, v in results.items():
+        v["accuracy_class"] = "fp64" if p == "fp64" else "fp32 (FP32-accumulated blocks)"
     r = results[use]
     # roofline of the dominant launch (the slower direction), algorithmic bytes
     dom = "inverse" if r["ms_inverse"] >= r["ms_forward"] else "forward"
@@ -541,7 +595,8 @@ def main():
         "value": r["fields_per_s"], "unit": "level-field transforms/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-split(f64 I/O)" if use == "split32" else "f64",
+        "dtype": {"fp64": "f64", "split32": "f32-split(f64 I/O)",
+                  "bf16x3": "bf16x3(f64 I/O)"}[use],
         "data": "synthetic",
         "config": {"workload": "T85 (N=85, 256x128) forward+inverse SHT of 61 level-fields",
                    "path": use},
