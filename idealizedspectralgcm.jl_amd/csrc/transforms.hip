@@ -591,6 +591,8 @@ typedef double dv4 __attribute__((ext_vector_type(4)));
 typedef float fv4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 64;  // contraction steps (latitudes or rows) staged per pass
+static_assert(KC <= 64, "the staging maps one lane to each contraction step of a chunk");
+static_assert((TROWS & (TROWS - 1)) == 0 && LTHREADS % 64 == 0, "tile shape");
 constexpr int KS1 = KC + 1;
 
 __device__ __forceinline__ dv4 mma64(double a, double b, dv4 c) {
@@ -860,28 +862,46 @@ __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int 
       const int KSc = (kcl + KSTEP - 1) / KSTEP;
       const int kpad = KSTEP * KSc;
       __syncthreads();  // the previous chunk's operands are consumed
-      staged_copy2(
-          (LCOLS / 2) * kpad,
-          [&](int idx) {
-            int jp = idx % kpad, fl = idx / kpad;
-            return (fl < nf && jp < kcl) ? srow[fl * fstride + lat0 + jp] : make_double2(0.0, 0.0);
-          },
-          [&](int idx, double2 v) {
-            int jp = idx % kpad, fl = idx / kpad;
-            double re = v.x, im = v.y;
+      // lane = latitude of the chunk (kpad <= KC = 64), the wave picks the
+      // fields and table rows: no index division, all loads issued first
+      {
+        constexpr int NWV = LTHREADS / 64;
+        constexpr int FPT = (LCOLS / 2 + NWV - 1) / NWV;   // fields per thread
+        constexpr int RPT = (TROWS + NWV - 1) / NWV;       // table rows per thread
+        const int jp = lane;
+        const bool jin = jp < kcl;
+        double2 xv[FPT];
+        TT pv[RPT];
+#pragma unroll
+        for (int i = 0; i < FPT; ++i) {
+          const int fl = wave + NWV * i;
+          xv[i] = (fl < nf && jin) ? srow[fl * fstride + lat0 + jp] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+          const int k = wave + NWV * i;
+          const bool ok = k < nrg && jin;
+          pv[i] = tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * k) * half + lat0 + jp : 0, ok);
+        }
+        if (jp < kpad) {
+#pragma unroll
+          for (int i = 0; i < FPT; ++i) {
+            const int fl = wave + NWV * i;
+            if (fl >= LCOLS / 2) break;
+            double re = xv[i].x, im = xv[i].y;
             if (mul_i) { double t = re; re = -im; im = t; }
             re *= sgn;
             im *= sgn;
             X[(2 * fl) * KS1 + jp] = to_lds<PREC>(re);
             X[(2 * fl + 1) * KS1 + jp] = to_lds<PREC>(im);
-          },
-          TROWS * kpad,
-          [&](int idx) {
-            int jp = idx % kpad, k = idx / kpad;
-            bool ok = k < nrg && jp < kcl;
-            return tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * k) * half + lat0 + jp : 0, ok);
-          },
-          [&](int idx, TT v) { Pt[(idx / kpad) * KS1 + idx % kpad] = v; });
+          }
+#pragma unroll
+          for (int i = 0; i < RPT; ++i) {
+            const int k = wave + NWV * i;
+            if (k < TROWS) Pt[k * KS1 + jp] = pv[i];
+          }
+        }
+      }
       __syncthreads();
       mfma_accumulate<PREC>(
           acc, ntile, KSc, [&](int ks, int e) { return Xc[KSTEP * ks + e]; },
@@ -1025,25 +1045,46 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
     const int KSc = (kcr + KSTEP - 1) / KSTEP;
     const int kpad = KSTEP * KSc;
     __syncthreads();
-    staged_copy2(
-        (LCOLS / 2) * kpad,
-        [&](int idx) {
-          int k = idx % kpad, fl = idx / kpad;
-          if (fl >= nf || k >= kcr) return make_double2(0.0, 0.0);
-          return staged_coefficient(in, ops, f0 + fl, spec_d, nfirst + 2 * (kk0 + k), m, nm, ns);
-        },
-        [&](int idx, double2 v) {
-          int k = idx % kpad, fl = idx / kpad;
-          S[(2 * fl) * KS1 + k] = to_lds<PREC>(v.x);
-          S[(2 * fl + 1) * KS1 + k] = to_lds<PREC>(v.y);
-        },
-        kpad * TROWS,
-        [&](int idx) {
-          int j = idx % TROWS, k = idx / TROWS;
-          bool ok = k < kcr && j < nlg;
-          return tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * (kk0 + k)) * half + j : 0, ok);
-        },
-        [&](int idx, TT v) { Pt[(idx / TROWS) * PSI + idx % TROWS] = v; });
+    {
+      // coefficients: lane = contraction row of the chunk (kpad <= KC = 64),
+      // the wave picks the fields (uniform segment lookup, no index division)
+      constexpr int NWV = LTHREADS / 64;
+      constexpr int FPT = (LCOLS / 2 + NWV - 1) / NWV;
+      const int kr = lane;
+      const bool kin = kr < kcr;
+      double2 sv[FPT];
+#pragma unroll
+      for (int i = 0; i < FPT; ++i) {
+        const int fl = wave + NWV * i;
+        sv[i] = (fl < nf && kin)
+                    ? staged_coefficient(in, ops, f0 + fl, spec_d, nfirst + 2 * (kk0 + kr), m, nm, ns)
+                    : make_double2(0.0, 0.0);
+      }
+      // table rows: idx -> (latitude, row) by shifts (TROWS is a power of two)
+      constexpr int TPT = (KC * TROWS + LTHREADS - 1) / LTHREADS;
+      TT pv[TPT];
+#pragma unroll
+      for (int i = 0; i < TPT; ++i) {
+        const int idx = (int)threadIdx.x + LTHREADS * i;
+        const int j = idx % TROWS, k = idx / TROWS;
+        const bool ok = k < kcr && j < nlg;
+        pv[i] = tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * (kk0 + k)) * half + j : 0, ok);
+      }
+      if (kr < kpad) {
+#pragma unroll
+        for (int i = 0; i < FPT; ++i) {
+          const int fl = wave + NWV * i;
+          if (fl >= LCOLS / 2) break;
+          S[(2 * fl) * KS1 + kr] = to_lds<PREC>(sv[i].x);
+          S[(2 * fl + 1) * KS1 + kr] = to_lds<PREC>(sv[i].y);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < TPT; ++i) {
+        const int idx = (int)threadIdx.x + LTHREADS * i;
+        if (idx < kpad * TROWS) Pt[(idx / TROWS) * PSI + idx % TROWS] = pv[i];
+      }
+    }
     __syncthreads();
     mfma_accumulate<PREC>(
         acc, nlt, KSc, [&](int ks, int e) { return Sc[KSTEP * ks + e]; },
