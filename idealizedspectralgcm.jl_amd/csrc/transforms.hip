@@ -28,6 +28,7 @@
 //             P_hi x_lo + P_lo x_hi are summed by FP32 MFMA, the block sums
 //             are accumulated in FP64 (compensated summation).
 #include <mutex>
+#include <type_traits>
 
 #include "fft.hpp"
 #include "jgcm_internal.hpp"
@@ -933,13 +934,26 @@ __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int 
 // Spectral coefficient (n, m) of global field b of an inverse-transform
 // segment map, after the segment's operator (SPEC_OP_*).  The arithmetic and
 // its order are those of the stand-alone operator kernels.
+__device__ __forceinline__ double2 staged_coefficient_at(const double2* h, const double2* h2,
+                                                         int op, const SpecOpTables& t, int n, int m,
+                                                         int nm, int ns);
+
 __device__ __forceinline__ double2 staged_coefficient(const SegMap& in, const SpecOpTables& t,
                                                       int b, size_t spec_d, int n, int m, int nm,
                                                       int ns) {
   const int k = seg_find(in, b);
   const size_t off = (size_t)(b - in.start[k]) * spec_d;
   const double2* h = reinterpret_cast<const double2*>(in.ptr[k] + off);
-  const int op = (in.flag[k] >> 1) & 7;
+  const double2* h2 = in.ptr2[k] ? reinterpret_cast<const double2*>(in.ptr2[k] + off) : nullptr;
+  return staged_coefficient_at(h, h2, (in.flag[k] >> 1) & 7, t, n, m, nm, ns);
+}
+
+// The same with the array of global field b already resolved: h (and h2,
+// the second operand of the u cos / v cos operators) point at its
+// coefficients, op is its segment's operator.
+__device__ __forceinline__ double2 staged_coefficient_at(const double2* h, const double2* h2,
+                                                         int op, const SpecOpTables& t, int n, int m,
+                                                         int nm, int ns) {
   const size_t i = (size_t)n * nm + m;
   if (op == SPEC_OP_NONE) return h[i];
   if (op == SPEC_OP_NEG_I) {
@@ -970,7 +984,7 @@ __device__ __forceinline__ double2 staged_coefficient(const SegMap& in, const Sp
   }
   // u cos θ / v cos θ: ptr = vor, ptr2 = div
   const double2* vor = h;
-  const double2* div = reinterpret_cast<const double2*>(in.ptr2[k] + off);
+  const double2* div = h2;
   const double cc = t.uvc[c];
   if (op == SPEC_OP_UCOS) {
     double2 d = div[i];
@@ -1005,7 +1019,9 @@ __device__ __forceinline__ double2 staged_coefficient(const SegMap& in, const Sp
 // Inverse: grid x = segment (m, par), y = chunk of 32 fields, z = group of
 // 64 latitudes (4 tiles); 256 threads, wave w owns columns 16w .. 16w+15.
 // LDS: S [64 cols][KC+1], P [KC rows][KC+1].
-template <int PREC>
+// PLAIN: every field comes from in.ptr[0] without an operator (checked by the
+// launcher): the staging skips the segment and operator logic.
+template <int PREC, bool PLAIN>
 __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2* __restrict__ eo,
                                                          FwdTables tab, int B, int nmax, int ns,
                                                          int nm, int half,
@@ -1040,6 +1056,15 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
   const LT* Sc = S + (16 * wave + li) * KS1 + lko;
   constexpr int PSI = TROWS + 1;
   const TT* Pc = Pt + lko * PSI + li;
+  // the segment (array) of the chunk's fields, looked up once when the chunk
+  // lies in a single one (a lookup per field costs more than the loads)
+  const int seg_a = seg_find(in, f0), seg_b = seg_find(in, f0 + max(nf, 1) - 1);
+  const bool seg_one = seg_a == seg_b;
+  const size_t seg_off = (size_t)(f0 - in.start[seg_a]) * spec_d;
+  const double2* seg_h = reinterpret_cast<const double2*>(in.ptr[seg_a] + seg_off);
+  const double2* seg_h2 =
+      in.ptr2[seg_a] ? reinterpret_cast<const double2*>(in.ptr2[seg_a] + seg_off) : nullptr;
+  const int seg_op = (in.flag[seg_a] >> 1) & 7;
   for (int kk0 = 0; kk0 < nr; kk0 += KC) {
     const int kcr = min(KC, nr - kk0);
     const int KSc = (kcr + KSTEP - 1) / KSTEP;
@@ -1052,13 +1077,49 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
       constexpr int FPT = (LCOLS / 2 + NWV - 1) / NWV;
       const int kr = lane;
       const bool kin = kr < kcr;
+      const int nrow = nfirst + 2 * (kk0 + kr);
       double2 sv[FPT];
+      if constexpr (PLAIN) {
+        // the plain transform of one array (the common case): direct loads
+        const double2* h0 = reinterpret_cast<const double2*>(in.ptr[0]);
 #pragma unroll
-      for (int i = 0; i < FPT; ++i) {
-        const int fl = wave + NWV * i;
-        sv[i] = (fl < nf && kin)
-                    ? staged_coefficient(in, ops, f0 + fl, spec_d, nfirst + 2 * (kk0 + kr), m, nm, ns)
-                    : make_double2(0.0, 0.0);
+        for (int i = 0; i < FPT; ++i) {
+          const int fl = wave + NWV * i;
+          sv[i] = (fl < nf && kin)
+                      ? h0[(size_t)(f0 + fl) * (spec_d / 2) + (size_t)nrow * nm + m]
+                      : make_double2(0.0, 0.0);
+        }
+      } else if (seg_one) {
+        // the chunk's fields lie in one array: pointer arithmetic per field,
+        // one straight loop per operator (the operator is the workgroup's)
+        auto stage_op = [&](auto opc) {
+          constexpr int OP = decltype(opc)::value;
+#pragma unroll
+          for (int i = 0; i < FPT; ++i) {
+            const int fl = wave + NWV * i;
+            const size_t o = (size_t)fl * (spec_d / 2);
+            sv[i] = (fl < nf && kin)
+                        ? staged_coefficient_at(seg_h + o, seg_h2 ? seg_h2 + o : nullptr, OP, ops,
+                                                nrow, m, nm, ns)
+                        : make_double2(0.0, 0.0);
+          }
+        };
+        switch (seg_op) {
+          case SPEC_OP_NONE: stage_op(std::integral_constant<int, SPEC_OP_NONE>{}); break;
+          case SPEC_OP_GRAD_LAMBDA: stage_op(std::integral_constant<int, SPEC_OP_GRAD_LAMBDA>{}); break;
+          case SPEC_OP_GRAD_THETA: stage_op(std::integral_constant<int, SPEC_OP_GRAD_THETA>{}); break;
+          case SPEC_OP_UCOS: stage_op(std::integral_constant<int, SPEC_OP_UCOS>{}); break;
+          case SPEC_OP_VCOS: stage_op(std::integral_constant<int, SPEC_OP_VCOS>{}); break;
+          case SPEC_OP_NEG_I: stage_op(std::integral_constant<int, SPEC_OP_NEG_I>{}); break;
+          default: stage_op(std::integral_constant<int, SPEC_OP_NEG>{}); break;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < FPT; ++i) {
+          const int fl = wave + NWV * i;
+          sv[i] = (fl < nf && kin) ? staged_coefficient(in, ops, f0 + fl, spec_d, nrow, m, nm, ns)
+                                   : make_double2(0.0, 0.0);
+        }
       }
       // table rows: idx -> (latitude, row) by shifts (TROWS is a power of two)
       constexpr int TPT = (KC * TROWS + LTHREADS - 1) / LTHREADS;
@@ -1274,28 +1335,23 @@ static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, 
   FwdTables tab = tables_for(M, table);
   size_t lds = legendre_lds(M->precision);
   SpecOpTables ops{M->d_dlam, M->d_dthm, M->d_dthp, M->d_uvc, M->d_uvm, M->d_uvp};
+  const bool plain = snm.nseg == 1 && ((snm.flag[0] >> 1) & 7) == SPEC_OP_NONE;
+#define JGCM_INV_LAUNCH(P)                                                                      \
+  do {                                                                                          \
+    if (plain)                                                                                  \
+      hipLaunchKernelGGL((legendre_inv_mfma<P, true>), grd, dim3(LTHREADS), lds, M->stream,     \
+                         snm, eo, tab, B, nmax, M->ns, M->nm, M->half, ops, out_mode);          \
+    else                                                                                        \
+      hipLaunchKernelGGL((legendre_inv_mfma<P, false>), grd, dim3(LTHREADS), lds, M->stream,    \
+                         snm, eo, tab, B, nmax, M->ns, M->nm, M->half, ops, out_mode);          \
+  } while (0)
   switch (M->precision) {
-    case JGCM_PREC_FP64:
-      hipLaunchKernelGGL(legendre_inv_mfma<P_FP64>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, ops,
-                         out_mode);
-      break;
-    case JGCM_PREC_FP32:
-      hipLaunchKernelGGL(legendre_inv_mfma<P_FP32>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, ops,
-                         out_mode);
-      break;
-    case JGCM_PREC_BF16X3:
-      hipLaunchKernelGGL(legendre_inv_mfma<P_BF16>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, ops,
-                         out_mode);
-      break;
-    default:
-      hipLaunchKernelGGL(legendre_inv_mfma<P_SPLIT>, grd, dim3(LTHREADS), lds, M->stream, snm, eo, tab, B,
-                         nmax, M->ns, M->nm, M->half, ops,
-                         out_mode);
-      break;
+    case JGCM_PREC_FP64: JGCM_INV_LAUNCH(P_FP64); break;
+    case JGCM_PREC_FP32: JGCM_INV_LAUNCH(P_FP32); break;
+    case JGCM_PREC_BF16X3: JGCM_INV_LAUNCH(P_BF16); break;
+    default: JGCM_INV_LAUNCH(P_SPLIT); break;
   }
+#undef JGCM_INV_LAUNCH
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
