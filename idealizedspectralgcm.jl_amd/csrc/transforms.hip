@@ -105,7 +105,9 @@ static inline SegMap one_seg(const void* p, int nlev, int flag) {
 }
 
 // ---------------------------------------------------------------- FFT stage
-template <int L>
+// PLAIN (all kernels of this stage): one array with no cos θ flag, so the
+// segment lookup and the per-element scaling branches compile away.
+template <int L, bool PLAIN>
 __global__ __launch_bounds__(256) void fft_fold_kernel(
     SegMap in,                         // grid fields [nlat][L] each
     double2* __restrict__ eo,          // [B][nm][2][half]
@@ -121,11 +123,11 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
   const int SP = 2 * nm + 1;
   const int b = blockIdx.y;
   const int j0 = blockIdx.x * ppw;
-  const int seg = seg_find(in, b);
+  const int seg = PLAIN ? 0 : seg_find(in, b);
   const double* __restrict__ grid = in.ptr[seg] + (size_t)(b - in.start[seg]) * nlat * L;
   // flag bit 0: rows scaled by cos θ; with bit 1 divided by it instead
-  const double* __restrict__ cosmul = (in.flag[seg] & 1) ? cosv : nullptr;
-  const bool divcos = (in.flag[seg] & 2) != 0;
+  const double* __restrict__ cosmul = (!PLAIN && (in.flag[seg] & 1)) ? cosv : nullptr;
+  const bool divcos = !PLAIN && (in.flag[seg] & 2) != 0;
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const double scale = 1.0 / L;
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
 // with flag & 2 the E/O values are first weighted 1/L (m = 0) and 1/(2L)
 // (m > 0), the transposed Hermitian weights of the forward FFT; flag & 4
 // (with & 1) multiplies the rows by cos θ instead of dividing.
-template <int L>
+template <int L, bool PLAIN>
 __global__ __launch_bounds__(256) void ifft_unfold_kernel(
     const double2* __restrict__ eo,    // [B][nm][2][half]
     SegMap out,                        // grid fields [nlat][L] each
@@ -233,11 +235,11 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
   const int SP = 2 * nm + 1;
   const int b = blockIdx.y;
   const int j0 = blockIdx.x * ppw;
-  const int seg = seg_find(out, b);
+  const int seg = PLAIN ? 0 : seg_find(out, b);
   double* __restrict__ grid = out.ptr[seg] + (size_t)(b - out.start[seg]) * nlat * L;
-  const double* __restrict__ divcos = (out.flag[seg] & 1) ? cosv : nullptr;
-  const bool adjw = (out.flag[seg] & 2) != 0;
-  const bool mulcos = (out.flag[seg] & 4) != 0;
+  const double* __restrict__ divcos = (!PLAIN && (out.flag[seg] & 1)) ? cosv : nullptr;
+  const bool adjw = !PLAIN && (out.flag[seg] & 2) != 0;
+  const bool mulcos = !PLAIN && (out.flag[seg] & 4) != 0;
   const double w0 = 1.0 / L, w1 = 1.0 / (2.0 * L);
   // twiddle loads in flight together with the E/O gather below
   constexpr int TWI = (L + 255) / 256;
@@ -1204,8 +1206,13 @@ static void launch_fold(const jgcm_mesh* M, const SegMap& in, double2* eo, int B
   int ppw = M->fold_ppw > 0 ? M->fold_ppw : pick_ppw(L, B, true);
   dim3 grd((M->half + ppw - 1) / ppw, B);
   size_t lds = (size_t)(M->nm * 2 + 1) * ppw * sizeof(double2);
-  hipLaunchKernelGGL(fft_fold_kernel<L>, grd, dim3(256), lds, M->stream, in, eo,
-                     reinterpret_cast<const double2*>(M->d_twiddle), M->d_cos, M->nlat, M->nm, ppw);
+  const double2* tw = reinterpret_cast<const double2*>(M->d_twiddle);
+  if (in.nseg == 1 && in.flag[0] == 0)
+    hipLaunchKernelGGL((fft_fold_kernel<L, true>), grd, dim3(256), lds, M->stream, in, eo, tw,
+                       M->d_cos, M->nlat, M->nm, ppw);
+  else
+    hipLaunchKernelGGL((fft_fold_kernel<L, false>), grd, dim3(256), lds, M->stream, in, eo, tw,
+                       M->d_cos, M->nlat, M->nm, ppw);
 }
 
 template <int L>
@@ -1213,8 +1220,13 @@ static void launch_unfold(const jgcm_mesh* M, const double2* eo, const SegMap& o
   int ppw = M->unfold_ppw > 0 ? M->unfold_ppw : pick_ppw(L, B, false);
   dim3 grd((M->half + ppw - 1) / ppw, B);
   size_t lds = (size_t)(M->nm * 2 + 1) * ppw * sizeof(double2);
-  hipLaunchKernelGGL(ifft_unfold_kernel<L>, grd, dim3(256), lds, M->stream, eo, out,
-                     reinterpret_cast<const double2*>(M->d_twiddle), M->d_cos, M->nlat, M->nm, ppw);
+  const double2* tw = reinterpret_cast<const double2*>(M->d_twiddle);
+  if (out.nseg == 1 && out.flag[0] == 0)
+    hipLaunchKernelGGL((ifft_unfold_kernel<L, true>), grd, dim3(256), lds, M->stream, eo, out, tw,
+                       M->d_cos, M->nlat, M->nm, ppw);
+  else
+    hipLaunchKernelGGL((ifft_unfold_kernel<L, false>), grd, dim3(256), lds, M->stream, eo, out, tw,
+                       M->d_cos, M->nlat, M->nm, ppw);
 }
 
 template <int L>
