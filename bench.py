@@ -41,7 +41,7 @@ def parse():
     p.add_argument("--no-extra", action="store_true",
                    help="skip the ensemble, Galewsky and adjoint legs")
     p.add_argument("--no-tune", action="store_true",
-                   help="take the FFT shapes from the tuning cache (no timing runs); "
+                   help="take the FFT and Legendre shapes from the tuning cache (no timing runs); "
                         "the built-in rule where the cache has none")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run the library's copy kernel over 256 MiB (PMC byte calibration)")
@@ -211,14 +211,15 @@ def ensemble_bench(jgcm, torch, steps=100):
     """Mixed shallow-water ensemble: 64 x T21 (64x32), 128 x T42 (128x64) and
     64 x T85 (256x128) instances, seeded n^-3/2 vorticity spectra, FP32
     Legendre with FP64 state, one step of all 256 captured into one HIP graph
-    (the truncations in sequence on one stream).
+    (each truncation on its own stream, forked from and joined back to the
+    first group's stream, so the three run as parallel graph branches).
     Reports instance-steps/s and the p50/p99 latency of one ensemble step,
     beside the cost of looping the plain FP64 path one instance at a time."""
     from jgcm_amd import ensemble as E
     from jgcm_amd import autotune
     groups = [(21, 32, 64, 1800), (42, 64, 128, 1200), (85, 128, 64, 600)]
     cache = autotune.Tuning_Cache(tuning_path())
-    ens = E.SW_Ensemble(groups, seed=11, precision="fp32", tuning=cache, concurrent=False)
+    ens = E.SW_Ensemble(groups, seed=11, precision="fp32", tuning=cache, concurrent=True)
     ens.step()
     ens.capture()
     for _ in range(10):
@@ -253,8 +254,8 @@ def ensemble_bench(jgcm, torch, steps=100):
         loop_ms += count * e0.elapsed_time(e1) / 20
     n = ens.instances
     return {"instances": n, "groups": "64xT21, 128xT42, 64xT85", "precision": "fp32 Legendre, fp64 state",
-            "truncations": "in sequence on one stream, one graph",
-            "fft_shape_fold_unfold": ens.tuning,
+            "truncations": "one stream each, parallel branches of one graph",
+            "shapes_fold_unfold_fwdtiles_invtiles": ens.tuning,
             "ms_per_ensemble_step": total, "instance_steps_per_s": n / (total * 1e-3),
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "loop_fp64_ms_per_step": loop_ms, "speedup_vs_fp64_loop": loop_ms / total,
@@ -520,13 +521,15 @@ def main():
     ref_grid = ref_spec = None
     for prec in ("fp64", "split32", "bf16x3"):
         mesh.set_precision(prec)
-        # FFT workgroup shape for this precision and batch, picked by the autotuner
+        # FFT workgroup and Legendre tile shapes for this precision and batch,
+        # picked by the autotuner
         if a.no_tune:
             autotune.set_fft_shape(mesh, 0, 0)
+            autotune.set_legendre_shape(mesh, 0, 0)
             shape = autotune.Apply_Tuning(mesh, NFIELDS, cache=cache) or \
-                {"fold_ppw": 0, "unfold_ppw": 0}
+                {"fold_ppw": 0, "unfold_ppw": 0, "fwd_tiles": 0, "inv_tiles": 0}
         else:
-            shape = autotune.Tune_FFT_Shape(mesh, NFIELDS, cache=cache)
+            shape = autotune.Tune_Shapes(mesh, NFIELDS, cache=cache)
         for _ in range(a.warmup):
             inv(spec_in, grid)
             fwd(grid, spec)
@@ -574,9 +577,12 @@ def main():
                          "ms_inverse": t_inv, "ms_forward": t_fwd, "wall_s": wall,
                          "max_rel_err_vs_fp64": err,
                          "fft_shape": {"fold_ppw": shape["fold_ppw"],
-                                       "unfold_ppw": shape["unfold_ppw"]}}
+                                       "unfold_ppw": shape["unfold_ppw"]},
+                         "legendre_tiles": {"forward": shape.get("fwd_tiles", 0),
+                                            "inverse": shape.get("inv_tiles", 0)}}
     mesh.set_precision("fp64")
     autotune.set_fft_shape(mesh, 0, 0)
+    autotune.set_legendre_shape(mesh, 0, 0)
     # the headline is the fastest Legendre precision within the metric's
     # 1e-6 gate (FP64 is exact by definition); split-FP32 and BF16x3
     # accumulate in FP32 blocks, so their accuracy is FP32-class (~1e-7)

@@ -57,6 +57,8 @@ SIGNATURES = {
     "jgcm_mesh_get_precision": [P, ctypes.POINTER(I)],
     "jgcm_mesh_set_fft_shape": [P, I, I],
     "jgcm_mesh_get_fft_shape": [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)],
+    "jgcm_mesh_set_legendre_shape": [P, I, I],
+    "jgcm_mesh_get_legendre_shape": [P, ctypes.POINTER(I), ctypes.POINTER(I)],
     "jgcm_mesh_table_size": [P, I, ctypes.POINTER(SZ)],
     "jgcm_mesh_get_table": [P, I, P, SZ],
     "jgcm_trans_grid_to_spherical": [P, P, P, I],

@@ -100,6 +100,9 @@ struct jgcm_mesh {
   int precision = JGCM_PREC_FP64;
   // latitude pairs per FFT workgroup (0: the built-in rule), set by an autotuner
   int fold_ppw = 0, unfold_ppw = 0;
+  // 16-row (forward) / 16-latitude (inverse) tiles per Legendre workgroup:
+  // 1, 2 or 4 (0: the built-in 2), set by an autotuner
+  int leg_tiles_fwd = 0, leg_tiles_inv = 0;
   hipStream_t stream = nullptr;
   bool stream_borrowed = false;   // stream owned by another mesh (jgcm_mesh_share_stream)
   hipEvent_t signal = nullptr;    // recorded on this mesh's stream by jgcm_mesh_wait

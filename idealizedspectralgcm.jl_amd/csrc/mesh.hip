@@ -560,6 +560,25 @@ int jgcm_mesh_set_fft_shape(jgcm_mesh_t* mesh, int fold_ppw, int unfold_ppw) {
   return JGCM_OK;
 }
 
+int jgcm_mesh_set_legendre_shape(jgcm_mesh_t* mesh, int fwd_tiles, int inv_tiles) {
+  if (int rc = check_mesh(mesh)) return rc;
+  auto ok = [](int t) { return t == 0 || t == 1 || t == 2 || t == 4; };
+  JGCM_REQUIRE(ok(fwd_tiles) && ok(inv_tiles),
+               "Legendre tiles per workgroup must be 0, 1, 2 or 4, got %d / %d", fwd_tiles,
+               inv_tiles);
+  mesh->leg_tiles_fwd = fwd_tiles;
+  mesh->leg_tiles_inv = inv_tiles;
+  return JGCM_OK;
+}
+
+int jgcm_mesh_get_legendre_shape(jgcm_mesh_t* mesh, int* fwd_tiles, int* inv_tiles) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(fwd_tiles && inv_tiles, "NULL argument");
+  *fwd_tiles = mesh->leg_tiles_fwd;
+  *inv_tiles = mesh->leg_tiles_inv;
+  return JGCM_OK;
+}
+
 int jgcm_mesh_get_fft_shape(jgcm_mesh_t* mesh, int* fold_ppw, int* unfold_ppw, int* max_ppw) {
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_REQUIRE(fold_ppw && unfold_ppw && max_ppw, "NULL argument");

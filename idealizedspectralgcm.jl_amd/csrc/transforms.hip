@@ -507,18 +507,16 @@ __global__ void fold_full_kernel(const double2* __restrict__ four, double2* __re
 
 // ------------------------------------------------------------ Legendre stage
 // Legendre tile shape: LCOLS real columns per workgroup (LCOLS / 2 complex
-// fields, one wave per 16 columns) and TILES result tiles of 16 rows (forward)
-// or 16 latitudes (inverse) per workgroup.
+// fields, one wave per 16 columns) and TL result tiles of 16 rows (forward)
+// or 16 latitudes (inverse) per workgroup.  TL is a run-time choice per mesh
+// and direction (jgcm_mesh_set_legendre_shape: 1, 2 or 4; 0 = DEF_TILES),
+// picked by the autotuner; the results do not depend on it.
 #ifndef JGCM_LCOLS
 #define JGCM_LCOLS 64
 #endif
-#ifndef JGCM_TILES
-#define JGCM_TILES 2
-#endif
 constexpr int LCOLS = JGCM_LCOLS;
 constexpr int LTHREADS = LCOLS * 4;
-constexpr int TILES = JGCM_TILES;
-constexpr int TROWS = 16 * TILES;
+constexpr int DEF_TILES = 2;
 
 // Table rows are read at wave-uniform addresses; going through the constant
 // address space lets the compiler use scalar loads (one load per wave,
@@ -595,7 +593,7 @@ typedef float fv4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 64;  // contraction steps (latitudes or rows) staged per pass
 static_assert(KC <= 64, "the staging maps one lane to each contraction step of a chunk");
-static_assert((TROWS & (TROWS - 1)) == 0 && LTHREADS % 64 == 0, "tile shape");
+static_assert(LTHREADS % 64 == 0, "tile shape");
 constexpr int KS1 = KC + 1;
 
 __device__ __forceinline__ dv4 mma64(double a, double b, dv4 c) {
@@ -808,9 +806,11 @@ __device__ __forceinline__ uint3 wg_coords() {
 // grid: x = segment (m, par); y = group of 64 rows (4 tiles); z = (job,
 // chunk of 32 fields).  256 threads: wave w owns columns 16w .. 16w+15.
 // LDS: X [64 cols][KC+1], P [64 rows][KC+1].
-template <int PREC>
+template <int PREC, int TL>
 __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int ns, int nm, int half,
                                                          int nmax, int zero_top) {
+  constexpr int TROWS = 16 * TL;
+  static_assert(TL == 1 || TL == 2 || TL == 4, "1, 2 or 4 tiles of 16 rows");
   using LT = typename LdsT<PREC>::T;
   using TT = typename TabT<PREC>::T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1023,11 +1023,13 @@ __device__ __forceinline__ double2 staged_coefficient_at(const double2* h, const
 // LDS: S [64 cols][KC+1], P [KC rows][KC+1].
 // PLAIN: every field comes from in.ptr[0] without an operator (checked by the
 // launcher): the staging skips the segment and operator logic.
-template <int PREC, bool PLAIN>
+template <int PREC, bool PLAIN, int TL>
 __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2* __restrict__ eo,
                                                          FwdTables tab, int B, int nmax, int ns,
                                                          int nm, int half,
                                                          SpecOpTables ops, int out_mode) {
+  constexpr int TROWS = 16 * TL;
+  static_assert(TL == 1 || TL == 2 || TL == 4, "1, 2 or 4 tiles of 16 latitudes");
   using LT = typename LdsT<PREC>::T;
   using TT = typename TabT<PREC>::T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1299,11 +1301,14 @@ static FwdTables tables_for(const jgcm_mesh* M, int which) {
 }
 
 // LDS of the Legendre kernels: operand block [64][KC+1] + table block [64][KC+1]
-static size_t legendre_lds(int precision) {
+static size_t legendre_lds(int precision, int tl) {
+  const size_t trows = 16 * (size_t)tl;
   size_t e = precision == JGCM_PREC_FP32 ? sizeof(float) : 8;  // double, float2
-  size_t pf = (size_t)TROWS * KS1, pi = (size_t)KC * (TROWS + 1);
+  size_t pf = trows * KS1, pi = (size_t)KC * (trows + 1);
   return align16(e * LCOLS * KS1) + e * (pf > pi ? pf : pi);
 }
+
+static int tiles_of(int t) { return t > 0 ? t : DEF_TILES; }
 
 static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int zero_top) {
   int z = 0;
@@ -1313,28 +1318,28 @@ static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int 
   }
   jobs.zstart[jobs.njobs] = z;
   if (z == 0) return JGCM_OK;
+  const int tl = tiles_of(M->leg_tiles_fwd);
   int maxrows = nmax / 2 + 1;
-  int rgroups = ((maxrows + 15) / 16 + TILES - 1) / TILES;
+  int rgroups = ((maxrows + 15) / 16 + tl - 1) / tl;
   dim3 grd(2 * M->nm, rgroups, z);
-  size_t lds = legendre_lds(M->precision);
+  size_t lds = legendre_lds(M->precision, tl);
+#define JGCM_FWD_LAUNCH(P, T)                                                                   \
+  hipLaunchKernelGGL((legendre_fwd_mfma<P, T>), grd, dim3(LTHREADS), lds, M->stream, jobs,     \
+                     M->ns, M->nm, M->half, nmax, zero_top)
+#define JGCM_FWD_TILES(P)                                                                       \
+  do {                                                                                          \
+    if (tl == 1) JGCM_FWD_LAUNCH(P, 1);                                                         \
+    else if (tl == 4) JGCM_FWD_LAUNCH(P, 4);                                                    \
+    else JGCM_FWD_LAUNCH(P, 2);                                                                 \
+  } while (0)
   switch (M->precision) {
-    case JGCM_PREC_FP64:
-      hipLaunchKernelGGL(legendre_fwd_mfma<P_FP64>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
-                         M->nm, M->half, nmax, zero_top);
-      break;
-    case JGCM_PREC_FP32:
-      hipLaunchKernelGGL(legendre_fwd_mfma<P_FP32>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
-                         M->nm, M->half, nmax, zero_top);
-      break;
-    case JGCM_PREC_BF16X3:
-      hipLaunchKernelGGL(legendre_fwd_mfma<P_BF16>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
-                         M->nm, M->half, nmax, zero_top);
-      break;
-    default:
-      hipLaunchKernelGGL(legendre_fwd_mfma<P_SPLIT>, grd, dim3(LTHREADS), lds, M->stream, jobs, M->ns,
-                         M->nm, M->half, nmax, zero_top);
-      break;
+    case JGCM_PREC_FP64: JGCM_FWD_TILES(P_FP64); break;
+    case JGCM_PREC_FP32: JGCM_FWD_TILES(P_FP32); break;
+    case JGCM_PREC_BF16X3: JGCM_FWD_TILES(P_BF16); break;
+    default: JGCM_FWD_TILES(P_SPLIT); break;
   }
+#undef JGCM_FWD_TILES
+#undef JGCM_FWD_LAUNCH
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
@@ -1342,20 +1347,27 @@ static int launch_legendre_fwd(const jgcm_mesh* M, FwdJobs& jobs, int nmax, int 
 static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, int table,
                             int nmax, int out_mode = 0) {
   int B = snm.start[snm.nseg];
+  const int tl = tiles_of(M->leg_tiles_inv);
+  const int trows = 16 * tl;
   int fchunks = (B + LCOLS / 2 - 1) / (LCOLS / 2);
-  dim3 grd(2 * M->nm, fchunks, (M->half + TROWS - 1) / TROWS);
+  dim3 grd(2 * M->nm, fchunks, (M->half + trows - 1) / trows);
   FwdTables tab = tables_for(M, table);
-  size_t lds = legendre_lds(M->precision);
+  size_t lds = legendre_lds(M->precision, tl);
   SpecOpTables ops{M->d_dlam, M->d_dthm, M->d_dthp, M->d_uvc, M->d_uvm, M->d_uvp};
   const bool plain = snm.nseg == 1 && ((snm.flag[0] >> 1) & 7) == SPEC_OP_NONE;
+#define JGCM_INV_LAUNCH1(P, PL, T)                                                              \
+  hipLaunchKernelGGL((legendre_inv_mfma<P, PL, T>), grd, dim3(LTHREADS), lds, M->stream, snm,   \
+                     eo, tab, B, nmax, M->ns, M->nm, M->half, ops, out_mode)
+#define JGCM_INV_TILES(P, PL)                                                                   \
+  do {                                                                                          \
+    if (tl == 1) JGCM_INV_LAUNCH1(P, PL, 1);                                                    \
+    else if (tl == 4) JGCM_INV_LAUNCH1(P, PL, 4);                                               \
+    else JGCM_INV_LAUNCH1(P, PL, 2);                                                            \
+  } while (0)
 #define JGCM_INV_LAUNCH(P)                                                                      \
   do {                                                                                          \
-    if (plain)                                                                                  \
-      hipLaunchKernelGGL((legendre_inv_mfma<P, true>), grd, dim3(LTHREADS), lds, M->stream,     \
-                         snm, eo, tab, B, nmax, M->ns, M->nm, M->half, ops, out_mode);          \
-    else                                                                                        \
-      hipLaunchKernelGGL((legendre_inv_mfma<P, false>), grd, dim3(LTHREADS), lds, M->stream,    \
-                         snm, eo, tab, B, nmax, M->ns, M->nm, M->half, ops, out_mode);          \
+    if (plain) JGCM_INV_TILES(P, true);                                                         \
+    else JGCM_INV_TILES(P, false);                                                              \
   } while (0)
   switch (M->precision) {
     case JGCM_PREC_FP64: JGCM_INV_LAUNCH(P_FP64); break;
@@ -1364,6 +1376,8 @@ static int legendre_inv_map(const jgcm_mesh* M, const SegMap& snm, double2* eo, 
     default: JGCM_INV_LAUNCH(P_SPLIT); break;
   }
 #undef JGCM_INV_LAUNCH
+#undef JGCM_INV_TILES
+#undef JGCM_INV_LAUNCH1
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

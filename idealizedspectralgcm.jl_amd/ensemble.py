@@ -110,8 +110,8 @@ class SW_Ensemble:
     """Mixed-truncation ensemble: groups = [(N, nθ, count, dt), ...]."""
 
     def __init__(self, groups, seed=0, precision="fp32", tuning=None, concurrent=False):
-        """`tuning`: None (built-in FFT shapes) or an autotune.Tuning_Cache; each
-        truncation's FFT shape (for its instance count) is then read from the
+        """`tuning`: None (built-in shapes) or an autotune.Tuning_Cache; each
+        truncation's FFT and Legendre shapes (for its instance count) are then read from the
         cache, or tuned and stored when missing, before any step is captured.
         By default the truncations share the first group's stream and run one
         after another in the captured step; with `concurrent=True` each
@@ -132,7 +132,8 @@ class SW_Ensemble:
                 owner = g
             if tuning is not None:
                 e = Apply_Tuning(g.mesh, count, cache=tuning, tune_missing=True)
-                self.tuning[f"T{N}"] = None if e is None else (e["fold_ppw"], e["unfold_ppw"])
+                self.tuning[f"T{N}"] = None if e is None else \
+                    (e["fold_ppw"], e["unfold_ppw"], e.get("fwd_tiles", 0), e.get("inv_tiles", 0))
             self.groups.append(g)
         self.owner = owner.mesh
         self.graph = None

@@ -146,6 +146,12 @@ int jgcm_mesh_get_precision(jgcm_mesh_t* mesh, int* precision);
  * (autotune.py), which times the candidates and keeps the fastest.          */
 int jgcm_mesh_set_fft_shape(jgcm_mesh_t* mesh, int fold_ppw, int unfold_ppw);
 int jgcm_mesh_get_fft_shape(jgcm_mesh_t* mesh, int* fold_ppw, int* unfold_ppw, int* max_ppw);
+/* Legendre-stage tile shape (no reference counterpart): 16-row (forward) or
+ * 16-latitude (inverse) result tiles per workgroup, 1, 2 or 4 (0 = the
+ * built-in 2), per direction; set by the autotuner.  Results do not depend
+ * on it.                                                                    */
+int jgcm_mesh_set_legendre_shape(jgcm_mesh_t* mesh, int fwd_tiles, int inv_tiles);
+int jgcm_mesh_get_legendre_shape(jgcm_mesh_t* mesh, int* fwd_tiles, int* inv_tiles);
 
 /* Host copies of the mesh's public fields (the Julia struct's fields). */
 #define JGCM_TABLE_SIN_THETA 0     /* [nθ]                         */

@@ -47,3 +47,18 @@ def test_default_path_follows_environment(monkeypatch, tmp_path):
     assert autotune.default_cache_path() == str(tmp_path / "x.json")
     monkeypatch.delenv("JGCM_TUNING_FILE")
     assert autotune.default_cache_path().endswith("jgcm_amd/tuning.json")
+
+
+def test_tile_candidates_are_the_library_shapes():
+    # jgcm_mesh_set_legendre_shape accepts 0 (built-in), 1, 2 and 4
+    assert autotune.TILE_CANDIDATES == (1, 2, 4)
+    assert autotune.select({1: 24.0, 2: 23.0, 4: 23.0}, {1: 20.0, 2: 21.0, 4: 22.0}) == (2, 1)
+
+
+def test_cache_of_the_previous_format_is_ignored(tmp_path):
+    # entries of the FFT-only format carry no Legendre tiles; they are not read
+    path = tmp_path / "t.json"
+    old = {"version": 1, "entries": {"d": {"k": {"fold_ppw": 4, "unfold_ppw": 8}}}}
+    path.write_text(json.dumps(old))
+    assert autotune.VERSION != 1
+    assert autotune.Tuning_Cache(str(path)).get("d", "k") is None
