@@ -10,8 +10,12 @@ namespace jgcm {
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// Complex product with the fused multiply-adds spelled out, so every kernel
+// instantiation rounds it the same way (left to the compiler, the choice of
+// the product that is fused can differ between instantiations, and paths that
+// must agree bit for bit, such as a batched and a single transform, would not).
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_double2(__fma_rn(a.x, b.x, -(a.y * b.y)), __fma_rn(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
@@ -133,6 +137,100 @@ __device__ __forceinline__ void fft_lds(double2* buf, const double2* tw, int j) 
   for (int s = 1; s < NR4; ++s) {
     stage_r4<L, SIGN>(buf, tw, j, Ns);
     Ns *= 4;
+  }
+}
+
+// ------------------------------------------------ register DFTs (four-step)
+// Forward DFTs of 4, 8 and 16 points held in registers, natural order in and
+// out: X[k] = sum_n x[n] exp(-2 pi i n k / N).  Constant twiddles are the
+// exact values cos(j pi / 8), sin(j pi / 8).
+__device__ __forceinline__ double2 mul_negi(double2 a) { return make_double2(a.y, -a.x); }
+
+__device__ __forceinline__ void dft4r(double2& x0, double2& x1, double2& x2, double2& x3) {
+  double2 a = cadd(x0, x2), b = csub(x0, x2), c = cadd(x1, x3), d = mul_negi(csub(x1, x3));
+  x0 = cadd(a, c);
+  x2 = csub(a, c);
+  x1 = cadd(b, d);
+  x3 = csub(b, d);
+}
+
+constexpr double COS_PI8 = 0.92387953251128675613;  // cos(pi / 8)
+constexpr double SIN_PI8 = 0.38268343236508977173;  // sin(pi / 8)
+constexpr double RSQRT2 = 0.70710678118654752440;   // cos(pi / 4)
+
+template <int N>
+__device__ __forceinline__ void dft_reg(double2* x) {
+  static_assert(N == 4 || N == 8 || N == 16, "4, 8 or 16 points");
+  if constexpr (N == 4) {
+    dft4r(x[0], x[1], x[2], x[3]);
+  } else if constexpr (N == 8) {
+    // radix 2 x 4: n = 2a + b; inner 4-point over a, twiddle W8^{bc}, outer 2-point
+    dft4r(x[0], x[2], x[4], x[6]);
+    dft4r(x[1], x[3], x[5], x[7]);
+    x[3] = cmul(x[3], make_double2(RSQRT2, -RSQRT2));
+    x[5] = mul_negi(x[5]);
+    x[7] = cmul(x[7], make_double2(-RSQRT2, -RSQRT2));
+    double2 y[8];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      y[c] = cadd(x[2 * c], x[2 * c + 1]);
+      y[c + 4] = csub(x[2 * c], x[2 * c + 1]);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] = y[i];
+  } else {
+    // radix 4 x 4: n = 4a + b; inner 4-point over a, twiddle W16^{bc}, outer over b
+#pragma unroll
+    for (int b = 0; b < 4; ++b) dft4r(x[b], x[4 + b], x[8 + b], x[12 + b]);
+    const double2 w1 = make_double2(COS_PI8, -SIN_PI8), w2 = make_double2(RSQRT2, -RSQRT2);
+    const double2 w3 = make_double2(SIN_PI8, -COS_PI8), w6 = make_double2(-RSQRT2, -RSQRT2);
+    const double2 w9 = make_double2(-COS_PI8, SIN_PI8);
+    x[5] = cmul(x[5], w1);
+    x[9] = cmul(x[9], w2);
+    x[13] = cmul(x[13], w3);
+    x[6] = cmul(x[6], w2);
+    x[10] = mul_negi(x[10]);
+    x[14] = cmul(x[14], w6);
+    x[7] = cmul(x[7], w3);
+    x[11] = cmul(x[11], w6);
+    x[15] = cmul(x[15], w9);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) dft4r(x[4 * c], x[4 * c + 1], x[4 * c + 2], x[4 * c + 3]);
+    double2 t[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) t[i] = x[i];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int d = 0; d < 4; ++d) x[c + 4 * d] = t[4 * c + d];
+  }
+}
+
+// Four-step FFT of L = 16 P points (P = 4, 8, 16) by the P lanes of one
+// group, n = P n1 + n2, k = k1 + 16 k2:
+//   step 1: lane n2 holds x[P n1 + n2], n1 = 0..15, and takes the 16-point
+//           DFT over n1 (Y[n2][k1]), times W_L^{n2 k1};
+//   step 2: through LDS, lane l takes the P-point DFTs over n2 of the 16 / P
+//           columns k1 = l + P r, giving X[k1 + 16 k2] in x[r P + k2].
+// ex: the group's exchange block, 16 (P + 1) entries.  tw[k] = exp(-2 pi i k / L).
+template <int P>
+__device__ __forceinline__ void fft_four_step(double2 (&x)[16], double2* ex,
+                                              const double2* __restrict__ tw, int l) {
+  constexpr int L = 16 * P, EP = P + 1, R = 16 / P;
+  double2 w[16];
+#pragma unroll
+  for (int k1 = 1; k1 < 16; ++k1) w[k1] = tw[(l * k1) & (L - 1)];
+  dft_reg<16>(x);
+#pragma unroll
+  for (int k1 = 1; k1 < 16; ++k1) x[k1] = cmul(x[k1], w[k1]);
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) ex[k1 * EP + l] = x[k1];
+  gsync<64>();  // the group's lanes are in one wavefront
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+#pragma unroll
+    for (int n2 = 0; n2 < P; ++n2) x[r * P + n2] = ex[(l + P * r) * EP + n2];
+    dft_reg<P>(x + r * P);
   }
 }
 

@@ -360,6 +360,160 @@ __global__ __launch_bounds__(256) void fft_full_kernel(const double* __restrict_
   }
 }
 
+// ------------------------------------------ four-step FFT kernels (L = 64..256)
+// The same fold / unfold as the radix-4 kernels above for nλ = 16 P, P = 4, 8
+// or 16: each latitude pair is one group of P lanes holding 16 points each in
+// registers (fft_four_step), so a transform costs two register DFTs and one
+// LDS exchange instead of log4(L) LDS passes.  G latitude pairs per
+// workgroup (P G threads, 64 <= P G <= 256).  Per group, LDS holds the
+// exchange block (16 (P + 1) entries, reused for the spectrum) and the
+// (m, parity) staging row of 2 nm + 1 entries.
+template <int P, int G, bool PLAIN>
+__global__ __launch_bounds__(P * G) void fold_four_step_kernel(
+    SegMap in, double2* __restrict__ eo, const double2* __restrict__ twg,
+    const double* __restrict__ cosv, int nlat, int nm) {
+  constexpr int L = 16 * P, EX = 16 * (P + 1), R = 16 / P;
+  __shared__ double2 ex[G][EX];
+  extern __shared__ double2 stage[];  // [2 nm][G]: (m, parity) rows of the G pairs
+  const int half = nlat / 2;
+  const int b = blockIdx.y;
+  const int g = threadIdx.x / P, l = threadIdx.x % P;
+  const int j0 = blockIdx.x * G;
+  const int jp = j0 + g;
+  const bool valid = jp < half;
+  const int seg = PLAIN ? 0 : seg_find(in, b);
+  const double* __restrict__ grid = in.ptr[seg] + (size_t)(b - in.start[seg]) * nlat * L;
+  const int jr = valid ? jp : 0;
+  const double* ra = grid + (size_t)jr * L;
+  const double* rc = grid + (size_t)(nlat - 1 - jr) * L;
+  double2 x[16];
+#pragma unroll
+  for (int n1 = 0; n1 < 16; ++n1) x[n1] = make_double2(ra[P * n1 + l], rc[P * n1 + l]);
+  if (!PLAIN && (in.flag[seg] & 1)) {  // rows times cos θ, or with bit 1 divided by it
+    const double c = cosv[jr];
+    const bool div = (in.flag[seg] & 2) != 0;
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1)
+      x[n1] = div ? make_double2(x[n1].x / c, x[n1].y / c) : make_double2(x[n1].x * c, x[n1].y * c);
+  }
+  fft_four_step<P>(x, ex[g], twg, l);
+  // x[r P + k2] = Z[k1 + 16 k2], k1 = l + P r.  Z_{L-m} for m = k1 + 16 k2:
+  // k1 != 0: lane (P - l) % P, register (R - 1 - r) P + (P - 1 - k2) when
+  // l != 0; for l = 0 the lane's own register (R - r) % R, (P - k2) % P when
+  // r = 0, else R - r, P - 1 - k2.
+  // z = a + i c: A = (Z_m + conj Z_{L-m}) / 2, C = (Z_m - conj Z_{L-m}) / (2i);
+  // E = (A + C) / L, O = (A - C) / L
+  const int src = (threadIdx.x & 63) - l + ((P - l) & (P - 1));
+  const double scale = 1.0 / L;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k2 = 0; k2 < P; ++k2) {
+      if (P * r + 16 * k2 >= nm) break;  // no lane has m < nm (wave-uniform)
+      const int m = l + P * r + 16 * k2;
+      const double2 q = x[(R - 1 - r) * P + (P - 1 - k2)];
+      double2 zc = make_double2(__shfl(q.x, src, 64), __shfl(q.y, src, 64));
+      if (l == 0) zc = r == 0 ? x[(P - k2) & (P - 1)] : x[(R - r) * P + (P - 1 - k2)];
+      zc.y = -zc.y;
+      const double2 zm = x[r * P + k2];
+      const double2 A = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      const double2 d = make_double2(0.5 * (zm.x - zc.x), 0.5 * (zm.y - zc.y));
+      const double2 C = make_double2(d.y, -d.x);
+      if (m < nm) {
+        stage[(2 * m + 0) * G + g] = make_double2((A.x + C.x) * scale, (A.y + C.y) * scale);
+        stage[(2 * m + 1) * G + g] = make_double2((A.x - C.x) * scale, (A.y - C.y) * scale);
+      }
+    }
+  __syncthreads();
+  const int total = 2 * nm * G;
+  for (int idx = threadIdx.x; idx < total; idx += P * G) {
+    const int jj = idx % G, mp = idx / G;
+    if (j0 + jj < half) eo[((size_t)b * nm * 2 + mp) * half + j0 + jj] = stage[idx];
+  }
+}
+
+template <int P, int G, bool PLAIN>
+__global__ __launch_bounds__(P * G) void unfold_four_step_kernel(
+    const double2* __restrict__ eo, SegMap out, const double2* __restrict__ twg,
+    const double* __restrict__ cosv, int nlat, int nm) {
+  constexpr int L = 16 * P, EX = 16 * (P + 1);
+  __shared__ double2 ex[G][EX];
+  extern __shared__ double2 stage[];  // [G][2 nm + 1]
+  const int half = nlat / 2;
+  const int SP = 2 * nm + 1;
+  const int b = blockIdx.y;
+  const int g = threadIdx.x / P, l = threadIdx.x % P;
+  const int j0 = blockIdx.x * G;
+  const int jp = j0 + g;
+  const int seg = PLAIN ? 0 : seg_find(out, b);
+  double* __restrict__ grid = out.ptr[seg] + (size_t)(b - out.start[seg]) * nlat * L;
+  const int flag = PLAIN ? 0 : out.flag[seg];
+  const double w0 = 1.0 / L, w1 = 1.0 / (2.0 * L);
+  const int total = 2 * nm * G;
+  staged_copy<8>(
+      total,
+      [&](int idx) {
+        const int jj = idx % G, mp = idx / G;
+        return (j0 + jj < half) ? eo[((size_t)b * nm * 2 + mp) * half + j0 + jj]
+                                : make_double2(0.0, 0.0);
+      },
+      [&](int idx, double2 v) {
+        const int jj = idx % G, mp = idx / G;
+        if (flag & 2) {  // transposed Hermitian weights of the forward FFT
+          const double s = mp < 2 ? w0 : w1;
+          v = make_double2(v.x * s, v.y * s);
+        }
+        stage[jj * SP + mp] = v;
+      });
+  __syncthreads();
+  // spectrum Z_m = g_a + i g_c (m >= 1) with Z_{L-m} = conj g_a + i conj g_c
+  // and Z_0 = (Re g_a, Re g_c), g_a = E + O, g_c = E - O; the inverse FFT is
+  // the conjugate of the forward four-step FFT of conj Z
+  const double2* st = stage + g * SP;
+  double2 x[16];
+#pragma unroll
+  for (int n1 = 0; n1 < 16; ++n1) {
+    const int m = P * n1 + l;
+    double2 z = make_double2(0.0, 0.0);
+    if (m == 0) {
+      const double2 E = st[0], O = st[1];
+      z = make_double2(E.x + O.x, E.x - O.x);
+    } else if (m < nm) {
+      const double2 ga = cadd(st[2 * m], st[2 * m + 1]), gc = csub(st[2 * m], st[2 * m + 1]);
+      z = make_double2(ga.x - gc.y, ga.y + gc.x);
+    } else if (L - m < nm) {
+      const int q = L - m;
+      const double2 ga = cadd(st[2 * q], st[2 * q + 1]), gc = csub(st[2 * q], st[2 * q + 1]);
+      z = make_double2(ga.x + gc.y, gc.x - ga.y);
+    }
+    x[n1] = cconj(z);
+  }
+  fft_four_step<P>(x, ex[g], twg, l);
+  if (jp < half) {
+    double* ra = grid + (size_t)jp * L;
+    double* rc = grid + (size_t)(nlat - 1 - jp) * L;
+    const double c = (flag & 1) ? cosv[jp] : 1.0;
+#pragma unroll
+    for (int r = 0; r < 16 / P; ++r)
+#pragma unroll
+      for (int k2 = 0; k2 < P; ++k2) {
+        const int i = l + P * r + 16 * k2;
+        double va = x[r * P + k2].x, vc = -x[r * P + k2].y;
+        if (flag & 1) {  // divided by cos θ, or with bit 2 multiplied
+          if (flag & 4) {
+            va *= c;
+            vc *= c;
+          } else {
+            va /= c;
+            vc /= c;
+          }
+        }
+        ra[i] = va;
+        rc[i] = vc;
+      }
+  }
+}
+
 // ------------------------------------------------- general-length longitude DFT
 // For nλ that is not a power of two (the reference's FFTW takes any length):
 // direct sums over the twiddle table tw[k] = exp(-2πi k/nλ), indexed by
@@ -1187,7 +1341,29 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
 // -1.3 us at 20, +0.9 us at 61).
 // Largest latitude-pair count whose staging buffer, with the kernels' static
 // LDS (FPW + 1 rows of L complex values), fits in 64 KB.
+// nλ = 64, 128, 256 with 2 (nm - 1) < nλ run the four-step kernels
+static bool use_four_step(const jgcm_mesh* M) {
+  const int L = M->nlon;
+  return (L == 64 || L == 128 || L == 256) && 2 * (M->nm - 1) < L;
+}
+
+// four-step kernels: latitude pairs per workgroup G = 64 / P, 128 / P or
+// 256 / P (one to four wavefronts), the largest limited to what fits in 64 KB
+static size_t four_step_lds_per_pair(const jgcm_mesh* M) {
+  const int P = M->nlon / 16;
+  return (size_t)(16 * (P + 1) + 2 * M->nm + 1) * sizeof(double2);
+}
+
+static int four_step_pairs(const jgcm_mesh* M, int ppw) {
+  const int P = M->nlon / 16;
+  const size_t per = four_step_lds_per_pair(M);
+  int g = 64 / P;
+  while (g < ppw && 2 * g * P <= 256 && (size_t)(2 * g) * per <= 64 * 1024) g *= 2;
+  return g;
+}
+
 int fft_max_ppw(const jgcm_mesh* M) {
+  if (use_four_step(M)) return four_step_pairs(M, 1 << 20);
   const int L = M->nlon;
   const int fpw = 256 / (L / 4);
   const long stat = (long)(fpw + 1) * L * 16;
@@ -1201,6 +1377,57 @@ static int pick_ppw(int L, int B, bool fold) {
   int fpw = 256 / (L / 4);
   if (fpw == 4 && (fold || B < 48)) return 4;
   return fpw >= 8 ? fpw : 8;
+}
+
+template <int P, int G>
+static void launch_fold_fs(const jgcm_mesh* M, const SegMap& in, double2* eo, int B) {
+  dim3 grd((M->half + G - 1) / G, B);
+  size_t lds = (size_t)(2 * M->nm) * G * sizeof(double2);
+  const double2* tw = reinterpret_cast<const double2*>(M->d_twiddle);
+  if (in.nseg == 1 && in.flag[0] == 0)
+    hipLaunchKernelGGL((fold_four_step_kernel<P, G, true>), grd, dim3(P * G), lds, M->stream, in,
+                       eo, tw, M->d_cos, M->nlat, M->nm);
+  else
+    hipLaunchKernelGGL((fold_four_step_kernel<P, G, false>), grd, dim3(P * G), lds, M->stream, in,
+                       eo, tw, M->d_cos, M->nlat, M->nm);
+}
+
+template <int P, int G>
+static void launch_unfold_fs(const jgcm_mesh* M, const double2* eo, const SegMap& out, int B) {
+  dim3 grd((M->half + G - 1) / G, B);
+  size_t lds = (size_t)(2 * M->nm + 1) * G * sizeof(double2);
+  const double2* tw = reinterpret_cast<const double2*>(M->d_twiddle);
+  if (out.nseg == 1 && out.flag[0] == 0)
+    hipLaunchKernelGGL((unfold_four_step_kernel<P, G, true>), grd, dim3(P * G), lds, M->stream,
+                       eo, out, tw, M->d_cos, M->nlat, M->nm);
+  else
+    hipLaunchKernelGGL((unfold_four_step_kernel<P, G, false>), grd, dim3(P * G), lds, M->stream,
+                       eo, out, tw, M->d_cos, M->nlat, M->nm);
+}
+
+// wavefronts per workgroup W = P G / 64 from the requested latitude pairs
+#define JGCM_FS_DISPATCH(P_, G_, CALL)                                         \
+  switch (P_ * 100 + G_) {                                                     \
+    case 1604: { constexpr int PP = 16, GG = 4; CALL; } break;                 \
+    case 1608: { constexpr int PP = 16, GG = 8; CALL; } break;                 \
+    case 1616: { constexpr int PP = 16, GG = 16; CALL; } break;                \
+    case 808: { constexpr int PP = 8, GG = 8; CALL; } break;                   \
+    case 816: { constexpr int PP = 8, GG = 16; CALL; } break;                  \
+    case 832: { constexpr int PP = 8, GG = 32; CALL; } break;                  \
+    case 416: { constexpr int PP = 4, GG = 16; CALL; } break;                  \
+    case 432: { constexpr int PP = 4, GG = 32; CALL; } break;                  \
+    case 464: { constexpr int PP = 4, GG = 64; CALL; } break;                  \
+    default: break;                                                            \
+  }
+
+static void fold_four_step(const jgcm_mesh* M, const SegMap& in, double2* eo, int B) {
+  const int P = M->nlon / 16, G = four_step_pairs(M, M->fold_ppw);
+  JGCM_FS_DISPATCH(P, G, (launch_fold_fs<PP, GG>(M, in, eo, B)));
+}
+
+static void unfold_four_step(const jgcm_mesh* M, const double2* eo, const SegMap& out, int B) {
+  const int P = M->nlon / 16, G = four_step_pairs(M, M->unfold_ppw);
+  JGCM_FS_DISPATCH(P, G, (launch_unfold_fs<PP, GG>(M, eo, out, B)));
 }
 
 template <int L>
@@ -1252,7 +1479,9 @@ static void launch_full(const jgcm_mesh* M, const double* grid, double2* four, i
   }
 
 static int fft_fold_map(const jgcm_mesh* M, const SegMap& in, double2* eo) {
-  if (pow2_fft(M->nlon)) {
+  if (use_four_step(M)) {
+    fold_four_step(M, in, eo, in.start[in.nseg]);
+  } else if (pow2_fft(M->nlon)) {
     JGCM_DISPATCH_L(M->nlon, launch_fold<LL>(M, in, eo, in.start[in.nseg]));
   } else {
     const size_t lds = (size_t)M->nlon * (sizeof(double2) + 2 * sizeof(double));
@@ -1265,7 +1494,9 @@ static int fft_fold_map(const jgcm_mesh* M, const SegMap& in, double2* eo) {
 }
 
 static int ifft_unfold_map(const jgcm_mesh* M, const double2* eo, const SegMap& out) {
-  if (pow2_fft(M->nlon)) {
+  if (use_four_step(M)) {
+    unfold_four_step(M, eo, out, out.start[out.nseg]);
+  } else if (pow2_fft(M->nlon)) {
     JGCM_DISPATCH_L(M->nlon, launch_unfold<LL>(M, eo, out, out.start[out.nseg]));
   } else {
     const size_t lds = ((size_t)M->nlon + 2 * (size_t)M->nm) * sizeof(double2);

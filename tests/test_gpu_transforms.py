@@ -13,8 +13,10 @@ from oracle import SpectralSphericalMesh
 pytestmark = pytest.mark.gpu
 
 RADIUS = 6371.0e3
+# 64, 128, 256 longitudes: the four-step register FFT; 32 and 512: the
+# radix-4 LDS FFT; 96 and 192: the direct-DFT path (not powers of two)
 TRUNCS = [(21, 32), (42, 64), (85, 128),
-          (31, 48), (63, 96)]   # 96 and 192 longitudes: the direct-DFT path (not powers of two)
+          (31, 48), (63, 96), (10, 16)]
 _CACHE = {}
 
 
@@ -71,7 +73,7 @@ def test_forward_inverse_fp64(jgcm, N, nlat):
     assert rel(pd.cpu().numpy(), p_ref) < 1e-12
 
 
-@pytest.mark.parametrize("N,nlat", TRUNCS)
+@pytest.mark.parametrize("N,nlat", TRUNCS + [(127, 256)])
 def test_batched_61_fields_fp64(jgcm, N, nlat):
     """61 level-fields in one launch (vor 20, div 20, T 20, lnps 1)."""
     B = 61
