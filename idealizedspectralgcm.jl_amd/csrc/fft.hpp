@@ -140,6 +140,15 @@ __device__ __forceinline__ void fft_lds(double2* buf, const double2* tw, int j) 
   }
 }
 
+// Streaming (non-temporal) stores for outputs written once in whole 128-byte
+// lines: they leave the L2 as the kernel runs instead of in the write-back at
+// its end, which otherwise delays the next kernel of the stream.
+__device__ __forceinline__ void store_stream(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_stream(double2* p, double2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+
 // ------------------------------------------------ register DFTs (four-step)
 // Forward DFTs of 4, 8 and 16 points held in registers, natural order in and
 // out: X[k] = sum_n x[n] exp(-2 pi i n k / N).  Constant twiddles are the

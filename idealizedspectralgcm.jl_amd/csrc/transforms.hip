@@ -207,11 +207,16 @@ __global__ __launch_bounds__(256) void fft_fold_kernel(
   }
   __syncthreads();
   const int total = nm * 2 * ppw;
+  const bool whole_lines = ppw >= 8;  // (m, parity) rows of >= 128 bytes
   for (int idx = threadIdx.x; idx < total; idx += 256) {
     int jj = idx % ppw;
     int mp = idx / ppw;
     int jp = j0 + jj;
-    if (jp < half) eo[((size_t)b * nm * 2 + mp) * half + jp] = stage[jj * SP + mp];
+    if (jp < half) {
+      double2* o = eo + ((size_t)b * nm * 2 + mp) * half + jp;
+      if (whole_lines) store_stream(o, stage[jj * SP + mp]);
+      else *o = stage[jj * SP + mp];
+    }
   }
 }
 
@@ -302,14 +307,14 @@ __global__ __launch_bounds__(256) void ifft_unfold_kernel(
         int i = t + r * T;
         double2 z = buf[g][i];
         if (divcos && mulcos) {
-          ra[i] = z.x * c;
-          rc[i] = z.y * c;
+          store_stream(ra + i, z.x * c);
+          store_stream(rc + i, z.y * c);
         } else if (divcos) {
-          ra[i] = z.x / c;
-          rc[i] = z.y / c;
+          store_stream(ra + i, z.x / c);
+          store_stream(rc + i, z.y / c);
         } else {
-          ra[i] = z.x;
-          rc[i] = z.y;
+          store_stream(ra + i, z.x);
+          store_stream(rc + i, z.y);
         }
       }
     }
@@ -360,7 +365,7 @@ __global__ __launch_bounds__(256) void fft_full_kernel(const double* __restrict_
   }
 }
 
-// ------------------------------------------ four-step FFT kernels (L = 64..256)
+// ------------------------------------------ four-step FFT kernels (L = 128, 256)
 // The same fold / unfold as the radix-4 kernels above for nλ = 16 P, P = 4, 8
 // or 16: each latitude pair is one group of P lanes holding 16 points each in
 // registers (fft_four_step), so a transform costs two register DFTs and one
@@ -425,10 +430,15 @@ __global__ __launch_bounds__(P * G) void fold_four_step_kernel(
       }
     }
   __syncthreads();
+  // G >= 8: each (m, parity) row of the workgroup is a whole 128-byte line
   const int total = 2 * nm * G;
   for (int idx = threadIdx.x; idx < total; idx += P * G) {
     const int jj = idx % G, mp = idx / G;
-    if (j0 + jj < half) eo[((size_t)b * nm * 2 + mp) * half + j0 + jj] = stage[idx];
+    if (j0 + jj < half) {
+      double2* o = eo + ((size_t)b * nm * 2 + mp) * half + j0 + jj;
+      if constexpr (G >= 8) store_stream(o, stage[idx]);
+      else *o = stage[idx];
+    }
   }
 }
 
@@ -508,8 +518,8 @@ __global__ __launch_bounds__(P * G) void unfold_four_step_kernel(
             vc /= c;
           }
         }
-        ra[i] = va;
-        rc[i] = vc;
+        store_stream(ra + i, va);
+        store_stream(rc + i, vc);
       }
   }
 }
@@ -1309,26 +1319,35 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
         acc, nlt, KSc, [&](int ks, int e) { return Sc[KSTEP * ks + e]; },
         [&](int t, int ks, int e) { return Pc[(KSTEP * ks + e) * PSI + 16 * t]; });
   }
+  // The tile goes through LDS ([field][latitude] complex) so that each
+  // field's latitudes leave as one contiguous run of whole lines.
   const int col = 16 * wave + li;
   const int fl = col >> 1, comp = col & 1;
-  // out_mode bit 0: write the opposite parity's block (transposed two-table
-  // contraction); bit 1: add to the block instead of overwriting it
-  const int pout = (out_mode & 1) ? par ^ 1 : par;
+  double* O = reinterpret_cast<double*>(smem_raw);
+  __syncthreads();  // the last chunk's operands are consumed
   if (fl < nf) {
-    double* base = reinterpret_cast<double*>(eo) +
-                   ((((size_t)(f0 + fl) * nm + m) * 2 + pout) * half + lat0) * 2 + comp;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       if (t >= nlt) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int j = 16 * t + tile_row<PREC>(lk, r);
-        if (j < nlg) {
-          double v = acc.v[t][r];
-          if (out_mode & 2) v += base[(size_t)j * 2];
-          base[(size_t)j * 2] = v;
-        }
+        const int j = 16 * t + tile_row<PREC>(lk, r);
+        O[(fl * TROWS + j) * 2 + comp] = acc.v[t][r];
       }
+    }
+  }
+  __syncthreads();
+  // out_mode bit 0: write the opposite parity's block (transposed two-table
+  // contraction); bit 1: add to the block instead of overwriting it
+  const int pout = (out_mode & 1) ? par ^ 1 : par;
+  const double2* O2 = reinterpret_cast<const double2*>(O);
+  for (int idx = threadIdx.x; idx < nf * TROWS; idx += LTHREADS) {
+    const int j = idx % TROWS, f = idx / TROWS;
+    if (j < nlg) {
+      double2* o = eo + (((size_t)(f0 + f) * nm + m) * 2 + pout) * half + lat0 + j;
+      double2 v = O2[idx];
+      if (out_mode & 2) v = cadd(*o, v);
+      store_stream(o, v);
     }
   }
 }
@@ -1341,10 +1360,12 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
 // -1.3 us at 20, +0.9 us at 61).
 // Largest latitude-pair count whose staging buffer, with the kernels' static
 // LDS (FPW + 1 rows of L complex values), fits in 64 KB.
-// nλ = 64, 128, 256 with 2 (nm - 1) < nλ run the four-step kernels
+// nλ = 128 and 256 with 2 (nm - 1) < nλ run the four-step kernels (at 64
+// the radix-4 kernels measured faster: 4 lanes per transform leave too
+// little work per wavefront)
 static bool use_four_step(const jgcm_mesh* M) {
   const int L = M->nlon;
-  return (L == 64 || L == 128 || L == 256) && 2 * (M->nm - 1) < L;
+  return (L == 128 || L == 256) && 2 * (M->nm - 1) < L;
 }
 
 // four-step kernels: latitude pairs per workgroup G = 64 / P, 128 / P or
@@ -1358,6 +1379,8 @@ static int four_step_pairs(const jgcm_mesh* M, int ppw) {
   const int P = M->nlon / 16;
   const size_t per = four_step_lds_per_pair(M);
   int g = 64 / P;
+  // built-in choice: at least 8 pairs, so that the E/O rows are whole lines
+  if (ppw <= 0) ppw = 8;
   while (g < ppw && 2 * g * P <= 256 && (size_t)(2 * g) * per <= 64 * 1024) g *= 2;
   return g;
 }
@@ -1414,9 +1437,6 @@ static void launch_unfold_fs(const jgcm_mesh* M, const double2* eo, const SegMap
     case 808: { constexpr int PP = 8, GG = 8; CALL; } break;                   \
     case 816: { constexpr int PP = 8, GG = 16; CALL; } break;                  \
     case 832: { constexpr int PP = 8, GG = 32; CALL; } break;                  \
-    case 416: { constexpr int PP = 4, GG = 16; CALL; } break;                  \
-    case 432: { constexpr int PP = 4, GG = 32; CALL; } break;                  \
-    case 464: { constexpr int PP = 4, GG = 64; CALL; } break;                  \
     default: break;                                                            \
   }
 
@@ -1536,7 +1556,9 @@ static size_t legendre_lds(int precision, int tl) {
   const size_t trows = 16 * (size_t)tl;
   size_t e = precision == JGCM_PREC_FP32 ? sizeof(float) : 8;  // double, float2
   size_t pf = trows * KS1, pi = (size_t)KC * (trows + 1);
-  return align16(e * LCOLS * KS1) + e * (pf > pi ? pf : pi);
+  size_t operands = align16(e * LCOLS * KS1) + e * (pf > pi ? pf : pi);
+  size_t out_tile = (size_t)(LCOLS / 2) * trows * sizeof(double2);  // inverse epilogue
+  return operands > out_tile ? operands : out_tile;
 }
 
 static int tiles_of(int t) { return t > 0 ? t : DEF_TILES; }

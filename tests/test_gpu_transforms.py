@@ -13,7 +13,7 @@ from oracle import SpectralSphericalMesh
 pytestmark = pytest.mark.gpu
 
 RADIUS = 6371.0e3
-# 64, 128, 256 longitudes: the four-step register FFT; 32 and 512: the
+# 128 and 256 longitudes: the four-step register FFT; 32, 64 and 512: the
 # radix-4 LDS FFT; 96 and 192: the direct-DFT path (not powers of two)
 TRUNCS = [(21, 32), (42, 64), (85, 128),
           (31, 48), (63, 96), (10, 16)]
