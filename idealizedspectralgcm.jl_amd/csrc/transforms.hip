@@ -811,6 +811,7 @@ __device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS,
     dv4 c[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) c[t] = dv4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
     for (int ks = 0; ks < KS; ++ks) {
       double b = bval(ks, 0);
 #pragma unroll
@@ -825,6 +826,7 @@ __device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS,
     fv4 c[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) c[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
     for (int ks = 0; ks < KS; ++ks) {
       float b = bval(ks, 0);
 #pragma unroll
@@ -1097,29 +1099,37 @@ __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int 
   }
 }
 
-// Spectral coefficient (n, m) of global field b of an inverse-transform
-// segment map, after the segment's operator (SPEC_OP_*).  The arithmetic and
-// its order are those of the stand-alone operator kernels.
-__device__ __forceinline__ double2 staged_coefficient_at(const double2* h, const double2* h2,
-                                                         int op, const SpecOpTables& t, int n, int m,
-                                                         int nm, int ns);
+// Operator table values of one spectral row (m, n), loaded once per
+// staging lane and shared by every field of the chunk (they do not depend on
+// the field).  Only the tables of the operators in `op_mask` (bit 1 << op)
+// are read.
+struct OpRow {
+  double dlam, dthm, dthp, uvc, uvm, uvp;
+};
 
-__device__ __forceinline__ double2 staged_coefficient(const SegMap& in, const SpecOpTables& t,
-                                                      int b, size_t spec_d, int n, int m, int nm,
-                                                      int ns) {
-  const int k = seg_find(in, b);
-  const size_t off = (size_t)(b - in.start[k]) * spec_d;
-  const double2* h = reinterpret_cast<const double2*>(in.ptr[k] + off);
-  const double2* h2 = in.ptr2[k] ? reinterpret_cast<const double2*>(in.ptr2[k] + off) : nullptr;
-  return staged_coefficient_at(h, h2, (in.flag[k] >> 1) & 7, t, n, m, nm, ns);
+__device__ __forceinline__ OpRow op_row(const SpecOpTables& t, int op_mask, int m, int n, int ns) {
+  const int c = m * (ns + 1) + n;
+  OpRow r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (op_mask & (1 << SPEC_OP_GRAD_LAMBDA)) r.dlam = t.dlam[c];
+  if (op_mask & (1 << SPEC_OP_GRAD_THETA)) {
+    r.dthm = t.dthm[c];
+    r.dthp = t.dthp[c];
+  }
+  if (op_mask & ((1 << SPEC_OP_UCOS) | (1 << SPEC_OP_VCOS))) {
+    r.uvc = t.uvc[c];
+    r.uvm = t.uvm[c];
+    r.uvp = t.uvp[c];
+  }
+  return r;
 }
 
-// The same with the array of global field b already resolved: h (and h2,
-// the second operand of the u cos / v cos operators) point at its
-// coefficients, op is its segment's operator.
-__device__ __forceinline__ double2 staged_coefficient_at(const double2* h, const double2* h2,
-                                                         int op, const SpecOpTables& t, int n, int m,
-                                                         int nm, int ns) {
+// Spectral coefficient (n, m) after the operator `op` (SPEC_OP_*); h (and h2,
+// the second operand of the u cos / v cos operators) point at the field's
+// coefficients, r holds the row's table values.  The arithmetic and its order
+// are those of the stand-alone operator kernels.
+__device__ __forceinline__ double2 staged_coefficient_row(const double2* h, const double2* h2, int op,
+                                                          const OpRow& r, int n, int m, int nm,
+                                                          int ns) {
   const size_t i = (size_t)n * nm + m;
   if (op == SPEC_OP_NONE) return h[i];
   if (op == SPEC_OP_NEG_I) {
@@ -1130,56 +1140,61 @@ __device__ __forceinline__ double2 staged_coefficient_at(const double2* h, const
     double2 v = h[i];
     return make_double2(-v.x, -v.y);
   }
-  const int c = m * (ns + 1) + n;
   if (op == SPEC_OP_GRAD_LAMBDA) {
     double2 v = h[i];
-    double a = t.dlam[c];
-    return make_double2(a * (-v.y), a * v.x);
+    return make_double2(r.dlam * (-v.y), r.dlam * v.x);
   }
   if (op == SPEC_OP_GRAD_THETA) {
-    double2 r = make_double2(0.0, 0.0);
+    double2 q = make_double2(0.0, 0.0);
     if (n <= ns - 1) {
       double2 hp = h[i + nm];
-      r = make_double2(t.dthp[c] * hp.x, t.dthp[c] * hp.y);
+      q = make_double2(r.dthp * hp.x, r.dthp * hp.y);
     }
     if (n >= 1) {
       double2 hm = h[i - nm];
-      r = make_double2(r.x + t.dthm[c] * hm.x, r.y + t.dthm[c] * hm.y);
+      q = make_double2(q.x + r.dthm * hm.x, q.y + r.dthm * hm.y);
     }
-    return r;
+    return q;
   }
-  // u cos θ / v cos θ: ptr = vor, ptr2 = div
+  // u cos θ / v cos θ: h = vor, h2 = div
   const double2* vor = h;
   const double2* div = h2;
-  const double cc = t.uvc[c];
+  const double cc = r.uvc;
   if (op == SPEC_OP_UCOS) {
     double2 d = div[i];
     double2 u = make_double2(cc * (-d.y), cc * d.x);
     if (n >= 1) {
-      double a = t.uvm[c];
       double2 vm = vor[i - nm];
-      u = make_double2(u.x - a * vm.x, u.y - a * vm.y);
+      u = make_double2(u.x - r.uvm * vm.x, u.y - r.uvm * vm.y);
     }
     if (n <= ns - 1) {
-      double a = t.uvp[c];
       double2 vp = vor[i + nm];
-      u = make_double2(u.x - a * vp.x, u.y - a * vp.y);
+      u = make_double2(u.x - r.uvp * vp.x, u.y - r.uvp * vp.y);
     }
     return u;
   }
   double2 v = vor[i];
   double2 w = make_double2(cc * (-v.y), cc * v.x);
   if (n >= 1) {
-    double a = t.uvm[c];
     double2 dm = div[i - nm];
-    w = make_double2(w.x + a * dm.x, w.y + a * dm.y);
+    w = make_double2(w.x + r.uvm * dm.x, w.y + r.uvm * dm.y);
   }
   if (n <= ns - 1) {
-    double a = t.uvp[c];
     double2 dp = div[i + nm];
-    w = make_double2(w.x + a * dp.x, w.y + a * dp.y);
+    w = make_double2(w.x + r.uvp * dp.x, w.y + r.uvp * dp.y);
   }
   return w;
+}
+
+// The same with the field given by its global index b in a segment map.
+__device__ __forceinline__ double2 staged_coefficient_seg(const SegMap& in, const OpRow& r, int b,
+                                                          size_t spec_d, int n, int m, int nm,
+                                                          int ns) {
+  const int k = seg_find(in, b);
+  const size_t off = (size_t)(b - in.start[k]) * spec_d;
+  const double2* h = reinterpret_cast<const double2*>(in.ptr[k] + off);
+  const double2* h2 = in.ptr2[k] ? reinterpret_cast<const double2*>(in.ptr2[k] + off) : nullptr;
+  return staged_coefficient_row(h, h2, (in.flag[k] >> 1) & 7, r, n, m, nm, ns);
 }
 
 // Inverse: grid x = segment (m, par), y = chunk of 32 fields, z = group of
@@ -1233,6 +1248,8 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
   const double2* seg_h2 =
       in.ptr2[seg_a] ? reinterpret_cast<const double2*>(in.ptr2[seg_a] + seg_off) : nullptr;
   const int seg_op = (in.flag[seg_a] >> 1) & 7;
+  int chunk_ops = 0;  // operators of the segments the chunk touches (bit 1 << op)
+  for (int q = seg_a; q <= seg_b; ++q) chunk_ops |= 1 << ((in.flag[q] >> 1) & 7);
   for (int kk0 = 0; kk0 < nr; kk0 += KC) {
     const int kcr = min(KC, nr - kk0);
     const int KSc = (kcr + KSTEP - 1) / KSTEP;
@@ -1260,6 +1277,7 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
       } else if (seg_one) {
         // the chunk's fields lie in one array: pointer arithmetic per field,
         // one straight loop per operator (the operator is the workgroup's)
+        const OpRow orow = op_row(ops, 1 << seg_op, m, kin ? nrow : m, ns);
         auto stage_op = [&](auto opc) {
           constexpr int OP = decltype(opc)::value;
 #pragma unroll
@@ -1267,8 +1285,8 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
             const int fl = wave + NWV * i;
             const size_t o = (size_t)fl * (spec_d / 2);
             sv[i] = (fl < nf && kin)
-                        ? staged_coefficient_at(seg_h + o, seg_h2 ? seg_h2 + o : nullptr, OP, ops,
-                                                nrow, m, nm, ns)
+                        ? staged_coefficient_row(seg_h + o, seg_h2 ? seg_h2 + o : nullptr, OP,
+                                                 orow, nrow, m, nm, ns)
                         : make_double2(0.0, 0.0);
           }
         };
@@ -1282,10 +1300,13 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
           default: stage_op(std::integral_constant<int, SPEC_OP_NEG>{}); break;
         }
       } else {
+        // fields from several arrays: the tables of every operator in the
+        // chunk are read once per lane, the array is resolved per field
+        const OpRow orow = op_row(ops, chunk_ops, m, kin ? nrow : m, ns);
 #pragma unroll
         for (int i = 0; i < FPT; ++i) {
           const int fl = wave + NWV * i;
-          sv[i] = (fl < nf && kin) ? staged_coefficient(in, ops, f0 + fl, spec_d, nrow, m, nm, ns)
+          sv[i] = (fl < nf && kin) ? staged_coefficient_seg(in, orow, f0 + fl, spec_d, nrow, m, nm, ns)
                                    : make_double2(0.0, 0.0);
         }
       }

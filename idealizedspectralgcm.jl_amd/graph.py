@@ -11,8 +11,12 @@ from ._lib import call
 
 
 class Step_Graph:
-    def __init__(self, mesh, fn, nfields=None):
-        """Capture ``fn()`` (which must only enqueue library calls on `mesh`).
+    def __init__(self, mesh, fn, nfields=None, repeat=1):
+        """Capture ``fn()`` (which must only enqueue library calls on `mesh`),
+        ``repeat`` times in a row: one launch then advances the state by
+        ``repeat`` steps, and the launch overhead between graphs is paid once
+        per ``repeat`` steps.  ``fn`` must enqueue the same work each time
+        (a later leapfrog step, not the init step).
 
         The cyclic garbage collector is flushed and paused for the capture: a
         collection inside it could finalize an unrelated mesh or solver, whose
@@ -28,7 +32,8 @@ class Step_Graph:
             mesh.synchronize()
             call("jgcm_graph_begin", mesh.handle)
             try:
-                fn()
+                for _ in range(int(repeat)):
+                    fn()
             except Exception:
                 g = ctypes.c_void_p()
                 try:
@@ -43,6 +48,7 @@ class Step_Graph:
             if was_enabled:
                 gc.enable()
         self._g = g
+        self.repeat = int(repeat)
 
     def launch(self):
         call("jgcm_graph_launch", self._g)

@@ -206,16 +206,22 @@ def test_update_output_matches_oracle(jgcm):
 def test_overlapped_fused_and_captured_steps_equal_serial_steps(jgcm):
     """The physics pass as its own serial pass, on the side stream, fused into
     the column pass (the default), and captured (fused, and overlapped as a
-    parallel branch) with replays that have eager work in between: all give
-    the same results bit for bit."""
+    parallel branch) with replays that have eager work in between, and the
+    three steps captured as one graph of three steps: all give the same
+    results bit for bit."""
     runs = []
     modes = {"separate": dict(fused=False), "overlap": dict(overlap=True), "fused": {},
-             "graph_fused": {}, "graph_overlap": dict(overlap=True)}
+             "graph_fused": {}, "graph_overlap": dict(overlap=True), "graph_three_steps": {}}
     for mode, kw in modes.items():
         _, (dev, dvc, datmo, dsi, dd) = build(jgcm, 42, 64, 20, 600)
         jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS, **kw)
         jgcm.Update_Init_Step(dsi)
-        if mode.startswith("graph"):
+        if mode == "graph_three_steps":
+            g = jgcm.Step_Graph(dev, lambda: jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS),
+                                nfields=5 * 20 + 1, repeat=3)
+            g.launch()
+            g.close()
+        elif mode.startswith("graph"):
             g = jgcm.Step_Graph(dev, lambda: jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS,
                                                                     **kw),
                                 nfields=5 * 20 + 1)
