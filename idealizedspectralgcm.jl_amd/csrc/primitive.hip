@@ -279,10 +279,10 @@ __device__ __forceinline__ void geopotential_tile(const VC& v, const TileIdx& t,
 
 // One workgroup's contribution to an area-weighted global mean: sum over its
 // 64 columns of value * w[latitude], written by lane 0 of wave 0.
-__device__ inline void block_area_partial(const TileIdx& t, int nlon, const double* w, double v0,
-                                          double v1, double2* partial) {
+// wj: the column's Gaussian weight (read by the caller with its other inputs)
+__device__ inline void block_area_partial(const TileIdx& t, double wj, double v0, double v1,
+                                          double2* partial) {
   if (t.q != 0) return;
-  double wj = t.col ? w[t.c / nlon] : 0.0;
   double s0 = wave_sum(t.col ? v0 * wj : 0.0);
   double s1 = wave_sum(t.col ? v1 * wj : 0.0);
   if (t.lane == 0) partial[blockIdx.x] = d2(s0, s1);
@@ -313,6 +313,8 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
   // column pass derives them from ps itself)
   if (p_half) store_pressure<KPT>(v, t, P, pr, p_half, dp, lnp_half, p_full, lnp_full);
   if (t.col && t.q == 0) dps[t.c] = 0.0;
+  const double psp = (t.col && (mass || energy)) ? ps_p[t.c] : 0.0;
+  const double wj = (t.col && t.q == 0 && (mass || energy)) ? wts[t.c / nlon] : 0.0;
 
   double U[KPT], V[KPT], T[KPT], DU[KPT], DV[KPT], DT[KPT];
 #pragma unroll
@@ -368,7 +370,6 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
   }
   if (mass || energy) {
     double part = 0.0;
-    const double psp = t.col ? ps_p[t.c] : 0.0;
     if (energy) {
 #pragma unroll
       for (int i = 0; i < KPT; ++i) {
@@ -382,7 +383,7 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
     }
     double2 above, below;
     level_sums(red, t, d2(part, 0.0), above, below);
-    block_area_partial(t, nlon, wts, psp, part + below.x, partial);
+    block_area_partial(t, wj, psp, part + below.x, partial);
   }
 }
 
@@ -465,6 +466,9 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
     }
   }
   double LH[KPT + 1], LF[KPT], PF[KPT], DIV[KPT], DU[KPT], DV[KPT], DT[KPT], VOR[KPT];
+  // the T gradients of the horizontal advection are used last; they are
+  // requested here with the other inputs so that no load waits at the end
+  double GLT[KPT], GTT[KPT];
 #pragma unroll
   for (int i = 0; i <= KPT; ++i) {
     int k = min(k0 + i, nd);
@@ -474,6 +478,8 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
   for (int i = 0; i < KPT; ++i) {
     bool ok = tx.col && k0 + i < nd;
     size_t o = (size_t)(k0 + i) * P + c;
+    GLT[i] = (ok && dl_t) ? dl_t[o] : 0.0;
+    GTT[i] = (ok && dl_t) ? dt_t[o] : 0.0;
     LF[i] = ok ? (own_p ? pr.lnf[i] : lnp_full[o]) : 0.0;
     PF[i] = ok ? (own_p ? pr.pf[i] : p_full[o]) : 0.0;
     DIV[i] = ok ? div[o] : 0.0;
@@ -497,6 +503,8 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
       Vp[i] = ok ? pa.v_p[o] : 0.0;
       Tp[i] = ok ? pa.t_p[o] : 0.0;
     }
+    const double psp = (tx.col && (pa.mass || pa.energy)) ? pa.ps_p[c] : 0.0;
+    const double wj = (tx.col && tx.q == 0 && (pa.mass || pa.energy)) ? pa.wts[c / nlon] : 0.0;
     const HSParams& hp = pa.hp;
     const double day = 86400;
     const double t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
@@ -529,7 +537,6 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
     if (isbad) atomicAdd(pa.bad, 1);
     if (pa.mass || pa.energy) {
       double part = 0.0;
-      const double psp = tx.col ? pa.ps_p[c] : 0.0;
       if (pa.energy) {
 #pragma unroll
         for (int i = 0; i < KPT; ++i) {
@@ -543,7 +550,7 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
       }
       double2 ab, bl;
       level_sums(red, tx, d2(part, 0.0), ab, bl);
-      block_area_partial(tx, nlon, pa.wts, psp, part + bl.x, pa.partial);
+      block_area_partial(tx, wj, psp, part + bl.x, pa.partial);
     }
   }
 
@@ -636,7 +643,7 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
     du[o] = du_k + av * vk;
     dv[o] = dv_k - av * uk;
     double dt_k = DT[i] + rt;
-    if (dl_t) dt_k = dt_k - (uk * dl_t[o] + vk * dt_t[o]);  // Add_Horizontal_Advection! (SSM:716)
+    if (dl_t) dt_k = dt_k - (uk * GLT[i] + vk * GTT[i]);  // Add_Horizontal_Advection! (SSM:716)
     dtt[o] = dt_k;
     absvor[o] = av;
     w_full[o] = W[i];
@@ -680,6 +687,7 @@ __global__ __launch_bounds__(512) void hs_corrections_sums_kernel(
   const size_t P = (size_t)nlat * nlon;
   const TileIdx tx = tile_idx<KPT>(P);
   const double p = tx.col ? exp(lnps[tx.c]) : 0.0;
+  const double wj = (sums && tx.col && tx.q == 0) ? wts[tx.c / nlon] : 0.0;
   double pa = 0.0, pb = 0.0;
   if (energy) {
 #pragma unroll
@@ -697,7 +705,6 @@ __global__ __launch_bounds__(512) void hs_corrections_sums_kernel(
   level_sums(red, tx, d2(pa, pb), above, below);
   if (tx.col && tx.q == 0) ps_n[tx.c] = p;
   if (sums && tx.q == 0) {
-    double wj = tx.col ? wts[tx.c / nlon] : 0.0;
     double s0 = wave_sum(tx.col ? p * wj : 0.0);
     double s1 = wave_sum(tx.col ? (pa + below.x) * wj : 0.0);
     double s2 = wave_sum(tx.col ? p * (pb + below.y) * wj : 0.0);
@@ -836,18 +843,31 @@ __global__ __launch_bounds__(APPLY_THREADS) void hs_correct_advance_kernel(
   bool need = kind == 1 || kind == 2;
   if (kind == 3) need = (lo + spec_level - 1) / spec_level * spec_level < hi;  // holds a multiple
   if (kind == 4) need = lo == 0;
+  double* __restrict__ p = s.p[fld];
+  double* __restrict__ c = s.c[fld];
+  double* __restrict__ n = s.n[fld];
+  // the block's c and n values are requested before the correction scalars
+  // are reduced, so that both round trips overlap
+  constexpr int IT = JGCM_CADV_PER_THREAD;
+  double xc[IT], xn[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t i = lo + threadIdx.x + (size_t)it * APPLY_THREADS;
+    xc[it] = i < hi ? c[i] : 0.0;
+    xn[it] = i < hi ? n[i] : 0.0;
+  }
   double sc[6] = {0.0, 0.0, 0.0, 1.0, 0.0, 0.0};
   if (need)
     correction_scalars(part_p, part_n, nparts, nlon, grav, cp_air, mass, energy, red, sc);
   const double f = sc[3], corr = sc[5];
   const double logf = (kind == 4 && mass) ? log(f) : 0.0;
-  double* __restrict__ p = s.p[fld];
-  double* __restrict__ c = s.c[fld];
-  double* __restrict__ n = s.n[fld];
-  for (size_t i = lo + threadIdx.x; i < hi; i += APPLY_THREADS) {
-    const double x = c[i];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t i = lo + threadIdx.x + (size_t)it * APPLY_THREADS;
+    if (i >= hi) break;
+    const double x = xc[it];
     p[i] = x;
-    double v = n[i];
+    double v = xn[it];
     if (kind == 1 && energy) {
       v = v + corr;
       n[i] = v;
@@ -888,18 +908,20 @@ __global__ __launch_bounds__(512) void hs_pressure_heights_kernel(
   __shared__ double2 red[MAXNQ * TILE];
   const size_t P = (size_t)nlat * nlon;
   const TileIdx tx = tile_idx<KPT>(P);
+  // T is requested before the pressure stores (the stores may alias it for
+  // all the compiler knows, so it would not move the loads above them)
+  double T[KPT], gfull[KPT], ghalf[KPT + 1];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    bool ok = z_full && tx.col && tx.k0 + i < v.nd;
+    T[i] = ok ? t[(size_t)(tx.k0 + i) * P + tx.c] : 0.0;
+  }
+  const double gs = (z_full && tx.col) ? geopots[tx.c] : 0.0;
   PressTile<KPT> pr;
   pressure_tile<KPT>(v, tx.k0, tx.col ? ps[tx.c] : 1.0e5, pr);
   store_pressure<KPT>(v, tx, P, pr, p_half, dp, lnp_half, p_full, lnp_full);
   if (!z_full) return;  // uniform over the grid
-  double T[KPT], gfull[KPT], ghalf[KPT + 1];
-#pragma unroll
-  for (int i = 0; i < KPT; ++i) {
-    bool ok = tx.col && tx.k0 + i < v.nd;
-    T[i] = ok ? t[(size_t)(tx.k0 + i) * P + tx.c] : 0.0;
-  }
-  geopotential_tile<KPT>(v, tx, red, rdgas, T, pr.lnh, pr.lnf, tx.col ? geopots[tx.c] : 0.0, gfull,
-                         ghalf, grav);
+  geopotential_tile<KPT>(v, tx, red, rdgas, T, pr.lnh, pr.lnf, gs, gfull, ghalf, grav);
   if (!tx.col) return;
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
