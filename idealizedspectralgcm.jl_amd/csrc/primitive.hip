@@ -1197,37 +1197,30 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     if (k0 + i < nd) lds[(k0 + i) * TILE + tx.lane] = A[i];
   __syncthreads();
   if (tx.col && n >= m && n <= num_wavenumbers) {
-    // two loops, so that each reads through one address space (an LDS/global
-    // select would turn every read into a generic access)
-    if (stage_w) {
-      const double* Wn = Wl + (size_t)(n - n_lo) * nnd;
+    // the level qq of the operand is read once for the KPT rows of this
+    // thread (each row still sums over qq in increasing order); two loops, so
+    // that each reads through one address space (an LDS/global select would
+    // turn every read into a generic access)
+    const int kmax = nd - 1;
+    auto solve = [&](const double* Wn) {
+      double2 s[KPT];
 #pragma unroll
-      for (int i = 0; i < KPT; ++i) {
-        int k = k0 + i;
-        if (k >= nd) continue;
-        double2 s = d2(0.0, 0.0);
-        for (int qq = 0; qq < nd; ++qq) {
-          double w = Wn[k * nd + qq];
-          double2 x = lds[qq * TILE + tx.lane];
-          s = d2(s.x + w * x.x, s.y + w * x.y);
-        }
-        A[i] = s;
-      }
-    } else {
-      const double* Wn = wave + (size_t)n * nnd;
+      for (int i = 0; i < KPT; ++i) s[i] = d2(0.0, 0.0);
+#pragma unroll 4
+      for (int qq = 0; qq < nd; ++qq) {
+        const double2 x = lds[qq * TILE + tx.lane];
 #pragma unroll
-      for (int i = 0; i < KPT; ++i) {
-        int k = k0 + i;
-        if (k >= nd) continue;
-        double2 s = d2(0.0, 0.0);
-        for (int qq = 0; qq < nd; ++qq) {
-          double w = Wn[k * nd + qq];
-          double2 x = lds[qq * TILE + tx.lane];
-          s = d2(s.x + w * x.x, s.y + w * x.y);
+        for (int i = 0; i < KPT; ++i) {
+          const double w = Wn[min(k0 + i, kmax) * nd + qq];
+          s[i] = d2(s[i].x + w * x.x, s[i].y + w * x.y);
         }
-        A[i] = s;
       }
-    }
+#pragma unroll
+      for (int i = 0; i < KPT; ++i)
+        if (k0 + i < nd) A[i] = s[i];
+    };
+    if (stage_w) solve(Wl + (size_t)(n - n_lo) * nnd);
+    else solve(wave + (size_t)n * nnd);
   }
   __syncthreads();
   double2 Dfin[KPT];
