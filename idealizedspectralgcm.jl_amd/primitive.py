@@ -337,9 +337,9 @@ def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, phys
     registers.  ``fused=False`` runs it as its own pass first; with
     ``overlap`` as well, that pass goes to the mesh's side stream beside the
     first transforms (parallel branches of a captured step).  The results
-    agree to rounding.  On MI355X at T85 L20: fused 2xx us, separate serial
-    225 us, separate overlapped 233 us (the branch's fork and join cost more
-    than the 13 us pass)."""
+    agree to rounding.  On MI355X at T85 L20 (one-step graphs): fused 204 us,
+    separate serial 210 us, separate overlapped 218 us (the branch's fork and
+    join cost more than the pass)."""
     Δt = Get_Δt(semi_implicit.integrator)
     if fused and not overlap:
         Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
