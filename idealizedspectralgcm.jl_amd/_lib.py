@@ -119,6 +119,7 @@ SIGNATURES = {
     "jgcm_shallow_water_physics": [P, P, P, P, P, D, D, P, P, P, I],
     "jgcm_map": [P, I, SZ, P, D, D, P],
     "jgcm_vert_coord_create": [P, I, I, I, I, D, I, ctypes.POINTER(P)],
+    "jgcm_vert_coord_create_ex": [P, I, I, I, I, D, I, D, D, D, D, D, ctypes.POINTER(P)],
     "jgcm_vert_coord_destroy": [P],
     "jgcm_vert_coord_get": [P, P, P],
     "jgcm_pressure_variables": [P, P, P, P, P, P, P],

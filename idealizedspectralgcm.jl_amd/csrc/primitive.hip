@@ -1399,6 +1399,33 @@ static const double kSigmaV197[19] = {0.0,       0.0089163, 0.0342936, 0.0740741
                                       0.5829904, 0.6639231, 0.7407407, 0.8113854, 0.8737997,
                                       0.9259259, 0.9657064, 0.9910837, 1.0};
 
+// Compute_Uneven_Sigma (Vert_Coordinate.jl:120-141) as its docstring states
+// it: ζ_k = 1 - k/nd, b_k = exp(-(surf_res ζ + (1 - surf_res) ζ^exponent)
+// scale_heights), b_nd = 1, b_0 = 0 when zero_top; a = 0.  (The code
+// evaluates exp(+…), which gives b > 1; the docstring's form is the one that
+// yields a coordinate.)
+static void uneven_sigma(int nd, double scale_heights, double surf_res, double exponent,
+                         bool zero_top, std::vector<double>& b) {
+  b.resize(nd + 1);
+  for (int k = 0; k <= nd; ++k) {
+    const double zeta = (double)(nd - k) / nd;
+    const double z = -(surf_res * zeta + (1.0 - surf_res) * std::pow(zeta, exponent));
+    b[k] = std::exp(z * scale_heights);
+  }
+  b[nd] = 1.0;
+  if (zero_top) b[0] = 0.0;
+}
+
+// Transition (Vert_Coordinate.jl:93-110): 0 at or above p_press, 1 at or
+// below p_sigma, sin² in between, on the σ values p.
+static double transition(double p, double p_sigma, double p_press) {
+  if (p <= p_press) return 0.0;
+  if (p >= p_sigma) return 1.0;
+  const double x = p - p_press, xx = p_sigma - p_press;
+  const double s = std::sin(0.5 * M_PI * x / xx);
+  return s * s;
+}
+
 }  // namespace jgcm
 
 using namespace jgcm;
@@ -1408,12 +1435,23 @@ extern "C" {
 int jgcm_vert_coord_create(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
                            int vert_difference_option, int vert_advect_scheme, double p_ref,
                            int zero_top, jgcm_vert_coord_t** out) {
+  return jgcm_vert_coord_create_ex(mesh, nd, vert_coord_option, vert_difference_option,
+                                   vert_advect_scheme, p_ref, zero_top, 4.0, 1.0, 0.1, 0.3, 2.5,
+                                   out);
+}
+
+int jgcm_vert_coord_create_ex(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
+                              int vert_difference_option, int vert_advect_scheme, double p_ref,
+                              int zero_top, double scale_heights, double surf_res,
+                              double p_press, double p_sigma, double exponent,
+                              jgcm_vert_coord_t** out) {
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_REQUIRE(out != nullptr, "out is NULL");
   JGCM_REQUIRE(nd >= 1 && nd <= MAXND, "nd must be in 1..%d", MAXND);
   // Compute_Vert_Coord (Vert_Coordinate.jl:61-87)
   const double* table = nullptr;
   int table_nd = 0;
+  std::vector<double> ua, ub;  // uneven_sigma / hybrid: a and b at the half levels
   switch (vert_coord_option) {
     case JGCM_VC_EVEN_SIGMA:
       break;
@@ -1426,13 +1464,28 @@ int jgcm_vert_coord_create(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
       table_nd = 18;
       break;
     case JGCM_VC_UNEVEN_SIGMA:
-      set_error("Compute_Vert_Coord uneven_sigma cannot run in the reference: "
-                "Vert_Coordinate.jl:70 passes a and b before they are defined");
-      return JGCM_ERR_STATE;
-    case JGCM_VC_HYBRID:
-      set_error("Compute_Vert_Coord hybrid cannot run in the reference: "
-                "Vert_Coordinate.jl:72-73 call Compute_Uneven_Sigma without nd");
-      return JGCM_ERR_STATE;
+      // :69-70 (the reference passes a, b before they exist; called here with nd)
+      JGCM_REQUIRE(scale_heights > 0.0 && exponent > 0.0, "scale_heights and exponent must be > 0");
+      uneven_sigma(nd, scale_heights, surf_res, exponent, zero_top != 0, ub);
+      ua.assign(nd + 1, 0.0);
+      break;
+    case JGCM_VC_HYBRID: {
+      // :71-77 with nd supplied: (a_σ, b_σ) = uneven σ without a zero top,
+      // (b_p, a_p) = the same pair swapped (pressure levels), blended by
+      // Transition(b_σ) — a = p_ref b_σ (1 - trans), b = b_σ trans
+      JGCM_REQUIRE(scale_heights > 0.0 && exponent > 0.0, "scale_heights and exponent must be > 0");
+      JGCM_REQUIRE(p_sigma > p_press, "p_sigma must exceed p_press");
+      std::vector<double> bs;
+      uneven_sigma(nd, scale_heights, surf_res, exponent, false, bs);
+      ua.resize(nd + 1);
+      ub.resize(nd + 1);
+      for (int k = 0; k <= nd; ++k) {
+        const double tr = transition(bs[k], p_sigma, p_press);
+        ua[k] = p_ref * (bs[k] * (1.0 - tr));
+        ub[k] = bs[k] * tr;
+      }
+      break;
+    }
     default:
       set_error("Compute_Vert_Coord option %d is not a valid value for option", vert_coord_option);
       return JGCM_ERR_STATE;
@@ -1456,11 +1509,15 @@ int jgcm_vert_coord_create(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
   v->zero_top = zero_top ? 1 : 0;
   v->p_ref = p_ref;
   // Compute_Even_Sigma (Vert_Coordinate.jl:112-117): a = 0, b = LinRange(0, 1, nd+1);
-  // the tables: a = 0, b as listed
+  // the tables: a = 0, b as listed; uneven σ and hybrid as computed above
   v->ak.assign(nd + 1, 0.0);
   v->bk.resize(nd + 1);
   for (int k = 0; k <= nd; ++k) v->bk[k] = table ? table[k] : (double)k / nd;
   v->bk[nd] = 1.0;
+  if (!ub.empty()) {
+    v->ak = ua;
+    v->bk = ub;
+  }
   v->dak.resize(nd);
   v->dbk.resize(nd);
   for (int k = 0; k < nd; ++k) {

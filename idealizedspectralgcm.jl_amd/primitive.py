@@ -54,11 +54,16 @@ def _hs(params):
 
 class Vert_Coordinate:
     """``Vert_Coordinate(nλ, nθ, nd, vert_coord_option, vert_difference_option,
-    vert_advect_scheme, p_ref=101325., zero_top=True)`` (Vert_Coordinate.jl:42-56).
-    The mesh defaults to the last mesh created on this (nθ, nλ) grid."""
+    vert_advect_scheme, p_ref=101325., zero_top=True, scale_heights=4.0,
+    surf_res=1.0, p_press=0.1, p_sigma=0.3, exponent=2.5)``
+    (Vert_Coordinate.jl:42-56).  uneven_sigma and hybrid follow the
+    reference's docstring and call structure (the reference itself cannot
+    run them; include/jgcm.h states the formulas).  The mesh defaults to the
+    last mesh created on this (nθ, nλ) grid."""
 
     def __init__(self, nλ, nθ, nd, vert_coord_option, vert_difference_option,
-                 vert_advect_scheme, p_ref=101325.0, zero_top=True, mesh=None):
+                 vert_advect_scheme, p_ref=101325.0, zero_top=True, scale_heights=4.0,
+                 surf_res=1.0, p_press=0.1, p_sigma=0.3, exponent=2.5, mesh=None):
         from .mesh import _MESHES
         if mesh is None:
             for ref in reversed(_MESHES):
@@ -78,9 +83,10 @@ class Vert_Coordinate:
         self.vert_advect_scheme = vert_advect_scheme
         self.p_ref, self.zero_top = float(p_ref), bool(zero_top)
         h = ctypes.c_void_p()
-        call("jgcm_vert_coord_create", mesh.handle, nd, _VC[vert_coord_option],
+        call("jgcm_vert_coord_create_ex", mesh.handle, nd, _VC[vert_coord_option],
              _VD[vert_difference_option], _VA[vert_advect_scheme], self.p_ref,
-             1 if zero_top else 0, ctypes.byref(h))
+             1 if zero_top else 0, float(scale_heights), float(surf_res), float(p_press),
+             float(p_sigma), float(exponent), ctypes.byref(h))
         self._h = h
         ak, bk = np.zeros(nd + 1), np.zeros(nd + 1)
         call("jgcm_vert_coord_get", h, ak.ctypes.data_as(ctypes.c_void_p),

@@ -467,16 +467,30 @@ int jgcm_shallow_water_physics(jgcm_mesh_t* mesh, const double* grid_u, const do
 /* ---- primitive equations (Held-Suarez) ------------------------------------- */
 /* Vert_Coordinate(nλ, nθ, nd, vert_coord_option, vert_difference_option,
  *                 vert_advect_scheme, p_ref, zero_top)   Vert_Coordinate.jl:42-56
- * Pure sigma coordinates (ak = 0): even sigma (:112-117, any nd <= 64) and
- * the fixed tables mcm (Compute_Old_Model_Sigma, :157-165, nd = 14) and v197
- * (Compute_V197_Sigma, :143-154, nd = 18); for the tables nd must equal their
- * length (the reference indexes the table with nd and silently truncates a
- * smaller one).  uneven_sigma and hybrid fail as in the reference, where
- * they cannot run (:70 passes undefined a, b; :72-73 omit nd):
- * JGCM_ERR_STATE with a message naming the defect.                         */
+ * with the reference's defaults scale_heights = 4, surf_res = 1, p_press =
+ * 0.1, p_sigma = 0.3, exponent = 2.5 (jgcm_vert_coord_create_ex takes them).
+ * Pure sigma coordinates (ak = 0): even sigma (:112-117, any nd <= 64), the
+ * fixed tables mcm (Compute_Old_Model_Sigma, :157-165, nd = 14) and v197
+ * (Compute_V197_Sigma, :143-154, nd = 18; for the tables nd must equal their
+ * length — the reference indexes the table with nd and silently truncates a
+ * smaller one) and uneven sigma (Compute_Uneven_Sigma, :120-141).  hybrid
+ * (:71-77, Transition :93-110): pressure levels aloft, sigma levels below
+ * p_sigma.  The reference cannot run uneven_sigma and hybrid (:70 passes a,
+ * b before they exist, :72-73 omit nd, :74 calls Transition without nd, and
+ * :131-133 evaluate exp(+...) against the docstring's exp(-...)); they are
+ * built here as the docstring and the call structure state them:
+ *   uneven: ζ_k = 1 - k/nd, b_k = exp(-(surf_res ζ + (1-surf_res) ζ^exponent)
+ *           scale_heights), b_nd = 1, b_0 = 0 when zero_top, a = 0;
+ *   hybrid: b_u = uneven without a zero top, t_k = Transition(b_u,k),
+ *           a_k = p_ref b_u,k (1 - t_k), b_k = b_u,k t_k.                    */
 int jgcm_vert_coord_create(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
                            int vert_difference_option, int vert_advect_scheme, double p_ref,
                            int zero_top, jgcm_vert_coord_t** out);
+int jgcm_vert_coord_create_ex(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
+                              int vert_difference_option, int vert_advect_scheme, double p_ref,
+                              int zero_top, double scale_heights, double surf_res,
+                              double p_press, double p_sigma, double exponent,
+                              jgcm_vert_coord_t** out);
 int jgcm_vert_coord_destroy(jgcm_vert_coord_t* vc);
 /* host copies of ak, bk ([nd+1] each; either may be NULL) */
 int jgcm_vert_coord_get(jgcm_vert_coord_t* vc, double* ak, double* bk);

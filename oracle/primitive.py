@@ -50,20 +50,61 @@ SIGMA_TABLES = {
 }
 
 
+def uneven_sigma(nd, scale_heights, surf_res, exponent, zero_top):
+    """Compute_Uneven_Sigma (Vert_Coordinate.jl:120-141) as its docstring
+    states it (the code's exp(+...) gives b > 1): ζ = LinRange(1, 0, nd+1),
+    b = exp(-(surf_res ζ + (1 - surf_res) ζ^exponent) scale_heights),
+    b[nd] = 1, b[0] = 0 when zero_top; a = 0."""
+    zeta = np.array([(nd - k) / nd for k in range(nd + 1)])
+    z = -(surf_res * zeta + (1.0 - surf_res) * zeta ** exponent)
+    b = np.exp(z * scale_heights)
+    b[nd] = 1.0
+    if zero_top:
+        b[0] = 0.0
+    return np.zeros(nd + 1), b
+
+
+def transition(p, p_sigma, p_press):
+    """Transition (Vert_Coordinate.jl:93-110) on the σ values p."""
+    t = np.zeros(len(p))
+    for k, x in enumerate(p):
+        if x <= p_press:
+            t[k] = 0.0
+        elif x >= p_sigma:
+            t[k] = 1.0
+        else:
+            t[k] = np.sin(0.5 * np.pi * (x - p_press) / (p_sigma - p_press)) ** 2
+    return t
+
+
 class VertCoordinate:
-    """Vert_Coordinate.jl:42-56 with Compute_Vert_Coord (:61-87) for the pure
-    sigma options: even_sigma (:112-117) and the fixed tables, whose nd must
-    equal their length.  uneven_sigma and hybrid cannot run in the reference
-    (:70, :72-73) and raise here."""
+    """Vert_Coordinate.jl:42-56 with Compute_Vert_Coord (:61-87): even_sigma
+    (:112-117), the fixed tables (their nd must equal their length),
+    uneven_sigma (:69-70) and hybrid (:71-77).  The reference cannot run the
+    last two (:70 passes a, b before they exist, :72-74 omit nd); they follow
+    its docstring and call structure with nd supplied:
+    hybrid: (a_σ, b_σ) = uneven σ without a zero top, (b_p, a_p) the same pair
+    swapped, trans = Transition(b_σ), a = p_ref (a_σ trans + a_p (1 - trans)),
+    b = b_σ trans + b_p (1 - trans)."""
 
     def __init__(self, nlon, nlat, nd, vert_coord_option, vert_difference_option,
-                 vert_advect_scheme, p_ref=101325.0, zero_top=True):
+                 vert_advect_scheme, p_ref=101325.0, zero_top=True, scale_heights=4.0,
+                 surf_res=1.0, p_press=0.1, p_sigma=0.3, exponent=2.5):
+        ak = np.zeros(nd + 1)
         if vert_coord_option == "even_sigma":
             bk = np.linspace(0.0, 1.0, nd + 1)
         elif vert_coord_option in SIGMA_TABLES:
             bk = np.array(SIGMA_TABLES[vert_coord_option])
             if nd != len(bk) - 1:
                 raise ValueError(f"{vert_coord_option} has {len(bk) - 1} levels, got nd = {nd}")
+        elif vert_coord_option == "uneven_sigma":
+            ak, bk = uneven_sigma(nd, scale_heights, surf_res, exponent, zero_top)
+        elif vert_coord_option == "hybrid":
+            a_sigma, b_sigma = uneven_sigma(nd, scale_heights, surf_res, exponent, False)
+            b_press, a_press = uneven_sigma(nd, scale_heights, surf_res, exponent, False)
+            trans = transition(b_sigma, p_sigma, p_press)
+            ak = p_ref * (a_sigma * trans + a_press * (1.0 - trans))
+            bk = b_sigma * trans + b_press * (1.0 - trans)
         else:
             raise ValueError(f"Compute_Vert_Coord {vert_coord_option} is not supported")
         self.nlon, self.nlat, self.nd = nlon, nlat, nd
@@ -71,7 +112,7 @@ class VertCoordinate:
         self.vert_difference_option = vert_difference_option
         self.vert_advect_scheme = vert_advect_scheme
         self.p_ref, self.zero_top = p_ref, zero_top
-        self.ak = np.zeros(nd + 1)
+        self.ak = ak
         self.bk = bk
         self.Δak = self.ak[1:] - self.ak[:-1]
         self.Δbk = self.bk[1:] - self.bk[:-1]

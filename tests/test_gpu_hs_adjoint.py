@@ -130,7 +130,7 @@ def test_gradient_cos_adjoint_dot_product(jgcm):
     (1, False, "even_sigma", 8, "second_centered_wts"),
     (3, True, "even_sigma", 8, "second_centered_wts"),
     (2, True, "mcm", 14, "second_centered_wts"), (2, True, "v197", 18, "second_centered_wts"),
-    (2, False, "mcm", 14, "second_centered")])
+    (2, False, "mcm", 14, "second_centered"), (2, True, "uneven_sigma", 12, "second_centered_wts")])
 def test_hs_gradient_matches_central_differences(jgcm, K, starts_init, coord, nd, scheme):
     mesh, vc, atmo, make_solver, d = model(jgcm, nd=nd, spinup=3, coord=coord, scheme=scheme)
     state = state_of(jgcm, d)
@@ -176,6 +176,19 @@ def test_hs_gradient_matches_central_differences(jgcm, K, starts_init, coord, nd
         else:
             assert abs(err[1]) <= 0.05 * abs(err[0]) + 1e-8 * tol, (seed, an, err)
         assert abs(err[2]) <= 1e-6 * tol, (seed, an, err)
+
+
+def test_hs_adjoint_rejects_hybrid_coordinate(jgcm):
+    """The step adjoint is built for pure sigma (a_k = 0); on the hybrid
+    coordinate the forward run works and the reverse sweep is an argument
+    error naming the requirement."""
+    mesh, vc, atmo, make_solver, d = model(jgcm, nd=12, spinup=2, coord="hybrid")
+    state = state_of(jgcm, d)
+    adj = jgcm.Held_Suarez_Adjoint(mesh, atmo, vc, make_solver(False), HS, checkpoint_every=2)
+    out = adj.forward(state, 2)
+    assert all(bool(torch.isfinite(t).all()) for t in out)
+    with pytest.raises(AssertionError, match="pure sigma"):
+        adj.backward([torch.ones_like(t) for t in out])
 
 
 def mass_weighted_mse(jgcm, mesh, vc, t_spec, lnps_spec, target, want_grad=False):

@@ -68,7 +68,8 @@ def compare(d, dd, tol, fields=FIELDS):
         assert e < (max(tol, DIV_TOL) if f == "spe_div_c" else tol), (f, errs)
 
 
-@pytest.mark.parametrize("coord,nd", [("even_sigma", 10), ("mcm", 14), ("v197", 18)])
+@pytest.mark.parametrize("coord,nd", [("even_sigma", 10), ("mcm", 14), ("v197", 18),
+                                      ("uneven_sigma", 20), ("hybrid", 20)])
 def test_semi_implicit_setup_matches_oracle(jgcm, coord, nd):
     ref, dev = build(jgcm, 21, 32, nd, 600, coord=coord)
     si, dsi = ref[3], dev[3]
@@ -91,7 +92,8 @@ def test_initial_state_matches_oracle(jgcm):
 @pytest.mark.parametrize("N,nlat,nd,dt,steps,coord", [
     (21, 32, 10, 600, 4, "even_sigma"), (42, 64, 20, 600, 2, "even_sigma"),
     (85, 128, 20, 300, 3, "even_sigma"),
-    (31, 48, 10, 600, 2, "even_sigma"), (21, 32, 14, 600, 3, "mcm"), (21, 32, 18, 600, 3, "v197")])
+    (31, 48, 10, 600, 2, "even_sigma"), (21, 32, 14, 600, 3, "mcm"), (21, 32, 18, 600, 3, "v197"),
+    (21, 32, 20, 600, 3, "uneven_sigma"), (21, 32, 20, 600, 3, "hybrid")])
 def test_hs_steps_match_oracle(jgcm, N, nlat, nd, dt, steps, coord):
     (M, vc, atmo, si, d), (dev, dvc, datmo, dsi, dd) = build(jgcm, N, nlat, nd, dt, coord=coord)
     for i in range(steps):
@@ -160,7 +162,9 @@ def test_vertical_coordinate_options(jgcm):
     """Compute_Vert_Coord (Vert_Coordinate.jl:61-87) through the C ABI: the
     mcm and v197 tables equal the fixture read from the reference's text; a
     table with the wrong level count is an argument error; uneven_sigma and
-    hybrid, which cannot run in the reference, fail with a state error."""
+    hybrid (which the reference cannot run; built from its docstring and
+    call structure) equal the oracle's, for the default and other
+    parameters."""
     import json
     import os
     with open(os.path.join(os.path.dirname(__file__), "golden", "sigma_tables.json")) as f:
@@ -174,9 +178,14 @@ def test_vertical_coordinate_options(jgcm):
             jgcm.Vert_Coordinate(64, 32, t["nd"] + 1, name, "simmons_and_burridge",
                                  "second_centered_wts", 1e5, mesh=mesh)
     for name in ("uneven_sigma", "hybrid"):
-        with pytest.raises(jgcm.JGCMError, match="cannot run in the reference"):
-            jgcm.Vert_Coordinate(64, 32, 10, name, "simmons_and_burridge", "second_centered_wts",
-                                 1e5, mesh=mesh)
+        for kw in ({}, dict(scale_heights=5.0, surf_res=0.3, p_press=0.05, p_sigma=0.4,
+                            exponent=2.0), dict(zero_top=False)):
+            vc = jgcm.Vert_Coordinate(64, 32, 16, name, "simmons_and_burridge",
+                                      "second_centered_wts", 1e5, mesh=mesh, **kw)
+            ref = P.VertCoordinate(64, 32, 16, name, "simmons_and_burridge",
+                                   "second_centered_wts", 1e5, **kw)
+            np.testing.assert_allclose(vc.ak, ref.ak, rtol=1e-15, atol=1e-12)
+            np.testing.assert_allclose(vc.bk, ref.bk, rtol=1e-15, atol=1e-17)
 
 
 def test_update_output_matches_oracle(jgcm):

@@ -42,7 +42,7 @@ from oracle import dynamics as odyn
 from oracle import primitive as P
 
 R = 287.04
-COORDS = [("even_sigma", 10), ("mcm", 14), ("v197", 18)]
+COORDS = [("even_sigma", 10), ("mcm", 14), ("v197", 18), ("uneven_sigma", 20)]
 
 
 # ------------------------------------------------------------ vertical advection

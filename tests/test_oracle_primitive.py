@@ -20,8 +20,8 @@ from oracle import primitive as P
 
 R, KAPPA = 287.04, 2.0 / 7.0
 HS = {"σ_b": 0.7, "k_f": 1.0, "k_a": 1.0 / 40.0, "k_s": 1.0 / 4.0, "ΔT_y": 60.0, "Δθ_z": 10.0}
-# the pure sigma coordinates with their level counts (even sigma: any nd)
-COORDS = [("even_sigma", 10), ("mcm", 14), ("v197", 18)]
+# the coordinates with their level counts (even sigma, uneven sigma and hybrid: any nd)
+COORDS = [("even_sigma", 10), ("mcm", 14), ("v197", 18), ("uneven_sigma", 20), ("hybrid", 20)]
 SIGMA_TABLES = os.path.join(os.path.dirname(__file__), "golden", "sigma_tables.json")
 
 
@@ -50,9 +50,8 @@ def test_pressure_variables_closed_forms():
 
 def test_sigma_tables_match_the_reference_text():
     """The mcm and v197 tables equal the fixture read from the reference's
-    source text; v197 is symmetric about σ = 1/2 to its 7 printed digits; the
-    coordinates that cannot run in the reference raise, as do table
-    coordinates with the wrong level count."""
+    source text; v197 is symmetric about σ = 1/2 to its 7 printed digits;
+    table coordinates with the wrong level count and unknown options raise."""
     with open(SIGMA_TABLES) as f:
         tables = json.load(f)
     for name, t in tables.items():
@@ -63,9 +62,57 @@ def test_sigma_tables_match_the_reference_text():
             P.VertCoordinate(1, 1, t["nd"] - 1, name, "simmons_and_burridge", "second_centered_wts")
     b = np.array(tables["v197"]["b"])
     np.testing.assert_allclose(b + b[::-1], 1.0, atol=1.5e-7)
-    for name in ("uneven_sigma", "hybrid"):
-        with pytest.raises(ValueError):
-            P.VertCoordinate(1, 1, 10, name, "simmons_and_burridge", "second_centered_wts")
+    with pytest.raises(ValueError):
+        P.VertCoordinate(1, 1, 10, "sigma_and_more", "simmons_and_burridge", "second_centered_wts")
+
+
+@pytest.mark.parametrize("surf_res,exponent", [(1.0, 2.5), (0.2, 2.5), (0.5, 3.0)])
+def test_uneven_sigma_closed_form(surf_res, exponent):
+    """Compute_Uneven_Sigma (Vert_Coordinate.jl:120-141, the docstring's form):
+    with surf_res = 1 the levels are even in ln σ, ln b_k = -H (1 - k/nd);
+    otherwise ln b_k = -H (s ζ + (1-s) ζ^e) at ζ = 1 - k/nd, checked at a few
+    levels by hand; b rises monotonically from 0 (zero top) to 1; a = 0."""
+    nd, H = 20, 4.0
+    vc = P.VertCoordinate(1, 1, nd, "uneven_sigma", "simmons_and_burridge", "second_centered_wts",
+                          surf_res=surf_res, exponent=exponent)
+    b = vc.bk
+    assert b[0] == 0.0 and b[nd] == 1.0 and not vc.ak.any()
+    assert np.all(np.diff(b) > 0)
+    for k in (1, 5, 10, 19):
+        zeta = 1.0 - k / nd
+        expect = -H * (surf_res * zeta + (1.0 - surf_res) * zeta ** exponent)
+        assert abs(np.log(b[k]) - expect) < 1e-13, k
+    if surf_res == 1.0:
+        np.testing.assert_allclose(np.diff(np.log(b[1:])), H / nd, rtol=1e-12)
+    top = P.VertCoordinate(1, 1, nd, "uneven_sigma", "simmons_and_burridge", "second_centered_wts",
+                           zero_top=False)
+    assert abs(top.bk[0] - np.exp(-H)) < 1e-16
+
+
+def test_hybrid_coordinate_blends_pressure_and_sigma_levels():
+    """hybrid (Vert_Coordinate.jl:71-77 with Transition :93-110): at the
+    reference surface pressure every half level sits where the uneven σ level
+    does (a_k + b_k p_ref = p_ref b_u,k); above σ = p_press the levels are pure
+    pressure (b = 0), below σ = p_sigma pure σ (a = 0), and between, the σ part
+    is sin² of the distance across the band."""
+    nd, p_ref = 20, 101325.0
+    vc = P.VertCoordinate(1, 1, nd, "hybrid", "simmons_and_burridge", "second_centered_wts",
+                          p_ref=p_ref)
+    _, bu = P.uneven_sigma(nd, 4.0, 1.0, 2.5, False)
+    np.testing.assert_allclose(vc.ak + vc.bk * p_ref, p_ref * bu, rtol=1e-15)
+    top, low = bu <= 0.1, bu >= 0.3
+    assert top.any() and low.any() and (~top & ~low).any()
+    assert not vc.bk[top].any() and not vc.ak[low].any()
+    np.testing.assert_allclose(vc.ak[top], p_ref * bu[top], rtol=1e-15)
+    mid = ~top & ~low
+    np.testing.assert_allclose(vc.bk[mid] / bu[mid],
+                               np.sin(0.5 * np.pi * (bu[mid] - 0.1) / 0.2) ** 2, rtol=1e-14)
+    assert vc.bk[-1] == 1.0 and vc.ak[-1] == 0.0
+    # half-level pressures increase downward over the range of surface
+    # pressures a Held-Suarez run visits (at ps below ≈0.57 p_ref the blend
+    # would fold levels over)
+    for ps in (6.0e4, 1.0e5, 1.1e5):
+        assert np.all(np.diff(vc.ak + vc.bk * ps) > 0)
 
 
 @pytest.mark.parametrize("coord,nd", [("even_sigma", 12), ("mcm", 14), ("v197", 18)])
