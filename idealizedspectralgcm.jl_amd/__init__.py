@@ -18,7 +18,7 @@ from .time_integrator import (Filtered_Leapfrog, Update_Init_Step, Get_Δt, Get_
                               Damped_Leapfrog_Batch)
 from .dyn_data import Atmo_Data, Compute_Abs_Vor, Dyn_Data, Time_Advance
 from .dynamics import (Barotropic_Dynamics, Shallow_Water_Physics, Implicit_Correction,
-                       Shallow_Water_Dynamics)
+                       Shallow_Water_Dynamics, Shallow_Water_Step)
 from .graph import Step_Graph
 from .initial_conditions import Galewsky_Jet, Rossby_Haurwitz_Wave4
 from .adjoint import Barotropic_Adjoint, Held_Suarez_Adjoint, Shallow_Water_Adjoint
@@ -39,6 +39,7 @@ __all__ = [
     "Filtered_Leapfrog", "Update_Init_Step", "Get_Δt", "Get_ξ", "Compute_Spectral_Damping",
     "Filtered_Leapfrog_step", "Damped_Leapfrog", "Damped_Leapfrog_Batch", "Atmo_Data", "Compute_Abs_Vor", "Dyn_Data",
     "Time_Advance", "Barotropic_Dynamics", "Shallow_Water_Physics", "Implicit_Correction",
+    "Shallow_Water_Step",
     "Shallow_Water_Dynamics", "Step_Graph", "Barotropic_Adjoint",
     "Shallow_Water_Adjoint", "Held_Suarez_Adjoint", "Galewsky_Jet", "Rossby_Haurwitz_Wave4",
 ]

@@ -121,6 +121,18 @@ __global__ void sw_implicit_kernel(const double2* __restrict__ div_c, const doub
 // flux  δu (op)= absvor v,  δv (op)= -absvor u  (Barotropic_Dynamics.jl:28-29,
 // Shallow_Water_Dynamics.jl:129-130, Spectral_Dynamics.jl:300-301).
 // accumulate = 0: assign, 1: add.
+// The explicit vorticity-flux residual and the kinetic energy with their
+// fused multiply-adds written out, shared by the stand-alone kernels and the
+// fused shallow-water kernel so that both round alike.
+__device__ __forceinline__ void vorticity_flux_acc(double a, double ui, double vi, double& du,
+                                                   double& dv) {
+  du = __fma_rn(a, vi, du);    // δu += absvor v
+  dv = __fma_rn(-a, ui, dv);   // δv += -absvor u
+}
+__device__ __forceinline__ double kinetic(double ui, double vi) {
+  return 0.5 * __fma_rn(ui, ui, vi * vi);
+}
+
 __global__ void vorticity_flux_kernel(const double* __restrict__ vor, const double* __restrict__ u,
                                       const double* __restrict__ v, const double* __restrict__ sint,
                                       double two_omega, double* __restrict__ absvor,
@@ -132,14 +144,14 @@ __global__ void vorticity_flux_kernel(const double* __restrict__ vor, const doub
     double a = vor[i] + f;
     if (absvor) absvor[i] = a;
     if (!du) continue;
-    double fu = a * v[i];
-    double fv = -a * u[i];
     if (accumulate) {
-      du[i] = du[i] + fu;
-      dv[i] = dv[i] + fv;
+      double x = du[i], y = dv[i];
+      vorticity_flux_acc(a, u[i], v[i], x, y);
+      du[i] = x;
+      dv[i] = y;
     } else {
-      du[i] = fu;
-      dv[i] = fv;
+      du[i] = a * v[i];
+      dv[i] = -a * u[i];
     }
   }
 }
@@ -172,8 +184,34 @@ __global__ void kinetic_energy_kernel(const double* __restrict__ u, const double
                                       const double* __restrict__ base, double* __restrict__ out,
                                       size_t n) {
   GRID_STRIDE(i, n) {
-    double k = 0.5 * (u[i] * u[i] + v[i] * v[i]);
+    double k = kinetic(u[i], v[i]);
     out[i] = base ? base[i] + k : k;
+  }
+}
+
+// Shallow_Water_Physics! (Shallow_Water_Dynamics.jl:7-21) and the first
+// grid steps of Shallow_Water_Dynamics! (:126-130 absolute vorticity and the
+// explicit residual, :139 kinetic energy) in one pass over the grid: the
+// arithmetic of sw_physics_kernel, vorticity_flux_kernel (accumulating) and
+// kinetic_energy_kernel.
+__global__ void sw_explicit_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                                   const double* __restrict__ h, const double* __restrict__ h_eq,
+                                   const double* __restrict__ vor, const double* __restrict__ sint,
+                                   double km, double kt, double two_omega,
+                                   double* __restrict__ du, double* __restrict__ dv,
+                                   double* __restrict__ dh, double* __restrict__ absvor,
+                                   double* __restrict__ kin, int nlat, int nlon, size_t n) {
+  GRID_STRIDE(i, n) {
+    const int j = (int)((i / nlon) % nlat);
+    const double ui = u[i], vi = v[i];
+    double x = -km * ui, y = -km * vi;
+    dh[i] = -kt * (h[i] - h_eq[i]);
+    const double a = vor[i] + two_omega * sint[j];
+    absvor[i] = a;
+    vorticity_flux_acc(a, ui, vi, x, y);
+    du[i] = x;
+    dv[i] = y;
+    kin[i] = kinetic(ui, vi);
   }
 }
 
@@ -413,6 +451,22 @@ int jgcm_kinetic_energy(jgcm_mesh_t* mesh, const double* grid_u, const double* g
   size_t n = (size_t)nlev * mesh->grid_elems();
   hipLaunchKernelGGL(kinetic_energy_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, grid_u, grid_v,
                      grid_base, grid_out, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_sw_explicit(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
+                     const double* grid_h, const double* h_eq, double kappa_m, double kappa_t,
+                     const double* grid_vor, double omega, double* grid_du, double* grid_dv,
+                     double* grid_dh, double* grid_absvor, double* grid_kin, int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(grid_u && grid_v && grid_h && h_eq && grid_vor && grid_du && grid_dv && grid_dh &&
+                   grid_absvor && grid_kin,
+               "NULL pointer");
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  hipLaunchKernelGGL(sw_explicit_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, grid_u, grid_v,
+                     grid_h, h_eq, grid_vor, mesh->d_sin, kappa_m, kappa_t, 2 * omega, grid_du,
+                     grid_dv, grid_dh, grid_absvor, grid_kin, mesh->nlat, mesh->nlon, n);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

@@ -107,6 +107,20 @@ def Time_Advance(dyn_data):
              [f"grid_{f}" for f in ("u", "v", "ps", "t")] + _tracer_names(dyn_data))
 
 
+# The prognostic arrays each stepper writes (Time_Advance! copies the others
+# too, but in these models they are never written: they stay zero, so their
+# copies change nothing and are skipped).
+SW_STATE = ["spe_vor", "spe_div", "spe_lnps", "grid_u", "grid_v", "grid_ps"]
+BAROTROPIC_STATE = ["spe_vor", "grid_u", "grid_v"]
+
+
+def Time_Advance_Model(dyn_data, names):
+    """Time_Advance! (Dyn_Data.jl:326-364) of the named prognostic arrays and
+    the tracers; for the barotropic and shallow-water steps, whose other
+    arrays are never written (no reference counterpart)."""
+    _advance(dyn_data, list(names) + _tracer_names(dyn_data))
+
+
 def Time_Advance_Tracers(dyn_data):
     """The tracer part of Time_Advance! alone (Dyn_Data.jl:341-344, 360-363);
     for steps whose other fields are advanced by a fused call.  No launch

@@ -11,10 +11,11 @@ synchronises the stream).
 import numpy as np
 
 from ._lib import call
-from .dyn_data import Time_Advance
-from .mesh import (_ptr, Vor_Div_From_Grid_UV, Trans_Spherical_To_Grid, Trans_Grid_To_Spherical,
-                   UV_Grid_From_Vor_Div, Apply_Laplacian, Add_Horizontal_Advection)
-from .time_integrator import Compute_Spectral_Damping, Filtered_Leapfrog_step
+from .dyn_data import BAROTROPIC_STATE, SW_STATE, Time_Advance_Model
+from .mesh import (_ptr, Vor_Div_From_Grid_UV, Apply_Laplacian, Add_Horizontal_Advection,
+                   Trans_Spherical_To_Grid_Batch, Trans_Grid_To_Spherical_Batch, BATCH_DIVCOS,
+                   BATCH_VORDIV, INV_UCOS, INV_VCOS)
+from .time_integrator import Damped_Leapfrog, Damped_Leapfrog_Batch
 
 
 def _show(*ts):
@@ -23,20 +24,25 @@ def _show(*ts):
 
 
 def Barotropic_Dynamics(mesh, atmo_data, dyn_data, integrator, verbose=False):
-    """Barotropic_Dynamics.jl:2-45."""
+    """Barotropic_Dynamics.jl:2-45.  Compute_Spectral_Damping! and
+    Filtered_Leapfrog! run as one kernel, and the three inverse transforms of
+    the new vorticity (ζ, u, v: :38-40) as one batched launch pair; the
+    results equal the separate calls bit for bit."""
     d = dyn_data
     n = mesh._nlev(d.grid_vor, "grid")
     # grid_absvor = vor + f;  δu = absvor v;  δv = -absvor u        (:26-29)
     call("jgcm_vorticity_flux", mesh.handle, _ptr(d.grid_vor), _ptr(d.grid_u_c), _ptr(d.grid_v_c),
          atmo_data.omega, _ptr(d.grid_absvor), _ptr(d.grid_δu), _ptr(d.grid_δv), n, 0)
     Vor_Div_From_Grid_UV(mesh, d.grid_δu, d.grid_δv, d.spe_δvor, d.spe_δdiv)
-    Compute_Spectral_Damping(integrator, d.spe_vor_c, d.spe_vor_p, d.spe_δvor)
-    Filtered_Leapfrog_step(integrator, d.spe_δvor, d.spe_vor_p, d.spe_vor_c, d.spe_vor_n)
-    Trans_Spherical_To_Grid(mesh, d.spe_vor_n, d.grid_vor)
-    UV_Grid_From_Vor_Div(mesh, d.spe_vor_n, d.spe_zeros, d.grid_u_n, d.grid_v_n)
+    Damped_Leapfrog(integrator, d.spe_δvor, d.spe_vor_p, d.spe_vor_c, d.spe_vor_n)  # :34-36
+    # :38-40 Trans_Spherical_To_Grid!(vor) and UV_Grid_From_Vor_Div!(vor, 0)
+    Trans_Spherical_To_Grid_Batch(
+        mesh, [d.spe_vor_n, d.spe_vor_n, d.spe_vor_n], [d.grid_vor, d.grid_u_n, d.grid_v_n],
+        [0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS],
+        snms_b=[None, d.spe_zeros, d.spe_zeros])
     if verbose:
         _show(d.grid_u_n, d.grid_v_n, d.grid_vor)
-    Time_Advance(d)
+    Time_Advance_Model(d, BAROTROPIC_STATE)  # :44 (the arrays this step writes)
 
 
 def Shallow_Water_Physics(dyn_data, kappa_m, kappa_t, h_eq):
@@ -66,42 +72,73 @@ def Implicit_Correction(*args):
 
 
 def Shallow_Water_Dynamics(mesh, atmo_data, h_0, dyn_data, integrator, verbose=False):
-    """Shallow_Water_Dynamics.jl:88-179."""
+    """Shallow_Water_Dynamics.jl:88-179.  The four forward transforms (δu, δv
+    -> δζ, δD at :133; KE at :140; δh at :149) run as one batched launch pair
+    once their grid fields exist (the height tendency is formed first, it
+    does not depend on δu, δv); the three dampings and leapfrogs (:159-165)
+    as one kernel; and the inverse transforms of the new state (ζ, D, h, and
+    u, v from ζ, D: :169-172) as one batched launch pair.  Each operation
+    keeps its arithmetic, so the results equal the call-by-call sequence bit
+    for bit."""
     d = dyn_data
     n = mesh._nlev(d.grid_vor, "grid")
-    spec = d.spe_energy.numel() * 2
-    grid = d.grid_u_c.numel()
     # abs vor and explicit residual δu += absvor v, δv += -absvor u    (:126-130)
     call("jgcm_vorticity_flux", mesh.handle, _ptr(d.grid_vor), _ptr(d.grid_u_c), _ptr(d.grid_v_c),
          atmo_data.omega, _ptr(d.grid_absvor), _ptr(d.grid_δu), _ptr(d.grid_δv), n, 1)
-    Vor_Div_From_Grid_UV(mesh, d.grid_δu, d.grid_δv, d.spe_δvor, d.spe_δdiv)
-    # kinetic energy, δdiv -= ∇²(KE + h)                                (:137-143)
-    grid_kin, spe_kin = d.grid_d_full1, d.spe_d1
+    # kinetic energy on the grid                                        (:139)
     call("jgcm_kinetic_energy", mesh.handle, _ptr(d.grid_u_c), _ptr(d.grid_v_c), None,
-         _ptr(grid_kin), n)
-    Trans_Grid_To_Spherical(mesh, grid_kin, spe_kin)
+         _ptr(d.grid_d_full1), n)
+    _sw_dynamics_rest(mesh, h_0, d, integrator, verbose)
+
+
+def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kappa_t, h_eq):
+    """Shallow_Water_Physics! then Shallow_Water_Dynamics! (the drivers' step,
+    Shallow_Water_Dynamics.jl:7-21 and :88-179) with the physics, the
+    absolute-vorticity residual and the kinetic energy in one grid pass
+    (jgcm_sw_explicit; no reference counterpart).  Equal to the two calls bit
+    for bit."""
+    d = dyn_data
+    n = mesh._nlev(d.grid_vor, "grid")
+    call("jgcm_sw_explicit", mesh.handle, _ptr(d.grid_u_c), _ptr(d.grid_v_c), _ptr(d.grid_ps_c),
+         _ptr(h_eq), float(kappa_m), float(kappa_t), _ptr(d.grid_vor), atmo_data.omega,
+         _ptr(d.grid_δu), _ptr(d.grid_δv), _ptr(d.grid_δlnps), _ptr(d.grid_absvor),
+         _ptr(d.grid_d_full1), n)
+    _sw_dynamics_rest(mesh, h_0, d, integrator, False)
+
+
+def _sw_dynamics_rest(mesh, h_0, d, integrator, verbose):
+    """Shallow_Water_Dynamics.jl:133-177 once the explicit residual and the
+    kinetic energy (in grid_d_full1) are on the grid."""
+    spec = d.spe_energy.numel() * 2
+    grid = d.grid_u_c.numel()
+    grid_kin, spe_kin = d.grid_d_full1, d.spe_d1
+    # height tendency on the grid                                       (:147-148)
+    Add_Horizontal_Advection(mesh, d.spe_lnps_n, d.grid_u_c, d.grid_v_c, d.grid_δlnps)
+    call("jgcm_xmy", mesh.handle, grid, -1.0, _ptr(d.grid_ps_c), _ptr(d.grid_div),
+         _ptr(d.grid_δlnps), 1)
+    # :133 Vor_Div_From_Grid_UV!, :140 and :149 Trans_Grid_To_Spherical!
+    Trans_Grid_To_Spherical_Batch(
+        mesh, [d.grid_δu, d.grid_δv, grid_kin, d.grid_δlnps],
+        [d.spe_δvor, d.spe_δdiv, spe_kin, d.spe_δlnps], [BATCH_VORDIV, 0, 0, 0])
+    # δdiv -= ∇²(KE + h)                                                (:141-143)
     call("jgcm_axpby", mesh.handle, spec, 1.0, _ptr(spe_kin), 1.0, _ptr(d.spe_lnps_c),
          _ptr(d.spe_energy))
     Apply_Laplacian(mesh, d.spe_energy)
     call("jgcm_axpby", mesh.handle, spec, 1.0, _ptr(d.spe_δdiv), -1.0, _ptr(d.spe_energy),
          _ptr(d.spe_δdiv))
-    # heights                                                           (:147-149)
-    Add_Horizontal_Advection(mesh, d.spe_lnps_n, d.grid_u_c, d.grid_v_c, d.grid_δlnps)
-    call("jgcm_xmy", mesh.handle, grid, -1.0, _ptr(d.grid_ps_c), _ptr(d.grid_div),
-         _ptr(d.grid_δlnps), 1)
-    Trans_Grid_To_Spherical(mesh, d.grid_δlnps, d.spe_δlnps)
     Implicit_Correction(integrator, d.spe_div_c, d.spe_div_p, d.spe_lnps_c, d.spe_lnps_p, h_0,
                         d.spe_δdiv, d.spe_δlnps)
-    Compute_Spectral_Damping(integrator, d.spe_vor_c, d.spe_vor_p, d.spe_δvor)
-    Compute_Spectral_Damping(integrator, d.spe_div_c, d.spe_div_p, d.spe_δdiv)
-    Compute_Spectral_Damping(integrator, d.spe_lnps_c, d.spe_lnps_p, d.spe_δlnps)
-    Filtered_Leapfrog_step(integrator, d.spe_δvor, d.spe_vor_p, d.spe_vor_c, d.spe_vor_n)
-    Filtered_Leapfrog_step(integrator, d.spe_δdiv, d.spe_div_p, d.spe_div_c, d.spe_div_n)
-    Filtered_Leapfrog_step(integrator, d.spe_δlnps, d.spe_lnps_p, d.spe_lnps_c, d.spe_lnps_n)
-    Trans_Spherical_To_Grid(mesh, d.spe_vor_n, d.grid_vor)
-    Trans_Spherical_To_Grid(mesh, d.spe_div_n, d.grid_div)
-    Trans_Spherical_To_Grid(mesh, d.spe_lnps_n, d.grid_ps_n)
-    UV_Grid_From_Vor_Div(mesh, d.spe_vor_n, d.spe_div_n, d.grid_u_n, d.grid_v_n)
+    # :159-165 Compute_Spectral_Damping! and Filtered_Leapfrog! of ζ, D, h
+    Damped_Leapfrog_Batch(integrator, [
+        (d.spe_δvor, d.spe_vor_p, d.spe_vor_c, d.spe_vor_n, True),
+        (d.spe_δdiv, d.spe_div_p, d.spe_div_c, d.spe_div_n, True),
+        (d.spe_δlnps, d.spe_lnps_p, d.spe_lnps_c, d.spe_lnps_n, True)])
+    # :169-172 the new state on the grid
+    Trans_Spherical_To_Grid_Batch(
+        mesh, [d.spe_vor_n, d.spe_div_n, d.spe_lnps_n, d.spe_vor_n, d.spe_vor_n],
+        [d.grid_vor, d.grid_div, d.grid_ps_n, d.grid_u_n, d.grid_v_n],
+        [0, 0, 0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS],
+        snms_b=[None, None, None, d.spe_div_n, d.spe_div_n])
     if verbose:
         _show(d.grid_u_n, d.grid_v_n, d.grid_ps_n)
-    Time_Advance(d)
+    Time_Advance_Model(d, SW_STATE)  # :177 (the arrays this step writes)

@@ -15,7 +15,7 @@ its zonal-mean wind and temperature; ``jet_maxima`` locates the jets.
 import torch
 
 from .dyn_data import Atmo_Data, Dyn_Data
-from .dynamics import Shallow_Water_Dynamics, Shallow_Water_Physics
+from .dynamics import Shallow_Water_Step
 from .graph import Step_Graph
 from .initial_conditions import Galewsky_Jet
 from .mesh import Spectral_Spherical_Mesh
@@ -43,8 +43,7 @@ class Galewsky_Experiment:
         self.graph = None
 
     def _step(self):
-        Shallow_Water_Physics(self.d, 0.0, 0.0, self.h_eq)
-        Shallow_Water_Dynamics(self.mesh, self.atmo, self.h_0, self.d, self.integ)
+        Shallow_Water_Step(self.mesh, self.atmo, self.h_0, self.d, self.integ, 0.0, 0.0, self.h_eq)
 
     def _prepare(self):
         """First (forward) step eagerly, then capture the later step once."""
@@ -53,7 +52,7 @@ class Galewsky_Experiment:
             Update_Init_Step(self.integ)
             self.steps = 1
         if self.graph is None:
-            self.graph = Step_Graph(self.mesh, self._step, nfields=4)
+            self.graph = Step_Graph(self.mesh, self._step, nfields=5)
 
     def snapshot(self):
         """(grid vorticity, divergence, geopotential, u) copies, after the stream drains."""

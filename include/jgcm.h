@@ -463,6 +463,16 @@ int jgcm_shallow_water_physics(jgcm_mesh_t* mesh, const double* grid_u, const do
                                const double* grid_h, const double* h_eq, double kappa_m,
                                double kappa_t, double* grid_du, double* grid_dv, double* grid_dh,
                                int nlev);
+/* No reference counterpart (a fusion of reference steps): Shallow_Water_Physics!
+ * (Shallow_Water_Dynamics.jl:7-21) followed by the first grid steps of
+ * Shallow_Water_Dynamics! (:126-130 absolute vorticity and δu += absvor v,
+ * δv -= absvor u; :139 KE = (u² + v²)/2) in one pass; results equal
+ * jgcm_shallow_water_physics + jgcm_vorticity_flux(accumulate) +
+ * jgcm_kinetic_energy bit for bit.                                          */
+int jgcm_sw_explicit(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
+                     const double* grid_h, const double* h_eq, double kappa_m, double kappa_t,
+                     const double* grid_vor, double omega, double* grid_du, double* grid_dv,
+                     double* grid_dh, double* grid_absvor, double* grid_kin, int nlev);
 
 /* ---- primitive equations (Held-Suarez) ------------------------------------- */
 /* Vert_Coordinate(nλ, nθ, nd, vert_coord_option, vert_difference_option,

@@ -130,6 +130,38 @@ def test_shallow_water_steps_match_oracle(jgcm, N, nlat):
         assert e < 1e-11, (f, e)
 
 
+def test_fused_shallow_water_step_equals_physics_and_dynamics(jgcm):
+    """Shallow_Water_Step (the physics, the absolute-vorticity residual and
+    the kinetic energy in one grid pass, then the batched transforms) equals
+    Shallow_Water_Physics! + Shallow_Water_Dynamics! bit for bit, over
+    several steps from the same state, with nonzero forcing."""
+    N, nlat = 42, 64
+    M, d, h_0, h_eq = sw_setup(N, nlat)
+    runs = []
+    for fused in (False, True):
+        dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 1, 6.371e6)
+        dd = jgcm.Dyn_Data("sw", N, N + 1, 2 * nlat, nlat, 1, mesh=dev)
+        to_dev(dd, d, STATE)
+        atmo = jgcm.Atmo_Data("sw", 2 * nlat, nlat, 1, False, False, False, False, dev.sinθ, A,
+                              OMEGA)
+        Id = jgcm.Filtered_Leapfrog(0.04, 4, 1e-4, dev.laplacian_eig, 0.5, 1200, True, 0, 691200)
+        heq_d = torch.from_numpy(h_eq).cuda()
+        km, kt = 1 / (20 * 86400.0), 1 / (10 * 86400.0)
+        for i in range(4):
+            if fused:
+                jgcm.Shallow_Water_Step(dev, atmo, h_0, dd, Id, km, kt, heq_d)
+            else:
+                jgcm.Shallow_Water_Physics(dd, km, kt, heq_d)
+                jgcm.Shallow_Water_Dynamics(dev, atmo, h_0, dd, Id)
+            if i == 0:
+                jgcm.Update_Init_Step(Id)
+        torch.cuda.synchronize()
+        runs.append({f: getattr(dd, f).clone() for f in STATE + ["grid_absvor", "spe_δdiv",
+                                                                  "spe_δlnps", "grid_δu"]})
+    for f, v in runs[0].items():
+        assert torch.equal(runs[1][f], v), f
+
+
 def test_graph_replay_matches_eager(jgcm):
     """A captured SW step replayed 3 times equals 3 eager steps bit for bit;
     eager work and synchronisation happen between the replays."""
