@@ -117,7 +117,10 @@ SIGNATURES = {
     "jgcm_xmy": [P, SZ, D, P, P, P, I],
     "jgcm_kinetic_energy": [P, P, P, P, P, I],
     "jgcm_shallow_water_physics": [P, P, P, P, P, D, D, P, P, P, I],
-    "jgcm_sw_explicit": [P, P, P, P, P, D, D, P, D, P, P, P, P, P, I],
+    "jgcm_sw_explicit": [P, P, P, P, P, D, D, P, D, P, P, P, P, P, P, P, P, I],
+    "jgcm_sw_height_tendency": [P, P, P, P, P, P, P, I],
+    "jgcm_sw_spectral_step": [P, D, P, P, P, P, P, ctypes.POINTER(P), ctypes.POINTER(P),
+                              ctypes.POINTER(P), I],
     "jgcm_map": [P, I, SZ, P, D, D, P],
     "jgcm_vert_coord_create": [P, I, I, I, I, D, I, ctypes.POINTER(P)],
     "jgcm_vert_coord_create_ex": [P, I, I, I, I, D, I, D, D, D, D, D, ctypes.POINTER(P)],

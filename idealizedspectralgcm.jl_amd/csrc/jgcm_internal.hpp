@@ -65,6 +65,12 @@ struct RowTables {
   ushort4* alpb_b3 = nullptr;
 };
 
+// Add_Horizontal_Advection!'s update δh - (u ∂λh + v ∂θh) (SSM:716) with its
+// multiply-add written out, so that every kernel using it rounds alike.
+__device__ __forceinline__ double advect(double dh, double u, double v, double gl, double gt) {
+  return dh - __fma_rn(u, gl, v * gt);
+}
+
 // BF16 (round to nearest even) from FP32 and back; finite values only.
 __host__ __device__ inline unsigned short f32_to_bf16(float x) {
   unsigned u;

@@ -168,7 +168,7 @@ __global__ void gradient_cos_kernel(const double2* __restrict__ h, double2* __re
 __global__ void advection_kernel(double* __restrict__ dh, const double* __restrict__ u,
                                  const double* __restrict__ v, const double* __restrict__ gl,
                                  const double* __restrict__ gt, size_t n) {
-  GRID_STRIDE(i, n) dh[i] = dh[i] - (u[i] * gl[i] + v[i] * gt[i]);
+  GRID_STRIDE(i, n) dh[i] = advect(dh[i], u[i], v[i], gl[i], gt[i]);
 }
 
 // SSM:746-760 (level 1): sum_i sum_j f[j][i] w_j / (2 nλ), one workgroup,

@@ -143,6 +143,22 @@ __device__ inline void four_in_one_column(const VC& v, double rdgas, double cp_a
 // Sums over levels are therefore grouped by wave rather than strictly
 // sequential; the FP64 results differ from a sequential sum only by rounding.
 constexpr int MAXNQ = 8;
+// Columns per workgroup of the grid-point column kernels (physics, column
+// dynamics, pressures, correction sums): one per lane of a wave.  (Half-wave
+// tiles with two levels per thread measured slower at T85 L20: 1024
+// workgroups of 148-156 registers per thread against 512 of 249.)
+#ifndef JGCM_GRID_COLS
+#define JGCM_GRID_COLS 64
+#endif
+constexpr int GCOLS = JGCM_GRID_COLS;
+
+// Sum over the GCOLS columns of a tile held by lanes 0 .. GCOLS-1 of a wave,
+// in a fixed order (result in lane 0).
+__device__ inline double tile_sum(double v) {
+#pragma unroll
+  for (int o = GCOLS / 2; o > 0; o >>= 1) v += __shfl_down(v, o, GCOLS);
+  return v;
+}
 
 __device__ __forceinline__ double2 d2(double x, double y) { return make_double2(x, y); }
 __device__ __forceinline__ double2 d2add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
@@ -153,16 +169,17 @@ __device__ __forceinline__ double2 d2keep(bool keep, double2 a) {
   return make_double2(keep ? a.x : 0.0, keep ? a.y : 0.0);
 }
 
-// Sums of a per-thread partial over the waves above (smaller level index) and
-// below this one, per lane.  `red` holds nq * 64 entries; every thread of the
-// workgroup must call it (two barriers, `red` is free again on return).
+// Sums of a per-thread partial over the level groups above (smaller level
+// index) and below this one, per column.  `red` holds blockDim.x entries;
+// every thread of the workgroup must call it (two barriers, `red` is free
+// again on return).
 __device__ inline void level_sums(double2* red, const TileIdx& t, double2 part, double2& above,
                                   double2& below) {
-  red[t.q * TILE + t.lane] = part;
+  red[t.q * t.cols + t.lane] = part;
   __syncthreads();
   double2 a = d2(0.0, 0.0), b = d2(0.0, 0.0);
   for (int i = 0; i < t.nq; ++i) {
-    double2 x = red[i * TILE + t.lane];
+    double2 x = red[i * t.cols + t.lane];
     if (i < t.q) a = d2add(a, x);
     if (i > t.q) b = d2add(b, x);
   }
@@ -278,13 +295,13 @@ __device__ __forceinline__ void geopotential_tile(const VC& v, const TileIdx& t,
 }
 
 // One workgroup's contribution to an area-weighted global mean: sum over its
-// 64 columns of value * w[latitude], written by lane 0 of wave 0.
+// GCOLS columns of value * w[latitude], written by thread 0.
 // wj: the column's Gaussian weight (read by the caller with its other inputs)
 __device__ inline void block_area_partial(const TileIdx& t, double wj, double v0, double v1,
                                           double2* partial) {
   if (t.q != 0) return;
-  double s0 = wave_sum(t.col ? v0 * wj : 0.0);
-  double s1 = wave_sum(t.col ? v1 * wj : 0.0);
+  double s0 = tile_sum(t.col ? v0 * wj : 0.0);
+  double s1 = tile_sum(t.col ? v1 * wj : 0.0);
   if (t.lane == 0) partial[blockIdx.x] = d2(s0, s1);
 }
 
@@ -303,7 +320,7 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
     double* dtt, double* dps, double2* partial, int forcing, int mass, int energy, int* bad) {
   __shared__ double2 red[MAXNQ * TILE];
   const size_t P = (size_t)nlat * nlon;
-  const TileIdx t = tile_idx<KPT>(P);
+  const TileIdx t = tile_idx<KPT, GCOLS>(P);
   const int nd = v.nd;
   const int j = t.col ? (int)(t.c / nlon) : 0;
   const double ps = t.col ? ps_c[t.c] : 1.0e5;
@@ -431,7 +448,7 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
                                                         JGCM_COLUMN_DYNAMICS_ARGS) {
   static_assert(OWN_P || !PHYS, "the fused physics pass evaluates the pressures itself");
   const size_t P = (size_t)nlat * nlon;
-  const TileIdx tx = tile_idx<KPT>(P);
+  const TileIdx tx = tile_idx<KPT, GCOLS>(P);
   const int nd = v.nd;
   const int k0 = tx.k0;
   const size_t c = tx.c;
@@ -685,7 +702,7 @@ __global__ __launch_bounds__(512) void hs_corrections_sums_kernel(
     int energy) {
   __shared__ double2 red[MAXNQ * TILE];
   const size_t P = (size_t)nlat * nlon;
-  const TileIdx tx = tile_idx<KPT>(P);
+  const TileIdx tx = tile_idx<KPT, GCOLS>(P);
   const double p = tx.col ? exp(lnps[tx.c]) : 0.0;
   const double wj = (sums && tx.col && tx.q == 0) ? wts[tx.c / nlon] : 0.0;
   double pa = 0.0, pb = 0.0;
@@ -705,9 +722,9 @@ __global__ __launch_bounds__(512) void hs_corrections_sums_kernel(
   level_sums(red, tx, d2(pa, pb), above, below);
   if (tx.col && tx.q == 0) ps_n[tx.c] = p;
   if (sums && tx.q == 0) {
-    double s0 = wave_sum(tx.col ? p * wj : 0.0);
-    double s1 = wave_sum(tx.col ? (pa + below.x) * wj : 0.0);
-    double s2 = wave_sum(tx.col ? p * (pb + below.y) * wj : 0.0);
+    double s0 = tile_sum(tx.col ? p * wj : 0.0);
+    double s1 = tile_sum(tx.col ? (pa + below.x) * wj : 0.0);
+    double s2 = tile_sum(tx.col ? p * (pb + below.y) * wj : 0.0);
     if (tx.lane == 0) {
       partial[3 * (size_t)blockIdx.x] = s0;
       partial[3 * (size_t)blockIdx.x + 1] = s1;
@@ -907,7 +924,7 @@ __global__ __launch_bounds__(512) void hs_pressure_heights_kernel(
     double* lnp_full, double* z_full, double* z_half) {
   __shared__ double2 red[MAXNQ * TILE];
   const size_t P = (size_t)nlat * nlon;
-  const TileIdx tx = tile_idx<KPT>(P);
+  const TileIdx tx = tile_idx<KPT, GCOLS>(P);
   // T is requested before the pressure stores (the stores may alias it for
   // all the compiler knows, so it would not move the loads above them)
   double T[KPT], gfull[KPT], ghalf[KPT + 1];
@@ -1077,26 +1094,26 @@ struct LFFuse {
   double dt, r;
 };
 
-template <int KPT>
+template <int KPT, int COLS>
 __global__ __launch_bounds__(512) void hs_implicit_kernel(
     int nd, int nm, int ns, size_t spec_per_level, double kappa, double rdgas, double ps_ref,
     double xi, SIRef rg, const double* __restrict__ eig, const double* __restrict__ wave,
     int num_wavenumbers, const double2* div_c, const double2* div_p, const double2* lnps_c,
     const double2* lnps_p, const double2* t_c, const double2* t_p, double2* ddiv, double2* dlnps,
     double2* dtt, double2* energy, int stage_w, LFFuse lf) {
-  // LDS: nd * 64 double2 (level sums, then the wave-matrix operand); the
+  // LDS: nd * COLS double2 (level sums, then the wave-matrix operand); the
   // reference profiles (6 x (nd+1) doubles); when stage_w, the wave matrices
   // W_n of the tile's degrees n_lo .. n_hi.  Everything a thread reads from
   // global memory is requested at entry, before the first barrier, so the
   // column work below waits for one round trip rather than one per stage.
   extern __shared__ double2 lds[];
-  const TileIdx tx = tile_idx<KPT>(spec_per_level);
+  const TileIdx tx = tile_idx<KPT, COLS>(spec_per_level);
   const size_t idx = tx.c;
   const int m = tx.col ? (int)(idx % nm) : 0;
   const int n = tx.col ? (int)(idx / nm) : 0;
   const int k0 = tx.k0;
   const int n1 = nd + 1;
-  double* refs = reinterpret_cast<double*>(lds + (size_t)nd * TILE);
+  double* refs = reinterpret_cast<double*>(lds + (size_t)nd * COLS);
   for (int k = threadIdx.x; k < n1; k += blockDim.x) {
     const bool in = k < nd;
     refs[k] = in ? rg.dp[k] : 0.0;
@@ -1107,9 +1124,9 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     refs[5 * n1 + k] = rg.bk[k];
   }
   const SIRef r{refs, refs + n1, refs + 2 * n1, refs + 3 * n1, refs + 4 * n1, refs + 5 * n1};
-  // the tile's 64 consecutive columns (idx = n nm + m) span degrees n_lo .. n_hi
-  const size_t first = (size_t)blockIdx.x * TILE;
-  const size_t last = min(first + TILE, spec_per_level) - 1;
+  // the tile's COLS consecutive columns (idx = n nm + m) span degrees n_lo .. n_hi
+  const size_t first = (size_t)blockIdx.x * COLS;
+  const size_t last = min(first + COLS, spec_per_level) - 1;
   const int n_lo = (int)(first / nm);
   const int nnd = nd * nd;
   double* Wl = refs + 6 * n1;
@@ -1194,7 +1211,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
   // every level of the column is needed, so the operand goes through LDS
 #pragma unroll
   for (int i = 0; i < KPT; ++i)
-    if (k0 + i < nd) lds[(k0 + i) * TILE + tx.lane] = A[i];
+    if (k0 + i < nd) lds[(k0 + i) * COLS + tx.lane] = A[i];
   __syncthreads();
   if (tx.col && n >= m && n <= num_wavenumbers) {
     // the level qq of the operand is read once for the KPT rows of this
@@ -1208,7 +1225,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
       for (int i = 0; i < KPT; ++i) s[i] = d2(0.0, 0.0);
 #pragma unroll 4
       for (int qq = 0; qq < nd; ++qq) {
-        const double2 x = lds[qq * TILE + tx.lane];
+        const double2 x = lds[qq * COLS + tx.lane];
 #pragma unroll
         for (int i = 0; i < KPT; ++i) {
           const double w = Wn[min(k0 + i, kmax) * nd + qq];
@@ -1286,13 +1303,18 @@ __global__ void geopotential_kernel(VC v, size_t P, double rdgas, const double* 
   geopotential_column(v, rdgas, lnp_half, lnp_full, t, geopots[c], g_full, g_half, P, c, 0.0);
 }
 
-// Tile shape for nd levels: KPT levels per thread, nd / KPT waves (<= MAXNQ).
-static int tile_kpt(int nd) { return nd <= 16 ? 2 : (nd <= 32 ? 4 : 8); }
+// Tile shape of the grid column kernels for nd levels: KPT levels per thread,
+// ceil(nd / KPT) level groups of GCOLS threads (at most 512 threads).
+static int tile_kpt(int nd) {
+  int k = 2;
+  while ((nd + k - 1) / k * GCOLS > MAXNQ * TILE) k *= 2;
+  return k;
+}
 static dim3 tile_block(int nd) {
   int kpt = tile_kpt(nd);
-  return dim3(TILE * ((nd + kpt - 1) / kpt));
+  return dim3(GCOLS * ((nd + kpt - 1) / kpt));
 }
-static dim3 tile_grid(size_t ncol) { return dim3((unsigned)((ncol + TILE - 1) / TILE)); }
+static dim3 tile_grid(size_t ncol) { return dim3((unsigned)((ncol + GCOLS - 1) / GCOLS)); }
 
 #define JGCM_TILE_DISPATCH(nd, KERNEL, ...)                                          \
   switch (tile_kpt(nd)) {                                                            \
@@ -1741,6 +1763,19 @@ int jgcm_semi_implicit_get(jgcm_semi_implicit_t* si, int which, double* host_out
   return JGCM_OK;
 }
 
+// Column tiles of the semi-implicit solve: SI_COLS spectral coefficients per
+// workgroup (half a wave, so that T85's 7482 coefficients per level make 234
+// workgroups), levels split into groups of si_kpt(nd) per thread.
+#ifndef JGCM_SI_COLS
+#define JGCM_SI_COLS 32
+#endif
+constexpr int SI_COLS = JGCM_SI_COLS;
+static int si_kpt(int nd) {
+  int k = 1;
+  while ((nd + k - 1) / k * SI_COLS > 512) k *= 2;
+  return k;
+}
+
 static SIRef siref(const jgcm_semi_implicit* si) {
   int nd = si->nd;
   const double* b = si->d_ref;
@@ -1757,18 +1792,29 @@ static int implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, co
   jgcm_mesh* M = si->vc->mesh;
   size_t S = M->spec_elems();
   double xi = xi_of(si->lf);
-  size_t lds = (size_t)si->nd * TILE * sizeof(double2) + (size_t)6 * (si->nd + 1) * sizeof(double);
-  // wave matrices of at most TILE / nm + 2 degrees per tile, staged when they fit
-  const size_t nw = std::min<size_t>((size_t)M->ns + 1, TILE / (size_t)M->nm + 2);
-  const size_t wbytes = nw * si->nd * si->nd * sizeof(double);
+  const int nd = si->nd;
+  size_t lds = (size_t)nd * SI_COLS * sizeof(double2) + (size_t)6 * (nd + 1) * sizeof(double);
+  // wave matrices of at most SI_COLS / nm + 2 degrees per tile, staged when they fit
+  const size_t nw = std::min<size_t>((size_t)M->ns + 1, SI_COLS / (size_t)M->nm + 2);
+  const size_t wbytes = nw * nd * nd * sizeof(double);
   const int stage_w = lds + wbytes <= 48 * 1024 ? 1 : 0;
   if (stage_w) lds += wbytes;
-  JGCM_TILE_DISPATCH(si->nd, hs_implicit_kernel, tile_grid(S), tile_block(si->nd), lds, M->stream,
-                     si->nd, M->nm, M->ns, S, si->atmo.kappa, si->atmo.rdgas, si->ps_ref, xi,
-                     siref(si), M->d_eig, si->d_wave, si->num_wavenumbers, (const double2*)div_c,
-                     (const double2*)div_p, (const double2*)lnps_c, (const double2*)lnps_p,
-                     (const double2*)t_c, (const double2*)t_p, (double2*)ddiv, (double2*)dlnps,
-                     (double2*)dt, (double2*)energy, stage_w, lf);
+  const int kpt = si_kpt(nd);
+  const dim3 grid((unsigned)((S + SI_COLS - 1) / SI_COLS)), block(SI_COLS * ((nd + kpt - 1) / kpt));
+#define JGCM_SI_LAUNCH(K)                                                                       \
+  hipLaunchKernelGGL((hs_implicit_kernel<K, SI_COLS>), grid, block, lds, M->stream, nd, M->nm,  \
+                     M->ns, S, si->atmo.kappa, si->atmo.rdgas, si->ps_ref, xi, siref(si),        \
+                     M->d_eig, si->d_wave, si->num_wavenumbers, (const double2*)div_c,           \
+                     (const double2*)div_p, (const double2*)lnps_c, (const double2*)lnps_p,      \
+                     (const double2*)t_c, (const double2*)t_p, (double2*)ddiv, (double2*)dlnps,  \
+                     (double2*)dt, (double2*)energy, stage_w, lf)
+  switch (kpt) {
+    case 1: JGCM_SI_LAUNCH(1); break;
+    case 2: JGCM_SI_LAUNCH(2); break;
+    case 4: JGCM_SI_LAUNCH(4); break;
+    default: JGCM_SI_LAUNCH(8); break;
+  }
+#undef JGCM_SI_LAUNCH
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

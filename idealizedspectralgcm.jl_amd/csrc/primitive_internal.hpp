@@ -63,24 +63,29 @@ inline VC vc_of(const jgcm_vert_coord* v) {
             c + 2 * n1 + nd, c + 2 * n1 + 2 * nd, c + 3 * n1 + 2 * nd, c + 3 * n1 + 3 * nd};
 }
 
-// Column tiles: a workgroup owns 64 consecutive columns (one per lane) and
-// nq = blockDim.x / 64 waves; wave q works on levels q*KPT .. q*KPT + KPT - 1.
+// Column tiles: a workgroup owns COLS consecutive columns (64 by default, one
+// per lane of a wave) and nq = blockDim.x / COLS level groups; group q works on
+// levels q*KPT .. q*KPT + KPT - 1.  With COLS < 64 one wave holds 64 / COLS
+// level groups of the same columns.
 constexpr int TILE = 64;
 
 struct TileIdx {
   int lane, q, nq, k0;
+  int cols;   // columns per tile (lane = column within the tile, 0 .. cols-1)
   size_t c;   // column (grid point or spectral coefficient)
   bool col;   // column exists
 };
 
-template <int KPT>
+template <int KPT, int COLS = TILE>
 __device__ __forceinline__ TileIdx tile_idx(size_t ncol) {
+  static_assert(COLS > 0 && TILE % COLS == 0, "a wave holds whole column tiles");
   TileIdx t;
-  t.lane = threadIdx.x & (TILE - 1);
-  t.q = threadIdx.x / TILE;
-  t.nq = blockDim.x / TILE;
+  t.lane = threadIdx.x % COLS;
+  t.q = threadIdx.x / COLS;
+  t.nq = blockDim.x / COLS;
   t.k0 = t.q * KPT;
-  t.c = (size_t)blockIdx.x * TILE + t.lane;
+  t.cols = COLS;
+  t.c = (size_t)blockIdx.x * COLS + t.lane;
   t.col = t.c < ncol;
   return t;
 }
@@ -92,21 +97,22 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// N per-thread partials summed, per lane, over the waves above (smaller level
-// index) and below this one, in wave order.  `red` holds N * nq * 64 doubles;
+// N per-thread partials summed, per column, over the level groups above
+// (smaller level index) and below this one, in group order.  `red` holds
+// N * blockDim.x doubles;
 // every thread of the workgroup must call it (two barriers, `red` is free
 // again on return).
 template <int N>
 __device__ inline void level_sums_n(double* red, const TileIdx& t, const double (&part)[N],
                                     double (&above)[N], double (&below)[N]) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) red[(i * t.nq + t.q) * TILE + t.lane] = part[i];
+  for (int i = 0; i < N; ++i) red[(i * t.nq + t.q) * t.cols + t.lane] = part[i];
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     double a = 0.0, b = 0.0;
     for (int w = 0; w < t.nq; ++w) {
-      const double x = red[(i * t.nq + w) * TILE + t.lane];
+      const double x = red[(i * t.nq + w) * t.cols + t.lane];
       if (w < t.q) a += x;
       if (w > t.q) b += x;
     }

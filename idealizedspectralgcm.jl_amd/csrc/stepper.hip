@@ -24,6 +24,34 @@ static inline int nblk(size_t n) {
 #define GRID_STRIDE(i, n) \
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
 
+// Compute_Spectral_Damping! (TI:76-80) of one element: δQ = (δQ - Q d) / (1 + Δt d)
+// with Q = Q_c on the first step, (δQ - Q_p d) / (1 + 2Δt d) afterwards.
+__device__ __forceinline__ double2 damp_reg(double2 q, double2 p, double2 c, double d,
+                                            int init_step, double dt) {
+  if (init_step) {
+    double den = 1.0 + dt * d;
+    return make_double2((q.x - c.x * d) / den, (q.y - c.y * d) / den);
+  }
+  double den = 1.0 + (2.0 * dt) * d;
+  return make_double2((q.x - p.x * d) / den, (q.y - p.y * d) / den);
+}
+
+// Filtered_Leapfrog! (TI:93-100) of one element: the filtered current value
+// and the new value.
+__device__ __forceinline__ void leapfrog_reg(double2 q, double2 p, double2 c, int init_step,
+                                             double dt, double r, double2& c_out, double2& n_out) {
+  if (init_step) {
+    double2 nx = make_double2(c.x + dt * q.x, c.y + dt * q.y);
+    n_out = nx;
+    c_out = make_double2(c.x + r * (-1.0 * c.x + nx.x), c.y + r * (-1.0 * c.y + nx.y));
+  } else {
+    double2 c1 = make_double2(c.x + r * (p.x - 2.0 * c.x), c.y + r * (p.y - 2.0 * c.y));
+    double2 nx = make_double2(p.x + (2.0 * dt) * q.x, p.y + (2.0 * dt) * q.y);
+    n_out = nx;
+    c_out = make_double2(c1.x + r * nx.x, c1.y + r * nx.y);
+  }
+}
+
 // Compute_Spectral_Damping! (TI:76-80) fused with Filtered_Leapfrog! (TI:93-100)
 // for element i.  `mode` bit 0: apply damping, bit 1: apply leapfrog.
 __device__ __forceinline__ void damped_leapfrog_elem(double2* __restrict__ dQ,
@@ -38,30 +66,17 @@ __device__ __forceinline__ void damped_leapfrog_elem(double2* __restrict__ dQ,
   double2 q = dQ[i];
   if (mode & 1) {
     double d = damping[m * (ns + 1) + n];
-    if (init_step) {
-      double2 c = Qc[i];
-      double den = 1.0 + dt * d;
-      q = make_double2((q.x - c.x * d) / den, (q.y - c.y * d) / den);
-    } else {
-      double2 p = Qp[i];
-      double den = 1.0 + (2.0 * dt) * d;
-      q = make_double2((q.x - p.x * d) / den, (q.y - p.y * d) / den);
-    }
+    q = damp_reg(q, init_step ? make_double2(0.0, 0.0) : Qp[i], init_step ? Qc[i] : make_double2(0.0, 0.0),
+                 d, init_step, dt);
     dQ[i] = q;
   }
   if (mode & 2) {
     double2 c = Qc[i];
-    if (init_step) {
-      double2 nx = make_double2(c.x + dt * q.x, c.y + dt * q.y);
-      Qn[i] = nx;
-      Qc[i] = make_double2(c.x + r * (-1.0 * c.x + nx.x), c.y + r * (-1.0 * c.y + nx.y));
-    } else {
-      double2 p = Qp[i];
-      double2 c1 = make_double2(c.x + r * (p.x - 2.0 * c.x), c.y + r * (p.y - 2.0 * c.y));
-      double2 nx = make_double2(p.x + (2.0 * dt) * q.x, p.y + (2.0 * dt) * q.y);
-      Qn[i] = nx;
-      Qc[i] = make_double2(c1.x + r * nx.x, c1.y + r * nx.y);
-    }
+    double2 p = init_step ? make_double2(0.0, 0.0) : Qp[i];
+    double2 cn, nx;
+    leapfrog_reg(q, p, c, init_step, dt, r, cn, nx);
+    Qn[i] = nx;
+    Qc[i] = cn;
   }
 }
 
@@ -92,7 +107,20 @@ __global__ void damped_leapfrog_batch_kernel(LeapfrogSet s, const double* __rest
                        s.mode[f]);
 }
 
-// SW Implicit_Correction! (Shallow_Water_Dynamics.jl:36-51)
+// SW Implicit_Correction! (Shallow_Water_Dynamics.jl:36-51) of one element
+// (D = δdiv, H = δh in and out; e = the Laplacian eigenvalue).
+__device__ __forceinline__ void sw_implicit_reg(double2& D, double2& H, double2 hc, double2 hp,
+                                                double2 dc, double2 dp, double e, double h0,
+                                                double mu, double mu2) {
+  D = make_double2(D.x + e * (hc.x - hp.x), D.y + e * (hc.y - hp.y));
+  H = make_double2(H.x + h0 * (dc.x - dp.x), H.y + h0 * (dc.y - dp.y));
+  double me = mu * e;
+  double den = 1.0 - (mu2 * e) * h0;
+  D = make_double2((D.x - me * H.x) / den, (D.y - me * H.y) / den);
+  double mh = mu * h0;
+  H = make_double2(H.x - mh * D.x, H.y - mh * D.y);
+}
+
 __global__ void sw_implicit_kernel(const double2* __restrict__ div_c, const double2* __restrict__ div_p,
                                    const double2* __restrict__ h_c, const double2* __restrict__ h_p,
                                    double h0, double2* __restrict__ ddiv, double2* __restrict__ dh,
@@ -104,16 +132,67 @@ __global__ void sw_implicit_kernel(const double2* __restrict__ div_c, const doub
     int n = (int)((i / nm) % (ns + 1));
     double e = eig[m * (ns + 1) + n];
     double2 D = ddiv[i], H = dh[i];
-    double2 hc = h_c[i], hp = h_p[i], dc = div_c[i], dp = div_p[i];
-    D = make_double2(D.x + e * (hc.x - hp.x), D.y + e * (hc.y - hp.y));
-    H = make_double2(H.x + h0 * (dc.x - dp.x), H.y + h0 * (dc.y - dp.y));
-    double me = mu * e;
-    double den = 1.0 - (mu2 * e) * h0;
-    D = make_double2((D.x - me * H.x) / den, (D.y - me * H.y) / den);
-    double mh = mu * h0;
-    H = make_double2(H.x - mh * D.x, H.y - mh * D.y);
+    sw_implicit_reg(D, H, h_c[i], h_p[i], div_c[i], div_p[i], e, h0, mu, mu2);
     ddiv[i] = D;
     dh[i] = H;
+  }
+}
+
+// The spectral part of a shallow-water step after the forward transforms
+// (Shallow_Water_Dynamics.jl:141-143 δdiv -= ∇²(KE + h), :150-157
+// Implicit_Correction!, :159-165 Compute_Spectral_Damping! + Filtered_Leapfrog!
+// of ζ, D, h) and the spectral part of Time_Advance! (Dyn_Data.jl:328-340:
+// p = the filtered c, c = n) for one element, with the arithmetic of
+// axpby_kernel, laplacian_kernel, sw_implicit_kernel and
+// damped_leapfrog_elem.  energy receives ∇²(KE + h_c) as spe_energy does.
+struct SWSpectral {
+  const double2* kin;
+  double2* energy;
+  double2* dvor;
+  double2* ddiv;
+  double2* dh;
+  double2* vor[3];  // p, c, n
+  double2* div[3];
+  double2* h[3];
+};
+
+__global__ void sw_spectral_kernel(SWSpectral s, const double* __restrict__ eig,
+                                   const double* __restrict__ damping, int nm, int ns, size_t n_all,
+                                   double h0, double mu, int init_step, double dt, double r) {
+  const double mu2 = mu * mu;
+  GRID_STRIDE(i, n_all) {
+    const int m = (int)(i % nm);
+    const int n = (int)((i / nm) % (ns + 1));
+    const int t = m * (ns + 1) + n;
+    const double e = eig[t], d = damping[t];
+    const double2 k = s.kin[i], hc = s.h[1][i], hp = s.h[0][i];
+    const double2 dc = s.div[1][i], dp = s.div[0][i];
+    const double2 vc = s.vor[1][i], vp = s.vor[0][i];
+    double2 D = s.ddiv[i], H = s.dh[i], Q = s.dvor[i];
+    double2 en = make_double2(1.0 * k.x + 1.0 * hc.x, 1.0 * k.y + 1.0 * hc.y);
+    en = make_double2(e * en.x, e * en.y);
+    s.energy[i] = en;
+    D = make_double2(1.0 * D.x + -1.0 * en.x, 1.0 * D.y + -1.0 * en.y);
+    sw_implicit_reg(D, H, hc, hp, dc, dp, e, h0, mu, mu2);
+    double2 cn, nx;
+    Q = damp_reg(Q, vp, vc, d, init_step, dt);
+    s.dvor[i] = Q;
+    leapfrog_reg(Q, vp, vc, init_step, dt, r, cn, nx);
+    s.vor[0][i] = cn;
+    s.vor[1][i] = nx;
+    s.vor[2][i] = nx;
+    D = damp_reg(D, dp, dc, d, init_step, dt);
+    s.ddiv[i] = D;
+    leapfrog_reg(D, dp, dc, init_step, dt, r, cn, nx);
+    s.div[0][i] = cn;
+    s.div[1][i] = nx;
+    s.div[2][i] = nx;
+    H = damp_reg(H, hp, hc, d, init_step, dt);
+    s.dh[i] = H;
+    leapfrog_reg(H, hp, hc, init_step, dt, r, cn, nx);
+    s.h[0][i] = cn;
+    s.h[1][i] = nx;
+    s.h[2][i] = nx;
   }
 }
 
@@ -200,18 +279,39 @@ __global__ void sw_explicit_kernel(const double* __restrict__ u, const double* _
                                    double km, double kt, double two_omega,
                                    double* __restrict__ du, double* __restrict__ dv,
                                    double* __restrict__ dh, double* __restrict__ absvor,
-                                   double* __restrict__ kin, int nlat, int nlon, size_t n) {
+                                   double* __restrict__ kin, double* __restrict__ u_p,
+                                   double* __restrict__ v_p, double* __restrict__ h_p, int nlat,
+                                   int nlon, size_t n) {
   GRID_STRIDE(i, n) {
     const int j = (int)((i / nlon) % nlat);
-    const double ui = u[i], vi = v[i];
+    const double ui = u[i], vi = v[i], hi = h[i];
     double x = -km * ui, y = -km * vi;
-    dh[i] = -kt * (h[i] - h_eq[i]);
+    dh[i] = -kt * (hi - h_eq[i]);
+    if (u_p) {  // the p = c half of Time_Advance! for u, v, h
+      u_p[i] = ui;
+      v_p[i] = vi;
+      h_p[i] = hi;
+    }
     const double a = vor[i] + two_omega * sint[j];
     absvor[i] = a;
     vorticity_flux_acc(a, ui, vi, x, y);
     du[i] = x;
     dv[i] = y;
     kin[i] = kinetic(ui, vi);
+  }
+}
+
+// The shallow-water height tendency's grid terms (Shallow_Water_Dynamics.jl:147-148):
+// Add_Horizontal_Advection!'s δh -= u ∂λh + v ∂θh (SSM:716, the arithmetic of
+// advection_kernel) and δh -= h div (xmy_kernel with sign -1, accumulating).
+__global__ void sw_height_tendency_kernel(double* __restrict__ dh, const double* __restrict__ u,
+                                          const double* __restrict__ v, const double* __restrict__ gl,
+                                          const double* __restrict__ gt, const double* __restrict__ h,
+                                          const double* __restrict__ div, size_t n) {
+  GRID_STRIDE(i, n) {
+    double x = advect(dh[i], u[i], v[i], gl[i], gt[i]);
+    double p = h[i] * div[i];
+    dh[i] = x + -p;
   }
 }
 
@@ -458,15 +558,66 @@ int jgcm_kinetic_energy(jgcm_mesh_t* mesh, const double* grid_u, const double* g
 int jgcm_sw_explicit(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
                      const double* grid_h, const double* h_eq, double kappa_m, double kappa_t,
                      const double* grid_vor, double omega, double* grid_du, double* grid_dv,
-                     double* grid_dh, double* grid_absvor, double* grid_kin, int nlev) {
+                     double* grid_dh, double* grid_absvor, double* grid_kin, double* grid_u_p,
+                     double* grid_v_p, double* grid_h_p, int nlev) {
   if (int rc = check_mesh(mesh)) return rc;
   JGCM_REQUIRE(grid_u && grid_v && grid_h && h_eq && grid_vor && grid_du && grid_dv && grid_dh &&
                    grid_absvor && grid_kin,
                "NULL pointer");
+  JGCM_REQUIRE((grid_u_p && grid_v_p && grid_h_p) || (!grid_u_p && !grid_v_p && !grid_h_p),
+               "give all three p arrays or none");
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
   size_t n = (size_t)nlev * mesh->grid_elems();
   hipLaunchKernelGGL(sw_explicit_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, grid_u, grid_v,
                      grid_h, h_eq, grid_vor, mesh->d_sin, kappa_m, kappa_t, 2 * omega, grid_du,
-                     grid_dv, grid_dh, grid_absvor, grid_kin, mesh->nlat, mesh->nlon, n);
+                     grid_dv, grid_dh, grid_absvor, grid_kin, grid_u_p, grid_v_p, grid_h_p,
+                     mesh->nlat, mesh->nlon, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_sw_height_tendency(jgcm_mesh_t* mesh, const double* spe_h, const double* grid_u,
+                            const double* grid_v, const double* grid_h, const double* grid_div,
+                            double* grid_dh, int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(spe_h && grid_u && grid_v && grid_h && grid_div && grid_dh, "NULL field pointer");
+  // the grid scratch for both gradients exists before they are handed out
+  if (int rc = ensure_scratch(mesh, 2 * nlev)) return rc;
+  if (int rc = jgcm_compute_gradients(mesh, spe_h, mesh->d_grid_a, mesh->d_grid_b, nlev)) return rc;
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  hipLaunchKernelGGL(sw_height_tendency_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, grid_dh,
+                     grid_u, grid_v, mesh->d_grid_a, mesh->d_grid_b, grid_h, grid_div, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+int jgcm_sw_spectral_step(jgcm_leapfrog_t* lf, double h_0, const double* spe_kin, double* spe_energy,
+                          double* spe_dvor, double* spe_ddiv, double* spe_dh, double* const* vor,
+                          double* const* div, double* const* h, int nlev) {
+  JGCM_REQUIRE(lf != nullptr, "leapfrog is NULL");
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(spe_kin && spe_energy && spe_dvor && spe_ddiv && spe_dh && vor && div && h,
+               "NULL argument");
+  SWSpectral s{};
+  s.kin = (const double2*)spe_kin;
+  s.energy = (double2*)spe_energy;
+  s.dvor = (double2*)spe_dvor;
+  s.ddiv = (double2*)spe_ddiv;
+  s.dh = (double2*)spe_dh;
+  for (int k = 0; k < 3; ++k) {
+    JGCM_REQUIRE(vor[k] && div[k] && h[k], "NULL field pointer (time level %d)", k);
+    s.vor[k] = (double2*)vor[k];
+    s.div[k] = (double2*)div[k];
+    s.h[k] = (double2*)h[k];
+  }
+  jgcm_mesh* M = lf->mesh;
+  size_t n = (size_t)nlev * M->spec_elems();
+  double dtv = lf->init_step ? (double)lf->dt : 2.0 * lf->dt;
+  double mu = lf->implicit_coef * dtv;
+  hipLaunchKernelGGL(sw_spectral_kernel, dim3(nblk(n)), dim3(256), 0, M->stream, s, M->d_eig,
+                     lf->d_damping, M->nm, M->ns, n, h_0, mu, lf->init_step ? 1 : 0,
+                     (double)lf->dt, lf->robert_coef);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

@@ -8,10 +8,12 @@ graph); no arithmetic happens on the host.  The per-step ``@show norm(...)``
 diagnostics of the reference are omitted (``verbose=True`` prints them, which
 synchronises the stream).
 """
+import ctypes
+
 import numpy as np
 
 from ._lib import call
-from .dyn_data import BAROTROPIC_STATE, SW_STATE, Time_Advance_Model
+from .dyn_data import BAROTROPIC_STATE, SW_STATE, Time_Advance_Model, Time_Advance_Tracers
 from .mesh import (_ptr, Vor_Div_From_Grid_UV, Apply_Laplacian, Add_Horizontal_Advection,
                    Trans_Spherical_To_Grid_Batch, Trans_Grid_To_Spherical_Batch, BATCH_DIVCOS,
                    BATCH_VORDIV, INV_UCOS, INV_VCOS)
@@ -93,17 +95,46 @@ def Shallow_Water_Dynamics(mesh, atmo_data, h_0, dyn_data, integrator, verbose=F
 
 def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kappa_t, h_eq):
     """Shallow_Water_Physics! then Shallow_Water_Dynamics! (the drivers' step,
-    Shallow_Water_Dynamics.jl:7-21 and :88-179) with the physics, the
-    absolute-vorticity residual and the kinetic energy in one grid pass
-    (jgcm_sw_explicit; no reference counterpart).  Equal to the two calls bit
+    Shallow_Water_Dynamics.jl:7-21 and :88-179) in fused passes (no reference
+    counterpart): the physics, the absolute-vorticity residual, the kinetic
+    energy and p = c of the grid state in one grid pass (jgcm_sw_explicit);
+    the advection and h div terms of the height tendency in one
+    (jgcm_sw_height_tendency); the spectral energy, implicit correction,
+    damped leapfrog and spectral Time_Advance! in one (jgcm_sw_spectral_step);
+    c = n of the grid state as one batched copy.  Equal to the two calls bit
     for bit."""
     d = dyn_data
     n = mesh._nlev(d.grid_vor, "grid")
     call("jgcm_sw_explicit", mesh.handle, _ptr(d.grid_u_c), _ptr(d.grid_v_c), _ptr(d.grid_ps_c),
          _ptr(h_eq), float(kappa_m), float(kappa_t), _ptr(d.grid_vor), atmo_data.omega,
          _ptr(d.grid_δu), _ptr(d.grid_δv), _ptr(d.grid_δlnps), _ptr(d.grid_absvor),
-         _ptr(d.grid_d_full1), n)
-    _sw_dynamics_rest(mesh, h_0, d, integrator, False)
+         _ptr(d.grid_d_full1), _ptr(d.grid_u_p), _ptr(d.grid_v_p), _ptr(d.grid_ps_p), n)
+    # :147-148 the height tendency's grid terms
+    call("jgcm_sw_height_tendency", mesh.handle, _ptr(d.spe_lnps_n), _ptr(d.grid_u_c),
+         _ptr(d.grid_v_c), _ptr(d.grid_ps_c), _ptr(d.grid_div), _ptr(d.grid_δlnps), n)
+    # :133 Vor_Div_From_Grid_UV!, :140 and :149 Trans_Grid_To_Spherical!
+    Trans_Grid_To_Spherical_Batch(
+        mesh, [d.grid_δu, d.grid_δv, d.grid_d_full1, d.grid_δlnps],
+        [d.spe_δvor, d.spe_δdiv, d.spe_d1, d.spe_δlnps], [BATCH_VORDIV, 0, 0, 0])
+    # :141-143, :150-165 and the spectral part of :177
+    P3 = ctypes.c_void_p * 3
+    lev = lambda f: P3(*(getattr(d, f + s).data_ptr() for s in ("_p", "_c", "_n")))
+    call("jgcm_sw_spectral_step", integrator.handle, float(h_0), _ptr(d.spe_d1),
+         _ptr(d.spe_energy), _ptr(d.spe_δvor), _ptr(d.spe_δdiv), _ptr(d.spe_δlnps),
+         lev("spe_vor"), lev("spe_div"), lev("spe_lnps"), n)
+    # :169-172 the new state on the grid
+    Trans_Spherical_To_Grid_Batch(
+        mesh, [d.spe_vor_n, d.spe_div_n, d.spe_lnps_n, d.spe_vor_n, d.spe_vor_n],
+        [d.grid_vor, d.grid_div, d.grid_ps_n, d.grid_u_n, d.grid_v_n],
+        [0, 0, 0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS],
+        snms_b=[None, None, None, d.spe_div_n, d.spe_div_n])
+    # :177 the c = n half of Time_Advance! for the grid state, then the tracers
+    fields = [(d.grid_u_c, d.grid_u_n), (d.grid_v_c, d.grid_v_n), (d.grid_ps_c, d.grid_ps_n)]
+    Pn = ctypes.c_void_p * len(fields)
+    nbytes = (ctypes.c_size_t * len(fields))(*(c.numel() * c.element_size() for c, _ in fields))
+    call("jgcm_copy_batch", mesh.handle, len(fields), Pn(*(c.data_ptr() for c, _ in fields)),
+         Pn(*(x.data_ptr() for _, x in fields)), nbytes)
+    Time_Advance_Tracers(d)
 
 
 def _sw_dynamics_rest(mesh, h_0, d, integrator, verbose):

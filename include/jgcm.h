@@ -468,11 +468,35 @@ int jgcm_shallow_water_physics(jgcm_mesh_t* mesh, const double* grid_u, const do
  * Shallow_Water_Dynamics! (:126-130 absolute vorticity and δu += absvor v,
  * δv -= absvor u; :139 KE = (u² + v²)/2) in one pass; results equal
  * jgcm_shallow_water_physics + jgcm_vorticity_flux(accumulate) +
- * jgcm_kinetic_energy bit for bit.                                          */
+ * jgcm_kinetic_energy bit for bit.  grid_u_p, grid_v_p, grid_h_p (all or
+ * none): also the p = c half of Time_Advance! (Dyn_Data.jl:328-362) for the
+ * grid u, v and h, which the step reads only as c.                          */
 int jgcm_sw_explicit(jgcm_mesh_t* mesh, const double* grid_u, const double* grid_v,
                      const double* grid_h, const double* h_eq, double kappa_m, double kappa_t,
                      const double* grid_vor, double omega, double* grid_du, double* grid_dv,
-                     double* grid_dh, double* grid_absvor, double* grid_kin, int nlev);
+                     double* grid_dh, double* grid_absvor, double* grid_kin, double* grid_u_p,
+                     double* grid_v_p, double* grid_h_p, int nlev);
+/* The shallow-water height tendency's grid terms (Shallow_Water_Dynamics.jl
+ * :147-148): jgcm_add_horizontal_advection(spe_h, u, v, δh) followed by
+ * δh -= h div (jgcm_xmy with sign -1, accumulating), the second in the same
+ * pass as the advection product.  Equal to the two calls bit for bit.  No
+ * reference counterpart (a fusion).                                         */
+int jgcm_sw_height_tendency(jgcm_mesh_t* mesh, const double* spe_h, const double* grid_u,
+                            const double* grid_v, const double* grid_h, const double* grid_div,
+                            double* grid_dh, int nlev);
+/* The spectral part of a shallow-water step after its forward transforms, in
+ * one pass: energy = ∇²(KE + h_c) and δdiv -= energy (Shallow_Water_Dynamics.jl
+ * :141-143), Implicit_Correction! (:150-157, Shallow_Water_Dynamics.jl:25-54),
+ * Compute_Spectral_Damping! + Filtered_Leapfrog! of ζ, D and h (:159-165,
+ * Time_Integrator.jl:58-101), and the spectral part of Time_Advance!
+ * (Dyn_Data.jl:328-340): vor/div/h are {p, c, n} arrays; on return p holds
+ * the filtered current value, c and n the new one.  Equal, bit for bit, to
+ * jgcm_axpby, jgcm_apply_laplacian, jgcm_axpby, jgcm_sw_implicit_correction,
+ * jgcm_damped_leapfrog_batch (damping on) and jgcm_time_advance of the three
+ * fields.  No reference counterpart (a fusion).                             */
+int jgcm_sw_spectral_step(jgcm_leapfrog_t* lf, double h_0, const double* spe_kin, double* spe_energy,
+                          double* spe_dvor, double* spe_ddiv, double* spe_dh, double* const* vor,
+                          double* const* div, double* const* h, int nlev);
 
 /* ---- primitive equations (Held-Suarez) ------------------------------------- */
 /* Vert_Coordinate(nλ, nθ, nd, vert_coord_option, vert_difference_option,
