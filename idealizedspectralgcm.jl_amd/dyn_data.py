@@ -58,11 +58,43 @@ _GRID_HALF = ["grid_p_half", "grid_lnp_half", "grid_M_half", "grid_geopot_half",
               "grid_d_half1", "grid_d_half2"]
 
 
+# Outputs of Compute_Pressures_And_Heights! (Press_And_Geopot.jl:3-19) at the
+# end of Atmosphere_Update! (Spectral_Dynamics.jl:485), in its argument order.
+# No later part of a step reads them (the column pass evaluates the pressures
+# of ps itself), so Atmosphere_Update defers them: they are computed from the
+# state when first read after a step (see Dyn_Data._defer_heights).
+HEIGHT_FIELDS = ("grid_p_half", "grid_Δp", "grid_lnp_half", "grid_p_full", "grid_lnp_full",
+                 "grid_z_full", "grid_z_half")
+
+
+def _deferred_field(name):
+    key = "_stored_" + name
+
+    def get(self):
+        src = self.__dict__.get("_heights_src")
+        if src is not None and self.__dict__.get("_heights_gen") != src[0]._state_gen:
+            self._materialize_heights()
+        return self.__dict__[key]
+
+    def put(self, value):
+        self.__dict__[key] = value
+
+    return property(get, put, doc=f"{name} (deferred after Atmosphere_Update; see HEIGHT_FIELDS)")
+
+
 class Dyn_Data:
     """``Dyn_Data(name, num_fourier, num_spherical, nλ, nθ, nd,
     num_grid_tracters=0, num_spe_tracters=0)`` (Dyn_Data.jl:158-324).  The
     mesh whose stream orders Time_Advance defaults to the last mesh created
-    with the same shape (``mesh=`` selects one explicitly)."""
+    with the same shape (``mesh=`` selects one explicitly).
+
+    The fields of HEIGHT_FIELDS hold, when read, what
+    Compute_Pressures_And_Heights! left at the end of the last
+    Atmosphere_Update: the step only records that they are due, and the first
+    read after it (after an eager step or a graph launch on the mesh) computes
+    all seven from grid_ps_c, grid_t_c and grid_geopots and waits for them.
+    They therefore match the reference unless those three arrays are changed
+    between the step and that first read."""
 
     def __init__(self, name, num_fourier, num_spherical, nλ, nθ, nd, num_grid_tracters=0,
                  num_spe_tracters=0, mesh=None, device="cuda"):
@@ -93,6 +125,26 @@ class Dyn_Data:
                 device=dev))
             setattr(self, f"grid_tracers_{s}", torch.zeros((num_grid_tracters, nd, nθ, nλ),
                                                            dtype=torch.float64, device=dev))
+
+    def _defer_heights(self, mesh, atmo_data, vert_coord):
+        """Record that Compute_Pressures_And_Heights! is due (the caller has
+        just enqueued a step on `mesh`)."""
+        mesh._state_gen += 1
+        self._heights_src = (mesh, atmo_data, vert_coord)
+
+    def _materialize_heights(self):
+        mesh, atmo_data, vert_coord = self._heights_src
+        self._heights_gen = mesh._state_gen
+        from .primitive import Compute_Pressures_And_Heights
+        Compute_Pressures_And_Heights(atmo_data, vert_coord, self.grid_ps_c, self.grid_geopots,
+                                      self.grid_t_c,
+                                      *(self.__dict__["_stored_" + f] for f in HEIGHT_FIELDS))
+        if not mesh._capturing:
+            mesh.synchronize()
+
+
+for _f in HEIGHT_FIELDS:
+    setattr(Dyn_Data, _f, _deferred_field(_f))
 
 
 def _tracer_names(d):

@@ -31,6 +31,7 @@ class Step_Graph:
         try:
             mesh.synchronize()
             call("jgcm_graph_begin", mesh.handle)
+            mesh._capturing = True
             try:
                 for _ in range(int(repeat)):
                     fn()
@@ -45,6 +46,7 @@ class Step_Graph:
             g = ctypes.c_void_p()
             call("jgcm_graph_end", mesh.handle, ctypes.byref(g))
         finally:
+            mesh._capturing = False
             if was_enabled:
                 gc.enable()
         self._g = g
@@ -52,6 +54,7 @@ class Step_Graph:
 
     def launch(self):
         call("jgcm_graph_launch", self._g)
+        self.mesh._state_gen += 1   # the state advanced: deferred diagnostics are stale
 
     def close(self):
         if getattr(self, "_g", None) is not None and self._g.value:

@@ -76,6 +76,11 @@ class Spectral_Spherical_Mesh:
         call("jgcm_mesh_create", self.num_fourier, self.num_spherical, self.nθ, self.nλ,
              self.nd, self.radius, ctypes.byref(h))
         self._h = h
+        # host-side bookkeeping of deferred diagnostics (Dyn_Data): a counter of
+        # state-advancing work enqueued on this mesh (eager steps, graph
+        # launches), and whether a graph capture is in progress
+        self._state_gen = 0
+        self._capturing = False
         self.set_precision(precision)
         N1, S1 = self.num_fourier + 1, self.num_spherical + 1
         shapes = {"sinθ": (self.nθ,), "cosθ": (self.nθ,), "wts": (self.nθ,),

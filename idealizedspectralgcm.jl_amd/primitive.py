@@ -336,7 +336,7 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
 
 
 def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, physics_params,
-                      overlap=False, fused=True):
+                      overlap=False, fused=True, eager_heights=False):
     """Spectral_Dynamics.jl:470-490.  By default (``fused``) the physics pass
     (HS forcing on the p-fields, the correction partials) runs inside the
     step's column pass, after the gradient transforms, its tendencies kept in
@@ -345,7 +345,10 @@ def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, phys
     first transforms (parallel branches of a captured step).  The results
     agree to rounding.  On MI355X at T85 L20 (one-step graphs): fused 204 us,
     separate serial 210 us, separate overlapped 218 us (the branch's fork and
-    join cost more than the pass)."""
+    join cost more than the pass).  The closing Compute_Pressures_And_Heights!
+    (:485) is deferred to the first read of its outputs (Dyn_Data.HEIGHT_FIELDS;
+    nothing in the next step reads them); ``eager_heights`` runs it here, in
+    the reference's order."""
     Δt = Get_Δt(semi_implicit.integrator)
     if fused and not overlap:
         Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
@@ -359,9 +362,14 @@ def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, phys
         Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
                           pressure_variables=False)
     d = dyn_data
-    Compute_Pressures_And_Heights(atmo_data, vert_coord, d.grid_ps_c, d.grid_geopots, d.grid_t_c,
-                                  d.grid_p_half, d.grid_Δp, d.grid_lnp_half, d.grid_p_full,
-                                  d.grid_lnp_full, d.grid_z_full, d.grid_z_half)
+    if eager_heights:
+        d._heights_src = None
+        Compute_Pressures_And_Heights(atmo_data, vert_coord, d.grid_ps_c, d.grid_geopots,
+                                      d.grid_t_c, d.grid_p_half, d.grid_Δp, d.grid_lnp_half,
+                                      d.grid_p_full, d.grid_lnp_full, d.grid_z_full,
+                                      d.grid_z_half)
+    else:  # :485 computed when first read (Dyn_Data.HEIGHT_FIELDS)
+        d._defer_heights(mesh, atmo_data, vert_coord)
 
 
 def Get_Topography(grid_geopots):

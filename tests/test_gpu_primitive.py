@@ -250,6 +250,43 @@ def test_overlapped_fused_and_captured_steps_equal_serial_steps(jgcm):
             assert np.array_equal(other[f], v), f
 
 
+def test_deferred_pressures_and_heights_equal_eager(jgcm):
+    """Atmosphere_Update defers Compute_Pressures_And_Heights! to the first
+    read of its seven outputs: after eager steps and after graph launches they
+    equal the reference-ordered (eager_heights) run bit for bit; once read
+    they keep the step's values (a later change of ps leaves them alone) until
+    the next step or launch."""
+    from jgcm_amd.dyn_data import HEIGHT_FIELDS
+    runs = []
+    for eager in (True, False):
+        _, (dev, dvc, datmo, dsi, dd) = build(jgcm, 21, 32, 20, 600)
+        step = lambda: jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS, eager_heights=eager)
+        step()
+        jgcm.Update_Init_Step(dsi)
+        step()
+        dev.synchronize()
+        first = {f: getattr(dd, f).clone() for f in HEIGHT_FIELDS}
+        g = jgcm.Step_Graph(dev, step, nfields=5 * 20 + 1, repeat=2)
+        g.launch()
+        dev.synchronize()
+        after_graph = {f: getattr(dd, f).clone() for f in HEIGHT_FIELDS}
+        g.launch()
+        g.close()
+        dev.synchronize()
+        runs.append((first, after_graph, {f: getattr(dd, f).clone() for f in HEIGHT_FIELDS}))
+        if not eager:
+            kept = dd.grid_p_full.clone()
+            dd.grid_ps_c.mul_(1.01)
+            torch.cuda.synchronize()
+            assert torch.equal(dd.grid_p_full, kept)
+    for a, b in zip(*runs):
+        for f in HEIGHT_FIELDS:
+            assert torch.equal(a[f], b[f]), f
+    deferred = runs[1]   # each launch advanced the state, and the heights with it
+    assert not torch.equal(deferred[0]["grid_z_full"], deferred[1]["grid_z_full"])
+    assert not torch.equal(deferred[1]["grid_z_full"], deferred[2]["grid_z_full"])
+
+
 def test_hs_t42_l20_two_hundred_day_climate(jgcm):
     """The HSt42 case run for its 200 days (FP64, Δt = 600 s) reproduces the
     Held & Suarez (1994) climate: a westerly jet in each hemisphere of 25-40 m/s
