@@ -91,9 +91,24 @@ __device__ inline void vert_advection_add(const VC& v, const double* r, const do
   }
 }
 
+// HS_Forcing! parameters as the kernels use them (formed on the host once per
+// call): kv0 = k_f / (1 - σ_b), kt0 = (k_s - k_a) / (1 - σ_b) and
+// rsb = 1 / (1 - σ_b) with the rates per second, rcp = 1 / c_p.
 struct HSParams {
-  double sigma_b, k_f, k_a, k_s, dT_y, dth_z;
+  double sigma_b, kv0, ka, kt0, rsb, rcp, dT_y, dth_z;
 };
+
+static HSParams hs_params(const jgcm_hs_params* hp, double cp_air) {
+  const double day = 86400;
+  const double kf = hp->k_f / day, ka = hp->k_a / day, ks = hp->k_s / day;
+  return HSParams{hp->sigma_b, kf / (1.0 - hp->sigma_b), ka, (ks - ka) / (1.0 - hp->sigma_b),
+                  1.0 / (1.0 - hp->sigma_b), 1.0 / cp_air, hp->delta_T_y, hp->delta_theta_z};
+}
+
+// latitude row of grid column c (32-bit division: a grid has < 2^31 columns)
+__device__ __forceinline__ int lat_of(size_t c, int nlon) {
+  return (int)((unsigned)c / (unsigned)nlon);
+}
 
 // Four_In_One! (Spectral_Dynamics.jl:88-183) for one column.
 __device__ inline void four_in_one_column(const VC& v, double rdgas, double cp_air,
@@ -305,6 +320,74 @@ __device__ inline void block_area_partial(const TileIdx& t, double wj, double v0
   if (t.lane == 0) partial[blockIdx.x] = d2(s0, s1);
 }
 
+// HS_Forcing! (HS_Forcing.jl:1-33) of one level of a column, shared by the
+// physics pass and the fused column pass so that both round alike: Rayleigh
+// damping of u, v (Rayleigh_Damping!, :101-130) with its heating (:25) and
+// Newtonian cooling of T (Newtonian_Damping!, :37-97).  The column's
+// constants are formed once (reciprocals of 1 - σ_b, p_half[nd] and c_p;
+// on pure sigma (p_full/p_ref)^κ = skap_k (ps/p_ref)^κ, one exponential per
+// column).
+struct HSCol {
+  double kv0, ka, kt0, sb, rsb, s2, c2, c4, rpsb, epk, rcp, dt, kappa, dT_y, dth_z;
+  const double* skap;
+};
+
+__device__ __forceinline__ HSCol hs_column_constants(const VC& v, const HSParams& hp,
+                                                     const double* skap, double kappa,
+                                                     double dt, double ps, double sj) {
+  const double ln_pref = 11.512925464970229;   // ln(1e5)
+  HSCol h;
+  h.kv0 = hp.kv0;
+  h.ka = hp.ka;
+  h.kt0 = hp.kt0;
+  h.sb = hp.sigma_b;
+  h.rsb = hp.rsb;
+  h.s2 = sj * sj;
+  h.c2 = 1.0 - h.s2;
+  h.c4 = h.c2 * h.c2;
+  h.rpsb = 1.0 / (v.ak[v.nd] + v.bk[v.nd] * ps);   // 1 / p_half[nd]
+  h.epk = v.pure ? exp(kappa * (log(ps) - ln_pref)) : 0.0;   // (ps / p_ref)^κ
+  h.rcp = hp.rcp;
+  h.dt = dt;
+  h.kappa = kappa;
+  h.dT_y = hp.dT_y;
+  h.dth_z = hp.dth_z;
+  h.skap = skap;
+  return h;
+}
+
+// level k (clamped into range by the caller) with full-level pressure pf and
+// its logarithm lnf; p-field values U, V, T; tendencies and T_eq out
+__device__ __forceinline__ void hs_forcing_level(const VC& v, const HSCol& h, int k, double pf,
+                                                 double lnf, double U, double V, double T,
+                                                 bool ok, double& DU, double& DV, double& DT,
+                                                 double& teq, int& isbad) {
+  const double t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
+  const double ln_pref = 11.512925464970229;   // ln(1e5)
+  const double s = pf * h.rpsb;
+  if (ok && (s - h.sb) * h.rsb > 1.0) isbad = 1;
+  const double kv = h.kv0 * fmax(0.0, s - h.sb);
+  const double a = -kv * U, b = -kv * V;
+  DU = a;
+  DV = b;
+  const double d = -((U + 0.5 * a * h.dt) * a + (V + 0.5 * b * h.dt) * b) * h.rcp;
+  // ln(p/p_ref) and (p/p_ref)^κ; on pure sigma ln p_full is ln ps plus a
+  // level constant and the power a level constant times a column factor
+  double lpn, pk;
+  if (v.pure) {
+    lpn = lnf - ln_pref;
+    pk = h.skap[k] * h.epk;
+  } else {
+    const double pn = pf / p_ref;
+    lpn = log(pn);
+    pk = pow(pn, h.kappa);
+  }
+  const double te = (t_zero - h.dT_y * h.s2 - h.dth_z * h.c2 * lpn) * pk;
+  teq = fmax(t_strat, te);
+  const double kt = h.ka + h.kt0 * fmax(0.0, s - h.sb) * h.c4;
+  DT = d - kt * (T - teq);
+}
+
 // ------------------------------------------------------------------ kernels
 // Step part 1 (Spectral_Dynamics_Physics!, SD:452-467; Compute_Corrections_Init,
 // SD:5-28; Pressure_Variables!, SD:256): pressure variables of ps_c, HS_Forcing!
@@ -317,12 +400,13 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
     double kappa, double cp_air, double dt, HSParams hp, const double* ps_c, const double* u_p,
     const double* v_p, const double* t_p, const double* ps_p, double* p_half, double* dp,
     double* lnp_half, double* p_full, double* lnp_full, double* du, double* dv, double* t_eq,
-    double* dtt, double* dps, double2* partial, int forcing, int mass, int energy, int* bad) {
+    double* dtt, double* dps, double2* partial, int forcing, int mass, int energy, int* bad,
+    const double* __restrict__ skap) {
   __shared__ double2 red[MAXNQ * TILE];
   const size_t P = (size_t)nlat * nlon;
   const TileIdx t = tile_idx<KPT, GCOLS>(P);
   const int nd = v.nd;
-  const int j = t.col ? (int)(t.c / nlon) : 0;
+  const int j = t.col ? lat_of(t.c, nlon) : 0;
   const double ps = t.col ? ps_c[t.c] : 1.0e5;
   PressTile<KPT> pr;
   pressure_tile<KPT>(v, t.k0, ps, pr);
@@ -331,7 +415,7 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
   if (p_half) store_pressure<KPT>(v, t, P, pr, p_half, dp, lnp_half, p_full, lnp_full);
   if (t.col && t.q == 0) dps[t.c] = 0.0;
   const double psp = (t.col && (mass || energy)) ? ps_p[t.c] : 0.0;
-  const double wj = (t.col && t.q == 0 && (mass || energy)) ? wts[t.c / nlon] : 0.0;
+  const double wj = (t.col && t.q == 0 && (mass || energy)) ? wts[j] : 0.0;
 
   double U[KPT], V[KPT], T[KPT], DU[KPT], DV[KPT], DT[KPT];
 #pragma unroll
@@ -348,33 +432,14 @@ __global__ __launch_bounds__(512) void hs_physics_kernel(
     }
   }
   if (forcing) {
-    const double day = 86400;
-    const double t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
-    const double ln_pref = 11.512925464970229;   // ln(1e5)
-    const double kf = hp.k_f / day, ka = hp.k_a / day, ks = hp.k_s / day;
-    const double sj = sint[j];
-    const double s2 = sj * sj, c2 = 1.0 - s2, c4 = c2 * c2;
-    const double psb = v.ak[nd] + v.bk[nd] * ps;  // p_half[nd]
+    const HSCol h = hs_column_constants(v, hp, skap, kappa, dt, ps, sint[j]);
     int isbad = 0;
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
       bool ok = t.col && t.k0 + i < nd;
-      double s = pr.pf[i] / psb;
-      if (ok && (s - hp.sigma_b) / (1.0 - hp.sigma_b) > 1.0) isbad = 1;
-      double kv = kf / (1.0 - hp.sigma_b) * fmax(0.0, s - hp.sigma_b);
-      double a = -kv * U[i], b = -kv * V[i];
-      DU[i] = a;
-      DV[i] = b;
-      double d = -((U[i] + 0.5 * a * dt) * a + (V[i] + 0.5 * b * dt) * b) / cp_air;
-      double pn = pr.pf[i] / p_ref;
-      // ln(p/p_ref) and (p/p_ref)^κ; on pure sigma ln p_full is ln ps plus a
-      // level constant, so no further logarithm is needed
-      double lpn = v.pure ? pr.lnf[i] - ln_pref : log(pn);
-      double te = (t_zero - hp.dT_y * s2 - hp.dth_z * c2 * lpn) *
-                  (v.pure ? exp(kappa * lpn) : pow(pn, kappa));
-      double teq = fmax(t_strat, te);
-      double kt = ka + (ks - ka) / (1.0 - hp.sigma_b) * fmax(0.0, s - hp.sigma_b) * c4;
-      DT[i] = d - kt * (T[i] - teq);
+      double teq;
+      hs_forcing_level(v, h, min(t.k0 + i, nd - 1), pr.pf[i], pr.lnf[i], U[i], V[i], T[i], ok,
+                       DU[i], DV[i], DT[i], teq, isbad);
       if (ok) {
         size_t o = (size_t)(t.k0 + i) * P + t.c;
         du[o] = DU[i];
@@ -424,6 +489,7 @@ struct PhysArgs {
   double2* partial;
   int mass, energy;
   int* bad;
+  const double* skap;
 };
 
 #define JGCM_COLUMN_DYNAMICS_ARGS                                                            \
@@ -458,7 +524,8 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
   const double b = tx.col ? dtps[c] * p : 0.0;
   const double ps_tend = (!PHYS && tx.col) ? dps[c] : 0.0;   // the physics pass sets δps = 0
   const double gs = tx.col ? geopots[c] : 0.0;
-  const double f = two_omega * sint[tx.col ? (int)(c / nlon) : 0];
+  const int jlat = tx.col ? lat_of(c, nlon) : 0;
+  const double f = two_omega * sint[jlat];
   // Pressure_Variables! of ps (SD:256): read from the given fields, or, when
   // they are NULL, evaluated here with the arithmetic of the physics pass
   // (bit-identical to the stored values)
@@ -521,34 +588,17 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
       Tp[i] = ok ? pa.t_p[o] : 0.0;
     }
     const double psp = (tx.col && (pa.mass || pa.energy)) ? pa.ps_p[c] : 0.0;
-    const double wj = (tx.col && tx.q == 0 && (pa.mass || pa.energy)) ? pa.wts[c / nlon] : 0.0;
-    const HSParams& hp = pa.hp;
-    const double day = 86400;
-    const double t_zero = 315.0, t_strat = 200.0, p_ref = 1.0e5;
-    const double ln_pref = 11.512925464970229;   // ln(1e5)
-    const double kf = hp.k_f / day, ka = hp.k_a / day, ks = hp.k_s / day;
-    const double sj = sint[tx.col ? (int)(c / nlon) : 0];
-    const double s2 = sj * sj, c2 = 1.0 - s2, c4 = c2 * c2;
-    const double psb = v.ak[nd] + v.bk[nd] * p;  // p_half[nd]
+    const double wj = (tx.col && tx.q == 0 && (pa.mass || pa.energy)) ? pa.wts[jlat] : 0.0;
+    const HSCol h = hs_column_constants(v, pa.hp, pa.skap, pa.kappa, pa.dt, p,
+                                        sint[jlat]);
     const double dt = pa.dt;
     int isbad = 0;
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
       bool ok = tx.col && k0 + i < nd;
-      double sg = pr.pf[i] / psb;
-      if (ok && (sg - hp.sigma_b) / (1.0 - hp.sigma_b) > 1.0) isbad = 1;
-      double kv = kf / (1.0 - hp.sigma_b) * fmax(0.0, sg - hp.sigma_b);
-      double fa = -kv * Up[i], fb = -kv * Vp[i];
-      DU[i] = fa;
-      DV[i] = fb;
-      double dh = -((Up[i] + 0.5 * fa * dt) * fa + (Vp[i] + 0.5 * fb * dt) * fb) / pa.cp_air;
-      double pn = pr.pf[i] / p_ref;
-      double lpn = v.pure ? pr.lnf[i] - ln_pref : log(pn);
-      double te = (t_zero - hp.dT_y * s2 - hp.dth_z * c2 * lpn) *
-                  (v.pure ? exp(pa.kappa * lpn) : pow(pn, pa.kappa));
-      double teq = fmax(t_strat, te);
-      double kt = ka + (ks - ka) / (1.0 - hp.sigma_b) * fmax(0.0, sg - hp.sigma_b) * c4;
-      DT[i] = dh - kt * (Tp[i] - teq);
+      double teq;
+      hs_forcing_level(v, h, min(k0 + i, nd - 1), pr.pf[i], pr.lnf[i], Up[i], Vp[i], Tp[i], ok,
+                       DU[i], DV[i], DT[i], teq, isbad);
       if (ok) pa.t_eq[(size_t)(k0 + i) * P + c] = teq;
     }
     if (isbad) atomicAdd(pa.bad, 1);
@@ -573,6 +623,10 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
 
   // Four_In_One!: pressure-gradient force, mass-flux divergence (prefix sum
   // over levels), ω-α term and the σ-dot mass flux M_half.
+  // 1 / Δp of the tile's levels (each level divides by its Δp five times)
+  double rD[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) rD[i] = 1.0 / D[i + 1];
   double dl[KPT], dth[KPT], dmean[KPT];
   double part = 0.0;
 #pragma unroll
@@ -583,7 +637,7 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
     double Dp = D[i + 1];
     double dlnp_p = LH[i + 1] - LF[i];
     double dlnp_m = LF[i] - LH[i];
-    double x1 = (v.bk[kk] * dlnp_m + v.bk[kk + 1] * dlnp_p) / Dp;
+    double x1 = (v.bk[kk] * dlnp_m + v.bk[kk + 1] * dlnp_p) * rD[i];
     dl[i] = x1 * a;
     dth[i] = x1 * b;
     DU[i] = DU[i] - rdgas * T[i + 1] * dl[i];
@@ -604,8 +658,7 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
     int k = k0 + i;
     double dlnp_p = LH[i + 1] - LF[i];
     double dlnp = LH[i + 1] - LH[i];
-    double Dp = D[i + 1];
-    double x5 = -(run * dlnp + dmean[i] * dlnp_p) / Dp + U[i + 1] * dl[i] + V[i + 1] * dth[i];
+    double x5 = -(run * dlnp + dmean[i] * dlnp_p) * rD[i] + U[i + 1] * dl[i] + V[i + 1] * dth[i];
     DT[i] = DT[i] + kappa * T[i + 1] * x5;
     W[i] = x5 * PF[i];
     run = run + dmean[i];
@@ -628,32 +681,35 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
   for (int i = 0; i <= KPT; ++i) LHk[i] = LH[i];
   geopotential_tile<KPT>(v, tx, red, rdgas, Tk, LHk, LF, gs, gfull, ghalf, 0.0);
 
-  // vertical advection of u, v, T with w = M_half, dz = Δp
-  auto face = [&](const double (&R)[KPT + 2], int i) -> double {
-    // flux through half level k0+i (i = 0 .. KPT), between R[i] and R[i+1]
-    int h = k0 + i;
-    if (h <= 0 || h >= nd) return 0.0;
-    double r0 = R[i], r1 = R[i + 1];
-    if (v.scheme == 0) return Mh[i] * (r0 + (r1 - r0) * D[i] / (D[i] + D[i + 1]));
-    return Mh[i] * (r0 + r1) / 2.0;
-  };
+  // vertical advection of u, v, T with w = M_half, dz = Δp: fluxes through
+  // the half levels k0+i (i = 0 .. KPT), between R[i] and R[i+1], with the
+  // centred weight Δp_k / (Δp_k + Δp_k+1) formed once for the three fields
   double fu[KPT + 1], fv[KPT + 1], ft[KPT + 1];
 #pragma unroll
   for (int i = 0; i <= KPT; ++i) {
-    fu[i] = face(U, i);
-    fv[i] = face(V, i);
-    ft[i] = face(T, i);
+    const int h = k0 + i;
+    if (h <= 0 || h >= nd) {
+      fu[i] = fv[i] = ft[i] = 0.0;
+    } else if (v.scheme == 0) {
+      const double w = D[i] / (D[i] + D[i + 1]);
+      fu[i] = Mh[i] * (U[i] + (U[i + 1] - U[i]) * w);
+      fv[i] = Mh[i] * (V[i] + (V[i + 1] - V[i]) * w);
+      ft[i] = Mh[i] * (T[i] + (T[i + 1] - T[i]) * w);
+    } else {
+      fu[i] = Mh[i] * (U[i] + U[i + 1]) / 2.0;
+      fv[i] = Mh[i] * (V[i] + V[i + 1]) / 2.0;
+      ft[i] = Mh[i] * (T[i] + T[i + 1]) / 2.0;
+    }
   }
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
     int k = k0 + i;
     if (!(tx.col && k < nd)) continue;
     size_t o = (size_t)k * P + c;
-    double dz = D[i + 1];
     double wd = Mh[i + 1] - Mh[i];
-    double ru = (fu[i] - fu[i + 1] + U[i + 1] * wd) / dz;
-    double rv = (fv[i] - fv[i + 1] + V[i + 1] * wd) / dz;
-    double rt = (ft[i] - ft[i + 1] + T[i + 1] * wd) / dz;
+    double ru = (fu[i] - fu[i + 1] + U[i + 1] * wd) * rD[i];   // dz = Δp
+    double rv = (fv[i] - fv[i + 1] + V[i + 1] * wd) * rD[i];
+    double rt = (ft[i] - ft[i + 1] + T[i + 1] * wd) * rD[i];
     double du_k = DU[i] + ru, dv_k = DV[i] + rv;
     double av = VOR[i] + f;
     double uk = U[i + 1], vk = V[i + 1];
@@ -704,7 +760,7 @@ __global__ __launch_bounds__(512) void hs_corrections_sums_kernel(
   const size_t P = (size_t)nlat * nlon;
   const TileIdx tx = tile_idx<KPT, GCOLS>(P);
   const double p = tx.col ? exp(lnps[tx.c]) : 0.0;
-  const double wj = (sums && tx.col && tx.q == 0) ? wts[tx.c / nlon] : 0.0;
+  const double wj = (sums && tx.col && tx.q == 0) ? wts[lat_of(tx.c, nlon)] : 0.0;
   double pa = 0.0, pb = 0.0;
   if (energy) {
 #pragma unroll
@@ -961,6 +1017,7 @@ struct SIRef {
   const double* t_ref;
   const double* h;
   const double* bk;
+  const double* rdp;  // 1 / dp
 };
 
 // Linear_Ps_T_δdiv! (Semi_Implicit.jl:271-341) on one tile: δps and δT of
@@ -993,7 +1050,7 @@ __device__ __forceinline__ void linear_ps_t_tile(int nd, double kappa, const SIR
       double dlp = r.lnp_half[k + 1] - r.lnp_full[k];
       double dl = r.lnp_half[k + 1] - r.lnp_half[k];
       double s = -kappa * r.t_ref[k];
-      B[i] = d2(s * (run.x * dl + dm[i].x * dlp) / r.dp[k], s * (run.y * dl + dm[i].y * dlp) / r.dp[k]);
+      B[i] = d2(s * (run.x * dl + dm[i].x * dlp) * r.rdp[k], s * (run.y * dl + dm[i].y * dlp) * r.rdp[k]);
       run = d2add(run, dm[i]);
       Mh[i + 1] = (k + 1 < nd) ? d2add(d2(-run.x, -run.y), d2scale(r.bk[k + 1], total)) : d2(0.0, 0.0);
     } else {
@@ -1008,7 +1065,7 @@ __device__ __forceinline__ void linear_ps_t_tile(int nd, double kappa, const SIR
     if (k >= nd) continue;
     double2 next = (k + 1 < nd) ? d2scale(r.t_ref[k + 1] - r.t_ref[k], Mh[i + 1]) : d2(0.0, 0.0);
     double2 prev = (k >= 1) ? d2scale(r.t_ref[k] - r.t_ref[k - 1], Mh[i]) : d2(0.0, 0.0);
-    B[i] = d2(B[i].x - 0.5 * (next.x + prev.x) / r.dp[k], B[i].y - 0.5 * (next.y + prev.y) / r.dp[k]);
+    B[i] = d2(B[i].x - 0.5 * (next.x + prev.x) * r.rdp[k], B[i].y - 0.5 * (next.y + prev.y) * r.rdp[k]);
   }
 }
 
@@ -1109,8 +1166,8 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
   extern __shared__ double2 lds[];
   const TileIdx tx = tile_idx<KPT, COLS>(spec_per_level);
   const size_t idx = tx.c;
-  const int m = tx.col ? (int)(idx % nm) : 0;
-  const int n = tx.col ? (int)(idx / nm) : 0;
+  const int n = tx.col ? (int)((unsigned)idx / (unsigned)nm) : 0;
+  const int m = tx.col ? (int)idx - n * nm : 0;
   const int k0 = tx.k0;
   const int n1 = nd + 1;
   double* refs = reinterpret_cast<double*>(lds + (size_t)nd * COLS);
@@ -1122,14 +1179,16 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     refs[3 * n1 + k] = in ? rg.t_ref[k] : 0.0;
     refs[4 * n1 + k] = in ? rg.h[k] : 0.0;
     refs[5 * n1 + k] = rg.bk[k];
+    refs[6 * n1 + k] = in ? rg.rdp[k] : 0.0;
   }
-  const SIRef r{refs, refs + n1, refs + 2 * n1, refs + 3 * n1, refs + 4 * n1, refs + 5 * n1};
+  const SIRef r{refs, refs + n1, refs + 2 * n1, refs + 3 * n1, refs + 4 * n1, refs + 5 * n1,
+                refs + 6 * n1};
   // the tile's COLS consecutive columns (idx = n nm + m) span degrees n_lo .. n_hi
   const size_t first = (size_t)blockIdx.x * COLS;
   const size_t last = min(first + COLS, spec_per_level) - 1;
   const int n_lo = (int)(first / nm);
   const int nnd = nd * nd;
-  double* Wl = refs + 6 * n1;
+  double* Wl = refs + 7 * n1;
   if (stage_w) {
     const int nW = (int)(last / nm) - n_lo + 1;
     for (int q = threadIdx.x; q < nW * nnd; q += blockDim.x) {
@@ -1563,6 +1622,7 @@ int jgcm_vert_coord_create_ex(jgcm_mesh_t* mesh, int nd, int vert_coord_option,
       sf[k] = std::exp(cf[k]);
     }
   }
+  v->cf = cf;
   std::vector<double> all;
   all.insert(all.end(), v->ak.begin(), v->ak.end());
   all.insert(all.end(), v->bk.begin(), v->bk.end());
@@ -1712,6 +1772,9 @@ int jgcm_semi_implicit_create(jgcm_vert_coord_t* v, const jgcm_atmo_params* atmo
   dev.insert(dev.end(), lf_.begin(), lf_.end());
   dev.insert(dev.end(), si->t_ref.begin(), si->t_ref.end());
   dev.insert(dev.end(), si->h.begin(), si->h.end());
+  for (int k = 0; k < nd; ++k)
+    dev.push_back(v->pure_sigma ? std::exp(si->atmo.kappa * v->cf[k]) : 1.0);
+  for (int k = 0; k < nd; ++k) dev.push_back(1.0 / dp[k]);
   hipError_t e = hipMalloc(&si->d_ref, dev.size() * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(si->d_ref, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&si->d_scal, 16 * sizeof(double));
@@ -1780,8 +1843,12 @@ static SIRef siref(const jgcm_semi_implicit* si) {
   int nd = si->nd;
   const double* b = si->d_ref;
   return SIRef{b, b + nd, b + nd + (nd + 1), b + nd + (nd + 1) + nd, b + nd + (nd + 1) + 2 * nd,
-               si->vc->d_coef + (nd + 1)};
+               si->vc->d_coef + (nd + 1), b + 6 * nd + 1};
 }
+
+// (p_full_k / ps)^κ per level (pure sigma), after the reference profiles and
+// before their reciprocal Δp
+static const double* skap_of(const jgcm_semi_implicit* si) { return si->d_ref + 5 * si->nd + 1; }
 
 static int implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, const double* div_p,
                                const double* lnps_c, const double* lnps_p, const double* t_c,
@@ -1793,7 +1860,7 @@ static int implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, co
   size_t S = M->spec_elems();
   double xi = xi_of(si->lf);
   const int nd = si->nd;
-  size_t lds = (size_t)nd * SI_COLS * sizeof(double2) + (size_t)6 * (nd + 1) * sizeof(double);
+  size_t lds = (size_t)nd * SI_COLS * sizeof(double2) + (size_t)7 * (nd + 1) * sizeof(double);
   // wave matrices of at most SI_COLS / nm + 2 degrees per tile, staged when they fit
   const size_t nw = std::min<size_t>((size_t)M->ns + 1, SI_COLS / (size_t)M->nm + 2);
   const size_t wbytes = nw * nd * nd * sizeof(double);
@@ -1876,7 +1943,7 @@ int jgcm_hs_physics(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t 
   }
   jgcm_mesh* M = si->vc->mesh;
   size_t P = M->grid_elems();
-  HSParams h{hp->sigma_b, hp->k_f, hp->k_a, hp->k_s, hp->delta_T_y, hp->delta_theta_z};
+  HSParams h = hs_params(hp, si->atmo.cp_air);
   int mass = si->atmo.do_mass_correction, energy = si->atmo.do_energy_correction;
   int* bad = reinterpret_cast<int*>(si->d_scal + 15);
   int nd = si->nd;
@@ -1884,7 +1951,8 @@ int jgcm_hs_physics(jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t 
                      vc_of(si->vc), M->nlat, M->nlon, M->d_sin, M->d_wts, si->atmo.kappa,
                      si->atmo.cp_air, (double)dt, h, grid_ps_c, grid_u_p, grid_v_p, grid_t_p,
                      grid_ps_p, p_half, dp, lnp_half, p_full, lnp_full, grid_du, grid_dv, grid_t_eq,
-                     grid_dt, grid_dps, area_partials(si), do_forcing, mass, energy, bad);
+                     grid_dt, grid_dps, area_partials(si), do_forcing, mass, energy, bad,
+                     skap_of(si));
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }
@@ -1953,10 +2021,10 @@ int jgcm_hs_physics_column_dynamics(
   size_t P = M->grid_elems();
   int nd = si->nd;
   PhysArgs pa{M->d_wts, si->atmo.kappa, si->atmo.cp_air, (double)dt,
-              HSParams{hp->sigma_b, hp->k_f, hp->k_a, hp->k_s, hp->delta_T_y, hp->delta_theta_z},
+              hs_params(hp, si->atmo.cp_air),
               grid_u_p, grid_v_p, grid_t_p, grid_ps_p, grid_t_eq, area_partials(si),
               si->atmo.do_mass_correction, si->atmo.do_energy_correction,
-              reinterpret_cast<int*>(si->d_scal + 15)};
+              reinterpret_cast<int*>(si->d_scal + 15), skap_of(si)};
   const double* none = nullptr;
   JGCM_TILE_DISPATCH(nd, hs_physics_column_dynamics_kernel, tile_grid(P), tile_block(nd), 0,
                      M->stream, pa, vc_of(si->vc), M->nlat, M->nlon, si->atmo.rdgas,

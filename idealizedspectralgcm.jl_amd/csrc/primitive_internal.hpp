@@ -13,6 +13,7 @@ struct jgcm_vert_coord {
   int zero_top = 1;
   double p_ref = 101325.0;
   std::vector<double> ak, bk, dak, dbk;
+  std::vector<double> cf;  // pure sigma: ln p_full_k - ln ps (host copy of the device table)
   // [ak(nd+1) | bk(nd+1) | dak(nd) | dbk(nd) | lnb(nd+1) | cf(nd) | sf(nd)]
   double* d_coef = nullptr;
   // pure sigma (every a_k = 0, the top half level usable): the pressure
@@ -31,7 +32,9 @@ struct jgcm_semi_implicit {
   std::vector<double> nu, tau, gamma;
   int num_wavenumbers = 0;
   std::vector<double> wave_matrix;  // [num_wavenumbers+1][nd][nd]
-  double* d_ref = nullptr;          // [dp_ref | lnp_half_ref | lnp_full_ref | t_ref | h] on device
+  // [dp_ref | lnp_half_ref | lnp_full_ref | t_ref | h | skap | 1 / dp_ref] on
+  // device; skap_k = (p_full_k / ps)^κ = exp(κ cf_k) on pure sigma (1 otherwise)
+  double* d_ref = nullptr;
   double* d_wave = nullptr;
   double* d_scal = nullptr;         // correction scalars
   double* d_colsum = nullptr;       // column sums [2][nlat][nlon]
