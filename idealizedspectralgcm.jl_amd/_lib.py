@@ -143,6 +143,7 @@ SIGNATURES = {
     "jgcm_hs_physics_column_dynamics": [P, P, ctypes.c_int64] + [P] * 27,
     "jgcm_hs_corrections": [P] * 8,
     "jgcm_hs_corrections_advance": [P, P, P, P, P, P],
+    "jgcm_hs_implicit_leapfrog_advance": [P] * 6 + [ctypes.POINTER(P)] * 4,
     "jgcm_hs_scalars": [P, P, I],
     "jgcm_hs_scalars_device": [P, P, I],
     "jgcm_zonal_mean_accumulate": [P, P, P, P, P, P, I, P, P, P, P, P],

@@ -895,7 +895,7 @@ constexpr int MAXCADV = 16;
 #define JGCM_CADV_PER_THREAD 4  // elements per thread of hs_correct_advance_kernel
 #endif
 struct CorrAdvanceSet {
-  double* p[MAXCADV];
+  double* p[MAXCADV];     // NULL: p already holds the filtered c; only c = n remains
   double* c[MAXCADV];
   double* n[MAXCADV];
   size_t count[MAXCADV];  // doubles
@@ -926,7 +926,7 @@ __global__ __launch_bounds__(APPLY_THREADS) void hs_correct_advance_kernel(
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const size_t i = lo + threadIdx.x + (size_t)it * APPLY_THREADS;
-    xc[it] = i < hi ? c[i] : 0.0;
+    xc[it] = (i < hi && p) ? c[i] : 0.0;
     xn[it] = i < hi ? n[i] : 0.0;
   }
   double sc[6] = {0.0, 0.0, 0.0, 1.0, 0.0, 0.0};
@@ -938,8 +938,7 @@ __global__ __launch_bounds__(APPLY_THREADS) void hs_correct_advance_kernel(
   for (int it = 0; it < IT; ++it) {
     const size_t i = lo + threadIdx.x + (size_t)it * APPLY_THREADS;
     if (i >= hi) break;
-    const double x = xc[it];
-    p[i] = x;
+    if (p) p[i] = xc[it];
     double v = xn[it];
     if (kind == 1 && energy) {
       v = v + corr;
@@ -1149,6 +1148,16 @@ struct LFFuse {
   const double* damping;  // [m][n]
   int on, init_step;
   double dt, r;
+  // advance: the p = c half of the spectral Time_Advance! as well — p
+  // receives the filtered current value instead of c, which keeps the old
+  // value until jgcm_hs_corrections_advance sets c = n (the p arrays are those
+  // read as div_p, t_p, lnps_p and vor_p; each element is read and written by
+  // the same thread)
+  int advance;
+  double2* vor_pw;
+  double2* div_pw;
+  double2* t_pw;
+  double2* lnps_pw;
 };
 
 template <int KPT, int COLS>
@@ -1314,15 +1323,18 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     if (lf.on) {
       double2 cn, nn;
       leapfrog_reg(qd, Dp[i], Dc[i], dmp, true, lf.init_step, lf.dt, lf.r, cn, nn);
-      lf.div_c[o] = cn;
+      if (lf.advance) lf.div_pw[o] = cn;
+      else lf.div_c[o] = cn;
       lf.div_n[o] = nn;
       leapfrog_reg(qt, Tp[i], Tc[i], dmp, true, lf.init_step, lf.dt, lf.r, cn, nn);
-      lf.t_c[o] = cn;
+      if (lf.advance) lf.t_pw[o] = cn;
+      else lf.t_c[o] = cn;
       lf.t_n[o] = nn;
       double2 qv = QV[i];
       leapfrog_reg(qv, VP[i], VCc[i], dmp, true, lf.init_step, lf.dt, lf.r, cn, nn);
       lf.dvor[o] = qv;
-      lf.vor_c[o] = cn;
+      if (lf.advance) lf.vor_pw[o] = cn;
+      else lf.vor_c[o] = cn;
       lf.vor_n[o] = nn;
     }
     ddiv[o] = qd;
@@ -1333,7 +1345,8 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     if (lf.on) {
       double2 cn, nn;
       leapfrog_reg(lq, lnp, lnc, 0.0, false, lf.init_step, lf.dt, lf.r, cn, nn);
-      lf.lnps_c[idx] = cn;
+      if (lf.advance) lf.lnps_pw[idx] = cn;
+      else lf.lnps_c[idx] = cn;
       lf.lnps_n[idx] = nn;
     }
   }
@@ -1910,6 +1923,24 @@ int jgcm_hs_energy_implicit_leapfrog(jgcm_semi_implicit_t* si, double* spe_energ
                              spe_energy, lf);
 }
 
+int jgcm_hs_implicit_leapfrog_advance(jgcm_semi_implicit_t* si, double* spe_energy, double* ddiv,
+                                      double* dlnps, double* dt, double* dvor,
+                                      double* const* vor, double* const* div,
+                                      double* const* lnps, double* const* t) {
+  JGCM_REQUIRE(si && spe_energy && ddiv && dlnps && dt && dvor && vor && div && lnps && t,
+               "NULL argument");
+  for (int k = 0; k < 3; ++k)
+    JGCM_REQUIRE(vor[k] && div[k] && lnps[k] && t[k], "NULL field pointer (time level %d)", k);
+  jgcm_leapfrog_t* L = si->lf;
+  LFFuse lf{(double2*)dvor, (const double2*)vor[0], (double2*)vor[1], (double2*)vor[2],
+            (double2*)div[1], (double2*)div[2], (double2*)t[1], (double2*)t[2], (double2*)lnps[1],
+            (double2*)lnps[2], L->d_damping, 1, L->init_step ? 1 : 0, (double)L->dt,
+            L->robert_coef, 1, (double2*)vor[0], (double2*)div[0], (double2*)t[0],
+            (double2*)lnps[0]};
+  return implicit_correction(si, div[1], div[0], lnps[1], lnps[0], t[1], t[0], ddiv, dlnps, dt,
+                             spe_energy, lf);
+}
+
 int jgcm_hs_energy_implicit_correction(jgcm_semi_implicit_t* si, double* spe_energy,
                                        const double* div_c, const double* div_p,
                                        const double* lnps_c, const double* lnps_p,
@@ -2075,7 +2106,7 @@ int jgcm_hs_corrections_advance(jgcm_semi_implicit_t* si, const double* grid_lnp
   JGCM_REQUIRE(si && grid_lnps && p && c && n && count, "NULL argument");
   jgcm_mesh* M = si->vc->mesh;
   for (int f = 0; f < 8; ++f)
-    JGCM_REQUIRE(p[f] && c[f] && n[f], "NULL field pointer (field %d)", f);
+    JGCM_REQUIRE((p[f] || f < 4) && c[f] && n[f], "NULL field pointer (field %d)", f);
   const size_t P = M->grid_elems(), S = M->spec_elems();
   const int nd = si->nd;
   JGCM_REQUIRE(count[0] == nd * S * 2 && count[1] == nd * S * 2 && count[2] == S * 2 &&

@@ -311,10 +311,12 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
     # :309-316 ∇²(energy), δdiv -= ∇²(energy), Implicit_Correction!, and in the same
     # pass :319-328 Compute_Spectral_Damping! (vor, div, T) and Filtered_Leapfrog!
     # (vor, div, lnps, T)
-    call("jgcm_hs_energy_implicit_leapfrog", si.handle, *map(_ptr, (
-        d.spe_energy, d.spe_div_c, d.spe_div_p, d.spe_lnps_c, d.spe_lnps_p, d.spe_t_c, d.spe_t_p,
-        d.spe_δdiv, d.spe_δlnps, d.spe_δt, d.spe_δvor, d.spe_vor_p, d.spe_vor_c, d.spe_vor_n,
-        d.spe_div_n, d.spe_lnps_n, d.spe_t_n)))
+    # and the spectral part of :346 Time_Advance! (p = the filtered c, c = n)
+    P3 = ctypes.c_void_p * 3
+    lev = lambda f: P3(*(getattr(d, f + s).data_ptr() for s in ("_p", "_c", "_n")))
+    call("jgcm_hs_implicit_leapfrog_advance", si.handle, *map(_ptr, (
+        d.spe_energy, d.spe_δdiv, d.spe_δlnps, d.spe_δt, d.spe_δvor)),
+        lev("spe_vor"), lev("spe_div"), lev("spe_lnps"), lev("spe_t"))
     # :331-335: vor, div, (u, v) = UV_Grid_From_Vor_Div!, T, ln ps back to the grid
     Trans_Spherical_To_Grid_Batch(
         mesh, [d.spe_vor_n, d.spe_div_n, d.spe_vor_n, d.spe_vor_n, d.spe_t_n, d.spe_lnps_n],
@@ -327,6 +329,8 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
         [f"grid_{f}" for f in ("u", "v", "ps", "t")]
     arrs = [[getattr(d, f + s) for f in names] for s in ("_p", "_c", "_n")]
     ptrs = [(ctypes.c_void_p * 8)(*[t.data_ptr() for t in ts]) for ts in arrs]
+    for k in range(4):   # spectral fields advanced above: their (0, 0) corrections only
+        ptrs[0][k] = None
     counts = (ctypes.c_size_t * 8)(*[t.numel() * t.element_size() // 8 for t in arrs[1]])
     call("jgcm_hs_corrections_advance", si.handle, _ptr(d.grid_lnps), *ptrs, counts)
     Time_Advance_Tracers(d)

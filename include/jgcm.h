@@ -654,6 +654,19 @@ int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, doubl
 int jgcm_hs_corrections_advance(jgcm_semi_implicit_t* si, const double* grid_lnps,
                                 double* const* p, double* const* c, double* const* n,
                                 const size_t* count);
+/* No reference counterpart (a fusion): jgcm_hs_energy_implicit_leapfrog with
+ * the p = c half of the spectral Time_Advance! (Dyn_Data.jl:328-340) in the
+ * same pass.  vor, div, lnps, t are {p, c, n} arrays: on return p holds the
+ * filtered current value and n the new one; c still holds the old value.
+ * Pass NULL as the p pointers of these four spectral fields to
+ * jgcm_hs_corrections_advance afterwards: for them it then only sets c = n,
+ * with the (0, 0) corrections of T and ln ps.  The pair equals
+ * jgcm_hs_energy_implicit_leapfrog + jgcm_hs_corrections_advance bit for bit.
+ * Nothing may read the spectral c arrays between the two calls.            */
+int jgcm_hs_implicit_leapfrog_advance(jgcm_semi_implicit_t* si, double* spe_energy, double* ddiv,
+                                      double* dlnps, double* dt, double* dvor,
+                                      double* const* vor, double* const* div,
+                                      double* const* lnps, double* const* t);
 /* device correction scalars to the host: mean_ps_p, mean_energy_p, mean_ps_n,
  * factor, log(factor), mean_energy_n, temperature correction (synchronises) */
 int jgcm_hs_scalars(jgcm_semi_implicit_t* si, double* host_out, int count);

@@ -364,6 +364,59 @@ def test_fused_corrections_advance_equals_corrections_then_time_advance(jgcm):
 
 
 @pytest.mark.parametrize("init_step", [True, False])
+def test_implicit_leapfrog_advance_equals_leapfrog_then_advance(jgcm, init_step):
+    """jgcm_hs_implicit_leapfrog_advance (the spectral Time_Advance! in the
+    semi-implicit pass) followed by jgcm_hs_corrections_advance with NULL p for
+    the spectral fields equals jgcm_hs_energy_implicit_leapfrog followed by the
+    full jgcm_hs_corrections_advance, bit for bit, in every time level, the
+    tendencies and the seven correction scalars."""
+    import ctypes
+    from jgcm_amd._lib import call
+    from jgcm_amd.mesh import _ptr
+    _, (dev, dvc, datmo, dsi, dd) = build(jgcm, 42, 64, 20, 600)
+    jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS)
+    if not init_step:
+        jgcm.Update_Init_Step(dsi)
+        jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS)
+    dev.synchronize()
+    names = [f"spe_{f}" for f in ("vor", "div", "lnps", "t")] + \
+        [f"grid_{f}" for f in ("u", "v", "ps", "t")]
+    fields = [f + s for f in names for s in ("_p", "_c", "_n")] + \
+        ["grid_lnps", "spe_energy", "spe_δdiv", "spe_δlnps", "spe_δt", "spe_δvor"]
+    start = {f: getattr(dd, f).clone() for f in fields}
+    results = []
+    for split in (False, True):
+        for f in fields:
+            getattr(dd, f).copy_(start[f])
+        torch.cuda.synchronize()
+        arrs = [[getattr(dd, f + s) for f in names] for s in ("_p", "_c", "_n")]
+        ptrs = [(ctypes.c_void_p * 8)(*[t.data_ptr() for t in ts]) for ts in arrs]
+        counts = (ctypes.c_size_t * 8)(*[t.numel() * t.element_size() // 8 for t in arrs[1]])
+        if split:
+            P3 = ctypes.c_void_p * 3
+            lev = lambda f: P3(*(getattr(dd, f + s).data_ptr() for s in ("_p", "_c", "_n")))
+            call("jgcm_hs_implicit_leapfrog_advance", dsi.handle, *map(_ptr, (
+                dd.spe_energy, dd.spe_δdiv, dd.spe_δlnps, dd.spe_δt, dd.spe_δvor)),
+                lev("spe_vor"), lev("spe_div"), lev("spe_lnps"), lev("spe_t"))
+            for k in range(4):
+                ptrs[0][k] = None
+        else:
+            call("jgcm_hs_energy_implicit_leapfrog", dsi.handle, *map(_ptr, (
+                dd.spe_energy, dd.spe_div_c, dd.spe_div_p, dd.spe_lnps_c, dd.spe_lnps_p,
+                dd.spe_t_c, dd.spe_t_p, dd.spe_δdiv, dd.spe_δlnps, dd.spe_δt, dd.spe_δvor,
+                dd.spe_vor_p, dd.spe_vor_c, dd.spe_vor_n, dd.spe_div_n, dd.spe_lnps_n,
+                dd.spe_t_n)))
+        call("jgcm_hs_corrections_advance", dsi.handle, _ptr(dd.grid_lnps), *ptrs, counts)
+        dev.synchronize()
+        results.append(({f: getattr(dd, f).cpu().numpy() for f in fields}, dsi.scalars()))
+    (a, sa), (b, sb) = results
+    for f in fields:
+        assert np.array_equal(a[f], b[f]), f
+    assert np.array_equal(np.asarray(sa), np.asarray(sb))
+    assert not np.array_equal(a["spe_t_c"], start["spe_t_c"].cpu().numpy())
+
+
+@pytest.mark.parametrize("init_step", [True, False])
 def test_fused_implicit_leapfrog_equals_separate_calls(jgcm, init_step):
     """jgcm_hs_energy_implicit_leapfrog equals jgcm_hs_energy_implicit_correction
     followed by the damped leapfrog of vor, div, T (damped) and ln ps (not) bit
