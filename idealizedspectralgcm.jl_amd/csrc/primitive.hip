@@ -277,23 +277,33 @@ __device__ __forceinline__ void store_pressure(const VC& v, const TileIdx& t, si
 // Rd T (lnp_half[k'+1] - lnp_half[k']), g_full[k] = g_half[k+1] + Rd T
 // (lnp_half[k+1] - lnp_full[k]); with zero_top, g_half[0] = 0.  Results are
 // divided by `grav_div` when it is non-zero (Compute_Pressures_And_Heights!).
+// The first half: the tile's hydrostatic increments Rd T (lnp_half[k+1] -
+// lnp_half[k]) and their sum (the caller sums them over the tile's levels
+// below, with level_sums).
 template <int KPT>
-__device__ __forceinline__ void geopotential_tile(const VC& v, const TileIdx& t, double2* red,
-                                                  double rdgas, const double (&T)[KPT],
-                                                  const double (&lnh)[KPT + 1],
-                                                  const double (&lnf)[KPT], double geopots,
-                                                  double (&gfull)[KPT], double (&ghalf)[KPT + 1],
-                                                  double grav_div) {
-  double term[KPT];
+__device__ __forceinline__ double geopotential_terms(const VC& v, const TileIdx& t, double rdgas,
+                                                     const double (&T)[KPT],
+                                                     const double (&lnh)[KPT + 1],
+                                                     double (&term)[KPT]) {
   double part = 0.0;
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
     term[i] = (t.k0 + i < v.nd) ? rdgas * T[i] * (lnh[i + 1] - lnh[i]) : 0.0;
     part += term[i];
   }
-  double2 above, below;
-  level_sums(red, t, d2(part, 0.0), above, below);
-  double acc = geopots + below.x;  // g_half at level k0 + KPT (or nd)
+  return part;
+}
+
+// The second half, given the sum of the increments of the levels below the tile.
+template <int KPT>
+__device__ __forceinline__ void geopotential_finish(const VC& v, const TileIdx& t, double rdgas,
+                                                    const double (&T)[KPT],
+                                                    const double (&lnh)[KPT + 1],
+                                                    const double (&lnf)[KPT], double geopots,
+                                                    const double (&term)[KPT], double below,
+                                                    double (&gfull)[KPT],
+                                                    double (&ghalf)[KPT + 1], double grav_div) {
+  double acc = geopots + below;  // g_half at level k0 + KPT (or nd)
   ghalf[KPT] = acc;
 #pragma unroll
   for (int i = KPT - 1; i >= 0; --i) {
@@ -307,6 +317,21 @@ __device__ __forceinline__ void geopotential_tile(const VC& v, const TileIdx& t,
 #pragma unroll
     for (int i = 0; i <= KPT; ++i) ghalf[i] = ghalf[i] / grav_div;
   }
+}
+
+template <int KPT>
+__device__ __forceinline__ void geopotential_tile(const VC& v, const TileIdx& t, double2* red,
+                                                  double rdgas, const double (&T)[KPT],
+                                                  const double (&lnh)[KPT + 1],
+                                                  const double (&lnf)[KPT], double geopots,
+                                                  double (&gfull)[KPT], double (&ghalf)[KPT + 1],
+                                                  double grav_div) {
+  double term[KPT];
+  const double part = geopotential_terms<KPT>(v, t, rdgas, T, lnh, term);
+  double2 above, below;
+  level_sums(red, t, d2(part, 0.0), above, below);
+  geopotential_finish<KPT>(v, t, rdgas, T, lnh, lnf, geopots, term, below.x, gfull, ghalf,
+                           grav_div);
 }
 
 // One workgroup's contribution to an area-weighted global mean: sum over its
@@ -510,7 +535,7 @@ struct PhysArgs {
 // correction partials) runs first in the same tile, its tendencies kept in
 // registers instead of going through du, dv, dtt and dps (requires OWN_P).
 template <int KPT, bool OWN_P, bool PHYS>
-__device__ __forceinline__ void hs_column_dynamics_body(double2* red, const PhysArgs& pa,
+__device__ __forceinline__ void hs_column_dynamics_body(double* red, const PhysArgs& pa,
                                                         JGCM_COLUMN_DYNAMICS_ARGS) {
   static_assert(OWN_P || !PHYS, "the fused physics pass evaluates the pressures itself");
   const size_t P = (size_t)nlat * nlon;
@@ -574,6 +599,10 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
     }
     VOR[i] = ok ? vor[o] : 0.0;
   }
+  // the column energy of the physics pass (Compute_Corrections_Init), summed
+  // over the levels with the other two level sums below
+  double epart = 0.0;
+  double psp = 0.0, wj = 0.0;
   if constexpr (PHYS) {
     // Spectral_Dynamics_Physics! (SD:452-467): HS_Forcing! (HS_Forcing.jl:1-130)
     // on the p-fields with this column's pressures, the same arithmetic as
@@ -587,8 +616,8 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
       Vp[i] = ok ? pa.v_p[o] : 0.0;
       Tp[i] = ok ? pa.t_p[o] : 0.0;
     }
-    const double psp = (tx.col && (pa.mass || pa.energy)) ? pa.ps_p[c] : 0.0;
-    const double wj = (tx.col && tx.q == 0 && (pa.mass || pa.energy)) ? pa.wts[jlat] : 0.0;
+    psp = (tx.col && (pa.mass || pa.energy)) ? pa.ps_p[c] : 0.0;
+    wj = (tx.col && tx.q == 0 && (pa.mass || pa.energy)) ? pa.wts[jlat] : 0.0;
     const HSCol h = hs_column_constants(v, pa.hp, pa.skap, pa.kappa, pa.dt, p,
                                         sint[jlat]);
     const double dt = pa.dt;
@@ -602,22 +631,16 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
       if (ok) pa.t_eq[(size_t)(k0 + i) * P + c] = teq;
     }
     if (isbad) atomicAdd(pa.bad, 1);
-    if (pa.mass || pa.energy) {
-      double part = 0.0;
-      if (pa.energy) {
+    if (pa.energy) {
 #pragma unroll
-        for (int i = 0; i < KPT; ++i) {
-          int k = k0 + i;
-          if (tx.col && k < nd) {
-            double ea = Up[i] + dt * DU[i], eb = Vp[i] + dt * DV[i];
-            double e = 0.5 * (ea * ea + eb * eb) + pa.cp_air * (Tp[i] + dt * DT[i]);
-            part = part + e * (v.dak[k] + v.dbk[k] * psp);
-          }
+      for (int i = 0; i < KPT; ++i) {
+        int k = k0 + i;
+        if (tx.col && k < nd) {
+          double ea = Up[i] + dt * DU[i], eb = Vp[i] + dt * DV[i];
+          double e = 0.5 * (ea * ea + eb * eb) + pa.cp_air * (Tp[i] + dt * DT[i]);
+          epart = epart + e * (v.dak[k] + v.dbk[k] * psp);
         }
       }
-      double2 ab, bl;
-      level_sums(red, tx, d2(part, 0.0), ab, bl);
-      block_area_partial(tx, wj, psp, part + bl.x, pa.partial);
     }
   }
 
@@ -645,13 +668,26 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
     dmean[i] = ok ? DIV[i] * Dp + v.dbk[kk] * (U[i + 1] * a + V[i + 1] * b) : 0.0;
     part = part + dmean[i];
   }
-  double2 above2, below2;
-  level_sums(red, tx, d2(part, 0.0), above2, below2);
-  const double total = above2.x + part + below2.x;
+  // the hydrostatic increments of the geopotential
+  double Tk[KPT], LHk[KPT + 1], gterm[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) Tk[i] = T[i + 1];
+#pragma unroll
+  for (int i = 0; i <= KPT; ++i) LHk[i] = LH[i];
+  const double gpart = geopotential_terms<KPT>(v, tx, rdgas, Tk, LHk, gterm);
+  // the three sums over the levels (column energy, mass-flux divergence,
+  // geopotential) in one exchange: two barriers instead of six
+  const double parts[3] = {epart, part, gpart};
+  double sa[3], sb[3];
+  level_sums_n<3>(red, tx, parts, sa, sb);
+  if constexpr (PHYS) {
+    if (pa.mass || pa.energy) block_area_partial(tx, wj, psp, epart + sb[0], pa.partial);
+  }
+  const double total = sa[1] + part + sb[1];
   // M_half at half levels k0 .. k0+KPT
   double Mh[KPT + 1];
-  Mh[0] = (k0 == 0 || k0 >= nd) ? 0.0 : -above2.x + total * v.bk[k0];
-  double run = above2.x;
+  Mh[0] = (k0 == 0 || k0 >= nd) ? 0.0 : -sa[1] + total * v.bk[k0];
+  double run = sa[1];
   double W[KPT];
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
@@ -674,12 +710,8 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
   }
 
   // geopotential
-  double Tk[KPT], LHk[KPT + 1], gfull[KPT], ghalf[KPT + 1];
-#pragma unroll
-  for (int i = 0; i < KPT; ++i) Tk[i] = T[i + 1];
-#pragma unroll
-  for (int i = 0; i <= KPT; ++i) LHk[i] = LH[i];
-  geopotential_tile<KPT>(v, tx, red, rdgas, Tk, LHk, LF, gs, gfull, ghalf, 0.0);
+  double gfull[KPT], ghalf[KPT + 1];
+  geopotential_finish<KPT>(v, tx, rdgas, Tk, LHk, LF, gs, gterm, sb[2], gfull, ghalf, 0.0);
 
   // vertical advection of u, v, T with w = M_half, dz = Δp: fluxes through
   // the half levels k0+i (i = 0 .. KPT), between R[i] and R[i+1], with the
@@ -730,20 +762,20 @@ __device__ __forceinline__ void hs_column_dynamics_body(double2* red, const Phys
 
 template <int KPT>
 __global__ __launch_bounds__(512) void hs_column_dynamics_kernel(JGCM_COLUMN_DYNAMICS_ARGS) {
-  __shared__ double2 red[MAXNQ * TILE];
+  __shared__ double red[3 * MAXNQ * TILE];  // three level sums
   hs_column_dynamics_body<KPT, false, false>(red, PhysArgs{}, JGCM_COLUMN_DYNAMICS_PASS);
 }
 
 template <int KPT>
 __global__ __launch_bounds__(512) void hs_column_dynamics_own_p_kernel(JGCM_COLUMN_DYNAMICS_ARGS) {
-  __shared__ double2 red[MAXNQ * TILE];
+  __shared__ double red[3 * MAXNQ * TILE];  // three level sums
   hs_column_dynamics_body<KPT, true, false>(red, PhysArgs{}, JGCM_COLUMN_DYNAMICS_PASS);
 }
 
 template <int KPT>
 __global__ __launch_bounds__(512) void hs_physics_column_dynamics_kernel(
     PhysArgs pa, JGCM_COLUMN_DYNAMICS_ARGS) {
-  __shared__ double2 red[MAXNQ * TILE];
+  __shared__ double red[3 * MAXNQ * TILE];  // three level sums
   hs_column_dynamics_body<KPT, true, true>(red, pa, JGCM_COLUMN_DYNAMICS_PASS);
 }
 
