@@ -39,7 +39,7 @@ def trans_spherical_to_grid_adjoint(mesh, pfield, snm):
 
 
 def trans_grid_to_spherical_adjoint_batch(mesh, snms, pfields):
-    """trans_grid_to_spherical_adjoint of up to 8 arrays in two launches."""
+    """trans_grid_to_spherical_adjoint of up to 12 arrays in two launches."""
     k = len(snms)
     assert len(pfields) == k and 1 <= k <= 8
     nlev = []
@@ -54,7 +54,7 @@ def trans_grid_to_spherical_adjoint_batch(mesh, snms, pfields):
 
 
 def trans_spherical_to_grid_adjoint_batch(mesh, pfields, snms, divcos=None, accumulate=False):
-    """trans_spherical_to_grid_adjoint of up to 8 arrays in two launches;
+    """trans_spherical_to_grid_adjoint of up to 12 arrays in two launches;
     ``divcos[k]``: array k is divided by cos θ first; ``accumulate``: the
     results are added to ``snms`` (which must not overlap)."""
     k = len(pfields)

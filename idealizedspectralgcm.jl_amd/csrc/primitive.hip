@@ -515,6 +515,10 @@ struct PhysArgs {
   int mass, energy;
   int* bad;
   const double* skap;
+  // ∂λ ln ps, ∂θ ln ps from these arrays instead of dlps / dtps (which then
+  // only receive ∇ps); NULL: read from dlps / dtps
+  const double* dl_lnps;
+  const double* dt_lnps;
 };
 
 #define JGCM_COLUMN_DYNAMICS_ARGS                                                            \
@@ -545,8 +549,16 @@ __device__ __forceinline__ void hs_column_dynamics_body(double* red, const PhysA
   const size_t c = tx.c;
   const double kappa = rdgas / cp_air;
   const double p = tx.col ? ps[c] : 1.0;
-  const double a = tx.col ? dlps[c] * p : 0.0;
-  const double b = tx.col ? dtps[c] * p : 0.0;
+  const double* dl_in = dlps;
+  const double* dt_in = dtps;
+  if constexpr (PHYS) {
+    if (pa.dl_lnps) {
+      dl_in = pa.dl_lnps;
+      dt_in = pa.dt_lnps;
+    }
+  }
+  const double a = tx.col ? dl_in[c] * p : 0.0;
+  const double b = tx.col ? dt_in[c] * p : 0.0;
   const double ps_tend = (!PHYS && tx.col) ? dps[c] : 0.0;   // the physics pass sets δps = 0
   const double gs = tx.col ? geopots[c] : 0.0;
   const int jlat = tx.col ? lat_of(c, nlon) : 0;
@@ -2062,7 +2074,7 @@ int jgcm_hs_column_dynamics(jgcm_semi_implicit_t* si, const double* grid_div, co
 
 // Spectral_Dynamics_Physics! and the column part of Spectral_Dynamics! in one
 // pass (the physics tendencies stay in registers); see jgcm.h
-int jgcm_hs_physics_column_dynamics(
+static int physics_column_dynamics(
     jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t dt, const double* grid_u_p,
     const double* grid_v_p, const double* grid_t_p, const double* grid_ps_p, double* grid_t_eq,
     const double* grid_div, const double* grid_u, const double* grid_v, const double* grid_ps,
@@ -2070,7 +2082,7 @@ int jgcm_hs_physics_column_dynamics(
     const double* grid_geopots, double* grid_M_half, double* grid_w_full, double* grid_du,
     double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
     double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
-    const double* grid_dtheta_t) {
+    const double* grid_dtheta_t, const double* dlambda_lnps, const double* dtheta_lnps) {
   JGCM_REQUIRE(si && hp, "NULL argument");
   JGCM_REQUIRE(grid_u_p && grid_v_p && grid_t_p && grid_ps_p && grid_t_eq && grid_div && grid_u &&
                    grid_v && grid_ps && grid_dlambda_ps && grid_dtheta_ps && grid_t && grid_vor &&
@@ -2087,7 +2099,7 @@ int jgcm_hs_physics_column_dynamics(
               hs_params(hp, si->atmo.cp_air),
               grid_u_p, grid_v_p, grid_t_p, grid_ps_p, grid_t_eq, area_partials(si),
               si->atmo.do_mass_correction, si->atmo.do_energy_correction,
-              reinterpret_cast<int*>(si->d_scal + 15), skap_of(si)};
+              reinterpret_cast<int*>(si->d_scal + 15), skap_of(si), dlambda_lnps, dtheta_lnps};
   const double* none = nullptr;
   JGCM_TILE_DISPATCH(nd, hs_physics_column_dynamics_kernel, tile_grid(P), tile_block(nd), 0,
                      M->stream, pa, vc_of(si->vc), M->nlat, M->nlon, si->atmo.rdgas,
@@ -2098,6 +2110,41 @@ int jgcm_hs_physics_column_dynamics(
                      grid_dlambda_t, grid_dtheta_t);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
+}
+
+int jgcm_hs_physics_column_dynamics(
+    jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t dt, const double* grid_u_p,
+    const double* grid_v_p, const double* grid_t_p, const double* grid_ps_p, double* grid_t_eq,
+    const double* grid_div, const double* grid_u, const double* grid_v, const double* grid_ps,
+    double* grid_dlambda_ps, double* grid_dtheta_ps, const double* grid_t, const double* grid_vor,
+    const double* grid_geopots, double* grid_M_half, double* grid_w_full, double* grid_du,
+    double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
+    double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
+    const double* grid_dtheta_t) {
+  return physics_column_dynamics(si, hp, dt, grid_u_p, grid_v_p, grid_t_p, grid_ps_p, grid_t_eq,
+                                 grid_div, grid_u, grid_v, grid_ps, grid_dlambda_ps, grid_dtheta_ps,
+                                 grid_t, grid_vor, grid_geopots, grid_M_half, grid_w_full, grid_du,
+                                 grid_dv, grid_dps, grid_dt, geopot_half, geopot_full, grid_dlnps,
+                                 grid_absvor, grid_energy, grid_dlambda_t, grid_dtheta_t, nullptr,
+                                 nullptr);
+}
+
+int jgcm_hs_physics_column_dynamics_carried(
+    jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t dt, const double* grid_u_p,
+    const double* grid_v_p, const double* grid_t_p, const double* grid_ps_p, double* grid_t_eq,
+    const double* grid_div, const double* grid_u, const double* grid_v, const double* grid_ps,
+    double* grid_dlambda_ps, double* grid_dtheta_ps, const double* grid_t, const double* grid_vor,
+    const double* grid_geopots, double* grid_M_half, double* grid_w_full, double* grid_du,
+    double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
+    double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
+    const double* grid_dtheta_t, const double* dlambda_lnps, const double* dtheta_lnps) {
+  JGCM_REQUIRE(dlambda_lnps && dtheta_lnps, "NULL ln ps gradient");
+  return physics_column_dynamics(si, hp, dt, grid_u_p, grid_v_p, grid_t_p, grid_ps_p, grid_t_eq,
+                                 grid_div, grid_u, grid_v, grid_ps, grid_dlambda_ps, grid_dtheta_ps,
+                                 grid_t, grid_vor, grid_geopots, grid_M_half, grid_w_full, grid_du,
+                                 grid_dv, grid_dps, grid_dt, geopot_half, geopot_full, grid_dlnps,
+                                 grid_absvor, grid_energy, grid_dlambda_t, grid_dtheta_t,
+                                 dlambda_lnps, dtheta_lnps);
 }
 
 // Compute_Corrections! (SD:30-68) after the back-transforms, with ps_n = exp(lnps) (SD:336)

@@ -44,7 +44,7 @@ enum { P_FP64 = 0, P_FP32 = 1, P_SPLIT = 2, P_BF16 = 3 };
 // multiply (forward FFT) or divide (inverse FFT) for that segment; for the
 // inverse Legendre stage bits 1..3 name a spectral operator applied while the
 // coefficients are staged (SPEC_OP_*), ptr2[k] being its second operand.
-constexpr int MAXSEG = 8;
+constexpr int MAXSEG = 12;
 struct SegMap {
   int nseg;
   int start[MAXSEG + 1];

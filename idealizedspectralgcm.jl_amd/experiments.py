@@ -156,10 +156,13 @@ def held_suarez_climate(N=42, nθ=64, nd=20, dt=600, days=200, average_from=100,
     noise = torch.from_numpy(0.1 * np.random.default_rng(seed).standard_normal((nd, nθ, nλ)))
     Spectral_Initialize_Fields(mesh, atmo, vc, 1.0e5, 300.0, d.grid_geopots, d,
                                grid_t_noise=noise.to(mesh.device))
-    Atmosphere_Update(mesh, atmo, vc, si, d, HS_FORCING)
+    # every step carries the next step's gradient transforms (the driver reads
+    # the state between days but never changes it)
+    Atmosphere_Update(mesh, atmo, vc, si, d, HS_FORCING, carry_gradients=True)
     Update_Init_Step(si)
-    g = Step_Graph(mesh, lambda: Atmosphere_Update(mesh, atmo, vc, si, d, HS_FORCING),
-                   nfields=5 * nd + 1)
+    g = Step_Graph(mesh, lambda: Atmosphere_Update(mesh, atmo, vc, si, d, HS_FORCING,
+                                                   carry_gradients=True),
+                   nfields=8 * nd + 4)
     per_day = 86400 // dt
     um = torch.zeros(nd, nθ, dtype=torch.float64, device=mesh.device)
     tm = torch.zeros_like(um)

@@ -236,7 +236,7 @@ def _ptrs(ts):
 
 def _batch_args(ts_a, ts_b, flags):
     n = len(ts_a)
-    assert 1 <= n <= 8 and len(ts_b) == n, "batched transforms take 1..8 arrays"
+    assert 1 <= n <= 12 and len(ts_b) == n, "batched transforms take 1..12 arrays"
     nl = (ctypes.c_int * n)(*[t.shape[0] for t in ts_a])
     fl = (ctypes.c_int * n)(*(flags or [0] * n))
     return n, _ptrs(ts_a), _ptrs(ts_b), nl, fl

@@ -239,11 +239,13 @@ def Spectral_Dynamics_Physics(atmo_data, mesh, dyn_data, Δt, physics_params, se
 
 class _StepScratch:
     """Device temporaries of one HS step that the reference keeps inside its
-    mesh helpers: the grid gradients of T used by Add_Horizontal_Advection!."""
+    mesh helpers: the grid gradients of T used by Add_Horizontal_Advection!,
+    and the gradients of ln ps a step carries to the next one."""
 
     def __init__(self, mesh, nd):
-        g = mesh.grid_zeros(2 * nd)
+        g = mesh.grid_zeros(2 * nd + 2)
         self.grid_dl_t, self.grid_dt_t = g[0:nd], g[nd:2 * nd]
+        self.carry_dl_lnps, self.carry_dt_lnps = g[2 * nd:2 * nd + 1], g[2 * nd + 1:]
 
 
 def _scratch(mesh, d):
@@ -254,7 +256,7 @@ def _scratch(mesh, d):
 
 
 def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
-                      pressure_variables=True, physics_params=None):
+                      pressure_variables=True, physics_params=None, carry_gradients=False):
     """Spectral_Dynamics.jl:206-365.  The transforms of the step are batched:
     the gradients of ln ps and T (:259, :293) go back to the grid in one
     launch pair with the spectral gradient operators applied on the way in,
@@ -268,13 +270,26 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
     when the caller recomputes the same fields next (Atmosphere_Update!).
     ``physics_params`` (Atmosphere_Update!) runs Spectral_Dynamics_Physics!
     (:452-467) inside the step's column pass, its tendencies kept in
-    registers (jgcm_hs_physics_column_dynamics)."""
+    registers (jgcm_hs_physics_column_dynamics).
+
+    ``carry_gradients`` (with ``physics_params``): the closing inverse batch
+    also forms the next step's gradients of ln ps and T (:259, :293) from the
+    new state, and the next step that also carries uses them instead of
+    transforming its current state again (one launch pair fewer per step).
+    The current state differs from the new one only in the (0, 0)
+    coefficients that Compute_Corrections! adjusts, which the gradient
+    operators multiply by exact zeros, so the results are the same bit for
+    bit.  Every step of a run must then carry, and nothing may change the
+    state between the steps."""
     d, si = dyn_data, semi_implicit
     integrator = si.integrator
     Δt = Get_Δt(integrator)
     sc = _scratch(mesh, d)
     h = mesh.handle
     fused = physics_params is not None
+    assert fused or not carry_gradients, "carried gradients need the fused physics pass"
+    carried = carry_gradients and getattr(d, "_carried_gradients", False)
+    d._carried_gradients = False
     if not fused and not si._corrections_ready:
         # Compute_Corrections_Init + Pressure_Variables! without forcing (:250-256)
         _physics(si, d, _ZERO_HS, Δt, False)
@@ -283,11 +298,13 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
     # enqueues it there): the gradient batch below does not depend on it
     call("jgcm_mesh_branch", h, 0)
     # :259 Compute_Gradients!(lnps) and the gradient half of :293 Add_Horizontal_Advection!(T)
-    Trans_Spherical_To_Grid_Batch(
-        mesh, [d.spe_lnps_c, d.spe_lnps_c, d.spe_t_c, d.spe_t_c],
-        [d.grid_dλ_ps, d.grid_dθ_ps, sc.grid_dl_t, sc.grid_dt_t],
-        [BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA,
-         BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA])
+    # (carried: formed by the previous step's closing batch)
+    if not carried:
+        Trans_Spherical_To_Grid_Batch(
+            mesh, [d.spe_lnps_c, d.spe_lnps_c, d.spe_t_c, d.spe_t_c],
+            [d.grid_dλ_ps, d.grid_dθ_ps, sc.grid_dl_t, sc.grid_dt_t],
+            [BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA,
+             BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA])
     call("jgcm_mesh_join", h)                  # the column pass needs both
     # :256 Pressure_Variables!(ps_c) is evaluated inside the column pass (NULL fields)
     P = lambda t: None if t is None else _ptr(t)
@@ -295,10 +312,16 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
             d.grid_δt, d.grid_geopot_half, d.grid_geopot_full, d.grid_δlnps, d.grid_absvor,
             d.grid_energy_full, sc.grid_dl_t, sc.grid_dt_t)
     if fused:                                                                  # :452-467, :260-307
-        call("jgcm_hs_physics_column_dynamics", si.handle, ctypes.byref(_hs(physics_params)),
-             int(Δt), *map(P, (d.grid_u_p, d.grid_v_p, d.grid_t_p, d.grid_ps_p, d.grid_t_eq,
-                               d.grid_div, d.grid_u_c, d.grid_v_c, d.grid_ps_c, d.grid_dλ_ps,
-                               d.grid_dθ_ps, d.grid_t_c, d.grid_vor) + cols))
+        args = (d.grid_u_p, d.grid_v_p, d.grid_t_p, d.grid_ps_p, d.grid_t_eq, d.grid_div,
+                d.grid_u_c, d.grid_v_c, d.grid_ps_c, d.grid_dλ_ps, d.grid_dθ_ps, d.grid_t_c,
+                d.grid_vor) + cols
+        if carried:
+            call("jgcm_hs_physics_column_dynamics_carried", si.handle,
+                 ctypes.byref(_hs(physics_params)), int(Δt),
+                 *map(P, args + (sc.carry_dl_lnps, sc.carry_dt_lnps)))
+        else:
+            call("jgcm_hs_physics_column_dynamics", si.handle, ctypes.byref(_hs(physics_params)),
+                 int(Δt), *map(P, args))
     else:
         call("jgcm_hs_column_dynamics", si.handle, *map(P, (                   # :260-307
             d.grid_div, d.grid_u_c, d.grid_v_c, d.grid_ps_c, None, None,
@@ -318,11 +341,18 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
         d.spe_energy, d.spe_δdiv, d.spe_δlnps, d.spe_δt, d.spe_δvor)),
         lev("spe_vor"), lev("spe_div"), lev("spe_lnps"), lev("spe_t"))
     # :331-335: vor, div, (u, v) = UV_Grid_From_Vor_Div!, T, ln ps back to the grid
-    Trans_Spherical_To_Grid_Batch(
-        mesh, [d.spe_vor_n, d.spe_div_n, d.spe_vor_n, d.spe_vor_n, d.spe_t_n, d.spe_lnps_n],
-        [d.grid_vor, d.grid_div, d.grid_u_n, d.grid_v_n, d.grid_t_n, d.grid_lnps],
-        [0, 0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS, 0, 0],
-        snms_b=[None, None, d.spe_div_n, d.spe_div_n, None, None])
+    # (carrying: with the next step's gradients of ln ps and T)
+    snms = [d.spe_vor_n, d.spe_div_n, d.spe_vor_n, d.spe_vor_n, d.spe_t_n, d.spe_lnps_n]
+    grids = [d.grid_vor, d.grid_div, d.grid_u_n, d.grid_v_n, d.grid_t_n, d.grid_lnps]
+    flags = [0, 0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS, 0, 0]
+    if carry_gradients:
+        snms += [d.spe_lnps_n, d.spe_lnps_n, d.spe_t_n, d.spe_t_n]
+        grids += [sc.carry_dl_lnps, sc.carry_dt_lnps, sc.grid_dl_t, sc.grid_dt_t]
+        flags += [BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA,
+                  BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA]
+    Trans_Spherical_To_Grid_Batch(mesh, snms, grids, flags,
+                                  snms_b=[None, None, d.spe_div_n, d.spe_div_n] +
+                                  [None] * (len(snms) - 4))
     # :336-343 Compute_Corrections! and :346 Time_Advance!, the corrections applied
     # to the new fields while they are copied (one launch after the sums)
     names = [f"spe_{f}" for f in ("vor", "div", "lnps", "t")] + \
@@ -334,13 +364,14 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
     counts = (ctypes.c_size_t * 8)(*[t.numel() * t.element_size() // 8 for t in arrs[1]])
     call("jgcm_hs_corrections_advance", si.handle, _ptr(d.grid_lnps), *ptrs, counts)
     Time_Advance_Tracers(d)
+    d._carried_gradients = carry_gradients
     if pressure_variables:
         Pressure_Variables(vert_coord, d.grid_ps_c, d.grid_p_half, d.grid_Δp, d.grid_lnp_half,
                            d.grid_p_full, d.grid_lnp_full)                        # :355
 
 
 def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, physics_params,
-                      overlap=False, fused=True, eager_heights=False):
+                      overlap=False, fused=True, eager_heights=False, carry_gradients=False):
     """Spectral_Dynamics.jl:470-490.  By default (``fused``) the physics pass
     (HS forcing on the p-fields, the correction partials) runs inside the
     step's column pass, after the gradient transforms, its tendencies kept in
@@ -352,11 +383,16 @@ def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, phys
     join cost more than the pass).  The closing Compute_Pressures_And_Heights!
     (:485) is deferred to the first read of its outputs (Dyn_Data.HEIGHT_FIELDS;
     nothing in the next step reads them); ``eager_heights`` runs it here, in
-    the reference's order."""
+    the reference's order.  ``carry_gradients`` (fused only): each step's
+    closing inverse batch also forms the next step's gradients of ln ps and T
+    (Spectral_Dynamics); for runs whose steps all carry and that do not touch
+    the state between steps (driver loops, captured multi-step graphs)."""
     Δt = Get_Δt(semi_implicit.integrator)
+    assert not carry_gradients or (fused and not overlap), "carried gradients need the fused step"
     if fused and not overlap:
         Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
-                          pressure_variables=False, physics_params=physics_params)
+                          pressure_variables=False, physics_params=physics_params,
+                          carry_gradients=carry_gradients)
     else:
         if overlap:
             call("jgcm_mesh_branch", mesh.handle, 1)
@@ -386,6 +422,7 @@ def Spectral_Initialize_Fields(mesh, atmo_data, vert_coord, sea_level_ps_ref, in
     """Spectral_Dynamics.jl:372-449.  ``grid_t_noise`` (optional device grid
     field) is added to the isothermal temperature before the spectral
     filtering (the seeded-noise Held-Suarez benchmark state)."""
+    dyn_data._carried_gradients = False   # a new state: nothing carried
     d = dyn_data
     nd = mesh.nd
     G = d.grid_t_c.numel()

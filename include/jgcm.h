@@ -204,7 +204,7 @@ int jgcm_trans_grid_to_fourier(jgcm_mesh_t* mesh, const double* pfield,
                                double* fourier_g, int nlev);
 
 /* Batched forms of the two transforms (no reference counterpart: the
- * reference transforms one array per call).  nseg (1..8) arrays of nlev[k]
+ * reference transforms one array per call).  nseg (1..12) arrays of nlev[k]
  * levels go through ONE Legendre launch and ONE FFT launch, which is how a
  * time step pushes all its level-fields through the transform at once.
  *   inverse: flags[k] & JGCM_BATCH_DIVCOS divides the output rows of array k
@@ -245,14 +245,14 @@ int jgcm_trans_grid_to_spherical_batch(jgcm_mesh_t* mesh, int nseg, const double
  * written, lower triangle set to zero).                                     */
 int jgcm_trans_grid_to_spherical_adjoint(jgcm_mesh_t* mesh, const double* snm, double* pfield,
                                          int nlev);
-/* grid_to_spherical_adjoint for up to 8 arrays in one Legendre and one FFT
+/* grid_to_spherical_adjoint for up to 12 arrays in one Legendre and one FFT
  * launch (no reference counterpart: the adjoint of the forward batch).     */
 int jgcm_trans_grid_to_spherical_adjoint_batch(jgcm_mesh_t* mesh, int nseg,
                                                const double* const* snm, double* const* pfield,
                                                const int* nlev);
 int jgcm_trans_spherical_to_grid_adjoint(jgcm_mesh_t* mesh, const double* pfield, double* snm,
                                          int nlev);
-/* spherical_to_grid_adjoint for up to 8 arrays in one FFT and one Legendre
+/* spherical_to_grid_adjoint for up to 12 arrays in one FFT and one Legendre
  * launch (no reference counterpart: the adjoint of the inverse batch).
  * flags[k] (or NULL): JGCM_BATCH_DIVCOS divides array k by cos θ on the way
  * in.  accumulate = 0: snm[k] overwritten (lower triangle zeroed);
@@ -640,6 +640,21 @@ int jgcm_hs_physics_column_dynamics(
     double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
     double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
     const double* grid_dtheta_t);
+/* No reference counterpart: jgcm_hs_physics_column_dynamics with ∂λ ln ps
+ * and ∂θ ln ps read from dlambda_lnps / dtheta_lnps (the gradients of the
+ * step's current ln ps, carried from the previous step's closing inverse
+ * batch) instead of from grid_dlambda_ps / grid_dtheta_ps, which then only
+ * receive ∇ps (SD:260-261).  Equal to it bit for bit when the given arrays
+ * hold what grid_dlambda_ps / grid_dtheta_ps would.                         */
+int jgcm_hs_physics_column_dynamics_carried(
+    jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t dt, const double* grid_u_p,
+    const double* grid_v_p, const double* grid_t_p, const double* grid_ps_p, double* grid_t_eq,
+    const double* grid_div, const double* grid_u, const double* grid_v, const double* grid_ps,
+    double* grid_dlambda_ps, double* grid_dtheta_ps, const double* grid_t, const double* grid_vor,
+    const double* grid_geopots, double* grid_M_half, double* grid_w_full, double* grid_du,
+    double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
+    double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
+    const double* grid_dtheta_t, const double* dlambda_lnps, const double* dtheta_lnps);
 /* ps_n = exp(lnps) then Compute_Corrections!  Spectral_Dynamics.jl:336-343, :30-68 */
 int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, double* grid_ps_n,
                         const double* grid_u_n, const double* grid_v_n, double* grid_t_n,

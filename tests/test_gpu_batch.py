@@ -105,8 +105,8 @@ def test_batch_argument_errors(jgcm):
         jgcm.Trans_Spherical_To_Grid_Batch(dev, s[:1], g[:1], [16])
     with pytest.raises(AssertionError):   # u cos θ without the divergence
         jgcm.Trans_Spherical_To_Grid_Batch(dev, s[:1], g[:1], [jgcm.INV_UCOS])
-    with pytest.raises(AssertionError):   # more than 8 arrays
-        jgcm.Trans_Spherical_To_Grid_Batch(dev, s * 2, g * 2)
+    with pytest.raises(AssertionError):   # more than 12 arrays
+        jgcm.Trans_Spherical_To_Grid_Batch(dev, s * 3, g * 3)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "split32", "bf16x3"])

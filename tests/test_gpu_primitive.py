@@ -287,6 +287,30 @@ def test_deferred_pressures_and_heights_equal_eager(jgcm):
     assert not torch.equal(deferred[1]["grid_z_full"], deferred[2]["grid_z_full"])
 
 
+def test_carried_gradients_equal_recomputed(jgcm):
+    """Steps that carry the next step's gradients of ln ps and T in their
+    closing inverse batch (eager, then a graph of three steps launched twice,
+    then eager again) leave every array of Dyn_Data equal, bit for bit, to
+    steps that transform the current state at their start."""
+    runs = []
+    for carry in (False, True):
+        _, (dev, dvc, datmo, dsi, dd) = build(jgcm, 42, 64, 20, 600)
+        step = lambda: jgcm.Atmosphere_Update(dev, datmo, dvc, dsi, dd, HS, carry_gradients=carry)
+        step()
+        jgcm.Update_Init_Step(dsi)
+        step()
+        g = jgcm.Step_Graph(dev, step, nfields=8 * 20 + 4, repeat=3)
+        g.launch()
+        g.launch()
+        g.close()
+        step()
+        dev.synchronize()
+        runs.append({f: t.clone() for f, t in vars(dd).items() if isinstance(t, torch.Tensor)})
+    assert len(runs[0]) > 50 and set(runs[0]) == set(runs[1])
+    for f, v in runs[0].items():
+        assert torch.equal(runs[1][f], v), f
+
+
 def test_hs_t42_l20_two_hundred_day_climate(jgcm):
     """The HSt42 case run for its 200 days (FP64, Δt = 600 s) reproduces the
     Held & Suarez (1994) climate: a westerly jet in each hemisphere of 25-40 m/s
