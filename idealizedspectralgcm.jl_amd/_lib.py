@@ -119,6 +119,7 @@ SIGNATURES = {
     "jgcm_shallow_water_physics": [P, P, P, P, P, D, D, P, P, P, I],
     "jgcm_sw_explicit": [P, P, P, P, P, D, D, P, D, P, P, P, P, P, P, P, P, I],
     "jgcm_sw_height_tendency": [P, P, P, P, P, P, P, I],
+    "jgcm_sw_height_tendency_carried": [P, P, P, P, P, P, P, P, I],
     "jgcm_sw_spectral_step": [P, D, P, P, P, P, P, ctypes.POINTER(P), ctypes.POINTER(P),
                               ctypes.POINTER(P), I],
     "jgcm_map": [P, I, SZ, P, D, D, P],

@@ -592,6 +592,21 @@ int jgcm_sw_height_tendency(jgcm_mesh_t* mesh, const double* spe_h, const double
   return JGCM_OK;
 }
 
+int jgcm_sw_height_tendency_carried(jgcm_mesh_t* mesh, const double* grid_dlambda_h,
+                                    const double* grid_dtheta_h, const double* grid_u,
+                                    const double* grid_v, const double* grid_h,
+                                    const double* grid_div, double* grid_dh, int nlev) {
+  if (int rc = check_mesh(mesh)) return rc;
+  JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
+  JGCM_REQUIRE(grid_dlambda_h && grid_dtheta_h && grid_u && grid_v && grid_h && grid_div && grid_dh,
+               "NULL field pointer");
+  size_t n = (size_t)nlev * mesh->grid_elems();
+  hipLaunchKernelGGL(sw_height_tendency_kernel, dim3(nblk(n)), dim3(256), 0, mesh->stream, grid_dh,
+                     grid_u, grid_v, grid_dlambda_h, grid_dtheta_h, grid_h, grid_div, n);
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
 int jgcm_sw_spectral_step(jgcm_leapfrog_t* lf, double h_0, const double* spe_kin, double* spe_energy,
                           double* spe_dvor, double* spe_ddiv, double* spe_dh, double* const* vor,
                           double* const* div, double* const* h, int nlev) {

@@ -16,7 +16,7 @@ from ._lib import call
 from .dyn_data import BAROTROPIC_STATE, SW_STATE, Time_Advance_Model, Time_Advance_Tracers
 from .mesh import (_ptr, Vor_Div_From_Grid_UV, Apply_Laplacian, Add_Horizontal_Advection,
                    Trans_Spherical_To_Grid_Batch, Trans_Grid_To_Spherical_Batch, BATCH_DIVCOS,
-                   BATCH_VORDIV, INV_UCOS, INV_VCOS)
+                   BATCH_VORDIV, INV_UCOS, INV_VCOS, INV_GRAD_LAMBDA, INV_GRAD_THETA)
 from .time_integrator import Damped_Leapfrog, Damped_Leapfrog_Batch
 
 
@@ -93,7 +93,8 @@ def Shallow_Water_Dynamics(mesh, atmo_data, h_0, dyn_data, integrator, verbose=F
     _sw_dynamics_rest(mesh, h_0, d, integrator, verbose)
 
 
-def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kappa_t, h_eq):
+def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kappa_t, h_eq,
+                       carry_gradients=False):
     """Shallow_Water_Physics! then Shallow_Water_Dynamics! (the drivers' step,
     Shallow_Water_Dynamics.jl:7-21 and :88-179) in fused passes (no reference
     counterpart): the physics, the absolute-vorticity residual, the kinetic
@@ -102,16 +103,31 @@ def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kapp
     (jgcm_sw_height_tendency); the spectral energy, implicit correction,
     damped leapfrog and spectral Time_Advance! in one (jgcm_sw_spectral_step);
     c = n of the grid state as one batched copy.  Equal to the two calls bit
-    for bit."""
+    for bit.
+
+    ``carry_gradients``: the closing inverse batch also forms the gradient of
+    the new h, which is the h whose gradient the next step's height tendency
+    takes (:147), and the next step that also carries uses it instead of two
+    more inverse transforms.  Same results bit for bit; every step of the run
+    must carry, and nothing may change the state between the steps."""
     d = dyn_data
     n = mesh._nlev(d.grid_vor, "grid")
+    carried = carry_gradients and getattr(d, "_carried_sw", False)
+    d._carried_sw = False
+    if carry_gradients and getattr(d, "_sw_carry", None) is None:
+        d._sw_carry = (mesh.grid_zeros(n), mesh.grid_zeros(n))   # before any capture
     call("jgcm_sw_explicit", mesh.handle, _ptr(d.grid_u_c), _ptr(d.grid_v_c), _ptr(d.grid_ps_c),
          _ptr(h_eq), float(kappa_m), float(kappa_t), _ptr(d.grid_vor), atmo_data.omega,
          _ptr(d.grid_δu), _ptr(d.grid_δv), _ptr(d.grid_δlnps), _ptr(d.grid_absvor),
          _ptr(d.grid_d_full1), _ptr(d.grid_u_p), _ptr(d.grid_v_p), _ptr(d.grid_ps_p), n)
     # :147-148 the height tendency's grid terms
-    call("jgcm_sw_height_tendency", mesh.handle, _ptr(d.spe_lnps_n), _ptr(d.grid_u_c),
-         _ptr(d.grid_v_c), _ptr(d.grid_ps_c), _ptr(d.grid_div), _ptr(d.grid_δlnps), n)
+    if carried:
+        call("jgcm_sw_height_tendency_carried", mesh.handle, _ptr(d._sw_carry[0]),
+             _ptr(d._sw_carry[1]), _ptr(d.grid_u_c), _ptr(d.grid_v_c), _ptr(d.grid_ps_c),
+             _ptr(d.grid_div), _ptr(d.grid_δlnps), n)
+    else:
+        call("jgcm_sw_height_tendency", mesh.handle, _ptr(d.spe_lnps_n), _ptr(d.grid_u_c),
+             _ptr(d.grid_v_c), _ptr(d.grid_ps_c), _ptr(d.grid_div), _ptr(d.grid_δlnps), n)
     # :133 Vor_Div_From_Grid_UV!, :140 and :149 Trans_Grid_To_Spherical!
     Trans_Grid_To_Spherical_Batch(
         mesh, [d.grid_δu, d.grid_δv, d.grid_d_full1, d.grid_δlnps],
@@ -122,12 +138,17 @@ def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kapp
     call("jgcm_sw_spectral_step", integrator.handle, float(h_0), _ptr(d.spe_d1),
          _ptr(d.spe_energy), _ptr(d.spe_δvor), _ptr(d.spe_δdiv), _ptr(d.spe_δlnps),
          lev("spe_vor"), lev("spe_div"), lev("spe_lnps"), n)
-    # :169-172 the new state on the grid
-    Trans_Spherical_To_Grid_Batch(
-        mesh, [d.spe_vor_n, d.spe_div_n, d.spe_lnps_n, d.spe_vor_n, d.spe_vor_n],
-        [d.grid_vor, d.grid_div, d.grid_ps_n, d.grid_u_n, d.grid_v_n],
-        [0, 0, 0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS],
-        snms_b=[None, None, None, d.spe_div_n, d.spe_div_n])
+    # :169-172 the new state on the grid (carrying: and the gradient of the new h)
+    snms = [d.spe_vor_n, d.spe_div_n, d.spe_lnps_n, d.spe_vor_n, d.spe_vor_n]
+    grids = [d.grid_vor, d.grid_div, d.grid_ps_n, d.grid_u_n, d.grid_v_n]
+    flags = [0, 0, 0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS]
+    snms_b = [None, None, None, d.spe_div_n, d.spe_div_n]
+    if carry_gradients:
+        snms += [d.spe_lnps_n, d.spe_lnps_n]
+        grids += [d._sw_carry[0], d._sw_carry[1]]
+        flags += [BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA]
+        snms_b += [None, None]
+    Trans_Spherical_To_Grid_Batch(mesh, snms, grids, flags, snms_b=snms_b)
     # :177 the c = n half of Time_Advance! for the grid state, then the tracers
     fields = [(d.grid_u_c, d.grid_u_n), (d.grid_v_c, d.grid_v_n), (d.grid_ps_c, d.grid_ps_n)]
     Pn = ctypes.c_void_p * len(fields)
@@ -135,6 +156,7 @@ def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kapp
     call("jgcm_copy_batch", mesh.handle, len(fields), Pn(*(c.data_ptr() for c, _ in fields)),
          Pn(*(x.data_ptr() for _, x in fields)), nbytes)
     Time_Advance_Tracers(d)
+    d._carried_sw = carry_gradients
 
 
 def _sw_dynamics_rest(mesh, h_0, d, integrator, verbose):

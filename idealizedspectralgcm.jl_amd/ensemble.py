@@ -99,7 +99,8 @@ class SW_Group:
     def step(self):
         """One shallow-water step of every instance of the group."""
         d, m = self.d, self.mesh
-        Shallow_Water_Step(m, self.atmo, self.h_0, d, self.integ, 0.0, 0.0, self.h_eq)
+        Shallow_Water_Step(m, self.atmo, self.h_0, d, self.integ, 0.0, 0.0, self.h_eq,
+                           carry_gradients=True)
         if self.steps == 0:
             Update_Init_Step(self.integ)
         self.steps += 1
@@ -155,7 +156,7 @@ class SW_Ensemble:
         """Capture one later step of all groups into one graph (after the first step)."""
         assert all(g.steps >= 1 for g in self.groups), "take the first (init) step eagerly"
         for g in self.groups:
-            g.mesh.reserve(5 * g.count)
+            g.mesh.reserve(7 * g.count)
         self.synchronize()
         others = [g for g in self.groups if g.mesh.stream() != self.owner.stream()]
 
@@ -163,7 +164,8 @@ class SW_Ensemble:
             for g in others:                       # fork: branches start after the owner
                 call("jgcm_mesh_wait", g.mesh.handle, self.owner.handle)
             for g in self.groups:
-                Shallow_Water_Step(g.mesh, g.atmo, g.h_0, g.d, g.integ, 0.0, 0.0, g.h_eq)
+                Shallow_Water_Step(g.mesh, g.atmo, g.h_0, g.d, g.integ, 0.0, 0.0, g.h_eq,
+                                   carry_gradients=True)
             for g in others:                       # join: the graph ends after every branch
                 call("jgcm_mesh_wait", self.owner.handle, g.mesh.handle)
 

@@ -43,7 +43,8 @@ class Galewsky_Experiment:
         self.graph = None
 
     def _step(self):
-        Shallow_Water_Step(self.mesh, self.atmo, self.h_0, self.d, self.integ, 0.0, 0.0, self.h_eq)
+        Shallow_Water_Step(self.mesh, self.atmo, self.h_0, self.d, self.integ, 0.0, 0.0, self.h_eq,
+                           carry_gradients=True)
 
     def _prepare(self):
         """First (forward) step eagerly, then capture the later step once."""
@@ -52,7 +53,7 @@ class Galewsky_Experiment:
             Update_Init_Step(self.integ)
             self.steps = 1
         if self.graph is None:
-            self.graph = Step_Graph(self.mesh, self._step, nfields=5)
+            self.graph = Step_Graph(self.mesh, self._step, nfields=7)
 
     def snapshot(self):
         """(grid vorticity, divergence, geopotential, u) copies, after the stream drains."""

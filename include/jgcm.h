@@ -484,6 +484,14 @@ int jgcm_sw_explicit(jgcm_mesh_t* mesh, const double* grid_u, const double* grid
 int jgcm_sw_height_tendency(jgcm_mesh_t* mesh, const double* spe_h, const double* grid_u,
                             const double* grid_v, const double* grid_h, const double* grid_div,
                             double* grid_dh, int nlev);
+/* jgcm_sw_height_tendency with the gradient of h given on the grid
+ * (cos-weighted ∂λh, ∂θh divided by cos θ, as Compute_Gradients! leaves them;
+ * e.g. formed by the previous step's closing inverse batch).  No reference
+ * counterpart (a fusion).                                                   */
+int jgcm_sw_height_tendency_carried(jgcm_mesh_t* mesh, const double* grid_dlambda_h,
+                                    const double* grid_dtheta_h, const double* grid_u,
+                                    const double* grid_v, const double* grid_h,
+                                    const double* grid_div, double* grid_dh, int nlev);
 /* The spectral part of a shallow-water step after its forward transforms, in
  * one pass: energy = ∇²(KE + h_c) and δdiv -= energy (Shallow_Water_Dynamics.jl
  * :141-143), Implicit_Correction! (:150-157, Shallow_Water_Dynamics.jl:25-54),

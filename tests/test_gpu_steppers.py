@@ -163,6 +163,40 @@ def test_fused_shallow_water_step_equals_physics_and_dynamics(jgcm):
         assert torch.equal(runs[1][f], v), f
 
 
+def test_shallow_water_carried_gradients_equal_recomputed(jgcm):
+    """Shallow_Water_Step carrying the gradient of the new h from one step's
+    closing inverse batch to the next step's height tendency (eager steps,
+    then a graph of two steps launched twice) leaves every array of Dyn_Data
+    equal, bit for bit, to steps that transform h at their start."""
+    N, nlat = 42, 64
+    M, d, h_0, h_eq = sw_setup(N, nlat)
+    runs = []
+    for carry in (False, True):
+        dev = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, 2 * nlat, 1, 6.371e6)
+        dd = jgcm.Dyn_Data("sw", N, N + 1, 2 * nlat, nlat, 1, mesh=dev)
+        to_dev(dd, d, STATE)
+        atmo = jgcm.Atmo_Data("sw", 2 * nlat, nlat, 1, False, False, False, False, dev.sinθ, A,
+                              OMEGA)
+        Id = jgcm.Filtered_Leapfrog(0.04, 4, 1e-4, dev.laplacian_eig, 0.5, 1200, True, 0, 691200)
+        heq_d = torch.from_numpy(h_eq).cuda()
+        km, kt = 1 / (20 * 86400.0), 1 / (10 * 86400.0)
+        step = lambda: jgcm.Shallow_Water_Step(dev, atmo, h_0, dd, Id, km, kt, heq_d,
+                                               carry_gradients=carry)
+        step()
+        jgcm.Update_Init_Step(Id)
+        step()
+        g = jgcm.Step_Graph(dev, step, nfields=7, repeat=2)
+        g.launch()
+        g.launch()
+        g.close()
+        step()
+        torch.cuda.synchronize()
+        runs.append({f: t.clone() for f, t in vars(dd).items() if isinstance(t, torch.Tensor)})
+    assert len(runs[0]) > 30 and set(runs[0]) == set(runs[1])
+    for f, v in runs[0].items():
+        assert torch.equal(runs[1][f], v), f
+
+
 def test_graph_replay_matches_eager(jgcm):
     """A captured SW step replayed 3 times equals 3 eager steps bit for bit;
     eager work and synchronisation happen between the replays."""
