@@ -142,7 +142,7 @@ SIGNATURES = {
     "jgcm_hs_physics": [P, P, ctypes.c_int64] + [P] * 15 + [I],
     "jgcm_hs_column_dynamics": [P] * 27,
     "jgcm_hs_physics_column_dynamics": [P, P, ctypes.c_int64] + [P] * 27,
-    "jgcm_hs_physics_column_dynamics_carried": [P, P, ctypes.c_int64] + [P] * 29,
+    "jgcm_hs_physics_column_dynamics_ex": [P, P, ctypes.c_int64] + [P] * 29 + [I],
     "jgcm_hs_corrections": [P] * 8,
     "jgcm_hs_corrections_advance": [P, P, P, P, P, P],
     "jgcm_hs_implicit_leapfrog_advance": [P] * 6 + [ctypes.POINTER(P)] * 4,

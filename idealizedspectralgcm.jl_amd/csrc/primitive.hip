@@ -519,6 +519,10 @@ struct PhysArgs {
   // only receive ∇ps); NULL: read from dlps / dtps
   const double* dl_lnps;
   const double* dt_lnps;
+  // the p = c half of Time_Advance! for the grid u, v, T, ps: u_p, v_p, t_p,
+  // ps_p receive the current values once the forcing has read them (each
+  // element by the thread that read it)
+  int advance_p;
 };
 
 #define JGCM_COLUMN_DYNAMICS_ARGS                                                            \
@@ -719,6 +723,9 @@ __device__ __forceinline__ void hs_column_dynamics_body(double* red, const PhysA
     dps[c] = dps_new;
     dlnps[c] = dps_new / p;
     M_half[c] = 0.0;
+    if constexpr (PHYS) {
+      if (pa.advance_p) const_cast<double*>(pa.ps_p)[c] = p;
+    }
   }
 
   // geopotential
@@ -769,6 +776,13 @@ __device__ __forceinline__ void hs_column_dynamics_body(double* red, const PhysA
     g_half[o] = ghalf[i];
     energy[o] = gfull[i] + 0.5 * (uk * uk + vk * vk);
     if (k + 1 == nd) g_half[(size_t)nd * P + c] = ghalf[i + 1];
+    if constexpr (PHYS) {
+      if (pa.advance_p) {
+        const_cast<double*>(pa.u_p)[o] = uk;
+        const_cast<double*>(pa.v_p)[o] = vk;
+        const_cast<double*>(pa.t_p)[o] = T[i + 1];
+      }
+    }
   }
 }
 
@@ -2082,7 +2096,8 @@ static int physics_column_dynamics(
     const double* grid_geopots, double* grid_M_half, double* grid_w_full, double* grid_du,
     double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
     double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
-    const double* grid_dtheta_t, const double* dlambda_lnps, const double* dtheta_lnps) {
+    const double* grid_dtheta_t, const double* dlambda_lnps, const double* dtheta_lnps,
+    int advance_p) {
   JGCM_REQUIRE(si && hp, "NULL argument");
   JGCM_REQUIRE(grid_u_p && grid_v_p && grid_t_p && grid_ps_p && grid_t_eq && grid_div && grid_u &&
                    grid_v && grid_ps && grid_dlambda_ps && grid_dtheta_ps && grid_t && grid_vor &&
@@ -2099,7 +2114,8 @@ static int physics_column_dynamics(
               hs_params(hp, si->atmo.cp_air),
               grid_u_p, grid_v_p, grid_t_p, grid_ps_p, grid_t_eq, area_partials(si),
               si->atmo.do_mass_correction, si->atmo.do_energy_correction,
-              reinterpret_cast<int*>(si->d_scal + 15), skap_of(si), dlambda_lnps, dtheta_lnps};
+              reinterpret_cast<int*>(si->d_scal + 15), skap_of(si), dlambda_lnps, dtheta_lnps,
+              advance_p ? 1 : 0};
   const double* none = nullptr;
   JGCM_TILE_DISPATCH(nd, hs_physics_column_dynamics_kernel, tile_grid(P), tile_block(nd), 0,
                      M->stream, pa, vc_of(si->vc), M->nlat, M->nlon, si->atmo.rdgas,
@@ -2126,25 +2142,27 @@ int jgcm_hs_physics_column_dynamics(
                                  grid_t, grid_vor, grid_geopots, grid_M_half, grid_w_full, grid_du,
                                  grid_dv, grid_dps, grid_dt, geopot_half, geopot_full, grid_dlnps,
                                  grid_absvor, grid_energy, grid_dlambda_t, grid_dtheta_t, nullptr,
-                                 nullptr);
+                                 nullptr, 0);
 }
 
-int jgcm_hs_physics_column_dynamics_carried(
-    jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t dt, const double* grid_u_p,
-    const double* grid_v_p, const double* grid_t_p, const double* grid_ps_p, double* grid_t_eq,
+int jgcm_hs_physics_column_dynamics_ex(
+    jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t dt, double* grid_u_p,
+    double* grid_v_p, double* grid_t_p, double* grid_ps_p, double* grid_t_eq,
     const double* grid_div, const double* grid_u, const double* grid_v, const double* grid_ps,
     double* grid_dlambda_ps, double* grid_dtheta_ps, const double* grid_t, const double* grid_vor,
     const double* grid_geopots, double* grid_M_half, double* grid_w_full, double* grid_du,
     double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
     double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
-    const double* grid_dtheta_t, const double* dlambda_lnps, const double* dtheta_lnps) {
-  JGCM_REQUIRE(dlambda_lnps && dtheta_lnps, "NULL ln ps gradient");
+    const double* grid_dtheta_t, const double* dlambda_lnps, const double* dtheta_lnps,
+    int advance_p) {
+  JGCM_REQUIRE((dlambda_lnps == nullptr) == (dtheta_lnps == nullptr),
+               "dlambda_lnps and dtheta_lnps must both be given or both be NULL");
   return physics_column_dynamics(si, hp, dt, grid_u_p, grid_v_p, grid_t_p, grid_ps_p, grid_t_eq,
                                  grid_div, grid_u, grid_v, grid_ps, grid_dlambda_ps, grid_dtheta_ps,
                                  grid_t, grid_vor, grid_geopots, grid_M_half, grid_w_full, grid_du,
                                  grid_dv, grid_dps, grid_dt, geopot_half, geopot_full, grid_dlnps,
                                  grid_absvor, grid_energy, grid_dlambda_t, grid_dtheta_t,
-                                 dlambda_lnps, dtheta_lnps);
+                                 dlambda_lnps, dtheta_lnps, advance_p);
 }
 
 // Compute_Corrections! (SD:30-68) after the back-transforms, with ps_n = exp(lnps) (SD:336)
@@ -2185,7 +2203,7 @@ int jgcm_hs_corrections_advance(jgcm_semi_implicit_t* si, const double* grid_lnp
   JGCM_REQUIRE(si && grid_lnps && p && c && n && count, "NULL argument");
   jgcm_mesh* M = si->vc->mesh;
   for (int f = 0; f < 8; ++f)
-    JGCM_REQUIRE((p[f] || f < 4) && c[f] && n[f], "NULL field pointer (field %d)", f);
+    JGCM_REQUIRE(c[f] && n[f], "NULL field pointer (field %d)", f);
   const size_t P = M->grid_elems(), S = M->spec_elems();
   const int nd = si->nd;
   JGCM_REQUIRE(count[0] == nd * S * 2 && count[1] == nd * S * 2 && count[2] == S * 2 &&

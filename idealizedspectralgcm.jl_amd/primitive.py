@@ -315,13 +315,10 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
         args = (d.grid_u_p, d.grid_v_p, d.grid_t_p, d.grid_ps_p, d.grid_t_eq, d.grid_div,
                 d.grid_u_c, d.grid_v_c, d.grid_ps_c, d.grid_dλ_ps, d.grid_dθ_ps, d.grid_t_c,
                 d.grid_vor) + cols
-        if carried:
-            call("jgcm_hs_physics_column_dynamics_carried", si.handle,
-                 ctypes.byref(_hs(physics_params)), int(Δt),
-                 *map(P, args + (sc.carry_dl_lnps, sc.carry_dt_lnps)))
-        else:
-            call("jgcm_hs_physics_column_dynamics", si.handle, ctypes.byref(_hs(physics_params)),
-                 int(Δt), *map(P, args))
+        # with the p = c half of :346 Time_Advance! for the grid u, v, T, ps
+        carry_in = (sc.carry_dl_lnps, sc.carry_dt_lnps) if carried else (None, None)
+        call("jgcm_hs_physics_column_dynamics_ex", si.handle, ctypes.byref(_hs(physics_params)),
+             int(Δt), *map(P, args + carry_in), 1)
     else:
         call("jgcm_hs_column_dynamics", si.handle, *map(P, (                   # :260-307
             d.grid_div, d.grid_u_c, d.grid_v_c, d.grid_ps_c, None, None,
@@ -359,7 +356,9 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
         [f"grid_{f}" for f in ("u", "v", "ps", "t")]
     arrs = [[getattr(d, f + s) for f in names] for s in ("_p", "_c", "_n")]
     ptrs = [(ctypes.c_void_p * 8)(*[t.data_ptr() for t in ts]) for ts in arrs]
-    for k in range(4):   # spectral fields advanced above: their (0, 0) corrections only
+    # p = c done above for the spectral fields and (fused) the grid fields: c = n and
+    # the corrections remain
+    for k in range(8 if fused else 4):
         ptrs[0][k] = None
     counts = (ctypes.c_size_t * 8)(*[t.numel() * t.element_size() // 8 for t in arrs[1]])
     call("jgcm_hs_corrections_advance", si.handle, _ptr(d.grid_lnps), *ptrs, counts)

@@ -648,21 +648,26 @@ int jgcm_hs_physics_column_dynamics(
     double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
     double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
     const double* grid_dtheta_t);
-/* No reference counterpart: jgcm_hs_physics_column_dynamics with ∂λ ln ps
- * and ∂θ ln ps read from dlambda_lnps / dtheta_lnps (the gradients of the
- * step's current ln ps, carried from the previous step's closing inverse
- * batch) instead of from grid_dlambda_ps / grid_dtheta_ps, which then only
- * receive ∇ps (SD:260-261).  Equal to it bit for bit when the given arrays
- * hold what grid_dlambda_ps / grid_dtheta_ps would.                         */
-int jgcm_hs_physics_column_dynamics_carried(
-    jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t dt, const double* grid_u_p,
-    const double* grid_v_p, const double* grid_t_p, const double* grid_ps_p, double* grid_t_eq,
+/* No reference counterpart: jgcm_hs_physics_column_dynamics with two options.
+ * dlambda_lnps, dtheta_lnps (both or neither): ∂λ ln ps and ∂θ ln ps are read
+ * from them (the gradients of the step's current ln ps, carried from the
+ * previous step's closing inverse batch) instead of from grid_dlambda_ps /
+ * grid_dtheta_ps, which then only receive ∇ps (SD:260-261).  advance_p: the
+ * p = c half of Time_Advance! (Dyn_Data.jl:346-362) for the grid u, v, T, ps
+ * in the same pass — grid_u_p, grid_v_p, grid_t_p, grid_ps_p receive
+ * grid_u, grid_v, grid_t, grid_ps once the forcing has read them; pass NULL
+ * as their p pointers to jgcm_hs_corrections_advance afterwards.  Equal to
+ * jgcm_hs_physics_column_dynamics (and the full advance) bit for bit.      */
+int jgcm_hs_physics_column_dynamics_ex(
+    jgcm_semi_implicit_t* si, const jgcm_hs_params* hp, int64_t dt, double* grid_u_p,
+    double* grid_v_p, double* grid_t_p, double* grid_ps_p, double* grid_t_eq,
     const double* grid_div, const double* grid_u, const double* grid_v, const double* grid_ps,
     double* grid_dlambda_ps, double* grid_dtheta_ps, const double* grid_t, const double* grid_vor,
     const double* grid_geopots, double* grid_M_half, double* grid_w_full, double* grid_du,
     double* grid_dv, double* grid_dps, double* grid_dt, double* geopot_half, double* geopot_full,
     double* grid_dlnps, double* grid_absvor, double* grid_energy, const double* grid_dlambda_t,
-    const double* grid_dtheta_t, const double* dlambda_lnps, const double* dtheta_lnps);
+    const double* grid_dtheta_t, const double* dlambda_lnps, const double* dtheta_lnps,
+    int advance_p);
 /* ps_n = exp(lnps) then Compute_Corrections!  Spectral_Dynamics.jl:336-343, :30-68 */
 int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, double* grid_ps_n,
                         const double* grid_u_n, const double* grid_v_n, double* grid_t_n,
@@ -683,7 +688,9 @@ int jgcm_hs_corrections_advance(jgcm_semi_implicit_t* si, const double* grid_lnp
  * filtered current value and n the new one; c still holds the old value.
  * Pass NULL as the p pointers of these four spectral fields to
  * jgcm_hs_corrections_advance afterwards: for them it then only sets c = n,
- * with the (0, 0) corrections of T and ln ps.  The pair equals
+ * with the (0, 0) corrections of T and ln ps (so for any field whose p is
+ * NULL: the grid fields after jgcm_hs_physics_column_dynamics_ex with
+ * advance_p).  The pair equals
  * jgcm_hs_energy_implicit_leapfrog + jgcm_hs_corrections_advance bit for bit.
  * Nothing may read the spectral c arrays between the two calls.            */
 int jgcm_hs_implicit_leapfrog_advance(jgcm_semi_implicit_t* si, double* spe_energy, double* ddiv,
