@@ -70,6 +70,13 @@ SIGNATURES = {
     "jgcm_trans_grid_to_spherical_batch": [P, I, ctypes.POINTER(P), ctypes.POINTER(P),
                                            ctypes.POINTER(I), ctypes.POINTER(I)],
     "jgcm_trans_grid_to_fourier": [P, P, P, I],
+    "jgcm_plan_spherical_to_grid_create": [I, P, P, P, P, P, P, P, I, ctypes.POINTER(P)],
+    "jgcm_plan_grid_to_spherical_create": [I, P, P, P, P, P, P, I, ctypes.POINTER(P)],
+    "jgcm_plan_execute": [P],
+    "jgcm_plan_info": [P, ctypes.POINTER(I), ctypes.POINTER(I)],
+    "jgcm_plan_work_list": [P, P, P],
+    "jgcm_plan_destroy": [P],
+    "jgcm_lpt_schedule": [I, P, I, P, P, ctypes.POINTER(I)],
     "jgcm_trans_grid_to_spherical_adjoint": [P, P, P, I],
     "jgcm_trans_spherical_to_grid_adjoint": [P, P, P, I],
     "jgcm_trans_spherical_to_grid_adjoint_batch": [P, I, ctypes.POINTER(P), ctypes.POINTER(P),

@@ -147,81 +147,99 @@ def _inputs(mesh, nfields, seed):
     return grid, spec
 
 
-def _sweep(mesh, cands, apply, grid, spec, reps):
-    """Forward and inverse transform times (us) per candidate."""
+def _sweep(mesh, cands, apply, grid, spec, reps, workload=None):
+    """Forward and inverse times (us) per candidate: of plain transforms of
+    `grid` / `spec`, or of the workload's two callables."""
     from .mesh import Trans_Grid_To_Spherical, Trans_Spherical_To_Grid
+    if workload is not None:
+        _, run_fwd, run_inv = workload
+    else:
+        run_fwd = lambda: Trans_Grid_To_Spherical(mesh, grid, spec, batch=True)
+        run_inv = lambda: Trans_Spherical_To_Grid(mesh, spec, grid, batch=True)
     fwd, inv = {}, {}
     for c in cands:
         apply(c)
-        fwd[c] = _time_us(mesh, lambda: Trans_Grid_To_Spherical(mesh, grid, spec, batch=True), reps)
-        inv[c] = _time_us(mesh, lambda: Trans_Spherical_To_Grid(mesh, spec, grid, batch=True), reps)
+        fwd[c] = _time_us(mesh, run_fwd, reps)
+        inv[c] = _time_us(mesh, run_inv, reps)
     return fwd, inv
+
+
+def _key(mesh, nfields, workload):
+    k = tuning_key(mesh, nfields)
+    return k if workload is None else f"{k}|{workload[0]}"
 
 
 def _us(d):
     return {str(c): round(v, 3) for c, v in d.items()}
 
 
-def Tune_FFT_Shape(mesh, nfields, candidates=CANDIDATES, reps=30, cache=None, seed=0):
+def Tune_FFT_Shape(mesh, nfields, candidates=CANDIDATES, reps=30, cache=None, seed=0,
+                   workload=None):
     """Time forward and inverse transforms of `nfields` synthetic level-fields
     on `mesh` for every candidate FFT shape (those above the mesh's LDS limit
     are skipped; the mesh's Legendre tiles stay as they are), apply the
     fastest fold and unfold shapes and return the entry; stored in `cache`
-    (a Tuning_Cache) when given."""
+    (a Tuning_Cache) when given.  `workload` = (name, forward, inverse):
+    time those two callables (a model's own transform batches) instead; the
+    cache key then carries the name."""
     limit = get_fft_shape(mesh)[2]
     cands = [c for c in candidates if 1 <= c <= limit]
     if not cands:
         raise ValueError(f"no candidate shape fits (limit {limit})")
-    grid, spec = _inputs(mesh, nfields, seed)
-    fwd, inv = _sweep(mesh, cands, lambda c: set_fft_shape(mesh, c, c), grid, spec, reps)
+    grid, spec = _inputs(mesh, nfields, seed) if workload is None else (None, None)
+    fwd, inv = _sweep(mesh, cands, lambda c: set_fft_shape(mesh, c, c), grid, spec, reps,
+                      workload)
     fold, unfold = select(fwd, inv)
     set_fft_shape(mesh, fold, unfold)
     ft, it = get_legendre_shape(mesh)
     entry = {"fold_ppw": fold, "unfold_ppw": unfold, "fwd_tiles": ft, "inv_tiles": it,
              "forward_us": _us(fwd), "inverse_us": _us(inv)}
     if cache is not None:
-        cache.put(device_key(), tuning_key(mesh, nfields), entry)
+        cache.put(device_key(), _key(mesh, nfields, workload), entry)
     return entry
 
 
-def Tune_Legendre_Shape(mesh, nfields, candidates=TILE_CANDIDATES, reps=30, seed=0):
+def Tune_Legendre_Shape(mesh, nfields, candidates=TILE_CANDIDATES, reps=30, seed=0,
+                        workload=None):
     """Time forward and inverse transforms for every Legendre tile candidate
     (the mesh's FFT shapes stay as they are), apply the fastest per
     direction and return (fwd_tiles, inv_tiles, forward_us, inverse_us)."""
     cands = [c for c in candidates if c in TILE_CANDIDATES]
     if not cands:
         raise ValueError(f"Legendre tile candidates must be among {TILE_CANDIDATES}")
-    grid, spec = _inputs(mesh, nfields, seed)
-    fwd, inv = _sweep(mesh, cands, lambda c: set_legendre_shape(mesh, c, c), grid, spec, reps)
+    grid, spec = _inputs(mesh, nfields, seed) if workload is None else (None, None)
+    fwd, inv = _sweep(mesh, cands, lambda c: set_legendre_shape(mesh, c, c), grid, spec, reps,
+                      workload)
     ft, it = select(fwd, inv)
     set_legendre_shape(mesh, ft, it)
     return ft, it, fwd, inv
 
 
-def Tune_Shapes(mesh, nfields, reps=30, cache=None, seed=0):
+def Tune_Shapes(mesh, nfields, reps=30, cache=None, seed=0, workload=None):
     """FFT shapes (with the built-in Legendre tiles), then Legendre tiles
     (with the chosen FFT shapes); both applied to `mesh`, the entry returned
     and stored in `cache` when given."""
     set_legendre_shape(mesh, 0, 0)
-    entry = Tune_FFT_Shape(mesh, nfields, reps=reps, seed=seed)
-    ft, it, lf, li = Tune_Legendre_Shape(mesh, nfields, reps=reps, seed=seed)
+    entry = Tune_FFT_Shape(mesh, nfields, reps=reps, seed=seed, workload=workload)
+    ft, it, lf, li = Tune_Legendre_Shape(mesh, nfields, reps=reps, seed=seed, workload=workload)
     entry.update({"fwd_tiles": ft, "inv_tiles": it,
                   "legendre_forward_us": _us(lf), "legendre_inverse_us": _us(li)})
     if cache is not None:
-        cache.put(device_key(), tuning_key(mesh, nfields), entry)
+        cache.put(device_key(), _key(mesh, nfields, workload), entry)
     return entry
 
 
-def Apply_Tuning(mesh, nfields, cache=None, tune_missing=False, **kw):
+def Apply_Tuning(mesh, nfields, cache=None, tune_missing=False, workload=None, **kw):
     """Set the mesh's FFT and Legendre shapes from the cache entry for
-    (device, mesh, nfields); with `tune_missing`, tune and store when there
-    is none.  Returns the entry used, or None (the built-in shapes stay)."""
+    (device, mesh, nfields[, workload name]); with `tune_missing`, tune and
+    store when there is none.  Returns the entry used, or None (the built-in
+    shapes stay)."""
     cache = cache if cache is not None else Tuning_Cache()
-    entry = cache.get(device_key(), tuning_key(mesh, nfields))
+    entry = cache.get(device_key(), _key(mesh, nfields, workload))
     if entry is None:
         if not tune_missing:
             return None
-        return Tune_Shapes(mesh, nfields, cache=cache, **kw)
+        return Tune_Shapes(mesh, nfields, cache=cache, workload=workload, **kw)
     limit = get_fft_shape(mesh)[2]
     fold, unfold = int(entry["fold_ppw"]), int(entry["unfold_ppw"])
     ft, it = int(entry.get("fwd_tiles", 0)), int(entry.get("inv_tiles", 0))

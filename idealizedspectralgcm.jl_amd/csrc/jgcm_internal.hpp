@@ -124,7 +124,8 @@ struct jgcm_mesh {
   // events and buffers) is freed only after the last of them, whatever order
   // a garbage collector finalizes them in.
   std::atomic<int> refs{1};
-  std::atomic<int> live_graphs{0};     // graphs whose kernels use this mesh's scratch
+  std::atomic<int> live_graphs{0};     // graphs and plans whose kernels use this mesh's scratch
+  std::atomic<int> live_plans{0};      // of which plans (jgcm_plan_*_create)
   jgcm_mesh* stream_owner = nullptr;   // the mesh whose stream this one borrows
   std::vector<void*> retired;          // scratch replaced while graphs still held it
 

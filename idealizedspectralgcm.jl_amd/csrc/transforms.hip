@@ -27,8 +27,11 @@
 //             contraction steps the products P_hi x_hi and the cross terms
 //             P_hi x_lo + P_lo x_hi are summed by FP32 MFMA, the block sums
 //             are accumulated in FP64 (compensated summation).
+#include <algorithm>
 #include <mutex>
+#include <queue>
 #include <type_traits>
+#include <vector>
 
 #include "fft.hpp"
 #include "jgcm_internal.hpp"
@@ -969,20 +972,21 @@ __device__ __forceinline__ uint3 wg_coords() {
   return make_uint3(L % gx, (L / gx) % gy, L / (gx * gy));
 }
 
-// grid: x = segment (m, par); y = group of 64 rows (4 tiles); z = (job,
-// chunk of 32 fields).  256 threads: wave w owns columns 16w .. 16w+15.
-// LDS: X [64 cols][KC+1], P [64 rows][KC+1].
+// Forward tile wg: x = segment (m, par); y = group of TL 16-row tiles; z =
+// (job, chunk of 32 fields).  256 threads: wave w owns columns 16w .. 16w+15.
+// LDS: X [64 cols][KC+1], P [16 TL rows][KC+1].  Shared by the single-mesh
+// kernel (wg from the launch grid) and the multi-mesh work-list kernel (wg
+// from a work item).
 template <int PREC, int TL>
-__global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int ns, int nm, int half,
-                                                         int nmax, int zero_top) {
+__device__ __forceinline__ void legendre_fwd_tile(const FwdJobs& jobs, int ns, int nm, int half,
+                                                  int nmax, int zero_top, const uint3 wg,
+                                                  unsigned char* smem_raw) {
   constexpr int TROWS = 16 * TL;
   static_assert(TL == 1 || TL == 2 || TL == 4, "1, 2 or 4 tiles of 16 rows");
   using LT = typename LdsT<PREC>::T;
   using TT = typename TabT<PREC>::T;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
   LT* X = reinterpret_cast<LT*>(smem_raw);
   TT* Pt = reinterpret_cast<TT*>(smem_raw + align16(sizeof(LT) * LCOLS * KS1));
-  const uint3 wg = wg_coords();
   const int m = wg.x >> 1;
   const int par = wg.x & 1;
   int jobi = 0;
@@ -1099,6 +1103,13 @@ __global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int 
   }
 }
 
+template <int PREC, int TL>
+__global__ __launch_bounds__(LTHREADS) void legendre_fwd_mfma(FwdJobs jobs, int ns, int nm, int half,
+                                                         int nmax, int zero_top) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  legendre_fwd_tile<PREC, TL>(jobs, ns, nm, half, nmax, zero_top, wg_coords(), smem_raw);
+}
+
 // Operator table values of one spectral row (m, n), loaded once per
 // staging lane and shared by every field of the chunk (they do not depend on
 // the field).  Only the tables of the operators in `op_mask` (bit 1 << op)
@@ -1197,24 +1208,24 @@ __device__ __forceinline__ double2 staged_coefficient_seg(const SegMap& in, cons
   return staged_coefficient_row(h, h2, (in.flag[k] >> 1) & 7, r, n, m, nm, ns);
 }
 
-// Inverse: grid x = segment (m, par), y = chunk of 32 fields, z = group of
-// 64 latitudes (4 tiles); 256 threads, wave w owns columns 16w .. 16w+15.
-// LDS: S [64 cols][KC+1], P [KC rows][KC+1].
+// Inverse tile wg: x = segment (m, par), y = chunk of 32 fields, z = group of
+// 16 TL latitudes; 256 threads, wave w owns columns 16w .. 16w+15.
+// LDS: S [64 cols][KC+1], P [KC rows][16 TL + 1].  Shared like the forward
+// tile by the single-mesh and the work-list kernels.
 // PLAIN: every field comes from in.ptr[0] without an operator (checked by the
 // launcher): the staging skips the segment and operator logic.
 template <int PREC, bool PLAIN, int TL>
-__global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2* __restrict__ eo,
-                                                         FwdTables tab, int B, int nmax, int ns,
-                                                         int nm, int half,
-                                                         SpecOpTables ops, int out_mode) {
+__device__ __forceinline__ void legendre_inv_tile(const SegMap& in, double2* __restrict__ eo,
+                                                  const FwdTables& tab, int B, int nmax, int ns,
+                                                  int nm, int half, const SpecOpTables& ops,
+                                                  int out_mode, const uint3 wg,
+                                                  unsigned char* smem_raw) {
   constexpr int TROWS = 16 * TL;
   static_assert(TL == 1 || TL == 2 || TL == 4, "1, 2 or 4 tiles of 16 latitudes");
   using LT = typename LdsT<PREC>::T;
   using TT = typename TabT<PREC>::T;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
   LT* S = reinterpret_cast<LT*>(smem_raw);
   TT* Pt = reinterpret_cast<TT*>(smem_raw + align16(sizeof(LT) * LCOLS * KS1));
-  const uint3 wg = wg_coords();
   const int m = wg.x >> 1;
   const int par = wg.x & 1;
   const int f0 = wg.y * (LCOLS / 2);
@@ -1371,6 +1382,16 @@ __global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2
       store_stream(o, v);
     }
   }
+}
+
+template <int PREC, bool PLAIN, int TL>
+__global__ __launch_bounds__(LTHREADS) void legendre_inv_mfma(SegMap in, double2* __restrict__ eo,
+                                                         FwdTables tab, int B, int nmax, int ns,
+                                                         int nm, int half,
+                                                         SpecOpTables ops, int out_mode) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  legendre_inv_tile<PREC, PLAIN, TL>(in, eo, tab, B, nmax, ns, nm, half, ops, out_mode,
+                                     wg_coords(), smem_raw);
 }
 
 // ------------------------------------------------------------------ drivers
@@ -1741,13 +1762,14 @@ int fold_full(const jgcm_mesh* M, const double2* four, double2* eo, int B) {
   return JGCM_OK;
 }
 
-// Back-transform of several spectral arrays in one Legendre launch and one FFT
-// launch; flag & 1 divides the rows of that array by cos θ, flag >> 1 names a
-// spectral operator applied on the way in (SPEC_OP_*, second operand snm_b[k]).
-int inverse_batch(const jgcm_mesh* M, int nseg, const double* const* snm,
-                  const double* const* snm_b, double* const* grid, const int* nlev,
-                  const int* flags) {
-  SegMap in{}, out{};
+// Segment maps of a batched back-transform: `in` the spectral arrays with
+// their operators, `out` the grid arrays with their ÷cos θ flags.  Returns
+// the number of level-fields.
+static int build_inverse(int nseg, const double* const* snm, const double* const* snm_b,
+                         double* const* grid, const int* nlev, const int* flags, SegMap& in,
+                         SegMap& out) {
+  in = SegMap{};
+  out = SegMap{};
   in.nseg = out.nseg = nseg;
   int B = 0;
   for (int k = 0; k < nseg; ++k) {
@@ -1760,18 +1782,31 @@ int inverse_batch(const jgcm_mesh* M, int nseg, const double* const* snm,
     B += nlev[k];
   }
   in.start[nseg] = out.start[nseg] = B;
+  return B;
+}
+
+// Back-transform of several spectral arrays in one Legendre launch and one FFT
+// launch; flag & 1 divides the rows of that array by cos θ, flag >> 1 names a
+// spectral operator applied on the way in (SPEC_OP_*, second operand snm_b[k]).
+int inverse_batch(const jgcm_mesh* M, int nseg, const double* const* snm,
+                  const double* const* snm_b, double* const* grid, const int* nlev,
+                  const int* flags) {
+  SegMap in, out;
+  build_inverse(nseg, snm, snm_b, grid, nlev, flags, in, out);
   double2* eo = reinterpret_cast<double2*>(M->d_four_a);
   if (int rc = legendre_inv_map(M, in, eo, 3, M->ns)) return rc;
   return ifft_unfold_map(M, eo, out);
 }
 
-// Forward transform of several grid arrays in one FFT launch and one Legendre
-// launch.  flags[k] == JGCM_BATCH_VORDIV pairs array k (u) with array k+1 (v)
-// and writes vor to snm[k], div to snm[k+1] (Vor_Div_From_Grid_UV!); other
-// arrays get the plain transform.  Consecutive plain arrays share one job.
-int forward_batch(const jgcm_mesh* M, int nseg, const double* const* grid, double* const* snm,
-                  const int* nlev, const int* flags) {
-  SegMap in{};
+// Segment map of a batched forward transform's grid inputs and its Legendre
+// jobs reading the E/O block `eo`.  flags[k] == JGCM_BATCH_VORDIV pairs array
+// k (u) with array k+1 (v) and writes vor to snm[k], div to snm[k+1]
+// (Vor_Div_From_Grid_UV!); other arrays get the plain transform, and
+// consecutive plain arrays share one job.
+static int build_forward(const jgcm_mesh* M, int nseg, const double* const* grid,
+                         double* const* snm, const int* nlev, const int* flags, double2* eo,
+                         SegMap& in, FwdJobs& jobs) {
+  in = SegMap{};
   in.nseg = nseg;
   int B = 0;
   for (int k = 0; k < nseg; ++k) {
@@ -1783,9 +1818,8 @@ int forward_batch(const jgcm_mesh* M, int nseg, const double* const* grid, doubl
     B += nlev[k];
   }
   in.start[nseg] = B;
-  double2* eo = reinterpret_cast<double2*>(M->d_four_a);
   const size_t eo_field = (size_t)M->nm * 2 * M->half;
-  FwdJobs jobs{};
+  jobs = FwdJobs{};
   int nj = 0;
   for (int k = 0; k < nseg;) {
     if (nj >= MAXJOB) {
@@ -1817,8 +1851,350 @@ int forward_batch(const jgcm_mesh* M, int nseg, const double* const* grid, doubl
     jobs.job[nj++] = plain_job(M, eo + in.start[k0] * eo_field, nf, out);
   }
   jobs.njobs = nj;
+  int z = 0;
+  for (int j = 0; j < nj; ++j) {
+    jobs.zstart[j] = z;
+    z += (jobs.job[j].nf + LCOLS / 2 - 1) / (LCOLS / 2);
+  }
+  jobs.zstart[nj] = z;
+  return JGCM_OK;
+}
+
+// Forward transform of several grid arrays in one FFT launch and one Legendre
+// launch (build_forward gives the arrays' roles).
+int forward_batch(const jgcm_mesh* M, int nseg, const double* const* grid, double* const* snm,
+                  const int* nlev, const int* flags) {
+  double2* eo = reinterpret_cast<double2*>(M->d_four_a);
+  SegMap in;
+  FwdJobs jobs;
+  if (int rc = build_forward(M, nseg, grid, snm, nlev, flags, eo, in, jobs)) return rc;
   if (int rc = fft_fold_map(M, in, eo)) return rc;
   return launch_legendre_fwd(M, jobs, M->N, 1);
+}
+
+// ------------------------------------------------ plans: several meshes per launch
+// A plan holds the batched transform of several meshes that share one stream
+// (an ensemble's truncations) and runs the Legendre stage of all of them in
+// ONE launch, driven by a work list.  A work item is one tile of a one-mesh
+// Legendre launch: (mesh, segment (m, parity), y, z), with y, z the grid
+// coordinates that tile has in the one-mesh launch, so every tile computes
+// exactly what it computes there.  The items are sorted by an estimate of
+// their cost and dealt out longest-first to the least-loaded workgroup
+// (persistent workgroups, each running its own item list) or, with one
+// workgroup per item, launched heaviest first.
+
+constexpr int PLAN_MAXMESH = 8;
+
+struct InvDesc {
+  SegMap in;
+  double2* eo;
+  FwdTables tab;
+  SpecOpTables ops;
+  int B, nmax, ns, nm, half, tl;
+};
+
+struct FwdDesc {
+  FwdJobs jobs;
+  int ns, nm, half, nmax, tl;
+};
+
+template <int PREC>
+__global__ __launch_bounds__(LTHREADS) void legendre_inv_worklist(const InvDesc* __restrict__ desc,
+                                                             const int4* __restrict__ items,
+                                                             const int* __restrict__ wg_start) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int end = wg_start[blockIdx.x + 1];
+  for (int i = wg_start[blockIdx.x]; i < end; ++i) {
+    const int4 it = items[i];
+    const InvDesc& d = desc[it.x];
+    const uint3 wg = make_uint3(it.y, it.z, it.w);
+    // each tile starts with a barrier before its first LDS store, so the
+    // previous tile's epilogue has read the LDS it reuses
+    if (d.tl == 1)
+      legendre_inv_tile<PREC, false, 1>(d.in, d.eo, d.tab, d.B, d.nmax, d.ns, d.nm, d.half,
+                                        d.ops, 0, wg, smem_raw);
+    else if (d.tl == 4)
+      legendre_inv_tile<PREC, false, 4>(d.in, d.eo, d.tab, d.B, d.nmax, d.ns, d.nm, d.half,
+                                        d.ops, 0, wg, smem_raw);
+    else
+      legendre_inv_tile<PREC, false, 2>(d.in, d.eo, d.tab, d.B, d.nmax, d.ns, d.nm, d.half,
+                                        d.ops, 0, wg, smem_raw);
+  }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(LTHREADS) void legendre_fwd_worklist(const FwdDesc* __restrict__ desc,
+                                                             const int4* __restrict__ items,
+                                                             const int* __restrict__ wg_start) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int end = wg_start[blockIdx.x + 1];
+  for (int i = wg_start[blockIdx.x]; i < end; ++i) {
+    const int4 it = items[i];
+    const FwdDesc& d = desc[it.x];
+    const uint3 wg = make_uint3(it.y, it.z, it.w);
+    if (d.tl == 1)
+      legendre_fwd_tile<PREC, 1>(d.jobs, d.ns, d.nm, d.half, d.nmax, 1, wg, smem_raw);
+    else if (d.tl == 4)
+      legendre_fwd_tile<PREC, 4>(d.jobs, d.ns, d.nm, d.half, d.nmax, 1, wg, smem_raw);
+    else
+      legendre_fwd_tile<PREC, 2>(d.jobs, d.ns, d.nm, d.half, d.nmax, 1, wg, smem_raw);
+  }
+}
+
+// Longest processing time first: items in decreasing cost (ties in index
+// order), each to the workgroup with the least work so far (ties: the lowest
+// workgroup).  order = the item indices grouped by workgroup, each group in
+// assignment order; wg_start[w] .. wg_start[w+1] is workgroup w's range.
+// nwg <= 0 or >= n: one workgroup per item, heaviest first.
+int lpt_schedule(const std::vector<double>& cost, int nwg, std::vector<int>& order,
+                 std::vector<int>& wg_start) {
+  const int n = (int)cost.size();
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  if (nwg <= 0 || nwg >= n) {
+    order = idx;
+    wg_start.resize(n + 1);
+    for (int i = 0; i <= n; ++i) wg_start[i] = i;
+    return n;
+  }
+  using Load = std::pair<double, int>;
+  std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+  for (int w = 0; w < nwg; ++w) heap.push({0.0, w});
+  std::vector<std::vector<int>> bins(nwg);
+  for (int i : idx) {
+    Load l = heap.top();
+    heap.pop();
+    bins[l.second].push_back(i);
+    heap.push({l.first + cost[i], l.second});
+  }
+  order.clear();
+  wg_start.assign(1, 0);
+  for (int w = 0; w < nwg; ++w) {
+    order.insert(order.end(), bins[w].begin(), bins[w].end());
+    wg_start.push_back((int)order.size());
+  }
+  return nwg;
+}
+
+}  // namespace jgcm
+
+struct jgcm_plan {
+  int direction = 0;  // 0: spherical -> grid, 1: grid -> spherical
+  int nmesh = 0;
+  int precision = 0;
+  jgcm_mesh* mesh[jgcm::PLAN_MAXMESH] = {};
+  jgcm::SegMap fft[jgcm::PLAN_MAXMESH];    // the FFT stage's grid arrays per mesh
+  double2* eo[jgcm::PLAN_MAXMESH] = {};     // each mesh's E/O block
+  void* d_desc = nullptr;                   // InvDesc[] or FwdDesc[]
+  int4* d_items = nullptr;
+  int* d_wg_start = nullptr;
+  int nitems = 0, nwg = 0;
+  size_t lds = 0;
+  std::vector<int4> items;                  // host copy in launch order
+  std::vector<int> wg_start;
+};
+
+namespace jgcm {
+
+static int plan_occupancy(int direction, int precision, size_t lds, int* per_cu) {
+  const void* k = nullptr;
+  switch (precision) {
+#define JGCM_PLAN_K(P)                                                                       \
+  k = direction == 0 ? reinterpret_cast<const void*>(legendre_inv_worklist<P>)               \
+                     : reinterpret_cast<const void*>(legendre_fwd_worklist<P>)
+    case JGCM_PREC_FP64: JGCM_PLAN_K(P_FP64); break;
+    case JGCM_PREC_FP32: JGCM_PLAN_K(P_FP32); break;
+    case JGCM_PREC_BF16X3: JGCM_PLAN_K(P_BF16); break;
+    default: JGCM_PLAN_K(P_SPLIT); break;
+#undef JGCM_PLAN_K
+  }
+  JGCM_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, LTHREADS, lds));
+  if (*per_cu < 1) *per_cu = 1;
+  return JGCM_OK;
+}
+
+static void free_plan(jgcm_plan* p) {
+  if (!p) return;
+  jgcm_mesh* m0 = p->mesh[0];
+  void* bufs[] = {p->d_desc, p->d_items, p->d_wg_start};
+  if (m0) {
+    (void)hipStreamSynchronize(m0->stream);
+    // graphs that replay this plan's launches keep its buffers' addresses:
+    // while any graph of the first mesh lives, the buffers go with the mesh
+    const bool graphs = m0->live_graphs.load() > m0->live_plans.load();
+    for (void* b : bufs)
+      if (b) {
+        if (graphs) m0->retired.push_back(b);
+        else (void)hipFree(b);
+      }
+  } else {
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+  }
+  for (int g = 0; g < p->nmesh; ++g) {
+    jgcm_mesh* m = p->mesh[g];
+    if (!m) continue;
+    m->live_plans.fetch_sub(1);
+    m->live_graphs.fetch_sub(1);
+    mesh_release(m);
+  }
+  delete p;
+}
+
+// Shared part of the two create calls: meshes checked (one stream, one
+// precision, each once, no capture open) and retained; the plan's scratch
+// pinned like a graph's (growth keeps the buffers the plan addresses).
+static int plan_begin(int direction, int nmesh, jgcm_mesh_t* const* meshes, jgcm_plan** out) {
+  JGCM_REQUIRE(out != nullptr, "out is NULL");
+  *out = nullptr;
+  JGCM_REQUIRE(nmesh >= 1 && nmesh <= PLAN_MAXMESH, "nmesh must be in 1..%d, got %d",
+               PLAN_MAXMESH, nmesh);
+  JGCM_REQUIRE(meshes != nullptr, "meshes is NULL");
+  if (capture_ctx() != nullptr) {
+    set_error("a plan allocates and uploads its work list: create it before capturing");
+    return JGCM_ERR_STATE;
+  }
+  for (int g = 0; g < nmesh; ++g) {
+    if (int rc = check_mesh(meshes[g])) return rc;
+    JGCM_REQUIRE(meshes[g]->stream == meshes[0]->stream,
+                 "mesh %d does not share mesh 0's stream (jgcm_mesh_share_stream)", g);
+    JGCM_REQUIRE(meshes[g]->precision == meshes[0]->precision,
+                 "mesh %d has another Legendre precision than mesh 0", g);
+    for (int q = 0; q < g; ++q)
+      JGCM_REQUIRE(meshes[q] != meshes[g], "mesh %d appears twice (one batch per mesh)", g);
+  }
+  jgcm_plan* p = new (std::nothrow) jgcm_plan();
+  if (!p) {
+    set_error("out of host memory");
+    return JGCM_ERR_NOMEM;
+  }
+  p->direction = direction;
+  p->precision = meshes[0]->precision;
+  *out = p;
+  return JGCM_OK;
+}
+
+static void plan_hold(jgcm_plan* p, jgcm_mesh* m) {
+  mesh_retain(m);
+  m->live_graphs.fetch_add(1);
+  m->live_plans.fetch_add(1);
+  p->mesh[p->nmesh++] = m;
+}
+
+// Work list from the items' costs, uploaded with the descriptors.
+static int plan_finish(jgcm_plan* p, const void* desc, size_t desc_bytes,
+                       const std::vector<int4>& items, const std::vector<double>& cost,
+                       int workgroups, int tlmax) {
+  p->lds = legendre_lds(p->precision, tlmax);
+  int nwg = workgroups;
+  if (workgroups < 0) {
+    int per_cu = 1, cus = 1;
+    if (int rc = plan_occupancy(p->direction, p->precision, p->lds, &per_cu)) return rc;
+    JGCM_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                         p->mesh[0]->device));
+    nwg = per_cu * cus;
+  }
+  std::vector<int> order;
+  p->nwg = lpt_schedule(cost, nwg, order, p->wg_start);
+  p->nitems = (int)items.size();
+  p->items.resize(items.size());
+  for (size_t i = 0; i < order.size(); ++i) p->items[i] = items[order[i]];
+  JGCM_CHECK_HIP(hipMalloc(&p->d_desc, desc_bytes));
+  JGCM_CHECK_HIP(hipMalloc(&p->d_items, sizeof(int4) * std::max<size_t>(1, p->items.size())));
+  JGCM_CHECK_HIP(hipMalloc(&p->d_wg_start, sizeof(int) * p->wg_start.size()));
+  JGCM_CHECK_HIP(hipMemcpy(p->d_desc, desc, desc_bytes, hipMemcpyHostToDevice));
+  JGCM_CHECK_HIP(hipMemcpy(p->d_items, p->items.data(), sizeof(int4) * p->items.size(),
+                           hipMemcpyHostToDevice));
+  JGCM_CHECK_HIP(hipMemcpy(p->d_wg_start, p->wg_start.data(), sizeof(int) * p->wg_start.size(),
+                           hipMemcpyHostToDevice));
+  return JGCM_OK;
+}
+
+// Cost estimates of one tile in staged elements: both operand blocks over the
+// contraction, the MFMA steps, the stores and a fixed part per tile.
+static double inv_item_cost(int nr, int nf, int nlg) {
+  const int nlt = (nlg + 15) / 16;
+  return 256.0 + (double)nr * (nf + nlg + 4 * nlt) + (double)nf * nlg;
+}
+
+static double fwd_item_cost(int half, int terms, int nf, int nrg) {
+  const int ntile = (nrg + 15) / 16;
+  return 256.0 + (double)terms * half * (nf + nrg + 4 * ntile) + (double)nf * nrg;
+}
+
+static int plan_launch_legendre(const jgcm_plan* p) {
+  if (p->nitems == 0) return JGCM_OK;
+  hipStream_t st = p->mesh[0]->stream;
+  const int4* it = p->d_items;
+  const int* ws = p->d_wg_start;
+#define JGCM_PLAN_LAUNCH(P)                                                                   \
+  do {                                                                                        \
+    if (p->direction == 0)                                                                    \
+      hipLaunchKernelGGL(legendre_inv_worklist<P>, dim3(p->nwg), dim3(LTHREADS), p->lds, st,  \
+                         reinterpret_cast<const InvDesc*>(p->d_desc), it, ws);                \
+    else                                                                                      \
+      hipLaunchKernelGGL(legendre_fwd_worklist<P>, dim3(p->nwg), dim3(LTHREADS), p->lds, st,  \
+                         reinterpret_cast<const FwdDesc*>(p->d_desc), it, ws);                \
+  } while (0)
+  switch (p->precision) {
+    case JGCM_PREC_FP64: JGCM_PLAN_LAUNCH(P_FP64); break;
+    case JGCM_PREC_FP32: JGCM_PLAN_LAUNCH(P_FP32); break;
+    case JGCM_PREC_BF16X3: JGCM_PLAN_LAUNCH(P_BF16); break;
+    default: JGCM_PLAN_LAUNCH(P_SPLIT); break;
+  }
+#undef JGCM_PLAN_LAUNCH
+  JGCM_LAUNCH_CHECK();
+  return JGCM_OK;
+}
+
+}  // namespace jgcm
+
+namespace jgcm {
+
+// Argument checks of the batched transforms (the one-mesh calls and plans);
+// *total = the level-fields of the batch.
+static int check_inverse_args(int nseg, const double* const* snm, const double* const* snm_b,
+                              double* const* pfield, const int* nlev, const int* flags,
+                              int* total) {
+  JGCM_REQUIRE(nseg >= 1 && nseg <= MAXSEG, "nseg must be in 1..%d, got %d", MAXSEG, nseg);
+  JGCM_REQUIRE(snm && pfield && nlev, "NULL argument");
+  int t = 0;
+  for (int k = 0; k < nseg; ++k) {
+    JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
+    JGCM_REQUIRE(nlev[k] >= 1, "nlev[%d] must be >= 1, got %d", k, nlev[k]);
+    int f = flags ? flags[k] : 0;
+    int op = f >> 1;
+    JGCM_REQUIRE(f >= 0 && op <= SPEC_OP_VCOS, "flags[%d] = %d is not valid", k, f);
+    JGCM_REQUIRE(op < SPEC_OP_UCOS || (snm_b && snm_b[k]),
+                 "array %d: u cos / v cos needs the divergence in snm_b[%d]", k, k);
+    t += nlev[k];
+  }
+  *total = t;
+  return JGCM_OK;
+}
+
+static int check_forward_args(int nseg, const double* const* pfield, double* const* snm,
+                              const int* nlev, const int* flags, int* total) {
+  JGCM_REQUIRE(nseg >= 1 && nseg <= MAXSEG, "nseg must be in 1..%d, got %d", MAXSEG, nseg);
+  JGCM_REQUIRE(snm && pfield && nlev, "NULL argument");
+  int t = 0;
+  for (int k = 0; k < nseg; ++k) {
+    JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
+    JGCM_REQUIRE(nlev[k] >= 1, "nlev[%d] must be >= 1, got %d", k, nlev[k]);
+    int f = flags ? flags[k] : 0;
+    JGCM_REQUIRE(f == 0 || f == JGCM_BATCH_VORDIV, "flags[%d] = %d is not valid", k, f);
+    if (f == JGCM_BATCH_VORDIV) {
+      JGCM_REQUIRE(k + 1 < nseg && flags[k + 1] == 0 && nlev[k + 1] == nlev[k],
+                   "array %d (u) needs a following v array with flag 0 and the same nlev", k);
+      t += nlev[k];
+      ++k;
+      JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
+    }
+    t += nlev[k];
+  }
+  *total = t;
+  return JGCM_OK;
 }
 
 }  // namespace jgcm
@@ -1931,19 +2307,8 @@ int jgcm_trans_spherical_to_grid_batch(jgcm_mesh_t* mesh, int nseg, const double
                                        const double* const* snm_b, double* const* pfield,
                                        const int* nlev, const int* flags) {
   if (int rc = check_mesh(mesh)) return rc;
-  JGCM_REQUIRE(nseg >= 1 && nseg <= MAXSEG, "nseg must be in 1..%d, got %d", MAXSEG, nseg);
-  JGCM_REQUIRE(snm && pfield && nlev, "NULL argument");
   int total = 0;
-  for (int k = 0; k < nseg; ++k) {
-    JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
-    JGCM_REQUIRE(nlev[k] >= 1, "nlev[%d] must be >= 1, got %d", k, nlev[k]);
-    int f = flags ? flags[k] : 0;
-    int op = f >> 1;
-    JGCM_REQUIRE(f >= 0 && op <= SPEC_OP_VCOS, "flags[%d] = %d is not valid", k, f);
-    JGCM_REQUIRE(op < SPEC_OP_UCOS || (snm_b && snm_b[k]),
-                 "array %d: u cos / v cos needs the divergence in snm_b[%d]", k, k);
-    total += nlev[k];
-  }
+  if (int rc = check_inverse_args(nseg, snm, snm_b, pfield, nlev, flags, &total)) return rc;
   if (int rc = ensure_scratch(mesh, total)) return rc;
   return inverse_batch(mesh, nseg, snm, snm_b, pfield, nlev, flags);
 }
@@ -1951,23 +2316,8 @@ int jgcm_trans_spherical_to_grid_batch(jgcm_mesh_t* mesh, int nseg, const double
 int jgcm_trans_grid_to_spherical_batch(jgcm_mesh_t* mesh, int nseg, const double* const* pfield,
                                        double* const* snm, const int* nlev, const int* flags) {
   if (int rc = check_mesh(mesh)) return rc;
-  JGCM_REQUIRE(nseg >= 1 && nseg <= MAXSEG, "nseg must be in 1..%d, got %d", MAXSEG, nseg);
-  JGCM_REQUIRE(snm && pfield && nlev, "NULL argument");
   int total = 0;
-  for (int k = 0; k < nseg; ++k) {
-    JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
-    JGCM_REQUIRE(nlev[k] >= 1, "nlev[%d] must be >= 1, got %d", k, nlev[k]);
-    int f = flags ? flags[k] : 0;
-    JGCM_REQUIRE(f == 0 || f == JGCM_BATCH_VORDIV, "flags[%d] = %d is not valid", k, f);
-    if (f == JGCM_BATCH_VORDIV) {
-      JGCM_REQUIRE(k + 1 < nseg && flags[k + 1] == 0 && nlev[k + 1] == nlev[k],
-                   "array %d (u) needs a following v array with flag 0 and the same nlev", k);
-      total += nlev[k];
-      ++k;
-      JGCM_REQUIRE(snm[k] && pfield[k], "NULL field pointer (array %d)", k);
-    }
-    total += nlev[k];
-  }
+  if (int rc = check_forward_args(nseg, pfield, snm, nlev, flags, &total)) return rc;
   if (int rc = ensure_scratch(mesh, total)) return rc;
   return forward_batch(mesh, nseg, pfield, snm, nlev, flags);
 }
@@ -2083,6 +2433,193 @@ int jgcm_compute_alpha_operator(jgcm_mesh_t* mesh, const double* fourier_a, cons
   if (int rc = fold_full(mesh, reinterpret_cast<const double2*>(fourier_a), ea, nlev)) return rc;
   if (int rc = fold_full(mesh, reinterpret_cast<const double2*>(fourier_b), eb, nlev)) return rc;
   return legendre_alpha(mesh, ea, eb, isign, reinterpret_cast<double2*>(alpha), nlev);
+}
+
+// ------------------------------------------------------------------ plans
+int jgcm_plan_spherical_to_grid_create(int nmesh, jgcm_mesh_t* const* meshes, const int* nseg,
+                                       const double* const* const* snm,
+                                       const double* const* const* snm_b,
+                                       double* const* const* pfield, const int* const* nlev,
+                                       const int* const* flags, int workgroups,
+                                       jgcm_plan_t** out) {
+  JGCM_REQUIRE(nseg && snm && pfield && nlev, "NULL argument");
+  jgcm_plan* p = nullptr;
+  if (int rc = plan_begin(0, nmesh, meshes, &p)) return rc;
+  std::vector<InvDesc> desc(nmesh);
+  std::vector<int4> items;
+  std::vector<double> cost;
+  int tlmax = 1, rc = JGCM_OK;
+  for (int g = 0; g < nmesh && rc == JGCM_OK; ++g) {
+    jgcm_mesh* M = meshes[g];
+    int total = 0;
+    rc = check_inverse_args(nseg[g], snm[g], snm_b ? snm_b[g] : nullptr, pfield[g], nlev[g],
+                            flags ? flags[g] : nullptr, &total);
+    if (rc == JGCM_OK) rc = ensure_scratch(M, total);
+    if (rc != JGCM_OK) break;
+    plan_hold(p, M);
+    InvDesc& d = desc[g];
+    build_inverse(nseg[g], snm[g], snm_b ? snm_b[g] : nullptr, pfield[g], nlev[g],
+                  flags ? flags[g] : nullptr, d.in, p->fft[g]);
+    p->eo[g] = reinterpret_cast<double2*>(M->d_four_a);
+    d.eo = p->eo[g];
+    d.tab = tables_for(M, 3);
+    d.ops = SpecOpTables{M->d_dlam, M->d_dthm, M->d_dthp, M->d_uvc, M->d_uvm, M->d_uvp};
+    d.B = total;
+    d.nmax = d.ns = M->ns;
+    d.nm = M->nm;
+    d.half = M->half;
+    d.tl = tiles_of(M->leg_tiles_inv);
+    tlmax = std::max(tlmax, d.tl);
+    const int trows = 16 * d.tl;
+    const int fchunks = (total + LCOLS / 2 - 1) / (LCOLS / 2);
+    const int lgroups = (M->half + trows - 1) / trows;
+    for (int x = 0; x < 2 * M->nm; ++x) {
+      const int nfirst = (x >> 1) + (x & 1);
+      const int nr = nfirst <= d.nmax ? (d.nmax - nfirst) / 2 + 1 : 0;
+      for (int y = 0; y < fchunks; ++y) {
+        const int nf = std::min(LCOLS / 2, total - y * (LCOLS / 2));
+        for (int z = 0; z < lgroups; ++z) {
+          const int nlg = std::min(trows, M->half - z * trows);
+          items.push_back(make_int4(g, x, y, z));
+          cost.push_back(inv_item_cost(nr, nf, nlg));
+        }
+      }
+    }
+  }
+  if (rc == JGCM_OK)
+    rc = plan_finish(p, desc.data(), sizeof(InvDesc) * nmesh, items, cost, workgroups, tlmax);
+  if (rc != JGCM_OK) {
+    free_plan(p);
+    return rc;
+  }
+  *out = p;
+  return JGCM_OK;
+}
+
+int jgcm_plan_grid_to_spherical_create(int nmesh, jgcm_mesh_t* const* meshes, const int* nseg,
+                                       const double* const* const* pfield,
+                                       double* const* const* snm, const int* const* nlev,
+                                       const int* const* flags, int workgroups,
+                                       jgcm_plan_t** out) {
+  JGCM_REQUIRE(nseg && snm && pfield && nlev, "NULL argument");
+  jgcm_plan* p = nullptr;
+  if (int rc = plan_begin(1, nmesh, meshes, &p)) return rc;
+  std::vector<FwdDesc> desc(nmesh);
+  std::vector<int4> items;
+  std::vector<double> cost;
+  int tlmax = 1, rc = JGCM_OK;
+  for (int g = 0; g < nmesh && rc == JGCM_OK; ++g) {
+    jgcm_mesh* M = meshes[g];
+    int total = 0;
+    rc = check_forward_args(nseg[g], pfield[g], snm[g], nlev[g], flags ? flags[g] : nullptr,
+                            &total);
+    if (rc == JGCM_OK) rc = ensure_scratch(M, total);
+    if (rc != JGCM_OK) break;
+    plan_hold(p, M);
+    FwdDesc& d = desc[g];
+    p->eo[g] = reinterpret_cast<double2*>(M->d_four_a);
+    rc = build_forward(M, nseg[g], pfield[g], snm[g], nlev[g], flags ? flags[g] : nullptr,
+                       p->eo[g], p->fft[g], d.jobs);
+    if (rc != JGCM_OK) break;
+    d.ns = M->ns;
+    d.nm = M->nm;
+    d.half = M->half;
+    d.nmax = M->N;
+    d.tl = tiles_of(M->leg_tiles_fwd);
+    tlmax = std::max(tlmax, d.tl);
+    const int trows = 16 * d.tl;
+    const int maxrows = d.nmax / 2 + 1;
+    const int rgroups = ((maxrows + 15) / 16 + d.tl - 1) / d.tl;
+    for (int j = 0; j < d.jobs.njobs; ++j) {
+      const FwdJob& job = d.jobs.job[j];
+      const int terms = job.srcB ? 2 : 1;
+      for (int z = d.jobs.zstart[j]; z < d.jobs.zstart[j + 1]; ++z) {
+        const int nf = std::min(LCOLS / 2, job.nf - (z - d.jobs.zstart[j]) * (LCOLS / 2));
+        for (int x = 0; x < 2 * M->nm; ++x) {
+          const int nfirst = (x >> 1) + (x & 1);
+          const int nr = nfirst <= d.nmax ? (d.nmax - nfirst) / 2 + 1 : 0;
+          for (int y = 0; y < rgroups; ++y) {
+            const int nrg = std::min(trows, nr - y * trows);
+            if (nrg <= 0) continue;  // a tile without rows writes nothing
+            items.push_back(make_int4(g, x, y, z));
+            cost.push_back(fwd_item_cost(M->half, terms, nf, nrg));
+          }
+        }
+      }
+    }
+  }
+  if (rc == JGCM_OK)
+    rc = plan_finish(p, desc.data(), sizeof(FwdDesc) * nmesh, items, cost, workgroups, tlmax);
+  if (rc != JGCM_OK) {
+    free_plan(p);
+    return rc;
+  }
+  *out = p;
+  return JGCM_OK;
+}
+
+int jgcm_plan_execute(jgcm_plan_t* plan) {
+  JGCM_REQUIRE(plan != nullptr, "plan is NULL");
+  for (int g = 0; g < plan->nmesh; ++g) {
+    jgcm_mesh* M = plan->mesh[g];
+    if (int rc = check_mesh(M)) return rc;  // an open capture records the mesh
+    if (M->stream != plan->mesh[0]->stream) {
+      set_error("mesh %d no longer shares mesh 0's stream", g);
+      return JGCM_ERR_STATE;
+    }
+    if (M->precision != plan->precision) {
+      set_error("mesh %d's Legendre precision changed after the plan was created", g);
+      return JGCM_ERR_STATE;
+    }
+  }
+  if (plan->direction == 1) {
+    for (int g = 0; g < plan->nmesh; ++g)
+      if (int rc = fft_fold_map(plan->mesh[g], plan->fft[g], plan->eo[g])) return rc;
+    return plan_launch_legendre(plan);
+  }
+  if (int rc = plan_launch_legendre(plan)) return rc;
+  for (int g = 0; g < plan->nmesh; ++g)
+    if (int rc = ifft_unfold_map(plan->mesh[g], plan->eo[g], plan->fft[g])) return rc;
+  return JGCM_OK;
+}
+
+int jgcm_plan_info(jgcm_plan_t* plan, int* nitems, int* nworkgroups) {
+  JGCM_REQUIRE(plan != nullptr, "plan is NULL");
+  if (nitems) *nitems = plan->nitems;
+  if (nworkgroups) *nworkgroups = plan->nwg;
+  return JGCM_OK;
+}
+
+int jgcm_plan_work_list(jgcm_plan_t* plan, int* items, int* wg_start) {
+  JGCM_REQUIRE(plan != nullptr, "plan is NULL");
+  if (items)
+    for (int i = 0; i < plan->nitems; ++i) {
+      const int4 v = plan->items[i];
+      items[4 * i] = v.x;
+      items[4 * i + 1] = v.y;
+      items[4 * i + 2] = v.z;
+      items[4 * i + 3] = v.w;
+    }
+  if (wg_start)
+    for (int w = 0; w <= plan->nwg; ++w) wg_start[w] = plan->wg_start[w];
+  return JGCM_OK;
+}
+
+int jgcm_plan_destroy(jgcm_plan_t* plan) {
+  free_plan(plan);
+  return JGCM_OK;
+}
+
+int jgcm_lpt_schedule(int n, const double* cost, int nwg, int* order, int* wg_start,
+                      int* used) {
+  JGCM_REQUIRE(n >= 0 && (n == 0 || cost) && order && wg_start, "NULL argument");
+  std::vector<double> c(cost, cost + n);
+  std::vector<int> o, ws;
+  int u = lpt_schedule(c, nwg, o, ws);
+  std::copy(o.begin(), o.end(), order);
+  std::copy(ws.begin(), ws.end(), wg_start);
+  if (used) *used = u;
+  return JGCM_OK;
 }
 
 }  // extern "C"

@@ -111,16 +111,30 @@ def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kapp
     more inverse transforms.  Same results bit for bit; every step of the run
     must carry, and nothing may change the state between the steps."""
     d = dyn_data
-    n = mesh._nlev(d.grid_vor, "grid")
     carried = carry_gradients and getattr(d, "_carried_sw", False)
     d._carried_sw = False
     if carry_gradients and getattr(d, "_sw_carry", None) is None:
-        d._sw_carry = (mesh.grid_zeros(n), mesh.grid_zeros(n))   # before any capture
+        d._sw_carry = (mesh.grid_zeros(d.grid_u_c.shape[0]),
+                       mesh.grid_zeros(d.grid_u_c.shape[0]))   # before any capture
+    sw_grid_pass(mesh, atmo_data, d, kappa_m, kappa_t, h_eq, carried)
+    Trans_Grid_To_Spherical_Batch(mesh, *sw_forward_batch(d))
+    sw_spectral_pass(integrator, h_0, d)
+    snms, grids, flags, snms_b = sw_inverse_batch(d, carry_gradients)
+    Trans_Spherical_To_Grid_Batch(mesh, snms, grids, flags, snms_b=snms_b)
+    sw_closing_copy(mesh, d)
+    d._carried_sw = carry_gradients
+
+
+# The stages of Shallow_Water_Step, also run stage by stage over several
+# meshes by the mixed-truncation ensemble (one transform plan per direction).
+def sw_grid_pass(mesh, atmo_data, d, kappa_m, kappa_t, h_eq, carried):
+    """The explicit grid pass and the height tendency's grid terms (:147-148);
+    `carried`: the gradient of h comes from the previous step's inverse batch."""
+    n = mesh._nlev(d.grid_vor, "grid")
     call("jgcm_sw_explicit", mesh.handle, _ptr(d.grid_u_c), _ptr(d.grid_v_c), _ptr(d.grid_ps_c),
          _ptr(h_eq), float(kappa_m), float(kappa_t), _ptr(d.grid_vor), atmo_data.omega,
          _ptr(d.grid_δu), _ptr(d.grid_δv), _ptr(d.grid_δlnps), _ptr(d.grid_absvor),
          _ptr(d.grid_d_full1), _ptr(d.grid_u_p), _ptr(d.grid_v_p), _ptr(d.grid_ps_p), n)
-    # :147-148 the height tendency's grid terms
     if carried:
         call("jgcm_sw_height_tendency_carried", mesh.handle, _ptr(d._sw_carry[0]),
              _ptr(d._sw_carry[1]), _ptr(d.grid_u_c), _ptr(d.grid_v_c), _ptr(d.grid_ps_c),
@@ -128,17 +142,29 @@ def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kapp
     else:
         call("jgcm_sw_height_tendency", mesh.handle, _ptr(d.spe_lnps_n), _ptr(d.grid_u_c),
              _ptr(d.grid_v_c), _ptr(d.grid_ps_c), _ptr(d.grid_div), _ptr(d.grid_δlnps), n)
-    # :133 Vor_Div_From_Grid_UV!, :140 and :149 Trans_Grid_To_Spherical!
-    Trans_Grid_To_Spherical_Batch(
-        mesh, [d.grid_δu, d.grid_δv, d.grid_d_full1, d.grid_δlnps],
-        [d.spe_δvor, d.spe_δdiv, d.spe_d1, d.spe_δlnps], [BATCH_VORDIV, 0, 0, 0])
-    # :141-143, :150-165 and the spectral part of :177
+
+
+def sw_forward_batch(d):
+    """(grid arrays, spectral arrays, flags) of the step's forward batch:
+    :133 Vor_Div_From_Grid_UV!, :140 and :149 Trans_Grid_To_Spherical!."""
+    return ([d.grid_δu, d.grid_δv, d.grid_d_full1, d.grid_δlnps],
+            [d.spe_δvor, d.spe_δdiv, d.spe_d1, d.spe_δlnps], [BATCH_VORDIV, 0, 0, 0])
+
+
+def sw_spectral_pass(integrator, h_0, d):
+    """:141-143, :150-165 and the spectral part of :177 in one kernel."""
+    n = d.spe_δvor.shape[0]
     P3 = ctypes.c_void_p * 3
     lev = lambda f: P3(*(getattr(d, f + s).data_ptr() for s in ("_p", "_c", "_n")))
     call("jgcm_sw_spectral_step", integrator.handle, float(h_0), _ptr(d.spe_d1),
          _ptr(d.spe_energy), _ptr(d.spe_δvor), _ptr(d.spe_δdiv), _ptr(d.spe_δlnps),
          lev("spe_vor"), lev("spe_div"), lev("spe_lnps"), n)
-    # :169-172 the new state on the grid (carrying: and the gradient of the new h)
+
+
+def sw_inverse_batch(d, carry_gradients):
+    """(spectral arrays, grid arrays, flags, second operands) of the step's
+    closing inverse batch: :169-172 the new state on the grid (carrying: and
+    the gradient of the new h)."""
     snms = [d.spe_vor_n, d.spe_div_n, d.spe_lnps_n, d.spe_vor_n, d.spe_vor_n]
     grids = [d.grid_vor, d.grid_div, d.grid_ps_n, d.grid_u_n, d.grid_v_n]
     flags = [0, 0, 0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS]
@@ -148,15 +174,17 @@ def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kapp
         grids += [d._sw_carry[0], d._sw_carry[1]]
         flags += [BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA]
         snms_b += [None, None]
-    Trans_Spherical_To_Grid_Batch(mesh, snms, grids, flags, snms_b=snms_b)
-    # :177 the c = n half of Time_Advance! for the grid state, then the tracers
+    return snms, grids, flags, snms_b
+
+
+def sw_closing_copy(mesh, d):
+    """:177 the c = n half of Time_Advance! for the grid state, then the tracers."""
     fields = [(d.grid_u_c, d.grid_u_n), (d.grid_v_c, d.grid_v_n), (d.grid_ps_c, d.grid_ps_n)]
     Pn = ctypes.c_void_p * len(fields)
     nbytes = (ctypes.c_size_t * len(fields))(*(c.numel() * c.element_size() for c, _ in fields))
     call("jgcm_copy_batch", mesh.handle, len(fields), Pn(*(c.data_ptr() for c, _ in fields)),
          Pn(*(x.data_ptr() for _, x in fields)), nbytes)
     Time_Advance_Tracers(d)
-    d._carried_sw = carry_gradients
 
 
 def _sw_dynamics_rest(mesh, h_0, d, integrator, verbose):

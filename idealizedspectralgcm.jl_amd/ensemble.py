@@ -16,10 +16,12 @@ import torch
 
 from ._lib import call
 from .dyn_data import Atmo_Data, Dyn_Data
-from .dynamics import Shallow_Water_Step
+from .dynamics import (Shallow_Water_Step, sw_closing_copy, sw_forward_batch, sw_grid_pass,
+                       sw_inverse_batch, sw_spectral_pass)
 from .graph import Step_Graph
 from .mesh import (_ptr, Spectral_Spherical_Mesh, Trans_Spherical_To_Grid, UV_Grid_From_Vor_Div,
                    Vor_Div_From_Grid_UV)
+from .plan import FORWARD, INVERSE, Transform_Plan
 from .time_integrator import Filtered_Leapfrog, Update_Init_Step
 
 RADIUS, OMEGA = 6371.0e3, 7.292e-5
@@ -42,6 +44,29 @@ def random_vorticity(N, count, rng, amplitude=1e-5):
     power = np.sum(np.abs(s[:, :, 0]) ** 2, axis=1) + 2 * np.sum(np.abs(s[:, :, 1:]) ** 2, axis=(1, 2))
     s *= (amplitude / np.sqrt(power))[:, None, None]
     return s
+
+
+def sw_step_workload(mesh, count, seed=0):
+    """The shallow-water step's two transform batches at `count` instances
+    (forward: u, v -> vor, div plus two plain arrays; inverse: seven arrays
+    with the u cos θ, v cos θ and gradient operators), on arrays of their
+    own, for the autotuner: (name, forward, inverse)."""
+    from .mesh import (BATCH_DIVCOS, BATCH_VORDIV, INV_GRAD_LAMBDA, INV_GRAD_THETA, INV_UCOS,
+                       INV_VCOS, Trans_Grid_To_Spherical_Batch, Trans_Spherical_To_Grid_Batch)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    grids = [torch.randn((count, mesh.nθ, mesh.nλ), dtype=torch.float64, device="cuda",
+                         generator=gen) for _ in range(7)]
+    specs = [mesh.spectral_zeros(count) for _ in range(4)]
+    torch.cuda.synchronize()
+    fwd_flags = [BATCH_VORDIV, 0, 0, 0]
+    vor, div, h = specs[0], specs[1], specs[2]
+    inv_in = [vor, div, h, vor, vor, h, h]
+    inv_flags = [0, 0, 0, BATCH_DIVCOS | INV_UCOS, BATCH_DIVCOS | INV_VCOS,
+                 BATCH_DIVCOS | INV_GRAD_LAMBDA, BATCH_DIVCOS | INV_GRAD_THETA]
+    inv_b = [None, None, None, div, div, None, None]
+    forward = lambda: Trans_Grid_To_Spherical_Batch(mesh, grids[:4], specs, fwd_flags)
+    inverse = lambda: Trans_Spherical_To_Grid_Batch(mesh, inv_in, grids, inv_flags, snms_b=inv_b)
+    return "sw-step", forward, inverse
 
 
 class SW_Group:
@@ -109,7 +134,8 @@ class SW_Group:
 class SW_Ensemble:
     """Mixed-truncation ensemble: groups = [(N, nθ, count, dt), ...]."""
 
-    def __init__(self, groups, seed=0, precision="fp32", tuning=None, concurrent=False):
+    def __init__(self, groups, seed=0, precision="fp32", tuning=None, concurrent=False,
+                 persistent=False, workgroups=-1):
         """`tuning`: None (built-in shapes) or an autotune.Tuning_Cache; each
         truncation's FFT and Legendre shapes (for its instance count) are then read from the
         cache, or tuned and stored when missing, before any step is captured.
@@ -119,11 +145,22 @@ class SW_Ensemble:
         the first group's stream and joins them back (jgcm_mesh_wait), so the
         truncations run as parallel branches of one graph.  The graph holds a
         library reference to every mesh it recorded, so no group's stream,
-        events or scratch can be freed under it."""
+        events or scratch can be freed under it.
+
+        `persistent=True` (one stream): the later steps run stage by stage over
+        all truncations, and each transform direction is one plan
+        (plan.Transform_Plan): the Legendre stage of every truncation in ONE
+        launch from a cost-sorted work list dealt to `workgroups` persistent
+        workgroups (< 0: as many as the device holds at once; 0: one per work
+        item).  Each instance's results equal the other forms bit for bit."""
         from .autotune import Apply_Tuning
+        assert not (concurrent and persistent), "the persistent form runs on one stream"
         self.groups = []
         self.tuning = {}
         self.concurrent = bool(concurrent)
+        self.persistent = bool(persistent)
+        self.workgroups = int(workgroups)
+        self._plan_fwd = self._plan_inv = None
         owner = None
         for i, (N, nθ, count, dt) in enumerate(groups):
             share = owner.mesh if (owner is not None and not self.concurrent) else None
@@ -131,7 +168,8 @@ class SW_Ensemble:
             if owner is None:
                 owner = g
             if tuning is not None:
-                e = Apply_Tuning(g.mesh, count, cache=tuning, tune_missing=True)
+                e = Apply_Tuning(g.mesh, count, cache=tuning, tune_missing=True,
+                                 workload=sw_step_workload(g.mesh, count))
                 self.tuning[f"T{N}"] = None if e is None else \
                     (e["fold_ppw"], e["unfold_ppw"], e.get("fwd_tiles", 0), e.get("inv_tiles", 0))
             self.groups.append(g)
@@ -144,8 +182,38 @@ class SW_Ensemble:
 
     def step(self):
         """One eager step of every group (each on its stream; see synchronize)."""
+        if self.persistent and all(g.steps >= 1 for g in self.groups):
+            self._stage_step()
+            for g in self.groups:
+                g.steps += 1
+            return
         for g in self.groups:
             g.step()
+
+    def plans(self):
+        """The two transform plans of the persistent form (built on first use,
+        after the first step, whose arrays they address)."""
+        if self._plan_fwd is None:
+            fwd = [(g.mesh, *sw_forward_batch(g.d)) for g in self.groups]
+            inv = [(g.mesh, *sw_inverse_batch(g.d, True)) for g in self.groups]
+            self._plan_fwd = Transform_Plan(FORWARD, fwd, self.workgroups)
+            self._plan_inv = Transform_Plan(INVERSE, inv, self.workgroups)
+        return self._plan_fwd, self._plan_inv
+
+    def _stage_step(self):
+        """One later step of every group, stage by stage (the stages of
+        Shallow_Water_Step with carried gradients), the transforms as plans."""
+        assert all(getattr(g.d, "_carried_sw", False) for g in self.groups), \
+            "the persistent form follows a step that carried its gradients"
+        fwd, inv = self.plans()
+        for g in self.groups:
+            sw_grid_pass(g.mesh, g.atmo, g.d, 0.0, 0.0, g.h_eq, True)
+        fwd.execute()
+        for g in self.groups:
+            sw_spectral_pass(g.integ, g.h_0, g.d)
+        inv.execute()
+        for g in self.groups:
+            sw_closing_copy(g.mesh, g.d)
 
     def synchronize(self):
         """Wait for every group's stream (graph launches complete on the owner's)."""
@@ -159,6 +227,11 @@ class SW_Ensemble:
             g.mesh.reserve(7 * g.count)
         self.synchronize()
         others = [g for g in self.groups if g.mesh.stream() != self.owner.stream()]
+
+        if self.persistent:
+            self.plans()                           # created (and uploaded) before the capture
+            self.graph = Step_Graph(self.owner, self._stage_step)
+            return self.graph
 
         def body():
             for g in others:                       # fork: branches start after the owner

@@ -56,6 +56,7 @@ extern "C" {
 typedef struct jgcm_mesh jgcm_mesh_t;
 typedef struct jgcm_leapfrog jgcm_leapfrog_t;
 typedef struct jgcm_graph jgcm_graph_t;
+typedef struct jgcm_plan jgcm_plan_t;
 typedef struct jgcm_vert_coord jgcm_vert_coord_t;
 typedef struct jgcm_semi_implicit jgcm_semi_implicit_t;
 
@@ -235,6 +236,52 @@ int jgcm_trans_spherical_to_grid_batch(jgcm_mesh_t* mesh, int nseg, const double
                                        const int* nlev, const int* flags);
 int jgcm_trans_grid_to_spherical_batch(jgcm_mesh_t* mesh, int nseg, const double* const* pfield,
                                        double* const* snm, const int* nlev, const int* flags);
+
+/* Plans: the batched transforms of several meshes in one launch per stage
+ * (no reference counterpart: the reference steps one model at a time; this is
+ * the mixed-truncation ensemble's launch form).  A plan takes, per mesh g, the
+ * arguments of jgcm_trans_spherical_to_grid_batch / _grid_to_spherical_batch
+ * (nseg[g], snm[g], snm_b[g], pfield[g], nlev[g], flags[g]; snm_b and flags
+ * may be NULL as a whole or per mesh).  The meshes must share one stream
+ * (jgcm_mesh_share_stream) and one Legendre precision, and each appears once.
+ * Execute runs the FFT stage per mesh (the work of every latitude is the same)
+ * and the Legendre stage of ALL meshes in ONE launch: every tile of the
+ * one-mesh launches -- (mesh, m, parity, row or latitude group, field chunk),
+ * with each mesh's tile shape (jgcm_mesh_set_legendre_shape) -- is a work
+ * item; items are sorted by estimated cost and dealt longest-first to the
+ * least-loaded workgroup.  workgroups: > 0 that many persistent workgroups,
+ * < 0 as many as fit on the device at once, 0 one workgroup per item
+ * (heaviest launched first).  Results equal the one-mesh batched calls bit
+ * for bit.  Creation reserves each mesh's scratch, allocates and uploads the
+ * work list (so it must happen before a graph capture); the arrays named at
+ * creation are the ones every execute reads and writes, and the tile shapes
+ * and precision are those of creation time (execute refuses a changed
+ * precision).                                                              */
+int jgcm_plan_spherical_to_grid_create(int nmesh, jgcm_mesh_t* const* meshes, const int* nseg,
+                                       const double* const* const* snm,
+                                       const double* const* const* snm_b,
+                                       double* const* const* pfield, const int* const* nlev,
+                                       const int* const* flags, int workgroups,
+                                       jgcm_plan_t** out);
+int jgcm_plan_grid_to_spherical_create(int nmesh, jgcm_mesh_t* const* meshes, const int* nseg,
+                                       const double* const* const* pfield,
+                                       double* const* const* snm, const int* const* nlev,
+                                       const int* const* flags, int workgroups,
+                                       jgcm_plan_t** out);
+int jgcm_plan_execute(jgcm_plan_t* plan);
+/* Work items and workgroups of the Legendre launch; the work list itself as
+ * 4 ints per item (mesh, segment 2m + parity, y, z) in launch order, and
+ * wg_start[nworkgroups + 1] (workgroup w runs items wg_start[w] ..
+ * wg_start[w+1] - 1).                                                      */
+int jgcm_plan_info(jgcm_plan_t* plan, int* nitems, int* nworkgroups);
+int jgcm_plan_work_list(jgcm_plan_t* plan, int* items, int* wg_start);
+int jgcm_plan_destroy(jgcm_plan_t* plan);
+/* The plans' scheduler on the host (no GPU call): n costs dealt in
+ * decreasing order (ties in index order) to the least-loaded of nwg bins
+ * (ties: lowest bin); order[n] = the item indices grouped by bin,
+ * wg_start[used + 1] the bins' ranges.  nwg <= 0 or >= n: one bin per item,
+ * heaviest first.                                                          */
+int jgcm_lpt_schedule(int n, const double* cost, int nwg, int* order, int* wg_start, int* used);
 
 /* Adjoints of the transform pair (no reference counterpart; the building
  * block of the reverse-mode derivative of a step).  Transposes with respect
