@@ -807,109 +807,138 @@ __device__ __forceinline__ fv4 mma16(sv4 a, sv4 b, fv4 c) {
 // acc[t] += sum over MFMA steps ks < KS of A_t(ks) B(ks); bval(ks, e) and
 // aval(t, ks, e) give this lane's operand values (from LDS): e = 0 for the
 // 16x16x4 instructions, e = 0..3 (four consecutive contraction steps of the
-// lane) for the bf16 16x16x16 instruction.
-template <int PREC, typename BF, typename AF>
-__device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS, BF bval, AF aval) {
-  if constexpr (PREC == P_FP64) {
-    dv4 c[4];
+// lane) for the bf16 16x16x16 instruction.  In steps: mfma_run_begin, then
+// mfma_run_steps over consecutive ranges of ks covering 0..KS (the staging of
+// a chunk may arrive in pieces), then mfma_run_end; the operations and their
+// order are those of one pass over 0..KS whatever the pieces: FP64 and FP32
+// accumulate the whole chunk in the MFMA accumulators, split-FP32 and BF16x3
+// close a block sum into acc after every 4 MFMA steps from the chunk's start
+// and after the last one.
+template <int PREC> struct MfmaRun {
+  fv4 c[4];   // FP32: the sums; split: hi*hi; BF16x3: the block sums
+  fv4 c2[4];  // split: the cross terms
+};
+template <> struct MfmaRun<P_FP64> {
+  dv4 c[4];
+};
+
+template <int PREC>
+__device__ __forceinline__ void mfma_run_begin(MfmaRun<PREC>& run) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) c[t] = dv4{0.0, 0.0, 0.0, 0.0};
+  for (int t = 0; t < 4; ++t) {
+    if constexpr (PREC == P_FP64) {
+      run.c[t] = dv4{0.0, 0.0, 0.0, 0.0};
+    } else {
+      run.c[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
+      run.c2[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+  }
+}
+
+// block sum of the split / BF16x3 forms into acc, accumulators cleared
+template <int PREC>
+__device__ __forceinline__ void mfma_run_close_block(TileAcc& acc, MfmaRun<PREC>& run) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if constexpr (PREC == P_SPLIT) acc.v[t][r] += (double)run.c[t][r] + (double)run.c2[t][r];
+      else acc.v[t][r] += (double)run.c[t][r];
+    }
+    run.c[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (PREC == P_SPLIT) run.c2[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
+  }
+}
+
+template <int PREC, typename BF, typename AF>
+__device__ __forceinline__ void mfma_run_steps(TileAcc& acc, MfmaRun<PREC>& run, int ntile, int ks0,
+                                               int ks1, BF bval, AF aval) {
+  if constexpr (PREC == P_FP64) {
 #pragma unroll 4
-    for (int ks = 0; ks < KS; ++ks) {
+    for (int ks = ks0; ks < ks1; ++ks) {
       double b = bval(ks, 0);
 #pragma unroll
       for (int t = 0; t < 4; ++t)
-        if (t < ntile) c[t] = mma64(aval(t, ks, 0), b, c[t]);
+        if (t < ntile) run.c[t] = mma64(aval(t, ks, 0), b, run.c[t]);
     }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc.v[t][r] += c[t][r];
   } else if constexpr (PREC == P_FP32) {
-    fv4 c[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) c[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 4
-    for (int ks = 0; ks < KS; ++ks) {
+    for (int ks = ks0; ks < ks1; ++ks) {
       float b = bval(ks, 0);
 #pragma unroll
       for (int t = 0; t < 4; ++t)
-        if (t < ntile) c[t] = mma32(aval(t, ks, 0), b, c[t]);
+        if (t < ntile) run.c[t] = mma32(aval(t, ks, 0), b, run.c[t]);
     }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc.v[t][r] += (double)c[t][r];
   } else if constexpr (PREC == P_SPLIT) {
-    for (int kb = 0; kb < KS; kb += 4) {
-      fv4 hh[4], cr[4];
+    for (int ks = ks0; ks < ks1; ++ks) {
+      float2 b = bval(ks, 0);
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        hh[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
-        cr[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
-      }
-      const int ke = min(KS, kb + 4);
-      for (int ks = kb; ks < ke; ++ks) {
-        float2 b = bval(ks, 0);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          if (t < ntile) {
-            float2 a = aval(t, ks, 0);
-            hh[t] = mma32(a.x, b.x, hh[t]);
-            cr[t] = mma32(a.x, b.y, cr[t]);
-            cr[t] = mma32(a.y, b.x, cr[t]);
-          }
+        if (t < ntile) {
+          float2 a = aval(t, ks, 0);
+          run.c[t] = mma32(a.x, b.x, run.c[t]);
+          run.c2[t] = mma32(a.x, b.y, run.c2[t]);
+          run.c2[t] = mma32(a.y, b.x, run.c2[t]);
         }
       }
-#pragma unroll
-      for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc.v[t][r] += (double)hh[t][r] + (double)cr[t][r];
+      if ((ks & 3) == 3) mfma_run_close_block<PREC>(acc, run);
     }
   } else {
     // BF16 x 3: x = x1 + x2 + x3, P = p1 + p2 + p3; the six products with
     // piece orders summing to <= 2 (the rest are below 2^-24 |P x|), smallest
     // first, in FP32 over blocks of 64 contraction steps; block sums in FP64.
-    for (int kb = 0; kb < KS; kb += 4) {
-      fv4 c[4];
+    for (int ks = ks0; ks < ks1; ++ks) {
+      sv4 b1, b2, b3;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) c[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
-      const int ke = min(KS, kb + 4);
-      for (int ks = kb; ks < ke; ++ks) {
-        sv4 b1, b2, b3;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          ushort4 x = bval(ks, e);
-          b1[e] = (short)x.x;
-          b2[e] = (short)x.y;
-          b3[e] = (short)x.z;
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          if (t < ntile) {
-            sv4 a1, a2, a3;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              ushort4 p = aval(t, ks, e);
-              a1[e] = (short)p.x;
-              a2[e] = (short)p.y;
-              a3[e] = (short)p.z;
-            }
-            c[t] = mma16(a1, b3, c[t]);
-            c[t] = mma16(a2, b2, c[t]);
-            c[t] = mma16(a3, b1, c[t]);
-            c[t] = mma16(a1, b2, c[t]);
-            c[t] = mma16(a2, b1, c[t]);
-            c[t] = mma16(a1, b1, c[t]);
-          }
-        }
+      for (int e = 0; e < 4; ++e) {
+        ushort4 x = bval(ks, e);
+        b1[e] = (short)x.x;
+        b2[e] = (short)x.y;
+        b3[e] = (short)x.z;
       }
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
+      for (int t = 0; t < 4; ++t) {
+        if (t < ntile) {
+          sv4 a1, a2, a3;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc.v[t][r] += (double)c[t][r];
+          for (int e = 0; e < 4; ++e) {
+            ushort4 p = aval(t, ks, e);
+            a1[e] = (short)p.x;
+            a2[e] = (short)p.y;
+            a3[e] = (short)p.z;
+          }
+          run.c[t] = mma16(a1, b3, run.c[t]);
+          run.c[t] = mma16(a2, b2, run.c[t]);
+          run.c[t] = mma16(a3, b1, run.c[t]);
+          run.c[t] = mma16(a1, b2, run.c[t]);
+          run.c[t] = mma16(a2, b1, run.c[t]);
+          run.c[t] = mma16(a1, b1, run.c[t]);
+        }
+      }
+      if ((ks & 3) == 3) mfma_run_close_block<PREC>(acc, run);
     }
   }
+}
+
+// the chunk of KS MFMA steps is complete
+template <int PREC>
+__device__ __forceinline__ void mfma_run_end(TileAcc& acc, MfmaRun<PREC>& run, int KS) {
+  if constexpr (PREC == P_FP64 || PREC == P_FP32) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc.v[t][r] += (double)run.c[t][r];
+  } else {
+    if (KS & 3) mfma_run_close_block<PREC>(acc, run);
+  }
+}
+
+template <int PREC, typename BF, typename AF>
+__device__ __forceinline__ void mfma_accumulate(TileAcc& acc, int ntile, int KS, BF bval, AF aval) {
+  MfmaRun<PREC> run;
+  mfma_run_begin<PREC>(run);
+  mfma_run_steps<PREC>(acc, run, ntile, 0, KS, bval, aval);
+  mfma_run_end<PREC>(acc, run, KS);
 }
 
 // Two staged copies whose global loads are issued together (up to IT + IT
@@ -1021,6 +1050,20 @@ __device__ __forceinline__ void legendre_fwd_tile(const FwdJobs& jobs, int ns, i
   const int lko = KSTEP == 16 ? 4 * lk : lk;  // this lane's first contraction step
   const LT* Xc = X + (16 * wave + li) * KS1 + lko;
   const TT* Pc = Pt + li * KS1 + lko;
+  // Staging in sub-chunks of SUB contraction steps (FP64: the products of one
+  // sub-chunk overlap the arrival of the next; the faster products of the
+  // other precisions measured best with one sub-chunk per chunk).  NG lane
+  // groups of SUB lanes each; a lane loads FL fields and RL table rows of a
+  // sub-chunk.
+  constexpr int SUB = PREC == P_FP64 ? 16 : KC;
+  constexpr int NSUB = KC / SUB;
+  constexpr int NG = 64 / SUB;
+  constexpr int NWV = LTHREADS / 64;
+  constexpr int FPT = (LCOLS / 2 + NWV - 1) / NWV;   // fields per wave
+  constexpr int RPT = (TROWS + NWV - 1) / NWV;       // table rows per wave
+  constexpr int FL = FPT / NG, RL = (RPT + NG - 1) / NG;
+  static_assert(FPT == NG * FL && KC % SUB == 0, "fields split evenly over the lane groups");
+  const int jl = lane & (SUB - 1), fq = lane / SUB;
   for (int term = 0; term < 2; ++term) {
     const double2* src = term == 0 ? job.srcA : job.srcB;
     if (!src) break;
@@ -1034,51 +1077,63 @@ __device__ __forceinline__ void legendre_fwd_tile(const FwdJobs& jobs, int ns, i
       const int kcl = min(KC, half - lat0);
       const int KSc = (kcl + KSTEP - 1) / KSTEP;
       const int kpad = KSTEP * KSc;
+      const int nsub = (kcl + SUB - 1) / SUB;
       __syncthreads();  // the previous chunk's operands are consumed
-      // lane = latitude of the chunk (kpad <= KC = 64), the wave picks the
-      // fields and table rows: no index division, all loads issued first
-      {
-        constexpr int NWV = LTHREADS / 64;
-        constexpr int FPT = (LCOLS / 2 + NWV - 1) / NWV;   // fields per thread
-        constexpr int RPT = (TROWS + NWV - 1) / NWV;       // table rows per thread
-        const int jp = lane;
-        const bool jin = jp < kcl;
-        double2 xv[FPT];
-        TT pv[RPT];
+      // Every load of the chunk is issued first, in sub-chunks of SUB
+      // latitudes (lane = latitude jl of the sub-chunk and field / row group
+      // fq; the wave picks the fields and table rows: no index division);
+      // then each sub-chunk goes to LDS as soon as its own loads are in, and
+      // its MFMA steps run while the later sub-chunks are still arriving.
+      double2 xv[NSUB][FL];
+      TT pv[NSUB][RL];
 #pragma unroll
-        for (int i = 0; i < FPT; ++i) {
-          const int fl = wave + NWV * i;
-          xv[i] = (fl < nf && jin) ? srow[fl * fstride + lat0 + jp] : make_double2(0.0, 0.0);
+      for (int sc = 0; sc < NSUB; ++sc) {
+        const int j = SUB * sc + jl;
+        const bool jin = sc < nsub && j < kcl;
+#pragma unroll
+        for (int p = 0; p < FL; ++p) {
+          const int fl = wave + NWV * (fq + NG * p);
+          xv[sc][p] = (fl < nf && jin) ? srow[fl * fstride + lat0 + j] : make_double2(0.0, 0.0);
         }
 #pragma unroll
-        for (int i = 0; i < RPT; ++i) {
+        for (int p = 0; p < RL; ++p) {
+          const int i = fq + NG * p;
           const int k = wave + NWV * i;
-          const bool ok = k < nrg && jin;
-          pv[i] = tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * k) * half + lat0 + jp : 0, ok);
+          const bool ok = i < RPT && k < nrg && jin;
+          pv[sc][p] = tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * k) * half + lat0 + j : 0, ok);
         }
-        if (jp < kpad) {
+      }
+      MfmaRun<PREC> run;
+      mfma_run_begin<PREC>(run);
 #pragma unroll
-          for (int i = 0; i < FPT; ++i) {
-            const int fl = wave + NWV * i;
-            if (fl >= LCOLS / 2) break;
-            double re = xv[i].x, im = xv[i].y;
+      for (int sc = 0; sc < NSUB; ++sc) {
+        if (sc >= nsub) break;  // uniform over the workgroup
+        const int j = SUB * sc + jl;
+        if (j < kpad) {
+#pragma unroll
+          for (int p = 0; p < FL; ++p) {
+            const int fl = wave + NWV * (fq + NG * p);
+            double re = xv[sc][p].x, im = xv[sc][p].y;
             if (mul_i) { double t = re; re = -im; im = t; }
             re *= sgn;
             im *= sgn;
-            X[(2 * fl) * KS1 + jp] = to_lds<PREC>(re);
-            X[(2 * fl + 1) * KS1 + jp] = to_lds<PREC>(im);
+            X[(2 * fl) * KS1 + j] = to_lds<PREC>(re);
+            X[(2 * fl + 1) * KS1 + j] = to_lds<PREC>(im);
           }
 #pragma unroll
-          for (int i = 0; i < RPT; ++i) {
+          for (int p = 0; p < RL; ++p) {
+            const int i = fq + NG * p;
             const int k = wave + NWV * i;
-            if (k < TROWS) Pt[k * KS1 + jp] = pv[i];
+            if (i < RPT && k < TROWS) Pt[k * KS1 + j] = pv[sc][p];
           }
         }
+        __syncthreads();
+        const int ks0 = SUB * sc / KSTEP, ks1 = min(KSc, SUB * (sc + 1) / KSTEP);
+        mfma_run_steps<PREC>(
+            acc, run, ntile, ks0, ks1, [&](int ks, int e) { return Xc[KSTEP * ks + e]; },
+            [&](int t, int ks, int e) { return Pc[16 * t * KS1 + KSTEP * ks + e]; });
       }
-      __syncthreads();
-      mfma_accumulate<PREC>(
-          acc, ntile, KSc, [&](int ks, int e) { return Xc[KSTEP * ks + e]; },
-          [&](int t, int ks, int e) { return Pc[16 * t * KS1 + KSTEP * ks + e]; });
+      mfma_run_end<PREC>(acc, run, KSc);
     }
   }
   const int col = 16 * wave + li;
@@ -1261,95 +1316,166 @@ __device__ __forceinline__ void legendre_inv_tile(const SegMap& in, double2* __r
   const int seg_op = (in.flag[seg_a] >> 1) & 7;
   int chunk_ops = 0;  // operators of the segments the chunk touches (bit 1 << op)
   for (int q = seg_a; q <= seg_b; ++q) chunk_ops |= 1 << ((in.flag[q] >> 1) & 7);
-  for (int kk0 = 0; kk0 < nr; kk0 += KC) {
-    const int kcr = min(KC, nr - kk0);
-    const int KSc = (kcr + KSTEP - 1) / KSTEP;
-    const int kpad = KSTEP * KSc;
-    __syncthreads();
-    {
-      // coefficients: lane = contraction row of the chunk (kpad <= KC = 64),
-      // the wave picks the fields (uniform segment lookup, no index division)
-      constexpr int NWV = LTHREADS / 64;
-      constexpr int FPT = (LCOLS / 2 + NWV - 1) / NWV;
-      const int kr = lane;
-      const bool kin = kr < kcr;
-      const int nrow = nfirst + 2 * (kk0 + kr);
-      double2 sv[FPT];
-      if constexpr (PLAIN) {
-        // the plain transform of one array (the common case): direct loads
-        const double2* h0 = reinterpret_cast<const double2*>(in.ptr[0]);
+  if constexpr (PLAIN && PREC == P_FP64) {
+    // Plain FP64 transforms stage in sub-chunks of SUBI rows: every load of a
+    // chunk is issued first (lane = row jl of the sub-chunk and field group
+    // fq), each sub-chunk goes to LDS when its own loads are in, and its MFMA
+    // steps run while the later sub-chunks are still arriving.
+    constexpr int SUBI = 16, NSUBI = KC / SUBI, NGI = 64 / SUBI;
+    constexpr int NWV = LTHREADS / 64;
+    constexpr int FPT = (LCOLS / 2 + NWV - 1) / NWV;
+    constexpr int FLI = FPT / NGI;
+    constexpr int TPS = (SUBI * TROWS + LTHREADS - 1) / LTHREADS;  // table loads per sub-chunk
+    static_assert(FPT == NGI * FLI, "fields split evenly over the lane groups");
+    const int jl = lane & (SUBI - 1), fq = lane / SUBI;
+    const double2* h0 = reinterpret_cast<const double2*>(in.ptr[0]);
+    for (int kk0 = 0; kk0 < nr; kk0 += KC) {
+      const int kcr = min(KC, nr - kk0);
+      const int KSc = (kcr + KSTEP - 1) / KSTEP;
+      const int kpad = KSTEP * KSc;
+      const int nsub = (kcr + SUBI - 1) / SUBI;
+      __syncthreads();
+      double2 sv[NSUBI][FLI];
+      TT pv[NSUBI][TPS];
 #pragma unroll
-        for (int i = 0; i < FPT; ++i) {
-          const int fl = wave + NWV * i;
-          sv[i] = (fl < nf && kin)
-                      ? h0[(size_t)(f0 + fl) * (spec_d / 2) + (size_t)nrow * nm + m]
-                      : make_double2(0.0, 0.0);
+      for (int sc = 0; sc < NSUBI; ++sc) {
+        const int kr = SUBI * sc + jl;
+        const bool kin = sc < nsub && kr < kcr;
+        const int nrow = nfirst + 2 * (kk0 + kr);
+#pragma unroll
+        for (int p = 0; p < FLI; ++p) {
+          const int fl = wave + NWV * (fq + NGI * p);
+          sv[sc][p] = (fl < nf && kin)
+                          ? h0[(size_t)(f0 + fl) * (spec_d / 2) + (size_t)nrow * nm + m]
+                          : make_double2(0.0, 0.0);
         }
-      } else if (seg_one) {
-        // the chunk's fields lie in one array: pointer arithmetic per field,
-        // one straight loop per operator (the operator is the workgroup's)
-        const OpRow orow = op_row(ops, 1 << seg_op, m, kin ? nrow : m, ns);
-        auto stage_op = [&](auto opc) {
-          constexpr int OP = decltype(opc)::value;
 #pragma unroll
+        for (int i = 0; i < TPS; ++i) {
+          const int idx = (int)threadIdx.x + LTHREADS * i;
+          const int j = idx % TROWS, k = SUBI * sc + idx / TROWS;
+          const bool ok = sc < nsub && idx < SUBI * TROWS && k < kcr && j < nlg;
+          pv[sc][i] = tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * (kk0 + k)) * half + j : 0, ok);
+        }
+      }
+      MfmaRun<PREC> run;
+      mfma_run_begin<PREC>(run);
+#pragma unroll
+      for (int sc = 0; sc < NSUBI; ++sc) {
+        if (sc >= nsub) break;  // uniform over the workgroup
+        const int kr = SUBI * sc + jl;
+        if (kr < kpad) {
+#pragma unroll
+          for (int p = 0; p < FLI; ++p) {
+            const int fl = wave + NWV * (fq + NGI * p);
+            S[(2 * fl) * KS1 + kr] = to_lds<PREC>(sv[sc][p].x);
+            S[(2 * fl + 1) * KS1 + kr] = to_lds<PREC>(sv[sc][p].y);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < TPS; ++i) {
+          const int idx = (int)threadIdx.x + LTHREADS * i;
+          const int k = SUBI * sc + idx / TROWS;
+          if (idx < SUBI * TROWS && k < kpad) Pt[k * PSI + idx % TROWS] = pv[sc][i];
+        }
+        __syncthreads();
+        const int ks0 = SUBI * sc / KSTEP, ks1 = min(KSc, SUBI * (sc + 1) / KSTEP);
+        mfma_run_steps<PREC>(
+            acc, run, nlt, ks0, ks1, [&](int ks, int e) { return Sc[KSTEP * ks + e]; },
+            [&](int t, int ks, int e) { return Pc[(KSTEP * ks + e) * PSI + 16 * t]; });
+      }
+      mfma_run_end<PREC>(acc, run, KSc);
+    }
+  } else {
+    for (int kk0 = 0; kk0 < nr; kk0 += KC) {
+      const int kcr = min(KC, nr - kk0);
+      const int KSc = (kcr + KSTEP - 1) / KSTEP;
+      const int kpad = KSTEP * KSc;
+      __syncthreads();
+      {
+        // coefficients: lane = contraction row of the chunk (kpad <= KC = 64),
+        // the wave picks the fields (uniform segment lookup, no index division)
+        constexpr int NWV = LTHREADS / 64;
+        constexpr int FPT = (LCOLS / 2 + NWV - 1) / NWV;
+        const int kr = lane;
+        const bool kin = kr < kcr;
+        const int nrow = nfirst + 2 * (kk0 + kr);
+        double2 sv[FPT];
+        if constexpr (PLAIN) {
+          // the plain transform of one array (the common case): direct loads
+          const double2* h0 = reinterpret_cast<const double2*>(in.ptr[0]);
+  #pragma unroll
           for (int i = 0; i < FPT; ++i) {
             const int fl = wave + NWV * i;
-            const size_t o = (size_t)fl * (spec_d / 2);
             sv[i] = (fl < nf && kin)
-                        ? staged_coefficient_row(seg_h + o, seg_h2 ? seg_h2 + o : nullptr, OP,
-                                                 orow, nrow, m, nm, ns)
+                        ? h0[(size_t)(f0 + fl) * (spec_d / 2) + (size_t)nrow * nm + m]
                         : make_double2(0.0, 0.0);
           }
-        };
-        switch (seg_op) {
-          case SPEC_OP_NONE: stage_op(std::integral_constant<int, SPEC_OP_NONE>{}); break;
-          case SPEC_OP_GRAD_LAMBDA: stage_op(std::integral_constant<int, SPEC_OP_GRAD_LAMBDA>{}); break;
-          case SPEC_OP_GRAD_THETA: stage_op(std::integral_constant<int, SPEC_OP_GRAD_THETA>{}); break;
-          case SPEC_OP_UCOS: stage_op(std::integral_constant<int, SPEC_OP_UCOS>{}); break;
-          case SPEC_OP_VCOS: stage_op(std::integral_constant<int, SPEC_OP_VCOS>{}); break;
-          case SPEC_OP_NEG_I: stage_op(std::integral_constant<int, SPEC_OP_NEG_I>{}); break;
-          default: stage_op(std::integral_constant<int, SPEC_OP_NEG>{}); break;
+        } else if (seg_one) {
+          // the chunk's fields lie in one array: pointer arithmetic per field,
+          // one straight loop per operator (the operator is the workgroup's)
+          const OpRow orow = op_row(ops, 1 << seg_op, m, kin ? nrow : m, ns);
+          auto stage_op = [&](auto opc) {
+            constexpr int OP = decltype(opc)::value;
+  #pragma unroll
+            for (int i = 0; i < FPT; ++i) {
+              const int fl = wave + NWV * i;
+              const size_t o = (size_t)fl * (spec_d / 2);
+              sv[i] = (fl < nf && kin)
+                          ? staged_coefficient_row(seg_h + o, seg_h2 ? seg_h2 + o : nullptr, OP,
+                                                   orow, nrow, m, nm, ns)
+                          : make_double2(0.0, 0.0);
+            }
+          };
+          switch (seg_op) {
+            case SPEC_OP_NONE: stage_op(std::integral_constant<int, SPEC_OP_NONE>{}); break;
+            case SPEC_OP_GRAD_LAMBDA: stage_op(std::integral_constant<int, SPEC_OP_GRAD_LAMBDA>{}); break;
+            case SPEC_OP_GRAD_THETA: stage_op(std::integral_constant<int, SPEC_OP_GRAD_THETA>{}); break;
+            case SPEC_OP_UCOS: stage_op(std::integral_constant<int, SPEC_OP_UCOS>{}); break;
+            case SPEC_OP_VCOS: stage_op(std::integral_constant<int, SPEC_OP_VCOS>{}); break;
+            case SPEC_OP_NEG_I: stage_op(std::integral_constant<int, SPEC_OP_NEG_I>{}); break;
+            default: stage_op(std::integral_constant<int, SPEC_OP_NEG>{}); break;
+          }
+        } else {
+          // fields from several arrays: the tables of every operator in the
+          // chunk are read once per lane, the array is resolved per field
+          const OpRow orow = op_row(ops, chunk_ops, m, kin ? nrow : m, ns);
+  #pragma unroll
+          for (int i = 0; i < FPT; ++i) {
+            const int fl = wave + NWV * i;
+            sv[i] = (fl < nf && kin) ? staged_coefficient_seg(in, orow, f0 + fl, spec_d, nrow, m, nm, ns)
+                                     : make_double2(0.0, 0.0);
+          }
         }
-      } else {
-        // fields from several arrays: the tables of every operator in the
-        // chunk are read once per lane, the array is resolved per field
-        const OpRow orow = op_row(ops, chunk_ops, m, kin ? nrow : m, ns);
-#pragma unroll
-        for (int i = 0; i < FPT; ++i) {
-          const int fl = wave + NWV * i;
-          sv[i] = (fl < nf && kin) ? staged_coefficient_seg(in, orow, f0 + fl, spec_d, nrow, m, nm, ns)
-                                   : make_double2(0.0, 0.0);
+        // table rows: idx -> (latitude, row) by shifts (TROWS is a power of two)
+        constexpr int TPT = (KC * TROWS + LTHREADS - 1) / LTHREADS;
+        TT pv[TPT];
+  #pragma unroll
+        for (int i = 0; i < TPT; ++i) {
+          const int idx = (int)threadIdx.x + LTHREADS * i;
+          const int j = idx % TROWS, k = idx / TROWS;
+          const bool ok = k < kcr && j < nlg;
+          pv[i] = tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * (kk0 + k)) * half + j : 0, ok);
+        }
+        if (kr < kpad) {
+  #pragma unroll
+          for (int i = 0; i < FPT; ++i) {
+            const int fl = wave + NWV * i;
+            if (fl >= LCOLS / 2) break;
+            S[(2 * fl) * KS1 + kr] = to_lds<PREC>(sv[i].x);
+            S[(2 * fl + 1) * KS1 + kr] = to_lds<PREC>(sv[i].y);
+          }
+        }
+  #pragma unroll
+        for (int i = 0; i < TPT; ++i) {
+          const int idx = (int)threadIdx.x + LTHREADS * i;
+          if (idx < kpad * TROWS) Pt[(idx / TROWS) * PSI + idx % TROWS] = pv[i];
         }
       }
-      // table rows: idx -> (latitude, row) by shifts (TROWS is a power of two)
-      constexpr int TPT = (KC * TROWS + LTHREADS - 1) / LTHREADS;
-      TT pv[TPT];
-#pragma unroll
-      for (int i = 0; i < TPT; ++i) {
-        const int idx = (int)threadIdx.x + LTHREADS * i;
-        const int j = idx % TROWS, k = idx / TROWS;
-        const bool ok = k < kcr && j < nlg;
-        pv[i] = tab_load<PREC>(tab, ok ? rowbase + (size_t)(2 * (kk0 + k)) * half + j : 0, ok);
-      }
-      if (kr < kpad) {
-#pragma unroll
-        for (int i = 0; i < FPT; ++i) {
-          const int fl = wave + NWV * i;
-          if (fl >= LCOLS / 2) break;
-          S[(2 * fl) * KS1 + kr] = to_lds<PREC>(sv[i].x);
-          S[(2 * fl + 1) * KS1 + kr] = to_lds<PREC>(sv[i].y);
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < TPT; ++i) {
-        const int idx = (int)threadIdx.x + LTHREADS * i;
-        if (idx < kpad * TROWS) Pt[(idx / TROWS) * PSI + idx % TROWS] = pv[i];
-      }
+      __syncthreads();
+      mfma_accumulate<PREC>(
+          acc, nlt, KSc, [&](int ks, int e) { return Sc[KSTEP * ks + e]; },
+          [&](int t, int ks, int e) { return Pc[(KSTEP * ks + e) * PSI + 16 * t]; });
     }
-    __syncthreads();
-    mfma_accumulate<PREC>(
-        acc, nlt, KSc, [&](int ks, int e) { return Sc[KSTEP * ks + e]; },
-        [&](int t, int ks, int e) { return Pc[(KSTEP * ks + e) * PSI + 16 * t]; });
   }
   // The tile goes through LDS ([field][latitude] complex) so that each
   // field's latitudes leave as one contiguous run of whole lines.
