@@ -170,12 +170,13 @@ def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):
     """Held-Suarez primitive-equation step (Atmosphere_Update!) on the device,
     T85 L20 with Δt = 300 s by default; isothermal 300 K at rest, ps = 1e5 Pa,
     seeded N(0, 0.1 K) temperature noise.  Later steps are captured into HIP
-    graphs and replayed: one step per graph, and four steps per graph (the
-    launch overhead between graphs paid once per four steps; the headline).
-    The timed steps carry the next step's gradient transforms in their closing
-    inverse batch (Atmosphere_Update carry_gradients, bit-identical results);
-    the same four-step graphs without carrying are timed beside them.
-    Returns ms/step and sim-days/day."""
+    graphs and replayed.  The headline: graphs of twelve steps that carry the
+    next step's gradient transforms in their closing inverse batch
+    (carry_gradients) and advance the state by renaming p, c, n (rotate);
+    both leave p and c bit-identical to the reference-ordered step.  Beside
+    it: four-step graphs with the copying Time_Advance!, the same without
+    carried gradients, and one-step graphs.  Returns ms/step and
+    sim-days/day."""
     nlon = 2 * nlat
     a = 6371000.0
     mesh = jgcm.Spectral_Spherical_Mesh(N, N + 1, nlat, nlon, nd, a, precision=precision)
@@ -193,9 +194,9 @@ def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):
     jgcm.Update_Init_Step(si)
     stream = torch.cuda.ExternalStream(mesh.stream())
 
-    def timed(repeat, launches, carry=True):
+    def timed(repeat, launches, carry=True, rotate=False):
         step = lambda: jgcm.Atmosphere_Update(mesh, atmo, vc, si, d, HS_PARAMS,
-                                              carry_gradients=carry)
+                                              carry_gradients=carry, rotate=rotate)
         step()   # an eager step of the same kind before the capture (carried gradients)
         g = jgcm.Step_Graph(mesh, step, nfields=8 * nd + 4, repeat=repeat)
         for _ in range(max(1, warmup // repeat)):
@@ -212,11 +213,14 @@ def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):
 
     ms1 = timed(1, steps)
     ms_plain = timed(4, max(1, steps // 4), carry=False)
-    ms = timed(4, max(1, steps // 4))
+    ms_copy = timed(4, max(1, steps // 4))
+    ms = timed(12, max(1, steps // 12), rotate=True)
     t = d.grid_t_c
     finite = bool(torch.isfinite(t).all())
     return {"ms_per_step": ms, "sim_days_per_day": dt / (ms * 1e-3), "dt_s": dt,
-            "steps_per_graph": 4, "ms_per_step_one_step_graphs": ms1,
+            "steps_per_graph": 12, "time_advance": "rotated (p, c, n renamed)",
+            "ms_per_step_copying_time_advance_four_step_graphs": ms_copy,
+            "ms_per_step_one_step_graphs": ms1,
             "ms_per_step_recomputed_gradients": ms_plain,
             "config": f"T{N} L{nd} ({nlon}x{nlat})", "finite": finite,
             "mean_T": float(t.mean())}

@@ -847,66 +847,13 @@ __global__ __launch_bounds__(512) void hs_corrections_sums_kernel(
   }
 }
 
-// Compute_Corrections! (SD:30-68).  Every workgroup first finishes the area
-// means from the per-workgroup partials of the physics pass (Σ w ps_p, Σ w Σ E_p Δp)
-// and of the sums pass, in the same fixed order in every workgroup (so all
-// agree, bit for bit, and no separate reduction launch is needed); workgroup 0
-// stores the scalars: 0 mean ps_p, 1 mean energy_p, 2 mean ps_n, 3 factor f,
-// 4 log f, 5 mean energy_n, 6 temperature correction.  Then ps_n *= f,
-// T_n += corr, and the (0, 0) spectral coefficients of ln ps_n and T_n.
 constexpr int APPLY_THREADS = 256;
-__global__ __launch_bounds__(APPLY_THREADS) void hs_apply_corrections_kernel(
-    int nd, size_t P, size_t spec_per_level, int nlon, double* t_n, double* ps_n,
-    double2* spe_t_n, double2* spe_lnps_n, const double* part_p, const double* part_n, int nparts,
-    double* scal, double grav, double cp_air, int mass, int energy) {
-  __shared__ double red[5][APPLY_THREADS];
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < nparts; i += APPLY_THREADS) {
-    acc[0] = acc[0] + part_p[2 * (size_t)i];
-    acc[1] = acc[1] + part_p[2 * (size_t)i + 1];
-    acc[2] = acc[2] + part_n[3 * (size_t)i];
-    acc[3] = acc[3] + part_n[3 * (size_t)i + 1];
-    acc[4] = acc[4] + part_n[3 * (size_t)i + 2];
-  }
-#pragma unroll
-  for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = acc[q];
-  __syncthreads();
-  for (int st = APPLY_THREADS / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st)
-#pragma unroll
-      for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + st];
-    __syncthreads();
-  }
-  const double mps_p = red[0][0] / (2.0 * nlon);
-  const double e_p = red[1][0] / (2.0 * nlon) / grav;
-  const double mps_n = red[2][0] / (2.0 * nlon);
-  const double f = mass ? mps_p / mps_n : 1.0;
-  const double e_n = (red[3][0] + f * red[4][0]) / (2.0 * nlon) / grav;
-  const double corr = energy ? grav * (e_p - e_n) / (cp_air * mps_p) : 0.0;
-  size_t n = (size_t)nd * P;
-  GRID_STRIDE(i, n) {
-    if (energy) t_n[i] = t_n[i] + corr;
-    if (mass && i < P) ps_n[i] = ps_n[i] * f;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    scal[0] = mps_p;
-    if (energy) scal[1] = e_p;
-    if (mass) {
-      scal[2] = mps_n;
-      scal[3] = f;
-      scal[4] = log(f);
-      spe_lnps_n[0].x = spe_lnps_n[0].x + scal[4];
-    }
-    if (energy) {
-      scal[5] = e_n;
-      scal[6] = corr;
-      for (int k = 0; k < nd; ++k) spe_t_n[k * spec_per_level].x = spe_t_n[k * spec_per_level].x + corr;
-    }
-  }
-}
 
-// The correction means (every workgroup that needs them, same fixed order as
-// hs_apply_corrections_kernel): out = {mean ps_p, energy_p, mean ps_n, f, e_n, corr}.
+// The correction means from the per-workgroup partials of the physics pass
+// (Σ w ps_p, Σ w Σ E_p Δp) and of the sums pass, reduced by every workgroup
+// that needs them in the same fixed order (so all agree, bit for bit, and no
+// separate reduction launch is needed): out = {mean ps_p, energy_p, mean
+// ps_n, f, e_n, corr}.
 __device__ void correction_scalars(const double* part_p, const double* part_n, int nparts,
                                    int nlon, double grav, double cp_air, int mass, int energy,
                                    double (*red)[APPLY_THREADS], double* out) {
@@ -918,20 +865,28 @@ __device__ void correction_scalars(const double* part_p, const double* part_n, i
     acc[3] = acc[3] + part_n[3 * (size_t)i + 1];
     acc[4] = acc[4] + part_n[3 * (size_t)i + 2];
   }
+  // butterfly sums within each wavefront (every lane ends with the same
+  // value), then the wavefronts' sums in wave order: one barrier
 #pragma unroll
-  for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = acc[q];
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) acc[q] = acc[q] + __shfl_xor(acc[q], off, 64);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) red[q][wave] = acc[q];
   __syncthreads();
-  for (int st = APPLY_THREADS / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st)
+  double tot[5];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + st];
-    __syncthreads();
+  for (int q = 0; q < 5; ++q) {
+    tot[q] = red[q][0];
+    for (int w = 1; w < APPLY_THREADS / 64; ++w) tot[q] = tot[q] + red[q][w];
   }
-  const double mps_p = red[0][0] / (2.0 * nlon);
-  const double e_p = red[1][0] / (2.0 * nlon) / grav;
-  const double mps_n = red[2][0] / (2.0 * nlon);
+  const double mps_p = tot[0] / (2.0 * nlon);
+  const double e_p = tot[1] / (2.0 * nlon) / grav;
+  const double mps_n = tot[2] / (2.0 * nlon);
   const double f = mass ? mps_p / mps_n : 1.0;
-  const double e_n = (red[3][0] + f * red[4][0]) / (2.0 * nlon) / grav;
+  const double e_n = (tot[3] + f * tot[4]) / (2.0 * nlon) / grav;
   out[0] = mps_p;
   out[1] = e_p;
   out[2] = mps_n;
@@ -952,6 +907,61 @@ constexpr int MAXCADV = 16;
 #ifndef JGCM_CADV_PER_THREAD
 #define JGCM_CADV_PER_THREAD 4  // elements per thread of hs_correct_advance_kernel
 #endif
+// Compute_Corrections! on the new state in place (SD:30-68): T_n += corr and
+// ps_n *= f on the grid, the (0, 0) coefficient of every level of T_n += corr
+// and that of ln ps_n += log f.  Workgroup b covers elements b W .. (b+1) W - 1
+// of T_n followed by ps_n (W = JGCM_CADV_PER_THREAD x APPLY_THREADS); it
+// requests them before reducing the partials (both round trips overlap) and
+// then writes them; workgroup 0 also updates the spectral coefficients and
+// stores the seven scalars.  Every workgroup reduces the partials in the same
+// fixed order (correction_scalars), and the corrections are those of
+// hs_correct_advance_kernel, so both give the same values bit for bit.
+__global__ __launch_bounds__(APPLY_THREADS) void hs_apply_corrections_kernel(
+    int nd, size_t P, size_t spec_per_level, int nlon, double* t_n, double* ps_n,
+    double2* spe_t_n, double2* spe_lnps_n, const double* part_p, const double* part_n, int nparts,
+    double* scal, double grav, double cp_air, int mass, int energy) {
+  __shared__ double red[5][APPLY_THREADS];
+  constexpr int IT = JGCM_CADV_PER_THREAD;
+  const size_t nT = energy ? (size_t)nd * P : 0;
+  const size_t total = nT + (mass ? P : 0);
+  const size_t lo = (size_t)blockIdx.x * IT * APPLY_THREADS;
+  double x[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t i = lo + threadIdx.x + (size_t)it * APPLY_THREADS;
+    x[it] = i < nT ? t_n[i] : (i < total ? ps_n[i - nT] : 0.0);
+  }
+  double sc[6];
+  correction_scalars(part_p, part_n, nparts, nlon, grav, cp_air, mass, energy, red, sc);
+  const double f = sc[3], corr = sc[5];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t i = lo + threadIdx.x + (size_t)it * APPLY_THREADS;
+    if (i < nT) t_n[i] = x[it] + corr;
+    else if (i < total) ps_n[i - nT] = x[it] * f;
+  }
+  if (blockIdx.x != 0) return;
+  // one level's (0, 0) coefficient per thread: a loop in one thread would wait
+  // for each store before the next load (the compiler cannot rule out aliasing)
+  if (energy)
+    for (int k = threadIdx.x; k < nd; k += APPLY_THREADS)
+      spe_t_n[k * spec_per_level].x = spe_t_n[k * spec_per_level].x + corr;
+  if (threadIdx.x == 0) {
+    scal[0] = sc[0];
+    if (energy) scal[1] = sc[1];
+    if (mass) {
+      scal[2] = sc[2];
+      scal[3] = f;
+      scal[4] = log(f);
+      spe_lnps_n[0].x = spe_lnps_n[0].x + scal[4];
+    }
+    if (energy) {
+      scal[5] = sc[4];
+      scal[6] = corr;
+    }
+  }
+}
+
 struct CorrAdvanceSet {
   double* p[MAXCADV];     // NULL: p already holds the filtered c; only c = n remains
   double* c[MAXCADV];
@@ -2186,8 +2196,9 @@ int jgcm_hs_corrections(jgcm_semi_implicit_t* si, const double* grid_lnps, doubl
                      grid_v_n, grid_t_n, part_n, mass || energy, energy);
   JGCM_LAUNCH_CHECK();
   if (mass || energy) {
-    size_t n = (size_t)si->nd * P;
-    int blocks = (int)std::min<size_t>((n + APPLY_THREADS - 1) / APPLY_THREADS, 256);
+    const size_t total = (energy ? (size_t)si->nd * P : 0) + (mass ? P : 0);
+    const size_t per_block = (size_t)JGCM_CADV_PER_THREAD * APPLY_THREADS;
+    const int blocks = (int)((total + per_block - 1) / per_block);
     hipLaunchKernelGGL(hs_apply_corrections_kernel, dim3(blocks), dim3(APPLY_THREADS), 0, M->stream,
                        si->nd, P, M->spec_elems(), M->nlon, grid_t_n, grid_ps_n, (double2*)spe_t_n,
                        (double2*)spe_lnps_n, part_p, part_n, nparts, si->d_scal, si->atmo.grav,

@@ -147,6 +147,30 @@ for _f in HEIGHT_FIELDS:
     setattr(Dyn_Data, _f, _deferred_field(_f))
 
 
+# The prognostic arrays that Time_Advance! advances (Dyn_Data.jl:326-364), in
+# its order; a rotated step (Atmosphere_Update(..., rotate=True)) renames
+# their p, c, n arrays instead of copying them.
+ROTATED_STATE = ["spe_vor", "spe_div", "spe_lnps", "spe_t", "grid_u", "grid_v", "grid_ps",
+                 "grid_t"]
+
+
+def _rotate_state(d):
+    """p <- c, c <- n by renaming the ROTATED_STATE arrays: the array that held
+    p becomes the new n (its values are stale until the next step writes
+    it).  While a graph is being captured on the state's mesh the rotation is
+    logged, so that the graph can check that its steps return the names to
+    where they started (Step_Graph)."""
+    for f in ROTATED_STATE:
+        p, c, n = getattr(d, f + "_p"), getattr(d, f + "_c"), getattr(d, f + "_n")
+        setattr(d, f + "_p", c)
+        setattr(d, f + "_c", n)
+        setattr(d, f + "_n", p)
+    d._rotation = (getattr(d, "_rotation", 0) + 1) % 3
+    mesh = d.mesh
+    if getattr(mesh, "_capturing", False):
+        mesh._rotation_log = getattr(mesh, "_rotation_log", []) + [d]
+
+
 def _tracer_names(d):
     return [f"{f}_tracers" for f in ("spe", "grid") if getattr(d, f"num_{f}_tracters") > 0]
 

@@ -20,7 +20,7 @@ from .mesh import (_ptr, Trans_Grid_To_Spherical, Trans_Spherical_To_Grid,
                    Trans_Grid_To_Spherical_Batch, Trans_Spherical_To_Grid_Batch, BATCH_DIVCOS,
                    BATCH_VORDIV, INV_GRAD_LAMBDA, INV_GRAD_THETA, INV_UCOS, INV_VCOS,
                    Vor_Div_From_Grid_UV, UV_Grid_From_Vor_Div)
-from .dyn_data import Time_Advance_Tracers
+from .dyn_data import ROTATED_STATE, Time_Advance_Tracers, _rotate_state
 from .time_integrator import Get_Δt
 
 _VC = {"even_sigma": 0, "uneven_sigma": 1, "hybrid": 2, "mcm": 3, "v197": 4}
@@ -256,7 +256,8 @@ def _scratch(mesh, d):
 
 
 def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
-                      pressure_variables=True, physics_params=None, carry_gradients=False):
+                      pressure_variables=True, physics_params=None, carry_gradients=False,
+                      rotate=False):
     """Spectral_Dynamics.jl:206-365.  The transforms of the step are batched:
     the gradients of ln ps and T (:259, :293) go back to the grid in one
     launch pair with the spectral gradient operators applied on the way in,
@@ -280,7 +281,16 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
     coefficients that Compute_Corrections! adjusts, which the gradient
     operators multiply by exact zeros, so the results are the same bit for
     bit.  Every step of a run must then carry, and nothing may change the
-    state between the steps."""
+    state between the steps.
+
+    ``rotate`` (with ``physics_params``): Time_Advance! (:346) renames the p,
+    c, n arrays of the state (dyn_data.ROTATED_STATE) instead of copying
+    them: the Robert-filtered c is written in place (as Filtered_Leapfrog!
+    does), Compute_Corrections! adjusts n in place, and then p <- c, c <- n
+    by name.  p and c end as the copying step leaves them, bit for bit; the
+    n arrays hold stale values afterwards (the reference's n keeps the new
+    state), which nothing in a step reads before writing it.  Graphs of
+    rotated steps need a multiple of 3 steps (Step_Graph checks)."""
     d, si = dyn_data, semi_implicit
     integrator = si.integrator
     Δt = Get_Δt(integrator)
@@ -288,6 +298,7 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
     h = mesh.handle
     fused = physics_params is not None
     assert fused or not carry_gradients, "carried gradients need the fused physics pass"
+    assert fused or not rotate, "the rotated Time_Advance! needs the fused physics pass"
     carried = carry_gradients and getattr(d, "_carried_gradients", False)
     d._carried_gradients = False
     if not fused and not si._corrections_ready:
@@ -318,7 +329,7 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
         # with the p = c half of :346 Time_Advance! for the grid u, v, T, ps
         carry_in = (sc.carry_dl_lnps, sc.carry_dt_lnps) if carried else (None, None)
         call("jgcm_hs_physics_column_dynamics_ex", si.handle, ctypes.byref(_hs(physics_params)),
-             int(Δt), *map(P, args + carry_in), 1)
+             int(Δt), *map(P, args + carry_in), 0 if rotate else 1)
     else:
         call("jgcm_hs_column_dynamics", si.handle, *map(P, (                   # :260-307
             d.grid_div, d.grid_u_c, d.grid_v_c, d.grid_ps_c, None, None,
@@ -332,11 +343,17 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
     # pass :319-328 Compute_Spectral_Damping! (vor, div, T) and Filtered_Leapfrog!
     # (vor, div, lnps, T)
     # and the spectral part of :346 Time_Advance! (p = the filtered c, c = n)
-    P3 = ctypes.c_void_p * 3
-    lev = lambda f: P3(*(getattr(d, f + s).data_ptr() for s in ("_p", "_c", "_n")))
-    call("jgcm_hs_implicit_leapfrog_advance", si.handle, *map(_ptr, (
-        d.spe_energy, d.spe_δdiv, d.spe_δlnps, d.spe_δt, d.spe_δvor)),
-        lev("spe_vor"), lev("spe_div"), lev("spe_lnps"), lev("spe_t"))
+    if rotate:   # the filtered c in place (Filtered_Leapfrog!), renamed to p below
+        call("jgcm_hs_energy_implicit_leapfrog", si.handle, *map(_ptr, (
+            d.spe_energy, d.spe_div_c, d.spe_div_p, d.spe_lnps_c, d.spe_lnps_p, d.spe_t_c,
+            d.spe_t_p, d.spe_δdiv, d.spe_δlnps, d.spe_δt, d.spe_δvor, d.spe_vor_p, d.spe_vor_c,
+            d.spe_vor_n, d.spe_div_n, d.spe_lnps_n, d.spe_t_n)))
+    else:
+        P3 = ctypes.c_void_p * 3
+        lev = lambda f: P3(*(getattr(d, f + s).data_ptr() for s in ("_p", "_c", "_n")))
+        call("jgcm_hs_implicit_leapfrog_advance", si.handle, *map(_ptr, (
+            d.spe_energy, d.spe_δdiv, d.spe_δlnps, d.spe_δt, d.spe_δvor)),
+            lev("spe_vor"), lev("spe_div"), lev("spe_lnps"), lev("spe_t"))
     # :331-335: vor, div, (u, v) = UV_Grid_From_Vor_Div!, T, ln ps back to the grid
     # (carrying: with the next step's gradients of ln ps and T)
     snms = [d.spe_vor_n, d.spe_div_n, d.spe_vor_n, d.spe_vor_n, d.spe_t_n, d.spe_lnps_n]
@@ -350,18 +367,24 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
     Trans_Spherical_To_Grid_Batch(mesh, snms, grids, flags,
                                   snms_b=[None, None, d.spe_div_n, d.spe_div_n] +
                                   [None] * (len(snms) - 4))
-    # :336-343 Compute_Corrections! and :346 Time_Advance!, the corrections applied
-    # to the new fields while they are copied (one launch after the sums)
-    names = [f"spe_{f}" for f in ("vor", "div", "lnps", "t")] + \
-        [f"grid_{f}" for f in ("u", "v", "ps", "t")]
-    arrs = [[getattr(d, f + s) for f in names] for s in ("_p", "_c", "_n")]
-    ptrs = [(ctypes.c_void_p * 8)(*[t.data_ptr() for t in ts]) for ts in arrs]
-    # p = c done above for the spectral fields and (fused) the grid fields: c = n and
-    # the corrections remain
-    for k in range(8 if fused else 4):
-        ptrs[0][k] = None
-    counts = (ctypes.c_size_t * 8)(*[t.numel() * t.element_size() // 8 for t in arrs[1]])
-    call("jgcm_hs_corrections_advance", si.handle, _ptr(d.grid_lnps), *ptrs, counts)
+    if rotate:
+        # :336-343 Compute_Corrections! on n in place, :346 Time_Advance! by renaming
+        call("jgcm_hs_corrections", si.handle, *map(_ptr, (
+            d.grid_lnps, d.grid_ps_n, d.grid_u_n, d.grid_v_n, d.grid_t_n, d.spe_lnps_n,
+            d.spe_t_n)))
+        _rotate_state(d)
+    else:
+        # :336-343 Compute_Corrections! and :346 Time_Advance!, the corrections applied
+        # to the new fields while they are copied (one launch after the sums)
+        names = ROTATED_STATE
+        arrs = [[getattr(d, f + s) for f in names] for s in ("_p", "_c", "_n")]
+        ptrs = [(ctypes.c_void_p * 8)(*[t.data_ptr() for t in ts]) for ts in arrs]
+        # p = c done above for the spectral fields and (fused) the grid fields: c = n and
+        # the corrections remain
+        for k in range(8 if fused else 4):
+            ptrs[0][k] = None
+        counts = (ctypes.c_size_t * 8)(*[t.numel() * t.element_size() // 8 for t in arrs[1]])
+        call("jgcm_hs_corrections_advance", si.handle, _ptr(d.grid_lnps), *ptrs, counts)
     Time_Advance_Tracers(d)
     d._carried_gradients = carry_gradients
     if pressure_variables:
@@ -370,7 +393,8 @@ def Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
 
 
 def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, physics_params,
-                      overlap=False, fused=True, eager_heights=False, carry_gradients=False):
+                      overlap=False, fused=True, eager_heights=False, carry_gradients=False,
+                      rotate=False):
     """Spectral_Dynamics.jl:470-490.  By default (``fused``) the physics pass
     (HS forcing on the p-fields, the correction partials) runs inside the
     step's column pass, after the gradient transforms, its tendencies kept in
@@ -385,13 +409,18 @@ def Atmosphere_Update(mesh, atmo_data, vert_coord, semi_implicit, dyn_data, phys
     the reference's order.  ``carry_gradients`` (fused only): each step's
     closing inverse batch also forms the next step's gradients of ln ps and T
     (Spectral_Dynamics); for runs whose steps all carry and that do not touch
-    the state between steps (driver loops, captured multi-step graphs)."""
+    the state between steps (driver loops, captured multi-step graphs).
+    ``rotate`` (fused only): Time_Advance! by renaming the state's p, c, n
+    arrays instead of copying them (Spectral_Dynamics); the n arrays are
+    stale after the step, and graphs of rotated steps take a multiple of 3
+    steps."""
     Δt = Get_Δt(semi_implicit.integrator)
     assert not carry_gradients or (fused and not overlap), "carried gradients need the fused step"
+    assert not rotate or (fused and not overlap), "the rotated Time_Advance! needs the fused step"
     if fused and not overlap:
         Spectral_Dynamics(mesh, vert_coord, atmo_data, dyn_data, semi_implicit,
                           pressure_variables=False, physics_params=physics_params,
-                          carry_gradients=carry_gradients)
+                          carry_gradients=carry_gradients, rotate=rotate)
     else:
         if overlap:
             call("jgcm_mesh_branch", mesh.handle, 1)
