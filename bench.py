@@ -227,31 +227,35 @@ def hs_bench(jgcm, torch, N, nlat, nd, dt, steps, warmup, precision):
 
 
 def _time_ensemble(torch, ens, steps):
-    """Capture one later step of `ens`, replay it; (mean ms, per-step ms)."""
+    """Capture the later steps of `ens` (one step per graph, three when the
+    steps rotate), replay; (mean ms per step, ms per step of each launch)."""
     ens.step()
     ens.capture()
-    for _ in range(10):
+    k = ens.graph.repeat
+    launches = max(1, steps // k)
+    for _ in range(max(1, 10 // k)):
         ens.launch()
     ens.owner.synchronize()
     stream = torch.cuda.ExternalStream(ens.owner.stream())
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(launches + 1)]
     evs[0].record(stream)
-    for i in range(steps):
+    for i in range(launches):
         ens.launch()
         evs[i + 1].record(stream)
     ens.owner.synchronize()
-    lat = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(steps)])
+    lat = np.array([evs[i].elapsed_time(evs[i + 1]) / k for i in range(launches)])
     finite = all(bool(torch.isfinite(g.d.spe_vor_c).all()) for g in ens.groups)
     ens.graph.close()
-    return evs[0].elapsed_time(evs[-1]) / steps, lat, finite
+    return evs[0].elapsed_time(evs[-1]) / (launches * k), lat, finite
 
 
 def ensemble_bench(jgcm, torch, steps=100):
     """Mixed shallow-water ensemble: 64 x T21 (64x32), 128 x T42 (128x64) and
     64 x T85 (256x128) instances, seeded n^-3/2 vorticity spectra, FP32
     Legendre with FP64 state, one step of all 256 captured into one HIP graph,
-    in three launch forms: the truncations as parallel graph branches (each
-    on its own stream); stage by stage on one stream with each transform
+    in four launch forms: the truncations as parallel graph branches (each
+    on its own stream); the same with Time_Advance! by renaming p, c, n
+    (three steps per graph); stage by stage on one stream with each transform
     direction as one plan whose Legendre stage covers all truncations in ONE
     launch from a cost-sorted work list, on persistent workgroups; the same
     with one workgroup per work item.  Reports instance-steps/s and the
@@ -264,6 +268,7 @@ def ensemble_bench(jgcm, torch, steps=100):
     forms = {}
     tuning = None
     for name, kw in (("branches", {"concurrent": True}),
+                     ("branches_rotated", {"concurrent": True, "rotate": True}),
                      ("persistent_plans", {"persistent": True, "workgroups": -1}),
                      ("per_item_plans", {"persistent": True, "workgroups": 0})):
         ens = E.SW_Ensemble(groups, seed=11, precision="fp32", tuning=cache, **kw)

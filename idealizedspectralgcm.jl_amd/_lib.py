@@ -129,6 +129,8 @@ SIGNATURES = {
     "jgcm_sw_height_tendency_carried": [P, P, P, P, P, P, P, P, I],
     "jgcm_sw_spectral_step": [P, D, P, P, P, P, P, ctypes.POINTER(P), ctypes.POINTER(P),
                               ctypes.POINTER(P), I],
+    "jgcm_sw_spectral_step_ex": [P, D, P, P, P, P, P, ctypes.POINTER(P), ctypes.POINTER(P),
+                                 ctypes.POINTER(P), I, I],
     "jgcm_map": [P, I, SZ, P, D, D, P],
     "jgcm_vert_coord_create": [P, I, I, I, I, D, I, ctypes.POINTER(P)],
     "jgcm_vert_coord_create_ex": [P, I, I, I, I, D, I, D, D, D, D, D, ctypes.POINTER(P)],

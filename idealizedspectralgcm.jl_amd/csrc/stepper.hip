@@ -156,9 +156,13 @@ struct SWSpectral {
   double2* h[3];
 };
 
+// advance = 1: the spectral half of Time_Advance! (p = the filtered c, c = n =
+// the new value); advance = 0: Filtered_Leapfrog!'s own writes (the filtered
+// c in place, n = the new value), Time_Advance! left to the caller.
 __global__ void sw_spectral_kernel(SWSpectral s, const double* __restrict__ eig,
                                    const double* __restrict__ damping, int nm, int ns, size_t n_all,
-                                   double h0, double mu, int init_step, double dt, double r) {
+                                   double h0, double mu, int init_step, double dt, double r,
+                                   int advance) {
   const double mu2 = mu * mu;
   GRID_STRIDE(i, n_all) {
     const int m = (int)(i % nm);
@@ -178,20 +182,32 @@ __global__ void sw_spectral_kernel(SWSpectral s, const double* __restrict__ eig,
     Q = damp_reg(Q, vp, vc, d, init_step, dt);
     s.dvor[i] = Q;
     leapfrog_reg(Q, vp, vc, init_step, dt, r, cn, nx);
-    s.vor[0][i] = cn;
-    s.vor[1][i] = nx;
+    if (advance) {
+      s.vor[0][i] = cn;
+      s.vor[1][i] = nx;
+    } else {
+      s.vor[1][i] = cn;
+    }
     s.vor[2][i] = nx;
     D = damp_reg(D, dp, dc, d, init_step, dt);
     s.ddiv[i] = D;
     leapfrog_reg(D, dp, dc, init_step, dt, r, cn, nx);
-    s.div[0][i] = cn;
-    s.div[1][i] = nx;
+    if (advance) {
+      s.div[0][i] = cn;
+      s.div[1][i] = nx;
+    } else {
+      s.div[1][i] = cn;
+    }
     s.div[2][i] = nx;
     H = damp_reg(H, hp, hc, d, init_step, dt);
     s.dh[i] = H;
     leapfrog_reg(H, hp, hc, init_step, dt, r, cn, nx);
-    s.h[0][i] = cn;
-    s.h[1][i] = nx;
+    if (advance) {
+      s.h[0][i] = cn;
+      s.h[1][i] = nx;
+    } else {
+      s.h[1][i] = cn;
+    }
     s.h[2][i] = nx;
   }
 }
@@ -610,6 +626,14 @@ int jgcm_sw_height_tendency_carried(jgcm_mesh_t* mesh, const double* grid_dlambd
 int jgcm_sw_spectral_step(jgcm_leapfrog_t* lf, double h_0, const double* spe_kin, double* spe_energy,
                           double* spe_dvor, double* spe_ddiv, double* spe_dh, double* const* vor,
                           double* const* div, double* const* h, int nlev) {
+  return jgcm_sw_spectral_step_ex(lf, h_0, spe_kin, spe_energy, spe_dvor, spe_ddiv, spe_dh, vor, div,
+                                  h, nlev, 1);
+}
+
+int jgcm_sw_spectral_step_ex(jgcm_leapfrog_t* lf, double h_0, const double* spe_kin,
+                             double* spe_energy, double* spe_dvor, double* spe_ddiv, double* spe_dh,
+                             double* const* vor, double* const* div, double* const* h, int nlev,
+                             int advance) {
   JGCM_REQUIRE(lf != nullptr, "leapfrog is NULL");
   JGCM_REQUIRE(nlev >= 1, "nlev must be >= 1, got %d", nlev);
   JGCM_REQUIRE(spe_kin && spe_energy && spe_dvor && spe_ddiv && spe_dh && vor && div && h,
@@ -632,7 +656,7 @@ int jgcm_sw_spectral_step(jgcm_leapfrog_t* lf, double h_0, const double* spe_kin
   double mu = lf->implicit_coef * dtv;
   hipLaunchKernelGGL(sw_spectral_kernel, dim3(nblk(n)), dim3(256), 0, M->stream, s, M->d_eig,
                      lf->d_damping, M->nm, M->ns, n, h_0, mu, lf->init_step ? 1 : 0,
-                     (double)lf->dt, lf->robert_coef);
+                     (double)lf->dt, lf->robert_coef, advance ? 1 : 0);
   JGCM_LAUNCH_CHECK();
   return JGCM_OK;
 }

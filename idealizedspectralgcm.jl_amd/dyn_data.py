@@ -154,13 +154,14 @@ ROTATED_STATE = ["spe_vor", "spe_div", "spe_lnps", "spe_t", "grid_u", "grid_v", 
                  "grid_t"]
 
 
-def _rotate_state(d):
-    """p <- c, c <- n by renaming the ROTATED_STATE arrays: the array that held
-    p becomes the new n (its values are stale until the next step writes
-    it).  While a graph is being captured on the state's mesh the rotation is
-    logged, so that the graph can check that its steps return the names to
-    where they started (Step_Graph)."""
-    for f in ROTATED_STATE:
+def _rotate_state(d, names=None):
+    """p <- c, c <- n by renaming the arrays of `names` (default
+    ROTATED_STATE): the array that held p becomes the new n (its values are
+    stale until the next step writes it).  While a graph is being captured on
+    the state's mesh the rotation is logged, so that the graph can check that
+    its steps return the names to where they started (Step_Graph)."""
+    d._rotated_names = list(names or ROTATED_STATE)
+    for f in d._rotated_names:
         p, c, n = getattr(d, f + "_p"), getattr(d, f + "_c"), getattr(d, f + "_n")
         setattr(d, f + "_p", c)
         setattr(d, f + "_c", n)

@@ -13,7 +13,8 @@ import ctypes
 import numpy as np
 
 from ._lib import call
-from .dyn_data import BAROTROPIC_STATE, SW_STATE, Time_Advance_Model, Time_Advance_Tracers
+from .dyn_data import (BAROTROPIC_STATE, SW_STATE, Time_Advance_Model, Time_Advance_Tracers,
+                       _rotate_state)
 from .mesh import (_ptr, Vor_Div_From_Grid_UV, Apply_Laplacian, Add_Horizontal_Advection,
                    Trans_Spherical_To_Grid_Batch, Trans_Grid_To_Spherical_Batch, BATCH_DIVCOS,
                    BATCH_VORDIV, INV_UCOS, INV_VCOS, INV_GRAD_LAMBDA, INV_GRAD_THETA)
@@ -94,7 +95,7 @@ def Shallow_Water_Dynamics(mesh, atmo_data, h_0, dyn_data, integrator, verbose=F
 
 
 def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kappa_t, h_eq,
-                       carry_gradients=False):
+                       carry_gradients=False, rotate=False):
     """Shallow_Water_Physics! then Shallow_Water_Dynamics! (the drivers' step,
     Shallow_Water_Dynamics.jl:7-21 and :88-179) in fused passes (no reference
     counterpart): the physics, the absolute-vorticity residual, the kinetic
@@ -109,32 +110,45 @@ def Shallow_Water_Step(mesh, atmo_data, h_0, dyn_data, integrator, kappa_m, kapp
     the new h, which is the h whose gradient the next step's height tendency
     takes (:147), and the next step that also carries uses it instead of two
     more inverse transforms.  Same results bit for bit; every step of the run
-    must carry, and nothing may change the state between the steps."""
+    must carry, and nothing may change the state between the steps.
+
+    ``rotate``: Time_Advance! by renaming the p, c, n arrays of the state
+    (dyn_data.SW_STATE) instead of copying them: the grid pass skips p = c,
+    the spectral pass writes the filtered c in place, the closing copy goes.
+    p and c end bit for bit as the copying step leaves them; the n arrays
+    hold stale values after the step, and graphs of rotated steps take a
+    multiple of 3 steps (Step_Graph checks)."""
     d = dyn_data
     carried = carry_gradients and getattr(d, "_carried_sw", False)
     d._carried_sw = False
     if carry_gradients and getattr(d, "_sw_carry", None) is None:
         d._sw_carry = (mesh.grid_zeros(d.grid_u_c.shape[0]),
                        mesh.grid_zeros(d.grid_u_c.shape[0]))   # before any capture
-    sw_grid_pass(mesh, atmo_data, d, kappa_m, kappa_t, h_eq, carried)
+    sw_grid_pass(mesh, atmo_data, d, kappa_m, kappa_t, h_eq, carried, advance_p=not rotate)
     Trans_Grid_To_Spherical_Batch(mesh, *sw_forward_batch(d))
-    sw_spectral_pass(integrator, h_0, d)
+    sw_spectral_pass(integrator, h_0, d, advance=not rotate)
     snms, grids, flags, snms_b = sw_inverse_batch(d, carry_gradients)
     Trans_Spherical_To_Grid_Batch(mesh, snms, grids, flags, snms_b=snms_b)
-    sw_closing_copy(mesh, d)
+    if rotate:
+        Time_Advance_Tracers(d)
+        _rotate_state(d, SW_STATE)
+    else:
+        sw_closing_copy(mesh, d)
     d._carried_sw = carry_gradients
 
 
 # The stages of Shallow_Water_Step, also run stage by stage over several
 # meshes by the mixed-truncation ensemble (one transform plan per direction).
-def sw_grid_pass(mesh, atmo_data, d, kappa_m, kappa_t, h_eq, carried):
+def sw_grid_pass(mesh, atmo_data, d, kappa_m, kappa_t, h_eq, carried, advance_p=True):
     """The explicit grid pass and the height tendency's grid terms (:147-148);
-    `carried`: the gradient of h comes from the previous step's inverse batch."""
+    `carried`: the gradient of h comes from the previous step's inverse batch;
+    `advance_p`: the pass also sets p = c for the grid u, v, h."""
     n = mesh._nlev(d.grid_vor, "grid")
+    P = (lambda t: _ptr(t)) if advance_p else (lambda t: None)
     call("jgcm_sw_explicit", mesh.handle, _ptr(d.grid_u_c), _ptr(d.grid_v_c), _ptr(d.grid_ps_c),
          _ptr(h_eq), float(kappa_m), float(kappa_t), _ptr(d.grid_vor), atmo_data.omega,
          _ptr(d.grid_δu), _ptr(d.grid_δv), _ptr(d.grid_δlnps), _ptr(d.grid_absvor),
-         _ptr(d.grid_d_full1), _ptr(d.grid_u_p), _ptr(d.grid_v_p), _ptr(d.grid_ps_p), n)
+         _ptr(d.grid_d_full1), P(d.grid_u_p), P(d.grid_v_p), P(d.grid_ps_p), n)
     if carried:
         call("jgcm_sw_height_tendency_carried", mesh.handle, _ptr(d._sw_carry[0]),
              _ptr(d._sw_carry[1]), _ptr(d.grid_u_c), _ptr(d.grid_v_c), _ptr(d.grid_ps_c),
@@ -151,14 +165,15 @@ def sw_forward_batch(d):
             [d.spe_δvor, d.spe_δdiv, d.spe_d1, d.spe_δlnps], [BATCH_VORDIV, 0, 0, 0])
 
 
-def sw_spectral_pass(integrator, h_0, d):
-    """:141-143, :150-165 and the spectral part of :177 in one kernel."""
+def sw_spectral_pass(integrator, h_0, d, advance=True):
+    """:141-143, :150-165 and (`advance`) the spectral part of :177 in one
+    kernel; without `advance` the filtered c is written in place."""
     n = d.spe_δvor.shape[0]
     P3 = ctypes.c_void_p * 3
     lev = lambda f: P3(*(getattr(d, f + s).data_ptr() for s in ("_p", "_c", "_n")))
-    call("jgcm_sw_spectral_step", integrator.handle, float(h_0), _ptr(d.spe_d1),
+    call("jgcm_sw_spectral_step_ex", integrator.handle, float(h_0), _ptr(d.spe_d1),
          _ptr(d.spe_energy), _ptr(d.spe_δvor), _ptr(d.spe_δdiv), _ptr(d.spe_δlnps),
-         lev("spe_vor"), lev("spe_div"), lev("spe_lnps"), n)
+         lev("spe_vor"), lev("spe_div"), lev("spe_lnps"), n, 1 if advance else 0)
 
 
 def sw_inverse_batch(d, carry_gradients):

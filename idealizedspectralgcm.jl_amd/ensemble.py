@@ -73,8 +73,9 @@ class SW_Group:
     """`count` shallow-water instances at one truncation, stepped as one batch."""
 
     def __init__(self, N, nθ, count, dt, seed, precision="fp32", h_0=3.0e4, owner=None,
-                 vor0=None):
+                 vor0=None, rotate=False):
         nλ = 2 * nθ
+        self.rotate = bool(rotate)   # later steps: Time_Advance! by renaming p, c, n
         self.N, self.count, self.dt, self.h_0 = N, count, dt, h_0
         self.mesh = Spectral_Spherical_Mesh(N, N + 1, nθ, nλ, count, RADIUS, precision=precision)
         self._stream_owner = owner     # the library also keeps the owner alive
@@ -125,7 +126,7 @@ class SW_Group:
         """One shallow-water step of every instance of the group."""
         d, m = self.d, self.mesh
         Shallow_Water_Step(m, self.atmo, self.h_0, d, self.integ, 0.0, 0.0, self.h_eq,
-                           carry_gradients=True)
+                           carry_gradients=True, rotate=self.rotate)
         if self.steps == 0:
             Update_Init_Step(self.integ)
         self.steps += 1
@@ -135,7 +136,7 @@ class SW_Ensemble:
     """Mixed-truncation ensemble: groups = [(N, nθ, count, dt), ...]."""
 
     def __init__(self, groups, seed=0, precision="fp32", tuning=None, concurrent=False,
-                 persistent=False, workgroups=-1):
+                 persistent=False, workgroups=-1, rotate=False):
         """`tuning`: None (built-in shapes) or an autotune.Tuning_Cache; each
         truncation's FFT and Legendre shapes (for its instance count) are then read from the
         cache, or tuned and stored when missing, before any step is captured.
@@ -152,9 +153,15 @@ class SW_Ensemble:
         (plan.Transform_Plan): the Legendre stage of every truncation in ONE
         launch from a cost-sorted work list dealt to `workgroups` persistent
         workgroups (< 0: as many as the device holds at once; 0: one per work
-        item).  Each instance's results equal the other forms bit for bit."""
+        item).  Each instance's results equal the other forms bit for bit.
+
+        `rotate` (not with `persistent`, whose plans keep the arrays they were
+        built for): the steps advance each group's state by renaming p, c, n
+        (Shallow_Water_Step rotate) and a captured graph holds 3 steps."""
         from .autotune import Apply_Tuning
         assert not (concurrent and persistent), "the persistent form runs on one stream"
+        assert not (persistent and rotate), "plans keep the arrays they were built for"
+        self.rotate = bool(rotate)
         self.groups = []
         self.tuning = {}
         self.concurrent = bool(concurrent)
@@ -164,7 +171,8 @@ class SW_Ensemble:
         owner = None
         for i, (N, nθ, count, dt) in enumerate(groups):
             share = owner.mesh if (owner is not None and not self.concurrent) else None
-            g = SW_Group(N, nθ, count, dt, seed + 1000 * i, precision=precision, owner=share)
+            g = SW_Group(N, nθ, count, dt, seed + 1000 * i, precision=precision, owner=share,
+                         rotate=rotate)
             if owner is None:
                 owner = g
             if tuning is not None:
@@ -238,14 +246,15 @@ class SW_Ensemble:
                 call("jgcm_mesh_wait", g.mesh.handle, self.owner.handle)
             for g in self.groups:
                 Shallow_Water_Step(g.mesh, g.atmo, g.h_0, g.d, g.integ, 0.0, 0.0, g.h_eq,
-                                   carry_gradients=True)
+                                   carry_gradients=True, rotate=self.rotate)
             for g in others:                       # join: the graph ends after every branch
                 call("jgcm_mesh_wait", self.owner.handle, g.mesh.handle)
 
-        self.graph = Step_Graph(self.owner, body)
+        self.graph = Step_Graph(self.owner, body, repeat=3 if self.rotate else 1)
         return self.graph
 
     def launch(self):
+        """One graph launch: 1 step of every group, 3 with rotated steps."""
         self.graph.launch()
         for g in self.groups:
-            g.steps += 1
+            g.steps += self.graph.repeat

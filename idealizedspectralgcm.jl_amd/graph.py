@@ -16,7 +16,7 @@ def _unrotate(log):
     for d in {id(d): d for d in log}.values():
         k = sum(1 for x in log if x is d) % 3
         for _ in range((3 - k) % 3):
-            _rotate_state(d)
+            _rotate_state(d, d._rotated_names)
 
 
 class Step_Graph:

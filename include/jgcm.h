@@ -552,6 +552,14 @@ int jgcm_sw_height_tendency_carried(jgcm_mesh_t* mesh, const double* grid_dlambd
 int jgcm_sw_spectral_step(jgcm_leapfrog_t* lf, double h_0, const double* spe_kin, double* spe_energy,
                           double* spe_dvor, double* spe_ddiv, double* spe_dh, double* const* vor,
                           double* const* div, double* const* h, int nlev);
+/* The same with advance = 0: Filtered_Leapfrog!'s own writes (the filtered c
+ * in place, n = the new value) instead of the spectral Time_Advance!; for
+ * steps that advance the state by renaming p, c, n (no reference
+ * counterpart; the shallow-water step's rotate option).                     */
+int jgcm_sw_spectral_step_ex(jgcm_leapfrog_t* lf, double h_0, const double* spe_kin,
+                             double* spe_energy, double* spe_dvor, double* spe_ddiv, double* spe_dh,
+                             double* const* vor, double* const* div, double* const* h, int nlev,
+                             int advance);
 
 /* ---- primitive equations (Held-Suarez) ------------------------------------- */
 /* Vert_Coordinate(nλ, nθ, nd, vert_coord_option, vert_difference_option,
