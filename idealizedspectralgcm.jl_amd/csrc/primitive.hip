@@ -1234,7 +1234,9 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     double xi, SIRef rg, const double* __restrict__ eig, const double* __restrict__ wave,
     int num_wavenumbers, const double2* div_c, const double2* div_p, const double2* lnps_c,
     const double2* lnps_p, const double2* t_c, const double2* t_p, double2* ddiv, double2* dlnps,
-    double2* dtt, double2* energy, int stage_w, LFFuse lf) {
+    double2* dtt, double2* energy, int stage_w, int tri, LFFuse lf) {
+  // tri: only the coefficients n >= m are read and written (those with n < m
+  // keep their values), and a tile holding none of them returns at once.
   // LDS: nd * COLS double2 (level sums, then the wave-matrix operand); the
   // reference profiles (6 x (nd+1) doubles); when stage_w, the wave matrices
   // W_n of the tile's degrees n_lo .. n_hi.  Everything a thread reads from
@@ -1242,9 +1244,15 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
   // column work below waits for one round trip rather than one per stage.
   extern __shared__ double2 lds[];
   const TileIdx tx = tile_idx<KPT, COLS>(spec_per_level);
+  // the tile's COLS consecutive columns (idx = n nm + m) span degrees n_lo .. n_hi
+  const size_t first = (size_t)blockIdx.x * COLS;
+  const size_t last = min(first + COLS, spec_per_level) - 1;
+  const int n_lo = (int)(first / nm);
+  if (tri && (int)(last / nm) == n_lo && (int)(first % nm) > n_lo) return;  // all m > n
   const size_t idx = tx.c;
   const int n = tx.col ? (int)((unsigned)idx / (unsigned)nm) : 0;
   const int m = tx.col ? (int)idx - n * nm : 0;
+  const bool live = tx.col && (!tri || n >= m);
   const int k0 = tx.k0;
   const int n1 = nd + 1;
   double* refs = reinterpret_cast<double*>(lds + (size_t)nd * COLS);
@@ -1260,10 +1268,6 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
   }
   const SIRef r{refs, refs + n1, refs + 2 * n1, refs + 3 * n1, refs + 4 * n1, refs + 5 * n1,
                 refs + 6 * n1};
-  // the tile's COLS consecutive columns (idx = n nm + m) span degrees n_lo .. n_hi
-  const size_t first = (size_t)blockIdx.x * COLS;
-  const size_t last = min(first + COLS, spec_per_level) - 1;
-  const int n_lo = (int)(first / nm);
   const int nnd = nd * nd;
   double* Wl = refs + 7 * n1;
   if (stage_w) {
@@ -1282,7 +1286,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
   double2 Dp[KPT], Dc[KPT], Tp[KPT], Tc[KPT], DT[KPT], DV[KPT], EN[KPT];
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
-    const bool ok = tx.col && k0 + i < nd;
+    const bool ok = live && k0 + i < nd;
     const size_t o = ok ? (size_t)(k0 + i) * spec_per_level + idx : 0;
     Dp[i] = d2keep(ok, div_p[o]);
     Dc[i] = d2keep(ok, div_c[o]);
@@ -1297,7 +1301,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
   if (lf.on) {
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
-      const bool ok = tx.col && k0 + i < nd;
+      const bool ok = live && k0 + i < nd;
       const size_t o = ok ? (size_t)(k0 + i) * spec_per_level + idx : 0;
       QV[i] = d2keep(ok, lf.dvor[o]);
       VP[i] = d2keep(ok, lf.vor_p[o]);
@@ -1305,10 +1309,10 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     }
     dmp = lf.damping[m * (ns + 1) + n];
   }
-  const size_t ic = tx.col ? idx : 0;
-  const double2 dl0 = d2keep(tx.col, dlnps[ic]);
-  const double2 lnc = d2keep(tx.col, lnps_c[ic]);
-  const double2 lnp = d2keep(tx.col, lnps_p[ic]);
+  const size_t ic = live ? idx : 0;
+  const double2 dl0 = d2keep(live, dlnps[ic]);
+  const double2 lnc = d2keep(live, lnps_c[ic]);
+  const double2 lnp = d2keep(live, lnps_p[ic]);
   const double e = eig[m * (ns + 1) + n];
   __syncthreads();  // refs and Wl
   double2 A[KPT], B[KPT], G[KPT], DD[KPT];
@@ -1320,7 +1324,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
   const double2 dl = d2(dl0.x + dps.x / ps_ref, dl0.y + dps.y / ps_ref);
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
-    const bool ok = tx.col && k0 + i < nd;
+    const bool ok = live && k0 + i < nd;
     const double2 d = d2add(DT[i], B[i]);
     DD[i] = d;
     // δt_temp = t_p - t_c + ξ δt
@@ -1331,7 +1335,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
     const int k = k0 + i;
-    const bool ok = tx.col && k < nd;
+    const bool ok = live && k < nd;
     const size_t o = (size_t)k * spec_per_level + idx;
     const double hp = ok ? r.h[k] * ps_ref : 0.0;
     double2 d = DV[i];
@@ -1349,7 +1353,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
   for (int i = 0; i < KPT; ++i)
     if (k0 + i < nd) lds[(k0 + i) * COLS + tx.lane] = A[i];
   __syncthreads();
-  if (tx.col && n >= m && n <= num_wavenumbers) {
+  if (live && n >= m && n <= num_wavenumbers) {
     // the level qq of the operand is read once for the KPT rows of this
     // thread (each row still sums over qq in increasing order); two loops, so
     // that each reads through one address space (an LDS/global select would
@@ -1384,7 +1388,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
   double2 lq = d2(dl.x + fx * dps.x, dl.y + fx * dps.y);
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
-    if (!(tx.col && k0 + i < nd)) continue;
+    if (!(live && k0 + i < nd)) continue;
     const size_t o = (size_t)(k0 + i) * spec_per_level + idx;
     double2 qd = Dfin[i];
     double2 qt = d2(DD[i].x + xi * B[i].x, DD[i].y + xi * B[i].y);
@@ -1408,7 +1412,7 @@ __global__ __launch_bounds__(512) void hs_implicit_kernel(
     ddiv[o] = qd;
     dtt[o] = qt;
   }
-  if (tx.col && tx.q == 0) {
+  if (live && tx.q == 0) {
     dlnps[idx] = lq;
     if (lf.on) {
       double2 cn, nn;
@@ -1934,7 +1938,7 @@ static const double* skap_of(const jgcm_semi_implicit* si) { return si->d_ref + 
 static int implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, const double* div_p,
                                const double* lnps_c, const double* lnps_p, const double* t_c,
                                const double* t_p, double* ddiv, double* dlnps, double* dt,
-                               double* energy, LFFuse lf = LFFuse{}) {
+                               double* energy, LFFuse lf = LFFuse{}, int tri = 0) {
   JGCM_REQUIRE(si && div_c && div_p && lnps_c && lnps_p && t_c && t_p && ddiv && dlnps && dt,
                "NULL argument");
   jgcm_mesh* M = si->vc->mesh;
@@ -1955,7 +1959,7 @@ static int implicit_correction(jgcm_semi_implicit_t* si, const double* div_c, co
                      M->d_eig, si->d_wave, si->num_wavenumbers, (const double2*)div_c,           \
                      (const double2*)div_p, (const double2*)lnps_c, (const double2*)lnps_p,      \
                      (const double2*)t_c, (const double2*)t_p, (double2*)ddiv, (double2*)dlnps,  \
-                     (double2*)dt, (double2*)energy, stage_w, lf)
+                     (double2*)dt, (double2*)energy, stage_w, tri, lf)
   switch (kpt) {
     case 1: JGCM_SI_LAUNCH(1); break;
     case 2: JGCM_SI_LAUNCH(2); break;
@@ -1988,7 +1992,7 @@ int jgcm_hs_energy_implicit_leapfrog(jgcm_semi_implicit_t* si, double* spe_energ
             (double2*)lnps_n, L->d_damping, 1, L->init_step ? 1 : 0, (double)L->dt,
             L->robert_coef};
   return implicit_correction(si, div_c, div_p, lnps_c, lnps_p, t_c, t_p, ddiv, dlnps, dt,
-                             spe_energy, lf);
+                             spe_energy, lf, 1);
 }
 
 int jgcm_hs_implicit_leapfrog_advance(jgcm_semi_implicit_t* si, double* spe_energy, double* ddiv,
@@ -2006,7 +2010,7 @@ int jgcm_hs_implicit_leapfrog_advance(jgcm_semi_implicit_t* si, double* spe_ener
             L->robert_coef, 1, (double2*)vor[0], (double2*)div[0], (double2*)t[0],
             (double2*)lnps[0]};
   return implicit_correction(si, div[1], div[0], lnps[1], lnps[0], t[1], t[0], ddiv, dlnps, dt,
-                             spe_energy, lf);
+                             spe_energy, lf, 1);
 }
 
 int jgcm_hs_energy_implicit_correction(jgcm_semi_implicit_t* si, double* spe_energy,

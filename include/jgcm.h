@@ -647,9 +647,11 @@ int jgcm_hs_energy_implicit_correction(jgcm_semi_implicit_t* si, double* spe_ene
  * Filtered_Leapfrog! (Spectral_Dynamics.jl:319-328; vorticity, divergence and
  * T damped, ln ps not) in the same pass (no single reference function): the
  * c arrays of vor, div, ln ps and T receive the Robert filter, the n arrays
- * the new values, the tendencies the damped values.  Equal to
- * jgcm_hs_energy_implicit_correction followed by jgcm_damped_leapfrog_batch
- * bit for bit.                                                              */
+ * the new values, the tendencies the damped values.  Only the coefficients
+ * with n >= m are read and written: those with n < m, zero in every state a
+ * step produces (no transform or step writes them), keep their values.  On
+ * the others equal to jgcm_hs_energy_implicit_correction followed by
+ * jgcm_damped_leapfrog_batch bit for bit.                                   */
 int jgcm_hs_energy_implicit_leapfrog(jgcm_semi_implicit_t* si, double* spe_energy, double* div_c,
                                      const double* div_p, double* lnps_c, const double* lnps_p,
                                      double* t_c, const double* t_p, double* ddiv, double* dlnps,
