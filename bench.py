@@ -8,9 +8,10 @@ band-limited fields resident in HBM before timing starts.  Throughput counts
 each direction of each level-field as one transform.
 
 The Legendre stage is measured in FP64, split-FP32 and BF16x3; `value` is
-the split-FP32 path when its measured error against the FP64 path is <= 1e-6,
-otherwise the FP64 path.  The CPU baseline is the FP64 oracle restatement
-(numpy, single thread) on a bounded sample of the same workload.
+the fastest of them whose measured error against the FP64 path is <= 1e-6.
+The CPU baselines are the C++ FP64 restatements of the transform pair and of
+the Held-Suarez step (oracle/sht_cpu.cpp, oracle/hs_cpu.cpp), each timed on
+one thread and on all host cores on a bounded sample of the same workload.
 """
 import argparse
 import ctypes

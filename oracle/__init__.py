@@ -6,6 +6,9 @@ and ``src/Atmos_Param``).  It exists to check the HIP library: only
 ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import it.  The product path (``libjgcm_hip.so`` and the
 host mirror in ``idealizedspectralgcm.jl_amd``) never calls into it.
+``cpu.py`` binds C++ restatements of the transform pair and the Held-Suarez
+step (``sht_cpu.cpp``, ``hs_cpu.cpp``), checked against the numpy functions
+and timed by bench.py as the CPU baselines.
 
 Array layout is the reference's memory layout read in C order:
 
